@@ -1,0 +1,284 @@
+/*
+ * mbx.h - C ABI of the MI355X (gfx950) distance-estimation raymarcher.
+ *
+ * The library (libmbx.so) replaces the per-pixel hot path of Mandelbulber2:
+ *   RayMarching()        mandelbulber2/src/render_worker.cpp:708-951
+ *   CalculateDistance()  mandelbulber2/src/calculate_distance.cpp:60-64, 533-763
+ *   Compute<Mode>()      mandelbulber2/src/compute_fractal.cpp:67-654
+ *   FormulaCode()        mandelbulber2/formula/definition/fractal_{mandelbulb,mandelbox,
+ *                        kaleidoscopic_ifs}.cpp
+ * plus the normal, Lambert light, hard-shadow, ambient-occlusion, background and Monte Carlo
+ * GI shaders that RayRecursion() runs on each hit (render_worker.cpp:969-1819).
+ *
+ * Conventions: plain C linkage, POD structs, caller-owned buffers, int status codes
+ * (0 = success, negative = error; the message is read with mbx_last_error()).  No C++
+ * exception crosses this boundary.  One context per device stream; calls on one context are
+ * serialised by the caller.
+ *
+ * Parameter names follow the reference's .fract keys and sParamRender / sFractal field names
+ * (mandelbulber2/src/fractparams.cpp:80-356, mandelbulber2/src/fractal.cpp:41-120).
+ */
+#ifndef MBX_H_
+#define MBX_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MBX_ABI_VERSION 1
+
+/* array sizes as in mandelbulber2/src/fractal.h:52-54 */
+#define MBX_IFS_VECTOR_COUNT 9
+#define MBX_MANDELBOX_FOLDS 2
+/* upper bound of explicitly supplied ambient-occlusion directions */
+#define MBX_MAX_AO_VECTORS 64
+
+/* status codes */
+#define MBX_OK 0
+#define MBX_ERR_INVALID -1
+#define MBX_ERR_UNSUPPORTED -2
+#define MBX_ERR_HIP -3
+#define MBX_ERR_NOMEM -4
+
+/* enumFractalFormula ids (mandelbulber2/formula/definition/all_fractal_list_enums.hpp:45,51,53) */
+enum mbx_formula {
+	MBX_FORMULA_NONE = 0,
+	MBX_FORMULA_MANDELBULB = 2,
+	MBX_FORMULA_MANDELBOX = 8,
+	MBX_FORMULA_KALEIDOSCOPIC_IFS = 10
+};
+
+/* enumDEType (mandelbulber2/src/fractal_enums.h:70-75) */
+enum mbx_de_type { MBX_DE_ANALYTIC = 0, MBX_DE_DELTA = 1 };
+
+/* enumDEFunctionType (fractal_enums.h:47-61): used by the delta-DE finish */
+enum mbx_de_function { MBX_DE_FUNCTION_LINEAR = 1, MBX_DE_FUNCTION_LOGARITHMIC = 2 };
+
+/* enumDEAnalyticFunction (fractal_enums.h:77-88): used by the analytic-DE finish */
+enum mbx_analytic_function {
+	MBX_ANALYTIC_LINEAR = 1,
+	MBX_ANALYTIC_LOGARITHMIC = 2,
+	MBX_ANALYTIC_IFS = 3
+};
+
+/* enumAOMode (mandelbulber2/src/ao_modes.h:42-44) */
+enum mbx_ao_mode { MBX_AO_FAST = 0, MBX_AO_MULTIPLE_RAYS = 1 };
+
+/* RNG stream layout.  PER_PIXEL: one Park-Miller stream per pixel, seeded from
+ * (pixel index, random_seed); identical on CPU and GPU.  GLOBAL_SEQUENTIAL: one stream for the
+ * whole image in row-major order starting from seed 1, as the single-threaded reference renders
+ * (mandelbulber2/src/common_math.cpp:47-72); CPU oracle only. */
+enum mbx_rng_mode { MBX_RNG_PER_PIXEL = 0, MBX_RNG_GLOBAL_SEQUENTIAL = 1 };
+
+/* sFractalMandelbulb (mandelbulber2/src/fractal.h:227-234); angles in degrees as in .fract */
+typedef struct mbx_fractal_mandelbulb {
+	double power;            /* "power" */
+	double alphaAngleOffset; /* "alpha_angle_offset", degrees */
+	double betaAngleOffset;  /* "beta_angle_offset", degrees */
+} mbx_fractal_mandelbulb;
+
+/* sFractalMandelbox (fractal.h:185-219); rotations in degrees as in .fract */
+typedef struct mbx_fractal_mandelbox {
+	double scale;                 /* "mandelbox_scale" */
+	double foldingLimit;          /* "mandelbox_folding_limit" */
+	double foldingValue;          /* "mandelbox_folding_value" */
+	double foldingSphericalMin;   /* "mandelbox_folding_min_radius" */
+	double foldingSphericalFixed; /* "mandelbox_folding_fixed_radius" */
+	double offset[3];             /* "mandelbox_offset" */
+	double rotationMain[3];       /* "mandelbox_rotation_main" */
+	double rotation[MBX_MANDELBOX_FOLDS][3][3]; /* [0]="mandelbox_rotation_neg_i",
+	                                              [1]="mandelbox_rotation_pos_i" */
+	double colorFactor[3];        /* "mandelbox_color" */
+	double colorFactorSp1;        /* "mandelbox_color_Sp1" */
+	double colorFactorSp2;        /* "mandelbox_color_Sp2" */
+	int32_t rotationsEnabled;     /* "mandelbox_rotations_enabled" */
+	int32_t mainRotationEnabled;  /* "mandelbox_main_rotation_enabled" */
+} mbx_fractal_mandelbox;
+
+/* sFractalIFS (fractal.h:155-174) */
+typedef struct mbx_fractal_ifs {
+	double scale;                              /* "IFS_scale" */
+	double offset[3];                          /* "IFS_offset" */
+	double rotation[3];                        /* "IFS_rotation", degrees */
+	double edge[3];                            /* "IFS_edge" */
+	double direction[MBX_IFS_VECTOR_COUNT][3]; /* "IFS_direction_i" (normalised on upload) */
+	double rotations[MBX_IFS_VECTOR_COUNT][3]; /* "IFS_rotations_i", degrees */
+	double distance[MBX_IFS_VECTOR_COUNT];     /* "IFS_distance_i" */
+	double intensity[MBX_IFS_VECTOR_COUNT];    /* "IFS_intensity_i" */
+	int32_t enabled[MBX_IFS_VECTOR_COUNT];     /* "IFS_enabled_i" */
+	int32_t absX, absY, absZ;                  /* "IFS_abs_x/y/z" */
+	int32_t mengerSpongeMode;                  /* "IFS_menger_sponge_mode" */
+	int32_t rotationEnabled;                   /* "IFS_rotation_enabled" */
+	int32_t edgeEnabled;                       /* "IFS_edge_enabled" */
+} mbx_fractal_ifs;
+
+/* One single-formula fractal node: sFractal plus the sequence data that
+ * cHybridFractalSequences::CollectSequenceData derives for it
+ * (mandelbulber2/src/hybrid_fractal_sequences.cpp:181-287, 377-503). */
+typedef struct mbx_fractal {
+	int32_t formula;            /* enum mbx_formula ("formula_1") */
+	int32_t formulaMaxiter;     /* "N" / node formula_maxiter */
+	int32_t deltaDEMethod;      /* "delta_DE_method": 0 preferred, 1 force delta, 2 force analytic */
+	int32_t deltaDEFunction;    /* "delta_DE_function": 0 preferred, 1 linear, 2 logarithmic */
+	int32_t useDefaultBailout;  /* "use_default_bailout" */
+	int32_t juliaMode;          /* "julia_mode" */
+	int32_t dontAddCConstant;   /* "dont_add_c_constant" */
+	int32_t reserved0;
+	double bailout;             /* "bailout" (used when useDefaultBailout == 0) */
+	double juliaC[3];           /* "julia_c" */
+	double constantFactor[3];   /* "fractal_constant_factor" */
+	double initialWAxis;        /* "initial_waxis" */
+	mbx_fractal_mandelbulb bulb;
+	mbx_fractal_mandelbox mandelbox;
+	mbx_fractal_ifs IFS;
+} mbx_fractal;
+
+/* Light #1 (cLight, mandelbulber2/src/light.cpp:51-195).  Only the directional type
+ * is on the path. */
+typedef struct mbx_light {
+	int32_t enabled;          /* "light1_enabled" */
+	int32_t castShadows;      /* "light1_cast_shadows" */
+	int32_t penetrating;      /* "light1_penetrating" */
+	int32_t relativePosition; /* "light1_relative_position" */
+	double rotation[3];       /* "light1_rotation", degrees */
+	double intensity;         /* "light1_intensity" */
+	double visibility;        /* "light1_visibility" */
+	double size;              /* "light1_size" */
+	double softShadowCone;    /* "light1_soft_shadow_cone", degrees */
+	double contourSharpness;  /* "light1_contour_sharpness" */
+	double color[3];          /* "light1_color" as 0..1 */
+} mbx_light;
+
+/* Render parameters: the sParamRender subset on the path
+ * (mandelbulber2/src/fractparams.cpp:80-356), with material #1 and light #1. */
+typedef struct mbx_params {
+	int32_t imageWidth;             /* "image_width" */
+	int32_t imageHeight;            /* "image_height" */
+	int32_t perspectiveType;        /* "perspective_type": only 0 (three-point) */
+	int32_t maxRaymarchingSteps;    /* "max_raymarching_steps" */
+	int32_t minN;                   /* "minN" */
+	int32_t iterThreshMode;         /* "iteration_threshold_mode" */
+	int32_t interiorMode;           /* "interior_mode" */
+	int32_t constantDEThreshold;    /* "constant_DE_threshold" */
+	double camera[3];               /* "camera" */
+	double target[3];               /* "target" */
+	double topVector[3];            /* "camera_top" */
+	double fov;                     /* "fov", degrees */
+	double sweetSpotHAngle;         /* "sweet_spot_horizontal_angle", degrees */
+	double sweetSpotVAngle;         /* "sweet_spot_vertical_angle", degrees */
+	double detailLevel;             /* "detail_level" */
+	double DEThresh;                /* "DE_thresh" */
+	double DEFactor;                /* "DE_factor" */
+	double smoothness;              /* "smoothness" */
+	double viewDistanceMax;         /* "view_distance_max" */
+
+	/* material #1 (cMaterial, mandelbulber2/src/material.cpp) */
+	double surfaceColor[3];         /* "mat1_surface_color" as 0..1 */
+	double shading;                 /* "mat1_shading" */
+	double luminosity;              /* "mat1_luminosity" */
+	double luminosityColor[3];      /* "mat1_luminosity_color" */
+	double fillLightColor[3];       /* "fill_light_color" */
+
+	mbx_light light;
+
+	/* ambient occlusion (shader_ambient_occlusion.cpp, shader_fast_ambient_occlusion.cpp) */
+	int32_t ambientOcclusionEnabled; /* "ambient_occlusion_enabled" */
+	int32_t ambientOcclusionMode;    /* "ambient_occlusion_mode" */
+	int32_t ambientOcclusionQuality; /* "ambient_occlusion_quality" */
+	int32_t aoVectorsCount;          /* number of explicit AO directions (replaces the
+	                                    lightmap-pruned set of PrepareAOVectors) */
+	double ambientOcclusion;         /* "ambient_occlusion" */
+	double ambientOcclusionColor[3]; /* "ambient_occlusion_color" */
+	double ambientOcclusionFastTune; /* "ambient_occlusion_fast_tune" */
+	double aoVectors[MBX_MAX_AO_VECTORS][3];
+	double aoColors[MBX_MAX_AO_VECTORS][3];
+
+	/* background (shader_background.cpp) */
+	int32_t background3ColorsEnable;  /* "background_3_colors_enable" */
+	int32_t reserved1;
+	double backgroundColor1[3];       /* "background_color_1" */
+	double backgroundColor2[3];       /* "background_color_2" */
+	double backgroundColor3[3];       /* "background_color_3" */
+	double backgroundBrightness;      /* "background_brightness" */
+	double backgroundGamma;           /* "background_gamma" */
+
+	/* Monte Carlo (render_worker.cpp:190-233, shader_global_illumination.cpp) */
+	int32_t DOFMonteCarlo;                   /* "DOF_monte_carlo" */
+	int32_t DOFSamples;                      /* "DOF_samples" */
+	int32_t DOFMonteCarloGlobalIllumination; /* "DOF_MC_global_illumination" */
+	int32_t reflectionsMax;                  /* "reflections_max" (GI bounce depth) */
+	double monteCarloGIRadianceLimit;        /* "MC_GI_radiance_limit" */
+
+	/* random stream */
+	int32_t rngMode;     /* enum mbx_rng_mode */
+	uint32_t randomSeed; /* per-frame seed mixed into the per-pixel streams */
+
+	mbx_fractal fractal;
+} mbx_params;
+
+/* Per-pixel output planes (device pointers; any may be NULL except where noted).
+ * Layout: row-major, pixel (x, y) at index y*imageWidth + x. */
+typedef struct mbx_outputs {
+	float *rgb;      /* 3 floats per pixel: the pre-post-processing image buffer */
+	float *depth;    /* ray depth t; 1e20 on a miss (render_worker.cpp:353) */
+	int32_t *iters;  /* iteration count of the last DE evaluation at the returned point */
+	uint8_t *mask;   /* 1 = surface hit */
+	float *normal;   /* 3 floats per pixel, world-space unit normal (0 on a miss) */
+	int32_t *steps;  /* raymarching step count per pixel (coarse + binary search) */
+} mbx_outputs;
+
+/* Work statistics of the last launch on a context (cStatistics analogue,
+ * mandelbulber2/src/statistics.h:42-75). */
+typedef struct mbx_stats {
+	uint64_t rays;            /* primary rays (pixels x samples) */
+	uint64_t de_evals;        /* CalculateDistance calls */
+	uint64_t iterations;      /* formula iterations (totalNumberOfIterations) */
+	double kernel_ms;         /* device time of the last render launch */
+} mbx_stats;
+
+typedef struct mbx_ctx mbx_ctx;
+
+/* Library version (MBX_ABI_VERSION). */
+int mbx_version(void);
+
+/* Create a context on HIP device `device`.  Replaces the OpenCL engine's
+ * CreateKernel4Program / PreAllocateBuffers / CreateCommandQueue sequence
+ * (mandelbulber2/src/render_job.cpp:916-927, opencl_engine.cpp:277-375).  `stream` is a
+ * hipStream_t (NULL = default stream). */
+int mbx_ctx_create(int device, void *stream, mbx_ctx **out);
+void mbx_ctx_destroy(mbx_ctx *ctx);
+
+/* Last error message of the context (never NULL). */
+const char *mbx_last_error(const mbx_ctx *ctx);
+
+/* Validate a parameter block and derive the per-frame constants (camera rotation, light
+ * vector, fractal sequence data) on the host, then upload them.  Replaces
+ * cOpenClEngineRenderFractal::SetParameters (opencl_engine_render_fractal.cpp:1114-1245)
+ * and the sParamRender / cHybridFractalSequences set-up of cRenderJob::Execute
+ * (render_job.cpp:399-406, 477). */
+int mbx_scene_upload(mbx_ctx *ctx, const mbx_params *params);
+
+/* Render the uploaded scene into caller-owned device buffers.  Replaces RenderMulti
+ * (opencl_engine_render_fractal.cpp:2089-2425) and, on the CPU side, cRenderer::RenderImage
+ * (render_image.cpp:354) / cRenderWorker::doWork (render_worker.cpp:91-530).
+ * Asynchronous on the context stream; mbx_get_stats synchronises. */
+int mbx_render(mbx_ctx *ctx, const mbx_outputs *out);
+
+/* Point DE query: for n points (xyz doubles, host memory) return the distance and
+ * iteration count of CalculateDistance at detail size `detail_size` (host memory).  Mirrors
+ * the OpenCL distance engine (mandelbulber2/opencl/engines/distance_engine.cl:42) and
+ * cRenderWorker-free CalculateDistance (calculate_distance.cpp:60-64). */
+int mbx_distance(mbx_ctx *ctx, const double *points, int n, double detail_size,
+	float *dist_out, int32_t *iters_out);
+
+/* Statistics of the last render (synchronises the stream). */
+int mbx_get_stats(mbx_ctx *ctx, mbx_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MBX_H_ */

@@ -1,0 +1,186 @@
+"""ctypes mirror of include/mbx.h (the C ABI of libmbx.so).
+
+Field names follow the reference's sParamRender / sFractal members and .fract keys
+(mandelbulber2/src/fractparams.cpp:80-356, mandelbulber2/src/fractal.cpp:41-120); the C header
+documents each field.  Keep this file in the same order as the header: the layout is checked
+by tests against sizeof(mbx_params) reported by the compiled library.
+"""
+import ctypes as C
+
+MBX_IFS_VECTOR_COUNT = 9
+MBX_MANDELBOX_FOLDS = 2
+MBX_MAX_AO_VECTORS = 64
+
+MBX_FORMULA_NONE = 0
+MBX_FORMULA_MANDELBULB = 2
+MBX_FORMULA_MANDELBOX = 8
+MBX_FORMULA_KALEIDOSCOPIC_IFS = 10
+
+MBX_AO_FAST = 0
+MBX_AO_MULTIPLE_RAYS = 1
+
+MBX_RNG_PER_PIXEL = 0
+MBX_RNG_GLOBAL_SEQUENTIAL = 1
+
+D3 = C.c_double * 3
+
+
+class MbxFractalMandelbulb(C.Structure):
+    _fields_ = [
+        ("power", C.c_double),
+        ("alphaAngleOffset", C.c_double),
+        ("betaAngleOffset", C.c_double),
+    ]
+
+
+class MbxFractalMandelbox(C.Structure):
+    _fields_ = [
+        ("scale", C.c_double),
+        ("foldingLimit", C.c_double),
+        ("foldingValue", C.c_double),
+        ("foldingSphericalMin", C.c_double),
+        ("foldingSphericalFixed", C.c_double),
+        ("offset", D3),
+        ("rotationMain", D3),
+        ("rotation", (D3 * 3) * MBX_MANDELBOX_FOLDS),
+        ("colorFactor", D3),
+        ("colorFactorSp1", C.c_double),
+        ("colorFactorSp2", C.c_double),
+        ("rotationsEnabled", C.c_int32),
+        ("mainRotationEnabled", C.c_int32),
+    ]
+
+
+class MbxFractalIFS(C.Structure):
+    _fields_ = [
+        ("scale", C.c_double),
+        ("offset", D3),
+        ("rotation", D3),
+        ("edge", D3),
+        ("direction", D3 * MBX_IFS_VECTOR_COUNT),
+        ("rotations", D3 * MBX_IFS_VECTOR_COUNT),
+        ("distance", C.c_double * MBX_IFS_VECTOR_COUNT),
+        ("intensity", C.c_double * MBX_IFS_VECTOR_COUNT),
+        ("enabled", C.c_int32 * MBX_IFS_VECTOR_COUNT),
+        ("absX", C.c_int32),
+        ("absY", C.c_int32),
+        ("absZ", C.c_int32),
+        ("mengerSpongeMode", C.c_int32),
+        ("rotationEnabled", C.c_int32),
+        ("edgeEnabled", C.c_int32),
+    ]
+
+
+class MbxFractal(C.Structure):
+    _fields_ = [
+        ("formula", C.c_int32),
+        ("formulaMaxiter", C.c_int32),
+        ("deltaDEMethod", C.c_int32),
+        ("deltaDEFunction", C.c_int32),
+        ("useDefaultBailout", C.c_int32),
+        ("juliaMode", C.c_int32),
+        ("dontAddCConstant", C.c_int32),
+        ("reserved0", C.c_int32),
+        ("bailout", C.c_double),
+        ("juliaC", D3),
+        ("constantFactor", D3),
+        ("initialWAxis", C.c_double),
+        ("bulb", MbxFractalMandelbulb),
+        ("mandelbox", MbxFractalMandelbox),
+        ("IFS", MbxFractalIFS),
+    ]
+
+
+class MbxLight(C.Structure):
+    _fields_ = [
+        ("enabled", C.c_int32),
+        ("castShadows", C.c_int32),
+        ("penetrating", C.c_int32),
+        ("relativePosition", C.c_int32),
+        ("rotation", D3),
+        ("intensity", C.c_double),
+        ("visibility", C.c_double),
+        ("size", C.c_double),
+        ("softShadowCone", C.c_double),
+        ("contourSharpness", C.c_double),
+        ("color", D3),
+    ]
+
+
+class MbxParams(C.Structure):
+    _fields_ = [
+        ("imageWidth", C.c_int32),
+        ("imageHeight", C.c_int32),
+        ("perspectiveType", C.c_int32),
+        ("maxRaymarchingSteps", C.c_int32),
+        ("minN", C.c_int32),
+        ("iterThreshMode", C.c_int32),
+        ("interiorMode", C.c_int32),
+        ("constantDEThreshold", C.c_int32),
+        ("camera", D3),
+        ("target", D3),
+        ("topVector", D3),
+        ("fov", C.c_double),
+        ("sweetSpotHAngle", C.c_double),
+        ("sweetSpotVAngle", C.c_double),
+        ("detailLevel", C.c_double),
+        ("DEThresh", C.c_double),
+        ("DEFactor", C.c_double),
+        ("smoothness", C.c_double),
+        ("viewDistanceMax", C.c_double),
+        ("surfaceColor", D3),
+        ("shading", C.c_double),
+        ("luminosity", C.c_double),
+        ("luminosityColor", D3),
+        ("fillLightColor", D3),
+        ("light", MbxLight),
+        ("ambientOcclusionEnabled", C.c_int32),
+        ("ambientOcclusionMode", C.c_int32),
+        ("ambientOcclusionQuality", C.c_int32),
+        ("aoVectorsCount", C.c_int32),
+        ("ambientOcclusion", C.c_double),
+        ("ambientOcclusionColor", D3),
+        ("ambientOcclusionFastTune", C.c_double),
+        ("aoVectors", D3 * MBX_MAX_AO_VECTORS),
+        ("aoColors", D3 * MBX_MAX_AO_VECTORS),
+        ("background3ColorsEnable", C.c_int32),
+        ("reserved1", C.c_int32),
+        ("backgroundColor1", D3),
+        ("backgroundColor2", D3),
+        ("backgroundColor3", D3),
+        ("backgroundBrightness", C.c_double),
+        ("backgroundGamma", C.c_double),
+        ("DOFMonteCarlo", C.c_int32),
+        ("DOFSamples", C.c_int32),
+        ("DOFMonteCarloGlobalIllumination", C.c_int32),
+        ("reflectionsMax", C.c_int32),
+        ("monteCarloGIRadianceLimit", C.c_double),
+        ("rngMode", C.c_int32),
+        ("randomSeed", C.c_uint32),
+        ("fractal", MbxFractal),
+    ]
+
+
+class MbxOutputs(C.Structure):
+    _fields_ = [
+        ("rgb", C.c_void_p),
+        ("depth", C.c_void_p),
+        ("iters", C.c_void_p),
+        ("mask", C.c_void_p),
+        ("normal", C.c_void_p),
+        ("steps", C.c_void_p),
+    ]
+
+
+class MbxStats(C.Structure):
+    _fields_ = [
+        ("rays", C.c_uint64),
+        ("de_evals", C.c_uint64),
+        ("iterations", C.c_uint64),
+        ("kernel_ms", C.c_double),
+    ]
+
+
+def set3(arr, values):
+    for i, v in enumerate(values):
+        arr[i] = float(v)

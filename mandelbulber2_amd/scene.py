@@ -1,0 +1,242 @@
+"""Scene parameter blocks: reference defaults and the benchmark / parity workloads.
+
+Defaults are the values cParameterContainer registers in mandelbulber2/src/initparameters.cpp
+(line numbers cited per field); colours given there as 16-bit sRGB are divided by 65535 as
+toRGBFloat does.  Only parameters on the raymarch path are represented.
+"""
+import math
+import struct
+
+from . import abi
+
+
+def _u16(rgb16):
+    return [c / 65535.0 for c in rgb16]
+
+
+def default_params(width=640, height=480):
+    """Reference defaults (initparameters.cpp) for the fields of mbx_params."""
+    p = abi.MbxParams()
+    p.imageWidth = width
+    p.imageHeight = height
+    p.perspectiveType = 0                      # three-point (initparameters.cpp:121)
+    p.maxRaymarchingSteps = 10000              # :167
+    p.minN = 1                                 # :131
+    p.iterThreshMode = 0                       # :137
+    p.interiorMode = 0                         # :147
+    p.constantDEThreshold = 0                  # :148
+    abi.set3(p.camera, (3.0, -6.0, 2.0))       # :113
+    abi.set3(p.target, (0.0, 0.0, 0.0))        # :114
+    abi.set3(p.topVector, (-0.1277753, 0.2555506, 0.958314))  # :116
+    p.fov = 53.13                              # :120
+    p.sweetSpotHAngle = 0.0                    # :124
+    p.sweetSpotVAngle = 0.0                    # :125
+    p.detailLevel = 1.0                        # :133
+    p.DEThresh = 0.01                          # :135
+    p.DEFactor = 1.0                           # :139
+    p.smoothness = 1.0                         # :136
+    p.viewDistanceMax = 50.0                   # :141
+    # material #1 (initparameters.cpp:1640-1781)
+    abi.set3(p.surfaceColor, _u16((50000, 50000, 50000)))
+    p.shading = 1.0
+    p.luminosity = 0.0
+    abi.set3(p.luminosityColor, (1.0, 1.0, 1.0))
+    abi.set3(p.fillLightColor, (0.0, 0.0, 0.0))  # :309
+    # light #1 (initparameters.cpp:1841-1960)
+    L = p.light
+    L.enabled = 1
+    L.castShadows = 1
+    L.penetrating = 1
+    L.relativePosition = 1
+    abi.set3(L.rotation, (-45.0, 45.0, 0.0))
+    L.intensity = 1.0
+    L.visibility = 1.0
+    L.size = 0.5
+    L.softShadowCone = 1.0
+    L.contourSharpness = 1.0
+    abi.set3(L.color, (1.0, 1.0, 1.0))
+    # ambient occlusion (initparameters.cpp:213-219)
+    p.ambientOcclusionEnabled = 0
+    p.ambientOcclusionMode = abi.MBX_AO_MULTIPLE_RAYS
+    p.ambientOcclusionQuality = 4
+    p.aoVectorsCount = 0
+    p.ambientOcclusion = 1.0
+    abi.set3(p.ambientOcclusionColor, (1.0, 1.0, 1.0))
+    p.ambientOcclusionFastTune = 1.0
+    # background (initparameters.cpp:227-245)
+    p.background3ColorsEnable = 1
+    abi.set3(p.backgroundColor1, _u16((0, 38306, 65535)))
+    abi.set3(p.backgroundColor2, _u16((65535, 65535, 65535)))
+    abi.set3(p.backgroundColor3, _u16((0, 10000, 500)))
+    p.backgroundBrightness = 1.0
+    p.backgroundGamma = 1.0
+    # Monte Carlo (initparameters.cpp:220-231, 128)
+    p.DOFMonteCarlo = 0
+    p.DOFSamples = 100
+    p.DOFMonteCarloGlobalIllumination = 0
+    p.reflectionsMax = 5
+    p.monteCarloGIRadianceLimit = 10.0
+    p.rngMode = abi.MBX_RNG_PER_PIXEL
+    p.randomSeed = 0
+    # fractal #1 (initparameters.cpp:713-766)
+    f = p.fractal
+    f.formula = abi.MBX_FORMULA_MANDELBULB
+    f.formulaMaxiter = 250
+    f.deltaDEMethod = 0
+    f.deltaDEFunction = 0
+    f.useDefaultBailout = 1
+    f.juliaMode = 0
+    f.dontAddCConstant = 0
+    f.bailout = 100.0
+    abi.set3(f.juliaC, (0.0, 0.0, 0.0))
+    abi.set3(f.constantFactor, (1.0, 1.0, 1.0))
+    f.initialWAxis = 0.0
+    f.bulb.power = 9.0
+    f.bulb.alphaAngleOffset = 0.0
+    f.bulb.betaAngleOffset = 0.0
+    mb = f.mandelbox
+    mb.scale = 2.0
+    mb.foldingLimit = 1.0
+    mb.foldingValue = 2.0
+    mb.foldingSphericalMin = 0.5
+    mb.foldingSphericalFixed = 1.0
+    abi.set3(mb.colorFactor, (0.03, 0.05, 0.07))
+    mb.colorFactorSp1 = 0.2
+    mb.colorFactorSp2 = 0.2
+    ifs = f.IFS
+    ifs.scale = 2.0
+    abi.set3(ifs.offset, (1.0, 0.0, 0.0))
+    for i in range(abi.MBX_IFS_VECTOR_COUNT):
+        abi.set3(ifs.direction[i], (1.0, 0.0, 0.0))
+        ifs.distance[i] = 0.0
+        ifs.intensity[i] = 1.0
+        ifs.enabled[i] = 0
+    return p
+
+
+# ------------------------------------------------------------------------------------------
+# workloads
+# ------------------------------------------------------------------------------------------
+
+def mandelbulb8(width=640, height=480, seed=0):
+    """Plain CPU-runnable case: Mandelbulb power 8, analytic log DE, bailout 10, 250 iterations,
+    DE factor 1, detail 1, camera at distance 3.5 looking at the origin, FOV 53 degrees, one
+    directional light without shadows, Lambert shading, no AO/GI/glow, 1 sample per pixel.
+    The 1920x1080 variant is the headline throughput workload."""
+    p = default_params(width, height)
+    abi.set3(p.camera, (0.0, -3.5, 0.0))
+    abi.set3(p.target, (0.0, 0.0, 0.0))
+    abi.set3(p.topVector, (0.0, 0.0, 1.0))
+    p.fov = 53.0
+    p.light.castShadows = 0
+    p.randomSeed = seed
+    f = p.fractal
+    f.formula = abi.MBX_FORMULA_MANDELBULB
+    f.bulb.power = 8.0
+    f.formulaMaxiter = 250
+    return p
+
+
+def splitmix64(state):
+    """SplitMix64 step: returns (value, new_state)."""
+    state = (state + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return z ^ (z >> 31), state
+
+
+class SeededStream:
+    """Deterministic uniforms / normals from SplitMix64 (Box-Muller for normals)."""
+
+    def __init__(self, seed):
+        self.state = seed & 0xFFFFFFFFFFFFFFFF
+
+    def uniform(self, a=0.0, b=1.0):
+        v, self.state = splitmix64(self.state)
+        return a + (b - a) * ((v >> 11) * (1.0 / 9007199254740992.0))
+
+    def normal(self):
+        u1 = max(self.uniform(), 1e-300)
+        u2 = self.uniform()
+        return math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+
+
+def fibonacci_hemisphere(n):
+    """n unit directions on the upper (z >= 0) hemisphere by the Fibonacci spiral."""
+    golden = math.pi * (3.0 - math.sqrt(5.0))
+    out = []
+    for i in range(n):
+        z = 1.0 - (i + 0.5) / n
+        r = math.sqrt(max(0.0, 1.0 - z * z))
+        phi = golden * i
+        out.append((r * math.cos(phi), r * math.sin(phi), z))
+    return out
+
+
+def mandelbox_shadow_ao(width=1920, height=1080, seed=1, scale=2.0):
+    """Mandelbox (folding limit 1, min radius 0.5, fixed radius 1), delta DE with the linear DE
+    function, 50 iterations; hard shadows from the directional light and 16-ray AO with a fixed
+    Fibonacci-hemisphere direction set; camera on a seeded sphere of radius 6-10 looking at the
+    origin.  scale 2.0 for the first run, -1.5 for the second."""
+    p = default_params(width, height)
+    rs = SeededStream(seed)
+    d = [rs.normal() for _ in range(3)]
+    n = math.sqrt(sum(c * c for c in d))
+    radius = rs.uniform(6.0, 10.0)
+    abi.set3(p.camera, [c / n * radius for c in d])
+    abi.set3(p.target, (0.0, 0.0, 0.0))
+    abi.set3(p.topVector, (0.0, 0.0, 1.0))
+    p.light.castShadows = 1
+    p.light.softShadowCone = 0.0
+    p.light.penetrating = 1
+    p.ambientOcclusionEnabled = 1
+    p.ambientOcclusionMode = abi.MBX_AO_MULTIPLE_RAYS
+    dirs = fibonacci_hemisphere(16)
+    p.aoVectorsCount = len(dirs)
+    for i, v in enumerate(dirs):
+        abi.set3(p.aoVectors[i], v)
+        abi.set3(p.aoColors[i], (1.0, 1.0, 1.0))
+    p.randomSeed = seed
+    f = p.fractal
+    f.formula = abi.MBX_FORMULA_MANDELBOX
+    f.formulaMaxiter = 50
+    f.deltaDEMethod = 1
+    f.deltaDEFunction = 1
+    f.mandelbox.scale = scale
+    f.mandelbox.foldingLimit = 1.0
+    f.mandelbox.foldingValue = 2.0
+    f.mandelbox.foldingSphericalMin = 0.5
+    f.mandelbox.foldingSphericalFixed = 1.0
+    return p
+
+
+def menger_ifs(width=320, height=240, seed=2):
+    """Kaleidoscopic IFS in Menger-sponge mode (abs folds, scale 3, offset (1,1,1)) as in
+    mandelbulber2/deploy/share/mandelbulber2/examples/IFS_001.fract:38-49."""
+    p = default_params(width, height)
+    abi.set3(p.camera, (2.2, -2.6, 1.7))
+    abi.set3(p.target, (0.0, 0.0, 0.0))
+    abi.set3(p.topVector, (0.0, 0.0, 1.0))
+    p.light.castShadows = 0
+    p.randomSeed = seed
+    f = p.fractal
+    f.formula = abi.MBX_FORMULA_KALEIDOSCOPIC_IFS
+    f.formulaMaxiter = 30
+    ifs = f.IFS
+    ifs.scale = 3.0
+    abi.set3(ifs.offset, (1.0, 1.0, 1.0))
+    ifs.absX = ifs.absY = ifs.absZ = 1
+    ifs.mengerSpongeMode = 1
+    return p
+
+
+def params_bytes(p):
+    return bytes(memoryview(p))
+
+
+def copy_params(p):
+    q = abi.MbxParams()
+    struct_bytes = params_bytes(p)
+    abi.C.memmove(abi.C.addressof(q), struct_bytes, len(struct_bytes))
+    return q

@@ -1,0 +1,166 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg.  The product package never imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from mandelbulber2_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+_libs = {}
+
+
+def build():
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", HERE, "all"])
+
+
+def lib(fast=False):
+    name = "liboracle_fast.so" if fast else "liboracle.so"
+    if name in _libs:
+        return _libs[name]
+    path = os.path.join(HERE, name)
+    if not os.path.exists(path):
+        build()
+    L = C.CDLL(path)
+    P = C.POINTER
+    L.mbxo_render.argtypes = [P(abi.MbxParams), C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.mbxo_render.restype = C.c_int
+    L.mbxo_distance.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_int, C.c_double, C.c_int,
+                                C.c_void_p, C.c_void_p]
+    L.mbxo_compute.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                               C.c_void_p, C.c_void_p]
+    L.mbxo_formula_step.argtypes = [P(abi.MbxParams), C.c_int, C.c_void_p, C.c_void_p]
+    L.mbxo_frame_constants.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_void_p, C.c_void_p]
+    L.mbxo_march_pixel.argtypes = [P(abi.MbxParams), C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.mbxo_normal.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_double, C.c_void_p]
+    L.mbxo_random_int.argtypes = [P(C.c_uint32)]
+    L.mbxo_random.argtypes = [P(C.c_uint32), C.c_int]
+    L.mbxo_pixel_seed.argtypes = [C.c_uint32, C.c_uint32]
+    L.mbxo_pixel_seed.restype = C.c_uint32
+    L.mbxo_sizeof_params.restype = C.c_size_t
+    L.mbxo_offsetof_fractal.restype = C.c_size_t
+    _libs[name] = L
+    return L
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def render(p, threads=None, rows=None, fast=False):
+    """Render the whole image (or rows=(y0, y1)).  Returns a dict of numpy planes and stats."""
+    L = lib(fast)
+    W, H = p.imageWidth, p.imageHeight
+    y0, y1 = rows if rows else (0, H)
+    n = W * H
+    rgb = np.zeros((n, 3), np.float32)
+    depth = np.zeros(n, np.float32)
+    iters = np.zeros(n, np.int32)
+    mask = np.zeros(n, np.uint8)
+    normal = np.zeros((n, 3), np.float32)
+    steps = np.zeros(n, np.int32)
+    stats = np.zeros(3, np.uint64)
+    if threads is None:
+        threads = os.cpu_count() or 1
+    rc = L.mbxo_render(C.byref(p), y0, y1, threads, _ptr(rgb), _ptr(depth), _ptr(iters), _ptr(mask),
+                       _ptr(normal), _ptr(steps), _ptr(stats))
+    if rc != 0:
+        raise RuntimeError("oracle render failed: %d" % rc)
+    sl = slice(y0 * W, y1 * W)
+    return {"rgb": rgb[sl].reshape(-1, W, 3), "depth": depth[sl].reshape(-1, W),
+            "iters": iters[sl].reshape(-1, W), "mask": mask[sl].reshape(-1, W),
+            "normal": normal[sl].reshape(-1, W, 3), "steps": steps[sl].reshape(-1, W),
+            "rays": int(stats[0]), "de_evals": int(stats[1]), "iterations": int(stats[2])}
+
+
+def distance(p, pts, detail_size, normal_calc=False):
+    L = lib()
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    d = np.zeros(len(pts), np.float64)
+    it = np.zeros(len(pts), np.int32)
+    rc = L.mbxo_distance(C.byref(p), _ptr(pts), len(pts), float(detail_size), int(normal_calc),
+                         _ptr(d), _ptr(it))
+    if rc != 0:
+        raise RuntimeError("oracle distance failed: %d" % rc)
+    return d, it
+
+
+def compute(p, pts):
+    L = lib()
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    n = len(pts)
+    d = np.zeros(n, np.float64)
+    it = np.zeros(n, np.int32)
+    mx = np.zeros(n, np.int32)
+    z = np.zeros((n, 3), np.float64)
+    rc = L.mbxo_compute(C.byref(p), _ptr(pts), n, _ptr(d), _ptr(it), _ptr(mx), _ptr(z))
+    if rc != 0:
+        raise RuntimeError("oracle compute failed: %d" % rc)
+    return d, it, mx, z
+
+
+def formula_step(p, z4, aux3):
+    L = lib()
+    z4 = np.array(z4, np.float64).reshape(-1, 4).copy()
+    aux3 = np.array(aux3, np.float64).reshape(-1, 3).copy()
+    rc = L.mbxo_formula_step(C.byref(p), len(z4), _ptr(z4), _ptr(aux3))
+    if rc != 0:
+        raise RuntimeError("oracle formula_step failed: %d" % rc)
+    return z4, aux3
+
+
+def frame_constants(p):
+    L = lib()
+    m = np.zeros(9, np.float64)
+    fov = np.zeros(1, np.float64)
+    light = np.zeros(3, np.float64)
+    rc = L.mbxo_frame_constants(C.byref(p), _ptr(m), _ptr(fov), _ptr(light))
+    if rc != 0:
+        raise RuntimeError("oracle frame_constants failed: %d" % rc)
+    return m.reshape(3, 3), float(fov[0]), light
+
+
+def march_pixel(p, x, y):
+    L = lib()
+    d = np.zeros(3, np.float64)
+    depth = np.zeros(1, np.float64)
+    thr = np.zeros(1, np.float64)
+    found = np.zeros(1, np.int32)
+    iters = np.zeros(1, np.int32)
+    pt = np.zeros(3, np.float64)
+    rc = L.mbxo_march_pixel(C.byref(p), x, y, _ptr(d), _ptr(depth), _ptr(thr), _ptr(found),
+                            _ptr(iters), _ptr(pt))
+    if rc != 0:
+        raise RuntimeError("oracle march failed: %d" % rc)
+    return {"dir": d, "depth": float(depth[0]), "thr": float(thr[0]), "found": bool(found[0]),
+            "iters": int(iters[0]), "point": pt}
+
+
+def normal(p, pt, dist_thresh):
+    L = lib()
+    pt = np.ascontiguousarray(pt, np.float64)
+    n = np.zeros(3, np.float64)
+    rc = L.mbxo_normal(C.byref(p), _ptr(pt), float(dist_thresh), _ptr(n))
+    if rc != 0:
+        raise RuntimeError("oracle normal failed: %d" % rc)
+    return n
+
+
+def random_ints(seed, count, maxv=None):
+    L = lib()
+    s = C.c_uint32(seed)
+    if maxv is None:
+        return [L.mbxo_random_int(C.byref(s)) for _ in range(count)], s.value
+    return [L.mbxo_random(C.byref(s), maxv) for _ in range(count)], s.value
+
+
+def pixel_seed(pixel, frame_seed):
+    return lib().mbxo_pixel_seed(pixel, frame_seed)

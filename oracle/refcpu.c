@@ -1,0 +1,1634 @@
+/*
+ * refcpu.c - FP64 CPU restatement of Mandelbulber2's distance-estimation raymarch path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity oracle and the CPU baseline: it is
+ * compiled into oracle/liboracle.so and loaded only by tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg.  The product library (libmbx.so) never links or calls it.
+ *
+ * Every function restates the reference C++ in plain C, in double precision (the reference
+ * CPU path is FP64 throughout: CVector3/CVector4 are double, mandelbulber2/src/algebra.hpp:58,290),
+ * keeping the reference's order of operations so that FP64 results agree bit for bit where
+ * the same libm is used.  Each function cites the reference file:line it follows.
+ *
+ * Scope: one single-formula fractal node (identity transform, scale 1), formulas Mandelbulb,
+ * Mandelbox and Kaleidoscopic IFS, analytic and delta DE, the RayMarching coarse loop and
+ * binary search, 6-tap normals, ObjectShader with one directional light (Lambert, no
+ * specular), hard AuxShadow, multi-ray and fast AO, the 3-colour background with the sun
+ * disk, Monte Carlo sub-pixel sampling and the GI bounce loop.
+ *
+ * Pinning (see tests/): formulas against vectors produced by the reference's own formula
+ * sources (oracle/_ref); Park-Miller against the reference's common_math.cpp; Compute against
+ * the 2D Multibrot closed form on the z=0 plane and the Mandelbox linear regime; the march
+ * against dense FP64 ray sampling; normals against a high-order finite difference of the DE;
+ * camera against the closed-form orthonormal frame.  Shading functions (light, shadow, AO,
+ * background, GI) are restatements checked by invariants only: parity unpinned beyond them.
+ */
+#include <math.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/mbx.h"
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+#define M_PI_180 0.01745329251994329576923690768489
+
+/* ------------------------------------------------------------------------------------------ */
+/* vectors: CVector3 / CVector4 (mandelbulber2/src/algebra.hpp:58-260, 290-460)                */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct { double x, y, z; } v3;
+typedef struct { double x, y, z, w; } v4;
+
+static inline v3 V3(double x, double y, double z) { v3 r = {x, y, z}; return r; }
+static inline v4 V4(double x, double y, double z, double w) { v4 r = {x, y, z, w}; return r; }
+static inline v3 v3_add(v3 a, v3 b) { return V3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 v3_sub(v3 a, v3 b) { return V3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 v3_mul(v3 a, double s) { return V3(a.x * s, a.y * s, a.z * s); }
+static inline double v3_dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline double v3_len(v3 a) { return sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+/* algebra.hpp:188-195 */
+static inline v3 v3_cross(v3 a, v3 v)
+{
+	return V3(a.y * v.z - a.z * v.y, -a.x * v.z + a.z * v.x, a.x * v.y - a.y * v.x);
+}
+/* algebra.hpp:204-211: multiply by the reciprocal of the length */
+static inline v3 v3_normalize(v3 a)
+{
+	const double norm = 1.0 / v3_len(a);
+	return V3(a.x * norm, a.y * norm, a.z * norm);
+}
+/* algebra.hpp:214-221 */
+static inline int v3_notfinite(v3 a) { return !(isfinite(a.x) & isfinite(a.y) & isfinite(a.z)); }
+static inline int v3_eq(v3 a, v3 b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
+
+static inline v4 v4_add(v4 a, v4 b) { return V4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+static inline v4 v4_sub(v4 a, v4 b) { return V4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w); }
+static inline v4 v4_mul(v4 a, double s) { return V4(a.x * s, a.y * s, a.z * s, a.w * s); }
+static inline double v4_dot(v4 a, v4 b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
+/* algebra.hpp:401: the 4D length includes w */
+static inline double v4_len(v4 a) { return sqrt(a.x * a.x + a.y * a.y + a.z * a.z + a.w * a.w); }
+/* algebra.hpp:444-451 */
+static inline int v4_notfinite(v4 a)
+{
+	return !(isfinite(a.x) & isfinite(a.y) & isfinite(a.z) & isfinite(a.w));
+}
+
+/* CVector3::RotateAroundVectorByAngle (mandelbulber2/src/algebra.cpp:71-77) */
+static v3 v3_rotate_around(v3 v, v3 axis, double angle)
+{
+	v3 r = v3_mul(v, cos(angle));
+	r = v3_add(r, v3_mul(v3_cross(axis, v), sin(angle)));
+	r = v3_add(r, v3_mul(v3_mul(axis, v3_dot(axis, v)), 1 - cos(angle)));
+	return r;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* CRotationMatrix (mandelbulber2/src/algebra.cpp:178-410)                                     */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct { double m[9]; int zero; } rotm; /* row-major m11..m33 */
+
+static void rot_null(rotm *r)
+{
+	memset(r->m, 0, sizeof(r->m));
+	r->m[0] = r->m[4] = r->m[8] = 1.0;
+	r->zero = 1;
+}
+
+/* CMatrix33::operator* (algebra.cpp:153-166): matrix = matrix * rot */
+static void rot_mul_right(rotm *r, const double b[9])
+{
+	const double *a = r->m;
+	double c[9];
+	c[0] = a[0] * b[0] + a[1] * b[3] + a[2] * b[6];
+	c[1] = a[0] * b[1] + a[1] * b[4] + a[2] * b[7];
+	c[2] = a[0] * b[2] + a[1] * b[5] + a[2] * b[8];
+	c[3] = a[3] * b[0] + a[4] * b[3] + a[5] * b[6];
+	c[4] = a[3] * b[1] + a[4] * b[4] + a[5] * b[7];
+	c[5] = a[3] * b[2] + a[4] * b[5] + a[5] * b[8];
+	c[6] = a[6] * b[0] + a[7] * b[3] + a[8] * b[6];
+	c[7] = a[6] * b[1] + a[7] * b[4] + a[8] * b[7];
+	c[8] = a[6] * b[2] + a[7] * b[5] + a[8] * b[8];
+	memcpy(r->m, c, sizeof(c));
+	r->zero = 0;
+}
+
+/* algebra.cpp:198-247 */
+static void rot_x(rotm *r, double angle)
+{
+	if (angle != 0.0)
+	{
+		const double s = sin(angle), c = cos(angle);
+		double b[9] = {1.0, 0, 0, 0, c, -s, 0, s, c};
+		rot_mul_right(r, b);
+	}
+}
+static void rot_y(rotm *r, double angle)
+{
+	if (angle != 0.0)
+	{
+		const double s = sin(angle), c = cos(angle);
+		double b[9] = {c, 0, s, 0, 1.0, 0, -s, 0, c};
+		rot_mul_right(r, b);
+	}
+}
+static void rot_z(rotm *r, double angle)
+{
+	if (angle != 0.0)
+	{
+		const double s = sin(angle), c = cos(angle);
+		double b[9] = {c, -s, 0, s, c, 0, 0, 0, 1.0};
+		rot_mul_right(r, b);
+	}
+}
+/* algebra.cpp:262-268 SetRotation(CVector3): Z(x), X(y), Y(z) */
+static void rot_set1(rotm *r, v3 a) { rot_null(r); rot_z(r, a.x); rot_x(r, a.y); rot_y(r, a.z); }
+/* algebra.cpp:297-303 SetRotation2: Z(z), Y(y), X(x) */
+static void rot_set2(rotm *r, v3 a) { rot_null(r); rot_z(r, a.z); rot_y(r, a.y); rot_x(r, a.x); }
+/* algebra.cpp:305-311 SetRotation3: Z(x), Y(y), X(z) */
+static void rot_set3(rotm *r, v3 a) { rot_null(r); rot_z(r, a.x); rot_y(r, a.y); rot_x(r, a.z); }
+/* algebra.cpp:398-412 */
+static rotm rot_transpose(const rotm *r)
+{
+	rotm t;
+	const double *a = r->m;
+	double b[9] = {a[0], a[3], a[6], a[1], a[4], a[7], a[2], a[5], a[8]};
+	memcpy(t.m, b, sizeof(b));
+	t.zero = 0;
+	return t;
+}
+/* algebra.cpp:320-344, CMatrix33 * CVector3 (algebra.cpp:168-175) */
+static v3 rot_apply3(const rotm *r, v3 v)
+{
+	if (r->zero) return v;
+	const double *a = r->m;
+	return V3(a[0] * v.x + a[1] * v.y + a[2] * v.z, a[3] * v.x + a[4] * v.y + a[5] * v.z,
+		a[6] * v.x + a[7] * v.y + a[8] * v.z);
+}
+static v4 rot_apply4(const rotm *r, v4 v)
+{
+	if (r->zero) return v;
+	v3 xyz = rot_apply3(r, V3(v.x, v.y, v.z));
+	return V4(xyz.x, xyz.y, xyz.z, v.w);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Park-Miller RNG (mandelbulber2/src/common_math.cpp:47-72)                                   */
+/* ------------------------------------------------------------------------------------------ */
+int mbxo_random_int(uint32_t *seed)
+{
+	uint32_t lo = 16807u * (*seed & 0xFFFFu);
+	uint32_t hi = 16807u * (*seed >> 16);
+	lo += (hi & 0x7FFFu) << 16;
+	hi >>= 15;
+	lo += hi;
+	if (lo > 2147483647u) lo -= 2147483647u;
+	*seed = lo;
+	return (int)lo;
+}
+
+int mbxo_random(uint32_t *seed, int max) { return mbxo_random_int(seed) % (max + 1); }
+
+/* Per-pixel stream seed shared with the GPU kernels: an integer mix of (pixel, frame seed)
+ * folded into Park-Miller's state range [1, 2^31-2].  Project-defined (the reference shares one
+ * racy global seed between threads, common_math.cpp:47). */
+uint32_t mbxo_pixel_seed(uint32_t pixel_index, uint32_t frame_seed)
+{
+	uint32_t h = pixel_index * 0x9E3779B1u ^ (frame_seed + 0x7F4A7C15u);
+	h ^= h >> 16;
+	h *= 0x7FEB352Du;
+	h ^= h >> 15;
+	h *= 0x846CA68Bu;
+	h ^= h >> 16;
+	return 1u + h % 2147483646u;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* scene: sParamRender + cHybridFractalSequences::sSequence subset, derived on the host        */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct
+{
+	const mbx_params *p;
+	/* camera (render_worker.cpp:539-564, fractparams.cpp:145, render_job.cpp:477) */
+	rotm mRot;
+	v3 camera;
+	double fov;        /* CalcFOV: 2 tan(deg/360 pi) */
+	double resolution; /* 1 / imageHeight */
+	double aspectRatio;
+	/* sequence (hybrid_fractal_sequences.cpp:181-287, 377-503) */
+	int formula;
+	int formulaMaxiter;
+	int DEType;
+	int DEFunctionType;
+	int DEAnalyticFunction;
+	int addCConstant;
+	int juliaEnabled;
+	double bailout;
+	v3 juliaConstant;
+	v3 constantMultiplier;
+	double initialWAxis;
+	/* sFractal after RecalculateFractalParams (fractal.cpp:653-678) */
+	double bulbPower, bulbAlpha, bulbBeta;
+	double mbScale, mbFoldingLimit, mbFoldingValue, mbFR2, mbMR2, mbFactor1, mbSp1, mbSp2;
+	v4 mbOffset;
+	v3 mbColorFactor;
+	int mbRotationsEnabled, mbMainRotationEnabled;
+	rotm mbMainRot, mbRot[2][3], mbRotInv[2][3];
+	double ifsScale;
+	v4 ifsOffset;
+	v3 ifsEdge;
+	rotm ifsMainRot, ifsRot[MBX_IFS_VECTOR_COUNT];
+	v4 ifsDirection[MBX_IFS_VECTOR_COUNT];
+	double ifsDistance[MBX_IFS_VECTOR_COUNT], ifsIntensity[MBX_IFS_VECTOR_COUNT];
+	int ifsEnabled[MBX_IFS_VECTOR_COUNT];
+	int ifsAbsX, ifsAbsY, ifsAbsZ, ifsMenger, ifsRotationEnabled, ifsEdgeEnabled;
+	/* light #1 (light.cpp:51-195) */
+	v3 lightDirection;
+	double lightIntensity, lightVisibility, lightSize, lightSoftShadowCone, lightContourSharpness;
+	float lightColor[3];
+	/* AO */
+	int aoCount;
+} scene_t;
+
+/* cCameraTarget::SetCameraTargetTop (mandelbulber2/src/camera_target.cpp:86-126) */
+typedef struct { v3 forward, top, right; double yaw, pitch, roll; } camtarget_t;
+
+static double correct_angle(double angle) { return fmod(angle + 3 * M_PI, 2 * M_PI) - M_PI; }
+
+static camtarget_t camera_target_top(v3 camera, v3 target, v3 top)
+{
+	camtarget_t c;
+	v3 f = v3_eq(camera, target) ? V3(0.0, 1.0, 0.0) : v3_sub(target, camera);
+	f = v3_normalize(f);
+	c.forward = f;
+	double yaw = atan2(f.y, f.x) - 0.5 * M_PI;                /* GetAlpha, algebra.hpp:212 */
+	double pitch = atan2(f.z, sqrt(f.x * f.x + f.y * f.y)); /* GetBeta, algebra.hpp:213 */
+	v3 t = v3_normalize(top);
+	t = v3_rotate_around(t, V3(0.0, 0.0, 1.0), -yaw);
+	t = v3_rotate_around(t, V3(1.0, 0.0, 0.0), -pitch);
+	double roll = -atan2(t.z, t.x) + 0.5 * M_PI;
+	yaw = correct_angle(yaw);
+	pitch = correct_angle(pitch);
+	roll = correct_angle(roll);
+	v3 tv = V3(0.0, 0.0, 1.0);
+	tv = v3_rotate_around(tv, V3(0.0, 1.0, 0.0), roll);
+	tv = v3_rotate_around(tv, V3(1.0, 0.0, 0.0), pitch);
+	tv = v3_rotate_around(tv, V3(0.0, 0.0, 1.0), yaw);
+	c.top = tv;
+	c.right = v3_cross(f, tv);
+	c.yaw = yaw;
+	c.pitch = pitch;
+	c.roll = roll;
+	return c;
+}
+
+static v3 v3_from(const double a[3]) { return V3(a[0], a[1], a[2]); }
+
+/* Derive the per-frame constants.  Returns 0 on success, <0 on an unsupported setting. */
+static int scene_init(scene_t *s, const mbx_params *p)
+{
+	memset(s, 0, sizeof(*s));
+	s->p = p;
+	if (p->imageWidth <= 0 || p->imageHeight <= 0) return MBX_ERR_INVALID;
+	if (p->perspectiveType != 0) return MBX_ERR_UNSUPPORTED;
+
+	/* PrepareMainVectors (render_worker.cpp:539-564) */
+	camtarget_t ct = camera_target_top(v3_from(p->camera), v3_from(p->target), v3_from(p->topVector));
+	rot_null(&s->mRot);
+	rot_z(&s->mRot, ct.yaw);
+	rot_x(&s->mRot, ct.pitch);
+	rot_y(&s->mRot, ct.roll);
+	rot_z(&s->mRot, -(p->sweetSpotHAngle / 180.0 * M_PI));
+	rot_x(&s->mRot, p->sweetSpotVAngle / 180.0 * M_PI);
+	s->camera = v3_from(p->camera);
+	s->fov = 2.0 * tan(p->fov / 360.0 * M_PI); /* CalcFOV (projection_3d.cpp:140-159) */
+	s->resolution = 1.0 / p->imageHeight;      /* render_job.cpp:477 */
+	s->aspectRatio = (double)p->imageWidth / p->imageHeight;
+
+	/* sequence data for one formula (hybrid_fractal_sequences.cpp:181-287, 471-503) */
+	const mbx_fractal *f = &p->fractal;
+	s->formula = f->formula;
+	s->formulaMaxiter = f->formulaMaxiter;
+	double defaultBailout;
+	int cpixelDisabled = 0;
+	switch (f->formula)
+	{
+		case MBX_FORMULA_MANDELBULB: /* fractal_mandelbulb.cpp:16-27 */
+			s->DEType = MBX_DE_ANALYTIC;
+			s->DEFunctionType = MBX_DE_FUNCTION_LOGARITHMIC;
+			s->DEAnalyticFunction = MBX_ANALYTIC_LOGARITHMIC;
+			defaultBailout = 10.0;
+			break;
+		case MBX_FORMULA_MANDELBOX: /* fractal_mandelbox.cpp:17-29 */
+			s->DEType = MBX_DE_ANALYTIC;
+			s->DEFunctionType = MBX_DE_FUNCTION_LINEAR;
+			s->DEAnalyticFunction = MBX_ANALYTIC_LINEAR;
+			defaultBailout = 100.0;
+			break;
+		case MBX_FORMULA_KALEIDOSCOPIC_IFS: /* fractal_kaleidoscopic_ifs.cpp:17-28 */
+			s->DEType = MBX_DE_ANALYTIC;
+			s->DEFunctionType = MBX_DE_FUNCTION_LINEAR;
+			s->DEAnalyticFunction = MBX_ANALYTIC_IFS;
+			defaultBailout = 100.0;
+			cpixelDisabled = 1;
+			break;
+		default: return MBX_ERR_UNSUPPORTED;
+	}
+	if (f->deltaDEMethod == 1) s->DEType = MBX_DE_DELTA;
+	if (f->deltaDEMethod == 2) s->DEType = MBX_DE_ANALYTIC;
+	if (f->deltaDEFunction != 0)
+	{
+		s->DEFunctionType = f->deltaDEFunction;
+		if (f->deltaDEFunction == MBX_DE_FUNCTION_LOGARITHMIC)
+			s->DEAnalyticFunction = MBX_ANALYTIC_LOGARITHMIC;
+		else if (f->deltaDEFunction == MBX_DE_FUNCTION_LINEAR)
+			s->DEAnalyticFunction = MBX_ANALYTIC_LINEAR;
+		else
+			return MBX_ERR_UNSUPPORTED;
+	}
+	int addc = !f->dontAddCConstant;
+	if (cpixelDisabled) addc = !addc;
+	s->addCConstant = addc;
+	s->bailout = f->useDefaultBailout ? defaultBailout : f->bailout;
+	s->juliaEnabled = f->juliaMode;
+	s->juliaConstant = v3_from(f->juliaC);
+	s->constantMultiplier = v3_from(f->constantFactor);
+	s->initialWAxis = f->initialWAxis;
+
+	/* RecalculateFractalParams (fractal.cpp:653-678) */
+	s->bulbPower = f->bulb.power;
+	s->bulbAlpha = f->bulb.alphaAngleOffset * M_PI_180;
+	s->bulbBeta = f->bulb.betaAngleOffset * M_PI_180;
+
+	const mbx_fractal_mandelbox *mb = &f->mandelbox;
+	s->mbScale = mb->scale;
+	s->mbFoldingLimit = mb->foldingLimit;
+	s->mbFoldingValue = mb->foldingValue;
+	s->mbFR2 = mb->foldingSphericalFixed * mb->foldingSphericalFixed;
+	s->mbMR2 = mb->foldingSphericalMin * mb->foldingSphericalMin;
+	s->mbFactor1 = s->mbFR2 / s->mbMR2;
+	s->mbOffset = V4(mb->offset[0], mb->offset[1], mb->offset[2], 0.0);
+	s->mbColorFactor = v3_from(mb->colorFactor);
+	s->mbSp1 = mb->colorFactorSp1;
+	s->mbSp2 = mb->colorFactorSp2;
+	s->mbRotationsEnabled = mb->rotationsEnabled;
+	s->mbMainRotationEnabled = mb->mainRotationEnabled;
+	rot_set2(&s->mbMainRot, v3_mul(v3_from(mb->rotationMain), M_PI_180));
+	for (int fold = 0; fold < MBX_MANDELBOX_FOLDS; fold++)
+		for (int axis = 0; axis < 3; axis++)
+		{
+			rot_set2(&s->mbRot[fold][axis], v3_mul(v3_from(mb->rotation[fold][axis]), M_PI_180));
+			s->mbRotInv[fold][axis] = rot_transpose(&s->mbRot[fold][axis]);
+		}
+
+	const mbx_fractal_ifs *ifs = &f->IFS;
+	s->ifsScale = ifs->scale;
+	s->ifsOffset = V4(ifs->offset[0], ifs->offset[1], ifs->offset[2], 0.0);
+	s->ifsEdge = v3_from(ifs->edge);
+	rot_set3(&s->ifsMainRot, v3_mul(v3_from(ifs->rotation), M_PI_180));
+	for (int i = 0; i < MBX_IFS_VECTOR_COUNT; i++)
+	{
+		rot_set3(&s->ifsRot[i], v3_mul(v3_from(ifs->rotations[i]), M_PI_180));
+		/* CVector4::Normalize includes w (= 0) */
+		v4 d = V4(ifs->direction[i][0], ifs->direction[i][1], ifs->direction[i][2], 0.0);
+		double norm = 1.0 / v4_len(d);
+		s->ifsDirection[i] = V4(d.x * norm, d.y * norm, d.z * norm, d.w * norm);
+		s->ifsDistance[i] = ifs->distance[i];
+		s->ifsIntensity[i] = ifs->intensity[i];
+		s->ifsEnabled[i] = ifs->enabled[i];
+	}
+	s->ifsAbsX = ifs->absX;
+	s->ifsAbsY = ifs->absY;
+	s->ifsAbsZ = ifs->absZ;
+	s->ifsMenger = ifs->mengerSpongeMode;
+	s->ifsRotationEnabled = ifs->rotationEnabled;
+	s->ifsEdgeEnabled = ifs->edgeEnabled;
+
+	/* cLight::setParameters for a directional light (light.cpp:80-195) */
+	const mbx_light *L = &p->light;
+	/* light.cpp:90: rotation = deg / 180.8 * M_PI (the reference divides by 180.8);
+	 * light.cpp:92-95: a directional light flips the second angle */
+	const v3 rdeg = v3_from(L->rotation);
+	v3 rotation = V3(rdeg.x / 180.8 * M_PI, rdeg.y / 180.8 * M_PI, rdeg.z / 180.8 * M_PI);
+	rotation = V3(rotation.x * 1.0, rotation.y * -1.0, rotation.z * 1.0);
+	if (L->relativePosition)
+	{
+		/* light.cpp:130-148 (use_target_point false) */
+		v3 d = v3_mul(ct.forward, -1.0);
+		d = v3_rotate_around(d, ct.forward, rotation.z);
+		d = v3_rotate_around(d, ct.right, rotation.y);
+		d = v3_rotate_around(d, ct.top, rotation.x);
+		s->lightDirection = d;
+	}
+	else
+	{
+		/* light.cpp:181-185 */
+		rotm lr;
+		rot_set1(&lr, rotation);
+		s->lightDirection = rot_apply3(&lr, V3(0.0, -1.0, 0.0));
+	}
+	s->lightIntensity = L->intensity;
+	s->lightVisibility = L->visibility;
+	s->lightSize = L->size;
+	s->lightSoftShadowCone = L->softShadowCone / 180.0 * M_PI;
+	s->lightContourSharpness = L->contourSharpness;
+	for (int k = 0; k < 3; k++) s->lightColor[k] = (float)L->color[k];
+
+	s->aoCount = p->aoVectorsCount;
+	if (p->ambientOcclusionEnabled && p->ambientOcclusionMode == MBX_AO_MULTIPLE_RAYS)
+	{
+		if (s->aoCount < 1 || s->aoCount > MBX_MAX_AO_VECTORS) return MBX_ERR_INVALID;
+	}
+	return MBX_OK;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* formulas (FormulaCode)                                                                      */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct
+{
+	v4 c, const_c, old_z;
+	double pos_neg, r, DE, DE0, dist, pseudoKleinianDE, actualScale, actualScaleA, color,
+		colorHybrid, temp1000;
+	int i;
+} aux_t; /* sExtendedAux (mandelbulber2/src/fractal.h:77-104) */
+
+/* cFractalMandelbulb::FormulaCode (formula/definition/fractal_mandelbulb.cpp:29-43) */
+static void mandelbulb_iteration(const scene_t *s, v4 *z, aux_t *aux)
+{
+	const double th0 = asin(z->z / aux->r) + s->bulbBeta;
+	const double ph0 = atan2(z->y, z->x) + s->bulbAlpha;
+	double rp = pow(aux->r, s->bulbPower - 1.0);
+	const double th = th0 * s->bulbPower;
+	const double ph = ph0 * s->bulbPower;
+	const double cth = cos(th);
+	aux->DE = (rp * aux->DE) * s->bulbPower + 1.0;
+	rp *= aux->r;
+	z->x = cth * cos(ph) * rp;
+	z->y = cth * sin(ph) * rp;
+	z->z = sin(th) * rp;
+}
+
+static inline double sign_d(double v) { return (double)((0.0 < v) - (v < 0.0)); }
+
+/* cFractalMandelbox::FormulaCode (formula/definition/fractal_mandelbox.cpp:31-107) */
+static void mandelbox_iteration(const scene_t *s, v4 *zp, aux_t *aux)
+{
+	v4 z = *zp;
+	if (s->mbRotationsEnabled)
+	{
+		const double colP[3] = {s->mbColorFactor.x, s->mbColorFactor.y, s->mbColorFactor.z};
+		for (int dim = 0; dim < 3; dim++)
+		{
+			v4 zRot = rot_apply4(&s->mbRot[0][dim], z);
+			double *zr = (dim == 0) ? &zRot.x : ((dim == 1) ? &zRot.y : &zRot.z);
+			if (*zr > s->mbFoldingLimit)
+			{
+				*zr = s->mbFoldingValue - *zr;
+				z = rot_apply4(&s->mbRotInv[0][dim], zRot);
+				aux->color += colP[dim];
+			}
+			else
+			{
+				zRot = rot_apply4(&s->mbRot[1][dim], z);
+				zr = (dim == 0) ? &zRot.x : ((dim == 1) ? &zRot.y : &zRot.z);
+				if (*zr < -s->mbFoldingLimit)
+				{
+					*zr = -s->mbFoldingValue - *zr;
+					z = rot_apply4(&s->mbRotInv[1][dim], zRot);
+					aux->color += colP[dim];
+				}
+			}
+		}
+	}
+	else
+	{
+		if (fabs(z.x) > s->mbFoldingLimit)
+		{
+			z.x = sign_d(z.x) * s->mbFoldingValue - z.x;
+			aux->color += s->mbColorFactor.x;
+		}
+		if (fabs(z.y) > s->mbFoldingLimit)
+		{
+			z.y = sign_d(z.y) * s->mbFoldingValue - z.y;
+			aux->color += s->mbColorFactor.y;
+		}
+		if (fabs(z.z) > s->mbFoldingLimit)
+		{
+			z.z = sign_d(z.z) * s->mbFoldingValue - z.z;
+			aux->color += s->mbColorFactor.z;
+		}
+	}
+	const double r2 = v4_dot(z, z);
+	z = v4_add(z, s->mbOffset);
+	if (r2 < s->mbMR2)
+	{
+		z = v4_mul(z, s->mbFactor1);
+		aux->DE *= s->mbFactor1;
+		aux->color += s->mbSp1;
+	}
+	else if (r2 < s->mbFR2)
+	{
+		const double tglad_factor2 = s->mbFR2 / r2;
+		z = v4_mul(z, tglad_factor2);
+		aux->DE *= tglad_factor2;
+		aux->color += s->mbSp2;
+	}
+	z = v4_sub(z, s->mbOffset);
+	if (s->mbMainRotationEnabled) z = rot_apply4(&s->mbMainRot, z);
+	z = v4_mul(z, s->mbScale);
+	aux->DE = aux->DE * fabs(s->mbScale) + 1.0;
+	*zp = z;
+}
+
+/* cFractalKaleidoscopicIfs::FormulaCode (formula/definition/fractal_kaleidoscopic_ifs.cpp:30-74) */
+static void kifs_iteration(const scene_t *s, v4 *zp, aux_t *aux)
+{
+	v4 z = *zp;
+	if (s->ifsAbsX) z.x = fabs(z.x);
+	if (s->ifsAbsY) z.y = fabs(z.y);
+	if (s->ifsAbsZ) z.z = fabs(z.z);
+	for (int i = 0; i < MBX_IFS_VECTOR_COUNT; i++)
+	{
+		if (s->ifsEnabled[i])
+		{
+			z = rot_apply4(&s->ifsRot[i], z);
+			double length = v4_dot(z, s->ifsDirection[i]);
+			if (length < s->ifsDistance[i])
+			{
+				z = v4_sub(z, v4_mul(s->ifsDirection[i],
+												2.0 * (length - s->ifsDistance[i]) * s->ifsIntensity[i]));
+			}
+		}
+	}
+	if (s->ifsRotationEnabled)
+		z = v4_add(rot_apply4(&s->ifsMainRot, v4_sub(z, s->ifsOffset)), s->ifsOffset);
+	if (s->ifsEdgeEnabled)
+	{
+		if (s->ifsEdge.x > 0.0) z.x = s->ifsEdge.x - fabs(s->ifsEdge.x - z.x);
+		if (s->ifsEdge.y > 0.0) z.y = s->ifsEdge.y - fabs(s->ifsEdge.y - z.y);
+		if (s->ifsEdge.z > 0.0) z.z = s->ifsEdge.z - fabs(s->ifsEdge.z - z.z);
+	}
+	z = v4_mul(z, s->ifsScale);
+	if (s->ifsMenger)
+	{
+		z.x -= s->ifsOffset.x * (s->ifsScale - 1.0);
+		z.y -= s->ifsOffset.y * (s->ifsScale - 1.0);
+		if (z.z > 0.5 * s->ifsOffset.z * (s->ifsScale - 1.0)) z.z -= s->ifsOffset.z * (s->ifsScale - 1.0);
+	}
+	else
+	{
+		z = v4_sub(z, v4_mul(s->ifsOffset, s->ifsScale - 1.0));
+	}
+	aux->DE *= fabs(s->ifsScale);
+	*zp = z;
+}
+
+static void formula_code(const scene_t *s, v4 *z, aux_t *aux)
+{
+	switch (s->formula)
+	{
+		case MBX_FORMULA_MANDELBULB: mandelbulb_iteration(s, z, aux); break;
+		case MBX_FORMULA_MANDELBOX: mandelbox_iteration(s, z, aux); break;
+		case MBX_FORMULA_KALEIDOSCOPIC_IFS: kifs_iteration(s, z, aux); break;
+		default: break;
+	}
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Compute<Mode> (mandelbulber2/src/compute_fractal.cpp:67-654), single formula, no hybrid    */
+/* ------------------------------------------------------------------------------------------ */
+enum { MODE_NORMAL = 0, MODE_DELTA1 = 1 };
+
+typedef struct { v3 z; double distance; int iters; int maxiter; } fout_t;
+
+static void compute(const scene_t *s, int mode, v3 point, int forcedMaxiter, int maxiterMultiplier,
+	fout_t *out)
+{
+	v4 z = V4(point.x, point.y, point.z, 0.0);
+	z.w = s->initialWAxis;
+	out->maxiter = 1;
+	aux_t aux;
+	memset(&aux, 0, sizeof(aux));
+	aux.c = z;
+	aux.const_c = z;
+	aux.old_z = z;
+	aux.pos_neg = 1.0;
+	aux.r = v4_len(z);
+	aux.DE = 1.0;
+	aux.DE0 = 0.0;
+	aux.dist = 1000.0;
+	aux.pseudoKleinianDE = 1.0;
+	aux.actualScale = s->mbScale;
+	aux.actualScaleA = 0.0;
+	aux.color = 1.0;
+	aux.colorHybrid = 0.0;
+	aux.temp1000 = 1000.0;
+
+	v4 lastGoodZ = V4(0, 0, 0, 0), lastZ = V4(0, 0, 0, 0);
+	int maxN = s->formulaMaxiter * maxiterMultiplier;
+	if (forcedMaxiter >= 0) maxN = forcedMaxiter;
+	int i;
+	for (i = 0; i < maxN; i++)
+	{
+		lastGoodZ = lastZ;
+		lastZ = z;
+		aux.i = i;
+		formula_code(s, &z, &aux);
+		if (s->addCConstant)
+		{
+			if (s->juliaEnabled)
+			{
+				z = v4_add(z, V4(s->juliaConstant.x * s->constantMultiplier.x,
+												s->juliaConstant.y * s->constantMultiplier.y,
+												s->juliaConstant.z * s->constantMultiplier.z, 0.0));
+			}
+			else
+			{
+				/* CVector4 * CVector3 keeps w (algebra.hpp:340-343) */
+				z = v4_add(z, V4(aux.const_c.x * s->constantMultiplier.x,
+												aux.const_c.y * s->constantMultiplier.y,
+												aux.const_c.z * s->constantMultiplier.z, aux.const_c.w));
+			}
+		}
+		aux.r = v4_len(z);
+		if (v4_notfinite(z))
+		{
+			z = lastZ;
+			aux.r = v4_len(z);
+			out->maxiter = 1;
+			break;
+		}
+		if (aux.r > s->bailout)
+		{
+			out->maxiter = 0;
+			break;
+		}
+		if (v4_notfinite(z))
+		{
+			z = lastGoodZ;
+			break;
+		}
+	}
+
+	if (mode == MODE_NORMAL)
+	{
+		if (aux.DE > 0.0)
+		{
+			switch (s->DEAnalyticFunction)
+			{
+				case MBX_ANALYTIC_LOGARITHMIC:
+					out->distance = (aux.r > 1.0) ? 0.5 * aux.r * log(aux.r) / aux.DE : 0.0;
+					break;
+				case MBX_ANALYTIC_LINEAR: out->distance = aux.r / aux.DE; break;
+				case MBX_ANALYTIC_IFS: out->distance = (aux.r - 2.0) / aux.DE; break;
+				default: out->distance = aux.r; break;
+			}
+		}
+		else
+			out->distance = aux.r;
+	}
+	else
+		out->distance = 0.0;
+	out->iters = i + 1;
+	out->z = V3(z.x, z.y, z.z);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* CalculateDistance for one fractal node (calculate_distance.cpp:225-516 -> 533-763)          */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct { double distance; int iters; int totalIters; } dout_t;
+
+static double calculate_distance(const scene_t *s, v3 point, double detailSize, int normalCalc,
+	dout_t *out)
+{
+	const mbx_params *p = s->p;
+	double distance = 0;
+	const int maxiterMultiplier = (normalCalc && p->iterThreshMode) ? 5 : 1;
+	fout_t fo;
+	out->totalIters = 0;
+	if (s->DEType == MBX_DE_ANALYTIC)
+	{
+		compute(s, MODE_NORMAL, point, -1, maxiterMultiplier, &fo);
+		distance = fo.distance;
+		int maxiter = fo.maxiter;
+		out->iters = fo.iters;
+		out->totalIters += fo.iters;
+		if (!p->iterThreshMode) maxiter = 0;
+		if (normalCalc) maxiter = 0;
+		if (maxiter) distance = 0.0;
+		if (fo.iters < p->minN && distance < detailSize) distance = detailSize;
+		if (p->interiorMode && !normalCalc)
+		{
+			if (distance < 0.5 * detailSize || fo.maxiter) distance = detailSize;
+		}
+		else if (p->interiorMode && normalCalc)
+		{
+			if (distance < 0.9 * detailSize) distance = detailSize - distance;
+		}
+		if (p->iterThreshMode && !normalCalc && !fo.maxiter)
+		{
+			if (distance < detailSize) distance = detailSize * 1.01;
+		}
+		if (distance < 0.0) distance = 0.0;
+		if (distance > 10.0) distance = 10.0;
+	}
+	else
+	{
+		const double deltaDE = fmax(v3_len(point) * 1e-14, 1e-5 * detailSize);
+		compute(s, MODE_DELTA1, point, -1, maxiterMultiplier, &fo);
+		const double r = v3_len(fo.z);
+		v3 zFromIters = fo.z;
+		(void)zFromIters;
+		int outMaxiter = fo.maxiter;
+		int maxiter = fo.maxiter;
+		out->iters = fo.iters;
+		out->totalIters += fo.iters;
+		if (!p->iterThreshMode) maxiter = 0; /* limits disabled */
+		const int forced = fo.iters;
+		compute(s, MODE_DELTA1, v3_add(point, V3(deltaDE, 0.0, 0.0)), forced, maxiterMultiplier, &fo);
+		double r2 = v3_len(fo.z);
+		const double dr1 = fabs(r2 - r) / deltaDE;
+		out->totalIters += fo.iters;
+		compute(s, MODE_DELTA1, v3_add(point, V3(0.0, deltaDE, 0.0)), forced, maxiterMultiplier, &fo);
+		r2 = v3_len(fo.z);
+		const double dr2 = fabs(r2 - r) / deltaDE;
+		out->totalIters += fo.iters;
+		compute(s, MODE_DELTA1, v3_add(point, V3(0.0, 0.0, deltaDE)), forced, maxiterMultiplier, &fo);
+		r2 = v3_len(fo.z);
+		const double dr3 = fabs(r2 - r) / deltaDE;
+		out->totalIters += fo.iters;
+		const double dr = sqrt(dr1 * dr1 + dr2 * dr2 + dr3 * dr3);
+		if (dr > 0)
+		{
+			if (s->DEFunctionType == MBX_DE_FUNCTION_LINEAR)
+				distance = 0.5 * r / dr;
+			else if (s->DEFunctionType == MBX_DE_FUNCTION_LOGARITHMIC)
+				distance = 0.5 * r * log(r) / dr;
+		}
+		else
+			distance = r;
+		if (!p->iterThreshMode) outMaxiter = 0;
+		if (normalCalc) outMaxiter = 0;
+		if (outMaxiter) distance = 0.0;
+		if (fo.iters < p->minN && distance < detailSize) distance = detailSize;
+		if (p->interiorMode && !normalCalc)
+		{
+			if (distance < 0.5 * detailSize || maxiter) distance = detailSize;
+		}
+		else if (p->interiorMode && normalCalc)
+		{
+			if (distance < 0.9 * detailSize) distance = detailSize - distance;
+		}
+		if (p->iterThreshMode && !normalCalc && !maxiter)
+		{
+			if (distance < detailSize) distance = detailSize * 1.01;
+		}
+		if (distance < 0.0) distance = 0.0;
+		if (distance > 10.0) distance = 10.0;
+	}
+	/* single node: boolean-add merge with 1e20 and absScale 1 (calculate_distance.cpp:167-204) */
+	if (!(distance < 1e20)) distance = 1e20;
+	out->distance = distance;
+	return distance;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* per-ray context and counters                                                                */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct
+{
+	const scene_t *s;
+	uint32_t *seed;
+	uint64_t de_evals;
+	uint64_t iterations;
+} ray_ctx;
+
+static double de(ray_ctx *rc, v3 point, double detailSize, int normalCalc, int *iters)
+{
+	dout_t o;
+	double d = calculate_distance(rc->s, point, detailSize, normalCalc, &o);
+	rc->de_evals++;
+	rc->iterations += (uint64_t)o.totalIters;
+	if (iters) *iters = o.iters;
+	return d;
+}
+
+/* CalcDistThresh (render_worker.cpp:653-679) */
+static double calc_dist_thresh(const scene_t *s, v3 point)
+{
+	const mbx_params *p = s->p;
+	double distThresh;
+	if (p->iterThreshMode)
+		distThresh = v3_len(v3_sub(s->camera, point)) * s->resolution * s->fov;
+	else if (p->constantDEThreshold)
+		distThresh = p->DEThresh;
+	else
+		distThresh = v3_len(v3_sub(s->camera, point)) * s->resolution * s->fov / p->detailLevel;
+	distThresh /= 1.0; /* reduceDetail = 1 outside flight recording (render_job.cpp:1173-1184) */
+	return distThresh;
+}
+
+/* CalcDelta (render_worker.cpp:684-690) */
+static double calc_delta(const scene_t *s, v3 point)
+{
+	return v3_len(v3_sub(s->camera, point)) * s->resolution * s->fov;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* RayMarching (render_worker.cpp:708-951)                                                     */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct
+{
+	int found;
+	double depth, distThresh, lastDist;
+	v3 point;
+	int steps;
+	int iters;
+} march_t;
+
+static void ray_marching(ray_ctx *rc, v3 start, v3 direction, double minScan, double maxScan,
+	int binaryEnable, march_t *out)
+{
+	const scene_t *s = rc->s;
+	const mbx_params *p = s->p;
+	v3 point = V3(1e30, 1e30, 1e30);
+	int found = 0;
+	double scan = minScan;
+	double dist = 0;
+	const double search_accuracy = 0.001 * p->detailLevel;
+	const double search_limit = 1.0 - search_accuracy;
+	int counter = 0;
+	double step = 0.0;
+	double distThresh = 0;
+	v3 lastPoint;
+	int deadComputationFound = 0;
+	int iters = 0;
+	const double dirLength = v3_len(direction);
+
+	for (int i = 0; i < p->maxRaymarchingSteps; i++)
+	{
+		lastPoint = point;
+		counter++;
+		point = v3_add(start, v3_mul(direction, scan));
+		if (v3_eq(point, lastPoint) || v3_notfinite(point))
+		{
+			point = lastPoint;
+			found = 1;
+			deadComputationFound = 1;
+			break;
+		}
+		distThresh = calc_dist_thresh(s, point);
+		dist = de(rc, point, distThresh, 0, &iters);
+		if (dist < distThresh)
+		{
+			found = 1;
+			break;
+		}
+		if (p->interiorMode)
+			step = (dist - 0.8 * distThresh) * p->DEFactor * (1.0 - mbxo_random(rc->seed, 1000) / 10000.0);
+		else
+			step = (dist - 0.5 * distThresh) * p->DEFactor * (1.0 - mbxo_random(rc->seed, 1000) / 10000.0);
+		if (step > 3.0) step = 3.0;
+		scan += step / dirLength;
+		if (scan > maxScan) break;
+	}
+
+	point = v3_add(start, v3_mul(direction, scan));
+
+	if (found && binaryEnable && !deadComputationFound)
+	{
+		step *= 0.5;
+		for (int i = 0; i < 30; i++)
+		{
+			counter++;
+			if (dist < distThresh && dist > distThresh * search_limit)
+				break;
+			else
+			{
+				if (dist > distThresh)
+				{
+					scan += step;
+					point = v3_add(start, v3_mul(direction, scan));
+				}
+				else if (dist < distThresh * search_limit)
+				{
+					scan -= step;
+					point = v3_add(start, v3_mul(direction, scan));
+				}
+			}
+			distThresh = calc_dist_thresh(s, point);
+			dist = de(rc, point, distThresh, 0, &iters);
+			step *= 0.5;
+		}
+	}
+	if (p->iterThreshMode)
+	{
+		scan -= distThresh;
+		point = v3_add(start, v3_mul(direction, scan));
+		if (found) dist = de(rc, point, distThresh, 0, &iters);
+	}
+	out->found = found;
+	out->lastDist = dist;
+	out->depth = scan;
+	out->distThresh = distThresh;
+	out->point = point;
+	out->steps = counter;
+	out->iters = iters;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* shaders                                                                                     */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct { float R, G, B, A; } rgba;
+static rgba RGBA(float r, float g, float b, float a) { rgba c = {r, g, b, a}; return c; }
+
+typedef struct
+{
+	v3 point, normal, viewVector;
+	double distThresh, delta, depth;
+} shin_t; /* sShaderInputData subset (render_worker.hpp:57-88) */
+
+/* CalculateNormals, fast mode (shader_calculate_normals.cpp:40-130) */
+static v3 calculate_normals(ray_ctx *rc, const shin_t *in)
+{
+	const mbx_params *p = rc->s->p;
+	double delta = in->distThresh * p->smoothness;
+	if (p->interiorMode) delta = in->distThresh * 0.2 * p->smoothness;
+	double sx1 = de(rc, v3_add(in->point, V3(delta, 0.0, 0.0)), in->distThresh, 1, NULL);
+	double sx2 = de(rc, v3_sub(in->point, V3(delta, 0.0, 0.0)), in->distThresh, 1, NULL);
+	double sy1 = de(rc, v3_add(in->point, V3(0.0, delta, 0.0)), in->distThresh, 1, NULL);
+	double sy2 = de(rc, v3_sub(in->point, V3(0.0, delta, 0.0)), in->distThresh, 1, NULL);
+	double sz1 = de(rc, v3_add(in->point, V3(0.0, 0.0, delta)), in->distThresh, 1, NULL);
+	double sz2 = de(rc, v3_sub(in->point, V3(0.0, 0.0, delta)), in->distThresh, 1, NULL);
+	v3 normal = V3(sx1 - sx2, sy1 - sy2, sz1 - sz2);
+	if (fabs(normal.x) < 1e-50 && fabs(normal.y) < 1e-50 && fabs(normal.z) < 1e-50)
+		normal = V3(1.0, 0.0, 0.0);
+	else
+		normal = v3_normalize(normal);
+	if (v3_notfinite(normal)) normal = V3(1.0, 0.0, 0.0);
+	return normal;
+}
+
+/* AuxShadow, hard variant with one directional light (shader_aux_shadow.cpp:39-323) */
+static float aux_shadow(ray_ctx *rc, const shin_t *in, double distance, v3 lightVector)
+{
+	const scene_t *s = rc->s;
+	const mbx_params *p = s->p;
+	double totalOpacity = 0.0;
+	double shadowTemp = 1.0;
+	const double DEFactor = p->DEFactor;
+	double start = in->distThresh;
+	if (p->interiorMode) start = in->distThresh * DEFactor;
+	const double softRange = tan(s->lightSoftShadowCone);
+	const int bSoft = !p->iterThreshMode && !p->interiorMode && softRange > 0.0;
+	double maxSoft = 0.0;
+	int count = 0;
+	double step = in->distThresh;
+	for (double i = start; i < distance; i += step)
+	{
+		v3 point2 = v3_add(in->point, v3_mul(lightVector, i));
+		const float dist_thresh = (float)in->distThresh;
+		double dist = de(rc, point2, in->distThresh, 0, NULL);
+		if (bSoft)
+		{
+			double angle = (dist - dist_thresh) / i;
+			if (angle < 0) angle = 0;
+			if (dist < dist_thresh) angle = 0;
+			double softShadow = 1.0 - angle / softRange;
+			if (p->light.penetrating) softShadow *= (distance - i) / distance;
+			if (softShadow < 0) softShadow = 0;
+			if (softShadow > maxSoft) maxSoft = softShadow;
+		}
+		shadowTemp = 1.0 - totalOpacity;
+		if (dist < dist_thresh || shadowTemp < 0.0)
+		{
+			if (p->light.penetrating)
+			{
+				shadowTemp -= (distance - i) / distance;
+				if (shadowTemp < 0.0) shadowTemp = 0.0;
+			}
+			else
+				shadowTemp = 0.0;
+			break;
+		}
+		step = fmin(dist, 1e6) * DEFactor;
+		step = fmax(step, 1e-15);
+		count++;
+		if (count > p->maxRaymarchingSteps) break;
+	}
+	return bSoft ? (float)(1.0f - maxSoft) : (float)shadowTemp;
+}
+
+/* AmbientOcclusion, multiple rays (shader_ambient_occlusion.cpp:41-122) */
+static rgba ambient_occlusion(ray_ctx *rc, const shin_t *in)
+{
+	const scene_t *s = rc->s;
+	const mbx_params *p = s->p;
+	rgba AO = RGBA(0, 0, 0, 1.0f);
+	const double start_dist = in->distThresh;
+	const double end_dist = in->delta / s->resolution;
+	int start = 0, end = s->aoCount - 1;
+	if (p->DOFMonteCarlo)
+	{
+		int randomSample = mbxo_random(rc->seed, s->aoCount - 1);
+		start = randomSample;
+		end = randomSample;
+	}
+	for (int i = start; i <= end; i++)
+	{
+		v3 v = v3_from(p->aoVectors[i]);
+		double dist;
+		double opacity = 0.0;
+		double shadowTemp = 1.0;
+		for (double r = start_dist; r < end_dist; r += dist * 2.0)
+		{
+			v3 point2 = v3_add(in->point, v3_mul(v, r));
+			dist = de(rc, point2, in->distThresh, 0, NULL);
+			const float dist_thresh = (float)in->distThresh;
+			shadowTemp -= opacity * (end_dist - r) / end_dist;
+			if (dist < dist_thresh || shadowTemp < 0.0)
+			{
+				shadowTemp -= (end_dist - r) / end_dist;
+				if (shadowTemp < 0.0) shadowTemp = 0.0;
+				break;
+			}
+		}
+		const double intense = shadowTemp;
+		AO.R += intense * (float)p->aoColors[i][0];
+		AO.G += intense * (float)p->aoColors[i][1];
+		AO.B += intense * (float)p->aoColors[i][2];
+	}
+	if (!p->DOFMonteCarlo)
+	{
+		AO.R /= s->aoCount;
+		AO.G /= s->aoCount;
+		AO.B /= s->aoCount;
+	}
+	return AO;
+}
+
+/* FastAmbientOcclusion (shader_fast_ambient_occlusion.cpp:40-66) */
+static rgba fast_ambient_occlusion(ray_ctx *rc, const shin_t *in)
+{
+	const mbx_params *p = rc->s->p;
+	const double delta = in->distThresh;
+	float aoTemp = 0;
+	const double quality = p->ambientOcclusionQuality;
+	double lastDist = 1e20;
+	for (int i = 1; i < quality * quality; i++)
+	{
+		double scan = i * i * delta;
+		v3 pointTemp = v3_add(in->point, v3_mul(in->normal, scan));
+		double dist = de(rc, pointTemp, in->distThresh, 0, NULL);
+		if (dist > lastDist * 2) dist = lastDist * 2.0;
+		lastDist = dist;
+		aoTemp += 1.0 / pow(2.0, i) * (scan - p->ambientOcclusionFastTune * dist) / in->distThresh;
+	}
+	float ao = 1.0f - 0.2f * aoTemp;
+	if (ao < 0) ao = 0;
+	return RGBA(ao, ao, ao, 1.0f);
+}
+
+/* LightShading for one directional light without specular (shader_light_shading.cpp:40-122),
+ * cLight::CalculateLightVector (light.cpp:277-302), AuxLightsShader (shader_aux_light.cpp) */
+static rgba light_shading(ray_ctx *rc, const shin_t *in)
+{
+	const scene_t *s = rc->s;
+	const mbx_params *p = s->p;
+	if (!p->light.enabled) return RGBA(0, 0, 0, 0);
+	const v3 lightVector = s->lightDirection;
+	const double distance = p->light.penetrating ? in->delta / s->resolution : p->viewDistanceMax;
+	float intensity = (float)s->lightIntensity;
+	float shade = (float)v3_dot(in->normal, lightVector);
+	if (shade < 0) shade = 0;
+	const float matShading = (float)p->shading;
+	shade = 1.0f - matShading + shade * matShading;
+	shade = shade * intensity;
+	if (shade > 500.0f) shade = 500.0f;
+	float auxShadow = 1.0f;
+	if (p->light.castShadows)
+	{
+		if (shade > 0.001f)
+			auxShadow = aux_shadow(rc, in, distance, lightVector);
+		else
+			auxShadow = 0.0f;
+	}
+	rgba out;
+	out.R = shade * s->lightColor[0] * auxShadow;
+	out.G = shade * s->lightColor[1] * auxShadow;
+	out.B = shade * s->lightColor[2] * auxShadow;
+	out.A = 0;
+	return out;
+}
+
+/* ObjectShader (shader_object.cpp:40-229) with palette, textures, specular, env mapping,
+ * fake lights and iridescence off */
+static rgba object_shader(ray_ctx *rc, const shin_t *in, rgba *surfaceColour)
+{
+	const scene_t *s = rc->s;
+	const mbx_params *p = s->p;
+	rgba colour = RGBA((float)p->surfaceColor[0], (float)p->surfaceColor[1],
+		(float)p->surfaceColor[2], 1.0f);
+	*surfaceColour = colour;
+	rgba ambient = RGBA(0, 0, 0, 0);
+	if (p->ambientOcclusionEnabled)
+	{
+		if (p->ambientOcclusionMode == MBX_AO_FAST)
+			ambient = fast_ambient_occlusion(rc, in);
+		else if (p->ambientOcclusionMode == MBX_AO_MULTIPLE_RAYS)
+			ambient = ambient_occlusion(rc, in);
+	}
+	const float aoI = (float)p->ambientOcclusion;
+	rgba ambient2;
+	ambient2.R = ambient.R * aoI * (float)p->ambientOcclusionColor[0];
+	ambient2.G = ambient.G * aoI * (float)p->ambientOcclusionColor[1];
+	ambient2.B = ambient.B * aoI * (float)p->ambientOcclusionColor[2];
+	rgba auxLights = light_shading(rc, in);
+	rgba luminosity;
+	const float lum = (float)p->luminosity;
+	luminosity.R = lum * (float)p->luminosityColor[0];
+	luminosity.G = lum * (float)p->luminosityColor[1];
+	luminosity.B = lum * (float)p->luminosityColor[2];
+	rgba output;
+	output.R = ((float)p->fillLightColor[0] + ambient2.R) * colour.R;
+	output.G = ((float)p->fillLightColor[1] + ambient2.G) * colour.G;
+	output.B = ((float)p->fillLightColor[2] + ambient2.B) * colour.B;
+	output.R += auxLights.R * colour.R;
+	output.G += auxLights.G * colour.G;
+	output.B += auxLights.B * colour.B;
+	output.R += luminosity.R;
+	output.G += luminosity.G;
+	output.B += luminosity.B;
+	output.A = 1.0f;
+	return output;
+}
+
+/* BackgroundShader, 3-colour gradient + directional-light disk (shader_background.cpp:39-200) */
+static rgba background_shader(const scene_t *s, v3 viewVector)
+{
+	const mbx_params *p = s->p;
+	float pix[3];
+	if (p->background3ColorsEnable)
+	{
+		v3 vv = v3_normalize(V3(0.0, 0.0, 1.0));
+		v3 viewVectorNorm = v3_normalize(viewVector);
+		float grad = (float)(v3_dot(viewVectorNorm, vv) + 1.0f);
+		for (int k = 0; k < 3; k++)
+		{
+			const float c1 = (float)p->backgroundColor1[k], c2 = (float)p->backgroundColor2[k],
+									c3 = (float)p->backgroundColor3[k];
+			if (grad < 1.0f)
+			{
+				float gradN = 1.0f - grad;
+				pix[k] = c3 * gradN + c2 * grad;
+			}
+			else
+			{
+				float g = grad - 1.0f;
+				float gradN = 1.0f - g;
+				pix[k] = c2 * gradN + c1 * g;
+			}
+			pix[k] = (float)(pix[k] * p->backgroundBrightness);
+		}
+	}
+	else
+	{
+		for (int k = 0; k < 3; k++)
+			pix[k] = (float)((float)p->backgroundColor1[k] * p->backgroundBrightness);
+	}
+	rgba out;
+	const float invGamma = (float)(1.0f / p->backgroundGamma);
+	out.R = powf(pix[0], invGamma);
+	out.G = powf(pix[1], invGamma);
+	out.B = powf(pix[2], invGamma);
+	out.A = 0.0f;
+	if (p->light.enabled)
+	{
+		v3 viewVectorNorm = v3_normalize(viewVector);
+		double intensity = -(v3_dot(viewVectorNorm, s->lightDirection) - 1.0) * 360.0 / s->lightSize;
+		intensity = 1.0 / (1.0 + pow(intensity, 6.0 * s->lightContourSharpness)) * s->lightVisibility
+								* s->lightIntensity;
+		out.R = (float)(out.R + intensity * s->lightColor[0]);
+		out.G = (float)(out.G + intensity * s->lightColor[1]);
+		out.B = (float)(out.B + intensity * s->lightColor[2]);
+	}
+	return out;
+}
+
+/* GlobalIlumination (shader_global_illumination.cpp:41-209), volumetric mode off */
+static void global_illumination(ray_ctx *rc, const shin_t *input, rgba objectColor, float outRGB[3])
+{
+	const scene_t *s = rc->s;
+	const mbx_params *p = s->p;
+	float out[3] = {0, 0, 0};
+	shin_t ic = *input;
+	rgba objectColorTemp = objectColor;
+	rgba resultShader = RGBA(0, 0, 0, 0);
+	rgba newColor = objectColor;
+	double totalOpacity = 0.0;
+	int finished = 0;
+	const int maxDepth = p->reflectionsMax;
+	for (int rayDepth = 0; rayDepth < maxDepth; rayDepth++)
+	{
+		v3 reflectedDirection = ic.normal;
+		double randomX = (mbxo_random(rc->seed, 20000) - 10000) / 10000.0;
+		double randomY = (mbxo_random(rc->seed, 20000) - 10000) / 10000.0;
+		double randomZ = (mbxo_random(rc->seed, 20000) - 10000) / 10000.0;
+		v3 randomVector = V3(randomX * 1.2, randomY * 1.2, randomZ * 1.2);
+		v3 randomizedDirection = v3_normalize(v3_add(reflectedDirection, randomVector));
+		ic.viewVector = randomizedDirection;
+		double dist = 0.0;
+		int found = 0;
+		int stepCount = 0;
+		for (double scan = ic.distThresh; scan < p->viewDistanceMax; scan += dist * p->DEFactor)
+		{
+			if (stepCount >= p->maxRaymarchingSteps) break;
+			v3 point = v3_add(ic.point, v3_mul(randomizedDirection, scan));
+			double distThresh = calc_dist_thresh(s, point);
+			ic.distThresh = distThresh;
+			ic.delta = distThresh;
+			dist = de(rc, point, distThresh, 0, NULL);
+			if (dist < distThresh)
+			{
+				if (scan < distThresh * 2.0)
+				{
+					outRGB[0] = out[0];
+					outRGB[1] = out[1];
+					outRGB[2] = out[2];
+					return;
+				}
+				ic.point = point;
+				ic.depth = scan;
+				ic.normal = calculate_normals(rc, &ic);
+				found = 1;
+				break;
+			}
+			stepCount++;
+		}
+		if (found)
+		{
+			rgba oc;
+			rgba objectShader = object_shader(rc, &ic, &oc);
+			newColor = oc;
+			resultShader.R = objectShader.R;
+			resultShader.G = objectShader.G;
+			resultShader.B = objectShader.B;
+			out[0] += objectShader.R * objectColorTemp.R;
+			out[1] += objectShader.G * objectColorTemp.G;
+			out[2] += objectShader.B * objectColorTemp.B;
+		}
+		else
+		{
+			resultShader = background_shader(s, ic.viewVector);
+			finished = 1;
+		}
+		const double influence = fmin(fmax(1.0 - totalOpacity, 0.0), 1.0);
+		const float lim = (float)p->monteCarloGIRadianceLimit;
+		resultShader.R = fminf(fmaxf(resultShader.R, 0.0f), lim);
+		resultShader.G = fminf(fmaxf(resultShader.G, 0.0f), lim);
+		resultShader.B = fminf(fmaxf(resultShader.B, 0.0f), lim);
+		out[0] = (float)(out[0] + resultShader.R * objectColorTemp.R * influence);
+		out[1] = (float)(out[1] + resultShader.G * objectColorTemp.G * influence);
+		out[2] = (float)(out[2] + resultShader.B * objectColorTemp.B * influence);
+		totalOpacity += 0.0;
+		objectColorTemp = newColor;
+		if (finished || totalOpacity > 1.0) break;
+	}
+	outRGB[0] = out[0];
+	outRGB[1] = out[1];
+	outRGB[2] = out[2];
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* one primary ray: RayRecursion with reflections off (render_worker.cpp:969-1819)            */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct
+{
+	float rgb[3];
+	double depth;
+	int found;
+	int iters;
+	int steps;
+	v3 normal;
+} ray_result;
+
+static void trace_primary(ray_ctx *rc, v3 start, v3 viewVector, ray_result *res)
+{
+	const scene_t *s = rc->s;
+	const mbx_params *p = s->p;
+	v3 direction = v3_normalize(viewVector);
+	march_t m;
+	ray_marching(rc, start, direction, 0.0, p->viewDistanceMax, 1, &m);
+	shin_t in;
+	in.point = m.point;
+	in.distThresh = m.distThresh;
+	in.delta = calc_delta(s, m.point);
+	in.viewVector = direction;
+	in.depth = m.depth;
+	in.normal = V3(0, 0, 0);
+	res->found = m.found;
+	res->iters = m.iters;
+	res->steps = m.steps;
+	if (m.found)
+	{
+		in.normal = calculate_normals(rc, &in);
+		rgba objectColour;
+		rgba objectShader = object_shader(rc, &in, &objectColour);
+		if (p->DOFMonteCarlo && p->DOFMonteCarloGlobalIllumination)
+		{
+			float gi[3];
+			global_illumination(rc, &in, objectColour, gi);
+			objectShader.R += gi[0];
+			objectShader.G += gi[1];
+			objectShader.B += gi[2];
+		}
+		float R = objectShader.R, G = objectShader.G, B = objectShader.B;
+		if (R < 0.0f) R = 0.0f;
+		if (G < 0.0f) G = 0.0f;
+		if (B < 0.0f) B = 0.0f;
+		res->rgb[0] = R;
+		res->rgb[1] = G;
+		res->rgb[2] = B;
+		res->depth = m.depth;
+		res->normal = in.normal;
+	}
+	else
+	{
+		rgba bg = background_shader(s, direction);
+		res->rgb[0] = bg.R;
+		res->rgb[1] = bg.G;
+		res->rgb[2] = bg.B;
+		res->depth = 1e20;
+		res->normal = V3(0, 0, 0);
+	}
+}
+
+/* CalculateViewVector, three-point perspective (projection_3d.cpp:41-89) */
+static v3 view_vector(const scene_t *s, double ix, double iy)
+{
+	v3 v = V3(ix * s->fov, 1.0, iy * s->fov);
+	v = v3_normalize(v);
+	return rot_apply3(&s->mRot, v);
+}
+
+/* pixel loop body of cRenderWorker::doWork (render_worker.cpp:155-525) for one pixel */
+static void render_pixel(ray_ctx *rc, int xs, int ys, ray_result *out)
+{
+	const scene_t *s = rc->s;
+	const mbx_params *p = s->p;
+	const int W = p->imageWidth, H = p->imageHeight;
+	/* screenRegion (0,0,W,H) -> imageRegion (-0.5,0.5,0.5,-0.5) (render_job.cpp:308-318,
+	 * region.hpp:66-71) */
+	double ix = (1.0 / W) * xs + (-0.5);
+	double iy = (-1.0 / H) * ys + 0.5;
+	ix *= s->aspectRatio;
+	const double ox = ix, oy = iy;
+	int repeats = p->DOFMonteCarlo ? p->DOFSamples : 1;
+	if (repeats < 1) repeats = 1;
+	float acc[3] = {0, 0, 0};
+	ray_result r;
+	memset(out, 0, sizeof(*out));
+	for (int repeat = 0; repeat < repeats; repeat++)
+	{
+		if (p->DOFMonteCarlo)
+		{
+			ix = ox + ((double)mbxo_random(rc->seed, 1000) / 1000.0 - 0.5) / W * s->aspectRatio;
+			iy = oy + ((double)mbxo_random(rc->seed, 1000) / 1000.0 - 0.5) / H;
+		}
+		v3 viewVector = view_vector(s, ix, iy);
+		trace_primary(rc, s->camera, viewVector, &r);
+		acc[0] += r.rgb[0];
+		acc[1] += r.rgb[1];
+		acc[2] += r.rgb[2];
+		if (repeat == 0)
+		{
+			out->depth = r.depth;
+			out->found = r.found;
+			out->iters = r.iters;
+			out->normal = r.normal;
+		}
+		out->steps += r.steps;
+		/* depth of the last sample is what the reference writes (render_worker.cpp:346-348) */
+		out->depth = r.depth;
+	}
+	if (p->DOFMonteCarlo)
+	{
+		out->rgb[0] = acc[0] / repeats;
+		out->rgb[1] = acc[1] / repeats;
+		out->rgb[2] = acc[2] / repeats;
+	}
+	else
+	{
+		out->rgb[0] = acc[0];
+		out->rgb[1] = acc[1];
+		out->rgb[2] = acc[2];
+	}
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* exported entry points (ctypes)                                                              */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Render rows [y0, y1) of the image.  Any output pointer may be NULL.  Row-interleaved
+ * dynamic scheduling over `threads` OpenMP threads stands in for cScheduler's line hand-out
+ * (mandelbulber2/src/scheduler.hpp:51-88).  Counters (rays, DE evals, iterations) are
+ * accumulated into stats[0..2] when stats is non-NULL. */
+int mbxo_render(const mbx_params *p, int y0, int y1, int threads, float *rgb, float *depth,
+	int32_t *iters, uint8_t *mask, float *normal, int32_t *steps, uint64_t *stats)
+{
+	scene_t s;
+	int rc0 = scene_init(&s, p);
+	if (rc0) return rc0;
+	const int W = p->imageWidth;
+	if (y0 < 0) y0 = 0;
+	if (y1 > p->imageHeight) y1 = p->imageHeight;
+	uint64_t n_de = 0, n_it = 0, n_rays = 0;
+	if (p->rngMode == MBX_RNG_GLOBAL_SEQUENTIAL)
+	{
+		uint32_t gseed = 1;
+		ray_ctx rc = {&s, &gseed, 0, 0};
+		for (int y = y0; y < y1; y++)
+			for (int x = 0; x < W; x++)
+			{
+				ray_result r;
+				render_pixel(&rc, x, y, &r);
+				const size_t i = (size_t)y * W + x;
+				if (rgb) { rgb[3 * i] = r.rgb[0]; rgb[3 * i + 1] = r.rgb[1]; rgb[3 * i + 2] = r.rgb[2]; }
+				if (depth) depth[i] = (float)r.depth;
+				if (iters) iters[i] = r.iters;
+				if (mask) mask[i] = (uint8_t)r.found;
+				if (normal) { normal[3 * i] = (float)r.normal.x; normal[3 * i + 1] = (float)r.normal.y; normal[3 * i + 2] = (float)r.normal.z; }
+				if (steps) steps[i] = r.steps;
+				n_rays += (uint64_t)(p->DOFMonteCarlo ? p->DOFSamples : 1);
+			}
+		n_de = rc.de_evals;
+		n_it = rc.iterations;
+	}
+	else
+	{
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1) \
+	reduction(+ : n_de, n_it, n_rays)
+		for (int y = y0; y < y1; y++)
+		{
+			for (int x = 0; x < W; x++)
+			{
+				const uint32_t pix = (uint32_t)y * (uint32_t)W + (uint32_t)x;
+				uint32_t seed = mbxo_pixel_seed(pix, p->randomSeed);
+				ray_ctx rc = {&s, &seed, 0, 0};
+				ray_result r;
+				render_pixel(&rc, x, y, &r);
+				const size_t i = (size_t)y * W + x;
+				if (rgb) { rgb[3 * i] = r.rgb[0]; rgb[3 * i + 1] = r.rgb[1]; rgb[3 * i + 2] = r.rgb[2]; }
+				if (depth) depth[i] = (float)r.depth;
+				if (iters) iters[i] = r.iters;
+				if (mask) mask[i] = (uint8_t)r.found;
+				if (normal) { normal[3 * i] = (float)r.normal.x; normal[3 * i + 1] = (float)r.normal.y; normal[3 * i + 2] = (float)r.normal.z; }
+				if (steps) steps[i] = r.steps;
+				n_de += rc.de_evals;
+				n_it += rc.iterations;
+				n_rays += (uint64_t)(p->DOFMonteCarlo ? p->DOFSamples : 1);
+			}
+		}
+	}
+	if (stats)
+	{
+		stats[0] += n_rays;
+		stats[1] += n_de;
+		stats[2] += n_it;
+	}
+	return MBX_OK;
+}
+
+/* Point DE query: CalculateDistance at each point with the given detail size. */
+int mbxo_distance(const mbx_params *p, const double *pts, int n, double detailSize, int normalCalc,
+	double *dist, int32_t *iters)
+{
+	scene_t s;
+	int rc0 = scene_init(&s, p);
+	if (rc0) return rc0;
+	uint32_t seed = 1;
+	ray_ctx rc = {&s, &seed, 0, 0};
+	for (int k = 0; k < n; k++)
+	{
+		int it = 0;
+		dist[k] = de(&rc, V3(pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]), detailSize, normalCalc, &it);
+		if (iters) iters[k] = it;
+	}
+	return MBX_OK;
+}
+
+/* Compute<calcModeNormal> at each point: distance, iters, maxiter flag and final z. */
+int mbxo_compute(const mbx_params *p, const double *pts, int n, double *dist, int32_t *iters,
+	int32_t *maxiter, double *zout)
+{
+	scene_t s;
+	int rc0 = scene_init(&s, p);
+	if (rc0) return rc0;
+	for (int k = 0; k < n; k++)
+	{
+		fout_t fo;
+		compute(&s, MODE_NORMAL, V3(pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]), -1, 1, &fo);
+		if (dist) dist[k] = fo.distance;
+		if (iters) iters[k] = fo.iters;
+		if (maxiter) maxiter[k] = fo.maxiter;
+		if (zout) { zout[3 * k] = fo.z.x; zout[3 * k + 1] = fo.z.y; zout[3 * k + 2] = fo.z.z; }
+	}
+	return MBX_OK;
+}
+
+/* One FormulaCode call: z4 in/out (4 doubles per item); aux in/out as (r, DE, color). */
+int mbxo_formula_step(const mbx_params *p, int n, double *z4, double *aux3)
+{
+	scene_t s;
+	int rc0 = scene_init(&s, p);
+	if (rc0) return rc0;
+	for (int k = 0; k < n; k++)
+	{
+		v4 z = V4(z4[4 * k], z4[4 * k + 1], z4[4 * k + 2], z4[4 * k + 3]);
+		aux_t aux;
+		memset(&aux, 0, sizeof(aux));
+		aux.r = aux3[3 * k];
+		aux.DE = aux3[3 * k + 1];
+		aux.color = aux3[3 * k + 2];
+		formula_code(&s, &z, &aux);
+		z4[4 * k] = z.x;
+		z4[4 * k + 1] = z.y;
+		z4[4 * k + 2] = z.z;
+		z4[4 * k + 3] = z.w;
+		aux3[3 * k] = aux.r;
+		aux3[3 * k + 1] = aux.DE;
+		aux3[3 * k + 2] = aux.color;
+	}
+	return MBX_OK;
+}
+
+/* Camera/light derivation for host-code parity: mRot (9), fov, lightDirection (3). */
+int mbxo_frame_constants(const mbx_params *p, double *mrot9, double *fov, double *light3)
+{
+	scene_t s;
+	int rc0 = scene_init(&s, p);
+	if (rc0) return rc0;
+	rotm m = s.mRot;
+	if (m.zero) rot_null(&m);
+	memcpy(mrot9, m.m, 9 * sizeof(double));
+	*fov = s.fov;
+	light3[0] = s.lightDirection.x;
+	light3[1] = s.lightDirection.y;
+	light3[2] = s.lightDirection.z;
+	return MBX_OK;
+}
+
+/* Primary ray of one pixel: view direction (normalised) and the march result. */
+int mbxo_march_pixel(const mbx_params *p, int x, int y, double *dir3, double *depth, double *thr,
+	int32_t *found, int32_t *iters, double *point3)
+{
+	scene_t s;
+	int rc0 = scene_init(&s, p);
+	if (rc0) return rc0;
+	const int W = p->imageWidth, H = p->imageHeight;
+	double ix = (1.0 / W) * x + (-0.5);
+	double iy = (-1.0 / H) * y + 0.5;
+	ix *= s.aspectRatio;
+	v3 d = v3_normalize(view_vector(&s, ix, iy));
+	uint32_t seed = mbxo_pixel_seed((uint32_t)y * (uint32_t)W + (uint32_t)x, p->randomSeed);
+	ray_ctx rc = {&s, &seed, 0, 0};
+	march_t m;
+	ray_marching(&rc, s.camera, d, 0.0, p->viewDistanceMax, 1, &m);
+	dir3[0] = d.x;
+	dir3[1] = d.y;
+	dir3[2] = d.z;
+	*depth = m.depth;
+	*thr = m.distThresh;
+	*found = m.found;
+	*iters = m.iters;
+	point3[0] = m.point.x;
+	point3[1] = m.point.y;
+	point3[2] = m.point.z;
+	return MBX_OK;
+}
+
+/* Normal at a point (CalculateNormals) with the given distance threshold. */
+int mbxo_normal(const mbx_params *p, const double *pt3, double distThresh, double *n3)
+{
+	scene_t s;
+	int rc0 = scene_init(&s, p);
+	if (rc0) return rc0;
+	uint32_t seed = 1;
+	ray_ctx rc = {&s, &seed, 0, 0};
+	shin_t in;
+	memset(&in, 0, sizeof(in));
+	in.point = V3(pt3[0], pt3[1], pt3[2]);
+	in.distThresh = distThresh;
+	v3 n = calculate_normals(&rc, &in);
+	n3[0] = n.x;
+	n3[1] = n.y;
+	n3[2] = n.z;
+	return MBX_OK;
+}
+
+/* Struct sizes for binding-layout checks. */
+size_t mbxo_sizeof_params(void) { return sizeof(mbx_params); }
+size_t mbxo_offsetof_fractal(void) { return offsetof(mbx_params, fractal); }
