@@ -244,6 +244,9 @@ typedef struct mbx_ctx mbx_ctx;
 /* Library version (MBX_ABI_VERSION). */
 int mbx_version(void);
 
+/* sizeof(mbx_params) as compiled into the library, for binding-layout checks. */
+size_t mbx_sizeof_params(void);
+
 /* Create a context on HIP device `device`.  Replaces the OpenCL engine's
  * CreateKernel4Program / PreAllocateBuffers / CreateCommandQueue sequence
  * (mandelbulber2/src/render_job.cpp:916-927, opencl_engine.cpp:277-375).  `stream` is a
@@ -260,6 +263,14 @@ const char *mbx_last_error(const mbx_ctx *ctx);
  * and the sParamRender / cHybridFractalSequences set-up of cRenderJob::Execute
  * (render_job.cpp:399-406, 477). */
 int mbx_scene_upload(mbx_ctx *ctx, const mbx_params *params);
+
+/* Derived per-frame constants of the uploaded scene, as the host computed them in FP64:
+ * the camera rotation mRot (row-major 3x3, render_worker.cpp:539-564), the CalcFOV value
+ * (projection_3d.cpp:140-159), light #1's direction (light.cpp:130-185) and the device DE
+ * kernel kind chosen for the fractal (0 Mandelbulb-8 trig-free, 1 Mandelbulb trig, 2 Mandelbox
+ * analytic, 3 Mandelbox delta, 4 Kaleidoscopic IFS).  Any output pointer may be NULL. */
+int mbx_scene_info(const mbx_ctx *ctx, double *mrot9, double *fov, double *light_dir3,
+	int *de_kind);
 
 /* Render the uploaded scene into caller-owned device buffers.  Replaces RenderMulti
  * (opencl_engine_render_fractal.cpp:2089-2425) and, on the CPU side, cRenderer::RenderImage

@@ -1,0 +1,135 @@
+// mbx_frame.h - per-frame constants as the kernels read them.
+//
+// The host derives every value here from an mbx_params block in double precision
+// (mbx_host.cpp) and uploads one FrameConst per frame into device memory.  All threads of a
+// launch read the same FrameConst, so the loads are wave-uniform and become scalar (SGPR)
+// loads: this plays the role of the OpenCL engine's __constant sClInConstants
+// (mandelbulber2/opencl/input_data_structures.h:62-65).
+#pragma once
+
+#include <stdint.h>
+
+#include "../../include/mbx.h"
+
+namespace mbx
+{
+
+// Distance-estimator kinds the kernels are specialised for.
+enum DEKind : int32_t
+{
+	kBulb8 = 0,      // Mandelbulb power 8, analytic log DE, trig-free power (w = 0)
+	kBulbTrig = 1,   // Mandelbulb any power, analytic log DE, reference trig form
+	kBoxAnalytic = 2,
+	kBoxDelta = 3,   // Mandelbox, delta DE by forward-mode tangents
+	kIfsAnalytic = 4,
+	kNumDEKinds = 5
+};
+
+struct FrameConst
+{
+	// image and camera (render_worker.cpp:539-564, projection_3d.cpp:77-89)
+	int32_t width, height;
+	float invWidth, invHeight; // 1/W, 1/H (pixel -> image plane)
+	float aspectRatio;
+	float fov;                 // 2 tan(fov/2)
+	float mRot[9];             // row-major camera rotation
+	float camera[3];
+	double cameraD[3];         // camera in FP64 for the precision path
+	double mRotD[9];
+	double fovD, aspectRatioD, invWidthD, invHeightD;
+	float threshK;             // distThresh = |camera - p| * threshK   (CalcDistThresh)
+	float deltaK;              // delta      = |camera - p| * deltaK    (CalcDelta)
+	double threshKD, deltaKD;
+	int32_t constantDEThreshold;
+	float DEThresh;
+	double DEThreshD;
+	float DEFactor;
+	float searchLimit;         // 1 - 0.001 * detailLevel
+	float viewDistanceMax;
+	float smoothness;
+	int32_t maxRaymarchingSteps;
+	int32_t minN;
+
+	// fractal sequence (hybrid_fractal_sequences.cpp:181-287, 377-503)
+	int32_t deKind;
+	int32_t formulaMaxiter;
+	int32_t deFunction;        // delta DE finish: 1 linear, 2 logarithmic
+	int32_t analyticFunction;  // analytic finish: 1 linear, 2 log, 3 IFS
+	int32_t addC;
+	int32_t julia;
+	float bailout;
+	float juliaC[3];           // juliaConstant * constantMultiplier
+	float constMul[3];
+	float initialW;
+	// Mandelbulb
+	float power, alpha, beta;  // radians
+	// Mandelbox (after RecalculateFractalParams, fractal.cpp:673-675)
+	float mbScale, mbAbsScale, mbFoldingLimit, mbFoldingValue, mbFR2, mbMR2, mbFactor1;
+	float mbOffset[3];
+	int32_t mbRotationsEnabled, mbMainRotationEnabled;
+	float mbMainRot[9];
+	float mbRot[2][3][9];
+	float mbRotInv[2][3][9];
+	// IFS
+	float ifsScale, ifsAbsScale;
+	float ifsOffset[3];
+	float ifsEdge[3];
+	int32_t ifsAbs[3];
+	int32_t ifsMenger, ifsRotationEnabled, ifsEdgeEnabled, ifsEnabledCount;
+	float ifsMainRot[9];
+	float ifsRot[MBX_IFS_VECTOR_COUNT][9];
+	float ifsDirection[MBX_IFS_VECTOR_COUNT][3];
+	float ifsDistance[MBX_IFS_VECTOR_COUNT];
+	float ifsIntensity[MBX_IFS_VECTOR_COUNT];
+	int32_t ifsEnabled[MBX_IFS_VECTOR_COUNT];
+
+	// light #1, directional (light.cpp:80-195, shader_light_shading.cpp:40-122)
+	int32_t lightEnabled, castShadows, penetrating, softShadows;
+	float lightDir[3];
+	float lightIntensity, lightVisibility, lightSize, lightContourSharpness, softRange;
+	float lightColor[3];
+	float viewDistanceMaxLight;
+
+	// material #1
+	float surfaceColor[3];
+	float shading;
+	float luminosity[3];       // luminosity * luminosityColor
+	float fillLight[3];
+
+	// ambient occlusion
+	int32_t aoEnabled, aoMode, aoQuality, aoCount;
+	float aoIntensity;
+	float aoColor[3];
+	float aoFastTune;
+	float aoVectors[MBX_MAX_AO_VECTORS][3];
+	float aoColors[MBX_MAX_AO_VECTORS][3];
+
+	// background (shader_background.cpp:39-200)
+	int32_t background3Colors;
+	float bg1[3], bg2[3], bg3[3];
+	float bgBrightness, bgInvGamma;
+
+	// Monte Carlo
+	int32_t monteCarlo, samples, globalIllumination, reflectionsMax;
+	float giRadianceLimit;
+
+	uint32_t randomSeed;
+	int32_t pad[2];
+};
+
+// Per-pixel seed of the Park-Miller stream (identical on host and device).
+#if defined(__HIPCC__)
+__host__ __device__
+#endif
+inline uint32_t pixel_seed(uint32_t pixel_index, uint32_t frame_seed)
+{
+	uint32_t h = pixel_index * 0x9E3779B1u ^ (frame_seed + 0x7F4A7C15u);
+	h ^= h >> 16;
+	h *= 0x7FEB352Du;
+	h ^= h >> 15;
+	h *= 0x846CA68Bu;
+	h ^= h >> 16;
+	return 1u + h % 2147483646u;
+}
+
+} // namespace mbx

@@ -1,0 +1,595 @@
+// mbx_host.cpp - C ABI of the raymarcher (include/mbx.h): context, scene upload, launches.
+//
+// The host side of what cRenderJob / cOpenClEngineRenderFractal do before a launch
+// (render_job.cpp:399-477, opencl_engine_render_fractal.cpp:1114-1245): turn the user-level
+// parameters into per-frame constants in double precision (camera frame, light vector, formula
+// constants), then hand them to the device once per frame.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mbx.h"
+#include "mbx_frame.h"
+#include "mbx_launch.h"
+
+using mbx::FrameConst;
+
+struct mbx_ctx
+{
+	int device = 0;
+	hipStream_t stream = nullptr;
+	FrameConst frame{};
+	FrameConst *d_frame = nullptr;
+	unsigned long long *d_counters = nullptr;
+	hipEvent_t ev0 = nullptr, ev1 = nullptr;
+	bool sceneReady = false;
+	bool launched = false;
+	double mRot[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+	double fov = 0.0;
+	double lightDir[3] = {0, 0, 0};
+	uint64_t rays = 0;
+	std::string err;
+};
+
+namespace
+{
+
+int fail(mbx_ctx *ctx, int code, const std::string &msg)
+{
+	if (ctx) ctx->err = msg;
+	return code;
+}
+
+int hip_fail(mbx_ctx *ctx, hipError_t e, const char *where)
+{
+	return fail(ctx, MBX_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+// ---- double-precision vector helpers for the host-side derivation -------------------------
+struct D3
+{
+	double x, y, z;
+};
+D3 d3(const double *a) { return D3{a[0], a[1], a[2]}; }
+D3 add(D3 a, D3 b) { return D3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+D3 sub(D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+D3 mul(D3 a, double s) { return D3{a.x * s, a.y * s, a.z * s}; }
+double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+double len(D3 a) { return std::sqrt(dot(a, a)); }
+D3 cross(D3 a, D3 b) { return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+D3 unit(D3 a)
+{
+	const double k = 1.0 / len(a);
+	return mul(a, k);
+}
+// Rodrigues rotation of v about a unit axis (CVector3::RotateAroundVectorByAngle, algebra.cpp:71-77)
+D3 rotate_about(D3 v, D3 axis, double angle)
+{
+	const double c = std::cos(angle), s = std::sin(angle);
+	return add(add(mul(v, c), mul(cross(axis, v), s)), mul(axis, dot(axis, v) * (1.0 - c)));
+}
+
+// 3x3 rotation composed by right-multiplication as CRotationMatrix::RotateX/Y/Z do
+// (algebra.cpp:198-247)
+struct Rot
+{
+	double m[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+	void mulRight(const double b[9])
+	{
+		double c[9];
+		for (int r = 0; r < 3; r++)
+			for (int k = 0; k < 3; k++)
+				c[3 * r + k] = m[3 * r] * b[k] + m[3 * r + 1] * b[3 + k] + m[3 * r + 2] * b[6 + k];
+		std::memcpy(m, c, sizeof(c));
+	}
+	void x(double a)
+	{
+		if (a == 0.0) return;
+		const double s = std::sin(a), c = std::cos(a);
+		const double b[9] = {1, 0, 0, 0, c, -s, 0, s, c};
+		mulRight(b);
+	}
+	void y(double a)
+	{
+		if (a == 0.0) return;
+		const double s = std::sin(a), c = std::cos(a);
+		const double b[9] = {c, 0, s, 0, 1, 0, -s, 0, c};
+		mulRight(b);
+	}
+	void z(double a)
+	{
+		if (a == 0.0) return;
+		const double s = std::sin(a), c = std::cos(a);
+		const double b[9] = {c, -s, 0, s, c, 0, 0, 0, 1};
+		mulRight(b);
+	}
+	D3 apply(D3 v) const
+	{
+		return D3{m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
+			m[6] * v.x + m[7] * v.y + m[8] * v.z};
+	}
+};
+
+double wrap_angle(double a) { return std::fmod(a + 3.0 * M_PI, 2.0 * M_PI) - M_PI; }
+
+// Camera frame from (camera, target, top): yaw/pitch/roll as cCameraTarget::SetCameraTargetTop
+// (camera_target.cpp:86-126) and the forward/top/right vectors.
+struct CameraFrame
+{
+	D3 forward, top, right;
+	double yaw, pitch, roll;
+};
+
+CameraFrame camera_frame(D3 camera, D3 target, D3 top)
+{
+	CameraFrame cf;
+	const bool same = camera.x == target.x && camera.y == target.y && camera.z == target.z;
+	const D3 f = unit(same ? D3{0.0, 1.0, 0.0} : sub(target, camera));
+	double yaw = std::atan2(f.y, f.x) - 0.5 * M_PI;
+	double pitch = std::atan2(f.z, std::sqrt(f.x * f.x + f.y * f.y));
+	D3 t = unit(top);
+	t = rotate_about(t, D3{0, 0, 1}, -yaw);
+	t = rotate_about(t, D3{1, 0, 0}, -pitch);
+	double roll = 0.5 * M_PI - std::atan2(t.z, t.x);
+	yaw = wrap_angle(yaw);
+	pitch = wrap_angle(pitch);
+	roll = wrap_angle(roll);
+	D3 up{0, 0, 1};
+	up = rotate_about(up, D3{0, 1, 0}, roll);
+	up = rotate_about(up, D3{1, 0, 0}, pitch);
+	up = rotate_about(up, D3{0, 0, 1}, yaw);
+	cf.forward = f;
+	cf.top = up;
+	cf.right = cross(f, up);
+	cf.yaw = yaw;
+	cf.pitch = pitch;
+	cf.roll = roll;
+	return cf;
+}
+
+void set3f(float *dst, D3 v)
+{
+	dst[0] = (float)v.x;
+	dst[1] = (float)v.y;
+	dst[2] = (float)v.z;
+}
+void set3f(float *dst, const double *v)
+{
+	dst[0] = (float)v[0];
+	dst[1] = (float)v[1];
+	dst[2] = (float)v[2];
+}
+void setRot(float *dst, const Rot &r)
+{
+	for (int i = 0; i < 9; i++) dst[i] = (float)r.m[i];
+}
+
+// Derive the per-frame constants; returns MBX_OK or an error with ctx->err set.
+int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
+{
+	std::memset(&fc, 0, sizeof(fc));
+	if (p->imageWidth <= 0 || p->imageHeight <= 0 || p->imageWidth > 65536 || p->imageHeight > 65536)
+		return fail(ctx, MBX_ERR_INVALID, "image size out of range");
+	if (p->perspectiveType != 0)
+		return fail(ctx, MBX_ERR_UNSUPPORTED, "only the three-point perspective is supported");
+	if (p->iterThreshMode || p->interiorMode)
+		return fail(ctx, MBX_ERR_UNSUPPORTED, "iteration-threshold and interior modes are not supported");
+	if (p->maxRaymarchingSteps < 1) return fail(ctx, MBX_ERR_INVALID, "max_raymarching_steps < 1");
+	if (!(p->detailLevel > 0.0)) return fail(ctx, MBX_ERR_INVALID, "detail_level must be > 0");
+	const mbx_fractal &fr = p->fractal;
+	if (fr.initialWAxis != 0.0) return fail(ctx, MBX_ERR_UNSUPPORTED, "initial_waxis must be 0");
+	if (fr.formulaMaxiter < 1) return fail(ctx, MBX_ERR_INVALID, "formula maxiter < 1");
+
+	// camera: PrepareMainVectors (render_worker.cpp:539-564)
+	const CameraFrame cf = camera_frame(d3(p->camera), d3(p->target), d3(p->topVector));
+	Rot m;
+	m.z(cf.yaw);
+	m.x(cf.pitch);
+	m.y(cf.roll);
+	m.z(-(p->sweetSpotHAngle / 180.0 * M_PI));
+	m.x(p->sweetSpotVAngle / 180.0 * M_PI);
+	std::memcpy(ctx->mRot, m.m, sizeof(m.m));
+	const double fov = 2.0 * std::tan(p->fov / 360.0 * M_PI);
+	ctx->fov = fov;
+	const double resolution = 1.0 / p->imageHeight;
+	const double aspect = (double)p->imageWidth / p->imageHeight;
+
+	fc.width = p->imageWidth;
+	fc.height = p->imageHeight;
+	fc.invWidth = (float)(1.0 / p->imageWidth);
+	fc.invHeight = (float)(1.0 / p->imageHeight);
+	fc.aspectRatio = (float)aspect;
+	fc.fov = (float)fov;
+	setRot(fc.mRot, m);
+	set3f(fc.camera, p->camera);
+	for (int i = 0; i < 3; i++) fc.cameraD[i] = p->camera[i];
+	for (int i = 0; i < 9; i++) fc.mRotD[i] = m.m[i];
+	fc.fovD = fov;
+	fc.aspectRatioD = aspect;
+	fc.invWidthD = 1.0 / p->imageWidth;
+	fc.invHeightD = 1.0 / p->imageHeight;
+	fc.threshKD = resolution * fov / p->detailLevel;
+	fc.deltaKD = resolution * fov;
+	fc.threshK = (float)fc.threshKD;
+	fc.deltaK = (float)fc.deltaKD;
+	fc.constantDEThreshold = p->constantDEThreshold;
+	fc.DEThresh = (float)p->DEThresh;
+	fc.DEThreshD = p->DEThresh;
+	fc.DEFactor = (float)p->DEFactor;
+	fc.searchLimit = (float)(1.0 - 0.001 * p->detailLevel);
+	fc.viewDistanceMax = (float)p->viewDistanceMax;
+	fc.smoothness = (float)p->smoothness;
+	fc.maxRaymarchingSteps = p->maxRaymarchingSteps;
+	fc.minN = p->minN;
+
+	// fractal sequence data (hybrid_fractal_sequences.cpp:181-287, 471-503)
+	int deType = 0, deFunction = 0, analyticFunction = 0;
+	double defaultBailout = 0.0;
+	bool cpixelDisabled = false;
+	switch (fr.formula)
+	{
+		case MBX_FORMULA_MANDELBULB:
+			deFunction = MBX_DE_FUNCTION_LOGARITHMIC;
+			analyticFunction = MBX_ANALYTIC_LOGARITHMIC;
+			defaultBailout = 10.0;
+			break;
+		case MBX_FORMULA_MANDELBOX:
+			deFunction = MBX_DE_FUNCTION_LINEAR;
+			analyticFunction = MBX_ANALYTIC_LINEAR;
+			defaultBailout = 100.0;
+			break;
+		case MBX_FORMULA_KALEIDOSCOPIC_IFS:
+			deFunction = MBX_DE_FUNCTION_LINEAR;
+			analyticFunction = MBX_ANALYTIC_IFS;
+			defaultBailout = 100.0;
+			cpixelDisabled = true;
+			break;
+		default: return fail(ctx, MBX_ERR_UNSUPPORTED, "formula not supported");
+	}
+	if (fr.deltaDEMethod == 1) deType = MBX_DE_DELTA;
+	if (fr.deltaDEMethod == 2) deType = MBX_DE_ANALYTIC;
+	if (fr.deltaDEFunction != 0)
+	{
+		deFunction = fr.deltaDEFunction;
+		if (deFunction == MBX_DE_FUNCTION_LOGARITHMIC)
+			analyticFunction = MBX_ANALYTIC_LOGARITHMIC;
+		else if (deFunction == MBX_DE_FUNCTION_LINEAR)
+			analyticFunction = MBX_ANALYTIC_LINEAR;
+		else
+			return fail(ctx, MBX_ERR_UNSUPPORTED, "delta_DE_function not supported");
+	}
+	bool addc = !fr.dontAddCConstant;
+	if (cpixelDisabled) addc = !addc;
+	fc.addC = addc;
+	fc.julia = fr.juliaMode;
+	for (int i = 0; i < 3; i++)
+	{
+		fc.juliaC[i] = (float)(fr.juliaC[i] * fr.constantFactor[i]);
+		fc.constMul[i] = (float)fr.constantFactor[i];
+	}
+	fc.initialW = 0.0f;
+	fc.bailout = (float)(fr.useDefaultBailout ? defaultBailout : fr.bailout);
+	fc.formulaMaxiter = fr.formulaMaxiter;
+	fc.deFunction = deFunction;
+	fc.analyticFunction = analyticFunction;
+	fc.power = (float)fr.bulb.power;
+	fc.alpha = (float)(fr.bulb.alphaAngleOffset * M_PI / 180.0);
+	fc.beta = (float)(fr.bulb.betaAngleOffset * M_PI / 180.0);
+
+	const mbx_fractal_mandelbox &mb = fr.mandelbox;
+	fc.mbScale = (float)mb.scale;
+	fc.mbAbsScale = (float)std::fabs(mb.scale);
+	fc.mbFoldingLimit = (float)mb.foldingLimit;
+	fc.mbFoldingValue = (float)mb.foldingValue;
+	const double fR2 = mb.foldingSphericalFixed * mb.foldingSphericalFixed;
+	const double mR2 = mb.foldingSphericalMin * mb.foldingSphericalMin;
+	fc.mbFR2 = (float)fR2;
+	fc.mbMR2 = (float)mR2;
+	fc.mbFactor1 = (float)(fR2 / mR2);
+	set3f(fc.mbOffset, mb.offset);
+
+	const mbx_fractal_ifs &ifs = fr.IFS;
+	fc.ifsScale = (float)ifs.scale;
+	fc.ifsAbsScale = (float)std::fabs(ifs.scale);
+	set3f(fc.ifsOffset, ifs.offset);
+	set3f(fc.ifsEdge, ifs.edge);
+	fc.ifsAbs[0] = ifs.absX;
+	fc.ifsAbs[1] = ifs.absY;
+	fc.ifsAbs[2] = ifs.absZ;
+	fc.ifsMenger = ifs.mengerSpongeMode;
+	fc.ifsRotationEnabled = ifs.rotationEnabled;
+	fc.ifsEdgeEnabled = ifs.edgeEnabled;
+	{
+		// SetRotation3: Z(x), Y(y), X(z) (algebra.cpp:305-311)
+		Rot r;
+		r.z(ifs.rotation[0] * M_PI / 180.0);
+		r.y(ifs.rotation[1] * M_PI / 180.0);
+		r.x(ifs.rotation[2] * M_PI / 180.0);
+		setRot(fc.ifsMainRot, r);
+	}
+	int enabledCount = 0;
+	for (int i = 0; i < MBX_IFS_VECTOR_COUNT; i++)
+	{
+		Rot r;
+		r.z(ifs.rotations[i][0] * M_PI / 180.0);
+		r.y(ifs.rotations[i][1] * M_PI / 180.0);
+		r.x(ifs.rotations[i][2] * M_PI / 180.0);
+		setRot(fc.ifsRot[i], r);
+		const D3 d = d3(ifs.direction[i]);
+		const double l = len(d);
+		set3f(fc.ifsDirection[i], l > 0.0 ? mul(d, 1.0 / l) : d);
+		fc.ifsDistance[i] = (float)ifs.distance[i];
+		fc.ifsIntensity[i] = (float)ifs.intensity[i];
+		fc.ifsEnabled[i] = ifs.enabled[i];
+		enabledCount += ifs.enabled[i] ? 1 : 0;
+	}
+	fc.ifsEnabledCount = enabledCount;
+
+	// kernel specialisation
+	int kind = -1;
+	if (fr.formula == MBX_FORMULA_MANDELBULB && deType == MBX_DE_ANALYTIC)
+	{
+		const bool pow8 = fr.bulb.power == 8.0 && fr.bulb.alphaAngleOffset == 0.0 &&
+			fr.bulb.betaAngleOffset == 0.0 && analyticFunction == MBX_ANALYTIC_LOGARITHMIC;
+		kind = pow8 ? mbx::kBulb8 : mbx::kBulbTrig;
+		if (analyticFunction != MBX_ANALYTIC_LOGARITHMIC && analyticFunction != MBX_ANALYTIC_LINEAR)
+			kind = -1;
+	}
+	else if (fr.formula == MBX_FORMULA_MANDELBOX)
+	{
+		if (mb.rotationsEnabled || mb.mainRotationEnabled)
+			return fail(ctx, MBX_ERR_UNSUPPORTED, "Mandelbox rotated folds are not supported on the device");
+		kind = deType == MBX_DE_DELTA ? mbx::kBoxDelta : mbx::kBoxAnalytic;
+	}
+	else if (fr.formula == MBX_FORMULA_KALEIDOSCOPIC_IFS && deType == MBX_DE_ANALYTIC)
+		kind = mbx::kIfsAnalytic;
+	if (kind < 0) return fail(ctx, MBX_ERR_UNSUPPORTED, "formula / DE mode combination not supported");
+	fc.deKind = kind;
+
+	// light #1 (light.cpp:80-195)
+	const mbx_light &L = p->light;
+	D3 rotation{L.rotation[0] / 180.8 * M_PI, L.rotation[1] / 180.8 * M_PI, L.rotation[2] / 180.8 * M_PI};
+	rotation.y = -rotation.y; // directional light (light.cpp:92-95)
+	D3 ld;
+	if (L.relativePosition)
+	{
+		ld = mul(cf.forward, -1.0);
+		ld = rotate_about(ld, cf.forward, rotation.z);
+		ld = rotate_about(ld, cf.right, rotation.y);
+		ld = rotate_about(ld, cf.top, rotation.x);
+	}
+	else
+	{
+		Rot lr; // SetRotation(CVector3): Z(x), X(y), Y(z)
+		lr.z(rotation.x);
+		lr.x(rotation.y);
+		lr.y(rotation.z);
+		ld = lr.apply(D3{0.0, -1.0, 0.0});
+	}
+	ctx->lightDir[0] = ld.x;
+	ctx->lightDir[1] = ld.y;
+	ctx->lightDir[2] = ld.z;
+	fc.lightEnabled = L.enabled;
+	fc.castShadows = L.castShadows;
+	fc.penetrating = L.penetrating;
+	set3f(fc.lightDir, ld);
+	fc.lightIntensity = (float)L.intensity;
+	fc.lightVisibility = (float)L.visibility;
+	fc.lightSize = (float)L.size;
+	fc.lightContourSharpness = (float)L.contourSharpness;
+	const double softRange = std::tan(L.softShadowCone / 180.0 * M_PI);
+	fc.softRange = (float)softRange;
+	fc.softShadows = softRange > 0.0;
+	set3f(fc.lightColor, L.color);
+	fc.viewDistanceMaxLight = (float)p->viewDistanceMax;
+
+	// material, fill light
+	set3f(fc.surfaceColor, p->surfaceColor);
+	fc.shading = (float)p->shading;
+	for (int i = 0; i < 3; i++)
+	{
+		fc.luminosity[i] = (float)(p->luminosity * p->luminosityColor[i]);
+		fc.fillLight[i] = (float)p->fillLightColor[i];
+	}
+
+	// ambient occlusion
+	fc.aoEnabled = p->ambientOcclusionEnabled;
+	fc.aoMode = p->ambientOcclusionMode;
+	fc.aoQuality = p->ambientOcclusionQuality;
+	fc.aoCount = p->aoVectorsCount;
+	fc.aoIntensity = (float)p->ambientOcclusion;
+	for (int i = 0; i < 3; i++) fc.aoColor[i] = (float)(p->ambientOcclusion * p->ambientOcclusionColor[i]);
+	fc.aoFastTune = (float)p->ambientOcclusionFastTune;
+	if (p->ambientOcclusionEnabled && p->ambientOcclusionMode == MBX_AO_MULTIPLE_RAYS &&
+		(p->aoVectorsCount < 1 || p->aoVectorsCount > MBX_MAX_AO_VECTORS))
+		return fail(ctx, MBX_ERR_INVALID, "aoVectorsCount out of range");
+	if (p->ambientOcclusionEnabled && p->ambientOcclusionMode != MBX_AO_MULTIPLE_RAYS &&
+		p->ambientOcclusionMode != MBX_AO_FAST)
+		return fail(ctx, MBX_ERR_UNSUPPORTED, "screen-space AO is a post effect, not on this path");
+	for (int i = 0; i < MBX_MAX_AO_VECTORS; i++)
+	{
+		set3f(fc.aoVectors[i], p->aoVectors[i]);
+		set3f(fc.aoColors[i], p->aoColors[i]);
+	}
+
+	// background
+	fc.background3Colors = p->background3ColorsEnable;
+	set3f(fc.bg1, p->backgroundColor1);
+	set3f(fc.bg2, p->backgroundColor2);
+	set3f(fc.bg3, p->backgroundColor3);
+	fc.bgBrightness = (float)p->backgroundBrightness;
+	fc.bgInvGamma = (float)(1.0 / p->backgroundGamma);
+
+	// Monte Carlo
+	fc.monteCarlo = p->DOFMonteCarlo;
+	fc.samples = p->DOFMonteCarlo ? p->DOFSamples : 1;
+	if (fc.samples < 1) return fail(ctx, MBX_ERR_INVALID, "DOF_samples < 1");
+	fc.globalIllumination = p->DOFMonteCarloGlobalIllumination;
+	fc.reflectionsMax = p->reflectionsMax;
+	fc.giRadianceLimit = (float)p->monteCarloGIRadianceLimit;
+	if (p->rngMode != MBX_RNG_PER_PIXEL)
+		return fail(ctx, MBX_ERR_UNSUPPORTED, "the device uses per-pixel random streams only");
+	fc.randomSeed = p->randomSeed;
+	return MBX_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int mbx_version(void) { return MBX_ABI_VERSION; }
+
+int mbx_ctx_create(int device, void *stream, mbx_ctx **out)
+{
+	if (!out) return MBX_ERR_INVALID;
+	*out = nullptr;
+	mbx_ctx *ctx = new (std::nothrow) mbx_ctx();
+	if (!ctx) return MBX_ERR_NOMEM;
+	ctx->device = device;
+	hipError_t e = hipSetDevice(device);
+	if (e != hipSuccess)
+	{
+		delete ctx;
+		return MBX_ERR_HIP;
+	}
+	ctx->stream = (hipStream_t)stream;
+	if ((e = hipMalloc((void **)&ctx->d_frame, sizeof(FrameConst))) != hipSuccess ||
+		(e = hipMalloc((void **)&ctx->d_counters, 4 * sizeof(unsigned long long))) != hipSuccess ||
+		(e = hipEventCreate(&ctx->ev0)) != hipSuccess || (e = hipEventCreate(&ctx->ev1)) != hipSuccess)
+	{
+		mbx_ctx_destroy(ctx);
+		return MBX_ERR_HIP;
+	}
+	*out = ctx;
+	return MBX_OK;
+}
+
+void mbx_ctx_destroy(mbx_ctx *ctx)
+{
+	if (!ctx) return;
+	hipSetDevice(ctx->device);
+	if (ctx->stream) hipStreamSynchronize(ctx->stream);
+	else hipDeviceSynchronize();
+	if (ctx->d_frame) hipFree(ctx->d_frame);
+	if (ctx->d_counters) hipFree(ctx->d_counters);
+	if (ctx->ev0) hipEventDestroy(ctx->ev0);
+	if (ctx->ev1) hipEventDestroy(ctx->ev1);
+	delete ctx;
+}
+
+const char *mbx_last_error(const mbx_ctx *ctx)
+{
+	if (!ctx) return "null context";
+	return ctx->err.c_str();
+}
+
+int mbx_scene_upload(mbx_ctx *ctx, const mbx_params *params)
+{
+	if (!ctx || !params) return fail(ctx, MBX_ERR_INVALID, "null argument");
+	FrameConst fc;
+	int rc = derive_frame(ctx, params, fc);
+	if (rc != MBX_OK) return rc;
+	hipError_t e = hipSetDevice(ctx->device);
+	if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+	// the previous launch may still read the old constants
+	if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "sync");
+	if ((e = hipMemcpy(ctx->d_frame, &fc, sizeof(fc), hipMemcpyHostToDevice)) != hipSuccess)
+		return hip_fail(ctx, e, "upload frame constants");
+	ctx->frame = fc;
+	ctx->sceneReady = true;
+	ctx->err.clear();
+	return MBX_OK;
+}
+
+int mbx_scene_info(const mbx_ctx *ctx, double *mrot9, double *fov, double *light_dir3, int *de_kind)
+{
+	if (!ctx || !ctx->sceneReady) return MBX_ERR_INVALID;
+	if (mrot9) std::memcpy(mrot9, ctx->mRot, 9 * sizeof(double));
+	if (fov) *fov = ctx->fov;
+	if (light_dir3) std::memcpy(light_dir3, ctx->lightDir, 3 * sizeof(double));
+	if (de_kind) *de_kind = ctx->frame.deKind;
+	return MBX_OK;
+}
+
+int mbx_render(mbx_ctx *ctx, const mbx_outputs *out)
+{
+	if (!ctx || !out) return fail(ctx, MBX_ERR_INVALID, "null argument");
+	if (!ctx->sceneReady) return fail(ctx, MBX_ERR_INVALID, "no scene uploaded");
+	hipError_t e = hipSetDevice(ctx->device);
+	if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+	mbx::Outputs o{out->rgb, out->depth, out->iters, out->mask, out->normal, out->steps};
+	if ((e = hipMemsetAsync(ctx->d_counters, 0, 4 * sizeof(unsigned long long), ctx->stream)) != hipSuccess)
+		return hip_fail(ctx, e, "clear counters");
+	if ((e = hipEventRecord(ctx->ev0, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
+	e = mbx::launch_render(ctx->frame.deKind, ctx->d_frame, ctx->frame.width, ctx->frame.height, o,
+		ctx->d_counters, ctx->stream);
+	if (e != hipSuccess) return hip_fail(ctx, e, "render launch");
+	if ((e = hipEventRecord(ctx->ev1, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
+	ctx->launched = true;
+	ctx->rays = (uint64_t)ctx->frame.width * ctx->frame.height * (uint64_t)ctx->frame.samples;
+	return MBX_OK;
+}
+
+int mbx_get_stats(mbx_ctx *ctx, mbx_stats *stats)
+{
+	if (!ctx || !stats) return fail(ctx, MBX_ERR_INVALID, "null argument");
+	if (!ctx->launched) return fail(ctx, MBX_ERR_INVALID, "nothing rendered yet");
+	hipError_t e = hipEventSynchronize(ctx->ev1);
+	if (e != hipSuccess) return hip_fail(ctx, e, "render");
+	float ms = 0.0f;
+	if ((e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1)) != hipSuccess) return hip_fail(ctx, e, "timing");
+	unsigned long long c[4];
+	if ((e = hipMemcpy(c, ctx->d_counters, sizeof(c), hipMemcpyDeviceToHost)) != hipSuccess)
+		return hip_fail(ctx, e, "read counters");
+	stats->rays = ctx->rays;
+	stats->de_evals = c[0];
+	stats->iterations = c[1];
+	stats->kernel_ms = ms;
+	return MBX_OK;
+}
+
+int mbx_distance(mbx_ctx *ctx, const double *points, int n, double detail_size, float *dist_out,
+	int32_t *iters_out)
+{
+	if (!ctx || (!points && n > 0) || n < 0) return fail(ctx, MBX_ERR_INVALID, "bad argument");
+	if (!ctx->sceneReady) return fail(ctx, MBX_ERR_INVALID, "no scene uploaded");
+	if (n == 0) return MBX_OK;
+	hipError_t e = hipSetDevice(ctx->device);
+	if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+	std::vector<float> pf(3 * (size_t)n);
+	for (size_t i = 0; i < pf.size(); i++) pf[i] = (float)points[i];
+	float *d_pts = nullptr, *d_dist = nullptr;
+	int32_t *d_it = nullptr;
+	int rc = MBX_OK;
+	if ((e = hipMalloc((void **)&d_pts, pf.size() * sizeof(float))) != hipSuccess ||
+		(e = hipMalloc((void **)&d_dist, (size_t)n * sizeof(float))) != hipSuccess ||
+		(e = hipMalloc((void **)&d_it, (size_t)n * sizeof(int32_t))) != hipSuccess)
+		rc = hip_fail(ctx, e, "allocate");
+	if (rc == MBX_OK &&
+		(e = hipMemcpyAsync(d_pts, pf.data(), pf.size() * sizeof(float), hipMemcpyHostToDevice,
+			 ctx->stream)) != hipSuccess)
+		rc = hip_fail(ctx, e, "copy points");
+	if (rc == MBX_OK &&
+		(e = mbx::launch_distance(ctx->frame.deKind, ctx->d_frame, d_pts, n, (float)detail_size, d_dist,
+			 d_it, ctx->stream)) != hipSuccess)
+		rc = hip_fail(ctx, e, "distance launch");
+	if (rc == MBX_OK && (e = hipStreamSynchronize(ctx->stream)) != hipSuccess) rc = hip_fail(ctx, e, "distance");
+	if (rc == MBX_OK && dist_out &&
+		(e = hipMemcpy(dist_out, d_dist, (size_t)n * sizeof(float), hipMemcpyDeviceToHost)) != hipSuccess)
+		rc = hip_fail(ctx, e, "copy distances");
+	if (rc == MBX_OK && iters_out &&
+		(e = hipMemcpy(iters_out, d_it, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost)) != hipSuccess)
+		rc = hip_fail(ctx, e, "copy iterations");
+	if (d_pts) hipFree(d_pts);
+	if (d_dist) hipFree(d_dist);
+	if (d_it) hipFree(d_it);
+	return rc;
+}
+
+size_t mbx_sizeof_params(void) { return sizeof(mbx_params); }
+
+} // extern "C"

@@ -1,0 +1,160 @@
+"""Python binding of libmbx.so (the C ABI in include/mbx.h).
+
+The library is the product: HIP kernels for gfx950 plus the C++ host.  This module only
+loads it with ctypes and moves torch device buffers across the boundary; it has no compute
+path of its own and raises if the library is missing.
+"""
+import ctypes as C
+import os
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmbx.so")
+
+# every entry point include/mbx.h declares
+EXPORTS = ["mbx_version", "mbx_sizeof_params", "mbx_ctx_create", "mbx_ctx_destroy",
+           "mbx_last_error", "mbx_scene_upload", "mbx_scene_info", "mbx_render", "mbx_distance",
+           "mbx_get_stats"]
+
+_lib = None
+
+
+class MbxError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the in-tree libmbx.so (built by mandelbulber2_amd.build)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MbxError("libmbx.so is not built: run `python -m mandelbulber2_amd.build` "
+                       "(there is no fallback path)")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    L.mbx_version.restype = C.c_int
+    L.mbx_sizeof_params.restype = C.c_size_t
+    L.mbx_ctx_create.argtypes = [C.c_int, C.c_void_p, P(C.c_void_p)]
+    L.mbx_ctx_destroy.argtypes = [C.c_void_p]
+    L.mbx_ctx_destroy.restype = None
+    L.mbx_last_error.argtypes = [C.c_void_p]
+    L.mbx_last_error.restype = C.c_char_p
+    L.mbx_scene_upload.argtypes = [C.c_void_p, P(abi.MbxParams)]
+    L.mbx_scene_info.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.mbx_render.argtypes = [C.c_void_p, P(abi.MbxOutputs)]
+    L.mbx_distance.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_void_p]
+    L.mbx_get_stats.argtypes = [C.c_void_p, P(abi.MbxStats)]
+    _lib = L
+    return L
+
+
+class Context:
+    """One mbx context on a HIP device and stream (torch's current stream by default)."""
+
+    def __init__(self, device=0, stream=None):
+        import torch
+        self.torch = torch
+        self.L = load()
+        self.device = device
+        if stream is None:
+            stream = torch.cuda.current_stream(device)
+        self.stream = stream
+        handle = C.c_void_p()
+        rc = self.L.mbx_ctx_create(device, C.c_void_p(stream.cuda_stream), C.byref(handle))
+        if rc != 0:
+            raise MbxError("mbx_ctx_create failed: %d" % rc)
+        self.h = handle
+        self.params = None
+
+    def close(self):
+        if self.h:
+            self.L.mbx_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise MbxError("%s failed (%d): %s" % (what, rc,
+                                                  self.L.mbx_last_error(self.h).decode()))
+
+    def upload(self, params):
+        self._check(self.L.mbx_scene_upload(self.h, C.byref(params)), "mbx_scene_upload")
+        self.params = params
+
+    def scene_info(self):
+        import numpy as np
+        m = np.zeros(9, np.float64)
+        fov = np.zeros(1, np.float64)
+        light = np.zeros(3, np.float64)
+        kind = C.c_int()
+        self._check(self.L.mbx_scene_info(self.h, m.ctypes.data_as(C.c_void_p),
+                                          fov.ctypes.data_as(C.c_void_p),
+                                          light.ctypes.data_as(C.c_void_p), C.byref(kind)),
+                    "mbx_scene_info")
+        return m.reshape(3, 3), float(fov[0]), light, kind.value
+
+    def alloc_outputs(self, params=None):
+        torch = self.torch
+        p = params or self.params
+        n = p.imageWidth * p.imageHeight
+        dev = torch.device("cuda", self.device)
+        return {"rgb": torch.empty((p.imageHeight, p.imageWidth, 3), dtype=torch.float32, device=dev),
+                "depth": torch.empty((p.imageHeight, p.imageWidth), dtype=torch.float32, device=dev),
+                "iters": torch.empty((p.imageHeight, p.imageWidth), dtype=torch.int32, device=dev),
+                "mask": torch.empty((p.imageHeight, p.imageWidth), dtype=torch.uint8, device=dev),
+                "normal": torch.empty((p.imageHeight, p.imageWidth, 3), dtype=torch.float32, device=dev),
+                "steps": torch.empty((p.imageHeight, p.imageWidth), dtype=torch.int32, device=dev),
+                "_n": n}
+
+    @staticmethod
+    def outputs_struct(bufs):
+        o = abi.MbxOutputs()
+        for name in ("rgb", "depth", "iters", "mask", "normal", "steps"):
+            t = bufs.get(name)
+            setattr(o, name, t.data_ptr() if t is not None else None)
+        return o
+
+    def render(self, bufs):
+        o = self.outputs_struct(bufs)
+        self._check(self.L.mbx_render(self.h, C.byref(o)), "mbx_render")
+
+    def stats(self):
+        s = abi.MbxStats()
+        self._check(self.L.mbx_get_stats(self.h, C.byref(s)), "mbx_get_stats")
+        return {"rays": s.rays, "de_evals": s.de_evals, "iterations": s.iterations,
+                "kernel_ms": s.kernel_ms}
+
+    def distance(self, pts, detail_size):
+        import numpy as np
+        pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+        n = len(pts)
+        d = np.zeros(n, np.float32)
+        it = np.zeros(n, np.int32)
+        self._check(self.L.mbx_distance(self.h, pts.ctypes.data_as(C.c_void_p), n,
+                                        float(detail_size), d.ctypes.data_as(C.c_void_p),
+                                        it.ctypes.data_as(C.c_void_p)), "mbx_distance")
+        return d, it
+
+
+def render_to_host(params, device=0):
+    """Upload, render and copy all planes back as numpy arrays (plus work statistics)."""
+    import torch
+    ctx = Context(device)
+    try:
+        ctx.upload(params)
+        bufs = ctx.alloc_outputs()
+        ctx.render(bufs)
+        st = ctx.stats()
+        torch.cuda.synchronize(device)
+        out = {k: v.cpu().numpy() for k, v in bufs.items() if not k.startswith("_")}
+        out.update(st)
+        return out
+    finally:
+        ctx.close()

@@ -1,0 +1,47 @@
+"""The C-ABI library loads and exports every symbol include/mbx.h declares; the ctypes
+mirror of mbx_params matches the compiled layout (no GPU calls)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from mandelbulber2_amd import abi, runtime
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for fn in os.listdir(inc):
+        if fn.endswith(".h"):
+            text = open(os.path.join(inc, fn)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names.update(re.findall(r"\b(mbx_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_declared_symbols():
+    lib = runtime.load()
+    names = declared_symbols()
+    assert len(names) >= 10
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(runtime.EXPORTS)
+
+
+def test_params_layout_matches_library(oracle_mod):
+    lib = runtime.load()
+    assert lib.mbx_version() == 1
+    assert lib.mbx_sizeof_params() == ctypes.sizeof(abi.MbxParams)
+    assert oracle_mod.lib().mbxo_sizeof_params() == ctypes.sizeof(abi.MbxParams)
+
+
+def test_product_does_not_import_oracle():
+    pkg = os.path.join(ROOT, "mandelbulber2_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for fn in files:
+            if fn.endswith((".py", ".cpp", ".hip", ".h")):
+                text = open(os.path.join(dirpath, fn)).read()
+                assert "oracle" not in text.replace("oracle.", "").lower() or fn == "__init__.py", fn
