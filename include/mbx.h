@@ -285,6 +285,28 @@ int mbx_render(mbx_ctx *ctx, const mbx_outputs *out);
 int mbx_distance(mbx_ctx *ctx, const double *points, int n, double detail_size,
 	float *dist_out, int32_t *iters_out);
 
+/* Render n frames (each its own parameter block and output planes) in ONE persistent,
+ * work-stealing launch replayed from a captured HIP graph.  The frames may differ in
+ * resolution, camera and formula parameters but must share the fractal formula family.
+ * Replaces the per-frame job loop of cKeyframeAnimation::RenderKeyframes
+ * (mandelbulber2/src/animation_keyframes.cpp:1058-1111), which runs one full render (and one
+ * OpenCL launch) per frame.  Asynchronous; mbx_get_stats synchronises. */
+int mbx_render_batch(mbx_ctx *ctx, const mbx_params *frames, const mbx_outputs *outs, int n);
+
+/* Launch tuning keys for mbx_tune. */
+#define MBX_TUNE_KERNEL 0    /* 0 = persistent work-stealing kernel (default), 1 = plain per-frame kernel */
+#define MBX_TUNE_WAVES 1     /* persistent kernel: number of 64-lane waves launched (0 = default) */
+#define MBX_TUNE_GRAPH 2     /* persistent kernel: 1 = replay a captured HIP graph (default), 0 = direct */
+
+/* Set a launch tuning value (see MBX_TUNE_*).  No reference counterpart: the OpenCL engine's
+ * adaptive job size (opencl_engine.cpp:429-450) is the closest analogue. */
+int mbx_tune(mbx_ctx *ctx, int key, int value);
+
+/* Project autotuner: time the uploaded scene with the persistent kernel for a set of wave
+ * counts (`reps` renders each into `out`) and keep the fastest.  Returns the chosen wave count
+ * in *best_waves and its time per render in *best_ms (either may be NULL). */
+int mbx_autotune(mbx_ctx *ctx, const mbx_outputs *out, int reps, int *best_waves, double *best_ms);
+
 /* Statistics of the last render (synchronises the stream). */
 int mbx_get_stats(mbx_ctx *ctx, mbx_stats *stats);
 

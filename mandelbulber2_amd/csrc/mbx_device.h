@@ -372,3 +372,129 @@ __device__ __forceinline__ float calculate_distance(V3 p, float detailSize, cons
 }
 
 } // namespace mbx
+
+namespace mbx
+{
+
+// ------------------------------------------------------------------------------------------
+// Resumable DE evaluation for the persistent kernels: the Compute loop split into begin /
+// one-iteration step / finish, so that every lane of a wave advances its own evaluation by
+// one formula iteration per loop trip.
+// ------------------------------------------------------------------------------------------
+template <int KIND>
+struct DEState
+{
+	V3 p;       // evaluation point
+	V3 z;
+	float r2;   // |z|^2 (aux.r squared)
+	float DE;   // running derivative
+	int it;     // formula iterations done
+	int status; // 0 running, 1 escaped, 2 iteration limit, 3 non-finite orbit (reverted)
+	V3 J[KIND == kBoxDelta ? 3 : 1];
+};
+
+template <int KIND>
+__device__ __forceinline__ void de_begin(DEState<KIND> &s, V3 p)
+{
+	s.p = p;
+	s.z = p;
+	s.r2 = dot(p, p);
+	s.DE = 1.0f;
+	s.it = 0;
+	s.status = 0;
+	if (KIND == kBoxDelta)
+	{
+		s.J[0] = v3(1.0f, 0.0f, 0.0f);
+		s.J[1] = v3(0.0f, 1.0f, 0.0f);
+		s.J[2] = v3(0.0f, 0.0f, 1.0f);
+	}
+}
+
+template <int KIND>
+__device__ __forceinline__ void de_step(DEState<KIND> &s, const FrameConst &f)
+{
+	const V3 lastZ = s.z;
+	if (KIND == kBulb8)
+		MandelbulbPow8Iteration(s.z, s.r2, s.DE);
+	else if (KIND == kBulbTrig)
+		MandelbulbIteration(s.z, sqrtf(s.r2), s.DE, f);
+	else if (KIND == kBoxAnalytic)
+		MandelboxIteration<false>(s.z, s.DE, nullptr, f);
+	else if (KIND == kBoxDelta)
+		MandelboxIteration<true>(s.z, s.DE, s.J, f);
+	else
+		KaleidoscopicIfsIteration(s.z, s.DE, f);
+	s.z = add_c(s.z, s.p, f);
+	if (KIND == kBoxDelta && f.addC && !f.julia)
+	{
+		s.J[0].x += f.constMul[0];
+		s.J[1].y += f.constMul[1];
+		s.J[2].z += f.constMul[2];
+	}
+	s.r2 = dot(s.z, s.z);
+	s.it++;
+	if (!finite3(s.z))
+	{
+		s.z = lastZ;
+		s.r2 = dot(lastZ, lastZ);
+		s.status = 3;
+	}
+	else if (s.r2 > f.bailout * f.bailout)
+		s.status = 1;
+	else if (s.it >= f.formulaMaxiter)
+		s.status = 2;
+}
+
+// Distance of a finished evaluation after CalculateDistanceSimple's clamps; `iters` gets the
+// reference's iteration count (i + 1, maxN + 1 for a non-escaping orbit).
+template <int KIND>
+__device__ __forceinline__ float de_finish(DEState<KIND> &s, const FrameConst &f, float detailSize,
+	int &iters)
+{
+	iters = s.status == 2 ? s.it + 1 : s.it;
+	const float r = sqrtf(s.r2);
+	float d;
+	if (KIND == kBoxDelta)
+	{
+		float dr;
+		if (s.status == 1)
+		{
+			const float inv_r = 1.0f / r;
+			const float d1 = fabsf(dot(s.z, s.J[0])) * inv_r;
+			const float d2 = fabsf(dot(s.z, s.J[1])) * inv_r;
+			const float d3 = fabsf(dot(s.z, s.J[2])) * inv_r;
+			dr = sqrtf(d1 * d1 + d2 * d2 + d3 * d3);
+		}
+		else
+		{
+			V3 z2 = s.z;
+			float DE2 = s.DE;
+			MandelboxIteration<false>(z2, DE2, nullptr, f);
+			z2 = add_c(z2, s.p, f);
+			const float delta = fmaxf(length(s.p) * 1e-14f, 1e-5f * detailSize);
+			dr = 1.7320508f * fabsf(length(z2) - r) / delta;
+		}
+		if (dr > 0.0f)
+			d = (f.deFunction == MBX_DE_FUNCTION_LOGARITHMIC) ? 0.5f * r * __logf(r) / dr : 0.5f * r / dr;
+		else
+			d = r;
+	}
+	else if (s.DE > 0.0f)
+	{
+		if (f.analyticFunction == MBX_ANALYTIC_LOGARITHMIC)
+			d = (r > 1.0f) ? 0.5f * r * __logf(r) / s.DE : 0.0f;
+		else if (f.analyticFunction == MBX_ANALYTIC_IFS)
+			d = (r - 2.0f) / s.DE;
+		else
+			d = r / s.DE;
+	}
+	else
+		d = r;
+	if (iters < f.minN && d < detailSize) d = detailSize;
+	d = (d < 0.0f) ? 0.0f : d;
+	d = (d > 10.0f) ? 10.0f : d;
+	if (!(d < 1e20f)) d = 1e20f;
+	return d;
+}
+
+} // namespace mbx

@@ -33,6 +33,20 @@ struct mbx_ctx
 	double lightDir[3] = {0, 0, 0};
 	uint64_t rays = 0;
 	std::string err;
+	// persistent / batched launches
+	int kernelMode = 0; // 0 persistent, 1 plain
+	int waves = 0;      // 0 = default
+	int useGraph = 1;
+	int numCUs = 256;
+	FrameConst *d_frames = nullptr;
+	mbx::FrameJob *d_jobs = nullptr;
+	int capacity = 0;
+	unsigned int *d_queue = nullptr;
+	std::vector<mbx::FrameJob> jobsHost; // what d_jobs holds
+	std::vector<FrameConst> framesHost;  // what d_frames holds
+	int batchKind = -1, batchFeat = 0, batchFrames = 0;
+	hipGraphExec_t graphExec = nullptr;
+	int graphKind = -1, graphFeat = -1, graphFrames = -1, graphWaves = -1;
 };
 
 namespace
@@ -437,6 +451,168 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 	return MBX_OK;
 }
 
+int feature_mask(const FrameConst &fc)
+{
+	int feat = 0;
+	if (fc.castShadows && fc.lightEnabled) feat |= mbx::kFeatShadow;
+	if (fc.aoEnabled) feat |= mbx::kFeatAO;
+	if (fc.monteCarlo) feat |= mbx::kFeatMC;
+	return feat;
+}
+
+int default_waves(const mbx_ctx *ctx) { return ctx->numCUs * 16; }
+
+int ensure_capacity(mbx_ctx *ctx, int n)
+{
+	if (n <= ctx->capacity) return MBX_OK;
+	hipError_t e;
+	if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "sync");
+	if (ctx->d_frames) (void)hipFree(ctx->d_frames);
+	if (ctx->d_jobs) (void)hipFree(ctx->d_jobs);
+	ctx->d_frames = nullptr;
+	ctx->d_jobs = nullptr;
+	ctx->capacity = 0;
+	ctx->framesHost.clear();
+	ctx->jobsHost.clear();
+	if ((e = hipMalloc((void **)&ctx->d_frames, (size_t)n * sizeof(FrameConst))) != hipSuccess ||
+		(e = hipMalloc((void **)&ctx->d_jobs, (size_t)n * sizeof(mbx::FrameJob))) != hipSuccess)
+		return hip_fail(ctx, e, "allocate batch buffers");
+	ctx->capacity = n;
+	ctx->graphFrames = -1; // buffers moved: the captured graph is stale
+	return MBX_OK;
+}
+
+mbx::FrameJob make_job(const FrameConst &fc, const mbx_outputs &o, int tileBase)
+{
+	mbx::FrameJob j{};
+	j.tileEnd = tileBase + ((fc.width + 7) / 8) * ((fc.height + 7) / 8);
+	j.rgb = o.rgb;
+	j.depth = o.depth;
+	j.iters = o.iters;
+	j.mask = o.mask;
+	j.normal = o.normal;
+	j.steps = o.steps;
+	return j;
+}
+
+bool same_jobs(const std::vector<mbx::FrameJob> &a, const std::vector<mbx::FrameJob> &b)
+{
+	return a.size() == b.size() &&
+		(a.empty() || std::memcmp(a.data(), b.data(), a.size() * sizeof(mbx::FrameJob)) == 0);
+}
+
+bool same_frames(const std::vector<FrameConst> &a, const std::vector<FrameConst> &b)
+{
+	return a.size() == b.size() &&
+		(a.empty() || std::memcmp(a.data(), b.data(), a.size() * sizeof(FrameConst)) == 0);
+}
+
+// Upload frames and jobs when they differ from what the device holds.
+int stage_batch(mbx_ctx *ctx, const std::vector<FrameConst> &frames, const std::vector<mbx::FrameJob> &jobs)
+{
+	int rc = ensure_capacity(ctx, (int)frames.size());
+	if (rc != MBX_OK) return rc;
+	const bool fSame = same_frames(frames, ctx->framesHost);
+	const bool jSame = same_jobs(jobs, ctx->jobsHost);
+	if (fSame && jSame) return MBX_OK;
+	hipError_t e;
+	// an in-flight launch may still read the old batch
+	if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "sync");
+	if (!fSame &&
+		(e = hipMemcpy(ctx->d_frames, frames.data(), frames.size() * sizeof(FrameConst), hipMemcpyHostToDevice)) !=
+			hipSuccess)
+		return hip_fail(ctx, e, "upload frames");
+	if (!jSame &&
+		(e = hipMemcpy(ctx->d_jobs, jobs.data(), jobs.size() * sizeof(mbx::FrameJob), hipMemcpyHostToDevice)) !=
+			hipSuccess)
+		return hip_fail(ctx, e, "upload jobs");
+	ctx->framesHost = frames;
+	ctx->jobsHost = jobs;
+	return MBX_OK;
+}
+
+// Enqueue the clear + persistent launch, from a captured graph when enabled.
+int enqueue_persistent(mbx_ctx *ctx, int kind, int feat, int nFrames)
+{
+	const int waves = ctx->waves > 0 ? ctx->waves : default_waves(ctx);
+	hipError_t e;
+	if (!ctx->useGraph)
+	{
+		if ((e = mbx::launch_clear(ctx->d_queue, ctx->d_counters, ctx->stream)) != hipSuccess)
+			return hip_fail(ctx, e, "clear launch");
+		if ((e = mbx::launch_persistent(kind, feat, ctx->d_frames, ctx->d_jobs, nFrames, ctx->d_queue,
+				 ctx->d_counters, waves, ctx->stream)) != hipSuccess)
+			return hip_fail(ctx, e, "persistent launch");
+		return MBX_OK;
+	}
+	if (!ctx->graphExec || ctx->graphKind != kind || ctx->graphFeat != feat || ctx->graphFrames != nFrames ||
+		ctx->graphWaves != waves)
+	{
+		if (ctx->graphExec)
+		{
+			(void)hipGraphExecDestroy(ctx->graphExec);
+			ctx->graphExec = nullptr;
+		}
+		hipStream_t cs = nullptr;
+		if ((e = hipStreamCreate(&cs)) != hipSuccess) return hip_fail(ctx, e, "capture stream");
+		hipGraph_t g = nullptr;
+		int rc = MBX_OK;
+		if ((e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+			rc = hip_fail(ctx, e, "begin capture");
+		if (rc == MBX_OK)
+		{
+			hipError_t e1 = mbx::launch_clear(ctx->d_queue, ctx->d_counters, cs);
+			hipError_t e2 = mbx::launch_persistent(kind, feat, ctx->d_frames, ctx->d_jobs, nFrames, ctx->d_queue,
+				ctx->d_counters, waves, cs);
+			e = hipStreamEndCapture(cs, &g);
+			if (e1 != hipSuccess) rc = hip_fail(ctx, e1, "capture clear");
+			else if (e2 != hipSuccess) rc = hip_fail(ctx, e2, "capture persistent");
+			else if (e != hipSuccess) rc = hip_fail(ctx, e, "end capture");
+		}
+		if (rc == MBX_OK && (e = hipGraphInstantiate(&ctx->graphExec, g, nullptr, nullptr, 0)) != hipSuccess)
+			rc = hip_fail(ctx, e, "instantiate graph");
+		if (g) (void)hipGraphDestroy(g);
+		(void)hipStreamDestroy(cs);
+		if (rc != MBX_OK)
+		{
+			ctx->graphExec = nullptr;
+			return rc;
+		}
+		ctx->graphKind = kind;
+		ctx->graphFeat = feat;
+		ctx->graphFrames = nFrames;
+		ctx->graphWaves = waves;
+	}
+	if ((e = hipGraphLaunch(ctx->graphExec, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "graph launch");
+	return MBX_OK;
+}
+
+int render_frames(mbx_ctx *ctx, const std::vector<FrameConst> &frames, const mbx_outputs *outs)
+{
+	hipError_t e = hipSetDevice(ctx->device);
+	if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+	int kind = frames[0].deKind, feat = 0;
+	uint64_t rays = 0;
+	std::vector<mbx::FrameJob> jobs;
+	int tiles = 0;
+	for (size_t i = 0; i < frames.size(); i++)
+	{
+		feat |= feature_mask(frames[i]);
+		jobs.push_back(make_job(frames[i], outs[i], tiles));
+		tiles = jobs.back().tileEnd;
+		rays += (uint64_t)frames[i].width * frames[i].height * (uint64_t)frames[i].samples;
+	}
+	int rc = stage_batch(ctx, frames, jobs);
+	if (rc != MBX_OK) return rc;
+	if ((e = hipEventRecord(ctx->ev0, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
+	rc = enqueue_persistent(ctx, kind, feat, (int)frames.size());
+	if (rc != MBX_OK) return rc;
+	if ((e = hipEventRecord(ctx->ev1, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
+	ctx->launched = true;
+	ctx->rays = rays;
+	return MBX_OK;
+}
+
 } // namespace
 
 extern "C" {
@@ -459,11 +635,15 @@ int mbx_ctx_create(int device, void *stream, mbx_ctx **out)
 	ctx->stream = (hipStream_t)stream;
 	if ((e = hipMalloc((void **)&ctx->d_frame, sizeof(FrameConst))) != hipSuccess ||
 		(e = hipMalloc((void **)&ctx->d_counters, 4 * sizeof(unsigned long long))) != hipSuccess ||
+		(e = hipMalloc((void **)&ctx->d_queue, 64)) != hipSuccess ||
 		(e = hipEventCreate(&ctx->ev0)) != hipSuccess || (e = hipEventCreate(&ctx->ev1)) != hipSuccess)
 	{
 		mbx_ctx_destroy(ctx);
 		return MBX_ERR_HIP;
 	}
+	int cus = 0;
+	if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+		ctx->numCUs = cus;
 	*out = ctx;
 	return MBX_OK;
 }
@@ -478,6 +658,10 @@ void mbx_ctx_destroy(mbx_ctx *ctx)
 	if (ctx->d_counters) hipFree(ctx->d_counters);
 	if (ctx->ev0) hipEventDestroy(ctx->ev0);
 	if (ctx->ev1) hipEventDestroy(ctx->ev1);
+	if (ctx->graphExec) (void)hipGraphExecDestroy(ctx->graphExec);
+	if (ctx->d_frames) (void)hipFree(ctx->d_frames);
+	if (ctx->d_jobs) (void)hipFree(ctx->d_jobs);
+	if (ctx->d_queue) (void)hipFree(ctx->d_queue);
 	delete ctx;
 }
 
@@ -519,10 +703,15 @@ int mbx_render(mbx_ctx *ctx, const mbx_outputs *out)
 {
 	if (!ctx || !out) return fail(ctx, MBX_ERR_INVALID, "null argument");
 	if (!ctx->sceneReady) return fail(ctx, MBX_ERR_INVALID, "no scene uploaded");
+	if (ctx->kernelMode == 0)
+	{
+		std::vector<FrameConst> frames(1, ctx->frame);
+		return render_frames(ctx, frames, out);
+	}
 	hipError_t e = hipSetDevice(ctx->device);
 	if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
 	mbx::Outputs o{out->rgb, out->depth, out->iters, out->mask, out->normal, out->steps};
-	if ((e = hipMemsetAsync(ctx->d_counters, 0, 4 * sizeof(unsigned long long), ctx->stream)) != hipSuccess)
+	if ((e = mbx::launch_clear(ctx->d_queue, ctx->d_counters, ctx->stream)) != hipSuccess)
 		return hip_fail(ctx, e, "clear counters");
 	if ((e = hipEventRecord(ctx->ev0, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
 	e = mbx::launch_render(ctx->frame.deKind, ctx->d_frame, ctx->frame.width, ctx->frame.height, o,
@@ -531,6 +720,94 @@ int mbx_render(mbx_ctx *ctx, const mbx_outputs *out)
 	if ((e = hipEventRecord(ctx->ev1, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
 	ctx->launched = true;
 	ctx->rays = (uint64_t)ctx->frame.width * ctx->frame.height * (uint64_t)ctx->frame.samples;
+	return MBX_OK;
+}
+
+int mbx_render_batch(mbx_ctx *ctx, const mbx_params *params, const mbx_outputs *outs, int n)
+{
+	if (!ctx || !params || !outs || n < 1) return fail(ctx, MBX_ERR_INVALID, "bad argument");
+	std::vector<FrameConst> frames((size_t)n);
+	for (int i = 0; i < n; i++)
+	{
+		int rc = derive_frame(ctx, &params[i], frames[i]);
+		if (rc != MBX_OK) return rc;
+	}
+	// one kernel specialisation per launch: Mandelbulb frames of mixed powers use the general
+	// (trigonometric) iteration
+	int kind = frames[0].deKind;
+	for (int i = 1; i < n; i++)
+	{
+		int k = frames[i].deKind;
+		if (k == kind) continue;
+		const bool bulbs = (k == mbx::kBulb8 || k == mbx::kBulbTrig) && (kind == mbx::kBulb8 || kind == mbx::kBulbTrig);
+		if (!bulbs) return fail(ctx, MBX_ERR_UNSUPPORTED, "a batch must share one fractal formula and DE mode");
+		kind = mbx::kBulbTrig;
+	}
+	for (auto &f : frames) f.deKind = kind;
+	return render_frames(ctx, frames, outs);
+}
+
+int mbx_tune(mbx_ctx *ctx, int key, int value)
+{
+	if (!ctx) return MBX_ERR_INVALID;
+	switch (key)
+	{
+		case MBX_TUNE_KERNEL:
+			if (value != 0 && value != 1) return fail(ctx, MBX_ERR_INVALID, "kernel mode must be 0 or 1");
+			ctx->kernelMode = value;
+			return MBX_OK;
+		case MBX_TUNE_WAVES:
+			if (value < 0 || value > (1 << 22)) return fail(ctx, MBX_ERR_INVALID, "wave count out of range");
+			ctx->waves = value;
+			return MBX_OK;
+		case MBX_TUNE_GRAPH:
+			ctx->useGraph = value ? 1 : 0;
+			return MBX_OK;
+		default: return fail(ctx, MBX_ERR_INVALID, "unknown tuning key");
+	}
+}
+
+int mbx_autotune(mbx_ctx *ctx, const mbx_outputs *out, int reps, int *best_waves, double *best_ms)
+{
+	if (!ctx || !out || reps < 1) return fail(ctx, MBX_ERR_INVALID, "bad argument");
+	if (!ctx->sceneReady) return fail(ctx, MBX_ERR_INVALID, "no scene uploaded");
+	const int savedMode = ctx->kernelMode;
+	ctx->kernelMode = 0;
+	const int perCU[] = {4, 8, 12, 16, 20, 24, 32};
+	double bestT = 1e30;
+	int bestW = default_waves(ctx);
+	for (int k : perCU)
+	{
+		ctx->waves = ctx->numCUs * k;
+		double total = 0.0;
+		for (int r = 0; r < reps + 1; r++)
+		{
+			int rc = mbx_render(ctx, out);
+			if (rc != MBX_OK)
+			{
+				ctx->kernelMode = savedMode;
+				return rc;
+			}
+			mbx_stats st;
+			rc = mbx_get_stats(ctx, &st);
+			if (rc != MBX_OK)
+			{
+				ctx->kernelMode = savedMode;
+				return rc;
+			}
+			if (r > 0) total += st.kernel_ms; // first run of a setting: graph capture, warm-up
+		}
+		const double t = total / reps;
+		if (t < bestT)
+		{
+			bestT = t;
+			bestW = ctx->waves;
+		}
+	}
+	ctx->waves = bestW;
+	ctx->kernelMode = savedMode;
+	if (best_waves) *best_waves = bestW;
+	if (best_ms) *best_ms = bestT;
 	return MBX_OK;
 }
 

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libmbx.so")
 # every entry point include/mbx.h declares
 EXPORTS = ["mbx_version", "mbx_sizeof_params", "mbx_ctx_create", "mbx_ctx_destroy",
            "mbx_last_error", "mbx_scene_upload", "mbx_scene_info", "mbx_render", "mbx_distance",
-           "mbx_get_stats"]
+           "mbx_get_stats", "mbx_render_batch", "mbx_tune", "mbx_autotune"]
 
 _lib = None
 
@@ -46,8 +46,18 @@ def load():
     L.mbx_render.argtypes = [C.c_void_p, P(abi.MbxOutputs)]
     L.mbx_distance.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_void_p]
     L.mbx_get_stats.argtypes = [C.c_void_p, P(abi.MbxStats)]
+    L.mbx_render_batch.argtypes = [C.c_void_p, P(abi.MbxParams), P(abi.MbxOutputs), C.c_int]
+    L.mbx_tune.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.mbx_autotune.argtypes = [C.c_void_p, P(abi.MbxOutputs), C.c_int, P(C.c_int), P(C.c_double)]
     _lib = L
     return L
+
+
+MBX_TUNE_KERNEL = 0
+MBX_TUNE_WAVES = 1
+MBX_TUNE_GRAPH = 2
+KERNEL_PERSISTENT = 0
+KERNEL_PLAIN = 1
 
 
 class Context:
@@ -125,6 +135,23 @@ class Context:
         o = self.outputs_struct(bufs)
         self._check(self.L.mbx_render(self.h, C.byref(o)), "mbx_render")
 
+    def render_batch(self, params_list, bufs_list):
+        n = len(params_list)
+        arr = (abi.MbxParams * n)(*params_list)
+        outs = (abi.MbxOutputs * n)(*[self.outputs_struct(b) for b in bufs_list])
+        self._check(self.L.mbx_render_batch(self.h, arr, outs, n), "mbx_render_batch")
+
+    def tune(self, key, value):
+        self._check(self.L.mbx_tune(self.h, key, value), "mbx_tune")
+
+    def autotune(self, bufs, reps=3):
+        o = self.outputs_struct(bufs)
+        w = C.c_int()
+        ms = C.c_double()
+        self._check(self.L.mbx_autotune(self.h, C.byref(o), reps, C.byref(w), C.byref(ms)),
+                    "mbx_autotune")
+        return w.value, ms.value
+
     def stats(self):
         s = abi.MbxStats()
         self._check(self.L.mbx_get_stats(self.h, C.byref(s)), "mbx_get_stats")
@@ -143,11 +170,12 @@ class Context:
         return d, it
 
 
-def render_to_host(params, device=0):
+def render_to_host(params, device=0, kernel=KERNEL_PERSISTENT):
     """Upload, render and copy all planes back as numpy arrays (plus work statistics)."""
     import torch
     ctx = Context(device)
     try:
+        ctx.tune(MBX_TUNE_KERNEL, kernel)
         ctx.upload(params)
         bufs = ctx.alloc_outputs()
         ctx.render(bufs)

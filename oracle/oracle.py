@@ -41,6 +41,7 @@ def lib(fast=False):
     L.mbxo_march_pixel.argtypes = [P(abi.MbxParams), C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.mbxo_normal.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_double, C.c_void_p]
+    L.mbxo_background.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_void_p]
     L.mbxo_random_int.argtypes = [P(C.c_uint32)]
     L.mbxo_random.argtypes = [P(C.c_uint32), C.c_int]
     L.mbxo_pixel_seed.argtypes = [C.c_uint32, C.c_uint32]
@@ -164,3 +165,13 @@ def random_ints(seed, count, maxv=None):
 
 def pixel_seed(pixel, frame_seed):
     return lib().mbxo_pixel_seed(pixel, frame_seed)
+
+
+def background(p, direction):
+    L = lib()
+    d = np.ascontiguousarray(direction, np.float64)
+    rgb = np.zeros(3, np.float32)
+    rc = L.mbxo_background(C.byref(p), _ptr(d), _ptr(rgb))
+    if rc != 0:
+        raise RuntimeError("oracle background failed: %d" % rc)
+    return rgb

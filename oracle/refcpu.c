@@ -1632,3 +1632,16 @@ int mbxo_normal(const mbx_params *p, const double *pt3, double distThresh, doubl
 /* Struct sizes for binding-layout checks. */
 size_t mbxo_sizeof_params(void) { return sizeof(mbx_params); }
 size_t mbxo_offsetof_fractal(void) { return offsetof(mbx_params, fractal); }
+
+/* Background colour seen along a view direction (BackgroundShader). */
+int mbxo_background(const mbx_params *p, const double *dir3, float *rgb3)
+{
+	scene_t s;
+	int rc0 = scene_init(&s, p);
+	if (rc0) return rc0;
+	rgba c = background_shader(&s, V3(dir3[0], dir3[1], dir3[2]));
+	rgb3[0] = c.R;
+	rgb3[1] = c.G;
+	rgb3[2] = c.B;
+	return MBX_OK;
+}
