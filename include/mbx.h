@@ -297,8 +297,11 @@ int mbx_render_batch(mbx_ctx *ctx, const mbx_params *frames, const mbx_outputs *
 #define MBX_TUNE_KERNEL 0    /* 0 = persistent work-stealing kernel (default), 1 = plain per-frame kernel */
 #define MBX_TUNE_WAVES 1     /* persistent kernel: number of 64-lane waves launched (0 = default) */
 #define MBX_TUNE_GRAPH 2     /* persistent kernel: 1 = replay a captured HIP graph (default), 0 = direct */
+#define MBX_TUNE_DELTA_DE 3  /* delta DE: 0 = the reference's finite differences (default),
+                                1 = forward-mode tangents (smooth limit of the same quotient) */
 
-/* Set a launch tuning value (see MBX_TUNE_*).  No reference counterpart: the OpenCL engine's
+/* Set a launch or numerics option (see MBX_TUNE_*); scene options apply from the next
+ * mbx_scene_upload / mbx_render_batch.  No reference counterpart: the OpenCL engine's
  * adaptive job size (opencl_engine.cpp:429-450) is the closest analogue. */
 int mbx_tune(mbx_ctx *ctx, int key, int value);
 

@@ -19,17 +19,21 @@ struct V3
 	float x, y, z;
 };
 
+// Hardware square root / reciprocal / reciprocal square root (v_sqrt_f32, v_rcp_f32,
+// v_rsq_f32: about 1 ulp) instead of the correctly rounded multi-instruction sequences; the
+// kernels are compared against an FP64 reference, where these errors are far below tolerance.
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float fdiv(float a, float b) { return a * frcp(b); }
+
 __device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
 __device__ __forceinline__ V3 operator+(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
 __device__ __forceinline__ V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
 __device__ __forceinline__ V3 operator*(V3 a, float s) { return V3{a.x * s, a.y * s, a.z * s}; }
 __device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ float length(V3 a) { return sqrtf(dot(a, a)); }
-__device__ __forceinline__ V3 normalize(V3 a)
-{
-	const float k = 1.0f / length(a);
-	return a * k;
-}
+__device__ __forceinline__ float length(V3 a) { return fsqrt(dot(a, a)); }
+__device__ __forceinline__ V3 normalize(V3 a) { return a * frsq(dot(a, a)); }
 __device__ __forceinline__ bool finite3(V3 a)
 {
 	return isfinite(a.x) && isfinite(a.y) && isfinite(a.z);
@@ -89,8 +93,8 @@ __device__ __forceinline__ void unit_pow8(float c, float s, float &c8, float &s8
 __device__ __forceinline__ void MandelbulbPow8Iteration(V3 &z, float r2, float &DE)
 {
 	const float rho2 = z.x * z.x + z.y * z.y;
-	const float inv_r = __builtin_amdgcn_rsqf(r2);
-	const float inv_rho = __builtin_amdgcn_rsqf(rho2);
+	const float inv_r = frsq(r2);
+	const float inv_rho = frsq(rho2);
 	const float rho = rho2 * inv_rho;
 	float ct8, st8, cp8, sp8;
 	unit_pow8(rho * inv_r, z.z * inv_r, ct8, st8);
@@ -108,7 +112,7 @@ __device__ __forceinline__ void MandelbulbPow8Iteration(V3 &z, float r2, float &
 // Mandelbulb, any power and angle offsets: the reference's trigonometric form.
 __device__ __forceinline__ void MandelbulbIteration(V3 &z, float r, float &DE, const FrameConst &f)
 {
-	const float th0 = asinf(z.z / r) + f.beta;
+	const float th0 = asinf(z.z * frcp(r)) + f.beta;
 	const float ph0 = atan2f(z.y, z.x) + f.alpha;
 	float rp = __expf(__logf(r) * (f.power - 1.0f));
 	const float th = th0 * f.power;
@@ -149,12 +153,12 @@ __device__ __forceinline__ void MandelboxIteration(V3 &z, float &DE, V3 *J, cons
 	const V3 zo = z + o;
 	const bool inner = r2 < f.mbMR2;
 	const bool outer = !inner && r2 < f.mbFR2;
-	const float t2 = f.mbFR2 / r2;
+	const float t2 = f.mbFR2 * frcp(r2);
 	const float k = inner ? f.mbFactor1 : (outer ? t2 : 1.0f);
 	if (TANGENT)
 	{
 		// d[(z + o) t(z)] = t dz - 2 t (z.dz) / r2 (z + o) in the outer regime, k dz otherwise
-		const float g = outer ? -2.0f * t2 / r2 : 0.0f;
+		const float g = outer ? -2.0f * t2 * frcp(r2) : 0.0f;
 #pragma unroll
 		for (int c = 0; c < 3; c++)
 		{
@@ -164,6 +168,232 @@ __device__ __forceinline__ void MandelboxIteration(V3 &z, float &DE, V3 *J, cons
 	}
 	z = (zo * k - o) * f.mbScale;
 	DE = DE * k * f.mbAbsScale + 1.0f;
+}
+
+// ------------------------------------------------------------------------------------------
+// FP64 Mandelbox.  Each Mandelbox iteration scales the orbit by |scale| (2 in the benchmark
+// scenes), so FP32 rounding doubles every iteration and after ~20 iterations escape counts no
+// longer follow the reference's FP64 orbit.  gfx950 runs FP64 FMA at the same rate as
+// (unpacked) FP32, so the orbit runs in FP64 and only the final distance returns to FP32.
+// ------------------------------------------------------------------------------------------
+struct DV3
+{
+	double x, y, z;
+};
+__device__ __forceinline__ DV3 dv3(double x, double y, double z) { return DV3{x, y, z}; }
+__device__ __forceinline__ DV3 operator+(DV3 a, DV3 b) { return DV3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ DV3 operator-(DV3 a, DV3 b) { return DV3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ DV3 operator*(DV3 a, double s) { return DV3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ double ddot(DV3 a, DV3 b)
+{
+#pragma clang fp contract(off)
+	return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+__device__ __forceinline__ double dsgn(double v) { return (double)((0.0 < v) - (v < 0.0)); }
+
+template <bool TANGENT>
+__device__ __forceinline__ void MandelboxIterationD(DV3 &z, double &DE, DV3 *J, const FrameConst &f)
+{
+	// no FMA contraction: the orbit expands rounding differences by up to (4 |scale|)^n, so the
+	// escape counts follow the reference only with the reference's separately rounded products
+#pragma clang fp contract(off)
+	const double L = f.mbFoldingLimitD, V = f.mbFoldingValueD;
+	const bool fx = fabs(z.x) > L, fy = fabs(z.y) > L, fz = fabs(z.z) > L;
+	z.x = fx ? dsgn(z.x) * V - z.x : z.x;
+	z.y = fy ? dsgn(z.y) * V - z.y : z.y;
+	z.z = fz ? dsgn(z.z) * V - z.z : z.z;
+	if (TANGENT)
+	{
+#pragma unroll
+		for (int k = 0; k < 3; k++)
+		{
+			J[k].x = fx ? -J[k].x : J[k].x;
+			J[k].y = fy ? -J[k].y : J[k].y;
+			J[k].z = fz ? -J[k].z : J[k].z;
+		}
+	}
+	const double r2 = ddot(z, z);
+	const DV3 o = dv3(f.mbOffsetD[0], f.mbOffsetD[1], f.mbOffsetD[2]);
+	const DV3 zo = z + o;
+	const bool inner = r2 < f.mbMR2D;
+	const bool outer = !inner && r2 < f.mbFR2D;
+	const double t2 = f.mbFR2D / r2;
+	const double k = inner ? f.mbFactor1D : (outer ? t2 : 1.0);
+	if (TANGENT)
+	{
+		const double g = outer ? -2.0 * t2 / r2 : 0.0;
+#pragma unroll
+		for (int c = 0; c < 3; c++)
+		{
+			const double zd = ddot(z, J[c]) * g;
+			J[c] = (J[c] * k + zo * zd) * f.mbScaleD;
+		}
+	}
+	z = (zo * k - o) * f.mbScaleD;
+	DE = DE * k * f.mbAbsScaleD + 1.0;
+}
+
+// aux.r > bailout with aux.r = |z| as the reference compares it (compute_fractal.cpp:299,317):
+// the squared compare decides all but the orbits within rounding of the bailout radius, which
+// are exactly the ones the binary search converges to (the DE jumps where the escape count
+// changes), so those take the square root.
+__device__ __forceinline__ bool escaped_d(double r2, double bailout)
+{
+	const double b2 = bailout * bailout;
+	if (r2 > b2 * (1.0 + 1e-12)) return true;
+	if (r2 < b2 * (1.0 - 1e-12)) return false;
+	return sqrt(r2) > bailout;
+}
+
+__device__ __forceinline__ DV3 add_c_d(DV3 z, DV3 p, const FrameConst &f)
+{
+#pragma clang fp contract(off)
+	if (!f.addC) return z;
+	if (f.julia) return z + dv3(f.juliaCD[0], f.juliaCD[1], f.juliaCD[2]);
+	return z + dv3(p.x * f.constMulD[0], p.y * f.constMulD[1], p.z * f.constMulD[2]);
+}
+
+// Mandelbox orbit in FP64 with the analytic DE (DELTA = false) or the delta DE (DELTA = true).
+// The reference's delta DE differences |z_n| over three axis offsets of
+// delta = max(|p| 1e-14, 1e-5 detailSize) (calculate_distance.cpp:613-672); the device carries
+// the three columns of dz_n/dp along the orbit instead (forward mode) and takes
+// dr_k = |z_n . dz_n/dp_k| / |z_n|, the limit of the reference's quotient, so no cancellation
+// at screen-space detail sizes.
+template <bool DELTA>
+__device__ __forceinline__ double compute_box_d(DV3 p, const FrameConst &f, double detailSize, int &iters)
+{
+	DV3 z = p;
+	DV3 J[3] = {dv3(1.0, 0.0, 0.0), dv3(0.0, 1.0, 0.0), dv3(0.0, 0.0, 1.0)};
+	double DE = 1.0;
+	double r2 = ddot(z, z);
+	const int maxN = f.formulaMaxiter;
+	const bool addJ = f.addC && !f.julia;
+	int i;
+	bool escaped = false;
+	for (i = 0; i < maxN; i++)
+	{
+		const DV3 lastZ = z;
+		MandelboxIterationD<DELTA>(z, DE, J, f);
+		z = add_c_d(z, p, f);
+		if (DELTA && addJ)
+		{
+			J[0].x += f.constMulD[0];
+			J[1].y += f.constMulD[1];
+			J[2].z += f.constMulD[2];
+		}
+		r2 = ddot(z, z);
+		if (!(r2 <= 1.0e300))
+		{
+			if (!(isfinite(z.x) && isfinite(z.y) && isfinite(z.z)))
+			{
+				z = lastZ;
+				r2 = ddot(z, z);
+			}
+			else
+				escaped = true;
+			break;
+		}
+		if (escaped_d(r2, f.bailoutD))
+		{
+			escaped = true;
+			break;
+		}
+	}
+	iters = i + 1;
+	const double r = sqrt(r2);
+	double d;
+	if (!DELTA)
+	{
+		if (DE > 0.0)
+		{
+			if (f.analyticFunction == MBX_ANALYTIC_LOGARITHMIC)
+				d = (r > 1.0) ? 0.5 * r * log(r) / DE : 0.0;
+			else if (f.analyticFunction == MBX_ANALYTIC_IFS)
+				d = (r - 2.0) / DE;
+			else
+				d = r / DE;
+		}
+		else
+			d = r;
+		return d;
+	}
+	double dr;
+	if (escaped)
+	{
+		const double d1 = fabs(ddot(z, J[0])) / r;
+		const double d2 = fabs(ddot(z, J[1])) / r;
+		const double d3 = fabs(ddot(z, J[2])) / r;
+		dr = sqrt(d1 * d1 + d2 * d2 + d3 * d3);
+	}
+	else
+	{
+		// the reference's offset orbits run one iteration past a non-escaping orbit
+		// (forcedMaxiter = maxN + 1): the quotient is ||z_{N+1}| - |z_N|| / delta per axis
+		DV3 z2 = z;
+		double DE2 = DE;
+		MandelboxIterationD<false>(z2, DE2, nullptr, f);
+		z2 = add_c_d(z2, p, f);
+		const double delta = fmax(sqrt(ddot(p, p)) * 1e-14, 1e-5 * detailSize);
+		dr = 1.7320508075688772 * fabs(sqrt(ddot(z2, z2)) - r) / delta;
+	}
+	if (dr > 0.0)
+		d = (f.deFunction == MBX_DE_FUNCTION_LOGARITHMIC) ? 0.5 * r * log(r) / dr : 0.5 * r / dr;
+	else
+		d = r;
+	return d;
+}
+
+// Compute<calcModeDeltaDE1> of the Mandelbox in FP64 (compute_fractal.cpp:67-654): the final z
+// of the orbit of p, with bailout, at most maxN formula calls.  Returns the iteration count.
+__device__ __forceinline__ int box_orbit_d(DV3 p, const FrameConst &f, int maxN, DV3 &zOut)
+{
+#pragma clang fp contract(off)
+	DV3 z = p;
+	double DE = 1.0;
+	int i;
+	for (i = 0; i < maxN; i++)
+	{
+		const DV3 lastZ = z;
+		MandelboxIterationD<false>(z, DE, nullptr, f);
+		z = add_c_d(z, p, f);
+		const double r2 = ddot(z, z);
+		if (!(r2 <= 1.0e300))
+		{
+			if (!(isfinite(z.x) && isfinite(z.y) && isfinite(z.z))) z = lastZ;
+			break;
+		}
+		if (escaped_d(r2, f.bailoutD)) break;
+	}
+	zOut = z;
+	return i + 1;
+}
+
+// The reference's delta DE of the Mandelbox (calculate_distance.cpp:609-711): one orbit at p,
+// then three orbits at p + delta e_k capped at the first orbit's iteration count, dr_k =
+// | |z(p + delta e_k)| - |z(p)| | / delta, dr = |(dr_1, dr_2, dr_3)|, distance 0.5 r / dr (or
+// 0.5 r log r / dr).  FP64 throughout: the quotient's offset is below FP32 resolution.
+__device__ __forceinline__ double compute_box_fd(DV3 p, const FrameConst &f, double detailSize, int &iters,
+	int &itersLast, uint32_t &work)
+{
+#pragma clang fp contract(off)
+	const double deltaDE = fmax(sqrt(ddot(p, p)) * 1e-14, 1e-5 * detailSize);
+	DV3 z;
+	iters = box_orbit_d(p, f, f.formulaMaxiter, z);
+	const double r = sqrt(ddot(z, z));
+	const int forced = iters;
+	DV3 z1, z2, z3;
+	const int i1 = box_orbit_d(dv3(p.x + deltaDE, p.y, p.z), f, forced, z1);
+	const int i2 = box_orbit_d(dv3(p.x, p.y + deltaDE, p.z), f, forced, z2);
+	const int i3 = box_orbit_d(dv3(p.x, p.y, p.z + deltaDE), f, forced, z3);
+	itersLast = i3;
+	work += (uint32_t)(iters + i1 + i2 + i3);
+	const double dr1 = fabs(sqrt(ddot(z1, z1)) - r) / deltaDE;
+	const double dr2 = fabs(sqrt(ddot(z2, z2)) - r) / deltaDE;
+	const double dr3 = fabs(sqrt(ddot(z3, z3)) - r) / deltaDE;
+	const double dr = sqrt(dr1 * dr1 + dr2 * dr2 + dr3 * dr3);
+	if (dr > 0.0)
+		return (f.deFunction == MBX_DE_FUNCTION_LOGARITHMIC) ? 0.5 * r * log(r) / dr : 0.5 * r / dr;
+	return r;
 }
 
 // Kaleidoscopic IFS (fractal_kaleidoscopic_ifs.cpp:30-74)
@@ -243,257 +473,166 @@ __device__ __forceinline__ float compute_analytic(V3 p, const FrameConst &f, int
 		if (KIND == kBulb8)
 			MandelbulbPow8Iteration(z, r2, DE);
 		else if (KIND == kBulbTrig)
-			MandelbulbIteration(z, sqrtf(r2), DE, f);
+			MandelbulbIteration(z, fsqrt(r2), DE, f);
 		else if (KIND == kBoxAnalytic)
 			MandelboxIteration<false>(z, DE, nullptr, f);
 		else
 			KaleidoscopicIfsIteration(z, DE, f);
 		z = add_c(z, p, f);
 		r2 = dot(z, z);
-		if (!finite3(z))
+		// one compare catches a NaN or infinite component (r2 is then NaN or +inf); the rare
+		// branch tells an overflowing finite orbit (an escape) from a non-finite one (revert to
+		// the last z, compute_fractal.cpp:303-309)
+		if (!(r2 <= 3.0e38f))
 		{
-			z = lastZ;
-			r2 = dot(z, z);
+			if (!finite3(z))
+			{
+				z = lastZ;
+				r2 = dot(z, z);
+			}
 			break;
 		}
 		if (r2 > bailout2) break;
 	}
 	iters = i + 1;
-	const float r = sqrtf(r2);
+	const float r = fsqrt(r2);
 	float distance;
 	if (DE > 0.0f)
 	{
+		const float invDE = frcp(DE);
 		if (f.analyticFunction == MBX_ANALYTIC_LOGARITHMIC)
-			distance = (r > 1.0f) ? 0.5f * r * __logf(r) / DE : 0.0f;
+			distance = (r > 1.0f) ? 0.5f * r * __logf(r) * invDE : 0.0f;
 		else if (f.analyticFunction == MBX_ANALYTIC_IFS)
-			distance = (r - 2.0f) / DE;
+			distance = (r - 2.0f) * invDE;
 		else
-			distance = r / DE;
+			distance = r * invDE;
 	}
 	else
 		distance = r;
 	return distance;
 }
 
-// Delta-DE orbit of the Mandelbox.  The reference differences |z_n| over three axis offsets
-// of delta = max(|p| 1e-14, 1e-5 detailSize), below FP32 resolution at screen-space detail
-// sizes; the device carries dz_n/dp instead and takes dr_k = |z_n . dz_n/dp_k| / |z_n|, the
-// limit of the reference's quotient.  A non-escaping orbit reproduces the reference's
-// quirk: its offset orbits run one iteration further (forcedMaxiter = maxN + 1), so the quotient
-// is ||z_{N+1}| - |z_N|| / delta on every axis.
-__device__ __forceinline__ float compute_delta_box(V3 p, const FrameConst &f, float detailSize,
-	int &iters)
+// Position precision of a DE kind.  Mandelbox kinds march in FP64 (ray position, scan, normal
+// taps, secondary-ray points): the reference's delta DE with its tiny offset is chaotic at FP32
+// position resolution (rounding a hit point to FP32 changes the iteration count of ~0.7% of
+// Mandelbox hits in the reference's own FP64 arithmetic), and the deep-zoom scenes need
+// sub-1e-7 positions.  FP64 add/mul/FMA run at the unpacked FP32 rate on gfx950.
+template <int KIND>
+struct PosPrecision
 {
-	V3 z = p;
-	V3 J[3] = {v3(1.0f, 0.0f, 0.0f), v3(0.0f, 1.0f, 0.0f), v3(0.0f, 0.0f, 1.0f)};
-	float DE = 1.0f;
-	const int maxN = f.formulaMaxiter;
-	const float bailout2 = f.bailout * f.bailout;
-	const V3 cm = f.julia ? v3(0.0f, 0.0f, 0.0f) : v3(f.constMul[0], f.constMul[1], f.constMul[2]);
-	int i;
-	bool escaped = false;
-	for (i = 0; i < maxN; i++)
+	static constexpr bool kFP64 = (KIND == kBoxAnalytic || KIND == kBoxDelta);
+};
+
+template <bool D>
+struct Prec;
+
+template <>
+struct Prec<false>
+{
+	using R = float;
+	using P = V3;
+	static __device__ __forceinline__ P cam(const FrameConst &f) { return v3(f.camera[0], f.camera[1], f.camera[2]); }
+	static __device__ __forceinline__ R threshK(const FrameConst &f) { return f.threshK; }
+	static __device__ __forceinline__ R deltaK(const FrameConst &f) { return f.deltaK; }
+	static __device__ __forceinline__ R DEThresh(const FrameConst &f) { return f.DEThresh; }
+	static __device__ __forceinline__ R len(P a) { return fsqrt(dot(a, a)); }
+	static __device__ __forceinline__ R rcp(R x) { return frcp(x); }
+	static __device__ __forceinline__ R thresh(const FrameConst &f, P p) { return len(cam(f) - p) * f.threshK; }
+	static __device__ __forceinline__ P along(P start, P dir, R scan) { return start + dir * scan; }
+	static __device__ __forceinline__ P from(V3 a) { return a; }
+	static __device__ __forceinline__ V3 tof(P a) { return a; }
+	static __device__ __forceinline__ R invWidth(const FrameConst &f) { return f.invWidth; }
+	static __device__ __forceinline__ R invHeight(const FrameConst &f) { return f.invHeight; }
+	static __device__ __forceinline__ R aspect(const FrameConst &f) { return f.aspectRatio; }
+	// CalculateViewVector three-point (projection_3d.cpp:77-89), normalised again by doWork
+	static __device__ __forceinline__ P view(const FrameConst &f, R ix, R iy)
 	{
-		const V3 lastZ = z;
-		MandelboxIteration<true>(z, DE, J, f);
-		z = add_c(z, p, f);
-		if (f.addC)
-		{
-			J[0].x += cm.x;
-			J[1].y += cm.y;
-			J[2].z += cm.z;
-		}
-		if (!finite3(z))
-		{
-			z = lastZ;
-			break;
-		}
-		if (dot(z, z) > bailout2)
-		{
-			escaped = true;
-			break;
-		}
+		const V3 vv = normalize(v3(ix * f.fov, 1.0f, iy * f.fov));
+		return normalize(rot(f.mRot, vv));
 	}
-	iters = i + 1;
-	const float r = length(z);
-	float dr;
-	if (escaped)
+};
+
+template <>
+struct Prec<true>
+{
+	using R = double;
+	using P = DV3;
+	static __device__ __forceinline__ P cam(const FrameConst &f) { return dv3(f.cameraD[0], f.cameraD[1], f.cameraD[2]); }
+	static __device__ __forceinline__ R threshK(const FrameConst &f) { return f.threshKD; }
+	static __device__ __forceinline__ R deltaK(const FrameConst &f) { return f.deltaKD; }
+	static __device__ __forceinline__ R DEThresh(const FrameConst &f) { return f.DEThreshD; }
+	static __device__ __forceinline__ R len(P a) { return sqrt(ddot(a, a)); }
+	static __device__ __forceinline__ R rcp(R x) { return 1.0 / x; }
+	// CalcDistThresh in the reference's operation order: |camera - p| * resolution * fov / detail
+	static __device__ __forceinline__ R thresh(const FrameConst &f, P p)
 	{
-		const float inv_r = 1.0f / r;
-		const float d1 = fabsf(dot(z, J[0])) * inv_r;
-		const float d2 = fabsf(dot(z, J[1])) * inv_r;
-		const float d3 = fabsf(dot(z, J[2])) * inv_r;
-		dr = sqrtf(d1 * d1 + d2 * d2 + d3 * d3);
+#pragma clang fp contract(off)
+		return len(cam(f) - p) * f.resolutionD * f.fovD / f.detailLevelD;
 	}
-	else
+	// start + dir * scan without FMA contraction (CVector3 operator* then operator+)
+	static __device__ __forceinline__ P along(P start, P dir, R scan)
 	{
-		V3 z2 = z;
-		float DE2 = DE;
-		MandelboxIteration<false>(z2, DE2, nullptr, f);
-		z2 = add_c(z2, p, f);
-		const float delta = fmaxf(length(p) * 1e-14f, 1e-5f * detailSize);
-		dr = 1.7320508f * fabsf(length(z2) - r) / delta;
+#pragma clang fp contract(off)
+		return dv3(start.x + dir.x * scan, start.y + dir.y * scan, start.z + dir.z * scan);
 	}
-	float distance;
-	if (dr > 0.0f)
-		distance = (f.deFunction == MBX_DE_FUNCTION_LOGARITHMIC) ? 0.5f * r * __logf(r) / dr
-		                                                        : 0.5f * r / dr;
-	else
-		distance = r;
-	return distance;
-}
+	static __device__ __forceinline__ P from(V3 a) { return dv3(a.x, a.y, a.z); }
+	static __device__ __forceinline__ V3 tof(P a) { return v3((float)a.x, (float)a.y, (float)a.z); }
+	static __device__ __forceinline__ R invWidth(const FrameConst &f) { return f.invWidthD; }
+	static __device__ __forceinline__ R invHeight(const FrameConst &f) { return f.invHeightD; }
+	static __device__ __forceinline__ R aspect(const FrameConst &f) { return f.aspectRatioD; }
+	static __device__ __forceinline__ P view(const FrameConst &f, R ix, R iy)
+	{
+#pragma clang fp contract(off)
+		DV3 vv = dv3(ix * f.fovD, 1.0, iy * f.fovD);
+		vv = vv * (1.0 / sqrt(ddot(vv, vv)));
+		const double *m = f.mRotD;
+		DV3 d = dv3(m[0] * vv.x + m[1] * vv.y + m[2] * vv.z, m[3] * vv.x + m[4] * vv.y + m[5] * vv.z,
+			m[6] * vv.x + m[7] * vv.y + m[8] * vv.z);
+		return d * (1.0 / sqrt(ddot(d, d)));
+	}
+};
 
 // CalculateDistance for the single fractal node (calculate_distance.cpp:225-516 reduces to
 // CalculateDistanceSimple for one node with identity transform and scale 1).  iterThresh and
 // interior modes are off on the device path.
 template <int KIND>
-__device__ __forceinline__ float calculate_distance(V3 p, float detailSize, const FrameConst &f,
-	int &iters, Work &w)
+__device__ __forceinline__ typename Prec<PosPrecision<KIND>::kFP64>::R calculate_distance(
+	typename Prec<PosPrecision<KIND>::kFP64>::P p, typename Prec<PosPrecision<KIND>::kFP64>::R detailSize,
+	const FrameConst &f, int &iters, Work &w)
 {
-	float d;
-	if (KIND == kBoxDelta)
+	using R = typename Prec<PosPrecision<KIND>::kFP64>::R;
+	R d;
+	int itersMinN = 0;
+	if constexpr (KIND == kBoxDelta)
 	{
-		d = compute_delta_box(p, f, detailSize, iters);
-		w.iterations += 4u * (uint32_t)iters;
+		if (f.deltaTangent)
+		{
+			d = compute_box_d<true>(p, f, detailSize, iters);
+			w.iterations += 4u * (uint32_t)iters;
+			itersMinN = iters;
+		}
+		else
+			d = compute_box_fd(p, f, detailSize, iters, itersMinN, w.iterations);
+	}
+	else if constexpr (KIND == kBoxAnalytic)
+	{
+		d = compute_box_d<false>(p, f, detailSize, iters);
+		w.iterations += (uint32_t)iters;
+		itersMinN = iters;
 	}
 	else
 	{
 		d = compute_analytic<KIND>(p, f, iters);
 		w.iterations += (uint32_t)iters;
+		itersMinN = iters;
 	}
 	w.de_evals++;
-	if (iters < f.minN && d < detailSize) d = detailSize;
+	if (itersMinN < f.minN && d < detailSize) d = detailSize;
 	// clamp as the reference does; a NaN passes the clamp and the boolean-add merge of the
 	// objects tree then keeps its initial 1e20
-	d = (d < 0.0f) ? 0.0f : d;
-	d = (d > 10.0f) ? 10.0f : d;
-	if (!(d < 1e20f)) d = 1e20f;
-	return d;
-}
-
-} // namespace mbx
-
-namespace mbx
-{
-
-// ------------------------------------------------------------------------------------------
-// Resumable DE evaluation for the persistent kernels: the Compute loop split into begin /
-// one-iteration step / finish, so that every lane of a wave advances its own evaluation by
-// one formula iteration per loop trip.
-// ------------------------------------------------------------------------------------------
-template <int KIND>
-struct DEState
-{
-	V3 p;       // evaluation point
-	V3 z;
-	float r2;   // |z|^2 (aux.r squared)
-	float DE;   // running derivative
-	int it;     // formula iterations done
-	int status; // 0 running, 1 escaped, 2 iteration limit, 3 non-finite orbit (reverted)
-	V3 J[KIND == kBoxDelta ? 3 : 1];
-};
-
-template <int KIND>
-__device__ __forceinline__ void de_begin(DEState<KIND> &s, V3 p)
-{
-	s.p = p;
-	s.z = p;
-	s.r2 = dot(p, p);
-	s.DE = 1.0f;
-	s.it = 0;
-	s.status = 0;
-	if (KIND == kBoxDelta)
-	{
-		s.J[0] = v3(1.0f, 0.0f, 0.0f);
-		s.J[1] = v3(0.0f, 1.0f, 0.0f);
-		s.J[2] = v3(0.0f, 0.0f, 1.0f);
-	}
-}
-
-template <int KIND>
-__device__ __forceinline__ void de_step(DEState<KIND> &s, const FrameConst &f)
-{
-	const V3 lastZ = s.z;
-	if (KIND == kBulb8)
-		MandelbulbPow8Iteration(s.z, s.r2, s.DE);
-	else if (KIND == kBulbTrig)
-		MandelbulbIteration(s.z, sqrtf(s.r2), s.DE, f);
-	else if (KIND == kBoxAnalytic)
-		MandelboxIteration<false>(s.z, s.DE, nullptr, f);
-	else if (KIND == kBoxDelta)
-		MandelboxIteration<true>(s.z, s.DE, s.J, f);
-	else
-		KaleidoscopicIfsIteration(s.z, s.DE, f);
-	s.z = add_c(s.z, s.p, f);
-	if (KIND == kBoxDelta && f.addC && !f.julia)
-	{
-		s.J[0].x += f.constMul[0];
-		s.J[1].y += f.constMul[1];
-		s.J[2].z += f.constMul[2];
-	}
-	s.r2 = dot(s.z, s.z);
-	s.it++;
-	if (!finite3(s.z))
-	{
-		s.z = lastZ;
-		s.r2 = dot(lastZ, lastZ);
-		s.status = 3;
-	}
-	else if (s.r2 > f.bailout * f.bailout)
-		s.status = 1;
-	else if (s.it >= f.formulaMaxiter)
-		s.status = 2;
-}
-
-// Distance of a finished evaluation after CalculateDistanceSimple's clamps; `iters` gets the
-// reference's iteration count (i + 1, maxN + 1 for a non-escaping orbit).
-template <int KIND>
-__device__ __forceinline__ float de_finish(DEState<KIND> &s, const FrameConst &f, float detailSize,
-	int &iters)
-{
-	iters = s.status == 2 ? s.it + 1 : s.it;
-	const float r = sqrtf(s.r2);
-	float d;
-	if (KIND == kBoxDelta)
-	{
-		float dr;
-		if (s.status == 1)
-		{
-			const float inv_r = 1.0f / r;
-			const float d1 = fabsf(dot(s.z, s.J[0])) * inv_r;
-			const float d2 = fabsf(dot(s.z, s.J[1])) * inv_r;
-			const float d3 = fabsf(dot(s.z, s.J[2])) * inv_r;
-			dr = sqrtf(d1 * d1 + d2 * d2 + d3 * d3);
-		}
-		else
-		{
-			V3 z2 = s.z;
-			float DE2 = s.DE;
-			MandelboxIteration<false>(z2, DE2, nullptr, f);
-			z2 = add_c(z2, s.p, f);
-			const float delta = fmaxf(length(s.p) * 1e-14f, 1e-5f * detailSize);
-			dr = 1.7320508f * fabsf(length(z2) - r) / delta;
-		}
-		if (dr > 0.0f)
-			d = (f.deFunction == MBX_DE_FUNCTION_LOGARITHMIC) ? 0.5f * r * __logf(r) / dr : 0.5f * r / dr;
-		else
-			d = r;
-	}
-	else if (s.DE > 0.0f)
-	{
-		if (f.analyticFunction == MBX_ANALYTIC_LOGARITHMIC)
-			d = (r > 1.0f) ? 0.5f * r * __logf(r) / s.DE : 0.0f;
-		else if (f.analyticFunction == MBX_ANALYTIC_IFS)
-			d = (r - 2.0f) / s.DE;
-		else
-			d = r / s.DE;
-	}
-	else
-		d = r;
-	if (iters < f.minN && d < detailSize) d = detailSize;
-	d = (d < 0.0f) ? 0.0f : d;
-	d = (d > 10.0f) ? 10.0f : d;
-	if (!(d < 1e20f)) d = 1e20f;
+	d = (d < (R)0) ? (R)0 : d;
+	d = (d > (R)10) ? (R)10 : d;
+	if (!(d < (R)1e20)) d = (R)1e20;
 	return d;
 }
 

@@ -20,7 +20,7 @@ enum DEKind : int32_t
 	kBulb8 = 0,      // Mandelbulb power 8, analytic log DE, trig-free power (w = 0)
 	kBulbTrig = 1,   // Mandelbulb any power, analytic log DE, reference trig form
 	kBoxAnalytic = 2,
-	kBoxDelta = 3,   // Mandelbox, delta DE by forward-mode tangents
+	kBoxDelta = 3,   // Mandelbox, delta DE (reference finite differences, or tangents)
 	kIfsAnalytic = 4,
 	kNumDEKinds = 5
 };
@@ -40,6 +40,7 @@ struct FrameConst
 	float threshK;             // distThresh = |camera - p| * threshK   (CalcDistThresh)
 	float deltaK;              // delta      = |camera - p| * deltaK    (CalcDelta)
 	double threshKD, deltaKD;
+	double resolutionD, detailLevelD, DEFactorD;
 	int32_t constantDEThreshold;
 	float DEThresh;
 	double DEThreshD;
@@ -66,6 +67,12 @@ struct FrameConst
 	// Mandelbox (after RecalculateFractalParams, fractal.cpp:673-675)
 	float mbScale, mbAbsScale, mbFoldingLimit, mbFoldingValue, mbFR2, mbMR2, mbFactor1;
 	float mbOffset[3];
+	// the Mandelbox orbit runs in FP64 (full-rate FMA on gfx950): its constants in FP64
+	double mbScaleD, mbAbsScaleD, mbFoldingLimitD, mbFoldingValueD, mbFR2D, mbMR2D, mbFactor1D;
+	double mbOffsetD[3];
+	double juliaCD[3], constMulD[3];
+	double bailoutD;
+	int32_t deltaTangent; // delta DE by forward-mode tangents instead of the reference's differences
 	int32_t mbRotationsEnabled, mbMainRotationEnabled;
 	float mbMainRot[9];
 	float mbRot[2][3][9];

@@ -6,6 +6,7 @@
 // constants), then hand them to the device once per frame.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -37,6 +38,7 @@ struct mbx_ctx
 	int kernelMode = 0; // 0 persistent, 1 plain
 	int waves = 0;      // 0 = default
 	int useGraph = 1;
+	int deltaTangent = 0;
 	int numCUs = 256;
 	FrameConst *d_frames = nullptr;
 	mbx::FrameJob *d_jobs = nullptr;
@@ -44,9 +46,8 @@ struct mbx_ctx
 	unsigned int *d_queue = nullptr;
 	std::vector<mbx::FrameJob> jobsHost; // what d_jobs holds
 	std::vector<FrameConst> framesHost;  // what d_frames holds
-	int batchKind = -1, batchFeat = 0, batchFrames = 0;
 	hipGraphExec_t graphExec = nullptr;
-	int graphKind = -1, graphFeat = -1, graphFrames = -1, graphWaves = -1;
+	int graphKind = -1, graphFrames = -1, graphWaves = -1;
 };
 
 namespace
@@ -81,10 +82,13 @@ D3 unit(D3 a)
 	return mul(a, k);
 }
 // Rodrigues rotation of v about a unit axis (CVector3::RotateAroundVectorByAngle, algebra.cpp:71-77)
+// (same operation order as the reference: ((v cos) + (axis x v) sin) + (axis (axis.v)) (1 - cos))
 D3 rotate_about(D3 v, D3 axis, double angle)
 {
-	const double c = std::cos(angle), s = std::sin(angle);
-	return add(add(mul(v, c), mul(cross(axis, v), s)), mul(axis, dot(axis, v) * (1.0 - c)));
+	D3 r = mul(v, std::cos(angle));
+	r = add(r, mul(cross(axis, v), std::sin(angle)));
+	r = add(r, mul(mul(axis, dot(axis, v)), 1 - std::cos(angle)));
+	return r;
 }
 
 // 3x3 rotation composed by right-multiplication as CRotationMatrix::RotateX/Y/Z do
@@ -227,6 +231,9 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 	fc.invWidthD = 1.0 / p->imageWidth;
 	fc.invHeightD = 1.0 / p->imageHeight;
 	fc.threshKD = resolution * fov / p->detailLevel;
+	fc.resolutionD = resolution;
+	fc.detailLevelD = p->detailLevel;
+	fc.DEFactorD = p->DEFactor;
 	fc.deltaKD = resolution * fov;
 	fc.threshK = (float)fc.threshKD;
 	fc.deltaK = (float)fc.deltaKD;
@@ -287,6 +294,7 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 	}
 	fc.initialW = 0.0f;
 	fc.bailout = (float)(fr.useDefaultBailout ? defaultBailout : fr.bailout);
+	fc.bailoutD = fr.useDefaultBailout ? defaultBailout : fr.bailout;
 	fc.formulaMaxiter = fr.formulaMaxiter;
 	fc.deFunction = deFunction;
 	fc.analyticFunction = analyticFunction;
@@ -305,6 +313,20 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 	fc.mbMR2 = (float)mR2;
 	fc.mbFactor1 = (float)(fR2 / mR2);
 	set3f(fc.mbOffset, mb.offset);
+	fc.deltaTangent = ctx->deltaTangent;
+	fc.mbScaleD = mb.scale;
+	fc.mbAbsScaleD = std::fabs(mb.scale);
+	fc.mbFoldingLimitD = mb.foldingLimit;
+	fc.mbFoldingValueD = mb.foldingValue;
+	fc.mbFR2D = fR2;
+	fc.mbMR2D = mR2;
+	fc.mbFactor1D = fR2 / mR2;
+	for (int i = 0; i < 3; i++)
+	{
+		fc.mbOffsetD[i] = mb.offset[i];
+		fc.juliaCD[i] = fr.juliaC[i] * fr.constantFactor[i];
+		fc.constMulD[i] = fr.constantFactor[i];
+	}
 
 	const mbx_fractal_ifs &ifs = fr.IFS;
 	fc.ifsScale = (float)ifs.scale;
@@ -451,15 +473,6 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 	return MBX_OK;
 }
 
-int feature_mask(const FrameConst &fc)
-{
-	int feat = 0;
-	if (fc.castShadows && fc.lightEnabled) feat |= mbx::kFeatShadow;
-	if (fc.aoEnabled) feat |= mbx::kFeatAO;
-	if (fc.monteCarlo) feat |= mbx::kFeatMC;
-	return feat;
-}
-
 int default_waves(const mbx_ctx *ctx) { return ctx->numCUs * 16; }
 
 int ensure_capacity(mbx_ctx *ctx, int n)
@@ -482,10 +495,11 @@ int ensure_capacity(mbx_ctx *ctx, int n)
 	return MBX_OK;
 }
 
-mbx::FrameJob make_job(const FrameConst &fc, const mbx_outputs &o, int tileBase)
+mbx::FrameJob make_job(const FrameConst &fc, const mbx_outputs &o, int tileBase, int pixBase)
 {
 	mbx::FrameJob j{};
 	j.tileEnd = tileBase + ((fc.width + 7) / 8) * ((fc.height + 7) / 8);
+	j.pixBase = pixBase;
 	j.rgb = o.rgb;
 	j.depth = o.depth;
 	j.iters = o.iters;
@@ -531,22 +545,25 @@ int stage_batch(mbx_ctx *ctx, const std::vector<FrameConst> &frames, const std::
 	return MBX_OK;
 }
 
-// Enqueue the clear + persistent launch, from a captured graph when enabled.
-int enqueue_persistent(mbx_ctx *ctx, int kind, int feat, int nFrames)
+// Enqueue the batch launch (queue clear + persistent kernel), from a captured graph when enabled.
+int enqueue_persistent(mbx_ctx *ctx, int kind, int nFrames)
 {
 	const int waves = ctx->waves > 0 ? ctx->waves : default_waves(ctx);
+	mbx::BatchArgs a{};
+	a.frames = ctx->d_frames;
+	a.jobs = ctx->d_jobs;
+	a.numFrames = nFrames;
+	a.queue = ctx->d_queue;
+	a.counters = ctx->d_counters;
 	hipError_t e;
 	if (!ctx->useGraph)
 	{
-		if ((e = mbx::launch_clear(ctx->d_queue, ctx->d_counters, ctx->stream)) != hipSuccess)
-			return hip_fail(ctx, e, "clear launch");
-		if ((e = mbx::launch_persistent(kind, feat, ctx->d_frames, ctx->d_jobs, nFrames, ctx->d_queue,
-				 ctx->d_counters, waves, ctx->stream)) != hipSuccess)
-			return hip_fail(ctx, e, "persistent launch");
+		if ((e = hipEventRecord(ctx->ev0, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
+		if ((e = mbx::launch_batch(kind, a, waves, ctx->stream)) != hipSuccess)
+			return hip_fail(ctx, e, "batch launch");
 		return MBX_OK;
 	}
-	if (!ctx->graphExec || ctx->graphKind != kind || ctx->graphFeat != feat || ctx->graphFrames != nFrames ||
-		ctx->graphWaves != waves)
+	if (!ctx->graphExec || ctx->graphKind != kind || ctx->graphFrames != nFrames || ctx->graphWaves != waves)
 	{
 		if (ctx->graphExec)
 		{
@@ -561,12 +578,9 @@ int enqueue_persistent(mbx_ctx *ctx, int kind, int feat, int nFrames)
 			rc = hip_fail(ctx, e, "begin capture");
 		if (rc == MBX_OK)
 		{
-			hipError_t e1 = mbx::launch_clear(ctx->d_queue, ctx->d_counters, cs);
-			hipError_t e2 = mbx::launch_persistent(kind, feat, ctx->d_frames, ctx->d_jobs, nFrames, ctx->d_queue,
-				ctx->d_counters, waves, cs);
+			hipError_t e1 = mbx::launch_batch(kind, a, waves, cs);
 			e = hipStreamEndCapture(cs, &g);
-			if (e1 != hipSuccess) rc = hip_fail(ctx, e1, "capture clear");
-			else if (e2 != hipSuccess) rc = hip_fail(ctx, e2, "capture persistent");
+			if (e1 != hipSuccess) rc = hip_fail(ctx, e1, "capture batch");
 			else if (e != hipSuccess) rc = hip_fail(ctx, e, "end capture");
 		}
 		if (rc == MBX_OK && (e = hipGraphInstantiate(&ctx->graphExec, g, nullptr, nullptr, 0)) != hipSuccess)
@@ -579,10 +593,11 @@ int enqueue_persistent(mbx_ctx *ctx, int kind, int feat, int nFrames)
 			return rc;
 		}
 		ctx->graphKind = kind;
-		ctx->graphFeat = feat;
 		ctx->graphFrames = nFrames;
 		ctx->graphWaves = waves;
 	}
+	// timing starts after any (re)capture, which is host work
+	if ((e = hipEventRecord(ctx->ev0, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
 	if ((e = hipGraphLaunch(ctx->graphExec, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "graph launch");
 	return MBX_OK;
 }
@@ -591,21 +606,22 @@ int render_frames(mbx_ctx *ctx, const std::vector<FrameConst> &frames, const mbx
 {
 	hipError_t e = hipSetDevice(ctx->device);
 	if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
-	int kind = frames[0].deKind, feat = 0;
+	int kind = frames[0].deKind;
 	uint64_t rays = 0;
 	std::vector<mbx::FrameJob> jobs;
-	int tiles = 0;
+	int64_t tiles = 0;
+	size_t pixels = 0;
 	for (size_t i = 0; i < frames.size(); i++)
 	{
-		feat |= feature_mask(frames[i]);
-		jobs.push_back(make_job(frames[i], outs[i], tiles));
+		jobs.push_back(make_job(frames[i], outs[i], (int)tiles, (int)pixels));
 		tiles = jobs.back().tileEnd;
+		pixels += (size_t)frames[i].width * frames[i].height;
 		rays += (uint64_t)frames[i].width * frames[i].height * (uint64_t)frames[i].samples;
+		if (tiles > (int64_t)1 << 30) return fail(ctx, MBX_ERR_INVALID, "batch too large");
 	}
 	int rc = stage_batch(ctx, frames, jobs);
 	if (rc != MBX_OK) return rc;
-	if ((e = hipEventRecord(ctx->ev0, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
-	rc = enqueue_persistent(ctx, kind, feat, (int)frames.size());
+	rc = enqueue_persistent(ctx, kind, (int)frames.size());
 	if (rc != MBX_OK) return rc;
 	if ((e = hipEventRecord(ctx->ev1, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
 	ctx->launched = true;
@@ -762,6 +778,9 @@ int mbx_tune(mbx_ctx *ctx, int key, int value)
 			return MBX_OK;
 		case MBX_TUNE_GRAPH:
 			ctx->useGraph = value ? 1 : 0;
+			return MBX_OK;
+		case MBX_TUNE_DELTA_DE:
+			ctx->deltaTangent = value ? 1 : 0;
 			return MBX_OK;
 		default: return fail(ctx, MBX_ERR_INVALID, "unknown tuning key");
 	}

@@ -33,19 +33,11 @@ hipError_t launch_distance(int deKind, const FrameConst *frame, const float *pts
 namespace mbx
 {
 
-// Feature set compiled into a persistent-kernel instance.
-enum Feature : int
-{
-	kFeatShadow = 1,
-	kFeatAO = 2,
-	kFeatMC = 4, // Monte Carlo sub-pixel samples and the GI bounce loop
-};
-
 // One frame of a batch: its tile range in the global tile numbering and its output planes.
 struct FrameJob
 {
 	int tileEnd; // exclusive end of this frame's tiles (8x8 pixels each)
-	int pad;
+	int pixBase; // offset of this frame's first pixel in the batch
 	float *rgb;
 	float *depth;
 	int32_t *iters;
@@ -54,11 +46,22 @@ struct FrameJob
 	int32_t *steps;
 };
 
-// Persistent work-stealing launch over all tiles of `numFrames` frames.  queue[0] must be 0
-// and counters zeroed before the launch (launch_clear).
-hipError_t launch_persistent(int deKind, int feat, const FrameConst *frames, const FrameJob *jobs,
-	int numFrames, unsigned int *queue, unsigned long long *counters, int numWaves, hipStream_t stream);
+// Arguments of the persistent kernel (passed by value).
+struct BatchArgs
+{
+	const FrameConst *frames;
+	const FrameJob *jobs;
+	int numFrames;
+	int pad;
+	unsigned int *queue;           // [0] tile queue head
+	unsigned long long *counters;  // [0] DE evaluations, [1] formula iterations
+};
 
+// Enqueue the batch: a clear of the queue head and counters, then the persistent kernel with
+// `numWaves` 64-lane waves.
+hipError_t launch_batch(int deKind, BatchArgs a, int numWaves, hipStream_t stream);
+
+// Zero queue[0] and the work counters.
 hipError_t launch_clear(unsigned int *queue, unsigned long long *counters, hipStream_t stream);
 
 } // namespace mbx

@@ -17,81 +17,108 @@
 namespace mbx
 {
 
+template <int KIND>
+using PrecOf = Prec<PosPrecision<KIND>::kFP64>;
+
+template <int KIND>
 struct MarchOut
 {
-	V3 point;
-	float depth;
-	float distThresh;
-	float lastDist;
+	typename PrecOf<KIND>::P point;
+	typename PrecOf<KIND>::R depth;
+	typename PrecOf<KIND>::R distThresh;
+	typename PrecOf<KIND>::R lastDist;
 	int found;
 	int iters;
 	int steps;
 };
 
-__device__ __forceinline__ float dist_thresh(const FrameConst &f, V3 p)
-{
-	// CalcDistThresh (render_worker.cpp:653-679)
-	if (f.constantDEThreshold) return f.DEThresh;
-	return length(v3(f.camera[0], f.camera[1], f.camera[2]) - p) * f.threshK;
-}
-
-__device__ __forceinline__ float calc_delta(const FrameConst &f, V3 p)
-{
-	// CalcDelta (render_worker.cpp:684-690)
-	return length(v3(f.camera[0], f.camera[1], f.camera[2]) - p) * f.deltaK;
-}
-
-// RayMarching (render_worker.cpp:708-951), exterior mode, binary search on
+// CalcDistThresh (render_worker.cpp:653-679)
 template <int KIND>
-__device__ void ray_marching(const FrameConst &f, V3 start, V3 dir, float maxScan,
-	uint32_t &seed, MarchOut &out, Work &w)
+__device__ __forceinline__ typename PrecOf<KIND>::R dist_thresh(const FrameConst &f, typename PrecOf<KIND>::P p)
 {
-	V3 point = v3(1e30f, 1e30f, 1e30f);
+	using Q = PrecOf<KIND>;
+	if (f.constantDEThreshold) return Q::DEThresh(f);
+	return Q::thresh(f, p);
+}
+
+// CalcDelta (render_worker.cpp:684-690)
+template <int KIND>
+__device__ __forceinline__ typename PrecOf<KIND>::R calc_delta(const FrameConst &f, typename PrecOf<KIND>::P p)
+{
+	using Q = PrecOf<KIND>;
+	return Q::len(Q::cam(f) - p) * Q::deltaK(f);
+}
+
+// RayMarching (render_worker.cpp:708-951), exterior mode, binary search on.  In the FP64
+// position precision the arithmetic follows the reference's operation order without FMA
+// contraction, so that march decisions at the DE's discontinuities come out the same.
+template <int KIND>
+__device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start, typename PrecOf<KIND>::P dir,
+	typename PrecOf<KIND>::R maxScan, uint32_t &seed, MarchOut<KIND> &out, Work &w)
+{
+#pragma clang fp contract(off)
+	using Q = PrecOf<KIND>;
+	using R = typename Q::R;
+	using P = typename Q::P;
+	constexpr bool kExact = PosPrecision<KIND>::kFP64;
+	P point = Q::from(v3(1e30f, 1e30f, 1e30f));
 	bool found = false, dead = false;
-	float scan = 0.0f, dist = 0.0f, step = 0.0f, thr = 0.0f;
+	R scan = 0, dist = 0, step = 0, thr = 0;
 	int counter = 0, iters = 0;
-	const float dirLength = length(dir);
+	const R dirLength = Q::len(dir);
+	const R invDirLength = Q::rcp(dirLength);
+	const R searchLimit = kExact ? (R)(1.0 - 0.001 * f.detailLevelD) : (R)f.searchLimit;
 	for (int i = 0; i < f.maxRaymarchingSteps; i++)
 	{
-		const V3 last = point;
+		const P last = point;
 		counter++;
-		point = start + dir * scan;
-		if ((point.x == last.x && point.y == last.y && point.z == last.z) || !finite3(point))
+		point = Q::along(start, dir, scan);
+		if ((point.x == last.x && point.y == last.y && point.z == last.z) ||
+			!(isfinite(point.x) && isfinite(point.y) && isfinite(point.z)))
 		{
 			point = last;
 			found = true;
 			dead = true;
 			break;
 		}
-		thr = dist_thresh(f, point);
+		thr = dist_thresh<KIND>(f, point);
 		dist = calculate_distance<KIND>(point, thr, f, iters, w);
 		if (dist < thr)
 		{
 			found = true;
 			break;
 		}
-		const float jitter = 1.0f - (float)random_mod(seed, 1000) * 1e-4f;
-		step = (dist - 0.5f * thr) * f.DEFactor * jitter;
-		step = fminf(step, 3.0f);
-		scan += step / dirLength;
+		const int rnd = random_mod(seed, 1000);
+		if (kExact)
+		{
+			step = (dist - (R)0.5 * thr) * (R)f.DEFactorD * ((R)1 - (R)rnd / (R)10000);
+			step = step > (R)3 ? (R)3 : step;
+			scan += step / dirLength;
+		}
+		else
+		{
+			step = (dist - (R)0.5 * thr) * (R)f.DEFactor * ((R)1 - (R)rnd * (R)1e-4);
+			step = step > (R)3 ? (R)3 : step;
+			scan += step * invDirLength;
+		}
 		if (scan > maxScan) break;
 	}
-	point = start + dir * scan;
+	point = Q::along(start, dir, scan);
 	if (found && !dead)
 	{
-		step *= 0.5f;
+		step *= (R)0.5;
 		for (int i = 0; i < 30; i++)
 		{
 			counter++;
-			if (dist < thr && dist > thr * f.searchLimit) break;
+			if (dist < thr && dist > thr * searchLimit) break;
 			if (dist > thr)
 				scan += step;
-			else if (dist < thr * f.searchLimit)
+			else if (dist < thr * searchLimit)
 				scan -= step;
-			point = start + dir * scan;
-			thr = dist_thresh(f, point);
+			point = Q::along(start, dir, scan);
+			thr = dist_thresh<KIND>(f, point);
 			dist = calculate_distance<KIND>(point, thr, f, iters, w);
-			step *= 0.5f;
+			step *= (R)0.5;
 		}
 	}
 	out.point = point;
@@ -105,21 +132,39 @@ __device__ void ray_marching(const FrameConst &f, V3 start, V3 dir, float maxSca
 
 // CalculateNormals, fast mode: central differences of the DE (shader_calculate_normals.cpp:44-78)
 template <int KIND>
-__device__ V3 calculate_normals(const FrameConst &f, V3 p, float thr, Work &w)
+__device__ V3 calculate_normals(const FrameConst &f, typename PrecOf<KIND>::P p, typename PrecOf<KIND>::R thr,
+	Work &w)
 {
-	const float delta = thr * f.smoothness;
+#pragma clang fp contract(off)
+	using Q = PrecOf<KIND>;
+	using R = typename Q::R;
+	using P = typename Q::P;
+	const R delta = thr * (R)f.smoothness;
 	int it;
-	const float sx1 = calculate_distance<KIND>(p + v3(delta, 0.0f, 0.0f), thr, f, it, w);
-	const float sx2 = calculate_distance<KIND>(p - v3(delta, 0.0f, 0.0f), thr, f, it, w);
-	const float sy1 = calculate_distance<KIND>(p + v3(0.0f, delta, 0.0f), thr, f, it, w);
-	const float sy2 = calculate_distance<KIND>(p - v3(0.0f, delta, 0.0f), thr, f, it, w);
-	const float sz1 = calculate_distance<KIND>(p + v3(0.0f, 0.0f, delta), thr, f, it, w);
-	const float sz2 = calculate_distance<KIND>(p - v3(0.0f, 0.0f, delta), thr, f, it, w);
-	V3 n = v3(sx1 - sx2, sy1 - sy2, sz1 - sz2);
-	if (n.x == 0.0f && n.y == 0.0f && n.z == 0.0f)
+	P e = p;
+	e.x = p.x + delta;
+	const R sx1 = calculate_distance<KIND>(e, thr, f, it, w);
+	e.x = p.x - delta;
+	const R sx2 = calculate_distance<KIND>(e, thr, f, it, w);
+	e = p;
+	e.y = p.y + delta;
+	const R sy1 = calculate_distance<KIND>(e, thr, f, it, w);
+	e.y = p.y - delta;
+	const R sy2 = calculate_distance<KIND>(e, thr, f, it, w);
+	e = p;
+	e.z = p.z + delta;
+	const R sz1 = calculate_distance<KIND>(e, thr, f, it, w);
+	e.z = p.z - delta;
+	const R sz2 = calculate_distance<KIND>(e, thr, f, it, w);
+	const R nx = sx1 - sx2, ny = sy1 - sy2, nz = sz1 - sz2;
+	V3 n;
+	if (nx == (R)0 && ny == (R)0 && nz == (R)0)
 		n = v3(1.0f, 0.0f, 0.0f);
 	else
-		n = normalize(n);
+	{
+		const R k = (R)1 / Q::len(Q::from(v3(0.0f, 0.0f, 0.0f)) + P{nx, ny, nz});
+		n = v3((float)(nx * k), (float)(ny * k), (float)(nz * k));
+	}
 	if (!finite3(n)) n = v3(1.0f, 0.0f, 0.0f);
 	return n;
 }
@@ -127,49 +172,61 @@ __device__ V3 calculate_normals(const FrameConst &f, V3 p, float thr, Work &w)
 // AuxShadow for the directional light (shader_aux_shadow.cpp:39-323): hard, or the soft
 // cone variant when soft_shadow_cone > 0
 template <int KIND>
-__device__ float aux_shadow(const FrameConst &f, V3 p, float thr, float distance, V3 lv, Work &w)
+__device__ float aux_shadow(const FrameConst &f, typename PrecOf<KIND>::P p, typename PrecOf<KIND>::R thr,
+	typename PrecOf<KIND>::R distance, V3 lvf, Work &w)
 {
-	float shadowTemp = 1.0f;
-	float maxSoft = 0.0f;
-	float step = thr;
+	using Q = PrecOf<KIND>;
+	using R = typename Q::R;
+	using P = typename Q::P;
+	const P lv = Q::from(lvf);
+	R shadowTemp = 1;
+	R maxSoft = 0;
+	R step = thr;
 	int count = 0;
 	int it;
-	for (float i = thr; i < distance; i += step)
+	for (R i = thr; i < distance; i += step)
 	{
-		const V3 p2 = p + lv * i;
-		const float dist = calculate_distance<KIND>(p2, thr, f, it, w);
+		const P p2 = p + lv * i;
+		const R dist = calculate_distance<KIND>(p2, thr, f, it, w);
 		if (f.softShadows)
 		{
-			float angle = (dist - thr) / i;
-			if (angle < 0.0f || dist < thr) angle = 0.0f;
-			float soft = 1.0f - angle / f.softRange;
+			R angle = (dist - thr) / i;
+			if (angle < (R)0 || dist < thr) angle = 0;
+			R soft = (R)1 - angle / (R)f.softRange;
 			if (f.penetrating) soft *= (distance - i) / distance;
-			soft = fmaxf(soft, 0.0f);
-			maxSoft = fmaxf(maxSoft, soft);
+			soft = soft < (R)0 ? (R)0 : soft;
+			maxSoft = soft > maxSoft ? soft : maxSoft;
 		}
-		shadowTemp = 1.0f;
+		shadowTemp = 1;
 		if (dist < thr)
 		{
 			if (f.penetrating)
-				shadowTemp = fmaxf(shadowTemp - (distance - i) / distance, 0.0f);
+			{
+				shadowTemp = shadowTemp - (distance - i) / distance;
+				shadowTemp = shadowTemp < (R)0 ? (R)0 : shadowTemp;
+			}
 			else
-				shadowTemp = 0.0f;
+				shadowTemp = 0;
 			break;
 		}
-		step = fmaxf(fminf(dist, 1e6f) * f.DEFactor, 1e-15f);
+		step = (dist < (R)1e6 ? dist : (R)1e6) * (R)f.DEFactor;
+		step = step > (R)1e-15 ? step : (R)1e-15;
 		count++;
 		if (count > f.maxRaymarchingSteps) break;
 	}
-	return f.softShadows ? 1.0f - maxSoft : shadowTemp;
+	return f.softShadows ? (float)((R)1 - maxSoft) : (float)shadowTemp;
 }
 
 // AmbientOcclusion, multiple rays (shader_ambient_occlusion.cpp:41-122)
 template <int KIND>
-__device__ V3 ambient_occlusion(const FrameConst &f, V3 p, float thr, float delta, uint32_t &seed,
-	Work &w)
+__device__ V3 ambient_occlusion(const FrameConst &f, typename PrecOf<KIND>::P p, typename PrecOf<KIND>::R thr,
+	typename PrecOf<KIND>::R delta, uint32_t &seed, Work &w)
 {
+	using Q = PrecOf<KIND>;
+	using R = typename Q::R;
+	using P = typename Q::P;
 	V3 ao = v3(0.0f, 0.0f, 0.0f);
-	const float end_dist = delta * (float)f.height; // delta / resolution
+	const R end_dist = delta * (R)f.height; // delta / resolution
 	int start = 0, end = f.aoCount - 1;
 	if (f.monteCarlo)
 	{
@@ -180,39 +237,45 @@ __device__ V3 ambient_occlusion(const FrameConst &f, V3 p, float thr, float delt
 	int it;
 	for (int i = start; i <= end; i++)
 	{
-		const V3 v = v3(f.aoVectors[i][0], f.aoVectors[i][1], f.aoVectors[i][2]);
-		float shadowTemp = 1.0f;
-		float dist;
-		for (float r = thr; r < end_dist; r += dist * 2.0f)
+		const P v = Q::from(v3(f.aoVectors[i][0], f.aoVectors[i][1], f.aoVectors[i][2]));
+		R shadowTemp = 1;
+		R dist;
+		for (R r = thr; r < end_dist; r += dist * (R)2)
 		{
 			dist = calculate_distance<KIND>(p + v * r, thr, f, it, w);
 			if (dist < thr)
 			{
-				shadowTemp = fmaxf(shadowTemp - (end_dist - r) / end_dist, 0.0f);
+				shadowTemp = shadowTemp - (end_dist - r) / end_dist;
+				shadowTemp = shadowTemp < (R)0 ? (R)0 : shadowTemp;
 				break;
 			}
 		}
-		ao = ao + v3(f.aoColors[i][0], f.aoColors[i][1], f.aoColors[i][2]) * shadowTemp;
+		ao = ao + v3(f.aoColors[i][0], f.aoColors[i][1], f.aoColors[i][2]) * (float)shadowTemp;
 	}
-	if (!f.monteCarlo) ao = ao * (1.0f / (float)f.aoCount);
+	if (!f.monteCarlo) ao = ao * frcp((float)f.aoCount);
 	return ao;
 }
 
 // FastAmbientOcclusion (shader_fast_ambient_occlusion.cpp:40-66)
 template <int KIND>
-__device__ float fast_ambient_occlusion(const FrameConst &f, V3 p, V3 n, float thr, Work &w)
+__device__ float fast_ambient_occlusion(const FrameConst &f, typename PrecOf<KIND>::P p, V3 nf,
+	typename PrecOf<KIND>::R thr, Work &w)
 {
+	using Q = PrecOf<KIND>;
+	using R = typename Q::R;
+	using P = typename Q::P;
+	const P n = Q::from(nf);
 	float aoTemp = 0.0f;
-	float lastDist = 1e20f;
+	R lastDist = (R)1e20;
 	const int q2 = f.aoQuality * f.aoQuality;
 	int it;
 	for (int i = 1; i < q2; i++)
 	{
-		const float scan = (float)(i * i) * thr;
-		float dist = calculate_distance<KIND>(p + n * scan, thr, f, it, w);
-		if (dist > lastDist * 2.0f) dist = lastDist * 2.0f;
+		const R scan = (R)(i * i) * thr;
+		R dist = calculate_distance<KIND>(p + n * scan, thr, f, it, w);
+		if (dist > lastDist * (R)2) dist = lastDist * (R)2;
 		lastDist = dist;
-		aoTemp += exp2f(-(float)i) * (scan - f.aoFastTune * dist) / thr;
+		aoTemp += (float)((R)exp2f(-(float)i) * (scan - (R)f.aoFastTune * dist) / thr);
 	}
 	return fmaxf(1.0f - 0.2f * aoTemp, 0.0f);
 }
@@ -249,10 +312,10 @@ __device__ V3 background_shader(const FrameConst &f, V3 viewVector)
 	if (f.lightEnabled)
 	{
 		const V3 ld = v3(f.lightDir[0], f.lightDir[1], f.lightDir[2]);
-		const float a = -(dot(vn, ld) - 1.0f) * 360.0f / f.lightSize;
+		const float a = -(dot(vn, ld) - 1.0f) * 360.0f * frcp(f.lightSize);
 		const float pw = 6.0f * f.lightContourSharpness;
 		const float ap = (pw == 6.0f) ? (a * a) * (a * a) * (a * a) : powf(a, pw);
-		const float intensity = 1.0f / (1.0f + ap) * f.lightVisibility * f.lightIntensity;
+		const float intensity = frcp(1.0f + ap) * f.lightVisibility * f.lightIntensity;
 		out = out + v3(f.lightColor[0], f.lightColor[1], f.lightColor[2]) * intensity;
 	}
 	return out;
@@ -261,9 +324,10 @@ __device__ V3 background_shader(const FrameConst &f, V3 viewVector)
 // ObjectShader with one directional light, Lambert, optional shadow and AO
 // (shader_object.cpp:40-229, shader_light_shading.cpp:40-122)
 template <int KIND>
-__device__ V3 object_shader(const FrameConst &f, V3 p, V3 n, float thr, float delta,
-	uint32_t &seed, Work &w)
+__device__ V3 object_shader(const FrameConst &f, typename PrecOf<KIND>::P p, V3 n, typename PrecOf<KIND>::R thr,
+	typename PrecOf<KIND>::R delta, uint32_t &seed, Work &w)
 {
+	using R = typename PrecOf<KIND>::R;
 	const V3 colour = v3(f.surfaceColor[0], f.surfaceColor[1], f.surfaceColor[2]);
 	V3 ambient = v3(0.0f, 0.0f, 0.0f);
 	if (f.aoEnabled)
@@ -288,7 +352,7 @@ __device__ V3 object_shader(const FrameConst &f, V3 p, V3 n, float thr, float de
 		{
 			if (shade > 0.001f)
 			{
-				const float distance = f.penetrating ? delta * (float)f.height : f.viewDistanceMaxLight;
+				const R distance = f.penetrating ? delta * (R)f.height : (R)f.viewDistanceMaxLight;
 				shadow = aux_shadow<KIND>(f, p, thr, distance, lv, w);
 			}
 			else
@@ -305,12 +369,16 @@ __device__ V3 object_shader(const FrameConst &f, V3 p, V3 n, float thr, float de
 
 // GlobalIlumination (shader_global_illumination.cpp:41-209)
 template <int KIND>
-__device__ V3 global_illumination(const FrameConst &f, V3 p0, V3 n0, float thr0, uint32_t &seed,
-	Work &w)
+__device__ V3 global_illumination(const FrameConst &f, typename PrecOf<KIND>::P p0, V3 n0,
+	typename PrecOf<KIND>::R thr0, uint32_t &seed, Work &w)
 {
+	using Q = PrecOf<KIND>;
+	using R = typename Q::R;
+	using P = typename Q::P;
 	V3 out = v3(0.0f, 0.0f, 0.0f);
-	V3 p = p0, n = n0;
-	float thrCur = thr0;
+	P p = p0;
+	V3 n = n0;
+	R thrCur = thr0;
 	const V3 colour = v3(f.surfaceColor[0], f.surfaceColor[1], f.surfaceColor[2]);
 	V3 colourTemp = colour;
 	for (int depth = 0; depth < f.reflectionsMax; depth++)
@@ -318,22 +386,23 @@ __device__ V3 global_illumination(const FrameConst &f, V3 p0, V3 n0, float thr0,
 		const float rx = (float)(random_mod(seed, 20000) - 10000) * 1e-4f;
 		const float ry = (float)(random_mod(seed, 20000) - 10000) * 1e-4f;
 		const float rz = (float)(random_mod(seed, 20000) - 10000) * 1e-4f;
-		const V3 dir = normalize(n + v3(rx * 1.2f, ry * 1.2f, rz * 1.2f));
-		float dist = 0.0f;
+		const V3 dirf = normalize(n + v3(rx * 1.2f, ry * 1.2f, rz * 1.2f));
+		const P dir = Q::from(dirf);
+		R dist = 0;
 		bool found = false;
 		int stepCount = 0;
 		int it;
-		V3 hit = p;
-		for (float scan = thrCur; scan < f.viewDistanceMax; scan += dist * f.DEFactor)
+		P hit = p;
+		for (R scan = thrCur; scan < (R)f.viewDistanceMax; scan += dist * (R)f.DEFactor)
 		{
 			if (stepCount >= f.maxRaymarchingSteps) break;
-			const V3 q = p + dir * scan;
-			const float thr = dist_thresh(f, q);
+			const P q = p + dir * scan;
+			const R thr = dist_thresh<KIND>(f, q);
 			thrCur = thr;
 			dist = calculate_distance<KIND>(q, thr, f, it, w);
 			if (dist < thr)
 			{
-				if (scan < thr * 2.0f) return out;
+				if (scan < thr * (R)2) return out;
 				hit = q;
 				found = true;
 				break;
@@ -351,7 +420,7 @@ __device__ V3 global_illumination(const FrameConst &f, V3 p0, V3 n0, float thr0,
 		}
 		else
 		{
-			result = background_shader(f, dir);
+			result = background_shader(f, dirf);
 			finished = true;
 		}
 		const float lim = f.giRadianceLimit;
@@ -378,40 +447,43 @@ struct PixelOut
 template <int KIND>
 __device__ void render_pixel(const FrameConst &f, int xs, int ys, PixelOut &po, Work &w)
 {
+#pragma clang fp contract(off)
+	using Q = PrecOf<KIND>;
+	using R = typename Q::R;
+	using P = typename Q::P;
 	const uint32_t pix = (uint32_t)ys * (uint32_t)f.width + (uint32_t)xs;
 	uint32_t seed = pixel_seed(pix, f.randomSeed);
-	const float ox = (f.invWidth * (float)xs - 0.5f) * f.aspectRatio;
-	const float oy = 0.5f - f.invHeight * (float)ys;
-	const V3 cam = v3(f.camera[0], f.camera[1], f.camera[2]);
+	const R aspect = Q::aspect(f);
+	const R ox = (Q::invWidth(f) * (R)xs - (R)0.5) * aspect;
+	const R oy = (R)0.5 - Q::invHeight(f) * (R)ys;
+	const P cam = Q::cam(f);
 	const int repeats = f.monteCarlo ? f.samples : 1;
 	V3 acc = v3(0.0f, 0.0f, 0.0f);
 	po.steps = 0;
 	for (int rep = 0; rep < repeats; rep++)
 	{
-		float ix = ox, iy = oy;
+		R ix = ox, iy = oy;
 		if (f.monteCarlo)
 		{
-			ix = ox + ((float)random_mod(seed, 1000) * 1e-3f - 0.5f) * f.invWidth * f.aspectRatio;
-			iy = oy + ((float)random_mod(seed, 1000) * 1e-3f - 0.5f) * f.invHeight;
+			ix = ox + ((R)random_mod(seed, 1000) * (R)1e-3 - (R)0.5) * Q::invWidth(f) * aspect;
+			iy = oy + ((R)random_mod(seed, 1000) * (R)1e-3 - (R)0.5) * Q::invHeight(f);
 		}
-		// CalculateViewVector three-point (projection_3d.cpp:77-89), then normalise again
-		V3 vv = normalize(v3(ix * f.fov, 1.0f, iy * f.fov));
-		const V3 dir = normalize(rot(f.mRot, vv));
-		MarchOut m;
-		ray_marching<KIND>(f, cam, dir, f.viewDistanceMax, seed, m, w);
+		const P dir = Q::view(f, ix, iy);
+		MarchOut<KIND> m;
+		ray_marching<KIND>(f, cam, dir, (R)f.viewDistanceMax, seed, m, w);
 		V3 rgb;
 		V3 n = v3(0.0f, 0.0f, 0.0f);
 		if (m.found)
 		{
 			n = calculate_normals<KIND>(f, m.point, m.distThresh, w);
-			const float delta = calc_delta(f, m.point);
+			const R delta = calc_delta<KIND>(f, m.point);
 			rgb = object_shader<KIND>(f, m.point, n, m.distThresh, delta, seed, w);
 			if (f.monteCarlo && f.globalIllumination)
 				rgb = rgb + global_illumination<KIND>(f, m.point, n, m.distThresh, seed, w);
 			rgb = v3(fmaxf(rgb.x, 0.0f), fmaxf(rgb.y, 0.0f), fmaxf(rgb.z, 0.0f));
 		}
 		else
-			rgb = background_shader(f, dir);
+			rgb = background_shader(f, Q::tof(dir));
 		acc = acc + rgb;
 		if (rep == 0)
 		{
@@ -419,10 +491,10 @@ __device__ void render_pixel(const FrameConst &f, int xs, int ys, PixelOut &po, 
 			po.iters = m.iters;
 			po.normal = n;
 		}
-		po.depth = m.found ? m.depth : 1e20f;
+		po.depth = m.found ? (float)m.depth : 1e20f;
 		po.steps += m.steps;
 	}
-	po.rgb = repeats > 1 ? acc * (1.0f / (float)repeats) : acc;
+	po.rgb = repeats > 1 ? v3(acc.x / (float)repeats, acc.y / (float)repeats, acc.z / (float)repeats) : acc;
 }
 
 __device__ __forceinline__ void wave_add_u64(unsigned long long *dst, uint32_t v)
@@ -433,7 +505,27 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long *dst, uint32_t v
 	if ((threadIdx.x & 63) == 0) atomicAdd(dst, s);
 }
 
-// Plain per-frame kernel: 64 lanes = one 8x8 tile.
+__device__ __forceinline__ void write_pixel(const Outputs &out, size_t i, const PixelOut &po)
+{
+	if (out.rgb)
+	{
+		out.rgb[3 * i + 0] = po.rgb.x;
+		out.rgb[3 * i + 1] = po.rgb.y;
+		out.rgb[3 * i + 2] = po.rgb.z;
+	}
+	if (out.depth) out.depth[i] = po.depth;
+	if (out.iters) out.iters[i] = po.iters;
+	if (out.mask) out.mask[i] = (uint8_t)po.found;
+	if (out.normal)
+	{
+		out.normal[3 * i + 0] = po.normal.x;
+		out.normal[3 * i + 1] = po.normal.y;
+		out.normal[3 * i + 2] = po.normal.z;
+	}
+	if (out.steps) out.steps[i] = po.steps;
+}
+
+// Plain per-frame kernel: 64 lanes = one 8x8 tile, one launch per frame.
 template <int KIND>
 __global__ __launch_bounds__(64) void render_kernel(const FrameConst *__restrict__ frame,
 	Outputs out, unsigned long long *__restrict__ counters)
@@ -445,34 +537,69 @@ __global__ __launch_bounds__(64) void render_kernel(const FrameConst *__restrict
 	const int xs = tx * 8 + (threadIdx.x & 7);
 	const int ys = ty * 8 + (threadIdx.x >> 3);
 	Work w = {0u, 0u};
-	const bool active = xs < f.width && ys < f.height;
-	if (active)
+	if (xs < f.width && ys < f.height)
 	{
 		PixelOut po;
 		render_pixel<KIND>(f, xs, ys, po, w);
-		const size_t i = (size_t)ys * (size_t)f.width + (size_t)xs;
-		if (out.rgb)
-		{
-			out.rgb[3 * i + 0] = po.rgb.x;
-			out.rgb[3 * i + 1] = po.rgb.y;
-			out.rgb[3 * i + 2] = po.rgb.z;
-		}
-		if (out.depth) out.depth[i] = po.depth;
-		if (out.iters) out.iters[i] = po.iters;
-		if (out.mask) out.mask[i] = (uint8_t)po.found;
-		if (out.normal)
-		{
-			out.normal[3 * i + 0] = po.normal.x;
-			out.normal[3 * i + 1] = po.normal.y;
-			out.normal[3 * i + 2] = po.normal.z;
-		}
-		if (out.steps) out.steps[i] = po.steps;
+		write_pixel(out, (size_t)ys * (size_t)f.width + (size_t)xs, po);
 	}
 	if (counters)
 	{
 		wave_add_u64(&counters[0], w.de_evals);
 		wave_add_u64(&counters[1], w.iterations);
 	}
+}
+
+// Persistent work-stealing kernel: a fixed grid of waves drains one queue of 8x8 tiles that
+// covers every frame of a batch (frames of different resolution, camera and formula
+// parameters).  The wave-uniform tile index keeps each tile's frame constants in scalar
+// registers.  One launch replaces the reference's one-job-per-frame animation loop.
+// (frames and jobs are separate __restrict__ kernel arguments so that the compiler may read
+// them with scalar loads)
+template <int KIND>
+__global__ __launch_bounds__(64) void persistent_kernel(const FrameConst *__restrict__ frames,
+	const FrameJob *__restrict__ jobs, int numFrames, unsigned int *__restrict__ queue,
+	unsigned long long *__restrict__ counters)
+{
+	const int lane = threadIdx.x;
+	const unsigned int totalTiles = (unsigned int)jobs[numFrames - 1].tileEnd;
+	Work w = {0u, 0u};
+	int fr = 0;
+	while (true)
+	{
+		unsigned int t = 0;
+		if (lane == 0) t = atomicAdd(queue, 1u);
+		t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
+		if (t >= totalTiles) break;
+		while (t >= (unsigned int)jobs[fr].tileEnd) fr++; // tiles are handed out in order
+		const FrameConst &f = frames[fr];
+		const FrameJob &job = jobs[fr];
+		const unsigned int tileBegin = fr == 0 ? 0u : (unsigned int)jobs[fr - 1].tileEnd;
+		const int lt = (int)(t - tileBegin);
+		const int tilesX = (f.width + 7) >> 3;
+		const int xs = (lt % tilesX) * 8 + (lane & 7);
+		const int ys = (lt / tilesX) * 8 + (lane >> 3);
+		if (xs < f.width && ys < f.height)
+		{
+			PixelOut po;
+			render_pixel<KIND>(f, xs, ys, po, w);
+			const Outputs out{job.rgb, job.depth, job.iters, job.mask, job.normal, job.steps};
+			write_pixel(out, (size_t)ys * (size_t)f.width + (size_t)xs, po);
+		}
+	}
+	if (counters)
+	{
+		wave_add_u64(&counters[0], w.de_evals);
+		wave_add_u64(&counters[1], w.iterations);
+	}
+}
+
+// Zero the tile queue head and the work counters (a kernel, so that it can be captured in a
+// graph without a memset node).
+__global__ void clear_kernel(unsigned int *queue, unsigned long long *counters)
+{
+	if (threadIdx.x == 0) queue[0] = 0u;
+	if (threadIdx.x < 4) counters[threadIdx.x] = 0ull;
 }
 
 // Point DE query (distance engine analogue)
@@ -485,8 +612,9 @@ __global__ __launch_bounds__(64) void distance_kernel(const FrameConst *__restri
 	if (i >= n) return;
 	Work w = {0u, 0u};
 	int it = 0;
-	const V3 p = v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
-	dist[i] = calculate_distance<KIND>(p, detailSize, *frame, it, w);
+	using Q = PrecOf<KIND>;
+	const typename Q::P p = Q::from(v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
+	dist[i] = (float)calculate_distance<KIND>(p, (typename Q::R)detailSize, *frame, it, w);
 	iters[i] = it;
 }
 
@@ -520,6 +648,35 @@ hipError_t launch_render(int deKind, const FrameConst *frame, int width, int hei
 		case kBoxDelta: return launch_render_t<kBoxDelta>(frame, width, height, out, counters, stream);
 		case kIfsAnalytic:
 			return launch_render_t<kIfsAnalytic>(frame, width, height, out, counters, stream);
+		default: return hipErrorInvalidValue;
+	}
+}
+
+hipError_t launch_clear(unsigned int *queue, unsigned long long *counters, hipStream_t stream)
+{
+	hipLaunchKernelGGL(clear_kernel, dim3(1), dim3(64), 0, stream, queue, counters);
+	return hipGetLastError();
+}
+
+template <int KIND>
+static hipError_t launch_batch_t(const BatchArgs &a, int numWaves, hipStream_t stream)
+{
+	hipLaunchKernelGGL(persistent_kernel<KIND>, dim3(numWaves), dim3(64), 0, stream, a.frames, a.jobs,
+		a.numFrames, a.queue, a.counters);
+	return hipGetLastError();
+}
+
+hipError_t launch_batch(int deKind, BatchArgs a, int numWaves, hipStream_t stream)
+{
+	hipError_t e = launch_clear(a.queue, a.counters, stream);
+	if (e != hipSuccess) return e;
+	switch (deKind)
+	{
+		case kBulb8: return launch_batch_t<kBulb8>(a, numWaves, stream);
+		case kBulbTrig: return launch_batch_t<kBulbTrig>(a, numWaves, stream);
+		case kBoxAnalytic: return launch_batch_t<kBoxAnalytic>(a, numWaves, stream);
+		case kBoxDelta: return launch_batch_t<kBoxDelta>(a, numWaves, stream);
+		case kIfsAnalytic: return launch_batch_t<kIfsAnalytic>(a, numWaves, stream);
 		default: return hipErrorInvalidValue;
 	}
 }

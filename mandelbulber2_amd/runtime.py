@@ -56,6 +56,7 @@ def load():
 MBX_TUNE_KERNEL = 0
 MBX_TUNE_WAVES = 1
 MBX_TUNE_GRAPH = 2
+MBX_TUNE_DELTA_DE = 3
 KERNEL_PERSISTENT = 0
 KERNEL_PLAIN = 1
 
