@@ -1,27 +1,52 @@
 """Benchmark: Mrays/s of the primary DE raymarch on 1080p Mandelbulb-8, one MI355X.
 
 A step is one full frame (1920x1080, 1 ray per pixel: march + binary search + 6-tap normal +
-Lambert shading) rendered by libmbx.so from constants already resident in HBM.  The JSON line
-also carries the FP32 VALU roofline of the render kernel (algorithmic flops per launch over the
-launch's device time, HIP events on the launch stream) and the CPU oracle timed on this host.
+Lambert shading) rendered by libmbx.so from constants already resident in HBM, launched as the
+persistent tile-queue kernel through a captured HIP graph.  The JSON line also carries the FP32
+VALU roofline of that kernel (algorithmic flops per launch over the launch's device time, HIP
+events on the launch stream), the plain one-tile-per-block kernel timed on the same frame, and
+the CPU oracle timed on this host.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload mandelbulb8|mandelbox]
+
+`--workload mandelbox` times the Mandelbox scene instead (scale 2, delta DE by the reference's
+finite differences in FP64, hard shadow, 16-ray AO, seeded camera; 1920x1080).
 """
 import argparse
 import json
 import os
+import statistics
 import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-METRIC = "Mrays/s primary DE raymarch, 1080p Mandelbulb-8, 1 GPU; % FP32 VALU peak"
-FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
+FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak (packed)
+FP64_PEAK_TFLOPS = 78.6   # MI355X published FP64 peak
 # Algorithmic flop counts (fixed, independent of the instruction mix): per DE evaluation
-# (ray point, threshold, DE finish, step) and per Mandelbulb-8 iteration.
+# (ray point, threshold, DE finish, step) and per formula iteration.
 F_DE = 25
-F_IT = {"mandelbulb8": 48}
+F_IT = {"mandelbulb8": 48, "mandelbox": 29}
+
+WORKLOADS = {
+    "mandelbulb8": {
+        "metric": "Mrays/s primary DE raymarch, 1080p Mandelbulb-8, 1 GPU; % FP32 VALU peak",
+        "kernel": "persistent_kernel<kBulb8>", "dtype": "f32", "peak": FP32_PEAK_TFLOPS,
+        "config": {"formula": "mandelbulb power 8, analytic log DE, bailout 10, maxiter 250",
+                   "camera": "distance 3.5, fov 53 deg", "spp": 1,
+                   "shading": "6-tap normals + Lambert, 1 directional light, no shadows/AO/GI"},
+    },
+    "mandelbox": {
+        "metric": "Mrays/s DE raymarch with hard shadow + 16-ray AO, 1080p Mandelbox scale 2, "
+                  "1 GPU; % FP64 peak",
+        "kernel": "persistent_kernel<kBoxDelta>", "dtype": "f64", "peak": FP64_PEAK_TFLOPS,
+        "config": {"formula": "mandelbox scale 2, fold 1/2, radii 0.5/1, bailout 100, maxiter 50, "
+                              "delta DE (4-orbit finite difference), linear",
+                   "camera": "seed 1, radius U(6,10), target origin", "spp": 1,
+                   "shading": "hard penetrating shadow, 16 AO rays (Fibonacci hemisphere)"},
+    },
+}
 
 
 def parse():
@@ -29,32 +54,81 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="mandelbulb8")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU time budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip timing the plain kernel")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 256 MiB device-to-device copy, whose FETCH_SIZE/WRITE_SIZE "
+                         "counters calibrate the profiler's byte counts")
     return ap.parse_args()
+
+
+def measured_traffic(workload, w, h):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this
+    workload (copy-calibrated), or None when there is none for this size."""
+    name = {"mandelbulb8": "persistent_kernel_bulb8_mandelbulb8_1080p.md"}.get(workload)
+    if not name or (w, h) != (1920, 1080):
+        return None, None
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1]), "profiles/" + name
+    return None, None
+
+
+def make_scene(name, w, h):
+    from mandelbulber2_amd import scene
+    if name == "mandelbulb8":
+        return scene.mandelbulb8(w, h)
+    return scene.mandelbox_shadow_ao(w, h, seed=1, scale=2.0)
 
 
 def cpu_baseline(p, budget_s):
     """The FP64 oracle (built with the reference's release flags) on this host's cores, on
-    whole frames of the same workload until about budget_s has elapsed."""
+    rows of the same frame until about budget_s has elapsed."""
     from oracle import oracle
     threads = min(16, os.cpu_count() or 1)
+    H = p.imageHeight
+    band = max(8, H // 16)
     rays = 0
-    frames = 0
     t0 = time.time()
+    y = 0
     while True:
-        oracle.render(p, threads=threads, fast=True)
-        rays += p.imageWidth * p.imageHeight
-        frames += 1
-        if time.time() - t0 >= budget_s or frames >= 64:
+        y1 = min(H, y + band)
+        oracle.render(p, threads=threads, rows=(y, y1), fast=True)
+        rays += (y1 - y) * p.imageWidth
+        y = 0 if y1 >= H else y1
+        if time.time() - t0 >= budget_s or rays >= 64 * H * p.imageWidth:
             break
     dt = time.time() - t0
     return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": "%d full %dx%d frame(s) of the same scene, FP64 oracle built -O3 -ffast-math "
-                      "-fopenmp, %.1f s" % (frames, p.imageWidth, p.imageHeight, dt)}
+            "sample": "%d rays (row bands of the same %dx%d frame), FP64 oracle built -O3 "
+                      "-ffast-math -fopenmp, %d threads, %.1f s"
+                      % (rays, p.imageWidth, H, threads, dt)}
+
+
+def time_kernel(torch, ctx, bufs, stream, steps, warmup):
+    for _ in range(warmup):
+        ctx.render(bufs)
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        starts[i].record(stream)
+        ctx.render(bufs)
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    launch_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    return wall, launch_ms
 
 
 def main():
@@ -65,38 +139,43 @@ def main():
               file=sys.stderr)
         sys.exit(1)
     import torch
-    from mandelbulber2_amd import runtime, scene
+    from mandelbulber2_amd import runtime
 
-    p = scene.mandelbulb8(args.width, args.height)
+    wl = WORKLOADS[args.workload]
+    p = make_scene(args.workload, args.width, args.height)
     stream = torch.cuda.current_stream(0)
     ctx = runtime.Context(0, stream)
     ctx.upload(p)
     bufs = ctx.alloc_outputs()
-    for _ in range(args.warmup):
-        ctx.render(bufs)
-    torch.cuda.synchronize()
-
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        starts[i].record(stream)
-        ctx.render(bufs)
-        ends[i].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    launch_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_PERSISTENT)
+    wall, launch_ms = time_kernel(torch, ctx, bufs, stream, args.steps, args.warmup)
     st = ctx.stats()  # counters of the last launch (identical every frame)
+
+    plain = None
+    if not args.no_plain:
+        ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_PLAIN)
+        psteps = max(3, args.steps // 2)
+        pwall, pms = time_kernel(torch, ctx, bufs, stream, psteps, 1)
+        plain = {"kernel": wl["kernel"].replace("persistent_kernel", "render_kernel"),
+                 "ms_per_step": pwall * 1e3 / psteps, "median_launch_ms": statistics.median(pms)}
+        ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_PERSISTENT)
     ctx.close()
+    if args.calibrate_copy:
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     rays = p.imageWidth * p.imageHeight
     ms_per_step = wall * 1e3 / args.steps
     avg_launch_s = sum(launch_ms) / len(launch_ms) / 1e3
-    flops = F_DE * st["de_evals"] + F_IT["mandelbulb8"] * st["iterations"]
+    flops = F_DE * st["de_evals"] + F_IT[args.workload] * st["iterations"]
+    traffic, traffic_src = measured_traffic(args.workload, p.imageWidth, p.imageHeight)
     achieved = flops / avg_launch_s / 1e12
     out = {
-        "metric": METRIC,
+        "metric": wl["metric"],
         "value": rays / (ms_per_step / 1e3) / 1e6,
         "unit": "Mrays/s",
         "n_gpus": 1,
@@ -106,21 +185,24 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": wl["dtype"],
         "data": "synthetic",
-        "config": {"workload": "mandelbulb8_%dx%d" % (p.imageWidth, p.imageHeight),
-                   "formula": "mandelbulb power 8, analytic log DE, bailout 10, maxiter 250",
-                   "camera": "distance 3.5, fov 53 deg", "spp": 1,
-                   "shading": "6-tap normals + Lambert, 1 directional light, no shadows/AO/GI"},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "render_kernel<kBulb8>", "avg_launch_ms": avg_launch_s * 1e3,
+        "config": dict({"workload": "%s_%dx%d" % (args.workload, p.imageWidth, p.imageHeight)},
+                       **wl["config"]),
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": wl["peak"],
+                     "unit": "TFLOP/s", "frac": achieved / wl["peak"], "traffic": traffic,
+                     "traffic_source": traffic_src,
+                     "kernel": wl["kernel"], "avg_launch_ms": avg_launch_s * 1e3,
+                     "median_launch_ms": statistics.median(launch_ms),
                      "flops_per_launch": flops},
         "de_evals_per_s": st["de_evals"] / avg_launch_s,
         "iterations_per_s": st["iterations"] / avg_launch_s,
         "de_evals_per_frame": st["de_evals"],
         "iterations_per_frame": st["iterations"],
     }
+    if plain is not None:
+        plain["persistent_speedup"] = plain["ms_per_step"] / ms_per_step
+        out["plain_kernel"] = plain
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(p, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -443,33 +443,63 @@ __device__ __forceinline__ void KaleidoscopicIfsIteration(V3 &z, float &DE, cons
 // Compute<calcModeNormal> + CalculateDistanceSimple (compute_fractal.cpp:67-654,
 // calculate_distance.cpp:533-763) for one single-formula node.
 // ------------------------------------------------------------------------------------------
+// Per-lane state of the DE evaluator: the work counters, and the orbit constants of the
+// current frame copied out of FrameConst once per tile.  The constants are wave-uniform, so
+// they live in scalar registers and the march loops issue no constant loads.
 struct Work
 {
 	uint32_t de_evals;
 	uint32_t iterations;
+	int maxN, minN, addC, julia, analyticFunction;
+	float bailout2;
+	float cx, cy, cz; // Julia constant, or the constant multiplier of c = p
 };
 
-__device__ __forceinline__ V3 add_c(V3 z, V3 p, const FrameConst &f)
+__device__ __forceinline__ void load_orbit_constants(Work &w, const FrameConst &f)
 {
-	if (!f.addC) return z;
-	if (f.julia) return z + v3(f.juliaC[0], f.juliaC[1], f.juliaC[2]);
-	return z + v3(p.x * f.constMul[0], p.y * f.constMul[1], p.z * f.constMul[2]);
+	w.maxN = f.formulaMaxiter;
+	w.minN = f.minN;
+	w.addC = f.addC;
+	w.julia = f.julia;
+	w.analyticFunction = f.analyticFunction;
+	w.bailout2 = f.bailout * f.bailout;
+	w.cx = f.julia ? f.juliaC[0] : f.constMul[0];
+	w.cy = f.julia ? f.juliaC[1] : f.constMul[1];
+	w.cz = f.julia ? f.juliaC[2] : f.constMul[2];
 }
 
+__device__ __forceinline__ Work make_work(const FrameConst &f)
+{
+	Work w;
+	w.de_evals = 0u;
+	w.iterations = 0u;
+	load_orbit_constants(w, f);
+	return w;
+}
+
+// z + c with c = p * constantMultiplier, or the Julia constant (compute_fractal.cpp:246-281)
+__device__ __forceinline__ V3 add_c(V3 z, V3 p, const Work &w)
+{
+	if (!w.addC) return z;
+	if (w.julia) return z + v3(w.cx, w.cy, w.cz);
+	return v3(z.x + p.x * w.cx, z.y + p.y * w.cy, z.z + p.z * w.cz);
+}
+
+// ln(x) for x > 1 from the hardware log2 (v_log_f32)
+__device__ __forceinline__ float flog(float x) { return __builtin_amdgcn_logf(x) * 0.69314718055994531f; }
+
 // Analytic-DE orbit.  Returns the distance and the iteration count (i + 1, maxN + 1 when the
-// orbit never escapes).
+// orbit never escapes).  kBulb8 is only chosen for c = p * constantMultiplier.
 template <int KIND>
-__device__ __forceinline__ float compute_analytic(V3 p, const FrameConst &f, int &iters)
+__device__ __forceinline__ float compute_analytic(V3 p, const FrameConst &f, const Work &w, int &iters)
 {
 	V3 z = p;
 	float r2 = dot(z, z);
 	float DE = 1.0f;
-	const int maxN = f.formulaMaxiter;
-	const float bailout2 = f.bailout * f.bailout;
 	int i;
-	for (i = 0; i < maxN; i++)
+	for (i = 0; i < w.maxN; i++)
 	{
-		const V3 lastZ = z;
+		const float lastR2 = r2;
 		if (KIND == kBulb8)
 			MandelbulbPow8Iteration(z, r2, DE);
 		else if (KIND == kBulbTrig)
@@ -478,21 +508,20 @@ __device__ __forceinline__ float compute_analytic(V3 p, const FrameConst &f, int
 			MandelboxIteration<false>(z, DE, nullptr, f);
 		else
 			KaleidoscopicIfsIteration(z, DE, f);
-		z = add_c(z, p, f);
+		if (KIND == kBulb8)
+			z = v3(z.x + p.x * w.cx, z.y + p.y * w.cy, z.z + p.z * w.cz);
+		else
+			z = add_c(z, p, w);
 		r2 = dot(z, z);
-		// one compare catches a NaN or infinite component (r2 is then NaN or +inf); the rare
-		// branch tells an overflowing finite orbit (an escape) from a non-finite one (revert to
-		// the last z, compute_fractal.cpp:303-309)
-		if (!(r2 <= 3.0e38f))
+		// one compare ends the orbit: escape (r2 > bailout^2), or a NaN / infinite component
+		// (r2 NaN or +inf).  The rare branch tells an overflowing finite orbit (an escape) from
+		// a non-finite one, which reverts to the last z (compute_fractal.cpp:303-309); only
+		// |z| is used after the loop.
+		if (!(r2 <= w.bailout2))
 		{
-			if (!finite3(z))
-			{
-				z = lastZ;
-				r2 = dot(z, z);
-			}
+			if (!(r2 <= 3.0e38f) && !finite3(z)) r2 = lastR2;
 			break;
 		}
-		if (r2 > bailout2) break;
 	}
 	iters = i + 1;
 	const float r = fsqrt(r2);
@@ -500,9 +529,9 @@ __device__ __forceinline__ float compute_analytic(V3 p, const FrameConst &f, int
 	if (DE > 0.0f)
 	{
 		const float invDE = frcp(DE);
-		if (f.analyticFunction == MBX_ANALYTIC_LOGARITHMIC)
-			distance = (r > 1.0f) ? 0.5f * r * __logf(r) * invDE : 0.0f;
-		else if (f.analyticFunction == MBX_ANALYTIC_IFS)
+		if (KIND == kBulb8 || w.analyticFunction == MBX_ANALYTIC_LOGARITHMIC)
+			distance = (r > 1.0f) ? 0.5f * r * flog(r) * invDE : 0.0f;
+		else if (w.analyticFunction == MBX_ANALYTIC_IFS)
 			distance = (r - 2.0f) * invDE;
 		else
 			distance = r * invDE;
@@ -622,17 +651,22 @@ __device__ __forceinline__ typename Prec<PosPrecision<KIND>::kFP64>::R calculate
 	}
 	else
 	{
-		d = compute_analytic<KIND>(p, f, iters);
+		d = compute_analytic<KIND>(p, f, w, iters);
 		w.iterations += (uint32_t)iters;
 		itersMinN = iters;
 	}
 	w.de_evals++;
-	if (itersMinN < f.minN && d < detailSize) d = detailSize;
-	// clamp as the reference does; a NaN passes the clamp and the boolean-add merge of the
-	// objects tree then keeps its initial 1e20
-	d = (d < (R)0) ? (R)0 : d;
-	d = (d > (R)10) ? (R)10 : d;
-	if (!(d < (R)1e20)) d = (R)1e20;
+	if (itersMinN < w.minN && d < detailSize) d = detailSize;
+	// clamp to [0, 10] as the reference does; a NaN passes the clamp and the boolean-add merge
+	// of the objects tree then keeps its initial 1e20
+	if constexpr (PosPrecision<KIND>::kFP64)
+	{
+		d = (d < (R)0) ? (R)0 : d;
+		d = (d > (R)10) ? (R)10 : d;
+		if (!(d < (R)1e20)) d = (R)1e20;
+	}
+	else
+		d = (d != d) ? 1e20f : fminf(fmaxf(d, 0.0f), 10.0f);
 	return d;
 }
 

@@ -370,7 +370,8 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 	if (fr.formula == MBX_FORMULA_MANDELBULB && deType == MBX_DE_ANALYTIC)
 	{
 		const bool pow8 = fr.bulb.power == 8.0 && fr.bulb.alphaAngleOffset == 0.0 &&
-			fr.bulb.betaAngleOffset == 0.0 && analyticFunction == MBX_ANALYTIC_LOGARITHMIC;
+			fr.bulb.betaAngleOffset == 0.0 && analyticFunction == MBX_ANALYTIC_LOGARITHMIC &&
+			fc.addC && !fc.julia; // the power-8 kernel adds c = p * constantMultiplier
 		kind = pow8 ? mbx::kBulb8 : mbx::kBulbTrig;
 		if (analyticFunction != MBX_ANALYTIC_LOGARITHMIC && analyticFunction != MBX_ANALYTIC_LINEAR)
 			kind = -1;

@@ -68,20 +68,32 @@ __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start
 	const R dirLength = Q::len(dir);
 	const R invDirLength = Q::rcp(dirLength);
 	const R searchLimit = kExact ? (R)(1.0 - 0.001 * f.detailLevelD) : (R)f.searchLimit;
-	for (int i = 0; i < f.maxRaymarchingSteps; i++)
+	// FP32: the ray starts at the camera, so |camera - point| = scan |dir|; one multiply, and
+	// closer to the FP64 value than the difference of two rounded positions
+	const bool constThr = f.constantDEThreshold != 0;
+	const R thrScale = kExact ? (R)0 : dirLength * (R)f.threshK;
+	const R constThrValue = Q::DEThresh(f);
+	const int maxSteps = f.maxRaymarchingSteps;
+	for (int i = 0; i < maxSteps; i++)
 	{
 		const P last = point;
 		counter++;
 		point = Q::along(start, dir, scan);
-		if ((point.x == last.x && point.y == last.y && point.z == last.z) ||
-			!(isfinite(point.x) && isfinite(point.y) && isfinite(point.z)))
+		// a step below the position resolution leaves the point where it was (dead point).  In
+		// FP32 the point is always finite: every step is in [0.45 thr DEFactor, 3].
+		bool deadPoint = point.x == last.x && point.y == last.y && point.z == last.z;
+		if (kExact) deadPoint = deadPoint || !(isfinite(point.x) && isfinite(point.y) && isfinite(point.z));
+		if (deadPoint)
 		{
 			point = last;
 			found = true;
 			dead = true;
 			break;
 		}
-		thr = dist_thresh<KIND>(f, point);
+		if (kExact)
+			thr = dist_thresh<KIND>(f, point);
+		else
+			thr = constThr ? constThrValue : scan * thrScale;
 		dist = calculate_distance<KIND>(point, thr, f, iters, w);
 		if (dist < thr)
 		{
@@ -116,7 +128,10 @@ __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start
 			else if (dist < thr * searchLimit)
 				scan -= step;
 			point = Q::along(start, dir, scan);
-			thr = dist_thresh<KIND>(f, point);
+			if (kExact)
+				thr = dist_thresh<KIND>(f, point);
+			else
+				thr = constThr ? constThrValue : scan * thrScale;
 			dist = calculate_distance<KIND>(point, thr, f, iters, w);
 			step *= (R)0.5;
 		}
@@ -527,7 +542,7 @@ __device__ __forceinline__ void write_pixel(const Outputs &out, size_t i, const 
 
 // Plain per-frame kernel: 64 lanes = one 8x8 tile, one launch per frame.
 template <int KIND>
-__global__ __launch_bounds__(64) void render_kernel(const FrameConst *__restrict__ frame,
+__global__ __launch_bounds__(64, KIND >= kBoxAnalytic ? 2 : 1) void render_kernel(const FrameConst *__restrict__ frame,
 	Outputs out, unsigned long long *__restrict__ counters)
 {
 	const FrameConst &f = *frame;
@@ -536,7 +551,7 @@ __global__ __launch_bounds__(64) void render_kernel(const FrameConst *__restrict
 	const int tx = tile % tilesX, ty = tile / tilesX;
 	const int xs = tx * 8 + (threadIdx.x & 7);
 	const int ys = ty * 8 + (threadIdx.x >> 3);
-	Work w = {0u, 0u};
+	Work w = make_work(f);
 	if (xs < f.width && ys < f.height)
 	{
 		PixelOut po;
@@ -553,17 +568,20 @@ __global__ __launch_bounds__(64) void render_kernel(const FrameConst *__restrict
 // Persistent work-stealing kernel: a fixed grid of waves drains one queue of 8x8 tiles that
 // covers every frame of a batch (frames of different resolution, camera and formula
 // parameters).  The wave-uniform tile index keeps each tile's frame constants in scalar
-// registers.  One launch replaces the reference's one-job-per-frame animation loop.
+// registers.  One launch replaces the reference's one-job-per-frame animation loop, and the
+// fixed grid avoids dispatching one workgroup per tile.
 // (frames and jobs are separate __restrict__ kernel arguments so that the compiler may read
 // them with scalar loads)
+// At least two waves per SIMD for the Mandelbox and IFS kinds: uncapped, their kernels take
+// more than 256 registers and run one wave per SIMD.
 template <int KIND>
-__global__ __launch_bounds__(64) void persistent_kernel(const FrameConst *__restrict__ frames,
+__global__ __launch_bounds__(64, KIND >= kBoxAnalytic ? 2 : 1) void persistent_kernel(const FrameConst *__restrict__ frames,
 	const FrameJob *__restrict__ jobs, int numFrames, unsigned int *__restrict__ queue,
 	unsigned long long *__restrict__ counters)
 {
 	const int lane = threadIdx.x;
 	const unsigned int totalTiles = (unsigned int)jobs[numFrames - 1].tileEnd;
-	Work w = {0u, 0u};
+	Work w = make_work(frames[0]);
 	int fr = 0;
 	while (true)
 	{
@@ -574,6 +592,7 @@ __global__ __launch_bounds__(64) void persistent_kernel(const FrameConst *__rest
 		while (t >= (unsigned int)jobs[fr].tileEnd) fr++; // tiles are handed out in order
 		const FrameConst &f = frames[fr];
 		const FrameJob &job = jobs[fr];
+		load_orbit_constants(w, f);
 		const unsigned int tileBegin = fr == 0 ? 0u : (unsigned int)jobs[fr - 1].tileEnd;
 		const int lt = (int)(t - tileBegin);
 		const int tilesX = (f.width + 7) >> 3;
@@ -610,7 +629,7 @@ __global__ __launch_bounds__(64) void distance_kernel(const FrameConst *__restri
 {
 	const int i = blockIdx.x * 64 + threadIdx.x;
 	if (i >= n) return;
-	Work w = {0u, 0u};
+	Work w = make_work(*frame);
 	int it = 0;
 	using Q = PrecOf<KIND>;
 	const typename Q::P p = Q::from(v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
