@@ -7,10 +7,12 @@ VALU roofline of that kernel (algorithmic flops per launch over the launch's dev
 events on the launch stream), the plain one-tile-per-block kernel timed on the same frame, and
 the CPU oracle timed on this host.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload mandelbulb8|mandelbox]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload mandelbulb8|mandelbox|animation64]
 
 `--workload mandelbox` times the Mandelbox scene instead (scale 2, delta DE by the reference's
 finite differences in FP64, hard shadow, 16-ray AO, seeded camera; 1920x1080).
+`--workload animation64` times a 64-frame batch (power 2 -> 12, mixed resolutions, 4 spp with
+GI depth 2) in one persistent launch against one plain launch per frame.
 """
 import argparse
 import json
@@ -27,7 +29,7 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X published FP64 peak
 # Algorithmic flop counts (fixed, independent of the instruction mix): per DE evaluation
 # (ray point, threshold, DE finish, step) and per formula iteration.
 F_DE = 25
-F_IT = {"mandelbulb8": 48, "mandelbox": 29}
+F_IT = {"mandelbulb8": 48, "mandelbox": 29, "animation64": 32}
 
 WORKLOADS = {
     "mandelbulb8": {
@@ -36,6 +38,16 @@ WORKLOADS = {
         "config": {"formula": "mandelbulb power 8, analytic log DE, bailout 10, maxiter 250",
                    "camera": "distance 3.5, fov 53 deg", "spp": 1,
                    "shading": "6-tap normals + Lambert, 1 directional light, no shadows/AO/GI"},
+    },
+    "animation64": {
+        "metric": "Mrays/s (samples) of a 64-frame animation batch in one persistent launch, "
+                  "1 GPU; speedup over per-frame plain launches",
+        "kernel": "persistent_kernel<kBulbTrig>", "dtype": "f32", "peak": FP32_PEAK_TFLOPS,
+        "config": {"formula": "mandelbulb power 2 -> 12 over 64 frames (trigonometric form), "
+                              "analytic log DE, bailout 10, maxiter 250",
+                   "camera": "seed 3, random walk on the radius 2.5-4 shell",
+                   "resolutions": "per frame from 1920x1080 / 1280x720 / 640x480 (seeded)",
+                   "spp": 4, "shading": "Monte Carlo GI depth 2, Lambert, 1 directional light"},
     },
     "mandelbox": {
         "metric": "Mrays/s DE raymarch with hard shadow + 16-ray AO, 1080p Mandelbox scale 2, "
@@ -86,6 +98,8 @@ def make_scene(name, w, h):
     from mandelbulber2_amd import scene
     if name == "mandelbulb8":
         return scene.mandelbulb8(w, h)
+    if name == "animation64":
+        return scene.animation64(seed=3)
     return scene.mandelbox_shadow_ao(w, h, seed=1, scale=2.0)
 
 
@@ -113,9 +127,10 @@ def cpu_baseline(p, budget_s):
                       % (rays, p.imageWidth, H, threads, dt)}
 
 
-def time_kernel(torch, ctx, bufs, stream, steps, warmup):
+def time_kernel(torch, ctx, bufs, stream, steps, warmup, step_fn=None):
+    step_fn = step_fn or (lambda: ctx.render(bufs))
     for _ in range(warmup):
-        ctx.render(bufs)
+        step_fn()
     torch.cuda.synchronize()
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
@@ -123,12 +138,87 @@ def time_kernel(torch, ctx, bufs, stream, steps, warmup):
     t0 = time.perf_counter()
     for i in range(steps):
         starts[i].record(stream)
-        ctx.render(bufs)
+        step_fn()
         ends[i].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     launch_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
     return wall, launch_ms
+
+
+def cpu_baseline_batch(frames, budget_s):
+    """The FP64 oracle on row bands of the batch's first and last frames (power 2 and 12) until
+    about budget_s has elapsed; Mrays/s counts samples."""
+    from oracle import oracle
+    threads = min(16, os.cpu_count() or 1)
+    rays = 0
+    t0 = time.time()
+    k = 0
+    while True:
+        p = frames[0] if k % 2 == 0 else frames[-1]
+        y = (k // 2 * 8) % p.imageHeight
+        y1 = min(p.imageHeight, y + 8)
+        oracle.render(p, threads=threads, rows=(y, y1), fast=True)
+        rays += (y1 - y) * p.imageWidth * p.DOFSamples
+        k += 1
+        if time.time() - t0 >= budget_s:
+            break
+    dt = time.time() - t0
+    return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": "%d samples (8-row bands of frames 0 and 63, 4 spp, GI depth 2), FP64 "
+                      "oracle built -O3 -ffast-math -fopenmp, %d threads, %.1f s"
+                      % (rays, threads, dt)}
+
+
+def bench_batch(args, torch, runtime, wl):
+    """One step = the whole 64-frame batch in one persistent launch (captured graph); the
+    comparison renders the same frames one plain launch per frame."""
+    frames = make_scene(args.workload, 0, 0)
+    stream = torch.cuda.current_stream(0)
+    ctx = runtime.Context(0, stream)
+    bufs = [ctx.alloc_outputs(p) for p in frames]
+    ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_PERSISTENT)
+    wall, launch_ms = time_kernel(torch, ctx, None, stream, args.steps, args.warmup,
+                                  lambda: ctx.render_batch(frames, bufs))
+    st = ctx.stats()
+    plain = None
+    if not args.no_plain:
+        ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_PLAIN)
+
+        def per_frame():
+            for p, b in zip(frames, bufs):
+                ctx.upload(p)
+                ctx.render(b)
+        psteps = max(1, args.steps // 2)
+        pwall, pms = time_kernel(torch, ctx, None, stream, psteps, 1, per_frame)
+        plain = {"kernel": wl["kernel"].replace("persistent_kernel", "render_kernel") +
+                 " x 64 launches", "ms_per_step": pwall * 1e3 / psteps,
+                 "median_launch_ms": statistics.median(pms)}
+    ctx.close()
+    ms_per_step = wall * 1e3 / args.steps
+    avg_launch_s = sum(launch_ms) / len(launch_ms) / 1e3
+    flops = F_DE * st["de_evals"] + F_IT[args.workload] * st["iterations"]
+    achieved = flops / avg_launch_s / 1e12
+    out = {
+        "metric": wl["metric"], "value": st["rays"] / (ms_per_step / 1e3) / 1e6, "unit": "Mrays/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": wl["dtype"],
+        "data": "synthetic",
+        "config": dict({"workload": "animation64", "frames": len(frames),
+                        "samples_per_batch": st["rays"]}, **wl["config"]),
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": wl["peak"], "unit": "TFLOP/s",
+                     "frac": achieved / wl["peak"], "traffic": None, "kernel": wl["kernel"],
+                     "avg_launch_ms": avg_launch_s * 1e3,
+                     "median_launch_ms": statistics.median(launch_ms), "flops_per_launch": flops},
+        "de_evals_per_s": st["de_evals"] / avg_launch_s,
+        "iterations_per_s": st["iterations"] / avg_launch_s,
+    }
+    if plain is not None:
+        plain["persistent_speedup"] = plain["ms_per_step"] / ms_per_step
+        out["plain_kernel"] = plain
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_batch(frames, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -142,6 +232,8 @@ def main():
     from mandelbulber2_amd import runtime
 
     wl = WORKLOADS[args.workload]
+    if args.workload == "animation64":
+        return bench_batch(args, torch, runtime, wl)
     p = make_scene(args.workload, args.width, args.height)
     stream = torch.cuda.current_stream(0)
     ctx = runtime.Context(0, stream)

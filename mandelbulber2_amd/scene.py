@@ -231,6 +231,37 @@ def menger_ifs(width=320, height=240, seed=2):
     return p
 
 
+ANIMATION_RESOLUTIONS = ((1920, 1080), (1280, 720), (640, 480))
+
+
+def animation64(seed=3, frames=64, downscale=1):
+    """Batched animation: `frames` Mandelbulb frames with power_k = 2 + 10 k / (frames - 1)
+    (general-power trigonometric form), the camera on a seeded random walk over the shell of
+    radius 2.5-4 around the origin, each frame's resolution drawn from 1920x1080, 1280x720 and
+    640x480 by the seed (divided by `downscale` for small parity runs), 4 Monte Carlo samples
+    per pixel with global illumination of depth 2.  Returns the list of parameter blocks."""
+    rs = SeededStream(seed)
+    d = [rs.normal() for _ in range(3)]
+    radius = rs.uniform(2.5, 4.0)
+    out = []
+    for k in range(frames):
+        n = math.sqrt(sum(c * c for c in d))
+        d = [c / n for c in d]
+        w, h = ANIMATION_RESOLUTIONS[int(rs.uniform(0.0, 3.0)) % 3]
+        p = mandelbulb8(max(8, w // downscale), max(8, h // downscale), seed=seed * 1000 + k)
+        abi.set3(p.camera, [c * radius for c in d])
+        p.fractal.bulb.power = 2.0 + 10.0 * k / max(1, frames - 1)
+        p.DOFMonteCarlo = 1
+        p.DOFSamples = 4
+        p.DOFMonteCarloGlobalIllumination = 1
+        p.reflectionsMax = 2
+        out.append(p)
+        # random walk: perturb the direction, then the radius, re-projected onto the shell
+        d = [c + 0.06 * rs.normal() for c in d]
+        radius = min(4.0, max(2.5, radius + 0.05 * rs.normal()))
+    return out
+
+
 def params_bytes(p):
     return bytes(memoryview(p))
 

@@ -168,6 +168,28 @@ def test_batch_equals_single_frames(rt, powers):
     ctx.close()
 
 
+def test_animation_batch_parity(rt, oracle_mod):
+    """The batched-animation workload (power sweep, random-walk camera, mixed resolutions, 4 spp
+    Monte Carlo GI of depth 2) at 1/16 resolution, six frames in one persistent launch, each
+    frame against the oracle."""
+    import torch
+    frames = scene.animation64(seed=3, frames=6, downscale=16)
+    ctx = rt.Context(0)
+    bufs = [ctx.alloc_outputs(p) for p in frames]
+    ctx.render_batch(frames, bufs)
+    torch.cuda.synchronize()
+    for k, (p, b) in enumerate(zip(frames, bufs)):
+        g = {key: t.cpu().numpy() for key, t in b.items() if not key.startswith("_")}
+        r = oracle_mod.render(p)
+        st = compare(g, r)
+        _report("animation frame %d power %.2f %dx%d" % (k, p.fractal.bulb.power, p.imageWidth,
+                                                         p.imageHeight), st)
+        assert st["mask_agree"] >= 0.999, st
+        assert st["iters_agree"] >= 0.995, st  # general power: asin/atan2/pow in FP32
+        assert st["rgb_within_1e-2"] >= 0.995, st
+    ctx.close()
+
+
 def test_graph_replays_match_direct_launch(rt):
     """The captured-graph launch, replayed several times with other device work in between,
     gives the direct launch's result every time."""
