@@ -73,6 +73,9 @@ def parse():
                     help="approximate CPU time budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain kernel")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the launch-shape autotuner (tile shape x wave count) that runs "
+                         "before timing on the mandelbulb8 workload")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 256 MiB device-to-device copy, whose FETCH_SIZE/WRITE_SIZE "
                          "counters calibrate the profiler's byte counts")
@@ -240,6 +243,14 @@ def main():
     ctx.upload(p)
     bufs = ctx.alloc_outputs()
     ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_PERSISTENT)
+    launch_shape = None
+    if args.workload == "mandelbulb8" and not args.no_autotune:
+        waves, _ = ctx.autotune(bufs, reps=2)
+        launch_shape = {"waves": waves, "tile_width": ctx.get_tune(runtime.MBX_TUNE_TILE_WIDTH),
+                        "source": "mbx_autotune"}
+    else:
+        launch_shape = {"waves": ctx.get_tune(runtime.MBX_TUNE_WAVES),
+                        "tile_width": ctx.get_tune(runtime.MBX_TUNE_TILE_WIDTH), "source": "default"}
     wall, launch_ms = time_kernel(torch, ctx, bufs, stream, args.steps, args.warmup)
     st = ctx.stats()  # counters of the last launch (identical every frame)
 
@@ -248,6 +259,7 @@ def main():
         ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_PLAIN)
         psteps = max(3, args.steps // 2)
         pwall, pms = time_kernel(torch, ctx, bufs, stream, psteps, 1)
+        ctx.tune(runtime.MBX_TUNE_TILE_WIDTH, 8)
         plain = {"kernel": wl["kernel"].replace("persistent_kernel", "render_kernel"),
                  "ms_per_step": pwall * 1e3 / psteps, "median_launch_ms": statistics.median(pms)}
         ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_PERSISTENT)
@@ -279,8 +291,8 @@ def main():
         "vs_baseline": None,
         "dtype": wl["dtype"],
         "data": "synthetic",
-        "config": dict({"workload": "%s_%dx%d" % (args.workload, p.imageWidth, p.imageHeight)},
-                       **wl["config"]),
+        "config": dict({"workload": "%s_%dx%d" % (args.workload, p.imageWidth, p.imageHeight),
+                        "launch": launch_shape}, **wl["config"]),
         "roofline": {"bound": "valu", "achieved": achieved, "peak": wl["peak"],
                      "unit": "TFLOP/s", "frac": achieved / wl["peak"], "traffic": traffic,
                      "traffic_source": traffic_src,

@@ -299,15 +299,21 @@ int mbx_render_batch(mbx_ctx *ctx, const mbx_params *frames, const mbx_outputs *
 #define MBX_TUNE_GRAPH 2     /* persistent kernel: 1 = replay a captured HIP graph (default), 0 = direct */
 #define MBX_TUNE_DELTA_DE 3  /* delta DE: 0 = the reference's finite differences (default),
                                 1 = forward-mode tangents (smooth limit of the same quotient) */
+#define MBX_TUNE_TILE_WIDTH 4 /* pixels per tile row: 4, 8 (default), 16 or 32; a tile is 64
+                                 pixels (one wave), so its height is 64 / width */
 
 /* Set a launch or numerics option (see MBX_TUNE_*); scene options apply from the next
  * mbx_scene_upload / mbx_render_batch.  No reference counterpart: the OpenCL engine's
  * adaptive job size (opencl_engine.cpp:429-450) is the closest analogue. */
 int mbx_tune(mbx_ctx *ctx, int key, int value);
 
-/* Project autotuner: time the uploaded scene with the persistent kernel for a set of wave
- * counts (`reps` renders each into `out`) and keep the fastest.  Returns the chosen wave count
- * in *best_waves and its time per render in *best_ms (either may be NULL). */
+/* Current value of a tuning key (the wave count resolves the default). */
+int mbx_get_tune(mbx_ctx *ctx, int key, int *value);
+
+/* Project autotuner: time the uploaded scene with the persistent kernel for each tile shape
+ * (tile widths 8, 16, 4, 32) and a set of wave counts (`reps` renders each into `out`) and keep
+ * the fastest pair.  Returns the chosen wave count in *best_waves and its time per render in
+ * *best_ms (either may be NULL); mbx_get_tune(MBX_TUNE_TILE_WIDTH) gives the tile width. */
 int mbx_autotune(mbx_ctx *ctx, const mbx_outputs *out, int reps, int *best_waves, double *best_ms);
 
 /* Statistics of the last render (synchronises the stream). */

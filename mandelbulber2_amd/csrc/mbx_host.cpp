@@ -47,7 +47,8 @@ struct mbx_ctx
 	std::vector<mbx::FrameJob> jobsHost; // what d_jobs holds
 	std::vector<FrameConst> framesHost;  // what d_frames holds
 	hipGraphExec_t graphExec = nullptr;
-	int graphKind = -1, graphFrames = -1, graphWaves = -1;
+	int graphKind = -1, graphFrames = -1, graphWaves = -1, graphTile = -1;
+	int tileShift = 3; // 8x8-pixel tiles
 };
 
 namespace
@@ -496,10 +497,11 @@ int ensure_capacity(mbx_ctx *ctx, int n)
 	return MBX_OK;
 }
 
-mbx::FrameJob make_job(const FrameConst &fc, const mbx_outputs &o, int tileBase, int pixBase)
+mbx::FrameJob make_job(const FrameConst &fc, const mbx_outputs &o, int tileBase, int pixBase, int tileShift)
 {
 	mbx::FrameJob j{};
-	j.tileEnd = tileBase + ((fc.width + 7) / 8) * ((fc.height + 7) / 8);
+	const int tw = 1 << tileShift, th = 64 >> tileShift;
+	j.tileEnd = tileBase + ((fc.width + tw - 1) / tw) * ((fc.height + th - 1) / th);
 	j.pixBase = pixBase;
 	j.rgb = o.rgb;
 	j.depth = o.depth;
@@ -556,6 +558,7 @@ int enqueue_persistent(mbx_ctx *ctx, int kind, int nFrames)
 	a.numFrames = nFrames;
 	a.queue = ctx->d_queue;
 	a.counters = ctx->d_counters;
+	a.tileShift = ctx->tileShift;
 	hipError_t e;
 	if (!ctx->useGraph)
 	{
@@ -564,7 +567,8 @@ int enqueue_persistent(mbx_ctx *ctx, int kind, int nFrames)
 			return hip_fail(ctx, e, "batch launch");
 		return MBX_OK;
 	}
-	if (!ctx->graphExec || ctx->graphKind != kind || ctx->graphFrames != nFrames || ctx->graphWaves != waves)
+	if (!ctx->graphExec || ctx->graphKind != kind || ctx->graphFrames != nFrames || ctx->graphWaves != waves ||
+		ctx->graphTile != ctx->tileShift)
 	{
 		if (ctx->graphExec)
 		{
@@ -596,6 +600,7 @@ int enqueue_persistent(mbx_ctx *ctx, int kind, int nFrames)
 		ctx->graphKind = kind;
 		ctx->graphFrames = nFrames;
 		ctx->graphWaves = waves;
+		ctx->graphTile = ctx->tileShift;
 	}
 	// timing starts after any (re)capture, which is host work
 	if ((e = hipEventRecord(ctx->ev0, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
@@ -614,7 +619,7 @@ int render_frames(mbx_ctx *ctx, const std::vector<FrameConst> &frames, const mbx
 	size_t pixels = 0;
 	for (size_t i = 0; i < frames.size(); i++)
 	{
-		jobs.push_back(make_job(frames[i], outs[i], (int)tiles, (int)pixels));
+		jobs.push_back(make_job(frames[i], outs[i], (int)tiles, (int)pixels, ctx->tileShift));
 		tiles = jobs.back().tileEnd;
 		pixels += (size_t)frames[i].width * frames[i].height;
 		rays += (uint64_t)frames[i].width * frames[i].height * (uint64_t)frames[i].samples;
@@ -732,7 +737,7 @@ int mbx_render(mbx_ctx *ctx, const mbx_outputs *out)
 		return hip_fail(ctx, e, "clear counters");
 	if ((e = hipEventRecord(ctx->ev0, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
 	e = mbx::launch_render(ctx->frame.deKind, ctx->d_frame, ctx->frame.width, ctx->frame.height, o,
-		ctx->d_counters, ctx->stream);
+		ctx->d_counters, ctx->tileShift, ctx->stream);
 	if (e != hipSuccess) return hip_fail(ctx, e, "render launch");
 	if ((e = hipEventRecord(ctx->ev1, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
 	ctx->launched = true;
@@ -783,6 +788,29 @@ int mbx_tune(mbx_ctx *ctx, int key, int value)
 		case MBX_TUNE_DELTA_DE:
 			ctx->deltaTangent = value ? 1 : 0;
 			return MBX_OK;
+		case MBX_TUNE_TILE_WIDTH:
+		{
+			int shift = -1;
+			for (int k = 2; k <= 5; k++)
+				if (value == (1 << k)) shift = k;
+			if (shift < 0) return fail(ctx, MBX_ERR_INVALID, "tile width must be 4, 8, 16 or 32");
+			ctx->tileShift = shift;
+			return MBX_OK;
+		}
+		default: return fail(ctx, MBX_ERR_INVALID, "unknown tuning key");
+	}
+}
+
+int mbx_get_tune(mbx_ctx *ctx, int key, int *value)
+{
+	if (!ctx || !value) return fail(ctx, MBX_ERR_INVALID, "null argument");
+	switch (key)
+	{
+		case MBX_TUNE_KERNEL: *value = ctx->kernelMode; return MBX_OK;
+		case MBX_TUNE_WAVES: *value = ctx->waves > 0 ? ctx->waves : default_waves(ctx); return MBX_OK;
+		case MBX_TUNE_GRAPH: *value = ctx->useGraph; return MBX_OK;
+		case MBX_TUNE_DELTA_DE: *value = ctx->deltaTangent; return MBX_OK;
+		case MBX_TUNE_TILE_WIDTH: *value = 1 << ctx->tileShift; return MBX_OK;
 		default: return fail(ctx, MBX_ERR_INVALID, "unknown tuning key");
 	}
 }
@@ -794,11 +822,14 @@ int mbx_autotune(mbx_ctx *ctx, const mbx_outputs *out, int reps, int *best_waves
 	const int savedMode = ctx->kernelMode;
 	ctx->kernelMode = 0;
 	const int perCU[] = {4, 8, 12, 16, 20, 24, 32};
+	const int shifts[] = {3, 4, 2, 5}; // tile widths 8, 16, 4, 32 (64 pixels each)
 	double bestT = 1e30;
-	int bestW = default_waves(ctx);
+	int bestW = default_waves(ctx), bestShift = ctx->tileShift;
+	for (int sh : shifts)
 	for (int k : perCU)
 	{
 		ctx->waves = ctx->numCUs * k;
+		ctx->tileShift = sh;
 		double total = 0.0;
 		for (int r = 0; r < reps + 1; r++)
 		{
@@ -822,9 +853,11 @@ int mbx_autotune(mbx_ctx *ctx, const mbx_outputs *out, int reps, int *best_waves
 		{
 			bestT = t;
 			bestW = ctx->waves;
+			bestShift = sh;
 		}
 	}
 	ctx->waves = bestW;
+	ctx->tileShift = bestShift;
 	ctx->kernelMode = savedMode;
 	if (best_waves) *best_waves = bestW;
 	if (best_ms) *best_ms = bestT;

@@ -19,10 +19,11 @@ struct Outputs
 	int32_t *steps;
 };
 
-// Plain per-frame launch: one 64-lane workgroup per 8x8 pixel tile.  counters[0] receives the
+// Plain per-frame launch: one 64-lane workgroup per tile of (1 << tileShift) x (64 >> tileShift)
+// pixels.  counters[0] receives the
 // DE evaluations, counters[1] the formula iterations (atomically added).
 hipError_t launch_render(int deKind, const FrameConst *frame, int width, int height,
-	const Outputs &out, unsigned long long *counters, hipStream_t stream);
+	const Outputs &out, unsigned long long *counters, int tileShift, hipStream_t stream);
 
 // Point DE query at a fixed detail size.
 hipError_t launch_distance(int deKind, const FrameConst *frame, const float *pts, int n,
@@ -36,7 +37,7 @@ namespace mbx
 // One frame of a batch: its tile range in the global tile numbering and its output planes.
 struct FrameJob
 {
-	int tileEnd; // exclusive end of this frame's tiles (8x8 pixels each)
+	int tileEnd; // exclusive end of this frame's tiles (64 pixels each)
 	int pixBase; // offset of this frame's first pixel in the batch
 	float *rgb;
 	float *depth;
@@ -52,7 +53,7 @@ struct BatchArgs
 	const FrameConst *frames;
 	const FrameJob *jobs;
 	int numFrames;
-	int pad;
+	int tileShift; // tiles are (1 << tileShift) x (64 >> tileShift) pixels, tileShift in 2..5
 	unsigned int *queue;           // [0] tile queue head
 	unsigned long long *counters;  // [0] DE evaluations, [1] formula iterations
 };

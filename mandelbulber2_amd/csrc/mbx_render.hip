@@ -540,17 +540,18 @@ __device__ __forceinline__ void write_pixel(const Outputs &out, size_t i, const 
 	if (out.steps) out.steps[i] = po.steps;
 }
 
-// Plain per-frame kernel: 64 lanes = one 8x8 tile, one launch per frame.
+// Plain per-frame kernel: 64 lanes = one tile of (1 << tileShift) x (64 >> tileShift) pixels
+// (8x8 by default), one workgroup per tile, one launch per frame.
 template <int KIND>
 __global__ __launch_bounds__(64, KIND >= kBoxAnalytic ? 2 : 1) void render_kernel(const FrameConst *__restrict__ frame,
-	Outputs out, unsigned long long *__restrict__ counters)
+	Outputs out, unsigned long long *__restrict__ counters, int tileShift)
 {
 	const FrameConst &f = *frame;
-	const int tilesX = (f.width + 7) >> 3;
+	const int tilesX = (f.width + (1 << tileShift) - 1) >> tileShift;
 	const int tile = blockIdx.x;
 	const int tx = tile % tilesX, ty = tile / tilesX;
-	const int xs = tx * 8 + (threadIdx.x & 7);
-	const int ys = ty * 8 + (threadIdx.x >> 3);
+	const int xs = (tx << tileShift) + (threadIdx.x & ((1 << tileShift) - 1));
+	const int ys = ty * (64 >> tileShift) + (threadIdx.x >> tileShift);
 	Work w = make_work(f);
 	if (xs < f.width && ys < f.height)
 	{
@@ -565,7 +566,7 @@ __global__ __launch_bounds__(64, KIND >= kBoxAnalytic ? 2 : 1) void render_kerne
 	}
 }
 
-// Persistent work-stealing kernel: a fixed grid of waves drains one queue of 8x8 tiles that
+// Persistent work-stealing kernel: a fixed grid of waves drains one queue of 64-pixel tiles that
 // covers every frame of a batch (frames of different resolution, camera and formula
 // parameters).  The wave-uniform tile index keeps each tile's frame constants in scalar
 // registers.  One launch replaces the reference's one-job-per-frame animation loop, and the
@@ -577,7 +578,7 @@ __global__ __launch_bounds__(64, KIND >= kBoxAnalytic ? 2 : 1) void render_kerne
 template <int KIND>
 __global__ __launch_bounds__(64, KIND >= kBoxAnalytic ? 2 : 1) void persistent_kernel(const FrameConst *__restrict__ frames,
 	const FrameJob *__restrict__ jobs, int numFrames, unsigned int *__restrict__ queue,
-	unsigned long long *__restrict__ counters)
+	unsigned long long *__restrict__ counters, int tileShift)
 {
 	const int lane = threadIdx.x;
 	const unsigned int totalTiles = (unsigned int)jobs[numFrames - 1].tileEnd;
@@ -595,9 +596,9 @@ __global__ __launch_bounds__(64, KIND >= kBoxAnalytic ? 2 : 1) void persistent_k
 		load_orbit_constants(w, f);
 		const unsigned int tileBegin = fr == 0 ? 0u : (unsigned int)jobs[fr - 1].tileEnd;
 		const int lt = (int)(t - tileBegin);
-		const int tilesX = (f.width + 7) >> 3;
-		const int xs = (lt % tilesX) * 8 + (lane & 7);
-		const int ys = (lt / tilesX) * 8 + (lane >> 3);
+		const int tilesX = (f.width + (1 << tileShift) - 1) >> tileShift;
+		const int xs = ((lt % tilesX) << tileShift) + (lane & ((1 << tileShift) - 1));
+		const int ys = (lt / tilesX) * (64 >> tileShift) + (lane >> tileShift);
 		if (xs < f.width && ys < f.height)
 		{
 			PixelOut po;
@@ -639,10 +640,12 @@ __global__ __launch_bounds__(64) void distance_kernel(const FrameConst *__restri
 
 template <int KIND>
 static hipError_t launch_render_t(const FrameConst *frame, int width, int height, const Outputs &out,
-	unsigned long long *counters, hipStream_t stream)
+	unsigned long long *counters, int tileShift, hipStream_t stream)
 {
-	const int tiles = ((width + 7) / 8) * ((height + 7) / 8);
-	hipLaunchKernelGGL(render_kernel<KIND>, dim3(tiles), dim3(64), 0, stream, frame, out, counters);
+	const int tw = 1 << tileShift, th = 64 >> tileShift;
+	const int tiles = ((width + tw - 1) / tw) * ((height + th - 1) / th);
+	hipLaunchKernelGGL(render_kernel<KIND>, dim3(tiles), dim3(64), 0, stream, frame, out, counters,
+		tileShift);
 	return hipGetLastError();
 }
 
@@ -656,17 +659,17 @@ static hipError_t launch_distance_t(const FrameConst *frame, const float *pts, i
 }
 
 hipError_t launch_render(int deKind, const FrameConst *frame, int width, int height,
-	const Outputs &out, unsigned long long *counters, hipStream_t stream)
+	const Outputs &out, unsigned long long *counters, int tileShift, hipStream_t stream)
 {
 	switch (deKind)
 	{
-		case kBulb8: return launch_render_t<kBulb8>(frame, width, height, out, counters, stream);
-		case kBulbTrig: return launch_render_t<kBulbTrig>(frame, width, height, out, counters, stream);
+		case kBulb8: return launch_render_t<kBulb8>(frame, width, height, out, counters, tileShift, stream);
+		case kBulbTrig: return launch_render_t<kBulbTrig>(frame, width, height, out, counters, tileShift, stream);
 		case kBoxAnalytic:
-			return launch_render_t<kBoxAnalytic>(frame, width, height, out, counters, stream);
-		case kBoxDelta: return launch_render_t<kBoxDelta>(frame, width, height, out, counters, stream);
+			return launch_render_t<kBoxAnalytic>(frame, width, height, out, counters, tileShift, stream);
+		case kBoxDelta: return launch_render_t<kBoxDelta>(frame, width, height, out, counters, tileShift, stream);
 		case kIfsAnalytic:
-			return launch_render_t<kIfsAnalytic>(frame, width, height, out, counters, stream);
+			return launch_render_t<kIfsAnalytic>(frame, width, height, out, counters, tileShift, stream);
 		default: return hipErrorInvalidValue;
 	}
 }
@@ -681,7 +684,7 @@ template <int KIND>
 static hipError_t launch_batch_t(const BatchArgs &a, int numWaves, hipStream_t stream)
 {
 	hipLaunchKernelGGL(persistent_kernel<KIND>, dim3(numWaves), dim3(64), 0, stream, a.frames, a.jobs,
-		a.numFrames, a.queue, a.counters);
+		a.numFrames, a.queue, a.counters, a.tileShift);
 	return hipGetLastError();
 }
 

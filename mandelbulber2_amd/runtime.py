@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libmbx.so")
 # every entry point include/mbx.h declares
 EXPORTS = ["mbx_version", "mbx_sizeof_params", "mbx_ctx_create", "mbx_ctx_destroy",
            "mbx_last_error", "mbx_scene_upload", "mbx_scene_info", "mbx_render", "mbx_distance",
-           "mbx_get_stats", "mbx_render_batch", "mbx_tune", "mbx_autotune"]
+           "mbx_get_stats", "mbx_render_batch", "mbx_tune", "mbx_get_tune", "mbx_autotune"]
 
 _lib = None
 
@@ -48,6 +48,7 @@ def load():
     L.mbx_get_stats.argtypes = [C.c_void_p, P(abi.MbxStats)]
     L.mbx_render_batch.argtypes = [C.c_void_p, P(abi.MbxParams), P(abi.MbxOutputs), C.c_int]
     L.mbx_tune.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.mbx_get_tune.argtypes = [C.c_void_p, C.c_int, P(C.c_int)]
     L.mbx_autotune.argtypes = [C.c_void_p, P(abi.MbxOutputs), C.c_int, P(C.c_int), P(C.c_double)]
     _lib = L
     return L
@@ -57,6 +58,7 @@ MBX_TUNE_KERNEL = 0
 MBX_TUNE_WAVES = 1
 MBX_TUNE_GRAPH = 2
 MBX_TUNE_DELTA_DE = 3
+MBX_TUNE_TILE_WIDTH = 4
 KERNEL_PERSISTENT = 0
 KERNEL_PLAIN = 1
 
@@ -144,6 +146,11 @@ class Context:
 
     def tune(self, key, value):
         self._check(self.L.mbx_tune(self.h, key, value), "mbx_tune")
+
+    def get_tune(self, key):
+        v = C.c_int()
+        self._check(self.L.mbx_get_tune(self.h, key, C.byref(v)), "mbx_get_tune")
+        return v.value
 
     def autotune(self, bufs, reps=3):
         o = self.outputs_struct(bufs)

@@ -235,13 +235,42 @@ def test_unsupported_settings_are_rejected(rt):
     ctx.close()
 
 
-def test_autotune_picks_a_wave_count(rt):
+@pytest.mark.parametrize("kernel", [PERSISTENT, PLAIN])
+def test_tile_shapes_give_identical_pixels(rt, kernel):
+    """Tiles of 4x16, 16x4 and 32x2 pixels (one wave each) against the default 8x8 on a frame
+    whose size is a multiple of none of them: every plane is bit-identical."""
+    import torch
+    p = scene.mandelbulb8(150, 101)
+    ctx = rt.Context(0)
+    ctx.tune(rt.MBX_TUNE_KERNEL, kernel)
+    ctx.upload(p)
+    ref = ctx.alloc_outputs()
+    ctx.render(ref)
+    ref_st = ctx.stats()
+    for width in (4, 16, 32):
+        ctx.tune(rt.MBX_TUNE_TILE_WIDTH, width)
+        assert ctx.get_tune(rt.MBX_TUNE_TILE_WIDTH) == width
+        out = ctx.alloc_outputs()
+        ctx.render(out)
+        st = ctx.stats()
+        torch.cuda.synchronize()
+        for key in ("rgb", "depth", "iters", "mask", "normal", "steps"):
+            assert torch.equal(out[key], ref[key]), (width, key)
+        assert st["de_evals"] == ref_st["de_evals"]
+    with pytest.raises(rt.MbxError, match="tile width"):
+        ctx.tune(rt.MBX_TUNE_TILE_WIDTH, 12)
+    ctx.close()
+
+
+def test_autotune_picks_a_launch_shape(rt):
     p = scene.mandelbulb8(256, 192)
     ctx = rt.Context(0)
     ctx.upload(p)
     bufs = ctx.alloc_outputs()
     waves, ms = ctx.autotune(bufs, reps=1)
     assert waves > 0 and ms > 0
+    assert ctx.get_tune(rt.MBX_TUNE_WAVES) == waves
+    assert ctx.get_tune(rt.MBX_TUNE_TILE_WIDTH) in (4, 8, 16, 32)
     ctx.render(bufs)
     assert ctx.stats()["rays"] == 256 * 192
     ctx.close()
