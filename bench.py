@@ -7,12 +7,14 @@ VALU roofline of that kernel (algorithmic flops per launch over the launch's dev
 events on the launch stream), the plain one-tile-per-block kernel timed on the same frame, and
 the CPU oracle timed on this host.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload mandelbulb8|mandelbox|animation64]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+                    [--workload mandelbulb8|mandelbox|animation64|deepzoom]
 
 `--workload mandelbox` times the Mandelbox scene instead (scale 2, delta DE by the reference's
 finite differences in FP64, hard shadow, 16-ray AO, seeded camera; 1920x1080).
 `--workload animation64` times a 64-frame batch (power 2 -> 12, mixed resolutions, 4 spp with
 GI depth 2) in one persistent launch against one plain launch per frame.
+`--workload deepzoom` times the 1024x1024 deep-zoom Mandelbox (camera 1e-9 from the surface).
 """
 import argparse
 import json
@@ -29,7 +31,7 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X published FP64 peak
 # Algorithmic flop counts (fixed, independent of the instruction mix): per DE evaluation
 # (ray point, threshold, DE finish, step) and per formula iteration.
 F_DE = 25
-F_IT = {"mandelbulb8": 48, "mandelbox": 29, "animation64": 32}
+F_IT = {"mandelbulb8": 48, "mandelbox": 29, "animation64": 32, "deepzoom": 29}
 
 WORKLOADS = {
     "mandelbulb8": {
@@ -48,6 +50,15 @@ WORKLOADS = {
                    "camera": "seed 3, random walk on the radius 2.5-4 shell",
                    "resolutions": "per frame from 1920x1080 / 1280x720 / 640x480 (seeded)",
                    "spp": 4, "shading": "Monte Carlo GI depth 2, Lambert, 1 directional light"},
+    },
+    "deepzoom": {
+        "metric": "Mrays/s deep-zoom Mandelbox (camera 1e-9 from the surface, DE threshold 1e-11), "
+                  "1024x1024, 1 GPU; % FP64 peak",
+        "kernel": "persistent_kernel<kBoxAnalytic>", "dtype": "f64", "peak": FP64_PEAK_TFLOPS,
+        "config": {"formula": "mandelbox scale 2, analytic linear DE, maxiter 50",
+                   "camera": "seed 5: 1e-9 back from an FP64 surface-search point",
+                   "threshold": "constant 1e-11", "spp": 1,
+                   "shading": "6-tap normals + Lambert, 1 directional light"},
     },
     "mandelbox": {
         "metric": "Mrays/s DE raymarch with hard shadow + 16-ray AO, 1080p Mandelbox scale 2, "
@@ -103,6 +114,8 @@ def make_scene(name, w, h):
         return scene.mandelbulb8(w, h)
     if name == "animation64":
         return scene.animation64(seed=3)
+    if name == "deepzoom":
+        return scene.deep_zoom_mandelbox(1024, 1024, seed=5)
     return scene.mandelbox_shadow_ao(w, h, seed=1, scale=2.0)
 
 

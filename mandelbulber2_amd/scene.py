@@ -262,6 +262,77 @@ def animation64(seed=3, frames=64, downscale=1):
     return out
 
 
+def _mandelbox_de(p, scale=2.0, L=1.0, V=2.0, rmin=0.5, rfix=1.0, bailout=100.0, maxiter=50):
+    """Analytic linear DE r / DE of the Mandelbox orbit of p (c = p), FP64: the same formula and
+    orbit as cFractalMandelbox::FormulaCode (fractal_mandelbox.cpp:31-107) with Compute's
+    bailout (compute_fractal.cpp:299-321); used here only to place the deep-zoom camera."""
+    mR2, fR2 = rmin * rmin, rfix * rfix
+    factor1 = fR2 / mR2
+    x, y, z = p
+    DE = 1.0
+    r = math.sqrt(x * x + y * y + z * z)
+    for _ in range(maxiter):
+        if abs(x) > L:
+            x = math.copysign(V, x) - x
+        if abs(y) > L:
+            y = math.copysign(V, y) - y
+        if abs(z) > L:
+            z = math.copysign(V, z) - z
+        r2 = x * x + y * y + z * z
+        k = factor1 if r2 < mR2 else (fR2 / r2 if r2 < fR2 else 1.0)
+        x, y, z = x * k * scale + p[0], y * k * scale + p[1], z * k * scale + p[2]
+        DE = DE * k * abs(scale) + 1.0
+        r = math.sqrt(x * x + y * y + z * z)
+        if r > bailout:
+            break
+    return r / abs(DE)
+
+
+def deep_zoom_mandelbox(width=1024, height=1024, seed=5, surface_eps=1e-11, standoff=1e-9):
+    """Deep zoom: Mandelbox scale 2 (analytic linear DE, 50 iterations).  From a seeded exterior
+    point at radius 8 the FP64 DE is sphere-traced toward the origin until the distance is
+    below surface_eps; the camera sits `standoff` back along that ray and looks at the surface
+    point.  Constant DE threshold 1e-11, 1024x1024."""
+    rs = SeededStream(seed)
+    d = [rs.normal() for _ in range(3)]
+    n = math.sqrt(sum(c * c for c in d))
+    start = [8.0 * c / n for c in d]
+    aim = [0.25 * rs.normal() for _ in range(3)]
+    ray = [a - s0 for a, s0 in zip(aim, start)]
+    n = math.sqrt(sum(c * c for c in ray))
+    ray = [c / n for c in ray]
+    t = 0.0
+    for _ in range(200000):
+        q = [s0 + c * t for s0, c in zip(start, ray)]
+        dist = _mandelbox_de(q)
+        if dist < surface_eps:
+            break
+        t += dist
+    else:
+        raise RuntimeError("deep-zoom surface search did not converge")
+    surface = [s0 + c * t for s0, c in zip(start, ray)]
+    p = default_params(width, height)
+    abi.set3(p.camera, [sf - c * standoff for sf, c in zip(surface, ray)])
+    abi.set3(p.target, surface)
+    abi.set3(p.topVector, (0.0, 0.0, 1.0))
+    p.fov = 53.0
+    p.constantDEThreshold = 1
+    p.DEThresh = surface_eps
+    p.light.castShadows = 0
+    p.randomSeed = seed
+    f = p.fractal
+    f.formula = abi.MBX_FORMULA_MANDELBOX
+    f.formulaMaxiter = 50
+    f.deltaDEMethod = 0  # the formula's default: analytic
+    f.deltaDEFunction = 0
+    f.mandelbox.scale = 2.0
+    f.mandelbox.foldingLimit = 1.0
+    f.mandelbox.foldingValue = 2.0
+    f.mandelbox.foldingSphericalMin = 0.5
+    f.mandelbox.foldingSphericalFixed = 1.0
+    return p
+
+
 def params_bytes(p):
     return bytes(memoryview(p))
 

@@ -190,6 +190,38 @@ def test_animation_batch_parity(rt, oracle_mod):
     ctx.close()
 
 
+def _histogram(values, edges):
+    """Counts of values in [0, e0], (e0, e1], ..., (e_last, inf)."""
+    counts = []
+    lo = -1.0
+    for e in edges:
+        counts.append(int(((values > lo) & (values <= e)).sum()))
+        lo = e
+    counts.append(int((values > lo).sum()))
+    return counts
+
+
+def test_deep_zoom_mandelbox_parity(rt, oracle_mod):
+    """Deep zoom: camera 1e-9 from a Mandelbox surface point found by a seeded FP64 surface
+    search, constant DE threshold 1e-11, 256x256.  Ray positions, scan and orbits run in FP64 on
+    the device, so against the all-FP64 oracle the depth (about 1e-9) agrees to 1e-6 relative;
+    the per-pixel depth and RGB error histograms are printed."""
+    p = scene.deep_zoom_mandelbox(256, 256, seed=5)
+    g = rt.render_to_host(p)
+    r = oracle_mod.render(p)
+    st = compare(g, r)
+    _report("deep zoom mandelbox 256x256", st)
+    both = (g["mask"] > 0) & (r["mask"] > 0)
+    rel = np.abs(g["depth"][both].astype(np.float64) - r["depth"][both]) / r["depth"][both]
+    rgb = np.abs(g["rgb"].astype(np.float64) - r["rgb"]).max(axis=-1).ravel()
+    edges = [0.0, 1e-9, 1e-7, 1e-5, 1e-3]
+    print("[deep zoom depth rel err histogram] edges %s counts %s" % (edges, _histogram(rel, edges)))
+    print("[deep zoom rgb abs err histogram] edges %s counts %s" % (edges, _histogram(rgb, edges)))
+    print("[deep zoom] depth median %.4g, hits %d of %d" % (np.median(r["depth"][r["mask"] > 0]),
+                                                           int(both.sum()), both.size), flush=True)
+    _check(st, depth_tol=1e-6)
+
+
 def test_graph_replays_match_direct_launch(rt):
     """The captured-graph launch, replayed several times with other device work in between,
     gives the direct launch's result every time."""
