@@ -8,13 +8,14 @@ events on the launch stream), the plain one-tile-per-block kernel timed on the s
 the CPU oracle timed on this host.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3]
-                    [--workload mandelbulb8|mandelbox|animation64|deepzoom]
+                    [--workload mandelbulb8|mandelbox|animation64|deepzoom|adjoint512]
 
 `--workload mandelbox` times the Mandelbox scene instead (scale 2, delta DE by the reference's
 finite differences in FP64, hard shadow, 16-ray AO, seeded camera; 1920x1080).
 `--workload animation64` times a 64-frame batch (power 2 -> 12, mixed resolutions, 4 spp with
 GI depth 2) in one persistent launch against one plain launch per frame.
 `--workload deepzoom` times the 1024x1024 deep-zoom Mandelbox (camera 1e-9 from the surface).
+`--workload adjoint512` times a differentiable render plus its adjoint (dL/dtheta) at 512x512.
 """
 import argparse
 import json
@@ -59,6 +60,15 @@ WORKLOADS = {
                    "camera": "seed 5: 1e-9 back from an FP64 surface-search point",
                    "threshold": "constant 1e-11", "spp": 1,
                    "shading": "6-tap normals + Lambert, 1 directional light"},
+    },
+    "adjoint512": {
+        "metric": "Mpixels/s of one differentiable render + adjoint (dL/dtheta, 10 parameters) "
+                  "at 512x512, 1 GPU",
+        "kernel": "render_diff_kernel + adjoint_kernel", "dtype": "f64", "peak": FP64_PEAK_TFLOPS,
+        "config": {"formula": "mandelbulb, seeded power near 8 (trigonometric form), node rotation",
+                   "camera": "seed 0, distance 2.6-3.4", "loss": "L2 against a target at "
+                   "theta * (1 +- U(0, 0.05))", "parameters": "power, rotation xyz, camera xyz, "
+                   "target xyz", "shading": "Lambert, analytic normals, FP64 Newton-refined hits"},
     },
     "mandelbox": {
         "metric": "Mrays/s DE raymarch with hard shadow + 16-ray AO, 1080p Mandelbox scale 2, "
@@ -186,6 +196,82 @@ def cpu_baseline_batch(frames, budget_s):
                       % (rays, threads, dt)}
 
 
+def cpu_baseline_gradient(p, budget_s):
+    """The same gradient by central differences of the FP64 oracle render (21 renders per
+    gradient, the only way the reference's CPU renderer gives one), on a row band of the frame
+    scaled down to about budget_s; Mpixels/s per gradient."""
+    from oracle import oracle
+    from mandelbulber2_amd import diff
+    threads = min(16, os.cpu_count() or 1)
+    theta = diff.theta_of(p)
+    rows = 8
+    y0 = p.imageHeight // 2 - rows // 2
+    t0 = time.time()
+    grads = 0
+    while True:
+        for k in range(-1, 10):
+            for sgn in ((0.0,) if k < 0 else (1e-5, -1e-5)):
+                th = theta.copy()
+                if k >= 0:
+                    th[k] += sgn
+                oracle.render_diff(diff.with_theta(p, th), threads=threads, rows=(y0, y0 + rows))
+        grads += 1
+        if time.time() - t0 >= budget_s:
+            break
+    dt = time.time() - t0
+    px = grads * rows * p.imageWidth
+    return {"value": px / dt / 1e6, "unit": "Mpixels/s", "cores": threads, "kind": "port",
+            "sample": "%d gradient(s) of the middle %d rows of the 512x512 frame by 21 FP64 oracle "
+                      "renders each (central differences), %d threads, %.1f s" % (grads, rows, threads, dt)}
+
+
+def bench_adjoint(args, torch, runtime, wl):
+    """One step = differentiable render at theta + loss gradient + adjoint, 512x512."""
+    from mandelbulber2_amd import diff, scene
+    import numpy as np
+    p = scene.inverse_rendering(512, 512, seed=0)
+    theta = diff.theta_of(p)
+    rs = np.random.default_rng(100)
+    theta_t = theta * (1.0 + rs.choice([-1.0, 1.0], size=theta.size) * rs.uniform(0.0, 0.05, size=theta.size))
+    r = diff.DiffRenderer(p)
+    tgt_bufs, _ = r.render(theta_t)
+    target = tgt_bufs["rgb"].clone()
+    stream = torch.cuda.current_stream(0)
+
+    def step():
+        bufs, ck = r.render(theta)
+        g = r.adjoint(theta, ck, (2.0 * (bufs["rgb"] - target)).contiguous())
+        return g
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    fwd_ms, adj_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        bufs, ck = r.render(theta)
+        fwd_ms.append(r.ctx.stats()["kernel_ms"])
+        g = r.adjoint(theta, ck, (2.0 * (bufs["rgb"] - target)).contiguous())
+        adj_ms.append(r.ctx.stats()["kernel_ms"])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    r.close()
+    n = p.imageWidth * p.imageHeight
+    ms_per_step = wall * 1e3 / args.steps
+    out = {
+        "metric": wl["metric"], "value": n / (ms_per_step / 1e3) / 1e6, "unit": "Mpixels/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": wl["dtype"],
+        "data": "synthetic",
+        "config": dict({"workload": "adjoint512", "width": 512, "height": 512}, **wl["config"]),
+        "roofline": None,
+        "kernels_ms": {"render_diff": statistics.median(fwd_ms), "adjoint": statistics.median(adj_ms)},
+        "gradient": [float(v) for v in g],
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_gradient(p, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
 def bench_batch(args, torch, runtime, wl):
     """One step = the whole 64-frame batch in one persistent launch (captured graph); the
     comparison renders the same frames one plain launch per frame."""
@@ -250,6 +336,8 @@ def main():
     wl = WORKLOADS[args.workload]
     if args.workload == "animation64":
         return bench_batch(args, torch, runtime, wl)
+    if args.workload == "adjoint512":
+        return bench_adjoint(args, torch, runtime, wl)
     p = make_scene(args.workload, args.width, args.height)
     stream = torch.cuda.current_stream(0)
     ctx = runtime.Context(0, stream)

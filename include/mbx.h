@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MBX_ABI_VERSION 1
+#define MBX_ABI_VERSION 2
 
 /* array sizes as in mandelbulber2/src/fractal.h:52-54 */
 #define MBX_IFS_VECTOR_COUNT 9
@@ -217,6 +217,11 @@ typedef struct mbx_params {
 	uint32_t randomSeed; /* per-frame seed mixed into the per-pixel streams */
 
 	mbx_fractal fractal;
+
+	/* rotation of the fractal node, degrees ("formula_rotation", the node's "rotation",
+	 * objects_tree.cpp:292-382): points enter the formula as p_local = R p with
+	 * R = Rz(z) Ry(y) Rx(x) (CRotationMatrix::SetRotation2, algebra.cpp:296-302) */
+	double formulaRotation[3];
 } mbx_params;
 
 /* Per-pixel output planes (device pointers; any may be NULL except where noted).
@@ -318,6 +323,43 @@ int mbx_autotune(mbx_ctx *ctx, const mbx_outputs *out, int reps, int *best_waves
 
 /* Statistics of the last render (synchronises the stream). */
 int mbx_get_stats(mbx_ctx *ctx, mbx_stats *stats);
+
+/* ---- differentiable render and adjoint (inverse rendering) --------------------------------
+ * No counterpart in the reference, which renders only forward; these make the uploaded scene's
+ * image a differentiable function of MBX_DIFF_NPARAMS parameters, in this order: */
+#define MBX_DIFF_NPARAMS 10
+#define MBX_DIFF_POWER 0      /* fractal.bulb.power */
+#define MBX_DIFF_ROTATION 1   /* formulaRotation x, y, z (degrees): indices 1, 2, 3 */
+#define MBX_DIFF_CAMERA 4     /* camera x, y, z: indices 4, 5, 6 */
+#define MBX_DIFF_TARGET 7     /* target x, y, z: indices 7, 8, 9 */
+
+/* Per-ray checkpoint written by mbx_render_diff and read by mbx_adjoint (device memory, one per
+ * pixel): the refined hit depth, so the adjoint never re-marches. */
+typedef struct mbx_ray_checkpoint {
+	double t;        /* depth along the unit view vector (refined) */
+	int32_t found;   /* the march hit the surface */
+	int32_t refined; /* the hit was refined to DE(p) = distThresh(p); 0 = not differentiable */
+	int32_t iters;   /* formula iterations at the refined point */
+	int32_t pad;
+} mbx_ray_checkpoint;
+
+/* Differentiable render of the uploaded scene (Mandelbulb, analytic logarithmic DE, c = p; one
+ * directional light at an absolute angle, Lambert; no shadows, AO or Monte Carlo; background
+ * gamma 1): the usual march finds the hit, which is then refined in FP64 by Newton iterations on
+ * DE(camera + t dir) = distThresh(t), shaded with the analytic normal (the DE gradient by
+ * forward-mode dual numbers) in FP64, and checkpointed.  Outputs as mbx_render; `checkpoints`
+ * (device, imageWidth*imageHeight entries) is required.  MBX_ERR_UNSUPPORTED names the scene
+ * setting that is not differentiable here. */
+int mbx_render_diff(mbx_ctx *ctx, const mbx_outputs *out, mbx_ray_checkpoint *checkpoints);
+
+/* Adjoint of mbx_render_diff: given dL/drgb (device, 3 floats per pixel) and the checkpoints of
+ * the last mbx_render_diff of the same scene, writes dL/dtheta (host, MBX_DIFF_NPARAMS doubles)
+ * and, if pixel_grad is not NULL, each pixel's contribution (device, MBX_DIFF_NPARAMS doubles
+ * per pixel).  Hits use the implicit derivative of the refined depth and the derivative of the
+ * analytic normal (second derivatives of the DE by hyper-dual numbers); misses the derivative of
+ * the background along the view vector.  Synchronises the stream. */
+int mbx_adjoint(mbx_ctx *ctx, const mbx_ray_checkpoint *checkpoints, const float *dL_drgb, double *grad,
+	double *pixel_grad);
 
 #ifdef __cplusplus
 }

@@ -158,7 +158,18 @@ class MbxParams(C.Structure):
         ("rngMode", C.c_int32),
         ("randomSeed", C.c_uint32),
         ("fractal", MbxFractal),
+        ("formulaRotation", C.c_double * 3),
     ]
+
+
+MBX_DIFF_NPARAMS = 10
+DIFF_PARAM_NAMES = ("power", "rotation_x", "rotation_y", "rotation_z", "camera_x", "camera_y",
+                    "camera_z", "target_x", "target_y", "target_z")
+
+
+class MbxRayCheckpoint(C.Structure):
+    _fields_ = [("t", C.c_double), ("found", C.c_int32), ("refined", C.c_int32),
+                ("iters", C.c_int32), ("pad", C.c_int32)]
 
 
 class MbxOutputs(C.Structure):

@@ -450,7 +450,7 @@ struct Work
 {
 	uint32_t de_evals;
 	uint32_t iterations;
-	int maxN, minN, addC, julia, analyticFunction;
+	int maxN, minN, addC, julia, analyticFunction, nodeRotated;
 	float bailout2;
 	float cx, cy, cz; // Julia constant, or the constant multiplier of c = p
 };
@@ -462,6 +462,7 @@ __device__ __forceinline__ void load_orbit_constants(Work &w, const FrameConst &
 	w.addC = f.addC;
 	w.julia = f.julia;
 	w.analyticFunction = f.analyticFunction;
+	w.nodeRotated = f.nodeRotated;
 	w.bailout2 = f.bailout * f.bailout;
 	w.cx = f.julia ? f.juliaC[0] : f.constMul[0];
 	w.cy = f.julia ? f.juliaC[1] : f.constMul[1];
@@ -570,6 +571,8 @@ struct Prec<false>
 	static __device__ __forceinline__ P along(P start, P dir, R scan) { return start + dir * scan; }
 	static __device__ __forceinline__ P from(V3 a) { return a; }
 	static __device__ __forceinline__ V3 tof(P a) { return a; }
+	// world -> local point of the rotated fractal node (calculate_distance.cpp:296)
+	static __device__ __forceinline__ P node_local(const FrameConst &f, P p) { return rot(f.nodeRot, p); }
 	static __device__ __forceinline__ R invWidth(const FrameConst &f) { return f.invWidth; }
 	static __device__ __forceinline__ R invHeight(const FrameConst &f) { return f.invHeight; }
 	static __device__ __forceinline__ R aspect(const FrameConst &f) { return f.aspectRatio; }
@@ -606,6 +609,15 @@ struct Prec<true>
 	}
 	static __device__ __forceinline__ P from(V3 a) { return dv3(a.x, a.y, a.z); }
 	static __device__ __forceinline__ V3 tof(P a) { return v3((float)a.x, (float)a.y, (float)a.z); }
+	// world -> local point of the rotated fractal node (calculate_distance.cpp:296; CMatrix44
+	// TransformPoint sums left to right)
+	static __device__ __forceinline__ P node_local(const FrameConst &f, P p)
+	{
+#pragma clang fp contract(off)
+		const double *m = f.nodeRotD;
+		return dv3(m[0] * p.x + m[1] * p.y + m[2] * p.z, m[3] * p.x + m[4] * p.y + m[5] * p.z,
+			m[6] * p.x + m[7] * p.y + m[8] * p.z);
+	}
 	static __device__ __forceinline__ R invWidth(const FrameConst &f) { return f.invWidthD; }
 	static __device__ __forceinline__ R invHeight(const FrameConst &f) { return f.invHeightD; }
 	static __device__ __forceinline__ R aspect(const FrameConst &f) { return f.aspectRatioD; }
@@ -630,6 +642,7 @@ __device__ __forceinline__ typename Prec<PosPrecision<KIND>::kFP64>::R calculate
 	const FrameConst &f, int &iters, Work &w)
 {
 	using R = typename Prec<PosPrecision<KIND>::kFP64>::R;
+	if (w.nodeRotated) p = Prec<PosPrecision<KIND>::kFP64>::node_local(f, p);
 	R d;
 	int itersMinN = 0;
 	if constexpr (KIND == kBoxDelta)

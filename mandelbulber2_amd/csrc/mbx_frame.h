@@ -121,7 +121,10 @@ struct FrameConst
 	float giRadianceLimit;
 
 	uint32_t randomSeed;
-	int32_t pad[2];
+	// rotation of the fractal node: DE points enter the formula as R p (world -> local)
+	int32_t nodeRotated;
+	float nodeRot[9];
+	double nodeRotD[9];
 };
 
 // Per-pixel seed of the Park-Miller stream (identical on host and device).

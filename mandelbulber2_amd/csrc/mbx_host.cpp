@@ -15,6 +15,7 @@
 
 #include "../../include/mbx.h"
 #include "mbx_frame.h"
+#include "mbx_diff.h"
 #include "mbx_launch.h"
 
 using mbx::FrameConst;
@@ -49,6 +50,12 @@ struct mbx_ctx
 	hipGraphExec_t graphExec = nullptr;
 	int graphKind = -1, graphFrames = -1, graphWaves = -1, graphTile = -1;
 	int tileShift = 3; // 8x8-pixel tiles
+	// differentiable render
+	mbx::DiffConst diff{};
+	std::string diffWhy = "no scene uploaded"; // empty when the scene is differentiable
+	mbx::DiffConst *d_diff = nullptr;
+	double *d_grad = nullptr;
+	bool diffUploaded = false;
 };
 
 namespace
@@ -134,6 +141,109 @@ struct Rot
 };
 
 double wrap_angle(double a) { return std::fmod(a + 3.0 * M_PI, 2.0 * M_PI) - M_PI; }
+
+// ---- forward-mode dual numbers for the camera and node-rotation Jacobians -------------------
+struct Dual1
+{
+	double v, d;
+};
+Dual1 operator+(Dual1 a, Dual1 b) { return Dual1{a.v + b.v, a.d + b.d}; }
+Dual1 operator-(Dual1 a, Dual1 b) { return Dual1{a.v - b.v, a.d - b.d}; }
+Dual1 operator-(Dual1 a) { return Dual1{-a.v, -a.d}; }
+Dual1 operator*(Dual1 a, Dual1 b) { return Dual1{a.v * b.v, a.v * b.d + a.d * b.v}; }
+Dual1 operator/(Dual1 a, Dual1 b) { return Dual1{a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
+Dual1 dsin(Dual1 a) { return Dual1{std::sin(a.v), std::cos(a.v) * a.d}; }
+Dual1 dcos(Dual1 a) { return Dual1{std::cos(a.v), -std::sin(a.v) * a.d}; }
+Dual1 dsqrt(Dual1 a)
+{
+	const double s = std::sqrt(a.v);
+	return Dual1{s, 0.5 * a.d / s};
+}
+Dual1 datan2(Dual1 y, Dual1 x)
+{
+	return Dual1{std::atan2(y.v, x.v), (x.v * y.d - y.v * x.d) / (x.v * x.v + y.v * y.v)};
+}
+Dual1 dwrap(Dual1 a) { return Dual1{wrap_angle(a.v), a.d}; }
+Dual1 dc(double v) { return Dual1{v, 0.0}; }
+
+struct DD3
+{
+	Dual1 x, y, z;
+};
+DD3 dd_sub(DD3 a, DD3 b) { return DD3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+Dual1 dd_dot(DD3 a, DD3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+DD3 dd_mul(DD3 a, Dual1 s) { return DD3{a.x * s, a.y * s, a.z * s}; }
+DD3 dd_add(DD3 a, DD3 b) { return DD3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+DD3 dd_unit(DD3 a) { return dd_mul(a, dc(1.0) / dsqrt(dd_dot(a, a))); }
+DD3 dd_cross(DD3 a, DD3 b)
+{
+	return DD3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+DD3 dd_rotate_about(DD3 v, DD3 axis, Dual1 angle)
+{
+	DD3 r = dd_mul(v, dcos(angle));
+	r = dd_add(r, dd_mul(dd_cross(axis, v), dsin(angle)));
+	r = dd_add(r, dd_mul(dd_mul(axis, dd_dot(axis, v)), dc(1.0) - dcos(angle)));
+	return r;
+}
+// Rot with dual entries; an axis rotation is skipped only when its angle and tangent are 0
+struct DRot
+{
+	Dual1 m[9] = {dc(1), dc(0), dc(0), dc(0), dc(1), dc(0), dc(0), dc(0), dc(1)};
+	void mulRight(const Dual1 b[9])
+	{
+		Dual1 c[9];
+		for (int r = 0; r < 3; r++)
+			for (int k = 0; k < 3; k++) c[3 * r + k] = m[3 * r] * b[k] + m[3 * r + 1] * b[3 + k] + m[3 * r + 2] * b[6 + k];
+		for (int i = 0; i < 9; i++) m[i] = c[i];
+	}
+	void axis(int ax, Dual1 a)
+	{
+		if (a.v == 0.0 && a.d == 0.0) return;
+		const Dual1 s = dsin(a), c = dcos(a), o = dc(1.0), z = dc(0.0);
+		if (ax == 0)
+		{
+			const Dual1 b[9] = {o, z, z, z, c, -s, z, s, c};
+			mulRight(b);
+		}
+		else if (ax == 1)
+		{
+			const Dual1 b[9] = {c, z, s, z, o, z, -s, z, c};
+			mulRight(b);
+		}
+		else
+		{
+			const Dual1 b[9] = {c, -s, z, s, c, z, z, z, o};
+			mulRight(b);
+		}
+	}
+};
+
+// The camera rotation of derive_frame (camera_frame + PrepareMainVectors) in dual numbers: the
+// tangents of the result are its derivatives along the tangents of camera and target.
+void camera_rotation_dual(const Dual1 cam[3], const Dual1 tgt[3], const double top[3], double sweetH,
+	double sweetV, Dual1 out[9])
+{
+	const DD3 c{cam[0], cam[1], cam[2]}, t{tgt[0], tgt[1], tgt[2]};
+	const DD3 f = dd_unit(dd_sub(t, c));
+	Dual1 yaw = datan2(f.y, f.x) - dc(0.5 * M_PI);
+	Dual1 pitch = datan2(f.z, dsqrt(f.x * f.x + f.y * f.y));
+	DD3 tv{dc(top[0]), dc(top[1]), dc(top[2])};
+	tv = dd_unit(tv);
+	tv = dd_rotate_about(tv, DD3{dc(0), dc(0), dc(1)}, -yaw);
+	tv = dd_rotate_about(tv, DD3{dc(1), dc(0), dc(0)}, -pitch);
+	Dual1 roll = dc(0.5 * M_PI) - datan2(tv.z, tv.x);
+	yaw = dwrap(yaw);
+	pitch = dwrap(pitch);
+	roll = dwrap(roll);
+	DRot m;
+	m.axis(2, yaw);
+	m.axis(0, pitch);
+	m.axis(1, roll);
+	m.axis(2, dc(-(sweetH / 180.0 * M_PI)));
+	m.axis(0, dc(sweetV / 180.0 * M_PI));
+	for (int i = 0; i < 9; i++) out[i] = m.m[i];
+}
 
 // Camera frame from (camera, target, top): yaw/pitch/roll as cCameraTarget::SetCameraTargetTop
 // (camera_target.cpp:86-126) and the forward/top/right vectors.
@@ -472,6 +582,19 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 	if (p->rngMode != MBX_RNG_PER_PIXEL)
 		return fail(ctx, MBX_ERR_UNSUPPORTED, "the device uses per-pixel random streams only");
 	fc.randomSeed = p->randomSeed;
+
+	// fractal node rotation (objects_tree.cpp:292-382): local-to-world is R^T with
+	// R = SetRotation2(rotation * pi/180), so world-to-local (calculate_distance.cpp:296) is R
+	{
+		const double *fr3 = p->formulaRotation;
+		fc.nodeRotated = (fr3[0] != 0.0 || fr3[1] != 0.0 || fr3[2] != 0.0) ? 1 : 0;
+		Rot nr;
+		nr.z(fr3[2] * (M_PI / 180.0));
+		nr.y(fr3[1] * (M_PI / 180.0));
+		nr.x(fr3[0] * (M_PI / 180.0));
+		setRot(fc.nodeRot, nr);
+		for (int k = 0; k < 9; k++) fc.nodeRotD[k] = nr.m[k];
+	}
 	return MBX_OK;
 }
 
@@ -635,6 +758,85 @@ int render_frames(mbx_ctx *ctx, const std::vector<FrameConst> &frames, const mbx
 	return MBX_OK;
 }
 
+// Constants of the differentiable render, or the reason the scene is not differentiable.
+std::string derive_diff(const mbx_params *p, const FrameConst &fc, const double lightDir[3], mbx::DiffConst &dc_)
+{
+	const mbx_fractal &fr = p->fractal;
+	if (fr.formula != MBX_FORMULA_MANDELBULB || (fc.deKind != mbx::kBulb8 && fc.deKind != mbx::kBulbTrig) ||
+		fc.analyticFunction != MBX_ANALYTIC_LOGARITHMIC)
+		return "the differentiable render covers the Mandelbulb with the analytic logarithmic DE";
+	if (!fc.addC || fc.julia) return "the differentiable render needs c = p (no Julia mode)";
+	if (p->light.castShadows || p->ambientOcclusionEnabled || p->DOFMonteCarlo)
+		return "the differentiable render has no shadows, AO or Monte Carlo sampling";
+	if (p->backgroundGamma != 1.0) return "the differentiable render needs background gamma 1";
+	if (p->light.enabled && p->light.relativePosition)
+		return "the differentiable render needs the light at an absolute angle (relativePosition 0)";
+	mbx::DiffConst &c = dc_;
+	std::memset(&c, 0, sizeof(c));
+	c.power = fr.bulb.power;
+	c.alpha = fr.bulb.alphaAngleOffset * M_PI / 180.0;
+	c.beta = fr.bulb.betaAngleOffset * M_PI / 180.0;
+	c.bailout = fc.bailoutD;
+	for (int i = 0; i < 3; i++) c.cm[i] = fr.constantFactor[i];
+	c.maxN = fc.formulaMaxiter;
+	c.constThr = fc.constantDEThreshold;
+	c.threshK = fc.threshKD;
+	c.DEThresh = fc.DEThreshD;
+	// node rotation and its derivative per degree
+	for (int k = 0; k < 3; k++)
+	{
+		Dual1 a[3];
+		for (int j = 0; j < 3; j++) a[j] = Dual1{p->formulaRotation[j] * (M_PI / 180.0), j == k ? M_PI / 180.0 : 0.0};
+		DRot r;
+		r.axis(2, a[2]);
+		r.axis(1, a[1]);
+		r.axis(0, a[0]);
+		for (int i = 0; i < 9; i++) c.dR[k][i] = r.m[i].d;
+	}
+	for (int i = 0; i < 9; i++) c.R[i] = fc.nodeRotD[i];
+	// camera rotation and its derivatives per camera / target component
+	for (int k = 0; k < 6; k++)
+	{
+		Dual1 cam[3], tgt[3], m[9];
+		for (int j = 0; j < 3; j++)
+		{
+			cam[j] = Dual1{p->camera[j], k == j ? 1.0 : 0.0};
+			tgt[j] = Dual1{p->target[j], k == 3 + j ? 1.0 : 0.0};
+		}
+		camera_rotation_dual(cam, tgt, p->topVector, p->sweetSpotHAngle, p->sweetSpotVAngle, m);
+		for (int i = 0; i < 9; i++) c.dM[k][i] = m[i].d;
+	}
+	for (int i = 0; i < 9; i++) c.M[i] = fc.mRotD[i];
+	for (int i = 0; i < 3; i++) c.cam[i] = p->camera[i];
+	c.fov = fc.fovD;
+	c.aspect = fc.aspectRatioD;
+	c.invW = fc.invWidthD;
+	c.invH = fc.invHeightD;
+	c.width = fc.width;
+	c.height = fc.height;
+	// shading and background
+	c.lightEnabled = p->light.enabled;
+	c.bg3 = p->background3ColorsEnable;
+	for (int i = 0; i < 3; i++)
+	{
+		c.light[i] = lightDir[i];
+		c.lightColor[i] = p->light.color[i];
+		c.colour[i] = p->surfaceColor[i];
+		c.fill[i] = p->fillLightColor[i];
+		c.lum[i] = p->luminosity * p->luminosityColor[i];
+		c.bg1[i] = p->backgroundColor1[i];
+		c.bg2[i] = p->backgroundColor2[i];
+		c.bgc3[i] = p->backgroundColor3[i];
+	}
+	c.intensity = p->light.intensity;
+	c.shading = p->shading;
+	c.bgBrightness = p->backgroundBrightness;
+	c.lightSize = p->light.size;
+	c.sunPow = 6.0 * p->light.contourSharpness;
+	c.visibility = p->light.visibility;
+	return std::string();
+}
+
 } // namespace
 
 extern "C" {
@@ -684,6 +886,8 @@ void mbx_ctx_destroy(mbx_ctx *ctx)
 	if (ctx->d_frames) (void)hipFree(ctx->d_frames);
 	if (ctx->d_jobs) (void)hipFree(ctx->d_jobs);
 	if (ctx->d_queue) (void)hipFree(ctx->d_queue);
+	if (ctx->d_diff) (void)hipFree(ctx->d_diff);
+	if (ctx->d_grad) (void)hipFree(ctx->d_grad);
 	delete ctx;
 }
 
@@ -707,7 +911,75 @@ int mbx_scene_upload(mbx_ctx *ctx, const mbx_params *params)
 		return hip_fail(ctx, e, "upload frame constants");
 	ctx->frame = fc;
 	ctx->sceneReady = true;
+	ctx->diffWhy = derive_diff(params, fc, ctx->lightDir, ctx->diff);
+	ctx->diffUploaded = false;
 	ctx->err.clear();
+	return MBX_OK;
+}
+
+}
+
+namespace
+{
+// Validate the scene for the differentiable render and put its constants on the device.
+int ensure_diff(mbx_ctx *ctx)
+{
+	if (!ctx->sceneReady) return fail(ctx, MBX_ERR_INVALID, "no scene uploaded");
+	if (!ctx->diffWhy.empty()) return fail(ctx, MBX_ERR_UNSUPPORTED, ctx->diffWhy);
+	hipError_t e = hipSetDevice(ctx->device);
+	if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+	if (!ctx->d_diff && (e = hipMalloc((void **)&ctx->d_diff, sizeof(mbx::DiffConst))) != hipSuccess)
+		return hip_fail(ctx, e, "allocate differentiable-render constants");
+	if (!ctx->d_grad && (e = hipMalloc((void **)&ctx->d_grad, mbx::kDiffParams * sizeof(double))) != hipSuccess)
+		return hip_fail(ctx, e, "allocate gradient");
+	if (!ctx->diffUploaded)
+	{
+		if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "sync");
+		if ((e = hipMemcpy(ctx->d_diff, &ctx->diff, sizeof(mbx::DiffConst), hipMemcpyHostToDevice)) != hipSuccess)
+			return hip_fail(ctx, e, "upload differentiable-render constants");
+		ctx->diffUploaded = true;
+	}
+	return MBX_OK;
+}
+} // namespace
+
+extern "C" {
+
+int mbx_render_diff(mbx_ctx *ctx, const mbx_outputs *out, mbx_ray_checkpoint *checkpoints)
+{
+	if (!ctx || !out || !checkpoints) return fail(ctx, MBX_ERR_INVALID, "null argument");
+	int rc = ensure_diff(ctx);
+	if (rc != MBX_OK) return rc;
+	hipError_t e;
+	if ((e = hipEventRecord(ctx->ev0, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
+	if ((e = mbx::launch_clear(ctx->d_queue, ctx->d_counters, ctx->stream)) != hipSuccess)
+		return hip_fail(ctx, e, "clear");
+	e = mbx::launch_render_diff(ctx->frame.deKind, ctx->d_frame, ctx->d_diff, ctx->frame.width,
+		ctx->frame.height, out->rgb, out->depth, out->iters, out->mask, out->normal,
+		reinterpret_cast<mbx::RayCheckpoint *>(checkpoints), ctx->stream);
+	if (e != hipSuccess) return hip_fail(ctx, e, "differentiable render launch");
+	if ((e = hipEventRecord(ctx->ev1, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
+	ctx->launched = true;
+	ctx->rays = (uint64_t)ctx->frame.width * ctx->frame.height;
+	return MBX_OK;
+}
+
+int mbx_adjoint(mbx_ctx *ctx, const mbx_ray_checkpoint *checkpoints, const float *dL_drgb, double *grad,
+	double *pixel_grad)
+{
+	if (!ctx || !checkpoints || !dL_drgb || !grad) return fail(ctx, MBX_ERR_INVALID, "null argument");
+	int rc = ensure_diff(ctx);
+	if (rc != MBX_OK) return rc;
+	hipError_t e;
+	if ((e = hipEventRecord(ctx->ev0, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
+	e = mbx::launch_adjoint(ctx->d_frame, ctx->d_diff, ctx->frame.width, ctx->frame.height,
+		reinterpret_cast<const mbx::RayCheckpoint *>(checkpoints), dL_drgb, ctx->d_grad, pixel_grad, ctx->stream);
+	if (e != hipSuccess) return hip_fail(ctx, e, "adjoint launch");
+	if ((e = hipEventRecord(ctx->ev1, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
+	if ((e = hipMemcpyAsync(grad, ctx->d_grad, mbx::kDiffParams * sizeof(double), hipMemcpyDeviceToHost,
+			 ctx->stream)) != hipSuccess)
+		return hip_fail(ctx, e, "read gradient");
+	if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "adjoint");
 	return MBX_OK;
 }
 

@@ -1,0 +1,548 @@
+// mbx_pixel.h - per-pixel device path shared by the render and differentiable kernels:
+// RayMarching with its binary search, normals, shading (light, shadow, AO, background, GI),
+// the pixel sample loop and the output writes.
+//
+// One lane per primary ray.  A 64-lane wave covers a 64-pixel tile so that neighbouring rays,
+// which take similar march paths, share a wave.  Mandelbulb and IFS run in FP32, the Mandelbox
+// kinds march and iterate in FP64; the per-frame constants are wave-uniform scalar loads.
+//
+// Path (reference): cRenderWorker::doWork pixel body (render_worker.cpp:155-525) ->
+// RayRecursion with reflections off (render_worker.cpp:969-1819) -> RayMarching
+// (render_worker.cpp:708-951) -> CalculateDistance -> Compute -> FormulaCode; on a hit
+// CalculateNormals (shader_calculate_normals.cpp:40-130), ObjectShader (shader_object.cpp:40-229)
+// with LightShading / AuxShadow / AO, and the GI bounce loop (shader_global_illumination.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mbx_device.h"
+#include "mbx_launch.h"
+
+namespace mbx
+{
+
+
+template <int KIND>
+using PrecOf = Prec<PosPrecision<KIND>::kFP64>;
+
+template <int KIND>
+struct MarchOut
+{
+	typename PrecOf<KIND>::P point;
+	typename PrecOf<KIND>::R depth;
+	typename PrecOf<KIND>::R distThresh;
+	typename PrecOf<KIND>::R lastDist;
+	int found;
+	int iters;
+	int steps;
+};
+
+// CalcDistThresh (render_worker.cpp:653-679)
+template <int KIND>
+__device__ __forceinline__ typename PrecOf<KIND>::R dist_thresh(const FrameConst &f, typename PrecOf<KIND>::P p)
+{
+	using Q = PrecOf<KIND>;
+	if (f.constantDEThreshold) return Q::DEThresh(f);
+	return Q::thresh(f, p);
+}
+
+// CalcDelta (render_worker.cpp:684-690)
+template <int KIND>
+__device__ __forceinline__ typename PrecOf<KIND>::R calc_delta(const FrameConst &f, typename PrecOf<KIND>::P p)
+{
+	using Q = PrecOf<KIND>;
+	return Q::len(Q::cam(f) - p) * Q::deltaK(f);
+}
+
+// RayMarching (render_worker.cpp:708-951), exterior mode, binary search on.  In the FP64
+// position precision the arithmetic follows the reference's operation order without FMA
+// contraction, so that march decisions at the DE's discontinuities come out the same.
+template <int KIND>
+__device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start, typename PrecOf<KIND>::P dir,
+	typename PrecOf<KIND>::R maxScan, uint32_t &seed, MarchOut<KIND> &out, Work &w)
+{
+#pragma clang fp contract(off)
+	using Q = PrecOf<KIND>;
+	using R = typename Q::R;
+	using P = typename Q::P;
+	constexpr bool kExact = PosPrecision<KIND>::kFP64;
+	P point = Q::from(v3(1e30f, 1e30f, 1e30f));
+	bool found = false, dead = false;
+	R scan = 0, dist = 0, step = 0, thr = 0;
+	int counter = 0, iters = 0;
+	const R dirLength = Q::len(dir);
+	const R invDirLength = Q::rcp(dirLength);
+	const R searchLimit = kExact ? (R)(1.0 - 0.001 * f.detailLevelD) : (R)f.searchLimit;
+	// FP32: the ray starts at the camera, so |camera - point| = scan |dir|; one multiply, and
+	// closer to the FP64 value than the difference of two rounded positions
+	const bool constThr = f.constantDEThreshold != 0;
+	const R thrScale = kExact ? (R)0 : dirLength * (R)f.threshK;
+	const R constThrValue = Q::DEThresh(f);
+	const int maxSteps = f.maxRaymarchingSteps;
+	for (int i = 0; i < maxSteps; i++)
+	{
+		const P last = point;
+		counter++;
+		point = Q::along(start, dir, scan);
+		// a step below the position resolution leaves the point where it was (dead point).  In
+		// FP32 the point is always finite: every step is in [0.45 thr DEFactor, 3].
+		bool deadPoint = point.x == last.x && point.y == last.y && point.z == last.z;
+		if (kExact) deadPoint = deadPoint || !(isfinite(point.x) && isfinite(point.y) && isfinite(point.z));
+		if (deadPoint)
+		{
+			point = last;
+			found = true;
+			dead = true;
+			break;
+		}
+		if (kExact)
+			thr = dist_thresh<KIND>(f, point);
+		else
+			thr = constThr ? constThrValue : scan * thrScale;
+		dist = calculate_distance<KIND>(point, thr, f, iters, w);
+		if (dist < thr)
+		{
+			found = true;
+			break;
+		}
+		const int rnd = random_mod(seed, 1000);
+		if (kExact)
+		{
+			step = (dist - (R)0.5 * thr) * (R)f.DEFactorD * ((R)1 - (R)rnd / (R)10000);
+			step = step > (R)3 ? (R)3 : step;
+			scan += step / dirLength;
+		}
+		else
+		{
+			step = (dist - (R)0.5 * thr) * (R)f.DEFactor * ((R)1 - (R)rnd * (R)1e-4);
+			step = step > (R)3 ? (R)3 : step;
+			scan += step * invDirLength;
+		}
+		if (scan > maxScan) break;
+	}
+	point = Q::along(start, dir, scan);
+	if (found && !dead)
+	{
+		step *= (R)0.5;
+		for (int i = 0; i < 30; i++)
+		{
+			counter++;
+			if (dist < thr && dist > thr * searchLimit) break;
+			if (dist > thr)
+				scan += step;
+			else if (dist < thr * searchLimit)
+				scan -= step;
+			point = Q::along(start, dir, scan);
+			if (kExact)
+				thr = dist_thresh<KIND>(f, point);
+			else
+				thr = constThr ? constThrValue : scan * thrScale;
+			dist = calculate_distance<KIND>(point, thr, f, iters, w);
+			step *= (R)0.5;
+		}
+	}
+	out.point = point;
+	out.depth = scan;
+	out.distThresh = thr;
+	out.lastDist = dist;
+	out.found = found;
+	out.iters = iters;
+	out.steps = counter;
+}
+
+// CalculateNormals, fast mode: central differences of the DE (shader_calculate_normals.cpp:44-78)
+template <int KIND>
+__device__ V3 calculate_normals(const FrameConst &f, typename PrecOf<KIND>::P p, typename PrecOf<KIND>::R thr,
+	Work &w)
+{
+#pragma clang fp contract(off)
+	using Q = PrecOf<KIND>;
+	using R = typename Q::R;
+	using P = typename Q::P;
+	const R delta = thr * (R)f.smoothness;
+	int it;
+	P e = p;
+	e.x = p.x + delta;
+	const R sx1 = calculate_distance<KIND>(e, thr, f, it, w);
+	e.x = p.x - delta;
+	const R sx2 = calculate_distance<KIND>(e, thr, f, it, w);
+	e = p;
+	e.y = p.y + delta;
+	const R sy1 = calculate_distance<KIND>(e, thr, f, it, w);
+	e.y = p.y - delta;
+	const R sy2 = calculate_distance<KIND>(e, thr, f, it, w);
+	e = p;
+	e.z = p.z + delta;
+	const R sz1 = calculate_distance<KIND>(e, thr, f, it, w);
+	e.z = p.z - delta;
+	const R sz2 = calculate_distance<KIND>(e, thr, f, it, w);
+	const R nx = sx1 - sx2, ny = sy1 - sy2, nz = sz1 - sz2;
+	V3 n;
+	if (nx == (R)0 && ny == (R)0 && nz == (R)0)
+		n = v3(1.0f, 0.0f, 0.0f);
+	else
+	{
+		const R k = (R)1 / Q::len(Q::from(v3(0.0f, 0.0f, 0.0f)) + P{nx, ny, nz});
+		n = v3((float)(nx * k), (float)(ny * k), (float)(nz * k));
+	}
+	if (!finite3(n)) n = v3(1.0f, 0.0f, 0.0f);
+	return n;
+}
+
+// AuxShadow for the directional light (shader_aux_shadow.cpp:39-323): hard, or the soft
+// cone variant when soft_shadow_cone > 0
+template <int KIND>
+__device__ float aux_shadow(const FrameConst &f, typename PrecOf<KIND>::P p, typename PrecOf<KIND>::R thr,
+	typename PrecOf<KIND>::R distance, V3 lvf, Work &w)
+{
+	using Q = PrecOf<KIND>;
+	using R = typename Q::R;
+	using P = typename Q::P;
+	const P lv = Q::from(lvf);
+	R shadowTemp = 1;
+	R maxSoft = 0;
+	R step = thr;
+	int count = 0;
+	int it;
+	for (R i = thr; i < distance; i += step)
+	{
+		const P p2 = p + lv * i;
+		const R dist = calculate_distance<KIND>(p2, thr, f, it, w);
+		if (f.softShadows)
+		{
+			R angle = (dist - thr) / i;
+			if (angle < (R)0 || dist < thr) angle = 0;
+			R soft = (R)1 - angle / (R)f.softRange;
+			if (f.penetrating) soft *= (distance - i) / distance;
+			soft = soft < (R)0 ? (R)0 : soft;
+			maxSoft = soft > maxSoft ? soft : maxSoft;
+		}
+		shadowTemp = 1;
+		if (dist < thr)
+		{
+			if (f.penetrating)
+			{
+				shadowTemp = shadowTemp - (distance - i) / distance;
+				shadowTemp = shadowTemp < (R)0 ? (R)0 : shadowTemp;
+			}
+			else
+				shadowTemp = 0;
+			break;
+		}
+		step = (dist < (R)1e6 ? dist : (R)1e6) * (R)f.DEFactor;
+		step = step > (R)1e-15 ? step : (R)1e-15;
+		count++;
+		if (count > f.maxRaymarchingSteps) break;
+	}
+	return f.softShadows ? (float)((R)1 - maxSoft) : (float)shadowTemp;
+}
+
+// AmbientOcclusion, multiple rays (shader_ambient_occlusion.cpp:41-122)
+template <int KIND>
+__device__ V3 ambient_occlusion(const FrameConst &f, typename PrecOf<KIND>::P p, typename PrecOf<KIND>::R thr,
+	typename PrecOf<KIND>::R delta, uint32_t &seed, Work &w)
+{
+	using Q = PrecOf<KIND>;
+	using R = typename Q::R;
+	using P = typename Q::P;
+	V3 ao = v3(0.0f, 0.0f, 0.0f);
+	const R end_dist = delta * (R)f.height; // delta / resolution
+	int start = 0, end = f.aoCount - 1;
+	if (f.monteCarlo)
+	{
+		const int pick = random_mod(seed, f.aoCount - 1);
+		start = pick;
+		end = pick;
+	}
+	int it;
+	for (int i = start; i <= end; i++)
+	{
+		const P v = Q::from(v3(f.aoVectors[i][0], f.aoVectors[i][1], f.aoVectors[i][2]));
+		R shadowTemp = 1;
+		R dist;
+		for (R r = thr; r < end_dist; r += dist * (R)2)
+		{
+			dist = calculate_distance<KIND>(p + v * r, thr, f, it, w);
+			if (dist < thr)
+			{
+				shadowTemp = shadowTemp - (end_dist - r) / end_dist;
+				shadowTemp = shadowTemp < (R)0 ? (R)0 : shadowTemp;
+				break;
+			}
+		}
+		ao = ao + v3(f.aoColors[i][0], f.aoColors[i][1], f.aoColors[i][2]) * (float)shadowTemp;
+	}
+	if (!f.monteCarlo) ao = ao * frcp((float)f.aoCount);
+	return ao;
+}
+
+// FastAmbientOcclusion (shader_fast_ambient_occlusion.cpp:40-66)
+template <int KIND>
+__device__ float fast_ambient_occlusion(const FrameConst &f, typename PrecOf<KIND>::P p, V3 nf,
+	typename PrecOf<KIND>::R thr, Work &w)
+{
+	using Q = PrecOf<KIND>;
+	using R = typename Q::R;
+	using P = typename Q::P;
+	const P n = Q::from(nf);
+	float aoTemp = 0.0f;
+	R lastDist = (R)1e20;
+	const int q2 = f.aoQuality * f.aoQuality;
+	int it;
+	for (int i = 1; i < q2; i++)
+	{
+		const R scan = (R)(i * i) * thr;
+		R dist = calculate_distance<KIND>(p + n * scan, thr, f, it, w);
+		if (dist > lastDist * (R)2) dist = lastDist * (R)2;
+		lastDist = dist;
+		aoTemp += (float)((R)exp2f(-(float)i) * (scan - (R)f.aoFastTune * dist) / thr);
+	}
+	return fmaxf(1.0f - 0.2f * aoTemp, 0.0f);
+}
+
+// BackgroundShader: 3-colour gradient and the directional light's disk (shader_background.cpp)
+__device__ V3 background_shader(const FrameConst &f, V3 viewVector)
+{
+	const V3 vn = normalize(viewVector);
+	float c[3];
+	if (f.background3Colors)
+	{
+		const float grad = vn.z + 1.0f;
+#pragma unroll
+		for (int k = 0; k < 3; k++)
+		{
+			float v;
+			if (grad < 1.0f)
+				v = f.bg3[k] * (1.0f - grad) + f.bg2[k] * grad;
+			else
+				v = f.bg2[k] * (2.0f - grad) + f.bg1[k] * (grad - 1.0f);
+			c[k] = v * f.bgBrightness;
+		}
+	}
+	else
+	{
+#pragma unroll
+		for (int k = 0; k < 3; k++) c[k] = f.bg1[k] * f.bgBrightness;
+	}
+	V3 out;
+	if (f.bgInvGamma == 1.0f)
+		out = v3(c[0], c[1], c[2]);
+	else
+		out = v3(powf(c[0], f.bgInvGamma), powf(c[1], f.bgInvGamma), powf(c[2], f.bgInvGamma));
+	if (f.lightEnabled)
+	{
+		const V3 ld = v3(f.lightDir[0], f.lightDir[1], f.lightDir[2]);
+		const float a = -(dot(vn, ld) - 1.0f) * 360.0f * frcp(f.lightSize);
+		const float pw = 6.0f * f.lightContourSharpness;
+		const float ap = (pw == 6.0f) ? (a * a) * (a * a) * (a * a) : powf(a, pw);
+		const float intensity = frcp(1.0f + ap) * f.lightVisibility * f.lightIntensity;
+		out = out + v3(f.lightColor[0], f.lightColor[1], f.lightColor[2]) * intensity;
+	}
+	return out;
+}
+
+// ObjectShader with one directional light, Lambert, optional shadow and AO
+// (shader_object.cpp:40-229, shader_light_shading.cpp:40-122)
+template <int KIND>
+__device__ V3 object_shader(const FrameConst &f, typename PrecOf<KIND>::P p, V3 n, typename PrecOf<KIND>::R thr,
+	typename PrecOf<KIND>::R delta, uint32_t &seed, Work &w)
+{
+	using R = typename PrecOf<KIND>::R;
+	const V3 colour = v3(f.surfaceColor[0], f.surfaceColor[1], f.surfaceColor[2]);
+	V3 ambient = v3(0.0f, 0.0f, 0.0f);
+	if (f.aoEnabled)
+	{
+		if (f.aoMode == MBX_AO_FAST)
+		{
+			const float a = fast_ambient_occlusion<KIND>(f, p, n, thr, w);
+			ambient = v3(a, a, a);
+		}
+		else
+			ambient = ambient_occlusion<KIND>(f, p, thr, delta, seed, w);
+	}
+	V3 light = v3(0.0f, 0.0f, 0.0f);
+	if (f.lightEnabled)
+	{
+		const V3 lv = v3(f.lightDir[0], f.lightDir[1], f.lightDir[2]);
+		float shade = fmaxf(dot(n, lv), 0.0f);
+		shade = 1.0f - f.shading + shade * f.shading;
+		shade = fminf(shade * f.lightIntensity, 500.0f);
+		float shadow = 1.0f;
+		if (f.castShadows)
+		{
+			if (shade > 0.001f)
+			{
+				const R distance = f.penetrating ? delta * (R)f.height : (R)f.viewDistanceMaxLight;
+				shadow = aux_shadow<KIND>(f, p, thr, distance, lv, w);
+			}
+			else
+				shadow = 0.0f;
+		}
+		light = v3(f.lightColor[0], f.lightColor[1], f.lightColor[2]) * (shade * shadow);
+	}
+	V3 out;
+	out.x = (f.fillLight[0] + ambient.x * f.aoColor[0]) * colour.x + light.x * colour.x + f.luminosity[0];
+	out.y = (f.fillLight[1] + ambient.y * f.aoColor[1]) * colour.y + light.y * colour.y + f.luminosity[1];
+	out.z = (f.fillLight[2] + ambient.z * f.aoColor[2]) * colour.z + light.z * colour.z + f.luminosity[2];
+	return out;
+}
+
+// GlobalIlumination (shader_global_illumination.cpp:41-209)
+template <int KIND>
+__device__ V3 global_illumination(const FrameConst &f, typename PrecOf<KIND>::P p0, V3 n0,
+	typename PrecOf<KIND>::R thr0, uint32_t &seed, Work &w)
+{
+	using Q = PrecOf<KIND>;
+	using R = typename Q::R;
+	using P = typename Q::P;
+	V3 out = v3(0.0f, 0.0f, 0.0f);
+	P p = p0;
+	V3 n = n0;
+	R thrCur = thr0;
+	const V3 colour = v3(f.surfaceColor[0], f.surfaceColor[1], f.surfaceColor[2]);
+	V3 colourTemp = colour;
+	for (int depth = 0; depth < f.reflectionsMax; depth++)
+	{
+		const float rx = (float)(random_mod(seed, 20000) - 10000) * 1e-4f;
+		const float ry = (float)(random_mod(seed, 20000) - 10000) * 1e-4f;
+		const float rz = (float)(random_mod(seed, 20000) - 10000) * 1e-4f;
+		const V3 dirf = normalize(n + v3(rx * 1.2f, ry * 1.2f, rz * 1.2f));
+		const P dir = Q::from(dirf);
+		R dist = 0;
+		bool found = false;
+		int stepCount = 0;
+		int it;
+		P hit = p;
+		for (R scan = thrCur; scan < (R)f.viewDistanceMax; scan += dist * (R)f.DEFactor)
+		{
+			if (stepCount >= f.maxRaymarchingSteps) break;
+			const P q = p + dir * scan;
+			const R thr = dist_thresh<KIND>(f, q);
+			thrCur = thr;
+			dist = calculate_distance<KIND>(q, thr, f, it, w);
+			if (dist < thr)
+			{
+				if (scan < thr * (R)2) return out;
+				hit = q;
+				found = true;
+				break;
+			}
+			stepCount++;
+		}
+		V3 result;
+		bool finished = false;
+		if (found)
+		{
+			p = hit;
+			n = calculate_normals<KIND>(f, p, thrCur, w);
+			result = object_shader<KIND>(f, p, n, thrCur, thrCur, seed, w);
+			out = out + v3(result.x * colourTemp.x, result.y * colourTemp.y, result.z * colourTemp.z);
+		}
+		else
+		{
+			result = background_shader(f, dirf);
+			finished = true;
+		}
+		const float lim = f.giRadianceLimit;
+		result = v3(fminf(fmaxf(result.x, 0.0f), lim), fminf(fmaxf(result.y, 0.0f), lim),
+			fminf(fmaxf(result.z, 0.0f), lim));
+		out = out + v3(result.x * colourTemp.x, result.y * colourTemp.y, result.z * colourTemp.z);
+		colourTemp = colour;
+		if (finished) break;
+	}
+	return out;
+}
+
+struct PixelOut
+{
+	V3 rgb;
+	V3 normal;
+	float depth;
+	int found;
+	int iters;
+	int steps;
+};
+
+// One pixel: the sample loop of doWork and RayRecursion at depth 0
+template <int KIND>
+__device__ void render_pixel(const FrameConst &f, int xs, int ys, PixelOut &po, Work &w)
+{
+#pragma clang fp contract(off)
+	using Q = PrecOf<KIND>;
+	using R = typename Q::R;
+	using P = typename Q::P;
+	const uint32_t pix = (uint32_t)ys * (uint32_t)f.width + (uint32_t)xs;
+	uint32_t seed = pixel_seed(pix, f.randomSeed);
+	const R aspect = Q::aspect(f);
+	const R ox = (Q::invWidth(f) * (R)xs - (R)0.5) * aspect;
+	const R oy = (R)0.5 - Q::invHeight(f) * (R)ys;
+	const P cam = Q::cam(f);
+	const int repeats = f.monteCarlo ? f.samples : 1;
+	V3 acc = v3(0.0f, 0.0f, 0.0f);
+	po.steps = 0;
+	for (int rep = 0; rep < repeats; rep++)
+	{
+		R ix = ox, iy = oy;
+		if (f.monteCarlo)
+		{
+			ix = ox + ((R)random_mod(seed, 1000) * (R)1e-3 - (R)0.5) * Q::invWidth(f) * aspect;
+			iy = oy + ((R)random_mod(seed, 1000) * (R)1e-3 - (R)0.5) * Q::invHeight(f);
+		}
+		const P dir = Q::view(f, ix, iy);
+		MarchOut<KIND> m;
+		ray_marching<KIND>(f, cam, dir, (R)f.viewDistanceMax, seed, m, w);
+		V3 rgb;
+		V3 n = v3(0.0f, 0.0f, 0.0f);
+		if (m.found)
+		{
+			n = calculate_normals<KIND>(f, m.point, m.distThresh, w);
+			const R delta = calc_delta<KIND>(f, m.point);
+			rgb = object_shader<KIND>(f, m.point, n, m.distThresh, delta, seed, w);
+			if (f.monteCarlo && f.globalIllumination)
+				rgb = rgb + global_illumination<KIND>(f, m.point, n, m.distThresh, seed, w);
+			rgb = v3(fmaxf(rgb.x, 0.0f), fmaxf(rgb.y, 0.0f), fmaxf(rgb.z, 0.0f));
+		}
+		else
+			rgb = background_shader(f, Q::tof(dir));
+		acc = acc + rgb;
+		if (rep == 0)
+		{
+			po.found = m.found;
+			po.iters = m.iters;
+			po.normal = n;
+		}
+		po.depth = m.found ? (float)m.depth : 1e20f;
+		po.steps += m.steps;
+	}
+	po.rgb = repeats > 1 ? v3(acc.x / (float)repeats, acc.y / (float)repeats, acc.z / (float)repeats) : acc;
+}
+
+__device__ __forceinline__ void wave_add_u64(unsigned long long *dst, uint32_t v)
+{
+	unsigned long long s = v;
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+	if ((threadIdx.x & 63) == 0) atomicAdd(dst, s);
+}
+
+__device__ __forceinline__ void write_pixel(const Outputs &out, size_t i, const PixelOut &po)
+{
+	if (out.rgb)
+	{
+		out.rgb[3 * i + 0] = po.rgb.x;
+		out.rgb[3 * i + 1] = po.rgb.y;
+		out.rgb[3 * i + 2] = po.rgb.z;
+	}
+	if (out.depth) out.depth[i] = po.depth;
+	if (out.iters) out.iters[i] = po.iters;
+	if (out.mask) out.mask[i] = (uint8_t)po.found;
+	if (out.normal)
+	{
+		out.normal[3 * i + 0] = po.normal.x;
+		out.normal[3 * i + 1] = po.normal.y;
+		out.normal[3 * i + 2] = po.normal.z;
+	}
+	if (out.steps) out.steps[i] = po.steps;
+}
+
+} // namespace mbx

@@ -15,7 +15,8 @@ LIB_PATH = os.path.join(HERE, "libmbx.so")
 # every entry point include/mbx.h declares
 EXPORTS = ["mbx_version", "mbx_sizeof_params", "mbx_ctx_create", "mbx_ctx_destroy",
            "mbx_last_error", "mbx_scene_upload", "mbx_scene_info", "mbx_render", "mbx_distance",
-           "mbx_get_stats", "mbx_render_batch", "mbx_tune", "mbx_get_tune", "mbx_autotune"]
+           "mbx_get_stats", "mbx_render_batch", "mbx_tune", "mbx_get_tune", "mbx_autotune",
+           "mbx_render_diff", "mbx_adjoint"]
 
 _lib = None
 
@@ -49,6 +50,8 @@ def load():
     L.mbx_render_batch.argtypes = [C.c_void_p, P(abi.MbxParams), P(abi.MbxOutputs), C.c_int]
     L.mbx_tune.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.mbx_get_tune.argtypes = [C.c_void_p, C.c_int, P(C.c_int)]
+    L.mbx_render_diff.argtypes = [C.c_void_p, P(abi.MbxOutputs), C.c_void_p]
+    L.mbx_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.mbx_autotune.argtypes = [C.c_void_p, P(abi.MbxOutputs), C.c_int, P(C.c_int), P(C.c_double)]
     _lib = L
     return L
@@ -146,6 +149,28 @@ class Context:
 
     def tune(self, key, value):
         self._check(self.L.mbx_tune(self.h, key, value), "mbx_tune")
+
+    def alloc_checkpoints(self, params=None):
+        """Device buffer for one mbx_ray_checkpoint (24 bytes) per pixel."""
+        p = params or self.params
+        return self.torch.empty((p.imageHeight * p.imageWidth * 3,), dtype=self.torch.float64,
+                                device=self.torch.device("cuda", self.device))
+
+    def render_diff(self, bufs, checkpoints):
+        o = self.outputs_struct(bufs)
+        self._check(self.L.mbx_render_diff(self.h, C.byref(o), C.c_void_p(checkpoints.data_ptr())),
+                    "mbx_render_diff")
+
+    def adjoint(self, checkpoints, dL_drgb, pixel_grad=None):
+        """dL/dtheta (numpy, MBX_DIFF_NPARAMS) for dL_drgb (float32 device tensor, 3 per pixel)."""
+        import numpy as np
+        g = np.zeros(abi.MBX_DIFF_NPARAMS, np.float64)
+        assert dL_drgb.dtype == self.torch.float32 and dL_drgb.is_contiguous()
+        pg = C.c_void_p(pixel_grad.data_ptr()) if pixel_grad is not None else None
+        self._check(self.L.mbx_adjoint(self.h, C.c_void_p(checkpoints.data_ptr()),
+                                       C.c_void_p(dL_drgb.data_ptr()),
+                                       g.ctypes.data_as(C.c_void_p), pg), "mbx_adjoint")
+        return g
 
     def get_tune(self, key):
         v = C.c_int()

@@ -333,6 +333,26 @@ def deep_zoom_mandelbox(width=1024, height=1024, seed=5, surface_eps=1e-11, stan
     return p
 
 
+def inverse_rendering(width=512, height=512, seed=0):
+    """Differentiable-render scene: Mandelbulb (trigonometric form, analytic log DE) with a
+    seeded power near 8 and node rotation, camera at distance ~3 looking at the origin, one
+    light at an absolute angle, Lambert, no shadows/AO.  The ten differentiable parameters are
+    power, rotation x/y/z (degrees), camera x/y/z and target x/y/z."""
+    rs = SeededStream(1000 + seed)
+    p = mandelbulb8(width, height, seed=seed)
+    p.fractal.bulb.power = 8.0 + rs.uniform(-1.0, 1.0)
+    abi.set3(p.formulaRotation, [rs.uniform(-30.0, 30.0) for _ in range(3)])
+    d = [rs.normal() for _ in range(3)]
+    n = math.sqrt(sum(c * c for c in d))
+    radius = rs.uniform(2.6, 3.4)
+    abi.set3(p.camera, [c / n * radius for c in d])
+    abi.set3(p.target, [rs.uniform(-0.1, 0.1) for _ in range(3)])
+    abi.set3(p.topVector, (0.0, 0.0, 1.0))
+    p.light.relativePosition = 0
+    p.light.castShadows = 0
+    return p
+
+
 def params_bytes(p):
     return bytes(memoryview(p))
 

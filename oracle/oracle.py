@@ -42,6 +42,11 @@ def lib(fast=False):
                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.mbxo_normal.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_double, C.c_void_p]
     L.mbxo_background.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_void_p]
+    L.mbxo_render_diff.argtypes = [P(abi.MbxParams), C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_void_p]
+    L.mbxo_diff_de.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                               C.c_void_p, C.c_void_p]
     L.mbxo_random_int.argtypes = [P(C.c_uint32)]
     L.mbxo_random.argtypes = [P(C.c_uint32), C.c_int]
     L.mbxo_pixel_seed.argtypes = [C.c_uint32, C.c_uint32]
@@ -175,3 +180,44 @@ def background(p, direction):
     if rc != 0:
         raise RuntimeError("oracle background failed: %d" % rc)
     return rgb
+
+
+def render_diff(p, threads=None, rows=None):
+    """The FP64 differentiable render (refined hits, analytic normals, double shading); planes
+    in float64 so that central differences of a loss stay clean.  rows=(y0, y1) renders only
+    those rows of the frame (the others stay zero)."""
+    L = lib()
+    W, H = p.imageWidth, p.imageHeight
+    n = W * H
+    rgb = np.zeros((n, 3), np.float64)
+    depth = np.zeros(n, np.float64)
+    iters = np.zeros(n, np.int32)
+    mask = np.zeros(n, np.uint8)
+    normal = np.zeros((n, 3), np.float64)
+    refined = np.zeros(n, np.int32)
+    branch = np.zeros(n, np.int32)
+    if threads is None:
+        threads = os.cpu_count() or 1
+    y0, y1 = rows if rows else (0, H)
+    rc = L.mbxo_render_diff(C.byref(p), y0, y1, threads, _ptr(rgb), _ptr(depth), _ptr(iters),
+                            _ptr(mask), _ptr(normal), _ptr(refined), _ptr(branch))
+    if rc != 0:
+        raise RuntimeError("oracle render_diff failed: %d" % rc)
+    return {"rgb": rgb.reshape(H, W, 3), "depth": depth.reshape(H, W), "iters": iters.reshape(H, W),
+            "mask": mask.reshape(H, W), "normal": normal.reshape(H, W, 3),
+            "refined": refined.reshape(H, W), "branch": branch.reshape(H, W)}
+
+
+def diff_de(p, pts):
+    """DE of the differentiable render and its analytic gradient at world points."""
+    L = lib()
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    n = len(pts)
+    de = np.zeros(n, np.float64)
+    grad = np.zeros((n, 3), np.float64)
+    it = np.zeros(n, np.int32)
+    ok = np.zeros(n, np.int32)
+    rc = L.mbxo_diff_de(C.byref(p), _ptr(pts), n, _ptr(de), _ptr(grad), _ptr(it), _ptr(ok))
+    if rc != 0:
+        raise RuntimeError("oracle diff_de failed: %d" % rc)
+    return de, grad, it, ok

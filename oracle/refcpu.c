@@ -250,6 +250,9 @@ typedef struct
 	float lightColor[3];
 	/* AO */
 	int aoCount;
+	/* fractal node rotation: world -> local (objects_tree.cpp:292-382, calculate_distance.cpp:296) */
+	int nodeRotated;
+	rotm nodeRot;
 } scene_t;
 
 /* cCameraTarget::SetCameraTargetTop (mandelbulber2/src/camera_target.cpp:86-126) */
@@ -441,6 +444,14 @@ static int scene_init(scene_t *s, const mbx_params *p)
 	{
 		if (s->aoCount < 1 || s->aoCount > MBX_MAX_AO_VECTORS) return MBX_ERR_INVALID;
 	}
+
+	/* The node's local-to-world matrix is R^T (rotation part of objects_tree.cpp:295-309 with
+	 * R = SetRotation2(rotation deg * pi/180)); its inverse, the world-to-local matrix applied
+	 * to every DE point (calculate_distance.cpp:296), is R. */
+	s->nodeRotated = p->formulaRotation[0] != 0.0 || p->formulaRotation[1] != 0.0 ||
+		p->formulaRotation[2] != 0.0;
+	rot_set2(&s->nodeRot, V3(p->formulaRotation[0] * (M_PI / 180.0), p->formulaRotation[1] * (M_PI / 180.0),
+							 p->formulaRotation[2] * (M_PI / 180.0)));
 	return MBX_OK;
 }
 
@@ -709,6 +720,7 @@ static double calculate_distance(const scene_t *s, v3 point, double detailSize, 
 	const int maxiterMultiplier = (normalCalc && p->iterThreshMode) ? 5 : 1;
 	fout_t fo;
 	out->totalIters = 0;
+	if (s->nodeRotated) point = rot_apply3(&s->nodeRot, point);
 	if (s->DEType == MBX_DE_ANALYTIC)
 	{
 		compute(s, MODE_NORMAL, point, -1, maxiterMultiplier, &fo);
@@ -1643,5 +1655,252 @@ int mbxo_background(const mbx_params *p, const double *dir3, float *rgb3)
 	rgb3[0] = c.R;
 	rgb3[1] = c.G;
 	rgb3[2] = c.B;
+	return MBX_OK;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Differentiable render, FP64 (the function the adjoint differentiates).  Not a reference     */
+/* feature: the march above finds the hit, Newton iterations on DE(camera + t dir) = distThresh */
+/* refine it, the normal is the analytic DE gradient (3-tangent dual numbers through the        */
+/* Mandelbulb FormulaCode, fractal_mandelbulb.cpp:29-43, and the analytic logarithmic DE,      */
+/* compute_fractal.cpp:561-568), and Lambert / background shading run in double.  The tests take */
+/* central differences of this render's loss as the reference gradient.                         */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct { double v, d[3]; } dl3;
+
+static inline dl3 dl_c(double v) { dl3 r = {v, {0.0, 0.0, 0.0}}; return r; }
+static inline dl3 dl_add(dl3 a, dl3 b)
+{
+	dl3 r = {a.v + b.v, {a.d[0] + b.d[0], a.d[1] + b.d[1], a.d[2] + b.d[2]}};
+	return r;
+}
+static inline dl3 dl_addc(dl3 a, double c) { a.v += c; return a; }
+static inline dl3 dl_scale(dl3 a, double s)
+{
+	dl3 r = {a.v * s, {a.d[0] * s, a.d[1] * s, a.d[2] * s}};
+	return r;
+}
+static inline dl3 dl_mul(dl3 a, dl3 b)
+{
+	dl3 r;
+	r.v = a.v * b.v;
+	for (int k = 0; k < 3; k++) r.d[k] = a.v * b.d[k] + a.d[k] * b.v;
+	return r;
+}
+static inline dl3 dl_div(dl3 a, dl3 b)
+{
+	dl3 r;
+	r.v = a.v / b.v;
+	for (int k = 0; k < 3; k++) r.d[k] = (a.d[k] * b.v - a.v * b.d[k]) / (b.v * b.v);
+	return r;
+}
+static inline dl3 dl_fn(dl3 a, double f, double f1)
+{
+	dl3 r = {f, {f1 * a.d[0], f1 * a.d[1], f1 * a.d[2]}};
+	return r;
+}
+static inline dl3 dl_sqrt(dl3 a) { const double s = sqrt(a.v); return dl_fn(a, s, 0.5 / s); }
+static inline dl3 dl_log(dl3 a) { return dl_fn(a, log(a.v), 1.0 / a.v); }
+static inline dl3 dl_sin(dl3 a) { return dl_fn(a, sin(a.v), cos(a.v)); }
+static inline dl3 dl_cos(dl3 a) { return dl_fn(a, cos(a.v), -sin(a.v)); }
+static inline dl3 dl_asin(dl3 a) { return dl_fn(a, asin(a.v), 1.0 / sqrt(1.0 - a.v * a.v)); }
+static inline dl3 dl_powc(dl3 a, double p) { return dl_fn(a, pow(a.v, p), p * pow(a.v, p - 1.0)); }
+static inline dl3 dl_atan2(dl3 y, dl3 x)
+{
+	const double q = x.v * x.v + y.v * y.v;
+	dl3 r;
+	r.v = atan2(y.v, x.v);
+	for (int k = 0; k < 3; k++) r.d[k] = (x.v * y.d[k] - y.v * x.d[k]) / q;
+	return r;
+}
+
+/* analytic log DE of the Mandelbulb node at world point pw (tangents carried in pw) */
+static dl3 diff_de_o(const scene_t *s, const dl3 pw[3], int *iters, int *ok)
+{
+	const double *R = s->nodeRot.m;
+	dl3 p[3];
+	for (int i = 0; i < 3; i++)
+	{
+		if (s->nodeRotated)
+			p[i] = dl_add(dl_add(dl_scale(pw[0], R[3 * i]), dl_scale(pw[1], R[3 * i + 1])), dl_scale(pw[2], R[3 * i + 2]));
+		else
+			p[i] = pw[i];
+	}
+	dl3 z[3] = {p[0], p[1], p[2]};
+	dl3 DE = dl_c(1.0);
+	dl3 r = dl_sqrt(dl_add(dl_add(dl_mul(z[0], z[0]), dl_mul(z[1], z[1])), dl_mul(z[2], z[2])));
+	const double P = s->bulbPower;
+	int i;
+	*ok = 1;
+	for (i = 0; i < s->formulaMaxiter; i++)
+	{
+		if (!(r.v > 0.0)) { *ok = 0; break; }
+		const dl3 th0 = dl_addc(dl_asin(dl_div(z[2], r)), s->bulbBeta);
+		const dl3 ph0 = dl_addc(dl_atan2(z[1], z[0]), s->bulbAlpha);
+		dl3 rp = dl_powc(r, P - 1.0);
+		const dl3 th = dl_scale(th0, P);
+		const dl3 ph = dl_scale(ph0, P);
+		const dl3 cth = dl_cos(th);
+		DE = dl_addc(dl_scale(dl_mul(rp, DE), P), 1.0);
+		rp = dl_mul(rp, r);
+		z[0] = dl_add(dl_mul(dl_mul(cth, dl_cos(ph)), rp), dl_scale(p[0], s->constantMultiplier.x));
+		z[1] = dl_add(dl_mul(dl_mul(cth, dl_sin(ph)), rp), dl_scale(p[1], s->constantMultiplier.y));
+		z[2] = dl_add(dl_mul(dl_sin(th), rp), dl_scale(p[2], s->constantMultiplier.z));
+		r = dl_sqrt(dl_add(dl_add(dl_mul(z[0], z[0]), dl_mul(z[1], z[1])), dl_mul(z[2], z[2])));
+		if (!isfinite(r.v)) { *ok = 0; break; }
+		if (r.v > s->bailout) break;
+	}
+	*iters = i + 1;
+	if (!(DE.v > 0.0) || !(r.v > 1.0)) { *ok = 0; return dl_c(0.0); }
+	return dl_scale(dl_div(dl_mul(r, dl_log(r)), DE), 0.5);
+}
+
+static double diff_thresh_o(const scene_t *s, double t)
+{
+	return s->p->constantDEThreshold ? s->p->DEThresh : t * s->resolution * s->fov / s->p->detailLevel;
+}
+
+/* DE value and gradient at a world point */
+static double diff_grad_o(const scene_t *s, v3 q, double g[3], int *iters, int *ok)
+{
+	dl3 pw[3] = {{q.x, {1, 0, 0}}, {q.y, {0, 1, 0}}, {q.z, {0, 0, 1}}};
+	const dl3 de = diff_de_o(s, pw, iters, ok);
+	for (int k = 0; k < 3; k++) g[k] = de.d[k];
+	return de.v;
+}
+
+/* branch (optional): which side of each kink of the shading the pixel is on -- bit 0 n.l > 0,
+ * bit 1 shade clamped at 500, bit 2 (misses) below the horizon of the background gradient,
+ * bits 3-5 channel clamped at 0; a finite difference across a kink is no derivative */
+int mbxo_render_diff(const mbx_params *p, int y0, int y1, int threads, double *rgb, double *depth,
+	int32_t *iters, uint8_t *mask, double *normal, int32_t *refined, int32_t *branch)
+{
+	scene_t s;
+	int rc0 = scene_init(&s, p);
+	if (rc0) return rc0;
+	if (p->fractal.formula != MBX_FORMULA_MANDELBULB || s.DEType != MBX_DE_ANALYTIC ||
+		s.DEAnalyticFunction != MBX_ANALYTIC_LOGARITHMIC || !s.addCConstant || s.juliaEnabled)
+		return MBX_ERR_UNSUPPORTED;
+	const int W = p->imageWidth, H = p->imageHeight;
+	const double dthr = p->constantDEThreshold ? 0.0 : s.resolution * s.fov / p->detailLevel;
+	if (y0 < 0) y0 = 0;
+	if (y1 > H) y1 = H;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
+	for (int ys = y0; ys < y1; ys++)
+	{
+		for (int xs = 0; xs < W; xs++)
+		{
+			const size_t i = (size_t)ys * W + xs;
+			uint32_t seed = mbxo_pixel_seed((uint32_t)i, p->randomSeed);
+			ray_ctx rc = {&s, &seed, 0, 0};
+			const double ix = ((1.0 / W) * xs + (-0.5)) * s.aspectRatio;
+			const double iy = (-1.0 / H) * ys + 0.5;
+			const v3 d = v3_normalize(view_vector(&s, ix, iy));
+			march_t m;
+			ray_marching(&rc, s.camera, d, 0.0, p->viewDistanceMax, 1, &m);
+			double col[3], n[3] = {0, 0, 0}, t = m.depth;
+			int it = m.iters, ref = 0, br = 0;
+			if (m.found)
+			{
+				/* Newton on G(t) = DE(camera + t d) - distThresh(t) */
+				for (int k = 0; k < 24; k++)
+				{
+					double g[3];
+					int ok, nit;
+					const v3 q = v3_add(s.camera, v3_mul(d, t));
+					const double de = diff_grad_o(&s, q, g, &nit, &ok);
+					if (!ok) break;
+					const double thr = diff_thresh_o(&s, t);
+					const double Gt = g[0] * d.x + g[1] * d.y + g[2] * d.z - dthr;
+					if (!(Gt < 0.0)) break;
+					const double dt = -(de - thr) / Gt;
+					if (!(fabs(dt) <= 4.0 * thr)) break;
+					t += dt;
+					it = nit;
+					if (fabs(dt) <= 1e-15 * t) { ref = 1; break; }
+				}
+				if (!ref) t = m.depth;
+				double g[3];
+				int ok, nit;
+				diff_grad_o(&s, v3_add(s.camera, v3_mul(d, t)), g, &nit, &ok);
+				const double gl = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+				if (ok && gl > 0.0)
+					for (int k = 0; k < 3; k++) n[k] = g[k] / gl;
+				else
+				{
+					n[0] = 1.0;
+					ref = 0;
+				}
+				/* Lambert, as ObjectShader with shadows and AO off (shader_object.cpp:40-229) */
+				const double ndotl = n[0] * s.lightDirection.x + n[1] * s.lightDirection.y + n[2] * s.lightDirection.z;
+				double shade = 1.0 - p->shading + (ndotl > 0.0 ? ndotl : 0.0) * p->shading;
+				shade *= p->light.intensity;
+				br |= ndotl > 0.0 ? 1 : 0;
+				if (shade > 500.0)
+				{
+					shade = 500.0;
+					br |= 2;
+				}
+				for (int k = 0; k < 3; k++)
+				{
+					const double light = p->light.enabled ? p->light.color[k] * shade : 0.0;
+					const double v = p->fillLightColor[k] * p->surfaceColor[k] + light * p->surfaceColor[k] +
+						p->luminosity * p->luminosityColor[k];
+					col[k] = v > 0.0 ? v : 0.0;
+					br |= v > 0.0 ? 0 : 8 << k;
+				}
+			}
+			else
+			{
+				/* BackgroundShader in double: 3-colour gradient and the light's disk */
+				const double grad = d.z + 1.0;
+				br |= grad < 1.0 ? 4 : 0;
+				for (int k = 0; k < 3; k++)
+				{
+					double c;
+					if (p->background3ColorsEnable)
+						c = grad < 1.0 ? p->backgroundColor3[k] * (1.0 - grad) + p->backgroundColor2[k] * grad
+									   : p->backgroundColor2[k] * (2.0 - grad) + p->backgroundColor1[k] * (grad - 1.0);
+					else
+						c = p->backgroundColor1[k];
+					col[k] = c * p->backgroundBrightness;
+				}
+				if (p->light.enabled)
+				{
+					const double dl = d.x * s.lightDirection.x + d.y * s.lightDirection.y + d.z * s.lightDirection.z;
+					const double a = (1.0 - dl) * (360.0 / p->light.size);
+					const double inten = 1.0 / (pow(a, 6.0 * p->light.contourSharpness) + 1.0) * (p->light.visibility * p->light.intensity);
+					for (int k = 0; k < 3; k++) col[k] += p->light.color[k] * inten;
+				}
+			}
+			if (rgb) for (int k = 0; k < 3; k++) rgb[3 * i + k] = col[k];
+			if (depth) depth[i] = m.found ? t : 1e20;
+			if (iters) iters[i] = it;
+			if (mask) mask[i] = (uint8_t)m.found;
+			if (normal) for (int k = 0; k < 3; k++) normal[3 * i + k] = n[k];
+			if (refined) refined[i] = ref;
+			if (branch) branch[i] = br;
+		}
+	}
+	return MBX_OK;
+}
+
+/* DE and its analytic gradient (the differentiable render's function) at n world points */
+int mbxo_diff_de(const mbx_params *p, const double *pts, int n, double *de, double *grad, int32_t *iters,
+	int32_t *ok)
+{
+	scene_t s;
+	int rc0 = scene_init(&s, p);
+	if (rc0) return rc0;
+	for (int i = 0; i < n; i++)
+	{
+		int it, okk;
+		double g[3];
+		const double d = diff_grad_o(&s, V3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]), g, &it, &okk);
+		if (de) de[i] = d;
+		if (grad) for (int k = 0; k < 3; k++) grad[3 * i + k] = g[k];
+		if (iters) iters[i] = it;
+		if (ok) ok[i] = okk;
+	}
 	return MBX_OK;
 }

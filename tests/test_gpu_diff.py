@@ -1,0 +1,174 @@
+"""Differentiable render and adjoint on the GPU (mbx_render_diff / mbx_adjoint) against the FP64
+oracle.
+
+* The device's differentiable render equals the oracle's FP64 one on hit/miss, iteration count
+  and refinement, and on the refined depth and colour to FP64/float output precision.
+* dL/dtheta of an L2 image loss from the adjoint kernel against central differences of the
+  oracle's FP64 render (step 1e-5, five-point stencil) on 16 seeded parameter sets; target
+  images are rendered at theta * (1 +- U(0, 0.05)).  Pixels whose hit, iteration count or
+  refinement change within the difference step (silhouettes, escape-count boundaries) have no
+  derivative there and are left out of both sums, and so are pixels that cross a kink of the
+  shading within the step (n.l = 0, the background's horizon, the clamps).  Bar: relative error
+  <= 1e-3 on every component whose reference gradient is at least 1e-3 of the set's largest
+  (smaller ones: absolute error <= 1e-6 of that largest).
+"""
+import numpy as np
+import pytest
+
+from mandelbulber2_amd import abi, scene
+
+pytestmark = pytest.mark.gpu
+
+H_FD = 1e-5
+
+
+@pytest.fixture(scope="module")
+def dmod():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from mandelbulber2_amd import diff, runtime
+    runtime.load()
+    return diff
+
+
+def _checkpoints(ck, n):
+    raw = ck.cpu().numpy()
+    t = raw[0::3][:n]
+    ints = raw.view(np.int32).reshape(n, 6)
+    return t, ints[:, 2], ints[:, 3], ints[:, 4]  # t, found, refined, iters
+
+
+def test_diff_render_matches_oracle(dmod, oracle_mod):
+    p = scene.inverse_rendering(64, 64, seed=0)
+    r = dmod.DiffRenderer(p)
+    bufs, ck = r.render(dmod.theta_of(p))
+    n = 64 * 64
+    t, found, refined, iters = _checkpoints(ck, n)
+    o = oracle_mod.render_diff(p)
+    r.close()
+    assert (found == o["mask"].ravel()).mean() >= 0.999
+    assert (iters == o["iters"].ravel()).mean() >= 0.995
+    both = (refined > 0) & (o["refined"].ravel() > 0)
+    assert both.sum() >= 0.95 * (found > 0).sum()
+    rel = np.abs(t[both] - o["depth"].ravel()[both]) / o["depth"].ravel()[both]
+    print("\n[diff render] refined %d of %d hits, depth rel err p99 %.3g max %.3g" % (
+        both.sum(), (found > 0).sum(), np.quantile(rel, 0.99), rel.max()), flush=True)
+    assert np.quantile(rel, 0.99) < 1e-9
+    rgb = bufs["rgb"].cpu().numpy().reshape(n, 3).astype(np.float64)
+    err = np.abs(rgb - o["rgb"].reshape(n, 3)).max(axis=1)
+    assert np.quantile(err[both], 0.99) < 1e-5
+    miss = (found == 0) & (o["mask"].ravel() == 0)
+    assert err[miss].max() < 1e-5
+
+
+def _fd_gradient(oracle_mod, dmod, p, theta, target):
+    """Per-pixel central differences of the L2 loss and the per-parameter stable-pixel masks."""
+    n = p.imageWidth * p.imageHeight
+    base = oracle_mod.render_diff(dmod.with_theta(p, theta))
+    fd = np.zeros((n, abi.MBX_DIFF_NPARAMS))
+    stable = np.zeros((n, abi.MBX_DIFF_NPARAMS), bool)
+    tgt = target.reshape(n, 3)
+    for k in range(abi.MBX_DIFF_NPARAMS):
+        outs = []
+        for step in (1.0, -1.0, 2.0, -2.0):
+            th = theta.copy()
+            th[k] += step * H_FD
+            outs.append(oracle_mod.render_diff(dmod.with_theta(p, th)))
+        lp, lm, lp2, lm2 = (((o["rgb"].reshape(n, 3) - tgt) ** 2).sum(1) for o in outs)
+        # five-point central stencil (fourth order): the three-point stencil's truncation error
+        # (h^2/6 times the third derivative) is visible at grazing hits, where the depth curves
+        # strongly with the parameters
+        fd[:, k] = (8.0 * (lp - lm) - (lp2 - lm2)) / (12.0 * H_FD)
+        st = np.ones(n, bool)
+        for key in ("mask", "iters", "refined", "branch"):
+            for o in outs:
+                st &= o[key].ravel() == base[key].ravel()
+        stable[:, k] = st
+    return base, fd, stable
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_adjoint_matches_central_differences(dmod, oracle_mod, seed):
+    import torch
+    p = scene.inverse_rendering(40, 40, seed=seed)
+    n = p.imageWidth * p.imageHeight
+    theta = dmod.theta_of(p)
+    rs = np.random.default_rng(100 + seed)
+    theta_t = theta * (1.0 + rs.choice([-1.0, 1.0], size=theta.size) * rs.uniform(0.0, 0.05, size=theta.size))
+    target = oracle_mod.render_diff(dmod.with_theta(p, theta_t))["rgb"]
+
+    r = dmod.DiffRenderer(p)
+    bufs, ck = r.render(theta)
+    img = bufs["rgb"]
+    dLdI = (2.0 * (img - torch.tensor(target, dtype=torch.float32, device=img.device))).contiguous()
+    g, pg = r.adjoint(theta, ck, dLdI, pixel_grad=True)
+    pg = pg.cpu().numpy()
+    t, found, refined, iters = _checkpoints(ck, n)
+    r.close()
+    assert np.allclose(pg.sum(0), g, rtol=1e-9, atol=1e-12)
+
+    base, fd, stable = _fd_gradient(oracle_mod, dmod, p, theta, target)
+    # the device's hit must be the oracle's: same mask, iterations and refinement, same root
+    same = (found == base["mask"].ravel()) & (iters == base["iters"].ravel()) & \
+        (refined == base["refined"].ravel())
+    hit = found > 0
+    same[hit] &= np.abs(t[hit] - base["depth"].ravel()[hit]) <= 1e-9 * base["depth"].ravel()[hit]
+    use = stable & same[:, None]
+    g_dev = (pg * use).sum(0)
+    g_fd = (fd * use).sum(0)
+    scale = np.abs(g_fd).max()
+    rel = np.abs(g_dev - g_fd) / np.maximum(np.abs(g_fd), 1e-3 * scale)
+    print("\n[adjoint seed %d] pixels used %s\n  fd  %s\n  dev %s\n  rel %s" % (
+        seed, use.sum(0).tolist(), np.array2string(g_fd, precision=5),
+        np.array2string(g_dev, precision=5), np.array2string(rel, precision=2)), flush=True)
+    assert use.sum(0).min() >= 0.9 * n
+    big = np.abs(g_fd) >= 1e-3 * scale
+    for k in np.nonzero(big & (rel > 1e-3))[0]:
+        dev_k = np.where(use[:, k], pg[:, k], 0.0)
+        fd_k = np.where(use[:, k], fd[:, k], 0.0)
+        worst = np.argsort(-np.abs(dev_k - fd_k))[:4]
+        print("  param %d worst pixels %s dev %s fd %s mask %s branch %s" % (
+            k, worst.tolist(), dev_k[worst], fd_k[worst], base["mask"].ravel()[worst],
+            base["branch"].ravel()[worst]), flush=True)
+    assert (rel[big] <= 1e-3).all(), rel
+    assert (np.abs(g_dev - g_fd)[~big] <= 1e-6 * scale).all()
+
+
+def test_autograd_function_runs_the_adjoint(dmod, oracle_mod):
+    """RenderFunction's backward is mbx_adjoint; a small step against the gradient lowers the
+    loss of an inverse-rendering problem."""
+    import torch
+    p = scene.inverse_rendering(48, 48, seed=21)
+    theta0 = dmod.theta_of(p)
+    target = oracle_mod.render_diff(dmod.with_theta(p, theta0 * 1.02))["rgb"]
+    tgt = torch.tensor(target, dtype=torch.float32, device="cuda")
+    r = dmod.DiffRenderer(p)
+    fn = dmod.render_function()
+    theta = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+    img = fn.apply(theta, r)
+    loss = ((img - tgt) ** 2).sum()
+    loss.backward()
+    bufs, ck = r.render(theta0)
+    g = r.adjoint(theta0, ck, (2.0 * (bufs["rgb"] - tgt)).contiguous())
+    np.testing.assert_allclose(theta.grad.numpy(), g, rtol=1e-6, atol=1e-9)
+    step = -theta.grad.numpy() / np.linalg.norm(theta.grad.numpy()) * 1e-3
+    bufs2, _ = r.render(theta0 + step)
+    loss2 = float(((bufs2["rgb"] - tgt) ** 2).sum())
+    r.close()
+    assert loss2 < float(loss)
+
+
+def test_differentiable_render_rejects_other_scenes(dmod):
+    from mandelbulber2_amd import runtime
+    p = scene.mandelbox_shadow_ao(32, 32)
+    ctx = runtime.Context(0)
+    ctx.upload(p)
+    with pytest.raises(runtime.MbxError, match="Mandelbulb"):
+        ctx.render_diff(ctx.alloc_outputs(), ctx.alloc_checkpoints())
+    q = scene.inverse_rendering(32, 32)
+    q.light.relativePosition = 1
+    ctx.upload(q)
+    with pytest.raises(runtime.MbxError, match="absolute"):
+        ctx.render_diff(ctx.alloc_outputs(), ctx.alloc_checkpoints())
+    ctx.close()

@@ -115,6 +115,21 @@ def test_menger_ifs_parity(rt, oracle_mod):
     _check(st)
 
 
+@pytest.mark.parametrize("which", ["mandelbulb8", "mandelbox"])
+def test_rotated_node_parity(rt, oracle_mod, which):
+    """A rotated fractal node (world -> local R p before every DE evaluation)."""
+    if which == "mandelbulb8":
+        p = scene.mandelbulb8(128, 96)
+    else:
+        p = scene.mandelbox_shadow_ao(128, 72, seed=1)
+    abi.set3(p.formulaRotation, (17.0, -33.0, 71.0))
+    g = rt.render_to_host(p)
+    r = oracle_mod.render(p)
+    st = compare(g, r)
+    _report("rotated %s" % which, st)
+    _check(st, depth_tol=1e-3 if which == "mandelbulb8" else 1e-6)
+
+
 def test_mandelbulb_trig_power_parity(rt, oracle_mod):
     p = scene.mandelbulb8(128, 96)
     p.fractal.bulb.power = 5.5

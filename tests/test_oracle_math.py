@@ -213,3 +213,47 @@ def test_pixel_seed_range(oracle_mod):
         for fs in (0, 7, 2 ** 32 - 1):
             s = oracle_mod.pixel_seed(pix, fs)
             assert 1 <= s <= 2147483646
+
+
+@pytest.mark.parametrize("rotation,perm", [
+    ((0.0, 0.0, 90.0), lambda q: (-q[1], q[0], q[2])),   # Rz(90): (x, y, z) -> (-y, x, z)
+    ((90.0, 0.0, 0.0), lambda q: (q[0], -q[2], q[1])),   # Rx(90): (x, y, z) -> (x, -z, y)
+    ((0.0, 90.0, 0.0), lambda q: (q[2], q[1], -q[0])),   # Ry(90): (x, y, z) -> (z, y, -x)
+    ((90.0, 0.0, 90.0), lambda q: (q[2], q[0], q[1])),   # Rz(90) Rx(90): x first, then z
+])
+def test_node_rotation_convention(oracle_mod, rotation, perm):
+    """A rotated fractal node evaluates the formula at R p with R = Rz(z) Ry(y) Rx(x): quarter
+    turns permute coordinates exactly, which pins each axis' sign and the composition order."""
+    rs = np.random.default_rng(11)
+    pts = rs.uniform(-1.2, 1.2, size=(64, 3))
+    p0 = scene.mandelbulb8(64, 48)
+    p0.fractal.bulb.power = 5.0  # no symmetry that would hide a wrong permutation
+    pr = scene.copy_params(p0)
+    abi.set3(pr.formulaRotation, rotation)
+    d_rot, it_rot = oracle_mod.distance(pr, pts, 1e-4)
+    d_ref, it_ref = oracle_mod.distance(p0, np.array([perm(q) for q in pts]), 1e-4)
+    np.testing.assert_allclose(d_rot, d_ref, rtol=1e-9, atol=1e-9)  # cos(pi/2) = 6e-17, not 0
+    assert (it_rot == it_ref).mean() >= 0.98
+
+
+def test_rotated_node_with_counter_rotated_camera_is_invariant(oracle_mod):
+    """Rotating the fractal by R and the camera, target and top vector by R^T leaves every ray's
+    DE samples unchanged, so mask, iteration counts and depth agree with the unrotated scene."""
+    p0 = scene.mandelbulb8(64, 48)
+    rot = (23.0, -41.0, 67.0)
+    a, b, c = (math.radians(v) for v in rot)
+    Rx = np.array([[1, 0, 0], [0, math.cos(a), -math.sin(a)], [0, math.sin(a), math.cos(a)]])
+    Ry = np.array([[math.cos(b), 0, math.sin(b)], [0, 1, 0], [-math.sin(b), 0, math.cos(b)]])
+    Rz = np.array([[math.cos(c), -math.sin(c), 0], [math.sin(c), math.cos(c), 0], [0, 0, 1]])
+    R = Rz @ Ry @ Rx
+    pr = scene.copy_params(p0)
+    abi.set3(pr.formulaRotation, rot)
+    for name in ("camera", "target", "topVector"):
+        abi.set3(getattr(pr, name), R.T @ np.array(list(getattr(p0, name))))
+    r0 = oracle_mod.render(p0)
+    r1 = oracle_mod.render(pr)
+    assert (r0["mask"] == r1["mask"]).mean() >= 0.99
+    assert (r0["iters"] == r1["iters"]).mean() >= 0.97
+    both = (r0["mask"] > 0) & (r1["mask"] > 0)
+    rel = np.abs(r0["depth"][both] - r1["depth"][both]) / r0["depth"][both]
+    assert np.median(rel) < 1e-6
