@@ -2,10 +2,10 @@
 
 A step is one full frame (1920x1080, 1 ray per pixel: march + binary search + 6-tap normal +
 Lambert shading) rendered by libmbx.so from constants already resident in HBM, launched as the
-persistent tile-queue kernel through a captured HIP graph.  The JSON line also carries the FP32
-VALU roofline of that kernel (algorithmic flops per launch over the launch's device time, HIP
-events on the launch stream), the plain one-tile-per-block kernel timed on the same frame, and
-the CPU oracle timed on this host.
+batch grid kernel (one 64-lane workgroup per tile) through a captured HIP graph.  The JSON line
+also carries the FP32 VALU roofline of that kernel (algorithmic flops per launch over the
+launch's device time, HIP events on the launch stream), the plain per-frame kernel timed on the
+same frame, and the CPU oracle timed on this host.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3]
                     [--workload mandelbulb8|mandelbox|animation64|deepzoom|adjoint512]
@@ -13,7 +13,7 @@ the CPU oracle timed on this host.
 `--workload mandelbox` times the Mandelbox scene instead (scale 2, delta DE by the reference's
 finite differences in FP64, hard shadow, 16-ray AO, seeded camera; 1920x1080).
 `--workload animation64` times a 64-frame batch (power 2 -> 12, mixed resolutions, 4 spp with
-GI depth 2) in one persistent launch against one plain launch per frame.
+GI depth 2) in one grid launch against one plain launch per frame.
 `--workload deepzoom` times the 1024x1024 deep-zoom Mandelbox (camera 1e-9 from the surface).
 `--workload adjoint512` times a differentiable render plus its adjoint (dL/dtheta) at 512x512.
 """
@@ -37,15 +37,15 @@ F_IT = {"mandelbulb8": 48, "mandelbox": 29, "animation64": 32, "deepzoom": 29}
 WORKLOADS = {
     "mandelbulb8": {
         "metric": "Mrays/s primary DE raymarch, 1080p Mandelbulb-8, 1 GPU; % FP32 VALU peak",
-        "kernel": "persistent_kernel<kBulb8>", "dtype": "f32", "peak": FP32_PEAK_TFLOPS,
+        "kernel": "batch_kernel<kBulb8>", "dtype": "f32", "peak": FP32_PEAK_TFLOPS,
         "config": {"formula": "mandelbulb power 8, analytic log DE, bailout 10, maxiter 250",
                    "camera": "distance 3.5, fov 53 deg", "spp": 1,
                    "shading": "6-tap normals + Lambert, 1 directional light, no shadows/AO/GI"},
     },
     "animation64": {
-        "metric": "Mrays/s (samples) of a 64-frame animation batch in one persistent launch, "
+        "metric": "Mrays/s (samples) of a 64-frame animation batch in one launch, "
                   "1 GPU; speedup over per-frame plain launches",
-        "kernel": "persistent_kernel<kBulbTrig>", "dtype": "f32", "peak": FP32_PEAK_TFLOPS,
+        "kernel": "batch_kernel<kBulbTrig>", "dtype": "f32", "peak": FP32_PEAK_TFLOPS,
         "config": {"formula": "mandelbulb power 2 -> 12 over 64 frames (trigonometric form), "
                               "analytic log DE, bailout 10, maxiter 250",
                    "camera": "seed 3, random walk on the radius 2.5-4 shell",
@@ -55,7 +55,7 @@ WORKLOADS = {
     "deepzoom": {
         "metric": "Mrays/s deep-zoom Mandelbox (camera 1e-9 from the surface, DE threshold 1e-11), "
                   "1024x1024, 1 GPU; % FP64 peak",
-        "kernel": "persistent_kernel<kBoxAnalytic>", "dtype": "f64", "peak": FP64_PEAK_TFLOPS,
+        "kernel": "batch_kernel<kBoxAnalytic>", "dtype": "f64", "peak": FP64_PEAK_TFLOPS,
         "config": {"formula": "mandelbox scale 2, analytic linear DE, maxiter 50",
                    "camera": "seed 5: 1e-9 back from an FP64 surface-search point",
                    "threshold": "constant 1e-11", "spp": 1,
@@ -73,7 +73,7 @@ WORKLOADS = {
     "mandelbox": {
         "metric": "Mrays/s DE raymarch with hard shadow + 16-ray AO, 1080p Mandelbox scale 2, "
                   "1 GPU; % FP64 peak",
-        "kernel": "persistent_kernel<kBoxDelta>", "dtype": "f64", "peak": FP64_PEAK_TFLOPS,
+        "kernel": "batch_kernel<kBoxDelta>", "dtype": "f64", "peak": FP64_PEAK_TFLOPS,
         "config": {"formula": "mandelbox scale 2, fold 1/2, radii 0.5/1, bailout 100, maxiter 50, "
                               "delta DE (4-orbit finite difference), linear",
                    "camera": "seed 1, radius U(6,10), target origin", "spp": 1,
@@ -273,13 +273,13 @@ def bench_adjoint(args, torch, runtime, wl):
 
 
 def bench_batch(args, torch, runtime, wl):
-    """One step = the whole 64-frame batch in one persistent launch (captured graph); the
+    """One step = the whole 64-frame batch in one grid launch (captured graph); the
     comparison renders the same frames one plain launch per frame."""
     frames = make_scene(args.workload, 0, 0)
     stream = torch.cuda.current_stream(0)
     ctx = runtime.Context(0, stream)
     bufs = [ctx.alloc_outputs(p) for p in frames]
-    ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_PERSISTENT)
+    ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_GRID)
     wall, launch_ms = time_kernel(torch, ctx, None, stream, args.steps, args.warmup,
                                   lambda: ctx.render_batch(frames, bufs))
     st = ctx.stats()
@@ -293,7 +293,7 @@ def bench_batch(args, torch, runtime, wl):
                 ctx.render(b)
         psteps = max(1, args.steps // 2)
         pwall, pms = time_kernel(torch, ctx, None, stream, psteps, 1, per_frame)
-        plain = {"kernel": wl["kernel"].replace("persistent_kernel", "render_kernel") +
+        plain = {"kernel": wl["kernel"].replace("batch_kernel", "render_kernel") +
                  " x 64 launches", "ms_per_step": pwall * 1e3 / psteps,
                  "median_launch_ms": statistics.median(pms)}
     ctx.close()
@@ -316,7 +316,7 @@ def bench_batch(args, torch, runtime, wl):
         "iterations_per_s": st["iterations"] / avg_launch_s,
     }
     if plain is not None:
-        plain["persistent_speedup"] = plain["ms_per_step"] / ms_per_step
+        plain["batch_speedup"] = plain["ms_per_step"] / ms_per_step
         out["plain_kernel"] = plain
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_batch(frames, args.cpu_seconds)
@@ -343,7 +343,7 @@ def main():
     ctx = runtime.Context(0, stream)
     ctx.upload(p)
     bufs = ctx.alloc_outputs()
-    ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_PERSISTENT)
+    ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_GRID)
     launch_shape = None
     if args.workload == "mandelbulb8" and not args.no_autotune:
         waves, _ = ctx.autotune(bufs, reps=2)
@@ -361,9 +361,12 @@ def main():
         psteps = max(3, args.steps // 2)
         pwall, pms = time_kernel(torch, ctx, bufs, stream, psteps, 1)
         ctx.tune(runtime.MBX_TUNE_TILE_WIDTH, 8)
-        plain = {"kernel": wl["kernel"].replace("persistent_kernel", "render_kernel"),
+        plain = {"kernel": wl["kernel"].replace("batch_kernel", "render_kernel"),
                  "ms_per_step": pwall * 1e3 / psteps, "median_launch_ms": statistics.median(pms)}
         ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_PERSISTENT)
+        qwall, qms = time_kernel(torch, ctx, bufs, stream, psteps, 1)
+        plain["persistent_kernel_ms_per_step"] = qwall * 1e3 / psteps
+        ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_GRID)
     ctx.close()
     if args.calibrate_copy:
         a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
@@ -406,7 +409,7 @@ def main():
         "iterations_per_frame": st["iterations"],
     }
     if plain is not None:
-        plain["persistent_speedup"] = plain["ms_per_step"] / ms_per_step
+        plain["batch_speedup"] = plain["ms_per_step"] / ms_per_step
         out["plain_kernel"] = plain
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(p, args.cpu_seconds)

@@ -242,6 +242,9 @@ typedef struct mbx_stats {
 	uint64_t de_evals;        /* CalculateDistance calls */
 	uint64_t iterations;      /* formula iterations (totalNumberOfIterations) */
 	double kernel_ms;         /* device time of the last render launch */
+	uint64_t far_steps;       /* march steps taken without a DE evaluation: beyond the far-field
+	                             radius the step is provably the clamped 3 (see MBX_TUNE_FAR_FIELD);
+	                             de_evals + far_steps = the reference's CalculateDistance calls */
 } mbx_stats;
 
 typedef struct mbx_ctx mbx_ctx;
@@ -299,13 +302,19 @@ int mbx_distance(mbx_ctx *ctx, const double *points, int n, double detail_size,
 int mbx_render_batch(mbx_ctx *ctx, const mbx_params *frames, const mbx_outputs *outs, int n);
 
 /* Launch tuning keys for mbx_tune. */
-#define MBX_TUNE_KERNEL 0    /* 0 = persistent work-stealing kernel (default), 1 = plain per-frame kernel */
+#define MBX_TUNE_KERNEL 0    /* 0 = one grid launch over every tile of every frame of a render or batch
+                                (default), 1 = one plain launch per frame, 2 = persistent waves
+                                draining tile queues */
 #define MBX_TUNE_WAVES 1     /* persistent kernel: number of 64-lane waves launched (0 = default) */
-#define MBX_TUNE_GRAPH 2     /* persistent kernel: 1 = replay a captured HIP graph (default), 0 = direct */
+#define MBX_TUNE_GRAPH 2     /* modes 0 and 2: 1 = replay a captured HIP graph (default), 0 = direct */
 #define MBX_TUNE_DELTA_DE 3  /* delta DE: 0 = the reference's finite differences (default),
                                 1 = forward-mode tangents (smooth limit of the same quotient) */
 #define MBX_TUNE_TILE_WIDTH 4 /* pixels per tile row: 4, 8 (default), 16 or 32; a tile is 64
                                  pixels (one wave), so its height is 64 / width */
+#define MBX_TUNE_FAR_FIELD 5  /* 1 = Mandelbulb march steps beyond the far-field radius, where the
+                                 first formula iteration bounds the DE above the value at which the
+                                 reference's step clamps to 3, take that step without evaluating the
+                                 DE (pixel results are identical); 0 (default) = evaluate every step */
 
 /* Set a launch or numerics option (see MBX_TUNE_*); scene options apply from the next
  * mbx_scene_upload / mbx_render_batch.  No reference counterpart: the OpenCL engine's
@@ -315,10 +324,10 @@ int mbx_tune(mbx_ctx *ctx, int key, int value);
 /* Current value of a tuning key (the wave count resolves the default). */
 int mbx_get_tune(mbx_ctx *ctx, int key, int *value);
 
-/* Project autotuner: time the uploaded scene with the persistent kernel for each tile shape
- * (tile widths 8, 16, 4, 32) and a set of wave counts (`reps` renders each into `out`) and keep
- * the fastest pair.  Returns the chosen wave count in *best_waves and its time per render in
- * *best_ms (either may be NULL); mbx_get_tune(MBX_TUNE_TILE_WIDTH) gives the tile width. */
+/* Project autotuner: time the uploaded scene in the current kernel mode for each tile shape
+ * (tile widths 8, 16, 4, 32) and, for the persistent kernel, a set of wave counts (`reps` renders
+ * each into `out`), and keep the fastest.  Returns the wave count in *best_waves and the time per
+ * render in *best_ms (either may be NULL); mbx_get_tune(MBX_TUNE_TILE_WIDTH) gives the tile width. */
 int mbx_autotune(mbx_ctx *ctx, const mbx_outputs *out, int reps, int *best_waves, double *best_ms);
 
 /* Statistics of the last render (synchronises the stream). */

@@ -189,6 +189,7 @@ class MbxStats(C.Structure):
         ("de_evals", C.c_uint64),
         ("iterations", C.c_uint64),
         ("kernel_ms", C.c_double),
+        ("far_steps", C.c_uint64),
     ]
 
 

@@ -450,6 +450,7 @@ struct Work
 {
 	uint32_t de_evals;
 	uint32_t iterations;
+	uint32_t far_steps;
 	int maxN, minN, addC, julia, analyticFunction, nodeRotated;
 	float bailout2;
 	float cx, cy, cz; // Julia constant, or the constant multiplier of c = p
@@ -474,6 +475,7 @@ __device__ __forceinline__ Work make_work(const FrameConst &f)
 	Work w;
 	w.de_evals = 0u;
 	w.iterations = 0u;
+	w.far_steps = 0u;
 	load_orbit_constants(w, f);
 	return w;
 }

@@ -47,6 +47,8 @@ struct FrameConst
 	float DEFactor;
 	float searchLimit;         // 1 - 0.001 * detailLevel
 	float viewDistanceMax;
+	float farR2;               // march points with |p|^2 >= farR2 take the clamped step 3 without a
+	                           // DE evaluation (+inf: never)
 	float smoothness;
 	int32_t maxRaymarchingSteps;
 	int32_t minN;

@@ -7,9 +7,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -36,7 +39,7 @@ struct mbx_ctx
 	uint64_t rays = 0;
 	std::string err;
 	// persistent / batched launches
-	int kernelMode = 0; // 0 persistent, 1 plain
+	int kernelMode = 0; // 0 grid batch kernel, 1 plain kernel per frame, 2 persistent tile queues
 	int waves = 0;      // 0 = default
 	int useGraph = 1;
 	int deltaTangent = 0;
@@ -48,8 +51,11 @@ struct mbx_ctx
 	std::vector<mbx::FrameJob> jobsHost; // what d_jobs holds
 	std::vector<FrameConst> framesHost;  // what d_frames holds
 	hipGraphExec_t graphExec = nullptr;
-	int graphKind = -1, graphFrames = -1, graphWaves = -1, graphTile = -1;
+	int graphKind = -1, graphFrames = -1, graphWaves = -1, graphTile = -1, graphFull = -1, graphMode = -1;
+	unsigned int graphTiles = 0;
 	int tileShift = 3; // 8x8-pixel tiles
+	int farField = 0;  // far-field march steps without DE evaluation (MBX_TUNE_FAR_FIELD)
+	int batchFull = 1; // the staged frames need the Monte Carlo / GI / shadow / AO code
 	// differentiable render
 	mbx::DiffConst diff{};
 	std::string diffWhy = "no scene uploaded"; // empty when the scene is differentiable
@@ -297,6 +303,57 @@ void setRot(float *dst, const Rot &r)
 	for (int i = 0; i < 9; i++) dst[i] = (float)r.m[i];
 }
 
+// Far-field radius of the FP32 Mandelbulb march.  Beyond radius r the first formula iteration
+// gives |z_1| >= r^P - c r (|T(p)| = r^P for every angle offset, c = max |constantMultiplier|) and
+// DE' = P r^(P-1) + 1, so the analytic DE is at least
+//   log:    f(r) = 0.5 (r^P - c r) ln(r^P - c r) / (P r^(P-1) + 1)
+//   linear: f(r) = (r^P - c r) / (P r^(P-1) + 1)
+// once r^P - c r exceeds the bailout (escape at i = 0).  The march's step
+// (dist - 0.5 thr) DEFactor (1 - Random(1000)/10000) is then >= 3, i.e. exactly the clamp 3,
+// when f(r) >= (3 + 0.5 thrMax) / (0.9 DEFactor); the clamp of dist at 10 must not cap below
+// that.  Returns the smallest radius (squared, with 1% margins) from which the bound holds on a
+// dense logarithmic grid up to r = 1e4 (farther than any march reaches), or +inf.
+float far_field_r2_uncached(double P, bool logDE, double bailout, double c, double DEFactor, double thrMax);
+
+// memoised: batches derive the same bound for frame after frame
+float far_field_r2(double P, bool logDE, double bailout, double c, double DEFactor, double thrMax)
+{
+	thread_local std::map<std::array<double, 6>, float> memo;
+	const std::array<double, 6> key{P, logDE ? 1.0 : 0.0, bailout, c, DEFactor, thrMax};
+	auto it = memo.find(key);
+	if (it != memo.end()) return it->second;
+	const float r2 = far_field_r2_uncached(P, logDE, bailout, c, DEFactor, thrMax);
+	if (memo.size() > 4096) memo.clear();
+	memo.emplace(key, r2);
+	return r2;
+}
+
+float far_field_r2_uncached(double P, bool logDE, double bailout, double c, double DEFactor, double thrMax)
+{
+	const float inf = std::numeric_limits<float>::infinity();
+	if (!(P >= 2.0) || !(DEFactor > 0.0)) return inf;
+	const double need = (3.0 + 0.5 * thrMax) / (0.9 * DEFactor) * 1.01;
+	if (need >= 10.0) return inf;
+	auto ok = [&](double r) {
+		const double r1 = std::pow(r, P) - c * r;
+		if (!(r1 > bailout * 1.01) || !(r1 > 1.0)) return false;
+		const double de = P * std::pow(r, P - 1.0) + 1.0;
+		const double f = logDE ? 0.5 * r1 * std::log(r1) / de : r1 / de;
+		return f >= need;
+	};
+	// the largest grid radius at which the bound fails, scanning down from 1e4
+	double rFail = 0.0;
+	for (double r = 1e4; r >= 1.0; r /= 1.0005)
+		if (!ok(r))
+		{
+			rFail = r;
+			break;
+		}
+	if (rFail >= 1e3) return inf;
+	const double r0 = rFail * 1.01;
+	return (float)(r0 * r0);
+}
+
 // Derive the per-frame constants; returns MBX_OK or an error with ctx->err set.
 int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 {
@@ -497,6 +554,16 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 		kind = mbx::kIfsAnalytic;
 	if (kind < 0) return fail(ctx, MBX_ERR_UNSUPPORTED, "formula / DE mode combination not supported");
 	fc.deKind = kind;
+	fc.farR2 = std::numeric_limits<float>::infinity();
+	if (ctx->farField && (kind == mbx::kBulb8 || kind == mbx::kBulbTrig) && !fc.julia && fc.addC)
+	{
+		const double c = std::max(std::fabs(fr.constantFactor[0]),
+			std::max(std::fabs(fr.constantFactor[1]), std::fabs(fr.constantFactor[2])));
+		const double thrMax = p->constantDEThreshold ? p->DEThresh
+													 : (p->viewDistanceMax + 3.0) * fc.threshKD * 1.01;
+		fc.farR2 = far_field_r2(fr.bulb.power, analyticFunction == MBX_ANALYTIC_LOGARITHMIC, fc.bailoutD, c,
+			p->DEFactor, thrMax);
+	}
 
 	// light #1 (light.cpp:80-195)
 	const mbx_light &L = p->light;
@@ -647,6 +714,12 @@ bool same_frames(const std::vector<FrameConst> &a, const std::vector<FrameConst>
 		(a.empty() || std::memcmp(a.data(), b.data(), a.size() * sizeof(FrameConst)) == 0);
 }
 
+// Whether a frame needs the kernels' Monte Carlo / GI / shadow / AO code.
+bool frame_needs_full(const FrameConst &f)
+{
+	return f.monteCarlo || f.aoEnabled || (f.lightEnabled && f.castShadows);
+}
+
 // Upload frames and jobs when they differ from what the device holds.
 int stage_batch(mbx_ctx *ctx, const std::vector<FrameConst> &frames, const std::vector<mbx::FrameJob> &jobs)
 {
@@ -672,9 +745,10 @@ int stage_batch(mbx_ctx *ctx, const std::vector<FrameConst> &frames, const std::
 }
 
 // Enqueue the batch launch (queue clear + persistent kernel), from a captured graph when enabled.
-int enqueue_persistent(mbx_ctx *ctx, int kind, int nFrames)
+int enqueue_batch(mbx_ctx *ctx, int kind, int nFrames, unsigned int tiles)
 {
 	const int waves = ctx->waves > 0 ? ctx->waves : default_waves(ctx);
+	const int mode = ctx->kernelMode == 2 ? mbx::kBatchPersistent : mbx::kBatchGrid;
 	mbx::BatchArgs a{};
 	a.frames = ctx->d_frames;
 	a.jobs = ctx->d_jobs;
@@ -682,6 +756,9 @@ int enqueue_persistent(mbx_ctx *ctx, int kind, int nFrames)
 	a.queue = ctx->d_queue;
 	a.counters = ctx->d_counters;
 	a.tileShift = ctx->tileShift;
+	a.full = ctx->batchFull;
+	a.mode = mode;
+	a.totalTiles = tiles;
 	hipError_t e;
 	if (!ctx->useGraph)
 	{
@@ -691,7 +768,8 @@ int enqueue_persistent(mbx_ctx *ctx, int kind, int nFrames)
 		return MBX_OK;
 	}
 	if (!ctx->graphExec || ctx->graphKind != kind || ctx->graphFrames != nFrames || ctx->graphWaves != waves ||
-		ctx->graphTile != ctx->tileShift)
+		ctx->graphTile != ctx->tileShift || ctx->graphFull != ctx->batchFull || ctx->graphMode != mode ||
+		ctx->graphTiles != tiles)
 	{
 		if (ctx->graphExec)
 		{
@@ -724,6 +802,9 @@ int enqueue_persistent(mbx_ctx *ctx, int kind, int nFrames)
 		ctx->graphFrames = nFrames;
 		ctx->graphWaves = waves;
 		ctx->graphTile = ctx->tileShift;
+		ctx->graphFull = ctx->batchFull;
+		ctx->graphMode = mode;
+		ctx->graphTiles = tiles;
 	}
 	// timing starts after any (re)capture, which is host work
 	if ((e = hipEventRecord(ctx->ev0, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
@@ -750,8 +831,28 @@ int render_frames(mbx_ctx *ctx, const std::vector<FrameConst> &frames, const mbx
 	}
 	int rc = stage_batch(ctx, frames, jobs);
 	if (rc != MBX_OK) return rc;
-	rc = enqueue_persistent(ctx, kind, (int)frames.size());
-	if (rc != MBX_OK) return rc;
+	ctx->batchFull = 0;
+	for (const auto &f : frames) ctx->batchFull |= frame_needs_full(f) ? 1 : 0;
+	if (ctx->kernelMode == 1)
+	{
+		// one plain launch per frame (the reference engine's job-per-frame shape)
+		if ((e = mbx::launch_clear(ctx->d_queue, ctx->d_counters, ctx->stream)) != hipSuccess)
+			return hip_fail(ctx, e, "clear counters");
+		if ((e = hipEventRecord(ctx->ev0, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
+		for (size_t i = 0; i < frames.size(); i++)
+		{
+			const mbx::Outputs o{outs[i].rgb, outs[i].depth, outs[i].iters, outs[i].mask, outs[i].normal,
+				outs[i].steps};
+			e = mbx::launch_render(kind, ctx->d_frames + i, frames[i].width, frames[i].height, o, ctx->d_counters,
+				ctx->tileShift, frame_needs_full(frames[i]), ctx->stream);
+			if (e != hipSuccess) return hip_fail(ctx, e, "render launch");
+		}
+	}
+	else
+	{
+		rc = enqueue_batch(ctx, kind, (int)frames.size(), (unsigned int)tiles);
+		if (rc != MBX_OK) return rc;
+	}
 	if ((e = hipEventRecord(ctx->ev1, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
 	ctx->launched = true;
 	ctx->rays = rays;
@@ -858,8 +959,8 @@ int mbx_ctx_create(int device, void *stream, mbx_ctx **out)
 	}
 	ctx->stream = (hipStream_t)stream;
 	if ((e = hipMalloc((void **)&ctx->d_frame, sizeof(FrameConst))) != hipSuccess ||
-		(e = hipMalloc((void **)&ctx->d_counters, 4 * sizeof(unsigned long long))) != hipSuccess ||
-		(e = hipMalloc((void **)&ctx->d_queue, 64)) != hipSuccess ||
+		(e = hipMalloc((void **)&ctx->d_counters, mbx::kCounterBytes)) != hipSuccess ||
+		(e = hipMalloc((void **)&ctx->d_queue, mbx::kTileQueueBytes)) != hipSuccess ||
 		(e = hipEventCreate(&ctx->ev0)) != hipSuccess || (e = hipEventCreate(&ctx->ev1)) != hipSuccess)
 	{
 		mbx_ctx_destroy(ctx);
@@ -997,24 +1098,8 @@ int mbx_render(mbx_ctx *ctx, const mbx_outputs *out)
 {
 	if (!ctx || !out) return fail(ctx, MBX_ERR_INVALID, "null argument");
 	if (!ctx->sceneReady) return fail(ctx, MBX_ERR_INVALID, "no scene uploaded");
-	if (ctx->kernelMode == 0)
-	{
-		std::vector<FrameConst> frames(1, ctx->frame);
-		return render_frames(ctx, frames, out);
-	}
-	hipError_t e = hipSetDevice(ctx->device);
-	if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
-	mbx::Outputs o{out->rgb, out->depth, out->iters, out->mask, out->normal, out->steps};
-	if ((e = mbx::launch_clear(ctx->d_queue, ctx->d_counters, ctx->stream)) != hipSuccess)
-		return hip_fail(ctx, e, "clear counters");
-	if ((e = hipEventRecord(ctx->ev0, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
-	e = mbx::launch_render(ctx->frame.deKind, ctx->d_frame, ctx->frame.width, ctx->frame.height, o,
-		ctx->d_counters, ctx->tileShift, ctx->stream);
-	if (e != hipSuccess) return hip_fail(ctx, e, "render launch");
-	if ((e = hipEventRecord(ctx->ev1, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
-	ctx->launched = true;
-	ctx->rays = (uint64_t)ctx->frame.width * ctx->frame.height * (uint64_t)ctx->frame.samples;
-	return MBX_OK;
+	std::vector<FrameConst> frames(1, ctx->frame);
+	return render_frames(ctx, frames, out);
 }
 
 int mbx_render_batch(mbx_ctx *ctx, const mbx_params *params, const mbx_outputs *outs, int n)
@@ -1047,7 +1132,7 @@ int mbx_tune(mbx_ctx *ctx, int key, int value)
 	switch (key)
 	{
 		case MBX_TUNE_KERNEL:
-			if (value != 0 && value != 1) return fail(ctx, MBX_ERR_INVALID, "kernel mode must be 0 or 1");
+			if (value < 0 || value > 2) return fail(ctx, MBX_ERR_INVALID, "kernel mode must be 0, 1 or 2");
 			ctx->kernelMode = value;
 			return MBX_OK;
 		case MBX_TUNE_WAVES:
@@ -1059,6 +1144,9 @@ int mbx_tune(mbx_ctx *ctx, int key, int value)
 			return MBX_OK;
 		case MBX_TUNE_DELTA_DE:
 			ctx->deltaTangent = value ? 1 : 0;
+			return MBX_OK;
+		case MBX_TUNE_FAR_FIELD:
+			ctx->farField = value ? 1 : 0;
 			return MBX_OK;
 		case MBX_TUNE_TILE_WIDTH:
 		{
@@ -1083,6 +1171,7 @@ int mbx_get_tune(mbx_ctx *ctx, int key, int *value)
 		case MBX_TUNE_GRAPH: *value = ctx->useGraph; return MBX_OK;
 		case MBX_TUNE_DELTA_DE: *value = ctx->deltaTangent; return MBX_OK;
 		case MBX_TUNE_TILE_WIDTH: *value = 1 << ctx->tileShift; return MBX_OK;
+		case MBX_TUNE_FAR_FIELD: *value = ctx->farField; return MBX_OK;
 		default: return fail(ctx, MBX_ERR_INVALID, "unknown tuning key");
 	}
 }
@@ -1092,11 +1181,12 @@ int mbx_autotune(mbx_ctx *ctx, const mbx_outputs *out, int reps, int *best_waves
 	if (!ctx || !out || reps < 1) return fail(ctx, MBX_ERR_INVALID, "bad argument");
 	if (!ctx->sceneReady) return fail(ctx, MBX_ERR_INVALID, "no scene uploaded");
 	const int savedMode = ctx->kernelMode;
-	ctx->kernelMode = 0;
-	const int perCU[] = {4, 8, 12, 16, 20, 24, 32};
+	// the wave count matters only to the persistent kernel
+	const std::vector<int> perCU = ctx->kernelMode == 2 ? std::vector<int>{4, 8, 12, 16, 20, 24, 32}
+														: std::vector<int>{0};
 	const int shifts[] = {3, 4, 2, 5}; // tile widths 8, 16, 4, 32 (64 pixels each)
 	double bestT = 1e30;
-	int bestW = default_waves(ctx), bestShift = ctx->tileShift;
+	int bestW = ctx->waves, bestShift = ctx->tileShift;
 	for (int sh : shifts)
 	for (int k : perCU)
 	{
@@ -1131,7 +1221,7 @@ int mbx_autotune(mbx_ctx *ctx, const mbx_outputs *out, int reps, int *best_waves
 	ctx->waves = bestW;
 	ctx->tileShift = bestShift;
 	ctx->kernelMode = savedMode;
-	if (best_waves) *best_waves = bestW;
+	if (best_waves) *best_waves = bestW > 0 ? bestW : default_waves(ctx);
 	if (best_ms) *best_ms = bestT;
 	return MBX_OK;
 }
@@ -1144,13 +1234,17 @@ int mbx_get_stats(mbx_ctx *ctx, mbx_stats *stats)
 	if (e != hipSuccess) return hip_fail(ctx, e, "render");
 	float ms = 0.0f;
 	if ((e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1)) != hipSuccess) return hip_fail(ctx, e, "timing");
-	unsigned long long c[4];
-	if ((e = hipMemcpy(c, ctx->d_counters, sizeof(c), hipMemcpyDeviceToHost)) != hipSuccess)
+	std::vector<unsigned long long> slots((size_t)mbx::kCounterSlots * mbx::kCounterStride);
+	if ((e = hipMemcpy(slots.data(), ctx->d_counters, mbx::kCounterBytes, hipMemcpyDeviceToHost)) != hipSuccess)
 		return hip_fail(ctx, e, "read counters");
+	unsigned long long c[mbx::kCounters] = {0, 0, 0};
+	for (unsigned int sl = 0; sl < mbx::kCounterSlots; sl++)
+		for (unsigned int k = 0; k < mbx::kCounters; k++) c[k] += slots[(size_t)sl * mbx::kCounterStride + k];
 	stats->rays = ctx->rays;
 	stats->de_evals = c[0];
 	stats->iterations = c[1];
 	stats->kernel_ms = ms;
+	stats->far_steps = c[2];
 	return MBX_OK;
 }
 

@@ -20,10 +20,10 @@ struct Outputs
 };
 
 // Plain per-frame launch: one 64-lane workgroup per tile of (1 << tileShift) x (64 >> tileShift)
-// pixels.  counters[0] receives the
-// DE evaluations, counters[1] the formula iterations (atomically added).
+// pixels.  full = false launches the variant without Monte Carlo, GI, shadow and AO code.  The
+// work counters are added into `counters` (kCounterBytes, slot layout above).
 hipError_t launch_render(int deKind, const FrameConst *frame, int width, int height,
-	const Outputs &out, unsigned long long *counters, int tileShift, hipStream_t stream);
+	const Outputs &out, unsigned long long *counters, int tileShift, bool full, hipStream_t stream);
 
 // Point DE query at a fixed detail size.
 hipError_t launch_distance(int deKind, const FrameConst *frame, const float *pts, int n,
@@ -47,6 +47,20 @@ struct FrameJob
 	int32_t *steps;
 };
 
+// Work counters: kCounterSlots slots of kCounterStride u64, each slot on its own 256-byte line;
+// a workgroup adds into slot (block mod kCounterSlots) so that the atomics of thousands of
+// workgroups do not queue on one address.  Counter k of the launch = sum over the slots of
+// slot[k]: 0 DE evaluations, 1 formula iterations, 2 far-field steps.
+constexpr unsigned int kCounterSlots = 64;
+constexpr unsigned int kCounterStride = 32;
+constexpr unsigned int kCounters = 3;
+constexpr size_t kCounterBytes = (size_t)kCounterSlots * kCounterStride * sizeof(unsigned long long);
+
+// Tile queues of the persistent kernel: kTileQueues counters, kTileQueueStride uints apart.
+constexpr unsigned int kTileQueues = 32;
+constexpr unsigned int kTileQueueStride = 64;
+constexpr size_t kTileQueueBytes = (size_t)kTileQueues * kTileQueueStride * sizeof(unsigned int);
+
 // Arguments of the persistent kernel (passed by value).
 struct BatchArgs
 {
@@ -54,15 +68,22 @@ struct BatchArgs
 	const FrameJob *jobs;
 	int numFrames;
 	int tileShift; // tiles are (1 << tileShift) x (64 >> tileShift) pixels, tileShift in 2..5
-	unsigned int *queue;           // [0] tile queue head
-	unsigned long long *counters;  // [0] DE evaluations, [1] formula iterations
+	int full;      // 0: every frame renders without Monte Carlo, GI, shadows and AO
+	int mode;      // kBatchGrid or kBatchPersistent
+	unsigned int totalTiles;
+	unsigned int *queue;           // kTileQueues tile counters (kTileQueueBytes)
+	unsigned long long *counters;  // work counter slots (kCounterBytes)
 };
 
-// Enqueue the batch: a clear of the queue head and counters, then the persistent kernel with
-// `numWaves` 64-lane waves.
+// Batch launch modes: one workgroup per tile of every frame (grid), or a fixed grid of
+// `numWaves` waves draining tile queues (persistent).
+constexpr int kBatchGrid = 0;
+constexpr int kBatchPersistent = 2;
+
+// Enqueue the batch: a clear of the queues and counters, then the batch kernel of a.mode.
 hipError_t launch_batch(int deKind, BatchArgs a, int numWaves, hipStream_t stream);
 
-// Zero queue[0] and the work counters.
+// Zero the tile counters and the work counters.
 hipError_t launch_clear(unsigned int *queue, unsigned long long *counters, hipStream_t stream);
 
 } // namespace mbx

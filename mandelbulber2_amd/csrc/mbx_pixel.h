@@ -79,6 +79,11 @@ __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start
 	const R thrScale = kExact ? (R)0 : dirLength * (R)f.threshK;
 	const R constThrValue = Q::DEThresh(f);
 	const int maxSteps = f.maxRaymarchingSteps;
+	// far field (FP32 Mandelbulb kinds): at |p|^2 >= farR2 the host has bounded the DE from the
+	// first formula iteration (|z_1| >= |p|^P - |c|, escape at i = 0, iters = 1) above the value
+	// that makes the step below clamp to exactly 3, so the step is taken without evaluating it;
+	// the jitter's random number is still drawn, so every later value is unchanged
+	const float farR2 = kExact ? __builtin_inff() : f.farR2;
 	for (int i = 0; i < maxSteps; i++)
 	{
 		const P last = point;
@@ -94,6 +99,16 @@ __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start
 			found = true;
 			dead = true;
 			break;
+		}
+		if (!kExact && (float)dot(Q::tof(point), Q::tof(point)) >= farR2)
+		{
+			(void)random_mod(seed, 1000);
+			step = (R)3;
+			iters = 1;
+			w.far_steps++;
+			scan += step * invDirLength;
+			if (scan > maxScan) break;
+			continue;
 		}
 		if (kExact)
 			thr = dist_thresh<KIND>(f, point);
@@ -342,15 +357,16 @@ __device__ V3 background_shader(const FrameConst &f, V3 viewVector)
 }
 
 // ObjectShader with one directional light, Lambert, optional shadow and AO
-// (shader_object.cpp:40-229, shader_light_shading.cpp:40-122)
-template <int KIND>
+// (shader_object.cpp:40-229, shader_light_shading.cpp:40-122).  FULL = false compiles the
+// shadow and AO paths out (frames that use neither), which lowers the kernel's registers.
+template <int KIND, bool FULL = true>
 __device__ V3 object_shader(const FrameConst &f, typename PrecOf<KIND>::P p, V3 n, typename PrecOf<KIND>::R thr,
 	typename PrecOf<KIND>::R delta, uint32_t &seed, Work &w)
 {
 	using R = typename PrecOf<KIND>::R;
 	const V3 colour = v3(f.surfaceColor[0], f.surfaceColor[1], f.surfaceColor[2]);
 	V3 ambient = v3(0.0f, 0.0f, 0.0f);
-	if (f.aoEnabled)
+	if (FULL && f.aoEnabled)
 	{
 		if (f.aoMode == MBX_AO_FAST)
 		{
@@ -368,7 +384,7 @@ __device__ V3 object_shader(const FrameConst &f, typename PrecOf<KIND>::P p, V3 
 		shade = 1.0f - f.shading + shade * f.shading;
 		shade = fminf(shade * f.lightIntensity, 500.0f);
 		float shadow = 1.0f;
-		if (f.castShadows)
+		if (FULL && f.castShadows)
 		{
 			if (shade > 0.001f)
 			{
@@ -463,8 +479,9 @@ struct PixelOut
 	int steps;
 };
 
-// One pixel: the sample loop of doWork and RayRecursion at depth 0
-template <int KIND>
+// One pixel: the sample loop of doWork and RayRecursion at depth 0.  FULL = false is the
+// variant for frames without Monte Carlo sampling, GI, shadows and AO.
+template <int KIND, bool FULL>
 __device__ void render_pixel(const FrameConst &f, int xs, int ys, PixelOut &po, Work &w)
 {
 #pragma clang fp contract(off)
@@ -477,13 +494,13 @@ __device__ void render_pixel(const FrameConst &f, int xs, int ys, PixelOut &po, 
 	const R ox = (Q::invWidth(f) * (R)xs - (R)0.5) * aspect;
 	const R oy = (R)0.5 - Q::invHeight(f) * (R)ys;
 	const P cam = Q::cam(f);
-	const int repeats = f.monteCarlo ? f.samples : 1;
+	const int repeats = (FULL && f.monteCarlo) ? f.samples : 1;
 	V3 acc = v3(0.0f, 0.0f, 0.0f);
 	po.steps = 0;
 	for (int rep = 0; rep < repeats; rep++)
 	{
 		R ix = ox, iy = oy;
-		if (f.monteCarlo)
+		if (FULL && f.monteCarlo)
 		{
 			ix = ox + ((R)random_mod(seed, 1000) * (R)1e-3 - (R)0.5) * Q::invWidth(f) * aspect;
 			iy = oy + ((R)random_mod(seed, 1000) * (R)1e-3 - (R)0.5) * Q::invHeight(f);
@@ -497,8 +514,8 @@ __device__ void render_pixel(const FrameConst &f, int xs, int ys, PixelOut &po, 
 		{
 			n = calculate_normals<KIND>(f, m.point, m.distThresh, w);
 			const R delta = calc_delta<KIND>(f, m.point);
-			rgb = object_shader<KIND>(f, m.point, n, m.distThresh, delta, seed, w);
-			if (f.monteCarlo && f.globalIllumination)
+			rgb = object_shader<KIND, FULL>(f, m.point, n, m.distThresh, delta, seed, w);
+			if (FULL && f.monteCarlo && f.globalIllumination)
 				rgb = rgb + global_illumination<KIND>(f, m.point, n, m.distThresh, seed, w);
 			rgb = v3(fmaxf(rgb.x, 0.0f), fmaxf(rgb.y, 0.0f), fmaxf(rgb.z, 0.0f));
 		}

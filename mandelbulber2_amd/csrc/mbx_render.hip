@@ -9,7 +9,7 @@ namespace mbx
 
 // Plain per-frame kernel: 64 lanes = one tile of (1 << tileShift) x (64 >> tileShift) pixels
 // (8x8 by default), one workgroup per tile, one launch per frame.
-template <int KIND>
+template <int KIND, bool FULL>
 __global__ __launch_bounds__(64, KIND >= kBoxAnalytic ? 2 : 1) void render_kernel(const FrameConst *__restrict__ frame,
 	Outputs out, unsigned long long *__restrict__ counters, int tileShift)
 {
@@ -23,26 +23,76 @@ __global__ __launch_bounds__(64, KIND >= kBoxAnalytic ? 2 : 1) void render_kerne
 	if (xs < f.width && ys < f.height)
 	{
 		PixelOut po;
-		render_pixel<KIND>(f, xs, ys, po, w);
+		render_pixel<KIND, FULL>(f, xs, ys, po, w);
 		write_pixel(out, (size_t)ys * (size_t)f.width + (size_t)xs, po);
 	}
 	if (counters)
 	{
-		wave_add_u64(&counters[0], w.de_evals);
-		wave_add_u64(&counters[1], w.iterations);
+		unsigned long long *slot = counters + (blockIdx.x % kCounterSlots) * kCounterStride;
+		wave_add_u64(&slot[0], w.de_evals);
+		wave_add_u64(&slot[1], w.iterations);
+		wave_add_u64(&slot[2], w.far_steps);
 	}
 }
 
-// Persistent work-stealing kernel: a fixed grid of waves drains one queue of 64-pixel tiles that
-// covers every frame of a batch (frames of different resolution, camera and formula
-// parameters).  The wave-uniform tile index keeps each tile's frame constants in scalar
-// registers.  One launch replaces the reference's one-job-per-frame animation loop, and the
-// fixed grid avoids dispatching one workgroup per tile.
+// Batch grid kernel: one 64-lane workgroup per tile over every frame of a batch (frames of
+// different resolution, camera and formula parameters), in a single launch.  A workgroup finds
+// its frame by a binary search over the frames' tile ranges (wave-uniform scalar loads), so the
+// frame constants stay in scalar registers.  One launch (replayed from a captured graph)
+// replaces the reference's one-job-per-frame animation loop; the hardware dispatcher balances
+// the tiles without any atomic.
+template <int KIND, bool FULL>
+__global__ __launch_bounds__(64, KIND >= kBoxAnalytic ? 2 : 1) void batch_kernel(const FrameConst *__restrict__ frames,
+	const FrameJob *__restrict__ jobs, int numFrames, unsigned long long *__restrict__ counters, int tileShift)
+{
+	const int lane = threadIdx.x;
+	const unsigned int t = blockIdx.x;
+	int lo = 0, hi = numFrames - 1;
+	while (lo < hi)
+	{
+		const int mid = (lo + hi) >> 1;
+		if (t < (unsigned int)jobs[mid].tileEnd)
+			hi = mid;
+		else
+			lo = mid + 1;
+	}
+	const int fr = lo;
+	const FrameConst &f = frames[fr];
+	const FrameJob &job = jobs[fr];
+	Work w = make_work(f);
+	const unsigned int tileBegin = fr == 0 ? 0u : (unsigned int)jobs[fr - 1].tileEnd;
+	const int lt = (int)(t - tileBegin);
+	const int tilesX = (f.width + (1 << tileShift) - 1) >> tileShift;
+	const int xs = ((lt % tilesX) << tileShift) + (lane & ((1 << tileShift) - 1));
+	const int ys = (lt / tilesX) * (64 >> tileShift) + (lane >> tileShift);
+	if (xs < f.width && ys < f.height)
+	{
+		PixelOut po;
+		render_pixel<KIND, FULL>(f, xs, ys, po, w);
+		const Outputs out{job.rgb, job.depth, job.iters, job.mask, job.normal, job.steps};
+		write_pixel(out, (size_t)ys * (size_t)f.width + (size_t)xs, po);
+	}
+	if (counters)
+	{
+		unsigned long long *slot = counters + (blockIdx.x % kCounterSlots) * kCounterStride;
+		wave_add_u64(&slot[0], w.de_evals);
+		wave_add_u64(&slot[1], w.iterations);
+		wave_add_u64(&slot[2], w.far_steps);
+	}
+}
+
+// Persistent work-stealing kernel: a fixed grid of waves drains the 64-pixel tiles of every
+// frame of a batch (frames of different resolution, camera and formula parameters).  The
+// wave-uniform tile index keeps each tile's frame constants in scalar registers.  One launch
+// replaces the reference's one-job-per-frame animation loop, and the fixed grid avoids
+// dispatching one workgroup per tile.
+// The tiles are split into kTileQueues contiguous ranges, each with its own counter on its own
+// 256-byte line: a device-scope atomic on one address serialises at about 8 ns per operation,
+// which 4096 waves asking for a tile every ~25 us would saturate.  A wave starts on queue
+// (block mod kTileQueues) and, when that range is exhausted, moves on to the next queue.
 // (frames and jobs are separate __restrict__ kernel arguments so that the compiler may read
 // them with scalar loads)
-// At least two waves per SIMD for the Mandelbox and IFS kinds: uncapped, their kernels take
-// more than 256 registers and run one wave per SIMD.
-template <int KIND>
+template <int KIND, bool FULL>
 __global__ __launch_bounds__(64, KIND >= kBoxAnalytic ? 2 : 1) void persistent_kernel(const FrameConst *__restrict__ frames,
 	const FrameJob *__restrict__ jobs, int numFrames, unsigned int *__restrict__ queue,
 	unsigned long long *__restrict__ counters, int tileShift)
@@ -51,42 +101,57 @@ __global__ __launch_bounds__(64, KIND >= kBoxAnalytic ? 2 : 1) void persistent_k
 	const unsigned int totalTiles = (unsigned int)jobs[numFrames - 1].tileEnd;
 	Work w = make_work(frames[0]);
 	int fr = 0;
-	while (true)
+	const unsigned int home = blockIdx.x % kTileQueues;
+	for (unsigned int k = 0; k < kTileQueues; k++)
 	{
-		unsigned int t = 0;
-		if (lane == 0) t = atomicAdd(queue, 1u);
-		t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
-		if (t >= totalTiles) break;
-		while (t >= (unsigned int)jobs[fr].tileEnd) fr++; // tiles are handed out in order
-		const FrameConst &f = frames[fr];
-		const FrameJob &job = jobs[fr];
-		load_orbit_constants(w, f);
-		const unsigned int tileBegin = fr == 0 ? 0u : (unsigned int)jobs[fr - 1].tileEnd;
-		const int lt = (int)(t - tileBegin);
-		const int tilesX = (f.width + (1 << tileShift) - 1) >> tileShift;
-		const int xs = ((lt % tilesX) << tileShift) + (lane & ((1 << tileShift) - 1));
-		const int ys = (lt / tilesX) * (64 >> tileShift) + (lane >> tileShift);
-		if (xs < f.width && ys < f.height)
+		const unsigned int q = (home + k) % kTileQueues;
+		const unsigned int begin = (unsigned int)((unsigned long long)totalTiles * q / kTileQueues);
+		const unsigned int count = (unsigned int)((unsigned long long)totalTiles * (q + 1) / kTileQueues) - begin;
+		unsigned int *head = queue + q * kTileQueueStride;
+		// a plain load never reads more than the counter holds: skip ranges already exhausted
+		if (__builtin_amdgcn_readfirstlane(__atomic_load_n(head, __ATOMIC_RELAXED)) >= count) continue;
+		while (true)
 		{
-			PixelOut po;
-			render_pixel<KIND>(f, xs, ys, po, w);
-			const Outputs out{job.rgb, job.depth, job.iters, job.mask, job.normal, job.steps};
-			write_pixel(out, (size_t)ys * (size_t)f.width + (size_t)xs, po);
+			unsigned int c = 0;
+			if (lane == 0) c = atomicAdd(head, 1u);
+			c = __builtin_amdgcn_readfirstlane(c);
+			if (c >= count) break;
+			const unsigned int t = begin + c;
+			if (fr > 0 && t < (unsigned int)jobs[fr - 1].tileEnd) fr = 0;
+			while (t >= (unsigned int)jobs[fr].tileEnd) fr++;
+			const FrameConst &f = frames[fr];
+			const FrameJob &job = jobs[fr];
+			load_orbit_constants(w, f);
+			const unsigned int tileBegin = fr == 0 ? 0u : (unsigned int)jobs[fr - 1].tileEnd;
+			const int lt = (int)(t - tileBegin);
+			const int tilesX = (f.width + (1 << tileShift) - 1) >> tileShift;
+			const int xs = ((lt % tilesX) << tileShift) + (lane & ((1 << tileShift) - 1));
+			const int ys = (lt / tilesX) * (64 >> tileShift) + (lane >> tileShift);
+			if (xs < f.width && ys < f.height)
+			{
+				PixelOut po;
+				render_pixel<KIND, FULL>(f, xs, ys, po, w);
+				const Outputs out{job.rgb, job.depth, job.iters, job.mask, job.normal, job.steps};
+				write_pixel(out, (size_t)ys * (size_t)f.width + (size_t)xs, po);
+			}
 		}
 	}
 	if (counters)
 	{
-		wave_add_u64(&counters[0], w.de_evals);
-		wave_add_u64(&counters[1], w.iterations);
+		unsigned long long *slot = counters + (blockIdx.x % kCounterSlots) * kCounterStride;
+		wave_add_u64(&slot[0], w.de_evals);
+		wave_add_u64(&slot[1], w.iterations);
+		wave_add_u64(&slot[2], w.far_steps);
 	}
 }
 
-// Zero the tile queue head and the work counters (a kernel, so that it can be captured in a
+// Zero the tile queue heads and the work counters (a kernel, so that it can be captured in a
 // graph without a memset node).
 __global__ void clear_kernel(unsigned int *queue, unsigned long long *counters)
 {
-	if (threadIdx.x == 0) queue[0] = 0u;
-	if (threadIdx.x < 4) counters[threadIdx.x] = 0ull;
+	if (threadIdx.x < kTileQueues) queue[threadIdx.x * kTileQueueStride] = 0u;
+	if (threadIdx.x < kCounterSlots)
+		for (unsigned int k = 0; k < kCounters; k++) counters[threadIdx.x * kCounterStride + k] = 0ull;
 }
 
 // Point DE query (distance engine analogue)
@@ -107,12 +172,16 @@ __global__ __launch_bounds__(64) void distance_kernel(const FrameConst *__restri
 
 template <int KIND>
 static hipError_t launch_render_t(const FrameConst *frame, int width, int height, const Outputs &out,
-	unsigned long long *counters, int tileShift, hipStream_t stream)
+	unsigned long long *counters, int tileShift, bool full, hipStream_t stream)
 {
 	const int tw = 1 << tileShift, th = 64 >> tileShift;
 	const int tiles = ((width + tw - 1) / tw) * ((height + th - 1) / th);
-	hipLaunchKernelGGL(render_kernel<KIND>, dim3(tiles), dim3(64), 0, stream, frame, out, counters,
-		tileShift);
+	if (full)
+		hipLaunchKernelGGL((render_kernel<KIND, true>), dim3(tiles), dim3(64), 0, stream, frame, out, counters,
+			tileShift);
+	else
+		hipLaunchKernelGGL((render_kernel<KIND, false>), dim3(tiles), dim3(64), 0, stream, frame, out, counters,
+			tileShift);
 	return hipGetLastError();
 }
 
@@ -126,17 +195,17 @@ static hipError_t launch_distance_t(const FrameConst *frame, const float *pts, i
 }
 
 hipError_t launch_render(int deKind, const FrameConst *frame, int width, int height,
-	const Outputs &out, unsigned long long *counters, int tileShift, hipStream_t stream)
+	const Outputs &out, unsigned long long *counters, int tileShift, bool full, hipStream_t stream)
 {
 	switch (deKind)
 	{
-		case kBulb8: return launch_render_t<kBulb8>(frame, width, height, out, counters, tileShift, stream);
-		case kBulbTrig: return launch_render_t<kBulbTrig>(frame, width, height, out, counters, tileShift, stream);
+		case kBulb8: return launch_render_t<kBulb8>(frame, width, height, out, counters, tileShift, full, stream);
+		case kBulbTrig: return launch_render_t<kBulbTrig>(frame, width, height, out, counters, tileShift, full, stream);
 		case kBoxAnalytic:
-			return launch_render_t<kBoxAnalytic>(frame, width, height, out, counters, tileShift, stream);
-		case kBoxDelta: return launch_render_t<kBoxDelta>(frame, width, height, out, counters, tileShift, stream);
+			return launch_render_t<kBoxAnalytic>(frame, width, height, out, counters, tileShift, full, stream);
+		case kBoxDelta: return launch_render_t<kBoxDelta>(frame, width, height, out, counters, tileShift, full, stream);
 		case kIfsAnalytic:
-			return launch_render_t<kIfsAnalytic>(frame, width, height, out, counters, tileShift, stream);
+			return launch_render_t<kIfsAnalytic>(frame, width, height, out, counters, tileShift, full, stream);
 		default: return hipErrorInvalidValue;
 	}
 }
@@ -148,10 +217,26 @@ hipError_t launch_clear(unsigned int *queue, unsigned long long *counters, hipSt
 }
 
 template <int KIND>
-static hipError_t launch_batch_t(const BatchArgs &a, int numWaves, hipStream_t stream)
+static hipError_t launch_batch_t(const BatchArgs &a, int numWaves, bool full, hipStream_t stream)
 {
-	hipLaunchKernelGGL(persistent_kernel<KIND>, dim3(numWaves), dim3(64), 0, stream, a.frames, a.jobs,
-		a.numFrames, a.queue, a.counters, a.tileShift);
+	if (a.mode == kBatchPersistent)
+	{
+		if (full)
+			hipLaunchKernelGGL((persistent_kernel<KIND, true>), dim3(numWaves), dim3(64), 0, stream, a.frames,
+				a.jobs, a.numFrames, a.queue, a.counters, a.tileShift);
+		else
+			hipLaunchKernelGGL((persistent_kernel<KIND, false>), dim3(numWaves), dim3(64), 0, stream, a.frames,
+				a.jobs, a.numFrames, a.queue, a.counters, a.tileShift);
+	}
+	else
+	{
+		if (full)
+			hipLaunchKernelGGL((batch_kernel<KIND, true>), dim3(a.totalTiles), dim3(64), 0, stream, a.frames, a.jobs,
+				a.numFrames, a.counters, a.tileShift);
+		else
+			hipLaunchKernelGGL((batch_kernel<KIND, false>), dim3(a.totalTiles), dim3(64), 0, stream, a.frames, a.jobs,
+				a.numFrames, a.counters, a.tileShift);
+	}
 	return hipGetLastError();
 }
 
@@ -161,11 +246,11 @@ hipError_t launch_batch(int deKind, BatchArgs a, int numWaves, hipStream_t strea
 	if (e != hipSuccess) return e;
 	switch (deKind)
 	{
-		case kBulb8: return launch_batch_t<kBulb8>(a, numWaves, stream);
-		case kBulbTrig: return launch_batch_t<kBulbTrig>(a, numWaves, stream);
-		case kBoxAnalytic: return launch_batch_t<kBoxAnalytic>(a, numWaves, stream);
-		case kBoxDelta: return launch_batch_t<kBoxDelta>(a, numWaves, stream);
-		case kIfsAnalytic: return launch_batch_t<kIfsAnalytic>(a, numWaves, stream);
+		case kBulb8: return launch_batch_t<kBulb8>(a, numWaves, a.full != 0, stream);
+		case kBulbTrig: return launch_batch_t<kBulbTrig>(a, numWaves, a.full != 0, stream);
+		case kBoxAnalytic: return launch_batch_t<kBoxAnalytic>(a, numWaves, a.full != 0, stream);
+		case kBoxDelta: return launch_batch_t<kBoxDelta>(a, numWaves, a.full != 0, stream);
+		case kIfsAnalytic: return launch_batch_t<kIfsAnalytic>(a, numWaves, a.full != 0, stream);
 		default: return hipErrorInvalidValue;
 	}
 }

@@ -62,8 +62,10 @@ MBX_TUNE_WAVES = 1
 MBX_TUNE_GRAPH = 2
 MBX_TUNE_DELTA_DE = 3
 MBX_TUNE_TILE_WIDTH = 4
-KERNEL_PERSISTENT = 0
+MBX_TUNE_FAR_FIELD = 5
+KERNEL_GRID = 0
 KERNEL_PLAIN = 1
+KERNEL_PERSISTENT = 2
 
 
 class Context:
@@ -189,7 +191,7 @@ class Context:
         s = abi.MbxStats()
         self._check(self.L.mbx_get_stats(self.h, C.byref(s)), "mbx_get_stats")
         return {"rays": s.rays, "de_evals": s.de_evals, "iterations": s.iterations,
-                "kernel_ms": s.kernel_ms}
+                "kernel_ms": s.kernel_ms, "far_steps": s.far_steps}
 
     def distance(self, pts, detail_size):
         import numpy as np
@@ -203,7 +205,7 @@ class Context:
         return d, it
 
 
-def render_to_host(params, device=0, kernel=KERNEL_PERSISTENT):
+def render_to_host(params, device=0, kernel=KERNEL_GRID):
     """Upload, render and copy all planes back as numpy arrays (plus work statistics)."""
     import torch
     ctx = Context(device)

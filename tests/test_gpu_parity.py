@@ -15,7 +15,7 @@ from tests.parity_util import compare
 
 pytestmark = pytest.mark.gpu
 
-PERSISTENT, PLAIN = 0, 1
+GRID, PLAIN, PERSISTENT = 0, 1, 2
 
 
 @pytest.fixture(scope="module")
@@ -71,7 +71,7 @@ def test_distance_matches_oracle(rt, oracle_mod):
     assert np.quantile(rel, 0.99) < 1e-4, np.quantile(rel, 0.99)
 
 
-@pytest.mark.parametrize("kernel", [PERSISTENT, PLAIN])
+@pytest.mark.parametrize("kernel", [GRID, PLAIN, PERSISTENT])
 def test_mandelbulb8_small_parity(rt, oracle_mod, kernel):
     p = scene.mandelbulb8(160, 120)
     g = rt.render_to_host(p, kernel=kernel)
@@ -88,9 +88,11 @@ def test_mandelbulb8_640x480_parity(rt, oracle_mod):
     st = compare(g, r)
     _report("mandelbulb8 640x480", st)
     _check(st)
-    # the work counters agree to within what the FP32 march changes
-    assert abs(g["de_evals"] - r["de_evals"]) / r["de_evals"] < 0.001
-    assert abs(g["iterations"] - r["iterations"]) / r["iterations"] < 0.001
+    # the work counters agree to within what the FP32 march changes; far-field steps (when
+    # enabled) are the reference's DE evaluations the device proves unnecessary, one iteration each
+    dev_evals = g["de_evals"] + g["far_steps"]
+    assert abs(dev_evals - r["de_evals"]) / r["de_evals"] < 0.001
+    assert abs(g["iterations"] + g["far_steps"] - r["iterations"]) / r["iterations"] < 0.001
 
 
 @pytest.mark.parametrize("scale", [2.0, -1.5])
@@ -159,10 +161,12 @@ def test_monte_carlo_gi_parity(rt, oracle_mod):
 
 
 @pytest.mark.parametrize("powers", [(6.0, 7.0, 9.5), (8.0, 8.0, 8.0)])
-def test_batch_equals_single_frames(rt, powers):
-    """Frames of different resolution, camera and power rendered in one persistent launch equal
-    their single-frame renders bit for bit.  (A batch mixing power 8 with other powers runs
-    every frame through the general-power kernel, so the two cases are kept apart here.)"""
+@pytest.mark.parametrize("kernel", [GRID, PLAIN, PERSISTENT])
+def test_batch_equals_single_frames(rt, powers, kernel):
+    """Frames of different resolution, camera and power rendered in one batch (one grid launch,
+    one plain launch per frame, or one persistent launch) equal their single-frame renders bit for
+    bit.  (A batch mixing power 8 with other powers runs every frame through the general-power
+    kernel, so the two cases are kept apart here.)"""
     import torch
     frames = []
     for k, (w, h) in enumerate([(96, 72), (64, 48), (120, 68)]):
@@ -171,6 +175,7 @@ def test_batch_equals_single_frames(rt, powers):
         abi.set3(p.camera, (0.3 * k, -3.0 - 0.2 * k, 0.5))
         frames.append(p)
     ctx = rt.Context(0)
+    ctx.tune(rt.MBX_TUNE_KERNEL, kernel)
     bufs = [ctx.alloc_outputs(p) for p in frames]
     ctx.render_batch(frames, bufs)
     st = ctx.stats()
@@ -265,6 +270,41 @@ def test_graph_replays_match_direct_launch(rt):
     ctx.close()
 
 
+@pytest.mark.parametrize("case", ["bulb8", "power2", "power12", "mc_gi"])
+def test_far_field_steps_leave_pixels_unchanged(rt, case):
+    """March steps beyond the far-field radius taken without a DE evaluation give bit-identical
+    planes to evaluating every step (the step is provably the clamped 3 there)."""
+    import torch
+    p = scene.mandelbulb8(160, 120)
+    if case == "power2":
+        p.fractal.bulb.power = 2.0
+    elif case == "power12":
+        p.fractal.bulb.power = 12.0
+    elif case == "mc_gi":
+        p = scene.mandelbulb8(96, 72)
+        p.DOFMonteCarlo = 1
+        p.DOFSamples = 4
+        p.DOFMonteCarloGlobalIllumination = 1
+        p.reflectionsMax = 2
+    ctx = rt.Context(0)
+    outs, stats = [], []
+    for on in (0, 1):
+        ctx.tune(rt.MBX_TUNE_FAR_FIELD, on)
+        ctx.upload(p)
+        b = ctx.alloc_outputs()
+        ctx.render(b)
+        stats.append(ctx.stats())
+        outs.append(b)
+    torch.cuda.synchronize()
+    for key in ("rgb", "depth", "iters", "mask", "normal", "steps"):
+        assert torch.equal(outs[0][key], outs[1][key]), key
+    assert stats[0]["far_steps"] == 0 and stats[1]["far_steps"] > 0
+    assert stats[1]["de_evals"] + stats[1]["far_steps"] == stats[0]["de_evals"]
+    print("\n[far field %s] %d of %d DE evaluations skipped" % (
+        case, stats[1]["far_steps"], stats[0]["de_evals"]), flush=True)
+    ctx.close()
+
+
 def test_unsupported_settings_are_rejected(rt):
     ctx = rt.Context(0)
     p = scene.mandelbulb8(64, 48)
@@ -282,7 +322,7 @@ def test_unsupported_settings_are_rejected(rt):
     ctx.close()
 
 
-@pytest.mark.parametrize("kernel", [PERSISTENT, PLAIN])
+@pytest.mark.parametrize("kernel", [GRID, PLAIN, PERSISTENT])
 def test_tile_shapes_give_identical_pixels(rt, kernel):
     """Tiles of 4x16, 16x4 and 32x2 pixels (one wave each) against the default 8x8 on a frame
     whose size is a multiple of none of them: every plane is bit-identical."""
