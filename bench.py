@@ -33,6 +33,9 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X published FP64 peak
 # (ray point, threshold, DE finish, step) and per formula iteration.
 F_DE = 25
 F_IT = {"mandelbulb8": 48, "mandelbox": 29, "animation64": 32, "deepzoom": 29}
+# A far-field step (no DE evaluation: the step is provably the clamped 3) does the ray point (6),
+# |p|^2 (5) and the scan update (2).
+F_FAR = 13
 
 WORKLOADS = {
     "mandelbulb8": {
@@ -94,6 +97,8 @@ def parse():
                     help="approximate CPU time budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain kernel")
+    ap.add_argument("--far-field", type=int, default=None, choices=[0, 1],
+                    help="MBX_TUNE_FAR_FIELD for the run (default: the library's default)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the launch-shape autotuner (tile shape x wave count) that runs "
                          "before timing on the mandelbulb8 workload")
@@ -278,6 +283,8 @@ def bench_batch(args, torch, runtime, wl):
     frames = make_scene(args.workload, 0, 0)
     stream = torch.cuda.current_stream(0)
     ctx = runtime.Context(0, stream)
+    if args.far_field is not None:
+        ctx.tune(runtime.MBX_TUNE_FAR_FIELD, args.far_field)
     bufs = [ctx.alloc_outputs(p) for p in frames]
     ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_GRID)
     wall, launch_ms = time_kernel(torch, ctx, None, stream, args.steps, args.warmup,
@@ -299,7 +306,7 @@ def bench_batch(args, torch, runtime, wl):
     ctx.close()
     ms_per_step = wall * 1e3 / args.steps
     avg_launch_s = sum(launch_ms) / len(launch_ms) / 1e3
-    flops = F_DE * st["de_evals"] + F_IT[args.workload] * st["iterations"]
+    flops = F_DE * st["de_evals"] + F_IT[args.workload] * st["iterations"] + F_FAR * st["far_steps"]
     achieved = flops / avg_launch_s / 1e12
     out = {
         "metric": wl["metric"], "value": st["rays"] / (ms_per_step / 1e3) / 1e6, "unit": "Mrays/s",
@@ -341,6 +348,8 @@ def main():
     p = make_scene(args.workload, args.width, args.height)
     stream = torch.cuda.current_stream(0)
     ctx = runtime.Context(0, stream)
+    if args.far_field is not None:
+        ctx.tune(runtime.MBX_TUNE_FAR_FIELD, args.far_field)
     ctx.upload(p)
     bufs = ctx.alloc_outputs()
     ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_GRID)
@@ -379,7 +388,7 @@ def main():
     rays = p.imageWidth * p.imageHeight
     ms_per_step = wall * 1e3 / args.steps
     avg_launch_s = sum(launch_ms) / len(launch_ms) / 1e3
-    flops = F_DE * st["de_evals"] + F_IT[args.workload] * st["iterations"]
+    flops = F_DE * st["de_evals"] + F_IT[args.workload] * st["iterations"] + F_FAR * st["far_steps"]
     traffic, traffic_src = measured_traffic(args.workload, p.imageWidth, p.imageHeight)
     achieved = flops / avg_launch_s / 1e12
     out = {
@@ -407,6 +416,7 @@ def main():
         "iterations_per_s": st["iterations"] / avg_launch_s,
         "de_evals_per_frame": st["de_evals"],
         "iterations_per_frame": st["iterations"],
+        "far_steps_per_frame": st["far_steps"],
     }
     if plain is not None:
         plain["batch_speedup"] = plain["ms_per_step"] / ms_per_step

@@ -311,10 +311,10 @@ int mbx_render_batch(mbx_ctx *ctx, const mbx_params *frames, const mbx_outputs *
                                 1 = forward-mode tangents (smooth limit of the same quotient) */
 #define MBX_TUNE_TILE_WIDTH 4 /* pixels per tile row: 4, 8 (default), 16 or 32; a tile is 64
                                  pixels (one wave), so its height is 64 / width */
-#define MBX_TUNE_FAR_FIELD 5  /* 1 = Mandelbulb march steps beyond the far-field radius, where the
-                                 first formula iteration bounds the DE above the value at which the
-                                 reference's step clamps to 3, take that step without evaluating the
-                                 DE (pixel results are identical); 0 (default) = evaluate every step */
+#define MBX_TUNE_FAR_FIELD 5  /* 1 (default) = Mandelbulb march steps beyond the far-field radius,
+                                 where the first formula iteration bounds the DE above the value at
+                                 which the reference's step clamps to 3, take that step without
+                                 evaluating the DE (pixel results are identical); 0 = evaluate */
 
 /* Set a launch or numerics option (see MBX_TUNE_*); scene options apply from the next
  * mbx_scene_upload / mbx_render_batch.  No reference counterpart: the OpenCL engine's

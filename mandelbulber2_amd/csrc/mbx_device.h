@@ -64,7 +64,8 @@ __device__ __forceinline__ int random_int(uint32_t &seed)
 // except the AO sample pick.
 __device__ __forceinline__ int random_mod(uint32_t &seed, int max)
 {
-	return random_int(seed) % (max + 1);
+	// RandomInt() is in [1, 2^31 - 2]: the unsigned remainder equals the reference's signed one
+	return (int)((uint32_t)random_int(seed) % (uint32_t)(max + 1));
 }
 
 // ------------------------------------------------------------------------------------------
@@ -528,20 +529,17 @@ __device__ __forceinline__ float compute_analytic(V3 p, const FrameConst &f, con
 	}
 	iters = i + 1;
 	const float r = fsqrt(r2);
+	// DE finish (compute_fractal.cpp:561-617) as selects rather than branches: every lane of a
+	// wave takes it, and the log and reciprocal are needed by almost all of them
+	const float invDE = frcp(DE);
 	float distance;
-	if (DE > 0.0f)
-	{
-		const float invDE = frcp(DE);
-		if (KIND == kBulb8 || w.analyticFunction == MBX_ANALYTIC_LOGARITHMIC)
-			distance = (r > 1.0f) ? 0.5f * r * flog(r) * invDE : 0.0f;
-		else if (w.analyticFunction == MBX_ANALYTIC_IFS)
-			distance = (r - 2.0f) * invDE;
-		else
-			distance = r * invDE;
-	}
+	if (KIND == kBulb8 || w.analyticFunction == MBX_ANALYTIC_LOGARITHMIC)
+		distance = (r > 1.0f) ? 0.5f * r * flog(r) * invDE : 0.0f;
+	else if (w.analyticFunction == MBX_ANALYTIC_IFS)
+		distance = (r - 2.0f) * invDE;
 	else
-		distance = r;
-	return distance;
+		distance = r * invDE;
+	return (DE > 0.0f) ? distance : r;
 }
 
 // Position precision of a DE kind.  Mandelbox kinds march in FP64 (ray position, scan, normal
@@ -671,7 +669,7 @@ __device__ __forceinline__ typename Prec<PosPrecision<KIND>::kFP64>::R calculate
 		itersMinN = iters;
 	}
 	w.de_evals++;
-	if (itersMinN < w.minN && d < detailSize) d = detailSize;
+	if (w.minN > 1 && itersMinN < w.minN && d < detailSize) d = detailSize; // iters >= 1 always
 	// clamp to [0, 10] as the reference does; a NaN passes the clamp and the boolean-add merge
 	// of the objects tree then keeps its initial 1e20
 	if constexpr (PosPrecision<KIND>::kFP64)

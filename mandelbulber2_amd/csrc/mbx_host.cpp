@@ -54,7 +54,7 @@ struct mbx_ctx
 	int graphKind = -1, graphFrames = -1, graphWaves = -1, graphTile = -1, graphFull = -1, graphMode = -1;
 	unsigned int graphTiles = 0;
 	int tileShift = 3; // 8x8-pixel tiles
-	int farField = 0;  // far-field march steps without DE evaluation (MBX_TUNE_FAR_FIELD)
+	int farField = 1;  // far-field march steps without DE evaluation (MBX_TUNE_FAR_FIELD)
 	int batchFull = 1; // the staged frames need the Monte Carlo / GI / shadow / AO code
 	// differentiable render
 	mbx::DiffConst diff{};
