@@ -64,8 +64,9 @@ enum mbx_analytic_function {
 	MBX_ANALYTIC_IFS = 3
 };
 
-/* enumAOMode (mandelbulber2/src/ao_modes.h:42-44) */
-enum mbx_ao_mode { MBX_AO_FAST = 0, MBX_AO_MULTIPLE_RAYS = 1 };
+/* enumAOMode (mandelbulber2/src/ao_modes.h:42-44); screen-space AO is a post effect and is not
+ * rendered by this library */
+enum mbx_ao_mode { MBX_AO_FAST = 0, MBX_AO_MULTIPLE_RAYS = 1, MBX_AO_SCREEN_SPACE = 2 };
 
 /* RNG stream layout.  PER_PIXEL: one Park-Miller stream per pixel, seeded from
  * (pixel index, random_seed); identical on CPU and GPU.  GLOBAL_SEQUENTIAL: one stream for the
@@ -369,6 +370,66 @@ int mbx_render_diff(mbx_ctx *ctx, const mbx_outputs *out, mbx_ray_checkpoint *ch
  * the background along the view vector.  Synchronises the stream. */
 int mbx_adjoint(mbx_ctx *ctx, const mbx_ray_checkpoint *checkpoints, const float *dL_drgb, double *grad,
 	double *pixel_grad);
+
+/* ---- parameter defaults and .fract scene files ------------------------------------------- */
+
+/* Fill *p with the reference's defaults: the values InitParams, InitFractalParams,
+ * InitMaterialParams(1) and InitLightParams(1) register (mandelbulber2/src/initparameters.cpp),
+ * with colours as toRGBFloat converts them (c / 65536, color_structures.hpp:106-109).  Host only;
+ * needs no context or device.  The multi-ray AO direction set is left empty (aoVectorsCount 0). */
+void mbx_params_default(mbx_params *p);
+
+/* Effects a scene file turns on that this library does not render (mbx_fract_info.unsupported).
+ * A scene loads with them regardless; the image is the path's image without them. */
+#define MBX_FRACT_HYBRID 0x1u          /* hybrid sequence, or a boolean tree of several objects */
+#define MBX_FRACT_PRIMITIVES 0x2u      /* primitive objects (plane, sphere, water, ...) */
+#define MBX_FRACT_TRANSFORM 0x4u       /* node position, scale or repeat; the limits box */
+#define MBX_FRACT_PALETTE 0x8u         /* surface colour from the palette gradient */
+#define MBX_FRACT_SPECULAR 0x10u       /* specular highlights (material specular > 0) */
+#define MBX_FRACT_GLOW 0x20u           /* glow (volumetric shader) */
+#define MBX_FRACT_FOG 0x40u            /* basic, volumetric, iteration or distance fog; clouds */
+#define MBX_FRACT_REFLECTIONS 0x80u    /* ray-traced reflections or transparency */
+#define MBX_FRACT_LIGHTS 0x100u        /* auxiliary, random, fake or volumetric lights; light #1 not
+                                          directional */
+#define MBX_FRACT_TEXTURES 0x200u      /* textured background, environment map, material textures */
+#define MBX_FRACT_AO_SCREEN_SPACE 0x400u /* screen-space AO (a post effect) */
+#define MBX_FRACT_AO_LIGHTMAP 0x800u   /* multi-ray AO: its directions and colours come from the
+                                          lightmap texture (aoVectorsCount is left 0) */
+#define MBX_FRACT_POSTPROCESS 0x1000u  /* brightness, contrast, gamma, saturation, HDR, blur, post DOF */
+#define MBX_FRACT_MC_ADAPTIVE 0x2000u  /* Monte Carlo stops early (DOF_min_samples < DOF_samples) */
+#define MBX_FRACT_PERSPECTIVE 0x4000u  /* fish-eye or equirectangular projection, stereo */
+#define MBX_FRACT_FORMULA 0x8000u      /* a formula outside Mandelbulb, Mandelbox, Kaleidoscopic IFS */
+#define MBX_FRACT_ANIMATION 0x10000u   /* [frames] / [keyframes] sections (not read) */
+#define MBX_FRACT_MATERIAL 0x20000u    /* other material effects: iridescence, roughness, subsurface,
+                                          perlin noise, luminosity gradient */
+#define MBX_FRACT_INTERIOR 0x40000u    /* interior mode or iteration-threshold mode */
+#define MBX_FRACT_DISABLED 0x80000u    /* the fractal node is disabled or absent: nothing to render */
+#define MBX_FRACT_ANTIALIASING 0x100000u /* antialiasing repeats (antialiasing_size^2 per pixel) */
+
+/* What mbx_load_fract found besides the parameter block. */
+typedef struct mbx_fract_info {
+	double fileVersion;    /* "# version x.yy" of the header */
+	int32_t format;        /* 0 "all parameters", 1 "only modified parameters" */
+	int32_t lines;         /* parameter lines decoded */
+	int32_t errors;        /* malformed lines (no space, no ';', or the ';' before the space) */
+	int32_t formula;       /* enumFractalFormula id of the rendered fractal node */
+	uint32_t unsupported;  /* MBX_FRACT_* */
+	int32_t reserved;
+	char message[256];     /* the first error ("" if none) */
+} mbx_fract_info;
+
+/* Read a Mandelbulber settings file (the text of a .fract file, `len` bytes, UTF-8) into *params:
+ * the reference's defaults (mbx_params_default) overwritten by the file's [main_parameters] and
+ * [fractal_N] lines, with the version-dependent renames and migrations of cSettings
+ * (mandelbulber2/src/settings.cpp:513-977, 1015-1245, 1262-1492, 1494-1754, 2551-2760) that concern
+ * these parameters, and the objects tree reduced to its one fractal node (objects_tree.cpp:36-96).
+ * Image size, camera, light #1, material of the node, AO, background, Monte Carlo and the
+ * formula's parameters are read; everything else is reported in info->unsupported when it changes
+ * the image.  Returns MBX_OK, or MBX_ERR_INVALID when the header is not a Mandelbulber settings
+ * header, the file is newer than version 2.35, it holds application settings, or more than three
+ * lines are malformed (the reference's limit, settings.cpp:888-901); info->message says which.
+ * `info` may be NULL.  Host only; needs no context or device. */
+int mbx_load_fract(const char *text, size_t len, mbx_params *params, mbx_fract_info *info);
 
 #ifdef __cplusplus
 }

@@ -18,6 +18,7 @@ MBX_FORMULA_KALEIDOSCOPIC_IFS = 10
 
 MBX_AO_FAST = 0
 MBX_AO_MULTIPLE_RAYS = 1
+MBX_AO_SCREEN_SPACE = 2
 
 MBX_RNG_PER_PIXEL = 0
 MBX_RNG_GLOBAL_SEQUENTIAL = 1
@@ -191,6 +192,33 @@ class MbxStats(C.Structure):
         ("kernel_ms", C.c_double),
         ("far_steps", C.c_uint64),
     ]
+
+
+# effects a .fract file turns on that the library does not render (mbx_fract_info.unsupported)
+FRACT_FLAGS = {
+    "hybrid": 0x1, "primitives": 0x2, "transform": 0x4, "palette": 0x8, "specular": 0x10,
+    "glow": 0x20, "fog": 0x40, "reflections": 0x80, "lights": 0x100, "textures": 0x200,
+    "ao_screen_space": 0x400, "ao_lightmap": 0x800, "postprocess": 0x1000, "mc_adaptive": 0x2000,
+    "perspective": 0x4000, "formula": 0x8000, "animation": 0x10000, "material": 0x20000,
+    "interior": 0x40000, "disabled": 0x80000, "antialiasing": 0x100000,
+}
+
+
+class MbxFractInfo(C.Structure):
+    _fields_ = [
+        ("fileVersion", C.c_double),
+        ("format", C.c_int32),
+        ("lines", C.c_int32),
+        ("errors", C.c_int32),
+        ("formula", C.c_int32),
+        ("unsupported", C.c_uint32),
+        ("reserved", C.c_int32),
+        ("message", C.c_char * 256),
+    ]
+
+
+def fract_flag_names(mask):
+    return [k for k, v in FRACT_FLAGS.items() if mask & v]
 
 
 def set3(arr, values):

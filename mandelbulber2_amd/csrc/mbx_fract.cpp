@@ -1,0 +1,965 @@
+// mbx_fract.cpp - reference parameter defaults and the .fract scene reader (include/mbx.h).
+//
+// The reader follows what cSettings::LoadFromString + Decode do with a settings file
+// (mandelbulber2/src/settings.cpp) for the parameters the raymarch path consumes:
+//   header           DecodeHeader            settings.cpp:540-601
+//   sections, lines  Decode / CheckSection   settings.cpp:647-905, 1247-1256
+//   one line         DecodeOneLine           settings.cpp:1015-1245
+//   renames          Compatibility           settings.cpp:1262-1492
+//   migrations       Compatibility2          settings.cpp:1494-1754
+//   legacy fractals  TryResolveLegacyFractalParam, MigrateToObjectsTree
+//                                            settings.cpp:2268-2322, 2551-2760
+//   objects tree     cObjectsTree::CreateNodeDataFromParameters  objects_tree.cpp:36-96
+// Values are typed as cMultiVal stores them (multi_val.cpp:87-123): booleans are "true" or not,
+// decimals take ',' or '.', colours are 16-bit hexadecimal triples, and numeric parameters are
+// clamped to their registered limits (one_parameter.cpp:112-150).  The file is parsed into
+// per-section key/value maps (the last line for a key wins, as repeated Set calls do); the typed
+// getters apply each parameter's default, type and limits.  Unknown parameter names are not
+// errors here (the reference rejects names it has not registered; this reader knows only the
+// names on the path), so only malformed lines count toward the three-error limit.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "../../include/mbx.h"
+
+namespace
+{
+
+// ------------------------------------------------------------------------------------------
+// string helpers (QString semantics where they matter)
+// ------------------------------------------------------------------------------------------
+bool is_space(char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == '\v' || c == '\f'; }
+
+std::string trim(const std::string &s)
+{
+	size_t a = 0, b = s.size();
+	while (a < b && is_space(s[a])) a++;
+	while (b > a && is_space(s[b - 1])) b--;
+	return s.substr(a, b - a);
+}
+
+bool starts_with(const std::string &s, const char *p) { return s.compare(0, std::strlen(p), p) == 0; }
+bool ends_with(const std::string &s, const char *p)
+{
+	const size_t n = std::strlen(p);
+	return s.size() >= n && s.compare(s.size() - n, n, p) == 0;
+}
+bool contains(const std::string &s, const char *p) { return s.find(p) != std::string::npos; }
+
+// QString::split(c) with empty parts kept
+std::vector<std::string> split(const std::string &s, char c)
+{
+	std::vector<std::string> out;
+	size_t a = 0;
+	for (;;)
+	{
+		const size_t b = s.find(c, a);
+		if (b == std::string::npos)
+		{
+			out.push_back(s.substr(a));
+			return out;
+		}
+		out.push_back(s.substr(a, b - a));
+		a = b + 1;
+	}
+}
+
+void replace_all(std::string &s, const char *from, const char *to)
+{
+	const size_t n = std::strlen(from), m = std::strlen(to);
+	if (n == 0) return;
+	for (size_t pos = s.find(from); pos != std::string::npos; pos = s.find(from, pos + m))
+		s.replace(pos, n, to);
+}
+
+// QString::toInt(&ok) in base 10 or 16: the whole (trimmed) string must be a number
+bool parse_int(const std::string &t, int base, long long &out)
+{
+	const std::string s = trim(t);
+	if (s.empty()) return false;
+	char *end = nullptr;
+	const long long v = std::strtoll(s.c_str(), &end, base);
+	if (end == s.c_str() || *end != '\0') return false;
+	if (v < -2147483648LL || v > 2147483647LL) return false;
+	out = v;
+	return true;
+}
+long long to_int(const std::string &s, int base = 10)
+{
+	long long v = 0;
+	return parse_int(s, base, v) ? v : 0;
+}
+
+// QLocale::toDouble after everyLocaleDouble (settings.cpp:2063-2072): ',' and '.' both decimal
+// points; the whole string must be a number, otherwise 0
+double to_double(std::string s)
+{
+	s = trim(s);
+	for (char &c : s)
+		if (c == ',') c = '.';
+	if (s.empty()) return 0.0;
+	char *end = nullptr;
+	const double v = std::strtod(s.c_str(), &end);
+	if (end == s.c_str() || *end != '\0') return 0.0;
+	return v;
+}
+
+// ------------------------------------------------------------------------------------------
+// the decoded file
+// ------------------------------------------------------------------------------------------
+using KV = std::map<std::string, std::string>;
+
+struct Doc
+{
+	double version = 0.0;
+	int format = 0;
+	KV main;                    // [main_parameters], after renames
+	std::map<int, KV> fractal;  // [fractal_N] and legacy name_N lines, keyed by N (1-based)
+	int lines = 0, errors = 0;
+	bool animation = false;
+	std::string firstError;
+	std::set<int> materialsSeen; // matN_* names decoded (lazy InitMaterialParams)
+	std::set<int> lightsSeen;    // lightN_* names decoded (lazy InitLightParams)
+	bool primitives = false;
+	std::set<int> nodeIds;       // node_XXXX_definition names
+};
+
+// typed views of one container with defaults
+struct View
+{
+	const KV *kv;
+	bool has(const std::string &k) const { return kv && kv->count(k); }
+	const std::string *raw(const std::string &k) const
+	{
+		if (!kv) return nullptr;
+		auto it = kv->find(k);
+		return it == kv->end() ? nullptr : &it->second;
+	}
+	bool b(const std::string &k, bool def) const
+	{
+		const std::string *v = raw(k);
+		return v ? *v == "true" : def; // DecodeOneLine: value == "true" ? 1 : 0
+	}
+	long long i(const std::string &k, long long def, long long lo = -2147483648LL, long long hi = 2147483647LL) const
+	{
+		const std::string *v = raw(k);
+		if (!v) return def;
+		return std::min(std::max(to_int(*v), lo), hi);
+	}
+	double d(const std::string &k, double def, double lo = -INFINITY, double hi = INFINITY) const
+	{
+		const std::string *v = raw(k);
+		if (!v) return def;
+		return std::min(std::max(to_double(*v), lo), hi);
+	}
+	// vectors: components past the last one given keep their default (cMultiVal::Store(QString)
+	// assigns only the parts present)
+	void v3(const std::string &k, double out[3], double d0, double d1, double d2) const
+	{
+		out[0] = d0;
+		out[1] = d1;
+		out[2] = d2;
+		const std::string *v = raw(k);
+		if (!v) return;
+		std::vector<std::string> parts = split(*v, ' ');
+		for (size_t j = 0; j < parts.size() && j < 3; j++) out[j] = to_double(parts[j]);
+	}
+	// colours: 16-bit hexadecimal components, clamped to [0, 65535], returned as c / 65536
+	void rgb(const std::string &k, double out[3], int r, int g, int bl) const
+	{
+		long long c[3] = {r, g, bl};
+		const std::string *v = raw(k);
+		if (v)
+		{
+			std::vector<std::string> parts = split(*v, ' ');
+			for (size_t j = 0; j < parts.size() && j < 3; j++)
+				c[j] = std::min(std::max(to_int(parts[j], 16), 0LL), 65535LL);
+		}
+		for (int j = 0; j < 3; j++) out[j] = (double)(float)(c[j] / 65536.0);
+	}
+	// enumeration parameters: a name from the list or its index (cOneParameter::GetIndexByEnumeration)
+	long long e(const std::string &k, long long def, const std::vector<const char *> &names) const
+	{
+		const std::string *v = raw(k);
+		if (!v) return def;
+		for (size_t j = 0; j < names.size(); j++)
+			if (*v == names[j]) return (long long)j;
+		return to_int(*v);
+	}
+};
+
+// Compatibility (settings.cpp:1262-1492): renames and value rewrites by file version.  Only the
+// branches that reach parameters this reader uses or reports are restated.
+void compatibility(double ver, std::string &name, std::string &value)
+{
+	if (ver <= 2.01)
+	{
+		if (name == "volumetric_light_enabled_0") name = "main_light_volumetric_enabled";
+		else if (name == "volumetric_light_intensity_0") name = "main_light_volumetric_intensity";
+		else if (contains(name, "volumetric_light")) replace_all(name, "volumetric_light", "aux_light_volumetric");
+		if (contains(name, "aux_light_predefined")) replace_all(name, "aux_light_predefined", "aux_light");
+	}
+	if (ver <= 2.04 && name == "fractal_constant_factor") value = value + " " + value + " " + value;
+	if (ver <= 2.06 && name == "linear_DE_mode")
+	{
+		name = "delta_DE_function";
+		value = "1";
+	}
+	if (ver <= 2.071)
+	{
+		static const char *const materialNames[][2] = {{"shading", "mat1_shading"},
+			{"specular", "mat1_specular"}, {"reflect", "mat1_reflectance"},
+			{"transparency_of_surface", "mat1_transparency_of_surface"},
+			{"transparency_of_interior", "mat1_transparency_of_interior"},
+			{"transparency_index_of_refraction", "mat1_transparency_index_of_refraction"},
+			{"transparency_interior_color", "mat1_transparency_interior_color"},
+			{"fresnel_reflectance", "mat1_fresnel_reflectance"}, {"coloring_random_seed", "mat1_coloring_random_seed"},
+			{"coloring_saturation", "mat1_coloring_saturation"}, {"coloring_speed", "mat1_coloring_speed"},
+			{"coloring_palette_size", "mat1_coloring_palette_size"},
+			{"coloring_palette_offset", "mat1_coloring_palette_offset"},
+			{"fractal_color", "mat1_use_colors_from_palette"},
+			{"surface_color_palette", "mat1_surface_color_palette"},
+			{"fractal_coloring_algorithm", "mat1_fractal_coloring_algorithm"},
+			{"fractal_coloring_sphere_radius", "mat1_fractal_coloring_sphere_radius"},
+			{"fractal_coloring_line_direction", "mat1_fractal_coloring_line_direction"}};
+		for (const auto &m : materialNames)
+			if (name == m[0])
+			{
+				name = m[1];
+				break;
+			}
+	}
+	if (ver < 2.09 && name == "delta_DE_function")
+	{
+		if (value == "0") value = "2";
+		else if (value == "2") value = "0";
+	}
+	if (ver < 2.12) replace_all(name, "gpu_", "opencl_");
+	if (ver < 2.19)
+	{
+		if (contains(name, "surface_color_palette")) replace_all(name, "surface_color_palette", "surface_color_gradient");
+		if (contains(name, "luminosity_color_thesame"))
+			replace_all(name, "luminosity_color_thesame", "luminosity_gradient_enable");
+		if (contains(name, "coloring_palette_size") || contains(name, "coloring_random_seed") ||
+			contains(name, "coloring_saturation"))
+			name = "skip";
+	}
+	if (ver < 2.25)
+	{
+		static const char *const lightNames[][2] = {{"main_light_intensity", "light1_intensity"},
+			{"main_light_visibility_size", "light1_size"},
+			{"main_light_contour_sharpness", "light1_contour_sharpness"}, {"main_light_alpha", "light1_alpha"},
+			{"main_light_beta", "light1_beta"}, {"main_light_colour", "light1_color"},
+			{"penetrating_lights", "light1_penetrating"}, {"shadows_enabled", "light1_cast_shadows"},
+			{"shadows_cone_angle", "light1_soft_shadow_cone"}, {"main_light_enable", "light1_enabled"},
+			{"main_light_position_relative", "light1_relative_position"},
+			{"main_light_volumetric_intensity", "light1_volumetric_visibility"},
+			{"main_light_volumetric_enabled", "light1_volumetric"}};
+		if (name == "main_light_visibility") name = "light1_visibility";
+		for (const auto &m : lightNames)
+			if (contains(name, m[0])) replace_all(name, m[0], m[1]);
+		if (contains(name, "aux_light"))
+		{
+			if (contains(name, "visibility_size")) replace_all(name, "aux_light_visibility_size", "light2_size");
+			else if (contains(name, "visibility")) replace_all(name, "aux_light_visibility", "light2_visibility");
+			else if (!contains(name, "aux_light_place_behind"))
+			{
+				// aux_light_<what>_<i> -> light<i+1>_<what>
+				std::vector<std::string> parts = split(name, '_');
+				if (parts.size() >= 4)
+				{
+					const std::string prefix = "light" + std::to_string(to_int(parts.back()) + 1);
+					if (parts[2] == "intensity") name = prefix + "_intensity";
+					else if (parts[2] == "position") name = prefix + "_position";
+					else if (parts[2] == "enabled") name = prefix + "_enabled";
+					else if (parts[2] == "colour") name = prefix + "_color";
+					else if (parts[2] == "volumetric" && parts.size() >= 5)
+					{
+						if (parts[3] == "intensity") name = prefix + "_volumetric_visibility";
+						else if (parts[3] == "enabled") name = prefix + "_volumetric";
+					}
+				}
+			}
+		}
+	}
+}
+
+// Parameters of the per-fractal containers that this reader reads (InitFractalParams,
+// initparameters.cpp:675-768); a legacy main-section line "name_N" with one of these names goes
+// to fractal N (TryResolveLegacyFractalParam, settings.cpp:2268-2322).
+bool is_fractal_param(const std::string &base)
+{
+	static const std::set<std::string> names = {"formula", "formula_iterations", "formula_weight",
+		"formula_start_iteration", "formula_stop_iteration", "fractal_enable", "dont_add_c_constant",
+		"check_for_bailout", "power", "alpha_angle_offset", "beta_angle_offset", "gamma_angle_offset",
+		"IFS_scale", "IFS_rotation", "IFS_rotation_enabled", "IFS_offset", "IFS_edge", "IFS_edge_enabled",
+		"IFS_abs_x", "IFS_abs_y", "IFS_abs_z", "IFS_menger_sponge_mode", "mandelbox_scale",
+		"mandelbox_folding_limit", "mandelbox_folding_value", "mandelbox_folding_min_radius",
+		"mandelbox_folding_fixed_radius", "mandelbox_sharpness", "mandelbox_offset", "mandelbox_rotation_main",
+		"mandelbox_color", "mandelbox_color_Sp1", "mandelbox_color_Sp2", "mandelbox_main_rotation_enabled",
+		"mandelbox_rotations_enabled", "formula_position", "formula_rotation", "formula_repeat", "formula_scale",
+		"formula_material_id"};
+	return names.count(base) != 0;
+}
+
+// main-section names of the form base_N that are main parameters, not legacy fractal lines:
+// the indexed temporaries InjectTemporaryLegacyFractalParams and InjectTemporaryLegacyBooleanParams
+// add (settings.cpp:2128-2172; dont_add_c_constant_N for N = 1..8)
+bool is_main_indexed(const std::string &base, long long n)
+{
+	return base == "julia_mode" || base == "julia_c" || base == "fractal_constant_factor" ||
+		base == "formula_maxiter" || base == "initial_waxis" || (base == "dont_add_c_constant" && n <= 8);
+}
+
+bool decode_one_line(Doc &doc, const std::string &lineIn, KV *container, bool mainSection, int fractalIndex)
+{
+	const std::string line = trim(lineIn);
+	if (line.empty()) return true;
+	const size_t firstSpace = line.find(' ');
+	const size_t semicolon = line.find(';');
+	if (firstSpace == std::string::npos || semicolon == std::string::npos || semicolon <= firstSpace)
+		return false;
+	std::string name = trim(line.substr(0, firstSpace));
+	std::string value = trim(line.substr(firstSpace + 1, semicolon - firstSpace - 1));
+	auto strip = [](const std::string &t) -> std::string {
+		if (t.size() >= 2 && ((t.front() == '"' && t.back() == '"') || (t.front() == '\'' && t.back() == '\'')))
+			return trim(t.substr(1, t.size() - 2));
+		return t;
+	};
+	name = strip(name);
+	value = strip(value);
+	doc.lines++;
+
+	if (mainSection && doc.version < 2.35)
+	{
+		// legacy per-fractal line "name_N" in the main section
+		const size_t us = name.rfind('_');
+		long long idx = 0;
+		if (us != std::string::npos && us > 0 && parse_int(name.substr(us + 1), 10, idx) && idx >= 1)
+		{
+			const std::string base = name.substr(0, us);
+			if (!is_main_indexed(base, idx) && is_fractal_param(base))
+			{
+				std::string n = base, v = value;
+				compatibility(doc.version, n, v);
+				if (n != "skip") doc.fractal[(int)idx][n] = v;
+				return true;
+			}
+		}
+	}
+
+	compatibility(doc.version, name, value);
+	if (name == "skip") return true;
+
+	if (mainSection)
+	{
+		if (starts_with(name, "primitive_")) doc.primitives = true;
+		if (starts_with(name, "mat"))
+		{
+			const size_t us = name.find('_');
+			long long idx = 0;
+			if (us != std::string::npos && parse_int(name.substr(3, us - 3), 10, idx))
+			{
+				if (!doc.materialsSeen.count((int)idx) && doc.version < 2.15)
+				{
+					// PreCompatibilityMaterials (settings.cpp:2107-2119): pre-2.15 material defaults,
+					// set when the material is first seen; later lines override them
+					const std::string p = "mat" + std::to_string(idx) + "_";
+					container->emplace(p + "metallic", "false");
+					container->emplace(p + "specular", "1");
+					container->emplace(p + "specular_width", "1");
+				}
+				doc.materialsSeen.insert((int)idx);
+			}
+		}
+		if (starts_with(name, "light"))
+		{
+			const size_t us = name.find('_');
+			long long idx = 0;
+			if (us != std::string::npos && parse_int(name.substr(5, us - 5), 10, idx)) doc.lightsSeen.insert((int)idx);
+		}
+		if (starts_with(name, "node_") && ends_with(name, "_definition"))
+		{
+			std::vector<std::string> parts = split(name, '_');
+			long long id = 0;
+			if (parts.size() >= 3 && parse_int(parts[1], 10, id) && id > 0) doc.nodeIds.insert((int)id);
+		}
+	}
+	(void)fractalIndex;
+	(*container)[name] = value;
+	return true;
+}
+
+int fail(mbx_fract_info *info, const std::string &msg)
+{
+	if (info) std::snprintf(info->message, sizeof(info->message), "%s", msg.c_str());
+	return MBX_ERR_INVALID;
+}
+
+// ------------------------------------------------------------------------------------------
+// defaults (initparameters.cpp line numbers)
+// ------------------------------------------------------------------------------------------
+void set3(double *o, double a, double b, double c)
+{
+	o[0] = a;
+	o[1] = b;
+	o[2] = c;
+}
+void rgb16(double *o, int r, int g, int b)
+{
+	o[0] = (double)(float)(r / 65536.0);
+	o[1] = (double)(float)(g / 65536.0);
+	o[2] = (double)(float)(b / 65536.0);
+}
+
+} // namespace
+
+extern "C" void mbx_params_default(mbx_params *p)
+{
+	if (!p) return;
+	std::memset(p, 0, sizeof(*p));
+	p->imageWidth = 800;                   // :64
+	p->imageHeight = 600;                  // :65
+	set3(p->camera, 3.0, -6.0, 2.0);       // :113
+	set3(p->target, 0.0, 0.0, 0.0);        // :114
+	set3(p->topVector, -0.1277753, 0.2555506, 0.958314); // :116
+	p->fov = 53.13;                        // :120
+	p->perspectiveType = 0;                // :121 three_point
+	p->sweetSpotHAngle = 0.0;              // :124
+	p->sweetSpotVAngle = 0.0;              // :125
+	p->minN = 1;                           // :131
+	p->detailLevel = 1.0;                  // :133
+	p->DEThresh = 0.01;                    // :135
+	p->smoothness = 1.0;                   // :136
+	p->iterThreshMode = 0;                 // :137
+	p->DEFactor = 1.0;                     // :139
+	p->viewDistanceMax = 50.0;             // :141
+	p->interiorMode = 0;                   // :147
+	p->constantDEThreshold = 0;            // :148
+	p->maxRaymarchingSteps = 10000;        // :167
+	// ambient occlusion :213-219
+	p->ambientOcclusion = 1.0;
+	p->ambientOcclusionQuality = 4;
+	p->ambientOcclusionFastTune = 1.0;
+	p->ambientOcclusionEnabled = 0;
+	p->ambientOcclusionMode = MBX_AO_SCREEN_SPACE;
+	rgb16(p->ambientOcclusionColor, 65535, 65535, 65535);
+	p->aoVectorsCount = 0;
+	// background :227-245
+	p->backgroundBrightness = 1.0;
+	p->backgroundGamma = 1.0;
+	rgb16(p->backgroundColor1, 0, 38306, 65535);
+	rgb16(p->backgroundColor2, 65535, 65535, 65535);
+	rgb16(p->backgroundColor3, 0, 10000, 500);
+	p->background3ColorsEnable = 1;
+	p->reflectionsMax = 5;                 // :237
+	rgb16(p->fillLightColor, 0, 0, 0);     // :309
+	// Monte Carlo :329-340
+	p->DOFMonteCarlo = 0;
+	p->DOFSamples = 100;
+	p->DOFMonteCarloGlobalIllumination = 0;
+	p->monteCarloGIRadianceLimit = 10.0;
+	p->rngMode = MBX_RNG_PER_PIXEL;
+	p->randomSeed = 0;
+	// material #1 (InitMaterialParams, :1600-1810)
+	rgb16(p->surfaceColor, 50000, 50000, 50000);
+	p->shading = 1.0;
+	p->luminosity = 0.0;
+	rgb16(p->luminosityColor, 65535, 65535, 65535);
+	// light #1 (InitLightParams, :1841-1960)
+	mbx_light &L = p->light;
+	L.enabled = 1;
+	L.castShadows = 1;
+	L.penetrating = 1;
+	L.relativePosition = 1;
+	set3(L.rotation, -45.0, 45.0, 0.0);
+	L.intensity = 1.0;
+	L.visibility = 1.0;
+	L.size = 0.5;
+	L.softShadowCone = 1.0;
+	L.contourSharpness = 1.0;
+	rgb16(L.color, 65535, 65535, 65535);
+	// fractal (InitParams :128-156 for the sequence-wide values, InitFractalParams :695-768)
+	mbx_fractal &f = p->fractal;
+	f.formula = MBX_FORMULA_MANDELBULB;    // :695 (container fractal0)
+	f.formulaMaxiter = 250;                // node formula_maxiter :2123, N :130
+	f.deltaDEMethod = 0;                   // :154
+	f.deltaDEFunction = 0;                 // :152
+	f.useDefaultBailout = 1;               // :155
+	f.bailout = 100.0;                     // :150
+	f.juliaMode = 0;                       // :128
+	set3(f.juliaC, 0.0, 0.0, 0.0);         // :129
+	set3(f.constantFactor, 1.0, 1.0, 1.0); // :132
+	f.initialWAxis = 0.0;                  // :156
+	f.dontAddCConstant = 0;                // :712
+	f.bulb.power = 9.0;                    // :717
+	mbx_fractal_mandelbox &mb = f.mandelbox;
+	mb.scale = 2.0;                        // :746
+	mb.foldingLimit = 1.0;
+	mb.foldingValue = 2.0;
+	mb.foldingSphericalMin = 0.5;
+	mb.foldingSphericalFixed = 1.0;        // :750
+	set3(mb.colorFactor, 0.03, 0.05, 0.07); // :761
+	mb.colorFactorSp1 = 0.2;               // :765
+	mb.colorFactorSp2 = 0.2;
+	mbx_fractal_ifs &ifs = f.IFS;
+	ifs.scale = 2.0;                       // :725
+	set3(ifs.offset, 1.0, 0.0, 0.0);       // :728
+	for (int i = 0; i < MBX_IFS_VECTOR_COUNT; i++)
+	{
+		set3(ifs.direction[i], 1.0, 0.0, 0.0); // :738-742
+		ifs.intensity[i] = 1.0;
+	}
+}
+
+extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, mbx_fract_info *info)
+{
+	if (info)
+	{
+		std::memset(info, 0, sizeof(*info));
+		info->formula = MBX_FORMULA_NONE;
+	}
+	if (!text || !params) return fail(info, "null argument");
+
+	// split into lines (settings.cpp:656-666: trimmed text split on \n, \r\n or \r)
+	const std::string all = trim(std::string(text, len));
+	std::vector<std::string> lines;
+	{
+		size_t a = 0;
+		for (size_t i = 0; i <= all.size(); i++)
+		{
+			if (i == all.size() || all[i] == '\n' || all[i] == '\r')
+			{
+				lines.push_back(all.substr(a, i - a));
+				if (i < all.size() && all[i] == '\r' && i + 1 < all.size() && all[i + 1] == '\n') i++;
+				a = i + 1;
+			}
+		}
+	}
+
+	// header (DecodeHeader, settings.cpp:540-601)
+	Doc doc;
+	if (lines.size() < 3) return fail(info, "not a Mandelbulber settings file: no header");
+	if (!contains(lines[0], "Mandelbulber settings file"))
+		return fail(info, "not a Mandelbulber settings file: wrong header");
+	{
+		const size_t pos = lines[1].rfind("version");
+		if (pos == std::string::npos) return fail(info, "not a Mandelbulber settings file: no version");
+		std::string num = trim(lines[1].substr(pos + 7));
+		char *end = nullptr;
+		const double v = std::strtod(num.c_str(), &end);
+		if (num.empty() || end == num.c_str() || *end != '\0')
+			return fail(info, "not a Mandelbulber settings file: wrong version number");
+		doc.version = v;
+	}
+	if (doc.version > 2.35) return fail(info, "file saved by a newer Mandelbulber (version > 2.35)");
+	if (contains(lines[2], "all parameters")) doc.format = 0;
+	else if (contains(lines[2], "only modified parameters")) doc.format = 1;
+	else if (contains(lines[2], "application settings")) return fail(info, "application settings, not a scene");
+	else return fail(info, "not a Mandelbulber settings file: format not given in the header");
+	if (info)
+	{
+		info->fileVersion = doc.version;
+		info->format = doc.format;
+	}
+
+	// lines (Decode, settings.cpp:742-905)
+	std::string section;
+	for (size_t l = 3; l < lines.size(); l++)
+	{
+		const std::string &line = lines[l];
+		if (line.size() >= 2 && line.front() == '[' && line.back() == ']')
+		{
+			section = line.substr(1, line.size() - 2);
+			continue;
+		}
+		if (section == "description" || line.empty()) continue;
+		bool ok = true;
+		if (section == "main_parameters") ok = decode_one_line(doc, line, &doc.main, true, 0);
+		else if (contains(section, "fractal"))
+		{
+			const int idx = (int)to_int(section.size() > 8 ? section.substr(8) : std::string());
+			ok = decode_one_line(doc, line, &doc.fractal[idx], false, idx);
+		}
+		else if (section == "frames" || section == "keyframes") doc.animation = true;
+		if (!ok)
+		{
+			doc.errors++;
+			if (doc.firstError.empty()) doc.firstError = "malformed line " + std::to_string(l) + ": " + line;
+			if (doc.errors > 3)
+			{
+				if (info)
+				{
+					info->lines = doc.lines;
+					info->errors = doc.errors;
+				}
+				return fail(info, "too many errors in the settings file; first: " + doc.firstError);
+			}
+		}
+	}
+
+	const View M{&doc.main};
+	auto F = [&](int n) { auto it = doc.fractal.find(n); return View{it == doc.fractal.end() ? nullptr : &it->second}; };
+	const double ver = doc.version;
+	uint32_t flags = 0;
+
+	// ---- which fractal node is rendered -------------------------------------------------------
+	// fractal container formulas: container 1 defaults to Mandelbulb, the others to none
+	auto formula_of = [&](int n) { return (int)F(n).i("formula", n == 1 ? MBX_FORMULA_MANDELBULB : 0); };
+	int objectId = 1;      // fractal container the node renders
+	bool nodeEnabled = true;
+	double rotation[3] = {0, 0, 0}, position[3] = {0, 0, 0}, repeat[3] = {0, 0, 0};
+	double nodeScale = 1.0;
+	int material = 1;
+	bool juliaMode = false;
+	double juliaC[3] = {0, 0, 0}, constFactor[3] = {1, 1, 1};
+	double initialW = 0.0;
+	int maxiter = 250;
+	if (ver < 2.35)
+	{
+		// MigrateToObjectsTree (settings.cpp:2551-2760) for the cases with one fractal node
+		const bool hybrid = M.b("hybrid_fractal_enable", false);
+		const bool boolean = M.b("boolean_operators", false);
+		std::vector<int> enabled;
+		for (int n = 1; n <= 9; n++)
+			if (formula_of(n) != 0) enabled.push_back(n);
+		// legacy global transform (MigrateLegacyParamsToFractal, settings.cpp:2368-2402)
+		double globalRot[3], globalPos[3], globalRepeat[3];
+		M.v3("fractal_rotation", globalRot, 0, 0, 0);
+		M.v3("fractal_position", globalPos, 0, 0, 0);
+		M.v3("repeat", globalRepeat, 0, 0, 0);
+		auto transform_of = [&](int n) {
+			const View fv = F(n);
+			fv.v3("formula_rotation", rotation, 0, 0, 0);
+			fv.v3("formula_position", position, 0, 0, 0);
+			fv.v3("formula_repeat", repeat, 0, 0, 0);
+			nodeScale = fv.d("formula_scale", 1.0);
+			material = (int)fv.i("formula_material_id", 1);
+			if (M.has("formula_material_id") && M.i("formula_material_id", 1) != 1)
+				material = (int)M.i("formula_material_id", 1);
+			auto nonzero = [](const double *v) { return v[0] != 0.0 || v[1] != 0.0 || v[2] != 0.0; };
+			if (nonzero(globalRot)) std::memcpy(rotation, globalRot, sizeof(rotation));
+			if (nonzero(globalPos)) std::memcpy(position, globalPos, sizeof(position));
+			if (nonzero(globalRepeat)) std::memcpy(repeat, globalRepeat, sizeof(repeat));
+		};
+		if (boolean)
+		{
+			if (enabled.size() == 1)
+			{
+				objectId = enabled[0];
+				transform_of(objectId);
+				const std::string k = "_" + std::to_string(objectId);
+				juliaMode = M.b("julia_mode" + k, false);
+				M.v3("julia_c" + k, juliaC, 0, 0, 0);
+				M.v3("fractal_constant_factor" + k, constFactor, 1, 1, 1);
+				initialW = M.d("initial_waxis" + k, 0.0);
+				maxiter = (int)M.i("formula_maxiter" + k, 250, 1, 9999);
+				if (ver < 2.29) maxiter = (int)M.i("N", 250, 1, 99999); // N is always registered
+				nodeEnabled = M.has("fractal_enable" + k) ? M.b("fractal_enable" + k, true)
+				                                           : F(objectId).b("fractal_enable", true);
+			}
+			else
+			{
+				flags |= enabled.empty() ? MBX_FRACT_DISABLED : MBX_FRACT_HYBRID;
+				if (!enabled.empty()) objectId = enabled[0];
+			}
+		}
+		else if (!hybrid)
+		{
+			objectId = 1;
+			transform_of(1);
+			juliaMode = M.b("julia_mode", false);
+			M.v3("julia_c", juliaC, 0, 0, 0);
+			M.v3("fractal_constant_factor", constFactor, 1, 1, 1);
+			initialW = M.d("initial_waxis", 0.0);
+			maxiter = (int)M.i("N", 250, 1, 99999);
+			nodeEnabled = F(1).b("fractal_enable", true);
+		}
+		else
+		{
+			flags |= enabled.empty() ? MBX_FRACT_DISABLED : MBX_FRACT_HYBRID;
+			if (!enabled.empty()) objectId = enabled[0];
+		}
+		if (doc.primitives) flags |= MBX_FRACT_PRIMITIVES;
+	}
+	else
+	{
+		// objects tree (objects_tree.cpp:36-96): without node lines the tree is InitNodeParams(1)'s
+		// one Mandelbulb node on fractal 1 (settings.cpp:715-733, initparameters.cpp:2102-2125)
+		std::vector<int> ids(doc.nodeIds.begin(), doc.nodeIds.end());
+		if (ids.empty()) ids.push_back(1);
+		int fractalNodes = 0, otherNodes = 0, nodeId = -1;
+		for (int id : ids)
+		{
+			char pfx[32];
+			std::snprintf(pfx, sizeof(pfx), "node_%04d_", id);
+			const std::string *def = M.raw(std::string(pfx) + "definition");
+			const std::string d = def ? *def : std::string("mandelbulb 1,1,1,0,1");
+			std::vector<std::string> parts = split(d, ',');
+			const int type = parts.size() >= 5 ? (int)to_int(parts[2]) : 0;
+			const bool en = M.b(std::string(pfx) + "enabled", true);
+			if (type == 1)
+			{
+				fractalNodes++;
+				if (nodeId < 0 || (en && !nodeEnabled))
+				{
+					nodeId = id;
+					objectId = (int)to_int(parts[4]);
+					nodeEnabled = en;
+				}
+			}
+			else if (type == 2) flags |= MBX_FRACT_PRIMITIVES;
+			else otherNodes++;
+		}
+		if (fractalNodes > 1 || otherNodes > 0) flags |= MBX_FRACT_HYBRID;
+		if (nodeId < 0) flags |= MBX_FRACT_DISABLED;
+		else
+		{
+			char pfx[32];
+			std::snprintf(pfx, sizeof(pfx), "node_%04d_", nodeId);
+			const std::string p(pfx);
+			M.v3(p + "rotation", rotation, 0, 0, 0);
+			M.v3(p + "position", position, 0, 0, 0);
+			M.v3(p + "repeat", repeat, 0, 0, 0);
+			nodeScale = M.d(p + "scale", 1.0);
+			material = (int)M.i(p + "material", 1);
+			juliaMode = M.b(p + "julia_mode", false);
+			M.v3(p + "julia_c", juliaC, 0, 0, 0);
+			M.v3(p + "fractal_constant_factor", constFactor, 1, 1, 1);
+			initialW = M.d(p + "initial_waxis", 0.0);
+			maxiter = (int)M.i(p + "formula_maxiter", 250, 1, 9999);
+		}
+	}
+	if (!nodeEnabled) flags |= MBX_FRACT_DISABLED;
+	if (position[0] != 0.0 || position[1] != 0.0 || position[2] != 0.0 || repeat[0] != 0.0 ||
+		repeat[1] != 0.0 || repeat[2] != 0.0 || nodeScale != 1.0 || M.b("limits_enabled", false))
+		flags |= MBX_FRACT_TRANSFORM;
+
+	// ---- the parameter block ------------------------------------------------------------------
+	mbx_params p;
+	mbx_params_default(&p);
+	p.imageWidth = (int)M.i("image_width", 800, 5, 65535);
+	p.imageHeight = (int)M.i("image_height", 600, 5, 65535);
+	M.v3("camera", p.camera, 3.0, -6.0, 2.0);
+	M.v3("target", p.target, 0.0, 0.0, 0.0);
+	M.v3("camera_top", p.topVector, -0.1277753, 0.2555506, 0.958314);
+	p.perspectiveType = (int)M.e("perspective_type", 0, {"three_point", "fish_eye", "equirectangular", "fish_eye_cut"});
+	p.fov = M.d("fov", 53.13, 0.0, 720.0);
+	if (ver < 2.21)
+	{
+		// Compatibility2 (settings.cpp:1611-1627): the old fov was the view-plane half-width
+		double fov = p.fov;
+		if (fov == 53.13) fov = 1.0;
+		double deg = 0.0;
+		switch (p.perspectiveType)
+		{
+			case 0: deg = std::atan(fov / 2.0) * 360.0 / M_PI; break;
+			case 1:
+			case 3: deg = fov * 180.0; break;
+			case 2: deg = fov * 360.0; break;
+		}
+		p.fov = std::min(std::max(deg, 0.0), 720.0);
+	}
+	if (p.perspectiveType != 0 || M.b("stereo_enabled", false)) flags |= MBX_FRACT_PERSPECTIVE;
+	p.sweetSpotHAngle = M.d("sweet_spot_horizontal_angle", 0.0, -180.0, 180.0);
+	p.sweetSpotVAngle = M.d("sweet_spot_vertical_angle", 0.0, -90.0, 90.0);
+	p.minN = (int)M.i("minN", 1, 0, 99999);
+	p.detailLevel = M.d("detail_level", 1.0, 1e-8, 1e8);
+	p.DEThresh = M.d("DE_thresh", 0.01, 1e-15, 1e5);
+	p.smoothness = M.d("smoothness", 1.0, 1e-15, 1e15);
+	p.iterThreshMode = M.b("iteration_threshold_mode", false);
+	p.DEFactor = M.d("DE_factor", 1.0, 1e-15, 1e15);
+	p.viewDistanceMax = M.d("view_distance_max", 50.0, 1e-15, 1e15);
+	p.interiorMode = M.b("interior_mode", false);
+	p.constantDEThreshold = M.b("constant_DE_threshold", false);
+	p.maxRaymarchingSteps = (int)M.i("max_raymarching_steps", 10000, 1, 100000000);
+	if (p.iterThreshMode || p.interiorMode) flags |= MBX_FRACT_INTERIOR;
+
+	// ambient occlusion
+	p.ambientOcclusion = M.d("ambient_occlusion", 1.0, 0.0, 1e15);
+	p.ambientOcclusionQuality = (int)M.i("ambient_occlusion_quality", 4, 1, 10);
+	p.ambientOcclusionFastTune = M.d("ambient_occlusion_fast_tune", 1.0, 1e-5, 1e5);
+	p.ambientOcclusionEnabled = M.b("ambient_occlusion_enabled", false);
+	p.ambientOcclusionMode = (int)M.i("ambient_occlusion_mode", MBX_AO_SCREEN_SPACE);
+	M.rgb("ambient_occlusion_color", p.ambientOcclusionColor, 65535, 65535, 65535);
+	if (p.ambientOcclusionEnabled && p.ambientOcclusionMode == MBX_AO_SCREEN_SPACE) flags |= MBX_FRACT_AO_SCREEN_SPACE;
+	if (p.ambientOcclusionEnabled && p.ambientOcclusionMode == MBX_AO_MULTIPLE_RAYS) flags |= MBX_FRACT_AO_LIGHTMAP;
+
+	// background, fill light, Monte Carlo
+	p.backgroundBrightness = M.d("background_brightness", 1.0, 0.0, 1000.0);
+	p.backgroundGamma = M.d("background_gamma", 1.0, 0.0, 1000.0);
+	M.rgb("background_color_1", p.backgroundColor1, 0, 38306, 65535);
+	M.rgb("background_color_2", p.backgroundColor2, 65535, 65535, 65535);
+	M.rgb("background_color_3", p.backgroundColor3, 0, 10000, 500);
+	p.background3ColorsEnable = M.b("background_3_colors_enable", true);
+	p.reflectionsMax = (int)M.i("reflections_max", 5, 0, 1000);
+	M.rgb("fill_light_color", p.fillLightColor, 0, 0, 0);
+	p.DOFMonteCarlo = M.b("DOF_monte_carlo", false);
+	p.DOFSamples = (int)M.i("DOF_samples", 100);
+	p.DOFMonteCarloGlobalIllumination = M.b("DOF_MC_global_illumination", false);
+	p.monteCarloGIRadianceLimit = M.d("MC_GI_radiance_limit", 10.0, 0.001, 1e10);
+	if (p.DOFMonteCarlo && M.i("DOF_min_samples", 10) < p.DOFSamples) flags |= MBX_FRACT_MC_ADAPTIVE;
+
+	// material of the node (InitMaterialParams :1600-1810, material.cpp:330-349); a file that names
+	// no material at all gets material 1 with its defaults (CheckIfMaterialsAreDefined)
+	{
+		const std::string m = "mat" + std::to_string(material) + "_";
+		M.rgb(m + "surface_color", p.surfaceColor, 50000, 50000, 50000);
+		p.shading = M.d(m + "shading", 1.0, 0.0, 1e15);
+		p.luminosity = M.d(m + "luminosity", 0.0, 0.0, 1e15);
+		M.rgb(m + "luminosity_color", p.luminosityColor, 65535, 65535, 65535);
+		if (M.b(m + "use_colors_from_palette", true)) flags |= MBX_FRACT_PALETTE;
+		if (M.d(m + "specular", 5.0, 0.0, 1e15) > 0.0) flags |= MBX_FRACT_SPECULAR;
+		const bool rt = M.b("raytraced_reflections", false);
+		if (rt && (M.d(m + "reflectance", 0.0) > 0.0 || M.d(m + "transparency_of_surface", 0.0) > 0.0))
+			flags |= MBX_FRACT_REFLECTIONS;
+		if (M.b(m + "iridescence_enabled", false) || M.b(m + "rough_surface", false) ||
+			M.b(m + "subsurface_scattering", false) || M.b(m + "perlin_noise_enable", false) ||
+			M.b(m + "luminosity_gradient_enable", false))
+			flags |= MBX_FRACT_MATERIAL;
+		for (const char *t : {"use_color_texture", "use_diffusion_texture", "use_displacement_texture",
+				 "use_luminosity_texture", "use_normal_map_texture", "use_reflectance_texture",
+				 "use_transparency_texture"})
+			if (M.b(m + t, false)) flags |= MBX_FRACT_TEXTURES;
+	}
+
+	// light #1: defined when the file names it; pre-2.25 files always get it (settings.cpp:1643-1654)
+	{
+		mbx_light &L = p.light;
+		const bool defined = doc.lightsSeen.count(1) || ver < 2.25;
+		if (!defined) L.enabled = 0;
+		else
+		{
+			L.enabled = M.b("light1_enabled", true);
+			L.castShadows = M.b("light1_cast_shadows", true);
+			L.penetrating = M.b("light1_penetrating", true);
+			L.relativePosition = M.b("light1_relative_position", true);
+			if (ver < 2.25 && doc.lightsSeen.count(1))
+				set3(L.rotation, M.d("light1_alpha", -45.0), M.d("light1_beta", 45.0), 0.0);
+			else if (ver >= 2.25)
+				M.v3("light1_rotation", L.rotation, -45.0, 45.0, 0.0);
+			L.intensity = M.d("light1_intensity", 1.0, 0.0, 1e10);
+			L.visibility = M.d("light1_visibility", 1.0, 0.0, 1e10);
+			L.size = M.d("light1_size", 0.5, 0.0, 1e10);
+			L.softShadowCone = M.d("light1_soft_shadow_cone", 1.0, 0.0, 1e10);
+			L.contourSharpness = M.d("light1_contour_sharpness", 1.0, 0.0, 1e10);
+			M.rgb("light1_color", L.color, 65535, 65535, 65535);
+			// settings.cpp:1700-1712: pre-2.28 iteration fog scaled light #1 by its brightness boost
+			if (ver < 2.28 && M.b("iteration_fog_enable", false))
+			{
+				double boost = M.d("iteration_fog_brightness_boost", 1.0, 0.0, 1e6);
+				if (ver <= 2.12) boost = 100.0; // settings.cpp:1547-1551
+				L.intensity /= boost;
+			}
+			const long long type = M.e("light1_type", 0, {"directional", "point", "spot", "projection", "beam"});
+			if (L.enabled && (type != 0 || M.b("light1_volumetric", false))) flags |= MBX_FRACT_LIGHTS;
+		}
+		for (int id : doc.lightsSeen)
+			if (id >= 2 && M.b("light" + std::to_string(id) + "_enabled", false)) flags |= MBX_FRACT_LIGHTS;
+		if (M.b("random_lights_group", false) || M.b("fake_lights_enabled", false)) flags |= MBX_FRACT_LIGHTS;
+	}
+
+	// other effects the path does not render
+	if (M.b("glow_enabled", true) && M.d("glow_intensity", 0.2) > 0.0) flags |= MBX_FRACT_GLOW;
+	if (M.b("basic_fog_enabled", false) || M.b("volumetric_fog_enabled", false) ||
+		M.b("iteration_fog_enable", false) || M.b("clouds_enable", false))
+		flags |= MBX_FRACT_FOG;
+	if (M.b("textured_background", false) || M.b("env_mapping_enable", false)) flags |= MBX_FRACT_TEXTURES;
+	if (M.d("brightness", 1.0) != 1.0 || M.d("contrast", 1.0) != 1.0 || M.d("gamma", 1.0) != 1.0 ||
+		M.d("saturation", 1.0) != 1.0 || M.b("hdr", false) || M.b("hdr_blur_enabled", false) ||
+		M.b("DOF_enabled", false) || M.b("post_chromatic_aberration_enabled", false))
+		flags |= MBX_FRACT_POSTPROCESS;
+	if (doc.animation) flags |= MBX_FRACT_ANIMATION;
+	if (M.b("antialiasing_enabled", false) && M.i("antialiasing_size", 2, 1, 10) > 1) flags |= MBX_FRACT_ANTIALIASING;
+
+	// ---- the fractal (sFractal, fractal.cpp:41-120; sequence-wide values, fractparams.cpp) ----
+	{
+		mbx_fractal &fr = p.fractal;
+		const View fv = F(objectId);
+		fr.formula = formula_of(objectId);
+		fr.formulaMaxiter = maxiter;
+		fr.deltaDEMethod = (int)M.i("delta_DE_method", 0, 0, 2);
+		fr.deltaDEFunction = (int)M.i("delta_DE_function", 0, 0, 6);
+		// settings.cpp:1500-1505: before 2.07 every non-linear DE function became logarithmic;
+		// settings.cpp:1596-1606: before 2.20 "preferred" method meant "preferred" function
+		if (ver <= 2.06 && fr.deltaDEFunction != MBX_DE_FUNCTION_LINEAR) fr.deltaDEFunction = MBX_DE_FUNCTION_LOGARITHMIC;
+		if (ver < 2.20 && fr.deltaDEMethod == 0) fr.deltaDEFunction = 0;
+		fr.useDefaultBailout = M.b("use_default_bailout", true);
+		fr.bailout = M.d("bailout", 100.0, 1.0, 1e15);
+		fr.juliaMode = juliaMode;
+		std::memcpy(fr.juliaC, juliaC, sizeof(juliaC));
+		std::memcpy(fr.constantFactor, constFactor, sizeof(constFactor));
+		fr.initialWAxis = initialW;
+		fr.dontAddCConstant = fv.b("dont_add_c_constant", false);
+		// MigrateLegacyParamsToFractal (settings.cpp:2328-2340): a true main-section
+		// dont_add_c_constant_N sets fractal N's flag
+		if (ver < 2.35 && objectId <= 8 && M.b("dont_add_c_constant_" + std::to_string(objectId), false))
+			fr.dontAddCConstant = 1;
+		fr.bulb.power = fv.d("power", 9.0);
+		fr.bulb.alphaAngleOffset = fv.d("alpha_angle_offset", 0.0);
+		fr.bulb.betaAngleOffset = fv.d("beta_angle_offset", 0.0);
+		mbx_fractal_mandelbox &mb = fr.mandelbox;
+		mb.scale = fv.d("mandelbox_scale", 2.0);
+		mb.foldingLimit = fv.d("mandelbox_folding_limit", 1.0);
+		mb.foldingValue = fv.d("mandelbox_folding_value", 2.0);
+		mb.foldingSphericalMin = fv.d("mandelbox_folding_min_radius", 0.5);
+		mb.foldingSphericalFixed = fv.d("mandelbox_folding_fixed_radius", 1.0);
+		fv.v3("mandelbox_offset", mb.offset, 0, 0, 0);
+		fv.v3("mandelbox_rotation_main", mb.rotationMain, 0, 0, 0);
+		for (int i = 1; i <= 3; i++)
+		{
+			fv.v3("mandelbox_rotation_neg_" + std::to_string(i), mb.rotation[0][i - 1], 0, 0, 0);
+			fv.v3("mandelbox_rotation_pos_" + std::to_string(i), mb.rotation[1][i - 1], 0, 0, 0);
+		}
+		fv.v3("mandelbox_color", mb.colorFactor, 0.03, 0.05, 0.07);
+		mb.colorFactorSp1 = fv.d("mandelbox_color_Sp1", 0.2);
+		mb.colorFactorSp2 = fv.d("mandelbox_color_Sp2", 0.2);
+		mb.rotationsEnabled = fv.b("mandelbox_rotations_enabled", false);
+		mb.mainRotationEnabled = fv.b("mandelbox_main_rotation_enabled", false);
+		mbx_fractal_ifs &ifs = fr.IFS;
+		ifs.scale = fv.d("IFS_scale", 2.0);
+		fv.v3("IFS_offset", ifs.offset, 1, 0, 0);
+		fv.v3("IFS_rotation", ifs.rotation, 0, 0, 0);
+		fv.v3("IFS_edge", ifs.edge, 0, 0, 0);
+		for (int i = 0; i < MBX_IFS_VECTOR_COUNT; i++)
+		{
+			const std::string k = "_" + std::to_string(i);
+			fv.v3("IFS_direction" + k, ifs.direction[i], 1, 0, 0);
+			fv.v3("IFS_rotations" + k, ifs.rotations[i], 0, 0, 0);
+			ifs.distance[i] = fv.d("IFS_distance" + k, 0.0);
+			ifs.intensity[i] = fv.d("IFS_intensity" + k, 1.0);
+			ifs.enabled[i] = fv.b("IFS_enabled" + k, false);
+		}
+		ifs.absX = fv.b("IFS_abs_x", false);
+		ifs.absY = fv.b("IFS_abs_y", false);
+		ifs.absZ = fv.b("IFS_abs_z", false);
+		ifs.mengerSpongeMode = fv.b("IFS_menger_sponge_mode", false);
+		ifs.rotationEnabled = fv.b("IFS_rotation_enabled", false);
+		ifs.edgeEnabled = fv.b("IFS_edge_enabled", false);
+		// settings.cpp:1507-1513, 1629-1641
+		if (ver <= 2.06 && objectId <= 4) ifs.rotationEnabled = 1;
+		if (ver < 2.22 && (ifs.edge[0] != 0.0 || ifs.edge[1] != 0.0 || ifs.edge[2] != 0.0)) ifs.edgeEnabled = 1;
+		if (fr.formula != MBX_FORMULA_MANDELBULB && fr.formula != MBX_FORMULA_MANDELBOX &&
+			fr.formula != MBX_FORMULA_KALEIDOSCOPIC_IFS)
+			flags |= fr.formula == MBX_FORMULA_NONE ? MBX_FRACT_DISABLED : MBX_FRACT_FORMULA;
+		std::memcpy(p.formulaRotation, rotation, sizeof(rotation));
+	}
+
+	*params = p;
+	if (info)
+	{
+		info->lines = doc.lines;
+		info->errors = doc.errors;
+		info->formula = p.fractal.formula;
+		info->unsupported = flags;
+		std::snprintf(info->message, sizeof(info->message), "%s", doc.firstError.c_str());
+	}
+	return MBX_OK;
+}

@@ -976,13 +976,14 @@ int mbx_ctx_create(int device, void *stream, mbx_ctx **out)
 void mbx_ctx_destroy(mbx_ctx *ctx)
 {
 	if (!ctx) return;
-	hipSetDevice(ctx->device);
-	if (ctx->stream) hipStreamSynchronize(ctx->stream);
-	else hipDeviceSynchronize();
-	if (ctx->d_frame) hipFree(ctx->d_frame);
-	if (ctx->d_counters) hipFree(ctx->d_counters);
-	if (ctx->ev0) hipEventDestroy(ctx->ev0);
-	if (ctx->ev1) hipEventDestroy(ctx->ev1);
+	// teardown is best effort: a failure here has no caller left to report to
+	(void)hipSetDevice(ctx->device);
+	if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+	else (void)hipDeviceSynchronize();
+	if (ctx->d_frame) (void)hipFree(ctx->d_frame);
+	if (ctx->d_counters) (void)hipFree(ctx->d_counters);
+	if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+	if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
 	if (ctx->graphExec) (void)hipGraphExecDestroy(ctx->graphExec);
 	if (ctx->d_frames) (void)hipFree(ctx->d_frames);
 	if (ctx->d_jobs) (void)hipFree(ctx->d_jobs);
@@ -1280,9 +1281,9 @@ int mbx_distance(mbx_ctx *ctx, const double *points, int n, double detail_size, 
 	if (rc == MBX_OK && iters_out &&
 		(e = hipMemcpy(iters_out, d_it, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost)) != hipSuccess)
 		rc = hip_fail(ctx, e, "copy iterations");
-	if (d_pts) hipFree(d_pts);
-	if (d_dist) hipFree(d_dist);
-	if (d_it) hipFree(d_it);
+	if (d_pts) (void)hipFree(d_pts);
+	if (d_dist) (void)hipFree(d_dist);
+	if (d_it) (void)hipFree(d_it);
 	return rc;
 }
 

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libmbx.so")
 EXPORTS = ["mbx_version", "mbx_sizeof_params", "mbx_ctx_create", "mbx_ctx_destroy",
            "mbx_last_error", "mbx_scene_upload", "mbx_scene_info", "mbx_render", "mbx_distance",
            "mbx_get_stats", "mbx_render_batch", "mbx_tune", "mbx_get_tune", "mbx_autotune",
-           "mbx_render_diff", "mbx_adjoint"]
+           "mbx_render_diff", "mbx_adjoint", "mbx_params_default", "mbx_load_fract"]
 
 _lib = None
 
@@ -53,8 +53,31 @@ def load():
     L.mbx_render_diff.argtypes = [C.c_void_p, P(abi.MbxOutputs), C.c_void_p]
     L.mbx_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.mbx_autotune.argtypes = [C.c_void_p, P(abi.MbxOutputs), C.c_int, P(C.c_int), P(C.c_double)]
+    L.mbx_params_default.argtypes = [P(abi.MbxParams)]
+    L.mbx_params_default.restype = None
+    L.mbx_load_fract.argtypes = [C.c_char_p, C.c_size_t, P(abi.MbxParams), P(abi.MbxFractInfo)]
     _lib = L
     return L
+
+
+def params_default():
+    """The reference's parameter defaults as the library holds them (mbx_params_default)."""
+    p = abi.MbxParams()
+    load().mbx_params_default(C.byref(p))
+    return p
+
+
+def load_fract(text):
+    """Read a .fract settings text (str or bytes) with mbx_load_fract.  Returns (params, info);
+    raises MbxError with the reader's message when the file is rejected."""
+    if isinstance(text, str):
+        text = text.encode("utf-8")
+    p = abi.MbxParams()
+    info = abi.MbxFractInfo()
+    rc = load().mbx_load_fract(text, len(text), C.byref(p), C.byref(info))
+    if rc != 0:
+        raise MbxError("mbx_load_fract: %s (%d)" % (info.message.decode(errors="replace"), rc))
+    return p, info
 
 
 MBX_TUNE_KERNEL = 0

@@ -1,17 +1,20 @@
 """Scene parameter blocks: reference defaults and the benchmark / parity workloads.
 
 Defaults are the values cParameterContainer registers in mandelbulber2/src/initparameters.cpp
-(line numbers cited per field); colours given there as 16-bit sRGB are divided by 65535 as
-toRGBFloat does.  Only parameters on the raymarch path are represented.
+(line numbers cited per field); colours given there as 16-bit sRGB become float(c / 65536) as
+toRGBFloat makes them (color_structures.hpp:106-109).  Only parameters on the raymarch path are
+represented.  The library holds the same table (mbx_params_default); a test keeps them equal.
 """
 import math
 import struct
+
+import numpy as np
 
 from . import abi
 
 
 def _u16(rgb16):
-    return [c / 65535.0 for c in rgb16]
+    return [float(np.float32(c / 65536.0)) for c in rgb16]
 
 
 def default_params(width=640, height=480):
@@ -40,7 +43,7 @@ def default_params(width=640, height=480):
     abi.set3(p.surfaceColor, _u16((50000, 50000, 50000)))
     p.shading = 1.0
     p.luminosity = 0.0
-    abi.set3(p.luminosityColor, (1.0, 1.0, 1.0))
+    abi.set3(p.luminosityColor, _u16((65535, 65535, 65535)))
     abi.set3(p.fillLightColor, (0.0, 0.0, 0.0))  # :309
     # light #1 (initparameters.cpp:1841-1960)
     L = p.light
@@ -54,14 +57,14 @@ def default_params(width=640, height=480):
     L.size = 0.5
     L.softShadowCone = 1.0
     L.contourSharpness = 1.0
-    abi.set3(L.color, (1.0, 1.0, 1.0))
+    abi.set3(L.color, _u16((65535, 65535, 65535)))
     # ambient occlusion (initparameters.cpp:213-219)
     p.ambientOcclusionEnabled = 0
-    p.ambientOcclusionMode = abi.MBX_AO_MULTIPLE_RAYS
+    p.ambientOcclusionMode = abi.MBX_AO_SCREEN_SPACE
     p.ambientOcclusionQuality = 4
     p.aoVectorsCount = 0
     p.ambientOcclusion = 1.0
-    abi.set3(p.ambientOcclusionColor, (1.0, 1.0, 1.0))
+    abi.set3(p.ambientOcclusionColor, _u16((65535, 65535, 65535)))
     p.ambientOcclusionFastTune = 1.0
     # background (initparameters.cpp:227-245)
     p.background3ColorsEnable = 1

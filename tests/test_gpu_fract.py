@@ -1,0 +1,79 @@
+"""Example scenes loaded with mbx_load_fract, rendered through the C ABI on the GPU and compared
+with the FP64 oracle's render of the same parameter block.
+
+This mirrors the reference's example test (mandelbulber2/src/test.cpp:88-247), which loads every
+example scene, overrides its image size and renders it: here the fixtures under
+tests/golden/fract are loaded, shrunk to 192 pixels wide (aspect kept), and rendered.  Effects the
+path does not render (mbx_fract_info.unsupported) are absent from both renders; multi-ray and
+screen-space AO are switched off because the first takes its directions from a lightmap texture
+and the second is a post effect.  Bars as tests/test_gpu_parity.py: mask and iterations equal on
+>= 99.9% of pixels (>= 99.5% iterations for the general-power Mandelbulb, whose asin / atan2 / pow
+run in FP32), depth within 1e-3 relative on >= 99.9% of common hits (Mandelbox, FP64 orbits:
+99.9th percentile within 1e-6), RGB within 1e-2 on >= 99.9% of pixels.
+"""
+import os
+
+import pytest
+
+from mandelbulber2_amd import abi
+from tests.parity_util import compare
+
+pytestmark = pytest.mark.gpu
+
+FIXTURES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fract")
+
+SCENES = [
+    "mandelbulb001.fract",
+    "mandelbulb002.fract",
+    "mandelbulb_power_8.fract",
+    "mandelbox001.fract",
+    "mandelbox_22.fract",
+    "keyframe_anim_mandelbox_boxes.fract",
+    "IFS_001.fract",
+    "menger_sponge_001.fract",
+]
+
+
+@pytest.fixture(scope="module")
+def rt():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from mandelbulber2_amd import runtime
+    runtime.load()
+    return runtime
+
+
+def _load(rt, name, width=192):
+    with open(os.path.join(FIXTURES, name), "rb") as f:
+        p, info = rt.load_fract(f.read())
+    p.imageHeight = max(1, round(width * p.imageHeight / p.imageWidth))
+    p.imageWidth = width
+    if p.ambientOcclusionEnabled and p.ambientOcclusionMode != abi.MBX_AO_FAST:
+        p.ambientOcclusionEnabled = 0
+    return p, info
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_loaded_example_scene_parity(rt, oracle_mod, name):
+    p, info = _load(rt, name)
+    flags = abi.fract_flag_names(info.unsupported)
+    assert not {"hybrid", "primitives", "formula", "perspective", "interior", "disabled"} & set(flags)
+    print("\n[%s] %dx%d formula %d, not rendered: %s" % (name, p.imageWidth, p.imageHeight,
+                                                        p.fractal.formula, ",".join(flags)), flush=True)
+    g = rt.render_to_host(p)
+    r = oracle_mod.render(p)
+    st = compare(g, r)
+    print("  " + " ".join("%s=%s" % (k, ("%.6g" % v) if isinstance(v, float) else v)
+                          for k, v in st.items()), flush=True)
+    trig = p.fractal.formula == abi.MBX_FORMULA_MANDELBULB and not (
+        p.fractal.bulb.power == 8.0 and p.fractal.bulb.alphaAngleOffset == 0.0
+        and p.fractal.bulb.betaAngleOffset == 0.0)
+    box = p.fractal.formula == abi.MBX_FORMULA_MANDELBOX
+    assert st["mask_agree"] >= 0.999, st
+    assert st["iters_agree"] >= (0.995 if trig else 0.999), st
+    if box:
+        assert st.get("depth_rel_p999", 0.0) <= 1e-6, st
+    else:
+        assert st.get("depth_within_1e-3", 1.0) >= 0.999, st
+    assert st["rgb_within_1e-2"] >= 0.999, st

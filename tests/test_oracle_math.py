@@ -205,7 +205,8 @@ def test_background_gradient_and_sun(oracle_mod):
     p.light.enabled = 0
     sky = oracle_mod.background(p, light)
     # looking straight into the directional light adds intensity * visibility * colour
-    np.testing.assert_allclose(sun - sky, [1.0, 1.0, 1.0], atol=1e-6)
+    want = np.asarray(p.light.color[:]) * p.light.intensity * p.light.visibility
+    np.testing.assert_allclose(sun - sky, want, atol=1e-6)
 
 
 def test_pixel_seed_range(oracle_mod):
