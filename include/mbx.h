@@ -277,7 +277,8 @@ int mbx_scene_upload(mbx_ctx *ctx, const mbx_params *params);
  * the camera rotation mRot (row-major 3x3, render_worker.cpp:539-564), the CalcFOV value
  * (projection_3d.cpp:140-159), light #1's direction (light.cpp:130-185) and the device DE
  * kernel kind chosen for the fractal (0 Mandelbulb-8 trig-free, 1 Mandelbulb trig, 2 Mandelbox
- * analytic, 3 Mandelbox delta, 4 Kaleidoscopic IFS).  Any output pointer may be NULL. */
+ * analytic, 3 Mandelbox delta, 4 Kaleidoscopic IFS, 5 Mandelbulb in FP64).  Any output pointer
+ * may be NULL. */
 int mbx_scene_info(const mbx_ctx *ctx, double *mrot9, double *fov, double *light_dir3,
 	int *de_kind);
 
@@ -316,6 +317,11 @@ int mbx_render_batch(mbx_ctx *ctx, const mbx_params *frames, const mbx_outputs *
                                  where the first formula iteration bounds the DE above the value at
                                  which the reference's step clamps to 3, take that step without
                                  evaluating the DE (pixel results are identical); 0 = evaluate */
+#define MBX_TUNE_PRECISION 6  /* Mandelbulb march precision: 0 (default) = FP32, or FP64 position and
+                                 orbit when FP32 rounding of a coordinate near the nearest hit exceeds
+                                 1/256 of the view's smallest distance threshold (deep zooms);
+                                 1 = always FP32; 2 = always FP64.  Mandelbox scenes always march in
+                                 FP64, Kaleidoscopic IFS scenes in FP32 */
 
 /* Set a launch or numerics option (see MBX_TUNE_*); scene options apply from the next
  * mbx_scene_upload / mbx_render_batch.  No reference counterpart: the OpenCL engine's

@@ -397,6 +397,48 @@ __device__ __forceinline__ double compute_box_fd(DV3 p, const FrameConst &f, dou
 	return r;
 }
 
+// Mandelbulb orbit in FP64 with the analytic DE, in the reference's operation order:
+// cFractalMandelbulb::FormulaCode (fractal_mandelbulb.cpp:29-43) with libm-class asin, atan2,
+// pow, sin and cos, then z += c, aux.r = |z|, the NaN revert and the r > bailout test of
+// Compute (compute_fractal.cpp:246-321) and the DE finish (compute_fractal.cpp:561-617).
+__device__ __forceinline__ double compute_bulb_d(DV3 p, const FrameConst &f, int &iters)
+{
+#pragma clang fp contract(off)
+	DV3 z = p;
+	double r = sqrt(ddot(z, z));
+	double DE = 1.0;
+	const int maxN = f.formulaMaxiter;
+	const double power = f.powerD;
+	int i;
+	for (i = 0; i < maxN; i++)
+	{
+		const DV3 lastZ = z;
+		const double th0 = asin(z.z / r) + f.betaD;
+		const double ph0 = atan2(z.y, z.x) + f.alphaD;
+		double rp = pow(r, power - 1.0);
+		const double th = th0 * power;
+		const double ph = ph0 * power;
+		const double cth = cos(th);
+		DE = (rp * DE) * power + 1.0;
+		rp *= r;
+		z = dv3(cth * cos(ph) * rp, cth * sin(ph) * rp, sin(th) * rp);
+		z = add_c_d(z, p, f);
+		r = sqrt(ddot(z, z));
+		if (!(isfinite(z.x) && isfinite(z.y) && isfinite(z.z)))
+		{
+			z = lastZ;
+			r = sqrt(ddot(z, z));
+			break;
+		}
+		if (r > f.bailoutD) break;
+	}
+	iters = i + 1;
+	if (!(DE > 0.0)) return r;
+	if (f.analyticFunction == MBX_ANALYTIC_LOGARITHMIC) return (r > 1.0) ? 0.5 * r * log(r) / DE : 0.0;
+	if (f.analyticFunction == MBX_ANALYTIC_IFS) return (r - 2.0) / DE;
+	return r / DE;
+}
+
 // Kaleidoscopic IFS (fractal_kaleidoscopic_ifs.cpp:30-74)
 __device__ __forceinline__ void KaleidoscopicIfsIteration(V3 &z, float &DE, const FrameConst &f)
 {
@@ -546,11 +588,13 @@ __device__ __forceinline__ float compute_analytic(V3 p, const FrameConst &f, con
 // taps, secondary-ray points): the reference's delta DE with its tiny offset is chaotic at FP32
 // position resolution (rounding a hit point to FP32 changes the iteration count of ~0.7% of
 // Mandelbox hits in the reference's own FP64 arithmetic), and the deep-zoom scenes need
-// sub-1e-7 positions.  FP64 add/mul/FMA run at the unpacked FP32 rate on gfx950.
+// sub-1e-7 positions.  kBulbD is the Mandelbulb for views whose distance threshold is within a
+// few hundred FP32 rounding steps of the coordinates (the host picks it).  FP64 add/mul/FMA run
+// at the unpacked FP32 rate on gfx950.
 template <int KIND>
 struct PosPrecision
 {
-	static constexpr bool kFP64 = (KIND == kBoxAnalytic || KIND == kBoxDelta);
+	static constexpr bool kFP64 = (KIND == kBoxAnalytic || KIND == kBoxDelta || KIND == kBulbD);
 };
 
 template <bool D>
@@ -659,6 +703,12 @@ __device__ __forceinline__ typename Prec<PosPrecision<KIND>::kFP64>::R calculate
 	else if constexpr (KIND == kBoxAnalytic)
 	{
 		d = compute_box_d<false>(p, f, detailSize, iters);
+		w.iterations += (uint32_t)iters;
+		itersMinN = iters;
+	}
+	else if constexpr (KIND == kBulbD)
+	{
+		d = compute_bulb_d(p, f, iters);
 		w.iterations += (uint32_t)iters;
 		itersMinN = iters;
 	}

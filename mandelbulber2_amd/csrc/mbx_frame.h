@@ -22,7 +22,8 @@ enum DEKind : int32_t
 	kBoxAnalytic = 2,
 	kBoxDelta = 3,   // Mandelbox, delta DE (reference finite differences, or tangents)
 	kIfsAnalytic = 4,
-	kNumDEKinds = 5
+	kBulbD = 5,      // Mandelbulb any power, analytic DE, FP64 position and orbit (deep zooms)
+	kNumDEKinds = 6
 };
 
 struct FrameConst
@@ -74,6 +75,7 @@ struct FrameConst
 	double mbOffsetD[3];
 	double juliaCD[3], constMulD[3];
 	double bailoutD;
+	double powerD, alphaD, betaD; // Mandelbulb in FP64 (radians)
 	int32_t deltaTangent; // delta DE by forward-mode tangents instead of the reference's differences
 	int32_t mbRotationsEnabled, mbMainRotationEnabled;
 	float mbMainRot[9];

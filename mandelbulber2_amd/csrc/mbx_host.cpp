@@ -55,6 +55,7 @@ struct mbx_ctx
 	unsigned int graphTiles = 0;
 	int tileShift = 3; // 8x8-pixel tiles
 	int farField = 1;  // far-field march steps without DE evaluation (MBX_TUNE_FAR_FIELD)
+	int precision = 0; // Mandelbulb march precision (MBX_TUNE_PRECISION): 0 auto, 1 FP32, 2 FP64
 	int batchFull = 1; // the staged frames need the Monte Carlo / GI / shadow / AO code
 	// differentiable render
 	mbx::DiffConst diff{};
@@ -66,6 +67,9 @@ struct mbx_ctx
 
 namespace
 {
+
+// M_PI_180 of mandelbulber2/src/fractal.h:46, the factor RecalculateFractalParams uses
+constexpr double kPi180 = 0.01745329251994329576923690768489;
 
 int fail(mbx_ctx *ctx, int code, const std::string &msg)
 {
@@ -354,6 +358,37 @@ float far_field_r2_uncached(double P, bool logDE, double bailout, double c, doub
 	return (float)(r0 * r0);
 }
 
+// Mandelbulb DE at one point in FP64 (FormulaCode of fractal_mandelbulb.cpp:29-43 inside the
+// Compute loop), used to choose the march precision: the DE at the camera bounds the distance to
+// the nearest surface, and with it the smallest distance threshold of the view.
+double host_bulb_de(const double p[3], const mbx_fractal &fr, double bailout, int analyticFunction,
+	bool addc, bool julia)
+{
+	const double power = fr.bulb.power;
+	const double alpha = fr.bulb.alphaAngleOffset * kPi180, beta = fr.bulb.betaAngleOffset * kPi180;
+	double z[3] = {p[0], p[1], p[2]};
+	double r = std::sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2]);
+	double DE = 1.0;
+	for (int i = 0; i < fr.formulaMaxiter; i++)
+	{
+		const double th = (std::asin(z[2] / r) + beta) * power;
+		const double ph = (std::atan2(z[1], z[0]) + alpha) * power;
+		double rp = std::pow(r, power - 1.0);
+		DE = rp * DE * power + 1.0;
+		rp *= r;
+		z[0] = std::cos(th) * std::cos(ph) * rp;
+		z[1] = std::cos(th) * std::sin(ph) * rp;
+		z[2] = std::sin(th) * rp;
+		for (int k = 0; k < 3; k++)
+			if (addc) z[k] += julia ? fr.juliaC[k] * fr.constantFactor[k] : p[k] * fr.constantFactor[k];
+		r = std::sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2]);
+		if (!std::isfinite(r) || r > bailout) break;
+	}
+	if (!std::isfinite(r) || !(DE > 0.0)) return 0.0;
+	if (analyticFunction == MBX_ANALYTIC_LOGARITHMIC) return r > 1.0 ? 0.5 * r * std::log(r) / DE : 0.0;
+	return r / DE;
+}
+
 // Derive the per-frame constants; returns MBX_OK or an error with ctx->err set.
 int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 {
@@ -469,6 +504,10 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 	fc.power = (float)fr.bulb.power;
 	fc.alpha = (float)(fr.bulb.alphaAngleOffset * M_PI / 180.0);
 	fc.beta = (float)(fr.bulb.betaAngleOffset * M_PI / 180.0);
+	// RecalculateFractalParams (fractal.cpp:677-678)
+	fc.powerD = fr.bulb.power;
+	fc.alphaD = fr.bulb.alphaAngleOffset * kPi180;
+	fc.betaD = fr.bulb.betaAngleOffset * kPi180;
 
 	const mbx_fractal_mandelbox &mb = fr.mandelbox;
 	fc.mbScale = (float)mb.scale;
@@ -541,6 +580,18 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 			fr.bulb.betaAngleOffset == 0.0 && analyticFunction == MBX_ANALYTIC_LOGARITHMIC &&
 			fc.addC && !fc.julia; // the power-8 kernel adds c = p * constantMultiplier
 		kind = pow8 ? mbx::kBulb8 : mbx::kBulbTrig;
+		// FP64 march when FP32 rounding of a coordinate near the nearest hit (|camera| + DE at the
+		// camera) exceeds 1/256 of the smallest distance threshold of the view
+		bool fp64 = ctx->precision == 2;
+		if (ctx->precision == 0)
+		{
+			const double de0 = host_bulb_de(p->camera, fr, fc.bailoutD, analyticFunction, fc.addC, fc.julia);
+			const double thrMin = p->constantDEThreshold ? p->DEThresh : de0 * fc.threshKD;
+			const double mag = std::max(std::fabs(p->camera[0]), std::max(std::fabs(p->camera[1]),
+								   std::fabs(p->camera[2]))) + de0;
+			fp64 = mag * 0x1p-24 > thrMin / 256.0;
+		}
+		if (fp64) kind = mbx::kBulbD;
 		if (analyticFunction != MBX_ANALYTIC_LOGARITHMIC && analyticFunction != MBX_ANALYTIC_LINEAR)
 			kind = -1;
 	}
@@ -863,6 +914,8 @@ int render_frames(mbx_ctx *ctx, const std::vector<FrameConst> &frames, const mbx
 std::string derive_diff(const mbx_params *p, const FrameConst &fc, const double lightDir[3], mbx::DiffConst &dc_)
 {
 	const mbx_fractal &fr = p->fractal;
+	if (fr.formula == MBX_FORMULA_MANDELBULB && fc.deKind == mbx::kBulbD)
+		return "the differentiable render marches in FP32: this view needs the FP64 Mandelbulb (MBX_TUNE_PRECISION)";
 	if (fr.formula != MBX_FORMULA_MANDELBULB || (fc.deKind != mbx::kBulb8 && fc.deKind != mbx::kBulbTrig) ||
 		fc.analyticFunction != MBX_ANALYTIC_LOGARITHMIC)
 		return "the differentiable render covers the Mandelbulb with the analytic logarithmic DE";
@@ -1113,15 +1166,16 @@ int mbx_render_batch(mbx_ctx *ctx, const mbx_params *params, const mbx_outputs *
 		if (rc != MBX_OK) return rc;
 	}
 	// one kernel specialisation per launch: Mandelbulb frames of mixed powers use the general
-	// (trigonometric) iteration
+	// (trigonometric) iteration, and all of them the FP64 one if any frame needs it
 	int kind = frames[0].deKind;
+	auto bulb = [](int k) { return k == mbx::kBulb8 || k == mbx::kBulbTrig || k == mbx::kBulbD; };
 	for (int i = 1; i < n; i++)
 	{
 		int k = frames[i].deKind;
 		if (k == kind) continue;
-		const bool bulbs = (k == mbx::kBulb8 || k == mbx::kBulbTrig) && (kind == mbx::kBulb8 || kind == mbx::kBulbTrig);
-		if (!bulbs) return fail(ctx, MBX_ERR_UNSUPPORTED, "a batch must share one fractal formula and DE mode");
-		kind = mbx::kBulbTrig;
+		if (!bulb(k) || !bulb(kind))
+			return fail(ctx, MBX_ERR_UNSUPPORTED, "a batch must share one fractal formula and DE mode");
+		kind = (k == mbx::kBulbD || kind == mbx::kBulbD) ? mbx::kBulbD : mbx::kBulbTrig;
 	}
 	for (auto &f : frames) f.deKind = kind;
 	return render_frames(ctx, frames, outs);
@@ -1149,6 +1203,10 @@ int mbx_tune(mbx_ctx *ctx, int key, int value)
 		case MBX_TUNE_FAR_FIELD:
 			ctx->farField = value ? 1 : 0;
 			return MBX_OK;
+		case MBX_TUNE_PRECISION:
+			if (value < 0 || value > 2) return fail(ctx, MBX_ERR_INVALID, "precision must be 0, 1 or 2");
+			ctx->precision = value;
+			return MBX_OK;
 		case MBX_TUNE_TILE_WIDTH:
 		{
 			int shift = -1;
@@ -1173,6 +1231,7 @@ int mbx_get_tune(mbx_ctx *ctx, int key, int *value)
 		case MBX_TUNE_DELTA_DE: *value = ctx->deltaTangent; return MBX_OK;
 		case MBX_TUNE_TILE_WIDTH: *value = 1 << ctx->tileShift; return MBX_OK;
 		case MBX_TUNE_FAR_FIELD: *value = ctx->farField; return MBX_OK;
+		case MBX_TUNE_PRECISION: *value = ctx->precision; return MBX_OK;
 		default: return fail(ctx, MBX_ERR_INVALID, "unknown tuning key");
 	}
 }

@@ -206,6 +206,7 @@ hipError_t launch_render(int deKind, const FrameConst *frame, int width, int hei
 		case kBoxDelta: return launch_render_t<kBoxDelta>(frame, width, height, out, counters, tileShift, full, stream);
 		case kIfsAnalytic:
 			return launch_render_t<kIfsAnalytic>(frame, width, height, out, counters, tileShift, full, stream);
+		case kBulbD: return launch_render_t<kBulbD>(frame, width, height, out, counters, tileShift, full, stream);
 		default: return hipErrorInvalidValue;
 	}
 }
@@ -251,6 +252,7 @@ hipError_t launch_batch(int deKind, BatchArgs a, int numWaves, hipStream_t strea
 		case kBoxAnalytic: return launch_batch_t<kBoxAnalytic>(a, numWaves, a.full != 0, stream);
 		case kBoxDelta: return launch_batch_t<kBoxDelta>(a, numWaves, a.full != 0, stream);
 		case kIfsAnalytic: return launch_batch_t<kIfsAnalytic>(a, numWaves, a.full != 0, stream);
+		case kBulbD: return launch_batch_t<kBulbD>(a, numWaves, a.full != 0, stream);
 		default: return hipErrorInvalidValue;
 	}
 }
@@ -267,6 +269,7 @@ hipError_t launch_distance(int deKind, const FrameConst *frame, const float *pts
 		case kBoxDelta: return launch_distance_t<kBoxDelta>(frame, pts, n, detail, dist, iters, stream);
 		case kIfsAnalytic:
 			return launch_distance_t<kIfsAnalytic>(frame, pts, n, detail, dist, iters, stream);
+		case kBulbD: return launch_distance_t<kBulbD>(frame, pts, n, detail, dist, iters, stream);
 		default: return hipErrorInvalidValue;
 	}
 }

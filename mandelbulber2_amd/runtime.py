@@ -86,6 +86,7 @@ MBX_TUNE_GRAPH = 2
 MBX_TUNE_DELTA_DE = 3
 MBX_TUNE_TILE_WIDTH = 4
 MBX_TUNE_FAR_FIELD = 5
+MBX_TUNE_PRECISION = 6
 KERNEL_GRID = 0
 KERNEL_PLAIN = 1
 KERNEL_PERSISTENT = 2
@@ -228,12 +229,15 @@ class Context:
         return d, it
 
 
-def render_to_host(params, device=0, kernel=KERNEL_GRID):
-    """Upload, render and copy all planes back as numpy arrays (plus work statistics)."""
+def render_to_host(params, device=0, kernel=KERNEL_GRID, tune=None):
+    """Upload, render and copy all planes back as numpy arrays (plus work statistics and the
+    kernel kind the scene was specialised for).  `tune` maps MBX_TUNE_* keys to values."""
     import torch
     ctx = Context(device)
     try:
         ctx.tune(MBX_TUNE_KERNEL, kernel)
+        for k, v in (tune or {}).items():
+            ctx.tune(k, v)
         ctx.upload(params)
         bufs = ctx.alloc_outputs()
         ctx.render(bufs)
@@ -241,6 +245,7 @@ def render_to_host(params, device=0, kernel=KERNEL_GRID):
         torch.cuda.synchronize(device)
         out = {k: v.cpu().numpy() for k, v in bufs.items() if not k.startswith("_")}
         out.update(st)
+        out["de_kind"] = ctx.scene_info()[3]
         return out
     finally:
         ctx.close()
