@@ -2,7 +2,8 @@
 
 A step is one full frame (1920x1080, 1 ray per pixel: march + binary search + 6-tap normal +
 Lambert shading) rendered by libmbx.so from constants already resident in HBM, launched as the
-batch grid kernel (one 64-lane workgroup per tile) through a captured HIP graph.  The JSON line
+batch grid kernel (one 64-lane workgroup per tile; a counter-clear kernel then the render kernel,
+launched directly: a captured graph of the same two kernels costs ~10 us more per frame).  The JSON line
 also carries the FP32 VALU roofline of that kernel (algorithmic flops per launch over the
 launch's device time, HIP events on the launch stream), the plain per-frame kernel timed on the
 same frame, and the CPU oracle timed on this host.
@@ -99,9 +100,12 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain kernel")
     ap.add_argument("--far-field", type=int, default=None, choices=[0, 1],
                     help="MBX_TUNE_FAR_FIELD for the run (default: the library's default)")
-    ap.add_argument("--no-autotune", action="store_true",
-                    help="skip the launch-shape autotuner (tile shape x wave count) that runs "
-                         "before timing on the mandelbulb8 workload")
+    ap.add_argument("--tile-width", type=int, default=None, choices=[4, 8, 16, 32],
+                    help="MBX_TUNE_TILE_WIDTH for the run (default: the library's, 8)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the launch-shape autotuner (tile shape x wave count) before timing "
+                         "the mandelbulb8 workload instead of the library's default tile shape; "
+                         "its choice follows timing noise between shapes within ~3%% of each other")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 256 MiB device-to-device copy, whose FETCH_SIZE/WRITE_SIZE "
                          "counters calibrate the profiler's byte counts")
@@ -278,7 +282,7 @@ def bench_adjoint(args, torch, runtime, wl):
 
 
 def bench_batch(args, torch, runtime, wl):
-    """One step = the whole 64-frame batch in one grid launch (captured graph); the
+    """One step = the whole 64-frame batch in one grid launch; the
     comparison renders the same frames one plain launch per frame."""
     frames = make_scene(args.workload, 0, 0)
     stream = torch.cuda.current_stream(0)
@@ -354,13 +358,20 @@ def main():
     bufs = ctx.alloc_outputs()
     ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_GRID)
     launch_shape = None
-    if args.workload == "mandelbulb8" and not args.no_autotune:
+    if args.tile_width:
+        ctx.tune(runtime.MBX_TUNE_TILE_WIDTH, args.tile_width)
+        launch_shape = {"waves": ctx.get_tune(runtime.MBX_TUNE_WAVES), "tile_width": args.tile_width,
+                        "source": "--tile-width"}
+    elif args.workload == "mandelbulb8" and args.autotune:
         waves, _ = ctx.autotune(bufs, reps=2)
         launch_shape = {"waves": waves, "tile_width": ctx.get_tune(runtime.MBX_TUNE_TILE_WIDTH),
                         "source": "mbx_autotune"}
     else:
         launch_shape = {"waves": ctx.get_tune(runtime.MBX_TUNE_WAVES),
                         "tile_width": ctx.get_tune(runtime.MBX_TUNE_TILE_WIDTH), "source": "default"}
+    # the bench brackets each launch with its own events; the library's pair would add a second
+    # pair of stream markers per frame
+    ctx.tune(runtime.MBX_TUNE_TIMING, 0)
     wall, launch_ms = time_kernel(torch, ctx, bufs, stream, args.steps, args.warmup)
     st = ctx.stats()  # counters of the last launch (identical every frame)
 

@@ -308,7 +308,9 @@ int mbx_render_batch(mbx_ctx *ctx, const mbx_params *frames, const mbx_outputs *
                                 (default), 1 = one plain launch per frame, 2 = persistent waves
                                 draining tile queues */
 #define MBX_TUNE_WAVES 1     /* persistent kernel: number of 64-lane waves launched (0 = default) */
-#define MBX_TUNE_GRAPH 2     /* modes 0 and 2: 1 = replay a captured HIP graph (default), 0 = direct */
+#define MBX_TUNE_GRAPH 2     /* modes 0 and 2: 1 = replay a captured HIP graph, 0 = direct launches
+                                (default: with frame-sized kernels the graph launch costs ~10 us
+                                more per render than launching its two kernels directly) */
 #define MBX_TUNE_DELTA_DE 3  /* delta DE: 0 = the reference's finite differences (default),
                                 1 = forward-mode tangents (smooth limit of the same quotient) */
 #define MBX_TUNE_TILE_WIDTH 4 /* pixels per tile row: 4, 8 (default), 16 or 32; a tile is 64
@@ -322,6 +324,9 @@ int mbx_render_batch(mbx_ctx *ctx, const mbx_params *frames, const mbx_outputs *
                                  1/256 of the view's smallest distance threshold (deep zooms);
                                  1 = always FP32; 2 = always FP64.  Mandelbox scenes always march in
                                  FP64, Kaleidoscopic IFS scenes in FP32 */
+#define MBX_TUNE_TIMING 7     /* 1 (default) = bracket each mbx_render / mbx_render_batch with events so
+                                 that mbx_get_stats reports its device time; 0 = no events (each event
+                                 marker costs the stream ~10 us between launches; kernel_ms reads 0) */
 
 /* Set a launch or numerics option (see MBX_TUNE_*); scene options apply from the next
  * mbx_scene_upload / mbx_render_batch.  No reference counterpart: the OpenCL engine's

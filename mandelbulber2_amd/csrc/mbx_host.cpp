@@ -29,7 +29,10 @@ struct mbx_ctx
 	hipStream_t stream = nullptr;
 	FrameConst frame{};
 	FrameConst *d_frame = nullptr;
-	unsigned long long *d_counters = nullptr;
+	unsigned long long *d_counters = nullptr; // two banks of kCounterBytes
+	int bank = 0;           // counter bank of the next launch
+	bool bankClean = false; // that bank is known to be zero
+	int statsBank = 0;      // bank holding the last launch's counters
 	hipEvent_t ev0 = nullptr, ev1 = nullptr;
 	bool sceneReady = false;
 	bool launched = false;
@@ -41,7 +44,7 @@ struct mbx_ctx
 	// persistent / batched launches
 	int kernelMode = 0; // 0 grid batch kernel, 1 plain kernel per frame, 2 persistent tile queues
 	int waves = 0;      // 0 = default
-	int useGraph = 1;
+	int useGraph = 0;  // replay a captured HIP graph (MBX_TUNE_GRAPH); direct launches by default
 	int deltaTangent = 0;
 	int numCUs = 256;
 	FrameConst *d_frames = nullptr;
@@ -56,6 +59,8 @@ struct mbx_ctx
 	int tileShift = 3; // 8x8-pixel tiles
 	int farField = 1;  // far-field march steps without DE evaluation (MBX_TUNE_FAR_FIELD)
 	int precision = 0; // Mandelbulb march precision (MBX_TUNE_PRECISION): 0 auto, 1 FP32, 2 FP64
+	int timing = 1;     // time renders with events (MBX_TUNE_TIMING)
+	bool timedLast = false; // the last launch was bracketed by ev0/ev1
 	int batchFull = 1; // the staged frames need the Monte Carlo / GI / shadow / AO code
 	// differentiable render
 	mbx::DiffConst diff{};
@@ -80,6 +85,25 @@ int fail(mbx_ctx *ctx, int code, const std::string &msg)
 int hip_fail(mbx_ctx *ctx, hipError_t e, const char *where)
 {
 	return fail(ctx, MBX_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+// Event around a render launch, when kernel timing is on (each marker costs the stream ~10 us).
+hipError_t mark(mbx_ctx *ctx, hipEvent_t ev)
+{
+	return ctx->timing ? hipEventRecord(ev, ctx->stream) : hipSuccess;
+}
+
+unsigned long long *counter_bank(mbx_ctx *ctx, int b)
+{
+	return ctx->d_counters + (size_t)b * (mbx::kCounterBytes / sizeof(unsigned long long));
+}
+
+// Counters of a launch that clears its own bank first (clear kernel): the bank is dirty afterwards.
+unsigned long long *counters_cleared_by_launch(mbx_ctx *ctx)
+{
+	ctx->statsBank = ctx->bank;
+	ctx->bankClean = false;
+	return counter_bank(ctx, ctx->bank);
 }
 
 // ---- double-precision vector helpers for the host-side derivation -------------------------
@@ -805,15 +829,33 @@ int enqueue_batch(mbx_ctx *ctx, int kind, int nFrames, unsigned int tiles)
 	a.jobs = ctx->d_jobs;
 	a.numFrames = nFrames;
 	a.queue = ctx->d_queue;
-	a.counters = ctx->d_counters;
 	a.tileShift = ctx->tileShift;
+	if (mode == mbx::kBatchGrid && !ctx->useGraph)
+	{
+		// direct grid launch: accumulate into this bank (cleared by the previous grid launch, else
+		// by the clear kernel) while workgroup 0 zeroes the other bank for the next launch
+		a.counters = counter_bank(ctx, ctx->bank);
+		a.clearFirst = ctx->bankClean ? 0 : 1;
+		a.clearNext = counter_bank(ctx, ctx->bank ^ 1);
+		ctx->statsBank = ctx->bank;
+		ctx->bank ^= 1;
+		ctx->bankClean = true;
+	}
+	else
+	{
+		// graphs bake the counter pointer in: bank 0, cleared by the clear kernel inside the graph
+		if (ctx->useGraph) ctx->bank = 0;
+		a.counters = counters_cleared_by_launch(ctx);
+		a.clearFirst = 1;
+		a.clearNext = nullptr;
+	}
 	a.full = ctx->batchFull;
 	a.mode = mode;
 	a.totalTiles = tiles;
 	hipError_t e;
 	if (!ctx->useGraph)
 	{
-		if ((e = hipEventRecord(ctx->ev0, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
+		if ((e = mark(ctx, ctx->ev0)) != hipSuccess) return hip_fail(ctx, e, "event");
 		if ((e = mbx::launch_batch(kind, a, waves, ctx->stream)) != hipSuccess)
 			return hip_fail(ctx, e, "batch launch");
 		return MBX_OK;
@@ -858,7 +900,7 @@ int enqueue_batch(mbx_ctx *ctx, int kind, int nFrames, unsigned int tiles)
 		ctx->graphTiles = tiles;
 	}
 	// timing starts after any (re)capture, which is host work
-	if ((e = hipEventRecord(ctx->ev0, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
+	if ((e = mark(ctx, ctx->ev0)) != hipSuccess) return hip_fail(ctx, e, "event");
 	if ((e = hipGraphLaunch(ctx->graphExec, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "graph launch");
 	return MBX_OK;
 }
@@ -887,14 +929,15 @@ int render_frames(mbx_ctx *ctx, const std::vector<FrameConst> &frames, const mbx
 	if (ctx->kernelMode == 1)
 	{
 		// one plain launch per frame (the reference engine's job-per-frame shape)
-		if ((e = mbx::launch_clear(ctx->d_queue, ctx->d_counters, ctx->stream)) != hipSuccess)
+		unsigned long long *counters = counters_cleared_by_launch(ctx);
+		if ((e = mbx::launch_clear(ctx->d_queue, counters, ctx->stream)) != hipSuccess)
 			return hip_fail(ctx, e, "clear counters");
-		if ((e = hipEventRecord(ctx->ev0, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
+		if ((e = mark(ctx, ctx->ev0)) != hipSuccess) return hip_fail(ctx, e, "event");
 		for (size_t i = 0; i < frames.size(); i++)
 		{
 			const mbx::Outputs o{outs[i].rgb, outs[i].depth, outs[i].iters, outs[i].mask, outs[i].normal,
 				outs[i].steps};
-			e = mbx::launch_render(kind, ctx->d_frames + i, frames[i].width, frames[i].height, o, ctx->d_counters,
+			e = mbx::launch_render(kind, ctx->d_frames + i, frames[i].width, frames[i].height, o, counters,
 				ctx->tileShift, frame_needs_full(frames[i]), ctx->stream);
 			if (e != hipSuccess) return hip_fail(ctx, e, "render launch");
 		}
@@ -904,8 +947,9 @@ int render_frames(mbx_ctx *ctx, const std::vector<FrameConst> &frames, const mbx
 		rc = enqueue_batch(ctx, kind, (int)frames.size(), (unsigned int)tiles);
 		if (rc != MBX_OK) return rc;
 	}
-	if ((e = hipEventRecord(ctx->ev1, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
+	if ((e = mark(ctx, ctx->ev1)) != hipSuccess) return hip_fail(ctx, e, "event");
 	ctx->launched = true;
+	ctx->timedLast = ctx->timing != 0;
 	ctx->rays = rays;
 	return MBX_OK;
 }
@@ -1012,7 +1056,7 @@ int mbx_ctx_create(int device, void *stream, mbx_ctx **out)
 	}
 	ctx->stream = (hipStream_t)stream;
 	if ((e = hipMalloc((void **)&ctx->d_frame, sizeof(FrameConst))) != hipSuccess ||
-		(e = hipMalloc((void **)&ctx->d_counters, mbx::kCounterBytes)) != hipSuccess ||
+		(e = hipMalloc((void **)&ctx->d_counters, 2 * mbx::kCounterBytes)) != hipSuccess ||
 		(e = hipMalloc((void **)&ctx->d_queue, mbx::kTileQueueBytes)) != hipSuccess ||
 		(e = hipEventCreate(&ctx->ev0)) != hipSuccess || (e = hipEventCreate(&ctx->ev1)) != hipSuccess)
 	{
@@ -1107,7 +1151,7 @@ int mbx_render_diff(mbx_ctx *ctx, const mbx_outputs *out, mbx_ray_checkpoint *ch
 	if (rc != MBX_OK) return rc;
 	hipError_t e;
 	if ((e = hipEventRecord(ctx->ev0, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
-	if ((e = mbx::launch_clear(ctx->d_queue, ctx->d_counters, ctx->stream)) != hipSuccess)
+	if ((e = mbx::launch_clear(ctx->d_queue, counters_cleared_by_launch(ctx), ctx->stream)) != hipSuccess)
 		return hip_fail(ctx, e, "clear");
 	e = mbx::launch_render_diff(ctx->frame.deKind, ctx->d_frame, ctx->d_diff, ctx->frame.width,
 		ctx->frame.height, out->rgb, out->depth, out->iters, out->mask, out->normal,
@@ -1115,6 +1159,7 @@ int mbx_render_diff(mbx_ctx *ctx, const mbx_outputs *out, mbx_ray_checkpoint *ch
 	if (e != hipSuccess) return hip_fail(ctx, e, "differentiable render launch");
 	if ((e = hipEventRecord(ctx->ev1, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
 	ctx->launched = true;
+	ctx->timedLast = true;
 	ctx->rays = (uint64_t)ctx->frame.width * ctx->frame.height;
 	return MBX_OK;
 }
@@ -1207,6 +1252,9 @@ int mbx_tune(mbx_ctx *ctx, int key, int value)
 			if (value < 0 || value > 2) return fail(ctx, MBX_ERR_INVALID, "precision must be 0, 1 or 2");
 			ctx->precision = value;
 			return MBX_OK;
+		case MBX_TUNE_TIMING:
+			ctx->timing = value ? 1 : 0;
+			return MBX_OK;
 		case MBX_TUNE_TILE_WIDTH:
 		{
 			int shift = -1;
@@ -1232,6 +1280,7 @@ int mbx_get_tune(mbx_ctx *ctx, int key, int *value)
 		case MBX_TUNE_TILE_WIDTH: *value = 1 << ctx->tileShift; return MBX_OK;
 		case MBX_TUNE_FAR_FIELD: *value = ctx->farField; return MBX_OK;
 		case MBX_TUNE_PRECISION: *value = ctx->precision; return MBX_OK;
+		case MBX_TUNE_TIMING: *value = ctx->timing; return MBX_OK;
 		default: return fail(ctx, MBX_ERR_INVALID, "unknown tuning key");
 	}
 }
@@ -1240,6 +1289,14 @@ int mbx_autotune(mbx_ctx *ctx, const mbx_outputs *out, int reps, int *best_waves
 {
 	if (!ctx || !out || reps < 1) return fail(ctx, MBX_ERR_INVALID, "bad argument");
 	if (!ctx->sceneReady) return fail(ctx, MBX_ERR_INVALID, "no scene uploaded");
+	// the tuner times its renders with the context's events whatever MBX_TUNE_TIMING says
+	struct TimingOn
+	{
+		mbx_ctx *c;
+		int saved;
+		~TimingOn() { c->timing = saved; }
+	} timingOn{ctx, ctx->timing};
+	ctx->timing = 1;
 	const int savedMode = ctx->kernelMode;
 	// the wave count matters only to the persistent kernel
 	const std::vector<int> perCU = ctx->kernelMode == 2 ? std::vector<int>{4, 8, 12, 16, 20, 24, 32}
@@ -1290,12 +1347,18 @@ int mbx_get_stats(mbx_ctx *ctx, mbx_stats *stats)
 {
 	if (!ctx || !stats) return fail(ctx, MBX_ERR_INVALID, "null argument");
 	if (!ctx->launched) return fail(ctx, MBX_ERR_INVALID, "nothing rendered yet");
-	hipError_t e = hipEventSynchronize(ctx->ev1);
-	if (e != hipSuccess) return hip_fail(ctx, e, "render");
+	hipError_t e;
 	float ms = 0.0f;
-	if ((e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1)) != hipSuccess) return hip_fail(ctx, e, "timing");
+	if (ctx->timedLast)
+	{
+		if ((e = hipEventSynchronize(ctx->ev1)) != hipSuccess) return hip_fail(ctx, e, "render");
+		if ((e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1)) != hipSuccess) return hip_fail(ctx, e, "timing");
+	}
+	else if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+		return hip_fail(ctx, e, "render");
 	std::vector<unsigned long long> slots((size_t)mbx::kCounterSlots * mbx::kCounterStride);
-	if ((e = hipMemcpy(slots.data(), ctx->d_counters, mbx::kCounterBytes, hipMemcpyDeviceToHost)) != hipSuccess)
+	if ((e = hipMemcpy(slots.data(), counter_bank(ctx, ctx->statsBank), mbx::kCounterBytes, hipMemcpyDeviceToHost)) !=
+		hipSuccess)
 		return hip_fail(ctx, e, "read counters");
 	unsigned long long c[mbx::kCounters] = {0, 0, 0};
 	for (unsigned int sl = 0; sl < mbx::kCounterSlots; sl++)

@@ -73,6 +73,8 @@ struct BatchArgs
 	unsigned int totalTiles;
 	unsigned int *queue;           // kTileQueues tile counters (kTileQueueBytes)
 	unsigned long long *counters;  // work counter slots (kCounterBytes)
+	int clearFirst;                // launch the clear kernel (queue + counters) before the batch
+	unsigned long long *clearNext; // grid mode: counter slots workgroup 0 zeroes for the next launch
 };
 
 // Batch launch modes: one workgroup per tile of every frame (grid), or a fixed grid of

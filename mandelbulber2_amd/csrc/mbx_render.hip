@@ -41,12 +41,19 @@ __global__ __launch_bounds__(64, KIND >= kBoxAnalytic ? 2 : 1) void render_kerne
 // frame constants stay in scalar registers.  One launch (replayed from a captured graph)
 // replaces the reference's one-job-per-frame animation loop; the hardware dispatcher balances
 // the tiles without any atomic.
+static_assert(kCounterSlots == 64, "workgroup 0 of batch_kernel clears one counter slot per lane");
+
+// When clearNext is set, workgroup 0 zeroes those counter slots (another bank than `counters`):
+// the next launch accumulates there without a clear kernel in front of it.
 template <int KIND, bool FULL>
 __global__ __launch_bounds__(64, KIND >= kBoxAnalytic ? 2 : 1) void batch_kernel(const FrameConst *__restrict__ frames,
-	const FrameJob *__restrict__ jobs, int numFrames, unsigned long long *__restrict__ counters, int tileShift)
+	const FrameJob *__restrict__ jobs, int numFrames, unsigned long long *__restrict__ counters, int tileShift,
+	unsigned long long *__restrict__ clearNext)
 {
 	const int lane = threadIdx.x;
 	const unsigned int t = blockIdx.x;
+	if (clearNext && t == 0)
+		for (unsigned int k = 0; k < kCounters; k++) clearNext[lane * kCounterStride + k] = 0ull;
 	int lo = 0, hi = numFrames - 1;
 	while (lo < hi)
 	{
@@ -233,18 +240,21 @@ static hipError_t launch_batch_t(const BatchArgs &a, int numWaves, bool full, hi
 	{
 		if (full)
 			hipLaunchKernelGGL((batch_kernel<KIND, true>), dim3(a.totalTiles), dim3(64), 0, stream, a.frames, a.jobs,
-				a.numFrames, a.counters, a.tileShift);
+				a.numFrames, a.counters, a.tileShift, a.clearNext);
 		else
 			hipLaunchKernelGGL((batch_kernel<KIND, false>), dim3(a.totalTiles), dim3(64), 0, stream, a.frames, a.jobs,
-				a.numFrames, a.counters, a.tileShift);
+				a.numFrames, a.counters, a.tileShift, a.clearNext);
 	}
 	return hipGetLastError();
 }
 
 hipError_t launch_batch(int deKind, BatchArgs a, int numWaves, hipStream_t stream)
 {
-	hipError_t e = launch_clear(a.queue, a.counters, stream);
-	if (e != hipSuccess) return e;
+	if (a.clearFirst)
+	{
+		hipError_t e = launch_clear(a.queue, a.counters, stream);
+		if (e != hipSuccess) return e;
+	}
 	switch (deKind)
 	{
 		case kBulb8: return launch_batch_t<kBulb8>(a, numWaves, a.full != 0, stream);

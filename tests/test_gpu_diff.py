@@ -156,7 +156,7 @@ def test_autograd_function_runs_the_adjoint(dmod, oracle_mod):
     bufs2, _ = r.render(theta0 + step)
     loss2 = float(((bufs2["rgb"] - tgt) ** 2).sum())
     r.close()
-    assert loss2 < float(loss)
+    assert loss2 < float(loss.detach())
 
 
 def test_differentiable_render_rejects_other_scenes(dmod):

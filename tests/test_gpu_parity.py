@@ -361,3 +361,51 @@ def test_autotune_picks_a_launch_shape(rt):
     ctx.render(bufs)
     assert ctx.stats()["rays"] == 256 * 192
     ctx.close()
+
+
+def test_counters_report_one_launch(rt):
+    """Each launch reports only its own work.  Direct grid launches accumulate into one of two
+    counter banks and zero the other for the next launch; plain, graph and persistent launches
+    clear their bank with a clear kernel.  Any order of modes, and several renders before one
+    statistics read, give the single-launch counts."""
+    p = scene.mandelbulb8(160, 120)
+    ctx = rt.Context(0)
+    ctx.upload(p)
+    out = ctx.alloc_outputs()
+    ref = None
+    seq = [(GRID, 0), (GRID, 0), (GRID, 0), (PLAIN, 0), (GRID, 0), (GRID, 1), (GRID, 1), (GRID, 0),
+           (GRID, 0), (PERSISTENT, 0), (GRID, 0), (PERSISTENT, 1), (GRID, 0)]
+    for mode, graph in seq:
+        ctx.tune(rt.MBX_TUNE_KERNEL, mode)
+        ctx.tune(rt.MBX_TUNE_GRAPH, graph)
+        for _ in range(3):
+            ctx.render(out)
+        st = ctx.stats()
+        key = (st["de_evals"], st["iterations"], st["far_steps"])
+        if ref is None:
+            ref = key
+            assert ref[0] > 0
+        assert key == ref, (mode, graph, key, ref)
+    ctx.close()
+
+
+def test_untimed_renders_keep_counters(rt):
+    """MBX_TUNE_TIMING 0 drops the context's event pair: the counters are unchanged and
+    kernel_ms reads 0; switching timing back on reports a positive device time again."""
+    p = scene.mandelbulb8(160, 120)
+    ctx = rt.Context(0)
+    ctx.upload(p)
+    out = ctx.alloc_outputs()
+    ctx.render(out)
+    ref = ctx.stats()
+    assert ref["kernel_ms"] > 0
+    ctx.tune(rt.MBX_TUNE_TIMING, 0)
+    for _ in range(3):
+        ctx.render(out)
+    st = ctx.stats()
+    assert st["kernel_ms"] == 0
+    assert (st["de_evals"], st["iterations"]) == (ref["de_evals"], ref["iterations"])
+    ctx.tune(rt.MBX_TUNE_TIMING, 1)
+    ctx.render(out)
+    assert ctx.stats()["kernel_ms"] > 0
+    ctx.close()
