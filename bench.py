@@ -115,7 +115,8 @@ def parse():
 def measured_traffic(workload, w, h):
     """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this
     workload (copy-calibrated), or None when there is none for this size."""
-    name = {"mandelbulb8": "persistent_kernel_bulb8_mandelbulb8_1080p.md"}.get(workload)
+    name = {"mandelbulb8": "batch_kernel_bulb8_mandelbulb8_1080p.md",
+            "mandelbox": "batch_kernel_boxdelta_mandelbox_1080p.md"}.get(workload)
     if not name or (w, h) != (1920, 1080):
         return None, None
     path = os.path.join(ROOT, "profiles", name)
