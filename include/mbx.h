@@ -28,13 +28,16 @@
 extern "C" {
 #endif
 
-#define MBX_ABI_VERSION 2
+#define MBX_ABI_VERSION 3
 
 /* array sizes as in mandelbulber2/src/fractal.h:52-54 */
 #define MBX_IFS_VECTOR_COUNT 9
 #define MBX_MANDELBOX_FOLDS 2
 /* upper bound of explicitly supplied ambient-occlusion directions */
 #define MBX_MAX_AO_VECTORS 64
+/* upper bound of the stops of a colour gradient (the first colour counts twice: at its position
+ * and again at 1.0) */
+#define MBX_MAX_GRADIENT_STOPS 64
 
 /* status codes */
 #define MBX_OK 0
@@ -95,6 +98,7 @@ typedef struct mbx_fractal_mandelbox {
 	double colorFactor[3];        /* "mandelbox_color" */
 	double colorFactorSp1;        /* "mandelbox_color_Sp1" */
 	double colorFactorSp2;        /* "mandelbox_color_Sp2" */
+	double colorFactorR;          /* "mandelbox_color_R" */
 	int32_t rotationsEnabled;     /* "mandelbox_rotations_enabled" */
 	int32_t mainRotationEnabled;  /* "mandelbox_main_rotation_enabled" */
 } mbx_fractal_mandelbox;
@@ -223,6 +227,44 @@ typedef struct mbx_params {
 	 * objects_tree.cpp:292-382): points enter the formula as p_local = R p with
 	 * R = Rz(z) Ry(y) Rx(x) (CRotationMatrix::SetRotation2, algebra.cpp:296-302) */
 	double formulaRotation[3];
+
+	/* palette surface colouring of material #1 (SurfaceColour, shader_surface_color.cpp:39-108;
+	 * the colouring pass of Compute, compute_fractal.cpp:343-436; CalculateColorIndex,
+	 * fractal_coloring.cpp:111-343; cColorGradient, color_gradient.cpp:138-360) */
+	int32_t useColorsFromPalette;     /* "mat1_use_colors_from_palette" */
+	int32_t surfaceGradientEnable;    /* "mat1_surface_gradient_enable" */
+	int32_t coloringAlgorithm;        /* "mat1_fractal_coloring_algorithm": 0 standard |z|, 1 z.p,
+	                                     2 sphere, 3 cross, 4 line */
+	int32_t coloringPreV215;          /* "mat1_fractal_coloring_color_preV215_false" */
+	int32_t coloring4D;               /* "mat1_fractal_coloring_color_4D_enabled_false" */
+	int32_t surfaceGradientCount;     /* stops of "mat1_surface_color_gradient" as cColorGradient holds
+	                                     them: sorted by position, the first colour repeated at 1.0
+	                                     (2 .. MBX_MAX_GRADIENT_STOPS) */
+	double coloringSpeed;             /* "mat1_coloring_speed" */
+	double coloringPaletteOffset;     /* "mat1_coloring_palette_offset" */
+	double coloringSphereRadius;      /* "mat1_fractal_coloring_sphere_radius" */
+	double coloringLineDirection[3];  /* "mat1_fractal_coloring_line_direction" (x, y, z) */
+	double surfaceGradientPosition[MBX_MAX_GRADIENT_STOPS]; /* 0 .. 1 */
+	double surfaceGradientColor[MBX_MAX_GRADIENT_STOPS][3]; /* 8-bit channel values, 0 .. 255 */
+
+	/* specular highlights of material #1 (SpecularHighlightCombined / SpecularHighlight,
+	 * shader_specular_highlight_combined.cpp:39-68, shader_specular_highlight.cpp:37-73) */
+	int32_t specularPlasticEnable;    /* "mat1_specular_plastic_enable" */
+	int32_t metallic;                 /* "mat1_metallic" */
+	double specular;                  /* "mat1_specular" (plastic intensity) */
+	double specularWidth;             /* "mat1_specular_width" */
+	double specularMetallic;          /* "mat1_specular_metallic" */
+	double specularMetallicWidth;     /* "mat1_specular_metallic_width" */
+	double specularMetallicRoughness; /* "mat1_specular_metallic_roughness" */
+	double specularColor[3];          /* "mat1_specular_color" */
+	double luminosityEmissive;        /* "mat1_luminosity_emissive" (emission seen by GI bounces) */
+
+	/* glow (VolumetricShader, shader_volumetric.cpp:82-90, 196-204) */
+	int32_t glowEnabled;              /* "glow_enabled" */
+	int32_t reserved2;
+	double glowIntensity;             /* "glow_intensity" */
+	double glowColor1[3];             /* "glow_color_1" */
+	double glowColor2[3];             /* "glow_color_2" */
 } mbx_params;
 
 /* Per-pixel output planes (device pointers; any may be NULL except where noted).
@@ -395,9 +437,12 @@ void mbx_params_default(mbx_params *p);
 #define MBX_FRACT_HYBRID 0x1u          /* hybrid sequence, or a boolean tree of several objects */
 #define MBX_FRACT_PRIMITIVES 0x2u      /* primitive objects (plane, sphere, water, ...) */
 #define MBX_FRACT_TRANSFORM 0x4u       /* node position, scale or repeat; the limits box */
-#define MBX_FRACT_PALETTE 0x8u         /* surface colour from the palette gradient */
-#define MBX_FRACT_SPECULAR 0x10u       /* specular highlights (material specular > 0) */
-#define MBX_FRACT_GLOW 0x20u           /* glow (volumetric shader) */
+#define MBX_FRACT_PALETTE 0x8u         /* palette colouring beyond the surface gradient: colour by
+                                          numbers, or specular / diffuse / luminosity / roughness /
+                                          reflectance / transparency gradients */
+#define MBX_FRACT_SPECULAR 0x10u       /* (unused: specular highlights are rendered) */
+#define MBX_FRACT_GLOW 0x20u           /* glow with a volumetric DE factor other than 1, or on GI
+                                          bounces (MC_global_illumination_volumetric) */
 #define MBX_FRACT_FOG 0x40u            /* basic, volumetric, iteration or distance fog; clouds */
 #define MBX_FRACT_REFLECTIONS 0x80u    /* ray-traced reflections or transparency */
 #define MBX_FRACT_LIGHTS 0x100u        /* auxiliary, random, fake or volumetric lights; light #1 not

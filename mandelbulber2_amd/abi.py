@@ -10,6 +10,7 @@ import ctypes as C
 MBX_IFS_VECTOR_COUNT = 9
 MBX_MANDELBOX_FOLDS = 2
 MBX_MAX_AO_VECTORS = 64
+MBX_MAX_GRADIENT_STOPS = 64
 
 MBX_FORMULA_NONE = 0
 MBX_FORMULA_MANDELBULB = 2
@@ -47,6 +48,7 @@ class MbxFractalMandelbox(C.Structure):
         ("colorFactor", D3),
         ("colorFactorSp1", C.c_double),
         ("colorFactorSp2", C.c_double),
+        ("colorFactorR", C.c_double),
         ("rotationsEnabled", C.c_int32),
         ("mainRotationEnabled", C.c_int32),
     ]
@@ -160,6 +162,32 @@ class MbxParams(C.Structure):
         ("randomSeed", C.c_uint32),
         ("fractal", MbxFractal),
         ("formulaRotation", C.c_double * 3),
+        ("useColorsFromPalette", C.c_int32),
+        ("surfaceGradientEnable", C.c_int32),
+        ("coloringAlgorithm", C.c_int32),
+        ("coloringPreV215", C.c_int32),
+        ("coloring4D", C.c_int32),
+        ("surfaceGradientCount", C.c_int32),
+        ("coloringSpeed", C.c_double),
+        ("coloringPaletteOffset", C.c_double),
+        ("coloringSphereRadius", C.c_double),
+        ("coloringLineDirection", D3),
+        ("surfaceGradientPosition", C.c_double * MBX_MAX_GRADIENT_STOPS),
+        ("surfaceGradientColor", D3 * MBX_MAX_GRADIENT_STOPS),
+        ("specularPlasticEnable", C.c_int32),
+        ("metallic", C.c_int32),
+        ("specular", C.c_double),
+        ("specularWidth", C.c_double),
+        ("specularMetallic", C.c_double),
+        ("specularMetallicWidth", C.c_double),
+        ("specularMetallicRoughness", C.c_double),
+        ("specularColor", D3),
+        ("luminosityEmissive", C.c_double),
+        ("glowEnabled", C.c_int32),
+        ("reserved2", C.c_int32),
+        ("glowIntensity", C.c_double),
+        ("glowColor1", D3),
+        ("glowColor2", D3),
     ]
 
 

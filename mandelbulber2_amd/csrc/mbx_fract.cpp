@@ -244,7 +244,22 @@ void compatibility(double ver, std::string &name, std::string &value)
 	if (ver < 2.12) replace_all(name, "gpu_", "opencl_");
 	if (ver < 2.19)
 	{
-		if (contains(name, "surface_color_palette")) replace_all(name, "surface_color_palette", "surface_color_gradient");
+		if (contains(name, "surface_color_palette"))
+		{
+			// a list of colours becomes "position colour" pairs spread evenly over 0..10000
+			replace_all(name, "surface_color_palette", "surface_color_gradient");
+			std::vector<std::string> parts = split(value, ' ');
+			size_t numberOfColors = parts.size();
+			if (!parts.empty() && parts.back().size() < 6) numberOfColors--;
+			const double step = 1.0 / (double)numberOfColors;
+			std::string v;
+			for (size_t i = 0; i < numberOfColors; i++)
+			{
+				if (i > 0) v += " ";
+				v += std::to_string((int)(i * step * 10000.0)) + " " + parts[i];
+			}
+			value = v;
+		}
 		if (contains(name, "luminosity_color_thesame"))
 			replace_all(name, "luminosity_color_thesame", "luminosity_gradient_enable");
 		if (contains(name, "coloring_palette_size") || contains(name, "coloring_random_seed") ||
@@ -303,7 +318,7 @@ bool is_fractal_param(const std::string &base)
 		"IFS_abs_x", "IFS_abs_y", "IFS_abs_z", "IFS_menger_sponge_mode", "mandelbox_scale",
 		"mandelbox_folding_limit", "mandelbox_folding_value", "mandelbox_folding_min_radius",
 		"mandelbox_folding_fixed_radius", "mandelbox_sharpness", "mandelbox_offset", "mandelbox_rotation_main",
-		"mandelbox_color", "mandelbox_color_Sp1", "mandelbox_color_Sp2", "mandelbox_main_rotation_enabled",
+		"mandelbox_color", "mandelbox_color_Sp1", "mandelbox_color_Sp2", "mandelbox_color_R", "mandelbox_main_rotation_enabled",
 		"mandelbox_rotations_enabled", "formula_position", "formula_rotation", "formula_repeat", "formula_scale",
 		"formula_material_id"};
 	return names.count(base) != 0;
@@ -402,6 +417,56 @@ int fail(mbx_fract_info *info, const std::string &msg)
 	if (info) std::snprintf(info->message, sizeof(info->message), "%s", msg.c_str());
 	return MBX_ERR_INVALID;
 }
+
+// cColorGradient::SetColorsFromString + SortGradient (color_gradient.cpp:271-282, 305-354):
+// "position colour" pairs, positions in 1/10000, colours 24-bit hexadecimal; the first colour is
+// also placed at 1.0.  Fewer than two parts give white at 0 and 1.  Sorting keeps the order of
+// equal positions (the reference's std::sort does too for gradients of up to 16 stops, where it
+// runs an insertion sort).  Stops past MBX_MAX_GRADIENT_STOPS are dropped.
+int parse_gradient(const std::string &text, double pos[], double col[][3])
+{
+	struct Stop
+	{
+		float position;
+		int rgb[3];
+	};
+	std::vector<Stop> stops;
+	std::vector<std::string> parts = split(text, ' ');
+	if (parts.size() < 2)
+	{
+		stops.push_back({0.0f, {255, 255, 255}});
+		stops.push_back({1.0f, {255, 255, 255}});
+	}
+	else
+	{
+		float position = 0.0f;
+		for (size_t i = 0; i < parts.size(); i++)
+		{
+			if (parts[i].empty()) continue;
+			if (i % 2 == 0)
+				position = (float)to_int(parts[i]) / 10000.0f;
+			else
+			{
+				const long long hex = to_int(parts[i], 16);
+				stops.push_back({position, {(int)(hex / 65536), (int)((hex / 256) % 256), (int)(hex % 256)}});
+				if (i == 1) stops.push_back({1.0f, {stops[0].rgb[0], stops[0].rgb[1], stops[0].rgb[2]}});
+			}
+		}
+	}
+	std::stable_sort(stops.begin(), stops.end(), [](const Stop &a, const Stop &b) { return a.position < b.position; });
+	const int n = (int)std::min(stops.size(), (size_t)MBX_MAX_GRADIENT_STOPS);
+	for (int i = 0; i < MBX_MAX_GRADIENT_STOPS; i++)
+	{
+		pos[i] = i < n ? stops[i].position : 0.0;
+		for (int c = 0; c < 3; c++) col[i][c] = i < n ? stops[i].rgb[c] : 0.0;
+	}
+	return n;
+}
+
+// material surface gradient default (initparameters.cpp, InitMaterialParams)
+const char *const kDefaultSurfaceGradient =
+	"0 fd6029 1000 698403 2000 fff59b 3000 f5bd22 4000 0b5e87 5000 c68876 6000 a51c64 7000 3b9fee "
+	"8000 d4ffd4 9000 aba53c";
 
 // ------------------------------------------------------------------------------------------
 // defaults (initparameters.cpp line numbers)
@@ -509,6 +574,7 @@ extern "C" void mbx_params_default(mbx_params *p)
 	set3(mb.colorFactor, 0.03, 0.05, 0.07); // :761
 	mb.colorFactorSp1 = 0.2;               // :765
 	mb.colorFactorSp2 = 0.2;
+	mb.colorFactorR = 0.0;                 // :764
 	mbx_fractal_ifs &ifs = f.IFS;
 	ifs.scale = 2.0;                       // :725
 	set3(ifs.offset, 1.0, 0.0, 0.0);       // :728
@@ -517,6 +583,32 @@ extern "C" void mbx_params_default(mbx_params *p)
 		set3(ifs.direction[i], 1.0, 0.0, 0.0); // :738-742
 		ifs.intensity[i] = 1.0;
 	}
+	// palette colouring of material #1 (InitMaterialParams)
+	p->useColorsFromPalette = 1;
+	p->surfaceGradientEnable = 1;
+	p->coloringAlgorithm = 0;
+	p->coloringPreV215 = 0;
+	p->coloring4D = 0;
+	p->coloringSpeed = 1.0;
+	p->coloringPaletteOffset = 0.0;
+	p->coloringSphereRadius = 1.0;
+	set3(p->coloringLineDirection, 1.0, 0.0, 0.0);
+	p->surfaceGradientCount = parse_gradient(kDefaultSurfaceGradient, p->surfaceGradientPosition, p->surfaceGradientColor);
+	// specular of material #1 (InitMaterialParams)
+	p->specularPlasticEnable = 1;
+	p->metallic = 1;
+	p->specular = 5.0;
+	p->specularWidth = 0.05;
+	p->specularMetallic = 2.0;
+	p->specularMetallicWidth = 1.0;
+	p->specularMetallicRoughness = 0.1;
+	rgb16(p->specularColor, 65535, 65535, 65535);
+	p->luminosityEmissive = 1.0;
+	// glow :222-223, :240-241
+	p->glowEnabled = 1;
+	p->glowIntensity = 0.2;
+	rgb16(p->glowColor1, 40984, 44713, 49490);
+	rgb16(p->glowColor2, 57192, 60888, 62408);
 }
 
 extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, mbx_fract_info *info)
@@ -815,14 +907,47 @@ extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, 
 		p.shading = M.d(m + "shading", 1.0, 0.0, 1e15);
 		p.luminosity = M.d(m + "luminosity", 0.0, 0.0, 1e15);
 		M.rgb(m + "luminosity_color", p.luminosityColor, 65535, 65535, 65535);
-		if (M.b(m + "use_colors_from_palette", true)) flags |= MBX_FRACT_PALETTE;
-		if (M.d(m + "specular", 5.0, 0.0, 1e15) > 0.0) flags |= MBX_FRACT_SPECULAR;
+		p.luminosityEmissive = M.d(m + "luminosity_emissive", 1.0, 0.0, 1e15);
+		// palette colouring
+		p.useColorsFromPalette = M.b(m + "use_colors_from_palette", true);
+		p.surfaceGradientEnable = M.b(m + "surface_gradient_enable", true);
+		p.coloringAlgorithm = (int)M.i(m + "fractal_coloring_algorithm", 0, 0, 4);
+		p.coloringPreV215 = M.b(m + "fractal_coloring_color_preV215_false", false);
+		p.coloring4D = M.b(m + "fractal_coloring_color_4D_enabled_false", false);
+		p.coloringSpeed = M.d(m + "coloring_speed", 1.0, 0.0, 1e15);
+		p.coloringPaletteOffset = M.d(m + "coloring_palette_offset", 0.0, 0.0, 256.0);
+		p.coloringSphereRadius = M.d(m + "fractal_coloring_sphere_radius", 1.0, 0.0, 1e20);
+		M.v3(m + "fractal_coloring_line_direction", p.coloringLineDirection, 1.0, 0.0, 0.0);
+		const std::string *gradient = M.raw(m + "surface_color_gradient");
+		const std::string gradientText = gradient ? *gradient : std::string(kDefaultSurfaceGradient);
+		p.surfaceGradientCount = parse_gradient(gradientText, p.surfaceGradientPosition, p.surfaceGradientColor);
+		if (ver < 2.19)
+		{
+			// Compatibility2 (settings.cpp:1563-1593): palette offset and speed were per colour
+			const double paletteSize = (double)(split(gradientText, ' ').size() / 2);
+			p.coloringPaletteOffset = std::min(std::max(p.coloringPaletteOffset / paletteSize, 0.0), 256.0);
+			p.coloringSpeed = std::min(std::max(p.coloringSpeed * 10.0 / paletteSize, 0.0), 1e15);
+		}
+		if (p.useColorsFromPalette &&
+			(M.b(m + "fractal_coloring_extra_color_enabled_false", false) ||
+				M.b(m + "specular_gradient_enable", false) || M.b(m + "diffuse_gradient_enable", false) ||
+				M.b(m + "luminosity_gradient_enable", false) || M.b(m + "roughness_gradient_enable", false) ||
+				M.b(m + "reflectance_gradient_enable", false) || M.b(m + "transparency_gradient_enable", false)))
+			flags |= MBX_FRACT_PALETTE;
+		// specular
+		p.specularPlasticEnable = M.b(m + "specular_plastic_enable", true);
+		p.metallic = M.b(m + "metallic", true);
+		p.specular = M.d(m + "specular", 5.0, 0.0, 1e15);
+		p.specularWidth = M.d(m + "specular_width", 0.05, 1e-10, 1e15);
+		p.specularMetallic = M.d(m + "specular_metallic", 2.0, 0.0, 1e15);
+		p.specularMetallicWidth = M.d(m + "specular_metallic_width", 1.0, 1e-10, 1e15);
+		p.specularMetallicRoughness = M.d(m + "specular_metallic_roughness", 0.1, 0.0, 1e4);
+		M.rgb(m + "specular_color", p.specularColor, 65535, 65535, 65535);
 		const bool rt = M.b("raytraced_reflections", false);
 		if (rt && (M.d(m + "reflectance", 0.0) > 0.0 || M.d(m + "transparency_of_surface", 0.0) > 0.0))
 			flags |= MBX_FRACT_REFLECTIONS;
 		if (M.b(m + "iridescence_enabled", false) || M.b(m + "rough_surface", false) ||
-			M.b(m + "subsurface_scattering", false) || M.b(m + "perlin_noise_enable", false) ||
-			M.b(m + "luminosity_gradient_enable", false))
+			M.b(m + "subsurface_scattering", false) || M.b(m + "perlin_noise_enable", false))
 			flags |= MBX_FRACT_MATERIAL;
 		for (const char *t : {"use_color_texture", "use_diffusion_texture", "use_displacement_texture",
 				 "use_luminosity_texture", "use_normal_map_texture", "use_reflectance_texture",
@@ -866,8 +991,16 @@ extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, 
 		if (M.b("random_lights_group", false) || M.b("fake_lights_enabled", false)) flags |= MBX_FRACT_LIGHTS;
 	}
 
+	// glow (rendered); the volumetric shader's own step recalculation and glow on GI bounces are not
+	p.glowEnabled = M.b("glow_enabled", true);
+	p.glowIntensity = M.d("glow_intensity", 0.2, 0.0, 1e15);
+	M.rgb("glow_color_1", p.glowColor1, 40984, 44713, 49490);
+	M.rgb("glow_color_2", p.glowColor2, 57192, 60888, 62408);
+	if (p.glowEnabled && p.glowIntensity > 0.0 &&
+		(M.d("volumetric_light_DE_Factor", 1.0, 1e-15, 1e15) != 1.0 ||
+			(p.DOFMonteCarlo && p.DOFMonteCarloGlobalIllumination && M.b("MC_global_illumination_volumetric", false))))
+		flags |= MBX_FRACT_GLOW;
 	// other effects the path does not render
-	if (M.b("glow_enabled", true) && M.d("glow_intensity", 0.2) > 0.0) flags |= MBX_FRACT_GLOW;
 	if (M.b("basic_fog_enabled", false) || M.b("volumetric_fog_enabled", false) ||
 		M.b("iteration_fog_enable", false) || M.b("clouds_enable", false))
 		flags |= MBX_FRACT_FOG;
@@ -921,6 +1054,7 @@ extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, 
 		fv.v3("mandelbox_color", mb.colorFactor, 0.03, 0.05, 0.07);
 		mb.colorFactorSp1 = fv.d("mandelbox_color_Sp1", 0.2);
 		mb.colorFactorSp2 = fv.d("mandelbox_color_Sp2", 0.2);
+		mb.colorFactorR = fv.d("mandelbox_color_R", 0.0);
 		mb.rotationsEnabled = fv.b("mandelbox_rotations_enabled", false);
 		mb.mainRotationEnabled = fv.b("mandelbox_main_rotation_enabled", false);
 		mbx_fractal_ifs &ifs = fr.IFS;

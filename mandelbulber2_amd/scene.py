@@ -106,6 +106,7 @@ def default_params(width=640, height=480):
     abi.set3(mb.colorFactor, (0.03, 0.05, 0.07))
     mb.colorFactorSp1 = 0.2
     mb.colorFactorSp2 = 0.2
+    mb.colorFactorR = 0.0                      # :764
     ifs = f.IFS
     ifs.scale = 2.0
     abi.set3(ifs.offset, (1.0, 0.0, 0.0))
@@ -114,6 +115,61 @@ def default_params(width=640, height=480):
         ifs.distance[i] = 0.0
         ifs.intensity[i] = 1.0
         ifs.enabled[i] = 0
+    # palette colouring of material #1 (InitMaterialParams, initparameters.cpp:1600-1810)
+    p.useColorsFromPalette = 1
+    p.surfaceGradientEnable = 1
+    p.coloringAlgorithm = 0
+    p.coloringSpeed = 1.0
+    p.coloringPaletteOffset = 0.0
+    p.coloringSphereRadius = 1.0
+    abi.set3(p.coloringLineDirection, (1.0, 0.0, 0.0))
+    set_gradient(p, DEFAULT_GRADIENT)
+    # specular of material #1
+    p.specularPlasticEnable = 1
+    p.metallic = 1
+    p.specular = 5.0
+    p.specularWidth = 0.05
+    p.specularMetallic = 2.0
+    p.specularMetallicWidth = 1.0
+    p.specularMetallicRoughness = 0.1
+    abi.set3(p.specularColor, _u16((65535, 65535, 65535)))
+    p.luminosityEmissive = 1.0
+    # glow (initparameters.cpp:222-223, 240-241)
+    p.glowEnabled = 1
+    p.glowIntensity = 0.2
+    abi.set3(p.glowColor1, _u16((40984, 44713, 49490)))
+    abi.set3(p.glowColor2, _u16((57192, 60888, 62408)))
+    return p
+
+
+DEFAULT_GRADIENT = [(0, 0xfd6029), (1000, 0x698403), (2000, 0xfff59b), (3000, 0xf5bd22),
+                    (4000, 0x0b5e87), (5000, 0xc68876), (6000, 0xa51c64), (7000, 0x3b9fee),
+                    (8000, 0xd4ffd4), (9000, 0xaba53c)]
+
+
+def set_gradient(p, stops):
+    """Surface gradient from (position 0..10000, 0xRRGGBB) pairs as cColorGradient holds it
+    (color_gradient.cpp:305-354): positions / 10000 as float, the first colour again at 1.0,
+    sorted by position."""
+    pts = [(float(np.float32(pos) / np.float32(10000.0)), rgb) for pos, rgb in stops]
+    pts.insert(1, (1.0, pts[0][1]))
+    pts.sort(key=lambda s: s[0])
+    p.surfaceGradientCount = len(pts)
+    for i in range(abi.MBX_MAX_GRADIENT_STOPS):
+        pos, rgb = pts[i] if i < len(pts) else (0.0, 0)
+        p.surfaceGradientPosition[i] = pos
+        abi.set3(p.surfaceGradientColor[i], ((rgb >> 16) & 255, (rgb >> 8) & 255, rgb & 255))
+
+
+def plain_material(p):
+    """The benchmark scenes' material: the material colour (palette off), Lambert light without
+    specular highlights, no glow."""
+    p.useColorsFromPalette = 0
+    p.specular = 0.0
+    p.specularMetallic = 0.0
+    p.specularPlasticEnable = 0
+    p.metallic = 0
+    p.glowEnabled = 0
     return p
 
 
@@ -126,7 +182,7 @@ def mandelbulb8(width=640, height=480, seed=0):
     DE factor 1, detail 1, camera at distance 3.5 looking at the origin, FOV 53 degrees, one
     directional light without shadows, Lambert shading, no AO/GI/glow, 1 sample per pixel.
     The 1920x1080 variant is the headline throughput workload."""
-    p = default_params(width, height)
+    p = plain_material(default_params(width, height))
     abi.set3(p.camera, (0.0, -3.5, 0.0))
     abi.set3(p.target, (0.0, 0.0, 0.0))
     abi.set3(p.topVector, (0.0, 0.0, 1.0))
@@ -182,7 +238,7 @@ def mandelbox_shadow_ao(width=1920, height=1080, seed=1, scale=2.0):
     function, 50 iterations; hard shadows from the directional light and 16-ray AO with a fixed
     Fibonacci-hemisphere direction set; camera on a seeded sphere of radius 6-10 looking at the
     origin.  scale 2.0 for the first run, -1.5 for the second."""
-    p = default_params(width, height)
+    p = plain_material(default_params(width, height))
     rs = SeededStream(seed)
     d = [rs.normal() for _ in range(3)]
     n = math.sqrt(sum(c * c for c in d))
@@ -217,7 +273,7 @@ def mandelbox_shadow_ao(width=1920, height=1080, seed=1, scale=2.0):
 def menger_ifs(width=320, height=240, seed=2):
     """Kaleidoscopic IFS in Menger-sponge mode (abs folds, scale 3, offset (1,1,1)) as in
     mandelbulber2/deploy/share/mandelbulber2/examples/IFS_001.fract:38-49."""
-    p = default_params(width, height)
+    p = plain_material(default_params(width, height))
     abi.set3(p.camera, (2.2, -2.6, 1.7))
     abi.set3(p.target, (0.0, 0.0, 0.0))
     abi.set3(p.topVector, (0.0, 0.0, 1.0))
@@ -314,7 +370,7 @@ def deep_zoom_mandelbox(width=1024, height=1024, seed=5, surface_eps=1e-11, stan
     else:
         raise RuntimeError("deep-zoom surface search did not converge")
     surface = [s0 + c * t for s0, c in zip(start, ray)]
-    p = default_params(width, height)
+    p = plain_material(default_params(width, height))
     abi.set3(p.camera, [sf - c * standoff for sf, c in zip(surface, ray)])
     abi.set3(p.target, surface)
     abi.set3(p.topVector, (0.0, 0.0, 1.0))

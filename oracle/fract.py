@@ -149,6 +149,8 @@ def init_params():
     c.add("ambient_occlusion_color", RGB, (65535, 65535, 65535))    # :219
     c.add("glow_enabled", BOOL, True)                               # :222
     c.add("glow_intensity", DOUBLE, 0.2, 0.0, 1e15)                 # :223
+    c.add("glow_color_1", RGB, (40984, 44713, 49490))               # :240
+    c.add("glow_color_2", RGB, (57192, 60888, 62408))               # :241
     c.add("textured_background", BOOL, False)                       # :224
     c.add("background_brightness", DOUBLE, 1.0, 0.0, 1000.0)        # :227
     c.add("background_gamma", DOUBLE, 1.0, 0.0, 1000.0)             # :228
@@ -172,7 +174,9 @@ def init_params():
     c.add("DOF_samples", INT, 100)                                  # :330
     c.add("DOF_min_samples", INT, 10)                               # :331
     c.add("DOF_MC_global_illumination", BOOL, False)                # :333
+    c.add("MC_global_illumination_volumetric", BOOL, False)         # :334
     c.add("MC_GI_radiance_limit", DOUBLE, 10.0, 0.001, 1e10)        # :340
+    c.add("volumetric_light_DE_Factor", DOUBLE, 1.0, 1e-15, 1e15)   # :350
     c.add("random_lights_group", BOOL, False)                       # :360
     c.add("fake_lights_enabled", BOOL, False)                       # :374
     c.add("fractal_position", VEC3, (0.0, 0.0, 0.0))                # :400
@@ -229,6 +233,7 @@ def init_fractal(index):
         c.add("mandelbox_rotation_neg_%d" % i, VEC3, (0.0, 0.0, 0.0))
         c.add("mandelbox_rotation_pos_%d" % i, VEC3, (0.0, 0.0, 0.0))
     c.add("mandelbox_color", VEC3, (0.03, 0.05, 0.07))              # :761
+    c.add("mandelbox_color_R", DOUBLE, 0.0)                         # :764
     c.add("mandelbox_color_Sp1", DOUBLE, 0.2)
     c.add("mandelbox_color_Sp2", DOUBLE, 0.2)
     c.add("mandelbox_main_rotation_enabled", BOOL, False)           # :767
@@ -245,16 +250,62 @@ def init_material(c, m):
     c.add(p + "luminosity", DOUBLE, 0.0, 0.0, 1e15)
     c.add(p + "luminosity_color", RGB, (65535, 65535, 65535))
     c.add(p + "use_colors_from_palette", BOOL, True)                 # :1780-1781
+    c.add(p + "surface_gradient_enable", BOOL, True)
+    c.add(p + "surface_color_gradient", STRING, DEFAULT_GRADIENT)
+    c.add(p + "coloring_speed", DOUBLE, 1.0, 0.0, 1e15)
+    c.add(p + "coloring_palette_offset", DOUBLE, 0.0, 0.0, 256.0)
+    c.add(p + "fractal_coloring_algorithm", INT, 0, 0, 4)
+    c.add(p + "fractal_coloring_color_preV215_false", BOOL, False)
+    c.add(p + "fractal_coloring_color_4D_enabled_false", BOOL, False)
+    c.add(p + "fractal_coloring_extra_color_enabled_false", BOOL, False)
+    c.add(p + "fractal_coloring_sphere_radius", DOUBLE, 1.0, 0.0, 1e20)
+    c.add(p + "fractal_coloring_line_direction", VEC3, (1.0, 0.0, 0.0))
+    for g in ("specular", "diffuse", "luminosity", "roughness", "reflectance", "transparency"):
+        c.add(p + g + "_gradient_enable", BOOL, False)
     c.add(p + "specular", DOUBLE, 5.0, 0.0, 1e15)
     c.add(p + "specular_width", DOUBLE, 0.05, 1e-10, 1e15)
+    c.add(p + "specular_metallic", DOUBLE, 2.0, 0.0, 1e15)
+    c.add(p + "specular_metallic_width", DOUBLE, 1.0, 1e-10, 1e15)
+    c.add(p + "specular_metallic_roughness", DOUBLE, 0.1, 0.0, 1e4)
+    c.add(p + "specular_plastic_enable", BOOL, True)
+    c.add(p + "specular_color", RGB, (65535, 65535, 65535))
+    c.add(p + "luminosity_emissive", DOUBLE, 1.0, 0.0, 1e15)
     c.add(p + "metallic", BOOL, True)
     c.add(p + "reflectance", DOUBLE, 0.0, 0.0, 1e15)
     c.add(p + "transparency_of_surface", DOUBLE, 0.0, 0.0, 1.0)
     for flag in ("iridescence_enabled", "rough_surface", "subsurface_scattering",
-                 "perlin_noise_enable", "luminosity_gradient_enable", "use_color_texture",
+                 "perlin_noise_enable", "use_color_texture",
                  "use_diffusion_texture", "use_displacement_texture", "use_luminosity_texture",
                  "use_normal_map_texture", "use_reflectance_texture", "use_transparency_texture"):
         c.add(p + flag, BOOL, False)
+
+
+DEFAULT_GRADIENT = ("0 fd6029 1000 698403 2000 fff59b 3000 f5bd22 4000 0b5e87 5000 c68876 "
+                    "6000 a51c64 7000 3b9fee 8000 d4ffd4 9000 aba53c")
+
+
+def gradient_stops(text, limit=64):
+    """cColorGradient::SetColorsFromString then SortGradient (color_gradient.cpp:271-354): a list
+    of (position, (r, g, b)) with 8-bit channels, sorted by position (stable)."""
+    import numpy as np
+    parts = text.split(" ")
+    if len(parts) < 2:
+        stops = [(0.0, (255, 255, 255)), (1.0, (255, 255, 255))]
+    else:
+        stops = []
+        position = 0.0
+        for i, part in enumerate(parts):
+            if not part:
+                continue
+            if i % 2 == 0:
+                position = float(np.float32(_qtoint(part)) / np.float32(10000.0))
+            else:
+                h = _qtoint(part, 16)
+                stops.append((position, (h // 65536, (h // 256) % 256, h % 256)))
+                if i == 1:
+                    stops.append((1.0, stops[0][1]))
+    stops = sorted(stops, key=lambda s: s[0])
+    return stops[:limit]
 
 
 def init_light(c, n):
@@ -343,6 +394,12 @@ def compatibility(ver, name, value):
     if ver < 2.19:                                                           # :1375-1403
         if "surface_color_palette" in name:
             name = name.replace("surface_color_palette", "surface_color_gradient")
+            parts = value.split(" ")
+            count = len(parts)
+            if parts and len(parts[-1]) < 6:
+                count -= 1
+            step = 1.0 / count
+            value = " ".join("%d %s" % (int(i * step * 10000.0), parts[i]) for i in range(count))
         if "luminosity_color_thesame" in name:
             name = name.replace("luminosity_color_thesame", "luminosity_gradient_enable")
         if any(k in name for k in ("coloring_palette_size", "coloring_random_seed",
@@ -711,15 +768,44 @@ def load(text):
     out["surfaceColor"] = _rgbf(g(mp + "surface_color"))
     out["shading"], out["luminosity"] = g(mp + "shading"), g(mp + "luminosity")
     out["luminosityColor"] = _rgbf(g(mp + "luminosity_color"))
-    if g(mp + "use_colors_from_palette"):
+    out["luminosityEmissive"] = g(mp + "luminosity_emissive")
+    out["useColorsFromPalette"] = int(g(mp + "use_colors_from_palette"))
+    out["surfaceGradientEnable"] = int(g(mp + "surface_gradient_enable"))
+    out["coloringAlgorithm"] = g(mp + "fractal_coloring_algorithm")
+    out["coloringPreV215"] = int(g(mp + "fractal_coloring_color_preV215_false"))
+    out["coloring4D"] = int(g(mp + "fractal_coloring_color_4D_enabled_false"))
+    speed, offset = g(mp + "coloring_speed"), g(mp + "coloring_palette_offset")
+    gtext = g(mp + "surface_color_gradient")
+    if ver < 2.19:                                                   # Compatibility2 :1563-1593
+        size = len(gtext.split(" ")) // 2
+        offset = par[mp + "coloring_palette_offset"]._limit(offset / size)
+        speed = par[mp + "coloring_speed"]._limit(speed * 10.0 / size)
+    out["coloringSpeed"], out["coloringPaletteOffset"] = speed, offset
+    out["coloringSphereRadius"] = g(mp + "fractal_coloring_sphere_radius")
+    out["coloringLineDirection"] = g(mp + "fractal_coloring_line_direction")
+    stops = gradient_stops(gtext)
+    out["surfaceGradientCount"] = len(stops)
+    out["surfaceGradientPosition"] = [s[0] for s in stops] + [0.0] * (64 - len(stops))
+    out["surfaceGradientColor"] = [[float(c) for c in s[1]] for s in stops] + \
+        [[0.0, 0.0, 0.0]] * (64 - len(stops))
+    if g(mp + "use_colors_from_palette") and (
+            g(mp + "fractal_coloring_extra_color_enabled_false") or
+            any(g(mp + k + "_gradient_enable") for k in ("specular", "diffuse", "luminosity",
+                                                         "roughness", "reflectance", "transparency"))):
         flags |= FLAGS["palette"]
-    if g(mp + "specular") > 0.0:
-        flags |= FLAGS["specular"]
+    out["specularPlasticEnable"] = int(g(mp + "specular_plastic_enable"))
+    out["metallic"] = int(g(mp + "metallic"))
+    out["specular"] = g(mp + "specular")
+    out["specularWidth"] = g(mp + "specular_width")
+    out["specularMetallic"] = g(mp + "specular_metallic")
+    out["specularMetallicWidth"] = g(mp + "specular_metallic_width")
+    out["specularMetallicRoughness"] = g(mp + "specular_metallic_roughness")
+    out["specularColor"] = _rgbf(g(mp + "specular_color"))
     if g("raytraced_reflections") and (g(mp + "reflectance") > 0.0 or
                                        g(mp + "transparency_of_surface") > 0.0):
         flags |= FLAGS["reflections"]
     if any(g(mp + k) for k in ("iridescence_enabled", "rough_surface", "subsurface_scattering",
-                                "perlin_noise_enable", "luminosity_gradient_enable")):
+                                "perlin_noise_enable")):
         flags |= FLAGS["material"]
     if any(g(mp + k) for k in ("use_color_texture", "use_diffusion_texture",
                                 "use_displacement_texture", "use_luminosity_texture",
@@ -747,7 +833,14 @@ def load(text):
     for k, v in L.items():
         out["light." + k] = v
 
-    if g("glow_enabled") and g("glow_intensity") > 0.0:
+    out["glowEnabled"] = int(g("glow_enabled"))
+    out["glowIntensity"] = g("glow_intensity")
+    out["glowColor1"] = _rgbf(g("glow_color_1"))
+    out["glowColor2"] = _rgbf(g("glow_color_2"))
+    if g("glow_enabled") and g("glow_intensity") > 0.0 and (
+            g("volumetric_light_DE_Factor") != 1.0 or
+            (g("DOF_monte_carlo") and g("DOF_MC_global_illumination") and
+             g("MC_global_illumination_volumetric"))):
         flags |= FLAGS["glow"]
     if g("basic_fog_enabled") or g("volumetric_fog_enabled") or g("iteration_fog_enable") or \
             g("clouds_enable"):
@@ -790,6 +883,7 @@ def load(text):
         out["fractal.mandelbox." + k] = f.get(v)
     out["fractal.mandelbox.rotationsEnabled"] = int(f.get("mandelbox_rotations_enabled"))
     out["fractal.mandelbox.mainRotationEnabled"] = int(f.get("mandelbox_main_rotation_enabled"))
+    out["fractal.mandelbox.colorFactorR"] = f.get("mandelbox_color_R")
     out["fractal.mandelbox.rotation"] = [[f.get("mandelbox_rotation_%s_%d" % (s, i))
                                           for i in (1, 2, 3)] for s in ("neg", "pos")]
     out["fractal.IFS.scale"] = f.get("IFS_scale")

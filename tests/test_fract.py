@@ -126,7 +126,9 @@ def test_empty_scene_is_the_defaults(load):
         assert r[k] == d[k], k
     # a 2.33 file that names no light has none (the light list starts empty, settings.cpp:689-690)
     assert r["light.enabled"] == 0
-    assert r.flags() == {"palette", "specular", "glow"}
+    # palette colouring, specular highlights and glow are on by default and rendered
+    assert r["useColorsFromPalette"] == 1 and r["specular"] == 5.0 and r["glowEnabled"] == 1
+    assert r.flags() == set()
 
 
 # ------------------------------------------------------------------------------------------
@@ -147,7 +149,14 @@ def test_mandelbulb001(load):
     assert r["light.enabled"] == 1 and r["light.rotation"] == [-45.0, 45.0, 0.0]
     assert r["light.intensity"] == 1.0
     assert r["ambientOcclusionEnabled"] == 1 and r["ambientOcclusionMode"] == abi.MBX_AO_SCREEN_SPACE
-    assert r.flags() == {"palette", "specular", "glow", "fog", "ao_screen_space", "postprocess"}
+    assert r.flags() == {"fog", "ao_screen_space", "postprocess"}
+    # mat1_surface_color_gradient "0 fd6029 999 698403 1999 fff59b ... 9000 aba53c": 10 stops and
+    # the first colour again at 1.0, positions as float(n) / 10000
+    assert r["surfaceGradientCount"] == 11
+    pos = r["surfaceGradientPosition"]
+    assert pos[0] == 0.0 and pos[1] == float(np.float32(999) / np.float32(10000)) and pos[10] == 1.0
+    assert r["surfaceGradientColor"][0] == [0xfd, 0x60, 0x29] == r["surfaceGradientColor"][10]
+    assert r["surfaceGradientColor"][9] == [0xab, 0xa5, 0x3c]
 
 
 def test_mandelbox001(load):
@@ -159,7 +168,8 @@ def test_mandelbox001(load):
     assert r["light.enabled"] == 1 and r["light.intensity"] == 4.0
     assert r["light.rotation"] == [-45.0, 45.0, 0.0]
     assert r["reflectionsMax"] == 3
-    assert r.flags() == {"specular", "glow", "fog", "reflections", "postprocess"}
+    assert r.flags() == {"fog", "reflections", "postprocess"}
+    assert r["useColorsFromPalette"] == 0 and r["specular"] == 1.0 and r["metallic"] == 0
 
 
 def test_ifs001_pre_2_25_lights(load):
@@ -357,12 +367,44 @@ def test_light_versions(load):
 
 
 def test_pre_2_15_material_defaults(load):
-    # a pre-2.15 material gets specular 1 (settings.cpp:2107-2119); without material lines the
-    # default specular 5 stays: both report specular, so check the value through 0
-    r = load(_hdr("2.14", "mat1_specular 0;\n"))
-    assert "specular" not in r.flags()
+    # a material first named by a pre-2.15 file gets specular 1, width 1 and metallic off
+    # (settings.cpp:2107-2119); the file's own lines still win
     r = load(_hdr("2.14", "mat1_shading 0,5;\n"))
-    assert "specular" in r.flags() and r["shading"] == 0.5
+    assert r["shading"] == 0.5
+    assert (r["specular"], r["specularWidth"], r["metallic"]) == (1.0, 1.0, 0)
+    r = load(_hdr("2.14", "mat1_specular 0;\n"))
+    assert r["specular"] == 0.0
+    r = load(_hdr("2.15", "mat1_shading 0,5;\n"))
+    assert (r["specular"], r["specularWidth"], r["metallic"]) == (5.0, 0.05, 1)
+    r = load(_hdr("2.14", "fov 30;\n"))          # no material line: InitMaterialParams defaults
+    assert r["specular"] == 5.0
+
+
+def _stops(r):
+    n = r["surfaceGradientCount"]
+    return [(r["surfaceGradientPosition"][i], tuple(r["surfaceGradientColor"][i])) for i in range(n)]
+
+
+def test_gradient_strings(load):
+    # unsorted stops are sorted; the first colour is repeated at 1.0
+    r = load(_hdr("2.33", "mat1_surface_color_gradient 5000 00ff00 0 ff0000;\n"))
+    assert _stops(r) == [(0.0, (255, 0, 0)), (0.5, (0, 255, 0)), (1.0, (0, 255, 0))]
+    r = load(_hdr("2.33", "mat1_surface_color_gradient 0 ff0000 5000 00ff00;\n"))
+    assert _stops(r) == [(0.0, (255, 0, 0)), (0.5, (0, 255, 0)), (1.0, (255, 0, 0))]
+    # a single token: white at both ends (color_gradient.cpp:311-319)
+    r = load(_hdr("2.33", "mat1_surface_color_gradient 0;\n"))
+    assert _stops(r) == [(0.0, (255, 255, 255)), (1.0, (255, 255, 255))]
+
+
+def test_pre_2_19_palette_migration(load):
+    # surface_color_palette -> gradient with evenly spread positions (settings.cpp:1375-1396),
+    # then offset / size and speed * 10 / size with size = tokens / 2 (settings.cpp:1563-1593)
+    r = load(_hdr("2.18", "mat1_surface_color_palette ff0000 00ff00 0000ff;\n"
+                          "mat1_coloring_speed 3;\nmat1_coloring_palette_offset 6;\n"))
+    assert _stops(r) == [(0.0, (255, 0, 0)), (float(np.float32(3333) / np.float32(10000)), (0, 255, 0)),
+                         (float(np.float32(6666) / np.float32(10000)), (0, 0, 255)), (1.0, (255, 0, 0))]
+    assert r["coloringSpeed"] == pytest.approx(3.0 * 10.0 / 3.0, rel=1e-15)
+    assert r["coloringPaletteOffset"] == pytest.approx(2.0, rel=1e-15)
 
 
 # ------------------------------------------------------------------------------------------
