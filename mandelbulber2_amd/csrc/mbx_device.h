@@ -192,8 +192,11 @@ __device__ __forceinline__ double ddot(DV3 a, DV3 b)
 }
 __device__ __forceinline__ double dsgn(double v) { return (double)((0.0 < v) - (v < 0.0)); }
 
-template <bool TANGENT>
-__device__ __forceinline__ void MandelboxIterationD(DV3 &z, double &DE, DV3 *J, const FrameConst &f)
+// With COLOR, aux.color gains the fold colour factors in the reference's order: x, y, z box
+// folds, then the inner (Sp1) or outer (Sp2) sphere fold.
+template <bool TANGENT, bool COLOR = false>
+__device__ __forceinline__ void MandelboxIterationD(DV3 &z, double &DE, DV3 *J, const FrameConst &f,
+	double *color = nullptr)
 {
 	// no FMA contraction: the orbit expands rounding differences by up to (4 |scale|)^n, so the
 	// escape counts follow the reference only with the reference's separately rounded products
@@ -203,6 +206,12 @@ __device__ __forceinline__ void MandelboxIterationD(DV3 &z, double &DE, DV3 *J, 
 	z.x = fx ? dsgn(z.x) * V - z.x : z.x;
 	z.y = fy ? dsgn(z.y) * V - z.y : z.y;
 	z.z = fz ? dsgn(z.z) * V - z.z : z.z;
+	if (COLOR)
+	{
+		if (fx) *color += f.mbColorD[0];
+		if (fy) *color += f.mbColorD[1];
+		if (fz) *color += f.mbColorD[2];
+	}
 	if (TANGENT)
 	{
 #pragma unroll
@@ -220,6 +229,13 @@ __device__ __forceinline__ void MandelboxIterationD(DV3 &z, double &DE, DV3 *J, 
 	const bool outer = !inner && r2 < f.mbFR2D;
 	const double t2 = f.mbFR2D / r2;
 	const double k = inner ? f.mbFactor1D : (outer ? t2 : 1.0);
+	if (COLOR)
+	{
+		if (inner)
+			*color += f.mbSp1D;
+		else if (outer)
+			*color += f.mbSp2D;
+	}
 	if (TANGENT)
 	{
 		const double g = outer ? -2.0 * t2 / r2 : 0.0;
@@ -401,6 +417,21 @@ __device__ __forceinline__ double compute_box_fd(DV3 p, const FrameConst &f, dou
 // cFractalMandelbulb::FormulaCode (fractal_mandelbulb.cpp:29-43) with libm-class asin, atan2,
 // pow, sin and cos, then z += c, aux.r = |z|, the NaN revert and the r > bailout test of
 // Compute (compute_fractal.cpp:246-321) and the DE finish (compute_fractal.cpp:561-617).
+__device__ __forceinline__ void MandelbulbIterationD(DV3 &z, double r, double &DE, const FrameConst &f)
+{
+#pragma clang fp contract(off)
+	const double power = f.powerD;
+	const double th0 = asin(z.z / r) + f.betaD;
+	const double ph0 = atan2(z.y, z.x) + f.alphaD;
+	double rp = pow(r, power - 1.0);
+	const double th = th0 * power;
+	const double ph = ph0 * power;
+	const double cth = cos(th);
+	DE = (rp * DE) * power + 1.0;
+	rp *= r;
+	z = dv3(cth * cos(ph) * rp, cth * sin(ph) * rp, sin(th) * rp);
+}
+
 __device__ __forceinline__ double compute_bulb_d(DV3 p, const FrameConst &f, int &iters)
 {
 #pragma clang fp contract(off)
@@ -408,20 +439,11 @@ __device__ __forceinline__ double compute_bulb_d(DV3 p, const FrameConst &f, int
 	double r = sqrt(ddot(z, z));
 	double DE = 1.0;
 	const int maxN = f.formulaMaxiter;
-	const double power = f.powerD;
 	int i;
 	for (i = 0; i < maxN; i++)
 	{
 		const DV3 lastZ = z;
-		const double th0 = asin(z.z / r) + f.betaD;
-		const double ph0 = atan2(z.y, z.x) + f.alphaD;
-		double rp = pow(r, power - 1.0);
-		const double th = th0 * power;
-		const double ph = ph0 * power;
-		const double cth = cos(th);
-		DE = (rp * DE) * power + 1.0;
-		rp *= r;
-		z = dv3(cth * cos(ph) * rp, cth * sin(ph) * rp, sin(th) * rp);
+		MandelbulbIterationD(z, r, DE, f);
 		z = add_c_d(z, p, f);
 		r = sqrt(ddot(z, z));
 		if (!(isfinite(z.x) && isfinite(z.y) && isfinite(z.z)))
@@ -731,6 +753,179 @@ __device__ __forceinline__ typename Prec<PosPrecision<KIND>::kFP64>::R calculate
 	else
 		d = (d != d) ? 1e20f : fminf(fmaxf(d, 0.0f), 10.0f);
 	return d;
+}
+
+// ------------------------------------------------------------------------------------------
+// Compute<calcModeColouring> (compute_fractal.cpp:67-654, colouring branch :343-436) and
+// CalculateColorIndex for a single formula (fractal_coloring.cpp:318-340).  SurfaceColour runs it
+// with maxiterMultiplier 4 (shader_surface_color.cpp:51-55).  The orbit's w stays 0 on the device,
+// so the 4D measures reduce to the 3D ones and the 4D cross measure to 0.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ float colour_measure(const FrameConst &f, V3 z, V3 p, float r)
+{
+	switch (f.colorAlgorithm)
+	{
+		case 1: return fabsf(dot(z, p));
+		case 2: return fabsf(length(z - p) - f.colorSphereRadius);
+		case 3:
+		{
+			const float ax = fabsf(z.x), ay = fabsf(z.y), az = fabsf(z.z);
+			const float len = (ax < ay) ? (ax < az ? ax : az) : (ay < az ? ay : az); // dMin
+			return (f.color4D && 0.0f < len) ? 0.0f : len;                          // min(len, |w|)
+		}
+		case 4: return fabsf(dot(z, v3(f.colorLineDir[0], f.colorLineDir[1], f.colorLineDir[2])));
+		default: return r;
+	}
+}
+
+__device__ __forceinline__ double colour_measure_d(const FrameConst &f, DV3 z, DV3 p, double r)
+{
+#pragma clang fp contract(off)
+	switch (f.colorAlgorithm)
+	{
+		case 1: return fabs(ddot(z, p));
+		case 2: return fabs(sqrt(ddot(z - p, z - p)) - f.colorSphereRadiusD);
+		case 3:
+		{
+			const double ax = fabs(z.x), ay = fabs(z.y), az = fabs(z.z);
+			const double len = (ax < ay) ? (ax < az ? ax : az) : (ay < az ? ay : az);
+			return (f.color4D && 0.0 < len) ? 0.0 : len;
+		}
+		case 4: return fabs(ddot(z, dv3(f.colorLineDirD[0], f.colorLineDirD[1], f.colorLineDirD[2])));
+		default: return r;
+	}
+}
+
+// FP32 kinds (Mandelbulb, IFS): colorMin of the orbit, bailout at the formula's radius (or
+// 1e15 and the stall test before v2.15); colour index colorMin * 5000 (Mandelbulb) or * 1000 (IFS)
+template <int KIND>
+__device__ double colour_index_f(V3 p, const FrameConst &f, const Work &w)
+{
+	V3 z = p;
+	float r2 = dot(z, z);
+	float DE = 1.0f;
+	float colorMin = 1000.0f;
+	const int maxN = 4 * w.maxN;
+	const bool pre = f.colorPreV215 != 0;
+	for (int i = 0; i < maxN; i++)
+	{
+		const V3 lastZ = z;
+		if (KIND == kBulb8)
+			MandelbulbPow8Iteration(z, r2, DE);
+		else if (KIND == kBulbTrig)
+			MandelbulbIteration(z, fsqrt(r2), DE, f);
+		else
+			KaleidoscopicIfsIteration(z, DE, f);
+		z = add_c(z, p, w);
+		r2 = dot(z, z);
+		if (!finite3(z))
+		{
+			z = lastZ;
+			break;
+		}
+		const float r = fsqrt(r2);
+		const float len = colour_measure(f, z, p, r);
+		if (len < colorMin) colorMin = len;
+		if (!pre)
+		{
+			if (r > f.bailout) break;
+		}
+		else if (r > 1e15f || length(z - lastZ) / r < 1e-15f)
+			break;
+	}
+	return (KIND == kIfsAnalytic) ? (double)colorMin * 1000.0 : (double)colorMin * 5000.0;
+}
+
+// FP64 kinds (Mandelbox, FP64 Mandelbulb) in the reference's operation order.  The Mandelbox
+// index is aux.color * 100 + r * factorR / 1e13 (+ colorMin * 1000 for a non-standard measure);
+// with the standard measure its orbit runs to |z| > 1e15 or until it stalls.
+template <int KIND>
+__device__ double colour_index_d(DV3 p, const FrameConst &f)
+{
+#pragma clang fp contract(off)
+	constexpr bool box = KIND == kBoxAnalytic || KIND == kBoxDelta;
+	DV3 z = p;
+	double r = sqrt(ddot(z, z));
+	double DE = 1.0, color = 1.0, colorMin = 1000.0;
+	const int maxN = 4 * f.formulaMaxiter;
+	const int alg = f.colorAlgorithm;
+	const bool pre = f.colorPreV215 != 0;
+	for (int i = 0; i < maxN; i++)
+	{
+		const DV3 lastZ = z;
+		if constexpr (box)
+			MandelboxIterationD<false, true>(z, DE, nullptr, f, &color);
+		else
+			MandelbulbIterationD(z, r, DE, f);
+		z = add_c_d(z, p, f);
+		r = sqrt(ddot(z, z));
+		if (!(isfinite(z.x) && isfinite(z.y) && isfinite(z.z)))
+		{
+			z = lastZ;
+			r = sqrt(ddot(z, z));
+			break;
+		}
+		const double len = colour_measure_d(f, z, p, r);
+		const DV3 dz = z - lastZ;
+		const double change = sqrt(ddot(dz, dz)) / r;
+		if (!pre)
+		{
+			if (!box)
+			{
+				if (len < colorMin) colorMin = len;
+				if (r > f.bailoutD) break;
+			}
+			else if (alg == 0)
+			{
+				if (r > 1e15 || change < 1e-15) break;
+			}
+			else
+			{
+				if (len < colorMin) colorMin = len;
+				if (r > f.bailoutD || change < 1e-15) break;
+			}
+		}
+		else if (!box || alg != 0)
+		{
+			if (len < colorMin) colorMin = len;
+			if (r > 1e15 || change < 1e-15) break;
+		}
+		else if (r > 1e15 || change < 1e-15)
+			break;
+	}
+	if (box) return color * 100.0 + r * f.mbColorRD / 1e13 + ((alg != 0) ? colorMin * 1000.0 : 0.0);
+	return colorMin * 5000.0;
+}
+
+// Colour index of a fractal-local point in the kind's position precision
+template <int KIND>
+__device__ __forceinline__ double colour_index(typename Prec<PosPrecision<KIND>::kFP64>::P p,
+	const FrameConst &f, const Work &w)
+{
+	if constexpr (PosPrecision<KIND>::kFP64)
+		return colour_index_d<KIND>(p, f);
+	else
+		return colour_index_f<KIND>(p, f, w);
+}
+
+// cColorGradient::GetColorFloat without smoothing (color_gradient.cpp:128-136, 180-229): the
+// stop below the position, then linear interpolation of the 8-bit channels / 256; the last
+// stop and a zero-width interval give the 8-bit values unscaled, as the reference does
+__device__ __forceinline__ V3 gradient_colour(const FrameConst &f, float pos)
+{
+#pragma clang fp contract(off)
+	const int n = f.gradientCount;
+	int idx = 0;
+	while (idx < n - 1 && pos > f.gradientPos[idx + 1]) idx++;
+	if (idx == n - 1)
+		return v3(f.gradientColor[idx - 1][0], f.gradientColor[idx - 1][1], f.gradientColor[idx - 1][2]);
+	const float p1 = f.gradientPos[idx], p2 = f.gradientPos[idx + 1];
+	const V3 c1 = v3(f.gradientColor[idx][0], f.gradientColor[idx][1], f.gradientColor[idx][2]);
+	const V3 c2 = v3(f.gradientColor[idx + 1][0], f.gradientColor[idx + 1][1], f.gradientColor[idx + 1][2]);
+	if (!(p2 - p1 > 0.0f)) return c1;
+	const float d = (pos - p1) / (p2 - p1);
+	const float nd = 1.0f - d;
+	return v3((c1.x * nd + c2.x * d) / 256.0f, (c1.y * nd + c2.y * d) / 256.0f, (c1.z * nd + c2.z * d) / 256.0f);
 }
 
 } // namespace mbx

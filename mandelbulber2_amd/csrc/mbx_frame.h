@@ -106,6 +106,35 @@ struct FrameConst
 	float shading;
 	float luminosity[3];       // luminosity * luminosityColor
 	float fillLight[3];
+	float lumEmissive[3];      // luminosity * luminosityColor * luminosityEmissive (GI bounces)
+
+	// palette colouring (shader_surface_color.cpp:39-108, compute_fractal.cpp:343-436,
+	// fractal_coloring.cpp:318-340, color_gradient.cpp:128-229)
+	int32_t palette;           // use_colors_from_palette
+	int32_t paletteGradient;   // surface_gradient_enable
+	int32_t colorAlgorithm;    // 0 standard, 1 z.p, 2 sphere, 3 cross, 4 line
+	int32_t colorPreV215, color4D;
+	int32_t gradientCount;     // 2 .. MBX_MAX_GRADIENT_STOPS, positions non-decreasing
+	double coloringSpeedD, paletteOffsetD; // the material's float values, widened
+	double colorSphereRadiusD;
+	double colorLineDirD[3];
+	float colorSphereRadius;
+	float colorLineDir[3];
+	double mbColorD[3], mbSp1D, mbSp2D, mbColorRD; // Mandelbox fold colour factors
+	float gradientPos[MBX_MAX_GRADIENT_STOPS];
+	float gradientColor[MBX_MAX_GRADIENT_STOPS][3]; // 8-bit channel values
+
+	// specular highlights (shader_specular_highlight_combined.cpp:39-68,
+	// shader_specular_highlight.cpp:38-73)
+	int32_t specPlastic, specMetal;
+	float specPlasticExp, specMetalExp; // 30 / width
+	float specPlasticK, specMetalK, specRoughness;
+	float specColor[3];
+
+	// glow (shader_volumetric.cpp:86-92, 196-204)
+	int32_t glow;
+	float glowIntensity;
+	float glowColor1[3], glowColor2[3];
 
 	// ambient occlusion
 	int32_t aoEnabled, aoMode, aoQuality, aoCount;

@@ -684,7 +684,67 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 	{
 		fc.luminosity[i] = (float)(p->luminosity * p->luminosityColor[i]);
 		fc.fillLight[i] = (float)p->fillLightColor[i];
+		// ObjectShader's float products (shader_object.cpp:171-201)
+		fc.lumEmissive[i] = (float)p->luminosity * (float)p->luminosityColor[i] * (float)p->luminosityEmissive;
 	}
+
+	// palette colouring: the material's colouring speed and offset are floats (material.h:88-89)
+	fc.palette = p->useColorsFromPalette ? 1 : 0;
+	fc.paletteGradient = p->surfaceGradientEnable ? 1 : 0;
+	fc.colorAlgorithm = p->coloringAlgorithm;
+	fc.colorPreV215 = p->coloringPreV215 ? 1 : 0;
+	fc.color4D = p->coloring4D ? 1 : 0;
+	fc.coloringSpeedD = (double)(float)p->coloringSpeed;
+	fc.paletteOffsetD = (double)(float)p->coloringPaletteOffset;
+	fc.colorSphereRadiusD = p->coloringSphereRadius;
+	fc.colorSphereRadius = (float)p->coloringSphereRadius;
+	for (int i = 0; i < 3; i++)
+	{
+		fc.colorLineDirD[i] = p->coloringLineDirection[i];
+		fc.colorLineDir[i] = (float)p->coloringLineDirection[i];
+		fc.mbColorD[i] = mb.colorFactor[i];
+	}
+	fc.mbSp1D = mb.colorFactorSp1;
+	fc.mbSp2D = mb.colorFactorSp2;
+	fc.mbColorRD = mb.colorFactorR;
+	if (fc.palette && fc.paletteGradient)
+	{
+		if (p->coloringAlgorithm < 0 || p->coloringAlgorithm > 4)
+			return fail(ctx, MBX_ERR_INVALID, "coloringAlgorithm must be 0 .. 4");
+		const int n = p->surfaceGradientCount;
+		if (n < 2 || n > MBX_MAX_GRADIENT_STOPS)
+			return fail(ctx, MBX_ERR_INVALID, "surfaceGradientCount must be 2 .. MBX_MAX_GRADIENT_STOPS");
+		for (int i = 1; i < n; i++)
+			if (!((float)p->surfaceGradientPosition[i] >= (float)p->surfaceGradientPosition[i - 1]))
+				return fail(ctx, MBX_ERR_INVALID, "surfaceGradientPosition must be sorted (non-decreasing)");
+	}
+	fc.gradientCount = p->surfaceGradientCount;
+	for (int i = 0; i < MBX_MAX_GRADIENT_STOPS; i++)
+	{
+		const bool used = i < p->surfaceGradientCount;
+		fc.gradientPos[i] = used ? (float)p->surfaceGradientPosition[i] : 0.0f;
+		for (int c = 0; c < 3; c++) fc.gradientColor[i][c] = used ? (float)p->surfaceGradientColor[i][c] : 0.0f;
+	}
+
+	// specular highlights: widths, intensities and roughness are the material's floats
+	fc.specPlastic = p->specularPlasticEnable ? 1 : 0;
+	fc.specMetal = p->metallic ? 1 : 0;
+	if ((fc.specPlastic && !(p->specularWidth > 0.0)) || (fc.specMetal && !(p->specularMetallicWidth > 0.0)))
+		return fail(ctx, MBX_ERR_INVALID, "specular widths must be > 0");
+	fc.specPlasticExp = 30.0f / (float)p->specularWidth;
+	fc.specMetalExp = 30.0f / (float)p->specularMetallicWidth;
+	fc.specPlasticK = (float)p->specular;
+	fc.specMetalK = (float)p->specularMetallic;
+	fc.specRoughness = (float)p->specularMetallicRoughness;
+	set3f(fc.specColor, p->specularColor);
+
+	// glow
+	fc.glow = p->glowEnabled ? 1 : 0;
+	if (fc.glow && !(p->DEFactor >= 1e-9))
+		return fail(ctx, MBX_ERR_UNSUPPORTED, "glow needs DEFactor >= 1e-9 (the step-merging case is not covered)");
+	fc.glowIntensity = (float)p->glowIntensity;
+	set3f(fc.glowColor1, p->glowColor1);
+	set3f(fc.glowColor2, p->glowColor2);
 
 	// ambient occlusion
 	fc.aoEnabled = p->ambientOcclusionEnabled;
@@ -789,10 +849,12 @@ bool same_frames(const std::vector<FrameConst> &a, const std::vector<FrameConst>
 		(a.empty() || std::memcmp(a.data(), b.data(), a.size() * sizeof(FrameConst)) == 0);
 }
 
-// Whether a frame needs the kernels' Monte Carlo / GI / shadow / AO code.
+// Whether a frame needs the kernels' Monte Carlo / GI / shadow / AO / palette / specular / glow
+// code.
 bool frame_needs_full(const FrameConst &f)
 {
-	return f.monteCarlo || f.aoEnabled || (f.lightEnabled && f.castShadows);
+	return f.monteCarlo || f.aoEnabled || (f.lightEnabled && (f.castShadows || f.specPlastic || f.specMetal)) ||
+		(f.palette && f.paletteGradient) || f.glow;
 }
 
 // Upload frames and jobs when they differ from what the device holds.
@@ -966,6 +1028,9 @@ std::string derive_diff(const mbx_params *p, const FrameConst &fc, const double 
 	if (!fc.addC || fc.julia) return "the differentiable render needs c = p (no Julia mode)";
 	if (p->light.castShadows || p->ambientOcclusionEnabled || p->DOFMonteCarlo)
 		return "the differentiable render has no shadows, AO or Monte Carlo sampling";
+	if ((p->useColorsFromPalette && p->surfaceGradientEnable) || p->glowEnabled ||
+		(p->light.enabled && (p->specularPlasticEnable || p->metallic)))
+		return "the differentiable render has no palette colouring, specular highlights or glow";
 	if (p->backgroundGamma != 1.0) return "the differentiable render needs background gamma 1";
 	if (p->light.enabled && p->light.relativePosition)
 		return "the differentiable render needs the light at an absolute angle (relativePosition 0)";

@@ -35,6 +35,7 @@ struct MarchOut
 	int found;
 	int iters;
 	int steps;
+	int stepCount; // buffCount: march iterations that ended without a hit (render_worker.cpp:838)
 };
 
 // CalcDistThresh (render_worker.cpp:653-679)
@@ -69,7 +70,7 @@ __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start
 	P point = Q::from(v3(1e30f, 1e30f, 1e30f));
 	bool found = false, dead = false;
 	R scan = 0, dist = 0, step = 0, thr = 0;
-	int counter = 0, iters = 0;
+	int counter = 0, iters = 0, stepCount = 0;
 	const R dirLength = Q::len(dir);
 	const R invDirLength = Q::rcp(dirLength);
 	const R searchLimit = kExact ? (R)(1.0 - 0.001 * f.detailLevelD) : (R)f.searchLimit;
@@ -106,6 +107,7 @@ __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start
 			step = (R)3;
 			iters = 1;
 			w.far_steps++;
+			stepCount = i + 1;
 			scan += step * invDirLength;
 			if (scan > maxScan) break;
 			continue;
@@ -133,6 +135,7 @@ __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start
 			step = step > (R)3 ? (R)3 : step;
 			scan += step * invDirLength;
 		}
+		stepCount = i + 1;
 		if (scan > maxScan) break;
 	}
 	point = Q::along(start, dir, scan);
@@ -163,6 +166,7 @@ __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start
 	out.found = found;
 	out.iters = iters;
 	out.steps = counter;
+	out.stepCount = stepCount;
 }
 
 // CalculateNormals, fast mode: central differences of the DE (shader_calculate_normals.cpp:44-78)
@@ -356,15 +360,47 @@ __device__ V3 background_shader(const FrameConst &f, V3 viewVector)
 	return out;
 }
 
-// ObjectShader with one directional light, Lambert, optional shadow and AO
-// (shader_object.cpp:40-229, shader_light_shading.cpp:40-122).  FULL = false compiles the
-// shadow and AO paths out (frames that use neither), which lowers the kernel's registers.
+// SurfaceColour of the fractal (shader_surface_color.cpp:39-108): the gradient colour at the
+// palette position of the fractal-local point's colour index, or the material colour
+template <int KIND>
+__device__ V3 surface_colour(const FrameConst &f, typename PrecOf<KIND>::P p, const Work &w)
+{
+#pragma clang fp contract(off)
+	using Q = PrecOf<KIND>;
+	if (!f.palette || !f.paletteGradient) return v3(f.surfaceColor[0], f.surfaceColor[1], f.surfaceColor[2]);
+	const typename Q::P local = w.nodeRotated ? Q::node_local(f, p) : p;
+	const double colorIndex = colour_index<KIND>(local, f, w);
+	const double nrCol = fmod(fabs(colorIndex), 248.0 * 256.0);
+	const double position = fmod(nrCol / 256.0 / 10.0 * f.coloringSpeedD + f.paletteOffsetD, 1.0);
+	return gradient_colour(f, (float)position);
+}
+
+// SpecularHighlight (shader_specular_highlight.cpp:38-73) with diffuse 1 and no texture: the
+// Blinn term pow(n.h, 30 / width), the metallic roughness factor (one draw of the pixel's
+// random stream), clamped at 15
+__device__ __forceinline__ float specular_shade(V3 n, V3 view, V3 lv, float expo, float roughness, uint32_t &seed)
+{
+#pragma clang fp contract(off)
+	const V3 half = normalize(lv - view);
+	float s = dot(n, half);
+	if (s < 0.0f) s = 0.0f;
+	s = powf(s, expo);
+	if (roughness > 0.0f)
+		s = (float)((double)s * (1.0 + (double)((float)random_mod(seed, 1000) / 1000.0f * roughness)));
+	return s > 15.0f ? 15.0f : s;
+}
+
+// ObjectShader with one directional light, Lambert, optional shadow and AO, and the combined
+// plastic + metallic specular highlight (shader_object.cpp:40-229, shader_light_shading.cpp:40-122,
+// shader_specular_highlight_combined.cpp:39-68).  colour is the surface colour, view the ray
+// direction; the highlight (already shadowed) goes to spec.  FULL = false compiles the shadow,
+// AO and specular paths out (frames that use none of them), which lowers the kernel's registers.
 template <int KIND, bool FULL = true>
 __device__ V3 object_shader(const FrameConst &f, typename PrecOf<KIND>::P p, V3 n, typename PrecOf<KIND>::R thr,
-	typename PrecOf<KIND>::R delta, uint32_t &seed, Work &w)
+	typename PrecOf<KIND>::R delta, V3 view, V3 colour, V3 &spec, uint32_t &seed, Work &w)
 {
 	using R = typename PrecOf<KIND>::R;
-	const V3 colour = v3(f.surfaceColor[0], f.surfaceColor[1], f.surfaceColor[2]);
+	spec = v3(0.0f, 0.0f, 0.0f);
 	V3 ambient = v3(0.0f, 0.0f, 0.0f);
 	if (FULL && f.aoEnabled)
 	{
@@ -383,18 +419,42 @@ __device__ V3 object_shader(const FrameConst &f, typename PrecOf<KIND>::P p, V3 
 		float shade = fmaxf(dot(n, lv), 0.0f);
 		shade = 1.0f - f.shading + shade * f.shading;
 		shade = fminf(shade * f.lightIntensity, 500.0f);
+		V3 specular = v3(0.0f, 0.0f, 0.0f);
+		if (FULL && (f.specPlastic || f.specMetal))
+		{
+#pragma clang fp contract(off)
+			const V3 sc = v3(f.specColor[0], f.specColor[1], f.specColor[2]);
+			if (f.specPlastic)
+			{
+				const float s = specular_shade(n, view, lv, f.specPlasticExp, 0.0f, seed);
+				specular = v3(s * sc.x * f.specPlasticK, s * sc.y * f.specPlasticK, s * sc.z * f.specPlasticK);
+			}
+			if (f.specMetal)
+			{
+				const float s = specular_shade(n, view, lv, f.specMetalExp, f.specRoughness, seed);
+				specular = specular + v3(s * sc.x * (f.specMetalK * colour.x), s * sc.y * (f.specMetalK * colour.y),
+										  s * sc.z * (f.specMetalK * colour.z));
+			}
+			specular = specular * f.lightIntensity;
+		}
 		float shadow = 1.0f;
 		if (FULL && f.castShadows)
 		{
-			if (shade > 0.001f)
+			const float specMax = fmaxf(specular.x, fmaxf(specular.y, specular.z));
+			if (shade > 0.001f || specMax > 0.001f)
 			{
 				const R distance = f.penetrating ? delta * (R)f.height : (R)f.viewDistanceMaxLight;
 				shadow = aux_shadow<KIND>(f, p, thr, distance, lv, w);
+				specular = specular * shadow;
 			}
 			else
+			{
 				shadow = 0.0f;
+				specular = v3(0.0f, 0.0f, 0.0f);
+			}
 		}
 		light = v3(f.lightColor[0], f.lightColor[1], f.lightColor[2]) * (shade * shadow);
+		if (FULL) spec = v3(specular.x * f.lightColor[0], specular.y * f.lightColor[1], specular.z * f.lightColor[2]);
 	}
 	V3 out;
 	out.x = (f.fillLight[0] + ambient.x * f.aoColor[0]) * colour.x + light.x * colour.x + f.luminosity[0];
@@ -403,10 +463,11 @@ __device__ V3 object_shader(const FrameConst &f, typename PrecOf<KIND>::P p, V3 
 	return out;
 }
 
-// GlobalIlumination (shader_global_illumination.cpp:41-209)
+// GlobalIlumination (shader_global_illumination.cpp:41-209).  colour0 is the primary hit's
+// surface colour; each bounce is tinted by the colour of the surface it left.
 template <int KIND>
 __device__ V3 global_illumination(const FrameConst &f, typename PrecOf<KIND>::P p0, V3 n0,
-	typename PrecOf<KIND>::R thr0, uint32_t &seed, Work &w)
+	typename PrecOf<KIND>::R thr0, V3 colour0, uint32_t &seed, Work &w)
 {
 	using Q = PrecOf<KIND>;
 	using R = typename Q::R;
@@ -415,8 +476,8 @@ __device__ V3 global_illumination(const FrameConst &f, typename PrecOf<KIND>::P 
 	P p = p0;
 	V3 n = n0;
 	R thrCur = thr0;
-	const V3 colour = v3(f.surfaceColor[0], f.surfaceColor[1], f.surfaceColor[2]);
-	V3 colourTemp = colour;
+	V3 colourTemp = colour0;
+	V3 newColour = colour0;
 	for (int depth = 0; depth < f.reflectionsMax; depth++)
 	{
 		const float rx = (float)(random_mod(seed, 20000) - 10000) * 1e-4f;
@@ -451,8 +512,14 @@ __device__ V3 global_illumination(const FrameConst &f, typename PrecOf<KIND>::P 
 		{
 			p = hit;
 			n = calculate_normals<KIND>(f, p, thrCur, w);
-			result = object_shader<KIND>(f, p, n, thrCur, thrCur, seed, w);
-			out = out + v3(result.x * colourTemp.x, result.y * colourTemp.y, result.z * colourTemp.z);
+			const V3 colour = surface_colour<KIND>(f, p, w);
+			V3 spec;
+			result = object_shader<KIND>(f, p, n, thrCur, thrCur, dirf, colour, spec, seed, w);
+			newColour = colour;
+			out = out + v3((result.x + spec.x + f.lumEmissive[0]) * colourTemp.x,
+							(result.y + spec.y + f.lumEmissive[1]) * colourTemp.y,
+							(result.z + spec.z + f.lumEmissive[2]) * colourTemp.z);
+			result = result + spec;
 		}
 		else
 		{
@@ -463,10 +530,31 @@ __device__ V3 global_illumination(const FrameConst &f, typename PrecOf<KIND>::P 
 		result = v3(fminf(fmaxf(result.x, 0.0f), lim), fminf(fmaxf(result.y, 0.0f), lim),
 			fminf(fmaxf(result.z, 0.0f), lim));
 		out = out + v3(result.x * colourTemp.x, result.y * colourTemp.y, result.z * colourTemp.z);
-		colourTemp = colour;
+		colourTemp = newColour;
 		if (finished) break;
 	}
 	return out;
+}
+
+// The glow of VolumetricShader (shader_volumetric.cpp:86-92, 196-204) with the other volumetric
+// effects off: stepCount - 1 identical blends towards the glow colour.  The reference skips a
+// blend only when a march step is below 1e-10 of the distance threshold, which needs a DE factor
+// below 1e-9 (the host rejects glow there).
+__device__ __forceinline__ V3 glow_shader(const FrameConst &f, int stepCount, V3 rgb)
+{
+#pragma clang fp contract(off)
+	if (stepCount <= 0) return rgb;
+	const float glow = (float)stepCount * f.glowIntensity / 512.0f * f.DEFactor;
+	float glowN = 1.0f - glow;
+	if (glowN < 0.0f) glowN = 0.0f;
+	const V3 g = v3(f.glowColor1[0] * glowN + f.glowColor2[0] * glow, f.glowColor1[1] * glowN + f.glowColor2[1] * glow,
+		f.glowColor1[2] * glowN + f.glowColor2[2] * glow);
+	float opacity = glow / (float)stepCount;
+	if (opacity > 1.0f) opacity = 1.0f;
+	const float keep = 1.0f - opacity;
+	for (int k = 1; k < stepCount; k++)
+		rgb = v3(opacity * g.x + keep * rgb.x, opacity * g.y + keep * rgb.y, opacity * g.z + keep * rgb.z);
+	return rgb;
 }
 
 struct PixelOut
@@ -514,13 +602,18 @@ __device__ void render_pixel(const FrameConst &f, int xs, int ys, PixelOut &po, 
 		{
 			n = calculate_normals<KIND>(f, m.point, m.distThresh, w);
 			const R delta = calc_delta<KIND>(f, m.point);
-			rgb = object_shader<KIND, FULL>(f, m.point, n, m.distThresh, delta, seed, w);
+			const V3 colour = FULL ? surface_colour<KIND>(f, m.point, w)
+								   : v3(f.surfaceColor[0], f.surfaceColor[1], f.surfaceColor[2]);
+			V3 spec;
+			rgb = object_shader<KIND, FULL>(f, m.point, n, m.distThresh, delta, Q::tof(dir), colour, spec, seed, w);
 			if (FULL && f.monteCarlo && f.globalIllumination)
-				rgb = rgb + global_illumination<KIND>(f, m.point, n, m.distThresh, seed, w);
+				rgb = rgb + global_illumination<KIND>(f, m.point, n, m.distThresh, colour, seed, w);
+			if (FULL) rgb = rgb + spec; // resultShader = objectShader + specular (render_worker.cpp:1458-1462)
 			rgb = v3(fmaxf(rgb.x, 0.0f), fmaxf(rgb.y, 0.0f), fmaxf(rgb.z, 0.0f));
 		}
 		else
 			rgb = background_shader(f, Q::tof(dir));
+		if (FULL && f.glow) rgb = glow_shader(f, m.stepCount, rgb);
 		acc = acc + rgb;
 		if (rep == 0)
 		{

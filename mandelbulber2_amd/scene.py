@@ -5,6 +5,7 @@ Defaults are the values cParameterContainer registers in mandelbulber2/src/initp
 toRGBFloat makes them (color_structures.hpp:106-109).  Only parameters on the raymarch path are
 represented.  The library holds the same table (mbx_params_default); a test keeps them equal.
 """
+import ctypes
 import math
 import struct
 
@@ -170,6 +171,16 @@ def plain_material(p):
     p.specularPlasticEnable = 0
     p.metallic = 0
     p.glowEnabled = 0
+    return p
+
+
+def reference_material(p):
+    """Material #1 and glow as the reference initialises them (palette colouring from the default
+    surface gradient, plastic and metallic specular highlights, glow on), restored onto a scene
+    made with plain_material.  These fields close the parameter block."""
+    d = default_params(p.imageWidth, p.imageHeight)
+    off = abi.MbxParams.useColorsFromPalette.offset
+    ctypes.memmove(ctypes.addressof(p) + off, ctypes.addressof(d) + off, ctypes.sizeof(abi.MbxParams) - off)
     return p
 
 

@@ -47,6 +47,9 @@ def lib(fast=False):
                                    C.c_void_p]
     L.mbxo_diff_de.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                C.c_void_p, C.c_void_p]
+    L.mbxo_colour_index.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_int, C.c_void_p]
+    L.mbxo_gradient.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_int, C.c_void_p]
+    L.mbxo_surface_colour.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_int, C.c_void_p]
     L.mbxo_random_int.argtypes = [P(C.c_uint32)]
     L.mbxo_random.argtypes = [P(C.c_uint32), C.c_int]
     L.mbxo_pixel_seed.argtypes = [C.c_uint32, C.c_uint32]
@@ -121,6 +124,36 @@ def formula_step(p, z4, aux3):
     if rc != 0:
         raise RuntimeError("oracle formula_step failed: %d" % rc)
     return z4, aux3
+
+
+def colour_index(p, pts):
+    """Colour index of fractal-local points (the colouring pass of Compute)."""
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    out = np.zeros(len(pts), np.float64)
+    rc = lib().mbxo_colour_index(C.byref(p), _ptr(pts), len(pts), _ptr(out))
+    if rc != 0:
+        raise RuntimeError("oracle colour_index failed: %d" % rc)
+    return out
+
+
+def gradient(p, positions):
+    """Surface gradient colours (GetColorFloat) at float32 palette positions."""
+    pos = np.ascontiguousarray(positions, np.float32).ravel()
+    rgb = np.zeros((len(pos), 3), np.float32)
+    rc = lib().mbxo_gradient(C.byref(p), _ptr(pos), len(pos), _ptr(rgb))
+    if rc != 0:
+        raise RuntimeError("oracle gradient failed: %d" % rc)
+    return rgb
+
+
+def surface_colour(p, pts):
+    """SurfaceColour at world points."""
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    rgb = np.zeros((len(pts), 3), np.float32)
+    rc = lib().mbxo_surface_colour(C.byref(p), _ptr(pts), len(pts), _ptr(rgb))
+    if rc != 0:
+        raise RuntimeError("oracle surface_colour failed: %d" % rc)
+    return rgb
 
 
 def frame_constants(p):

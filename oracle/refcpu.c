@@ -707,6 +707,116 @@ static void compute(const scene_t *s, int mode, v3 point, int forcedMaxiter, int
 	out->z = V3(z.x, z.y, z.z);
 }
 
+/* Compute<calcModeColouring> (compute_fractal.cpp:67-654 with the colouring branch :343-436)
+ * followed by CalculateColorIndex for a single formula (fractal_coloring.cpp:318-340).  point is
+ * the fractal-local point (sShaderInputData::GetFractalPoint); SurfaceColour calls it with
+ * maxiterMultiplier 4 (shader_surface_color.cpp:51-55).  Colouring algorithm numbers follow
+ * enumFractalColoring (fractal_coloring.hpp:44-52). */
+static double compute_colour_index(const scene_t *s, v3 point)
+{
+	const mbx_params *p = s->p;
+	v4 z = V4(point.x, point.y, point.z, 0.0);
+	z.w = s->initialWAxis;
+	const v4 startPoint = V4(point.x, point.y, point.z, z.w); /* CVector4(in.point, initialWAxisColor) */
+	aux_t aux;
+	memset(&aux, 0, sizeof(aux));
+	aux.c = z;
+	aux.const_c = z;
+	aux.old_z = z;
+	aux.pos_neg = 1.0;
+	aux.r = v4_len(z);
+	aux.DE = 1.0;
+	aux.dist = 1000.0;
+	aux.pseudoKleinianDE = 1.0;
+	aux.actualScale = s->mbScale;
+	aux.color = 1.0;
+	aux.temp1000 = 1000.0;
+	double colorMin = 1000.0;
+	const int isBox = s->formula == MBX_FORMULA_MANDELBOX;
+	const int alg = p->coloringAlgorithm;
+	const v3 lineDir = v3_from(p->coloringLineDirection);
+	v4 lastZ = V4(0, 0, 0, 0);
+	const int maxN = s->formulaMaxiter * 4;
+	for (int i = 0; i < maxN; i++)
+	{
+		lastZ = z;
+		aux.i = i;
+		formula_code(s, &z, &aux);
+		if (s->addCConstant)
+		{
+			if (s->juliaEnabled)
+				z = v4_add(z, V4(s->juliaConstant.x * s->constantMultiplier.x,
+												s->juliaConstant.y * s->constantMultiplier.y,
+												s->juliaConstant.z * s->constantMultiplier.z, 0.0));
+			else
+				z = v4_add(z, V4(aux.const_c.x * s->constantMultiplier.x,
+												aux.const_c.y * s->constantMultiplier.y,
+												aux.const_c.z * s->constantMultiplier.z, aux.const_c.w));
+		}
+		aux.r = v4_len(z);
+		if (v4_notfinite(z))
+		{
+			z = lastZ;
+			aux.r = v4_len(z);
+			break;
+		}
+		v4 colorZ = z;
+		if (!p->coloring4D) colorZ.w = 0.0;
+		double len = 0.0;
+		switch (alg)
+		{
+			case 0: len = v4_len(colorZ); break;
+			case 1: len = fabs(v4_dot(colorZ, startPoint)); break;
+			case 2: len = fabs(v4_len(v4_sub(colorZ, startPoint)) - p->coloringSphereRadius); break;
+			case 3:
+			{
+				const double ax = fabs(colorZ.x), ay = fabs(colorZ.y), az = fabs(colorZ.z);
+				len = (ax < ay) ? (ax < az ? ax : az) : (ay < az ? ay : az); /* dMin */
+				if (p->coloring4D && fabs(colorZ.w) < len) len = fabs(colorZ.w); /* std::min */
+				break;
+			}
+			case 4: len = fabs(colorZ.x * lineDir.x + colorZ.y * lineDir.y + colorZ.z * lineDir.z); break;
+			default: len = aux.r; break;
+		}
+		const double change = v4_len(v4_sub(z, lastZ)) / aux.r;
+		if (!p->coloringPreV215)
+		{
+			if (!isBox)
+			{
+				if (len < colorMin) colorMin = len;
+				if (aux.r > s->bailout) break;
+			}
+			else if (alg == 0)
+			{
+				if (aux.r > 1e15 || change < 1e-15) break;
+			}
+			else
+			{
+				if (len < colorMin) colorMin = len;
+				if (aux.r > s->bailout || change < 1e-15) break;
+			}
+		}
+		else
+		{
+			if (!isBox || alg != 0)
+			{
+				if (len < colorMin) colorMin = len;
+				if (aux.r > 1e15 || change < 1e-15) break;
+			}
+			else if (aux.r > 1e15 || change < 1e-15)
+				break;
+		}
+	}
+	switch (s->formula)
+	{
+		case MBX_FORMULA_MANDELBOX: /* coloringFunctionABox */
+			return aux.color * 100.0 + aux.r * p->fractal.mandelbox.colorFactorR / 1e13 +
+				((alg != 0) ? colorMin * 1000.0 : 0.0);
+		case MBX_FORMULA_KALEIDOSCOPIC_IFS: return colorMin * 1000.0; /* coloringFunctionIFS */
+		default: return colorMin * 5000.0;                            /* coloringFunctionDefault */
+	}
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* CalculateDistance for one fractal node (calculate_distance.cpp:225-516 -> 533-763)          */
 /* ------------------------------------------------------------------------------------------ */
@@ -859,6 +969,11 @@ typedef struct
 	v3 point;
 	int steps;
 	int iters;
+	/* sRayMarchingInOut: buffCount and stepBuff[i].step / .point (render_worker.cpp:725-838);
+	 * stepStep / stepPoint may be NULL when the caller needs no step buffer */
+	int buffCount;
+	double *stepStep;
+	v3 *stepPoint;
 } march_t;
 
 static void ray_marching(ray_ctx *rc, v3 start, v3 direction, double minScan, double maxScan,
@@ -879,6 +994,7 @@ static void ray_marching(ray_ctx *rc, v3 start, v3 direction, double minScan, do
 	int deadComputationFound = 0;
 	int iters = 0;
 	const double dirLength = v3_len(direction);
+	out->buffCount = 0;
 
 	for (int i = 0; i < p->maxRaymarchingSteps; i++)
 	{
@@ -899,11 +1015,14 @@ static void ray_marching(ray_ctx *rc, v3 start, v3 direction, double minScan, do
 			found = 1;
 			break;
 		}
+		if (out->stepStep) out->stepStep[i] = step;
 		if (p->interiorMode)
 			step = (dist - 0.8 * distThresh) * p->DEFactor * (1.0 - mbxo_random(rc->seed, 1000) / 10000.0);
 		else
 			step = (dist - 0.5 * distThresh) * p->DEFactor * (1.0 - mbxo_random(rc->seed, 1000) / 10000.0);
 		if (step > 3.0) step = 3.0;
+		if (out->stepPoint) out->stepPoint[i] = point;
+		out->buffCount = i + 1;
 		scan += step / dirLength;
 		if (scan > maxScan) break;
 	}
@@ -1104,12 +1223,62 @@ static rgba fast_ambient_occlusion(ray_ctx *rc, const shin_t *in)
 	return RGBA(ao, ao, ao, 1.0f);
 }
 
-/* LightShading for one directional light without specular (shader_light_shading.cpp:40-122),
- * cLight::CalculateLightVector (light.cpp:277-302), AuxLightsShader (shader_aux_light.cpp) */
-static rgba light_shading(ray_ctx *rc, const shin_t *in)
+/* SpecularHighlight (shader_specular_highlight.cpp:38-73) without diffusion texture or diffuse
+ * gradient (diffuse = 1, texDiffuse = 1).  The material's widths and roughness are floats
+ * (material.h:76-80).  The roughness factor draws from the random stream (Random(1000)). */
+static rgba specular_highlight(ray_ctx *rc, const shin_t *in, v3 lightVector, float specularWidth,
+	float roughness)
+{
+	const mbx_params *p = rc->s->p;
+	v3 half = v3_normalize(v3_sub(lightVector, in->viewVector));
+	float shade2 = (float)v3_dot(in->normal, half);
+	if (shade2 < 0.0f) shade2 = 0.0f;
+	const float diffuse = 1.0f;
+	shade2 = (float)pow(shade2, 30.0f / specularWidth / diffuse) / diffuse;
+	if (roughness > 0.0f) shade2 = (float)(shade2 * (1.0 + mbxo_random(rc->seed, 1000) / 1000.0f * roughness));
+	if (shade2 > 15.0f) shade2 = 15.0f;
+	rgba spec;
+	spec.R = shade2 * (float)p->specularColor[0] * (1.0f * 0.5f + 0.5f);
+	spec.G = shade2 * (float)p->specularColor[1] * (1.0f * 0.5f + 0.5f);
+	spec.B = shade2 * (float)p->specularColor[2] * (1.0f * 0.5f + 0.5f);
+	spec.A = 0.0f;
+	return spec;
+}
+
+/* SpecularHighlightCombined (shader_specular_highlight_combined.cpp:39-68): plastic highlight,
+ * then the metallic one tinted by the surface colour */
+static rgba specular_combined(ray_ctx *rc, const shin_t *in, v3 lightVector, rgba surfaceColor)
+{
+	const mbx_params *p = rc->s->p;
+	rgba plastic = RGBA(0, 0, 0, 0), metal = RGBA(0, 0, 0, 0);
+	if (p->specularPlasticEnable)
+	{
+		plastic = specular_highlight(rc, in, lightVector, (float)p->specularWidth, 0.0f);
+		const float k = (float)p->specular;
+		plastic.R *= k;
+		plastic.G *= k;
+		plastic.B *= k;
+	}
+	if (p->metallic)
+	{
+		metal = specular_highlight(rc, in, lightVector, (float)p->specularMetallicWidth,
+			(float)p->specularMetallicRoughness);
+		const float k = (float)p->specularMetallic;
+		metal.R *= k * surfaceColor.R;
+		metal.G *= k * surfaceColor.G;
+		metal.B *= k * surfaceColor.B;
+	}
+	return RGBA(plastic.R + metal.R, plastic.G + metal.G, plastic.B + metal.B, 0.0f);
+}
+
+/* LightShading for one directional light (shader_light_shading.cpp:40-122) with its specular
+ * highlight, cLight::CalculateLightVector (light.cpp:277-302) and CalculateCone (light.cpp:214:
+ * 1 and white for a directional light), summed by AuxLightsShader (shader_aux_light.cpp:38-63) */
+static rgba light_shading(ray_ctx *rc, const shin_t *in, rgba surfaceColor, rgba *specOut)
 {
 	const scene_t *s = rc->s;
 	const mbx_params *p = s->p;
+	*specOut = RGBA(0, 0, 0, 0);
 	if (!p->light.enabled) return RGBA(0, 0, 0, 0);
 	const v3 lightVector = s->lightDirection;
 	const double distance = p->light.penetrating ? in->delta / s->resolution : p->viewDistanceMax;
@@ -1120,30 +1289,104 @@ static rgba light_shading(ray_ctx *rc, const shin_t *in)
 	shade = 1.0f - matShading + shade * matShading;
 	shade = shade * intensity;
 	if (shade > 500.0f) shade = 500.0f;
+	rgba specular = specular_combined(rc, in, lightVector, surfaceColor);
+	specular.R *= intensity;
+	specular.G *= intensity;
+	specular.B *= intensity;
+	const double specularMax = (specular.R > specular.G) ? (specular.R > specular.B ? specular.R : specular.B)
+		: (specular.G > specular.B ? specular.G : specular.B); /* dMax */
 	float auxShadow = 1.0f;
 	if (p->light.castShadows)
 	{
-		if (shade > 0.001f)
+		if (shade > 0.001f || specularMax > 0.001f)
+		{
 			auxShadow = aux_shadow(rc, in, distance, lightVector);
+			specular.R *= auxShadow;
+			specular.G *= auxShadow;
+			specular.B *= auxShadow;
+		}
 		else
+		{
 			auxShadow = 0.0f;
+			specular = RGBA(0, 0, 0, 0);
+		}
 	}
 	rgba out;
 	out.R = shade * s->lightColor[0] * auxShadow;
 	out.G = shade * s->lightColor[1] * auxShadow;
 	out.B = shade * s->lightColor[2] * auxShadow;
 	out.A = 0;
+	specOut->R = specular.R * s->lightColor[0];
+	specOut->G = specular.G * s->lightColor[1];
+	specOut->B = specular.B * s->lightColor[2];
+	specOut->A = 0;
 	return out;
 }
 
-/* ObjectShader (shader_object.cpp:40-229) with palette, textures, specular, env mapping,
- * fake lights and iridescence off */
-static rgba object_shader(ray_ctx *rc, const shin_t *in, rgba *surfaceColour)
+/* cColorGradient::GetColorFloat without smoothing (color_gradient.cpp:128-136, 180-229): the stop
+ * below the position by PaletteIterator, then linear interpolation of the 8-bit channels / 256.
+ * The last stop and a zero-width interval return the 8-bit values unscaled, as the reference does. */
+static rgba gradient_colour(const mbx_params *p, float position)
+{
+	const int n = p->surfaceGradientCount;
+	int idx = 0;
+	while (idx < n - 1 && position > (float)p->surfaceGradientPosition[idx + 1]) idx++;
+	rgba c;
+	c.A = 1.0f;
+	if (idx == n - 1)
+	{
+		c.R = (float)p->surfaceGradientColor[idx - 1][0];
+		c.G = (float)p->surfaceGradientColor[idx - 1][1];
+		c.B = (float)p->surfaceGradientColor[idx - 1][2];
+		return c;
+	}
+	const float c1[3] = {(float)p->surfaceGradientColor[idx][0], (float)p->surfaceGradientColor[idx][1],
+		(float)p->surfaceGradientColor[idx][2]};
+	const float c2[3] = {(float)p->surfaceGradientColor[idx + 1][0], (float)p->surfaceGradientColor[idx + 1][1],
+		(float)p->surfaceGradientColor[idx + 1][2]};
+	const float pos1 = (float)p->surfaceGradientPosition[idx], pos2 = (float)p->surfaceGradientPosition[idx + 1];
+	if (pos2 - pos1 > 0.0f)
+	{
+		const float delta = (position - pos1) / (pos2 - pos1);
+		const float nDelta = 1.0f - delta;
+		c.R = (c1[0] * nDelta + c2[0] * delta) / 256.0f;
+		c.G = (c1[1] * nDelta + c2[1] * delta) / 256.0f;
+		c.B = (c1[2] * nDelta + c2[2] * delta) / 256.0f;
+	}
+	else
+	{
+		c.R = c1[0];
+		c.G = c1[1];
+		c.B = c1[2];
+	}
+	return c;
+}
+
+/* SurfaceColour for the fractal object (shader_surface_color.cpp:39-108): the palette colour
+ * from the colour index of the fractal-local point, or the material colour */
+static rgba surface_colour(const scene_t *s, v3 point)
+{
+	const mbx_params *p = s->p;
+	rgba colour = RGBA((float)p->surfaceColor[0], (float)p->surfaceColor[1], (float)p->surfaceColor[2], 1.0f);
+	if (!p->useColorsFromPalette) return colour;
+	v3 local = s->nodeRotated ? rot_apply3(&s->nodeRot, point) : point;
+	const double colorIndex = compute_colour_index(s, local);
+	const double nrCol = fmod(fabs(colorIndex), 248.0 * 256.0);
+	const float speed = (float)p->coloringSpeed, offset = (float)p->coloringPaletteOffset;
+	const double colorPosition = fmod(nrCol / 256.0 / 10.0 * speed + offset, 1.0);
+	if (p->surfaceGradientEnable) colour = gradient_colour(p, (float)colorPosition);
+	return colour;
+}
+
+/* ObjectShader (shader_object.cpp:40-229) with textures, env mapping, fake lights, perlin noise,
+ * iridescence and the non-surface gradients off.  Returns the shaded colour; the surface colour,
+ * the specular highlight and the emissive luminosity go to the out arguments. */
+static rgba object_shader(ray_ctx *rc, const shin_t *in, rgba *surfaceColour, rgba *specularOut,
+	float luminosityEmissiveOut[3])
 {
 	const scene_t *s = rc->s;
 	const mbx_params *p = s->p;
-	rgba colour = RGBA((float)p->surfaceColor[0], (float)p->surfaceColor[1],
-		(float)p->surfaceColor[2], 1.0f);
+	rgba colour = surface_colour(s, in->point);
 	*surfaceColour = colour;
 	rgba ambient = RGBA(0, 0, 0, 0);
 	if (p->ambientOcclusionEnabled)
@@ -1158,12 +1401,17 @@ static rgba object_shader(ray_ctx *rc, const shin_t *in, rgba *surfaceColour)
 	ambient2.R = ambient.R * aoI * (float)p->ambientOcclusionColor[0];
 	ambient2.G = ambient.G * aoI * (float)p->ambientOcclusionColor[1];
 	ambient2.B = ambient.B * aoI * (float)p->ambientOcclusionColor[2];
-	rgba auxLights = light_shading(rc, in);
+	rgba auxLightsSpecular;
+	rgba auxLights = light_shading(rc, in, colour, &auxLightsSpecular);
 	rgba luminosity;
 	const float lum = (float)p->luminosity;
 	luminosity.R = lum * (float)p->luminosityColor[0];
 	luminosity.G = lum * (float)p->luminosityColor[1];
 	luminosity.B = lum * (float)p->luminosityColor[2];
+	const float emissive = (float)p->luminosityEmissive;
+	luminosityEmissiveOut[0] = luminosity.R * emissive;
+	luminosityEmissiveOut[1] = luminosity.G * emissive;
+	luminosityEmissiveOut[2] = luminosity.B * emissive;
 	rgba output;
 	output.R = ((float)p->fillLightColor[0] + ambient2.R) * colour.R;
 	output.G = ((float)p->fillLightColor[1] + ambient2.G) * colour.G;
@@ -1175,6 +1423,7 @@ static rgba object_shader(ray_ctx *rc, const shin_t *in, rgba *surfaceColour)
 	output.G += luminosity.G;
 	output.B += luminosity.B;
 	output.A = 1.0f;
+	*specularOut = RGBA(auxLightsSpecular.R, auxLightsSpecular.G, auxLightsSpecular.B, output.A);
 	return output;
 }
 
@@ -1282,15 +1531,16 @@ static void global_illumination(ray_ctx *rc, const shin_t *input, rgba objectCol
 		}
 		if (found)
 		{
-			rgba oc;
-			rgba objectShader = object_shader(rc, &ic, &oc);
+			rgba oc, specular;
+			float emissive[3];
+			rgba objectShader = object_shader(rc, &ic, &oc, &specular, emissive);
 			newColor = oc;
-			resultShader.R = objectShader.R;
-			resultShader.G = objectShader.G;
-			resultShader.B = objectShader.B;
-			out[0] += objectShader.R * objectColorTemp.R;
-			out[1] += objectShader.G * objectColorTemp.G;
-			out[2] += objectShader.B * objectColorTemp.B;
+			resultShader.R = objectShader.R + specular.R;
+			resultShader.G = objectShader.G + specular.G;
+			resultShader.B = objectShader.B + specular.B;
+			out[0] += (objectShader.R + specular.R + emissive[0]) * objectColorTemp.R;
+			out[1] += (objectShader.G + specular.G + emissive[1]) * objectColorTemp.G;
+			out[2] += (objectShader.B + specular.B + emissive[2]) * objectColorTemp.B;
 		}
 		else
 		{
@@ -1327,12 +1577,44 @@ typedef struct
 	v3 normal;
 } ray_result;
 
+/* The glow part of VolumetricShader (shader_volumetric.cpp:70-204) with every other volumetric
+ * effect off and volumetric_light_DE_Factor 1: stepCount - 1 blends towards the glow colour,
+ * walking the step buffer from the end, each blend taking the steps skipped before it. */
+static void glow_shader(const scene_t *s, const march_t *m, float rgb[3])
+{
+	const mbx_params *p = s->p;
+	const int stepCount = m->buffCount;
+	if (!p->glowEnabled || stepCount <= 0) return;
+	const float glow = stepCount * (float)p->glowIntensity / 512.0f * (float)p->DEFactor;
+	float glowN = 1.0f - glow;
+	if (glowN < 0.0f) glowN = 0.0f;
+	const float g[3] = {(float)p->glowColor1[0] * glowN + (float)p->glowColor2[0] * glow,
+		(float)p->glowColor1[1] * glowN + (float)p->glowColor2[1] * glow,
+		(float)p->glowColor1[2] * glowN + (float)p->glowColor2[2] * glow};
+	double totalStep = 0.0;
+	for (int index = stepCount - 1; index > 0; index--)
+	{
+		totalStep += m->stepStep[index];
+		if (totalStep < 1e-10 * calc_dist_thresh(s, m->stepPoint[index])) continue;
+		totalStep = 0.0;
+		float glowOpacity = (float)(glow / stepCount * 1.0);
+		if (glowOpacity > 1.0f) glowOpacity = 1.0f;
+		for (int k = 0; k < 3; k++) rgb[k] = glowOpacity * g[k] + (1.0f - glowOpacity) * rgb[k];
+	}
+}
+
 static void trace_primary(ray_ctx *rc, v3 start, v3 viewVector, ray_result *res)
 {
 	const scene_t *s = rc->s;
 	const mbx_params *p = s->p;
 	v3 direction = v3_normalize(viewVector);
-	march_t m;
+	march_t m = {0};
+	if (p->glowEnabled)
+	{
+		const size_t n = (size_t)(p->maxRaymarchingSteps > 0 ? p->maxRaymarchingSteps : 0) + 2;
+		m.stepStep = (double *)calloc(n, sizeof(double));
+		m.stepPoint = (v3 *)calloc(n, sizeof(v3));
+	}
 	ray_marching(rc, start, direction, 0.0, p->viewDistanceMax, 1, &m);
 	shin_t in;
 	in.point = m.point;
@@ -1347,8 +1629,9 @@ static void trace_primary(ray_ctx *rc, v3 start, v3 viewVector, ray_result *res)
 	if (m.found)
 	{
 		in.normal = calculate_normals(rc, &in);
-		rgba objectColour;
-		rgba objectShader = object_shader(rc, &in, &objectColour);
+		rgba objectColour, specular;
+		float emissive[3];
+		rgba objectShader = object_shader(rc, &in, &objectColour, &specular, emissive);
 		if (p->DOFMonteCarlo && p->DOFMonteCarloGlobalIllumination)
 		{
 			float gi[3];
@@ -1357,7 +1640,8 @@ static void trace_primary(ray_ctx *rc, v3 start, v3 viewVector, ray_result *res)
 			objectShader.G += gi[1];
 			objectShader.B += gi[2];
 		}
-		float R = objectShader.R, G = objectShader.G, B = objectShader.B;
+		/* resultShader = objectShader + specular, clamped at 0 (render_worker.cpp:1458-1602) */
+		float R = objectShader.R + specular.R, G = objectShader.G + specular.G, B = objectShader.B + specular.B;
 		if (R < 0.0f) R = 0.0f;
 		if (G < 0.0f) G = 0.0f;
 		if (B < 0.0f) B = 0.0f;
@@ -1376,6 +1660,10 @@ static void trace_primary(ray_ctx *rc, v3 start, v3 viewVector, ray_result *res)
 		res->depth = 1e20;
 		res->normal = V3(0, 0, 0);
 	}
+	/* VolumetricShader on the result, outside any object (render_worker.cpp:1775-1779) */
+	glow_shader(s, &m, res->rgb);
+	free(m.stepStep);
+	free(m.stepPoint);
 }
 
 /* CalculateViewVector, three-point perspective (projection_3d.cpp:41-89) */
@@ -1607,7 +1895,7 @@ int mbxo_march_pixel(const mbx_params *p, int x, int y, double *dir3, double *de
 	v3 d = v3_normalize(view_vector(&s, ix, iy));
 	uint32_t seed = mbxo_pixel_seed((uint32_t)y * (uint32_t)W + (uint32_t)x, p->randomSeed);
 	ray_ctx rc = {&s, &seed, 0, 0};
-	march_t m;
+	march_t m = {0};
 	ray_marching(&rc, s.camera, d, 0.0, p->viewDistanceMax, 1, &m);
 	dir3[0] = d.x;
 	dir3[1] = d.y;
@@ -1638,6 +1926,49 @@ int mbxo_normal(const mbx_params *p, const double *pt3, double distThresh, doubl
 	n3[0] = n.x;
 	n3[1] = n.y;
 	n3[2] = n.z;
+	return MBX_OK;
+}
+
+/* Colour index (Compute<calcModeColouring> + CalculateColorIndex) of fractal-local points. */
+int mbxo_colour_index(const mbx_params *p, const double *pts, int n, double *out)
+{
+	scene_t s;
+	int rc0 = scene_init(&s, p);
+	if (rc0) return rc0;
+	for (int k = 0; k < n; k++) out[k] = compute_colour_index(&s, V3(pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]));
+	return MBX_OK;
+}
+
+/* Surface gradient colour at palette positions (GetColorFloat), 3 floats per position. */
+int mbxo_gradient(const mbx_params *p, const float *positions, int n, float *rgb)
+{
+	if (p->surfaceGradientCount < 2 || p->surfaceGradientCount > MBX_MAX_GRADIENT_STOPS) return MBX_ERR_INVALID;
+	for (int k = 0; k < n; k++)
+	{
+		rgba c = gradient_colour(p, positions[k]);
+		rgb[3 * k] = c.R;
+		rgb[3 * k + 1] = c.G;
+		rgb[3 * k + 2] = c.B;
+	}
+	return MBX_OK;
+}
+
+/* SurfaceColour at world points, 3 floats per point. */
+int mbxo_surface_colour(const mbx_params *p, const double *pts, int n, float *rgb)
+{
+	scene_t s;
+	int rc0 = scene_init(&s, p);
+	if (rc0) return rc0;
+	if (p->useColorsFromPalette && p->surfaceGradientEnable &&
+		(p->surfaceGradientCount < 2 || p->surfaceGradientCount > MBX_MAX_GRADIENT_STOPS))
+		return MBX_ERR_INVALID;
+	for (int k = 0; k < n; k++)
+	{
+		rgba c = surface_colour(&s, V3(pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]));
+		rgb[3 * k] = c.R;
+		rgb[3 * k + 1] = c.G;
+		rgb[3 * k + 2] = c.B;
+	}
 	return MBX_OK;
 }
 
@@ -1796,7 +2127,7 @@ int mbxo_render_diff(const mbx_params *p, int y0, int y1, int threads, double *r
 			const double ix = ((1.0 / W) * xs + (-0.5)) * s.aspectRatio;
 			const double iy = (-1.0 / H) * ys + 0.5;
 			const v3 d = v3_normalize(view_vector(&s, ix, iy));
-			march_t m;
+			march_t m = {0};
 			ray_marching(&rc, s.camera, d, 0.0, p->viewDistanceMax, 1, &m);
 			double col[3], n[3] = {0, 0, 0}, t = m.depth;
 			int it = m.iters, ref = 0, br = 0;
