@@ -9,8 +9,10 @@ launch's device time, HIP events on the launch stream), the plain per-frame kern
 same frame, and the CPU oracle timed on this host.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3]
-                    [--workload mandelbulb8|mandelbox|animation64|deepzoom|adjoint512]
+                    [--workload mandelbulb8|material|mandelbox|animation64|deepzoom|adjoint512]
 
+`--workload material` times the same 1080p Mandelbulb-8 with the reference's default material:
+palette colouring from the default surface gradient, plastic and metallic highlights, glow.
 `--workload mandelbox` times the Mandelbox scene instead (scale 2, delta DE by the reference's
 finite differences in FP64, hard shadow, 16-ray AO, seeded camera; 1920x1080).
 `--workload animation64` times a 64-frame batch (power 2 -> 12, mixed resolutions, 4 spp with
@@ -33,7 +35,7 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X published FP64 peak
 # Algorithmic flop counts (fixed, independent of the instruction mix): per DE evaluation
 # (ray point, threshold, DE finish, step) and per formula iteration.
 F_DE = 25
-F_IT = {"mandelbulb8": 48, "mandelbox": 29, "animation64": 32, "deepzoom": 29}
+F_IT = {"mandelbulb8": 48, "material": 48, "mandelbox": 29, "animation64": 32, "deepzoom": 29}
 # A far-field step (no DE evaluation: the step is provably the clamped 3) does the ray point (6),
 # |p|^2 (5) and the scan update (2).
 F_FAR = 13
@@ -45,6 +47,17 @@ WORKLOADS = {
         "config": {"formula": "mandelbulb power 8, analytic log DE, bailout 10, maxiter 250",
                    "camera": "distance 3.5, fov 53 deg", "spp": 1,
                    "shading": "6-tap normals + Lambert, 1 directional light, no shadows/AO/GI"},
+    },
+    "material": {
+        "metric": "Mrays/s primary DE raymarch with the default material (palette colouring, "
+                  "plastic + metallic highlights, glow), 1080p Mandelbulb-8, 1 GPU; % FP32 VALU peak",
+        "kernel": "batch_kernel<kBulb8> (full variant)", "dtype": "f32", "peak": FP32_PEAK_TFLOPS,
+        "config": {"formula": "mandelbulb power 8, analytic log DE, bailout 10, maxiter 250",
+                   "camera": "distance 3.5, fov 53 deg", "spp": 1,
+                   "shading": "6-tap normals, Lambert + plastic and metallic highlights (roughness "
+                              "0.1), palette colour from a 4 x maxiter colouring orbit, glow over "
+                              "the march steps, 1 directional light, no shadows/AO/GI",
+                   "flops": "the DE model only: colouring orbit, highlights and glow not counted"},
     },
     "animation64": {
         "metric": "Mrays/s (samples) of a 64-frame animation batch in one launch, "
@@ -116,6 +129,7 @@ def measured_traffic(workload, w, h):
     """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this
     workload (copy-calibrated), or None when there is none for this size."""
     name = {"mandelbulb8": "batch_kernel_bulb8_mandelbulb8_1080p.md",
+            "material": "batch_kernel_bulb8_material_1080p.md",
             "mandelbox": "batch_kernel_boxdelta_mandelbox_1080p.md"}.get(workload)
     if not name or (w, h) != (1920, 1080):
         return None, None
@@ -132,6 +146,8 @@ def make_scene(name, w, h):
     from mandelbulber2_amd import scene
     if name == "mandelbulb8":
         return scene.mandelbulb8(w, h)
+    if name == "material":
+        return scene.reference_material(scene.mandelbulb8(w, h))
     if name == "animation64":
         return scene.animation64(seed=3)
     if name == "deepzoom":
