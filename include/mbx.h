@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MBX_ABI_VERSION 3
+#define MBX_ABI_VERSION 4
 
 /* array sizes as in mandelbulber2/src/fractal.h:52-54 */
 #define MBX_IFS_VECTOR_COUNT 9
@@ -38,6 +38,8 @@ extern "C" {
 /* upper bound of the stops of a colour gradient (the first colour counts twice: at its position
  * and again at 1.0) */
 #define MBX_MAX_GRADIENT_STOPS 64
+/* fractal containers of a hybrid sequence (NUMBER_OF_FRACTALS, mandelbulber2/src/fractal.h:52) */
+#define MBX_MAX_FRACTALS 9
 
 /* status codes */
 #define MBX_OK 0
@@ -46,12 +48,19 @@ extern "C" {
 #define MBX_ERR_HIP -3
 #define MBX_ERR_NOMEM -4
 
-/* enumFractalFormula ids (mandelbulber2/formula/definition/all_fractal_list_enums.hpp:45,51,53) */
+/* enumFractalFormula ids (mandelbulber2/formula/definition/all_fractal_list_enums.hpp:45-62) */
 enum mbx_formula {
 	MBX_FORMULA_NONE = 0,
 	MBX_FORMULA_MANDELBULB = 2,
+	MBX_FORMULA_MANDELBULB_POWER2 = 3,
+	MBX_FORMULA_HYPERCOMPLEX = 4,
+	MBX_FORMULA_QUATERNION = 5,
+	MBX_FORMULA_MENGER_SPONGE = 7,
 	MBX_FORMULA_MANDELBOX = 8,
-	MBX_FORMULA_KALEIDOSCOPIC_IFS = 10
+	MBX_FORMULA_KALEIDOSCOPIC_IFS = 10,
+	MBX_FORMULA_XENODREAMBUIE = 11,
+	MBX_FORMULA_BOX_FOLD_BULB_POW2 = 16,
+	MBX_FORMULA_AEXION = 19
 };
 
 /* enumDEType (mandelbulber2/src/fractal_enums.h:70-75) */
@@ -120,25 +129,46 @@ typedef struct mbx_fractal_ifs {
 	int32_t edgeEnabled;                       /* "IFS_edge_enabled" */
 } mbx_fractal_ifs;
 
-/* One single-formula fractal node: sFractal plus the sequence data that
- * cHybridFractalSequences::CollectSequenceData derives for it
- * (mandelbulber2/src/hybrid_fractal_sequences.cpp:181-287, 377-503). */
+/* Parameters of the other formulas on the path (sFractal members, fractal.cpp:93-363) */
+typedef struct mbx_fractal_transform {
+	double scale3;                /* "transf_scale_3" (Menger sponge scale) */
+	double aexionCadd;            /* "cadd" */
+	double foldingIntPowFoldFactor; /* "boxfold_bulbpow2_folding_factor" */
+	double foldingIntPowZFactor;  /* "boxfold_bulbpow2_z_factor" */
+	double analyticDEScale1;      /* "analyticDE_scale_1" */
+	double analyticDEOffset1;     /* "analyticDE_offset_1" */
+	double analyticDEOffset2;     /* "analyticDE_offset_2" */
+	int32_t analyticDEEnabled;    /* "analyticDE_enabled" */
+	int32_t analyticDEEnabledFalse; /* "analyticDE_enabled_false" */
+} mbx_fractal_transform;
+
+/* One fractal container (sFractal) with its hybrid-sequence fields, plus, in mbx_params.fractal,
+ * the node data that cHybridFractalSequences::CollectSequenceData takes from the rendered node
+ * (mandelbulber2/src/hybrid_fractal_sequences.cpp:181-287, 377-503): maxiter, Julia, constant
+ * factor, initial w, bailout and DE mode.  In a hybrid's further containers
+ * (mbx_params.hybridFractals) only the formula, its parameters and the sequence fields are read. */
 typedef struct mbx_fractal {
-	int32_t formula;            /* enum mbx_formula ("formula_1") */
+	int32_t formula;            /* enum mbx_formula ("formula") */
 	int32_t formulaMaxiter;     /* "N" / node formula_maxiter */
 	int32_t deltaDEMethod;      /* "delta_DE_method": 0 preferred, 1 force delta, 2 force analytic */
 	int32_t deltaDEFunction;    /* "delta_DE_function": 0 preferred, 1 linear, 2 logarithmic */
 	int32_t useDefaultBailout;  /* "use_default_bailout" */
 	int32_t juliaMode;          /* "julia_mode" */
 	int32_t dontAddCConstant;   /* "dont_add_c_constant" */
-	int32_t reserved0;
+	int32_t checkForBailout;    /* "check_for_bailout" (hybrid sequences; a single formula always checks) */
 	double bailout;             /* "bailout" (used when useDefaultBailout == 0) */
 	double juliaC[3];           /* "julia_c" */
 	double constantFactor[3];   /* "fractal_constant_factor" */
 	double initialWAxis;        /* "initial_waxis" */
+	int32_t formulaIterations;  /* "formula_iterations" (hybrid sequences) */
+	int32_t formulaStartIteration; /* "formula_start_iteration" */
+	int32_t formulaStopIteration;  /* "formula_stop_iteration" */
+	int32_t reserved0;
+	double formulaWeight;       /* "formula_weight" (hybrid sequences) */
 	mbx_fractal_mandelbulb bulb;
 	mbx_fractal_mandelbox mandelbox;
 	mbx_fractal_ifs IFS;
+	mbx_fractal_transform transform;
 } mbx_fractal;
 
 /* Light #1 (cLight, mandelbulber2/src/light.cpp:51-195).  Only the directional type
@@ -265,6 +295,14 @@ typedef struct mbx_params {
 	double glowIntensity;             /* "glow_intensity" */
 	double glowColor1[3];             /* "glow_color_1" */
 	double glowColor2[3];             /* "glow_color_2" */
+
+	/* hybrid sequence (cHybridFractalSequences::CreateSequence, hybrid_fractal_sequences.cpp:290-377):
+	 * hybridCount 0 or 1 renders `fractal` alone; 2 .. MBX_MAX_FRACTALS renders the hybrid of
+	 * `fractal` and hybridFractals[0 .. hybridCount - 2], in that order */
+	int32_t hybridCount;              /* fractal containers in the hybrid */
+	int32_t repeatFrom;               /* "repeat_from" (1-based) */
+	double linearDEOffset;            /* "linear_DE_offset" (linear DE finish of a hybrid) */
+	mbx_fractal hybridFractals[MBX_MAX_FRACTALS - 1];
 } mbx_params;
 
 /* Per-pixel output planes (device pointers; any may be NULL except where noted).

@@ -11,11 +11,19 @@ MBX_IFS_VECTOR_COUNT = 9
 MBX_MANDELBOX_FOLDS = 2
 MBX_MAX_AO_VECTORS = 64
 MBX_MAX_GRADIENT_STOPS = 64
+MBX_MAX_FRACTALS = 9
 
 MBX_FORMULA_NONE = 0
 MBX_FORMULA_MANDELBULB = 2
+MBX_FORMULA_MANDELBULB_POWER2 = 3
+MBX_FORMULA_HYPERCOMPLEX = 4
+MBX_FORMULA_QUATERNION = 5
+MBX_FORMULA_MENGER_SPONGE = 7
 MBX_FORMULA_MANDELBOX = 8
 MBX_FORMULA_KALEIDOSCOPIC_IFS = 10
+MBX_FORMULA_XENODREAMBUIE = 11
+MBX_FORMULA_BOX_FOLD_BULB_POW2 = 16
+MBX_FORMULA_AEXION = 19
 
 MBX_AO_FAST = 0
 MBX_AO_MULTIPLE_RAYS = 1
@@ -74,6 +82,20 @@ class MbxFractalIFS(C.Structure):
     ]
 
 
+class MbxFractalTransform(C.Structure):
+    _fields_ = [
+        ("scale3", C.c_double),
+        ("aexionCadd", C.c_double),
+        ("foldingIntPowFoldFactor", C.c_double),
+        ("foldingIntPowZFactor", C.c_double),
+        ("analyticDEScale1", C.c_double),
+        ("analyticDEOffset1", C.c_double),
+        ("analyticDEOffset2", C.c_double),
+        ("analyticDEEnabled", C.c_int32),
+        ("analyticDEEnabledFalse", C.c_int32),
+    ]
+
+
 class MbxFractal(C.Structure):
     _fields_ = [
         ("formula", C.c_int32),
@@ -83,14 +105,20 @@ class MbxFractal(C.Structure):
         ("useDefaultBailout", C.c_int32),
         ("juliaMode", C.c_int32),
         ("dontAddCConstant", C.c_int32),
-        ("reserved0", C.c_int32),
+        ("checkForBailout", C.c_int32),
         ("bailout", C.c_double),
         ("juliaC", D3),
         ("constantFactor", D3),
         ("initialWAxis", C.c_double),
+        ("formulaIterations", C.c_int32),
+        ("formulaStartIteration", C.c_int32),
+        ("formulaStopIteration", C.c_int32),
+        ("reserved0", C.c_int32),
+        ("formulaWeight", C.c_double),
         ("bulb", MbxFractalMandelbulb),
         ("mandelbox", MbxFractalMandelbox),
         ("IFS", MbxFractalIFS),
+        ("transform", MbxFractalTransform),
     ]
 
 
@@ -188,6 +216,10 @@ class MbxParams(C.Structure):
         ("glowIntensity", C.c_double),
         ("glowColor1", D3),
         ("glowColor2", D3),
+        ("hybridCount", C.c_int32),
+        ("repeatFrom", C.c_int32),
+        ("linearDEOffset", C.c_double),
+        ("hybridFractals", MbxFractal * (MBX_MAX_FRACTALS - 1)),
     ]
 
 

@@ -486,6 +486,57 @@ void rgb16(double *o, int r, int g, int b)
 
 } // namespace
 
+// One fractal container's defaults (InitFractalParams, initparameters.cpp:695-1010) with the
+// sequence-wide values of InitParams (:128-156) and the node defaults (:2102-2125)
+static void default_fractal(mbx_fractal &f, int formula)
+{
+	f.formula = formula;                   // :695 (container fractal0 Mandelbulb, the others none)
+	f.formulaMaxiter = 250;                // node formula_maxiter :2123, N :130
+	f.deltaDEMethod = 0;                   // :154
+	f.deltaDEFunction = 0;                 // :152
+	f.useDefaultBailout = 1;               // :155
+	f.bailout = 100.0;                     // :150
+	f.juliaMode = 0;                       // :128
+	set3(f.juliaC, 0.0, 0.0, 0.0);         // :129
+	set3(f.constantFactor, 1.0, 1.0, 1.0); // :132
+	f.initialWAxis = 0.0;                  // :156
+	f.formulaIterations = 1;               // :701
+	f.formulaWeight = 1.0;                 // :702
+	f.formulaStartIteration = 0;           // :703
+	f.formulaStopIteration = 250;          // :704
+	f.dontAddCConstant = 0;                // :713
+	f.checkForBailout = 1;                 // :714
+	f.bulb.power = 9.0;                    // :717
+	mbx_fractal_mandelbox &mb = f.mandelbox;
+	mb.scale = 2.0;                        // :746
+	mb.foldingLimit = 1.0;
+	mb.foldingValue = 2.0;
+	mb.foldingSphericalMin = 0.5;
+	mb.foldingSphericalFixed = 1.0;        // :750
+	set3(mb.colorFactor, 0.03, 0.05, 0.07); // :761
+	mb.colorFactorSp1 = 0.2;               // :765
+	mb.colorFactorSp2 = 0.2;
+	mb.colorFactorR = 0.0;                 // :764
+	mbx_fractal_ifs &ifs = f.IFS;
+	ifs.scale = 2.0;                       // :725
+	set3(ifs.offset, 1.0, 0.0, 0.0);       // :728
+	for (int i = 0; i < MBX_IFS_VECTOR_COUNT; i++)
+	{
+		set3(ifs.direction[i], 1.0, 0.0, 0.0); // :738-742
+		ifs.intensity[i] = 1.0;
+	}
+	mbx_fractal_transform &t = f.transform;
+	t.aexionCadd = -1.3;                   // :722
+	t.foldingIntPowFoldFactor = 2.0;       // :784
+	t.foldingIntPowZFactor = 5.0;          // :785
+	t.analyticDEEnabled = 1;               // :904
+	t.analyticDEEnabledFalse = 0;          // :905
+	t.analyticDEScale1 = 1.0;              // :906
+	t.analyticDEOffset1 = 1.0;             // :909
+	t.analyticDEOffset2 = 1.0;             // :910
+	t.scale3 = 3.0;                        // :1010
+}
+
 extern "C" void mbx_params_default(mbx_params *p)
 {
 	if (!p) return;
@@ -551,38 +602,13 @@ extern "C" void mbx_params_default(mbx_params *p)
 	L.softShadowCone = 1.0;
 	L.contourSharpness = 1.0;
 	rgb16(L.color, 65535, 65535, 65535);
-	// fractal (InitParams :128-156 for the sequence-wide values, InitFractalParams :695-768)
-	mbx_fractal &f = p->fractal;
-	f.formula = MBX_FORMULA_MANDELBULB;    // :695 (container fractal0)
-	f.formulaMaxiter = 250;                // node formula_maxiter :2123, N :130
-	f.deltaDEMethod = 0;                   // :154
-	f.deltaDEFunction = 0;                 // :152
-	f.useDefaultBailout = 1;               // :155
-	f.bailout = 100.0;                     // :150
-	f.juliaMode = 0;                       // :128
-	set3(f.juliaC, 0.0, 0.0, 0.0);         // :129
-	set3(f.constantFactor, 1.0, 1.0, 1.0); // :132
-	f.initialWAxis = 0.0;                  // :156
-	f.dontAddCConstant = 0;                // :712
-	f.bulb.power = 9.0;                    // :717
-	mbx_fractal_mandelbox &mb = f.mandelbox;
-	mb.scale = 2.0;                        // :746
-	mb.foldingLimit = 1.0;
-	mb.foldingValue = 2.0;
-	mb.foldingSphericalMin = 0.5;
-	mb.foldingSphericalFixed = 1.0;        // :750
-	set3(mb.colorFactor, 0.03, 0.05, 0.07); // :761
-	mb.colorFactorSp1 = 0.2;               // :765
-	mb.colorFactorSp2 = 0.2;
-	mb.colorFactorR = 0.0;                 // :764
-	mbx_fractal_ifs &ifs = f.IFS;
-	ifs.scale = 2.0;                       // :725
-	set3(ifs.offset, 1.0, 0.0, 0.0);       // :728
-	for (int i = 0; i < MBX_IFS_VECTOR_COUNT; i++)
-	{
-		set3(ifs.direction[i], 1.0, 0.0, 0.0); // :738-742
-		ifs.intensity[i] = 1.0;
-	}
+	// fractal containers; a single formula (hybrid_fractal_enable :149, repeat_from :151,
+	// linear_DE_offset :157)
+	default_fractal(p->fractal, MBX_FORMULA_MANDELBULB);
+	for (int k = 0; k < MBX_MAX_FRACTALS - 1; k++) default_fractal(p->hybridFractals[k], MBX_FORMULA_NONE);
+	p->hybridCount = 0;
+	p->repeatFrom = 1;
+	p->linearDEOffset = 0.0;
 	// palette colouring of material #1 (InitMaterialParams)
 	p->useColorsFromPalette = 1;
 	p->surfaceGradientEnable = 1;

@@ -18,6 +18,59 @@ def _u16(rgb16):
     return [float(np.float32(c / 65536.0)) for c in rgb16]
 
 
+def default_fractal(f, formula):
+    """One fractal container's defaults (InitFractalParams, initparameters.cpp:695-1010) with the
+    sequence-wide values (:128-156) and node defaults (:2102-2125)."""
+    f.formula = formula
+    f.formulaMaxiter = 250
+    f.deltaDEMethod = 0
+    f.deltaDEFunction = 0
+    f.useDefaultBailout = 1
+    f.juliaMode = 0
+    f.dontAddCConstant = 0
+    f.checkForBailout = 1                      # :714
+    f.bailout = 100.0
+    abi.set3(f.juliaC, (0.0, 0.0, 0.0))
+    abi.set3(f.constantFactor, (1.0, 1.0, 1.0))
+    f.initialWAxis = 0.0
+    f.formulaIterations = 1                    # :701-704
+    f.formulaWeight = 1.0
+    f.formulaStartIteration = 0
+    f.formulaStopIteration = 250
+    f.bulb.power = 9.0
+    f.bulb.alphaAngleOffset = 0.0
+    f.bulb.betaAngleOffset = 0.0
+    mb = f.mandelbox
+    mb.scale = 2.0
+    mb.foldingLimit = 1.0
+    mb.foldingValue = 2.0
+    mb.foldingSphericalMin = 0.5
+    mb.foldingSphericalFixed = 1.0
+    abi.set3(mb.colorFactor, (0.03, 0.05, 0.07))
+    mb.colorFactorSp1 = 0.2
+    mb.colorFactorSp2 = 0.2
+    mb.colorFactorR = 0.0                      # :764
+    ifs = f.IFS
+    ifs.scale = 2.0
+    abi.set3(ifs.offset, (1.0, 0.0, 0.0))
+    for i in range(abi.MBX_IFS_VECTOR_COUNT):
+        abi.set3(ifs.direction[i], (1.0, 0.0, 0.0))
+        ifs.distance[i] = 0.0
+        ifs.intensity[i] = 1.0
+        ifs.enabled[i] = 0
+    t = f.transform
+    t.scale3 = 3.0                             # :1010
+    t.aexionCadd = -1.3                        # :722
+    t.foldingIntPowFoldFactor = 2.0            # :784
+    t.foldingIntPowZFactor = 5.0               # :785
+    t.analyticDEEnabled = 1                    # :904-910
+    t.analyticDEEnabledFalse = 0
+    t.analyticDEScale1 = 1.0
+    t.analyticDEOffset1 = 1.0
+    t.analyticDEOffset2 = 1.0
+    return f
+
+
 def default_params(width=640, height=480):
     """Reference defaults (initparameters.cpp) for the fields of mbx_params."""
     p = abi.MbxParams()
@@ -82,40 +135,13 @@ def default_params(width=640, height=480):
     p.monteCarloGIRadianceLimit = 10.0
     p.rngMode = abi.MBX_RNG_PER_PIXEL
     p.randomSeed = 0
-    # fractal #1 (initparameters.cpp:713-766)
-    f = p.fractal
-    f.formula = abi.MBX_FORMULA_MANDELBULB
-    f.formulaMaxiter = 250
-    f.deltaDEMethod = 0
-    f.deltaDEFunction = 0
-    f.useDefaultBailout = 1
-    f.juliaMode = 0
-    f.dontAddCConstant = 0
-    f.bailout = 100.0
-    abi.set3(f.juliaC, (0.0, 0.0, 0.0))
-    abi.set3(f.constantFactor, (1.0, 1.0, 1.0))
-    f.initialWAxis = 0.0
-    f.bulb.power = 9.0
-    f.bulb.alphaAngleOffset = 0.0
-    f.bulb.betaAngleOffset = 0.0
-    mb = f.mandelbox
-    mb.scale = 2.0
-    mb.foldingLimit = 1.0
-    mb.foldingValue = 2.0
-    mb.foldingSphericalMin = 0.5
-    mb.foldingSphericalFixed = 1.0
-    abi.set3(mb.colorFactor, (0.03, 0.05, 0.07))
-    mb.colorFactorSp1 = 0.2
-    mb.colorFactorSp2 = 0.2
-    mb.colorFactorR = 0.0                      # :764
-    ifs = f.IFS
-    ifs.scale = 2.0
-    abi.set3(ifs.offset, (1.0, 0.0, 0.0))
-    for i in range(abi.MBX_IFS_VECTOR_COUNT):
-        abi.set3(ifs.direction[i], (1.0, 0.0, 0.0))
-        ifs.distance[i] = 0.0
-        ifs.intensity[i] = 1.0
-        ifs.enabled[i] = 0
+    # fractal containers: #1 Mandelbulb, the others none; a single formula (initparameters.cpp:149-157)
+    default_fractal(p.fractal, abi.MBX_FORMULA_MANDELBULB)
+    for k in range(abi.MBX_MAX_FRACTALS - 1):
+        default_fractal(p.hybridFractals[k], abi.MBX_FORMULA_NONE)
+    p.hybridCount = 0
+    p.repeatFrom = 1
+    p.linearDEOffset = 0.0
     # palette colouring of material #1 (InitMaterialParams, initparameters.cpp:1600-1810)
     p.useColorsFromPalette = 1
     p.surfaceGradientEnable = 1
@@ -177,10 +203,11 @@ def plain_material(p):
 def reference_material(p):
     """Material #1 and glow as the reference initialises them (palette colouring from the default
     surface gradient, plastic and metallic specular highlights, glow on), restored onto a scene
-    made with plain_material.  These fields close the parameter block."""
+    made with plain_material: the fields from useColorsFromPalette to glowColor2."""
     d = default_params(p.imageWidth, p.imageHeight)
     off = abi.MbxParams.useColorsFromPalette.offset
-    ctypes.memmove(ctypes.addressof(p) + off, ctypes.addressof(d) + off, ctypes.sizeof(abi.MbxParams) - off)
+    end = abi.MbxParams.glowColor2.offset + abi.MbxParams.glowColor2.size
+    ctypes.memmove(ctypes.addressof(p) + off, ctypes.addressof(d) + off, end - off)
     return p
 
 

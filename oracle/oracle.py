@@ -47,6 +47,8 @@ def lib(fast=False):
                                    C.c_void_p]
     L.mbxo_diff_de.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                C.c_void_p, C.c_void_p]
+    L.mbxo_formula_step_ex.argtypes = [P(abi.MbxParams), C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_void_p]
     L.mbxo_colour_index.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_int, C.c_void_p]
     L.mbxo_gradient.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_int, C.c_void_p]
     L.mbxo_surface_colour.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_int, C.c_void_p]
@@ -124,6 +126,19 @@ def formula_step(p, z4, aux3):
     if rc != 0:
         raise RuntimeError("oracle formula_step failed: %d" % rc)
     return z4, aux3
+
+
+def formula_step_ex(p, z4, aux3, iters=None, const_c=None, slot=0):
+    """One FormulaCode step of container `slot` with aux.i and aux.const_c; returns
+    (z4, aux3, const_c)."""
+    z4 = np.array(z4, np.float64).reshape(-1, 4).copy()
+    aux3 = np.array(aux3, np.float64).reshape(-1, 3).copy()
+    it = None if iters is None else np.ascontiguousarray(iters, np.int32)
+    cc = None if const_c is None else np.array(const_c, np.float64).reshape(-1, 4).copy()
+    rc = lib().mbxo_formula_step_ex(C.byref(p), slot, len(z4), _ptr(z4), _ptr(aux3), _ptr(it), _ptr(cc))
+    if rc != 0:
+        raise RuntimeError("oracle formula_step_ex failed: %d" % rc)
+    return z4, aux3, cc
 
 
 def colour_index(p, pts):

@@ -5,6 +5,8 @@
 //   mandelbulber2/formula/definition/fractal_mandelbulb.cpp
 //   mandelbulber2/formula/definition/fractal_mandelbox.cpp
 //   mandelbulber2/formula/definition/fractal_kaleidoscopic_ifs.cpp
+//   mandelbulber2/formula/definition/fractal_{mandelbulb_power2,hypercomplex,quaternion,
+//     menger_sponge,xenodreambuie,box_fold_bulb_pow2,aexion}.cpp
 //   mandelbulber2/src/common_math.cpp
 // compiled where they lie under the reference checkout.  abstract_fractal.cpp and algebra.cpp
 // need Qt and are not built: the link leaves their symbols (the cAbstractFractal base
@@ -174,6 +176,113 @@ void gen_kifs()
 	}
 }
 
+// 4D orbit points for the formulas that use w
+CVector4 point4(double r, bool withW)
+{
+	return CVector4(uniform(-r, r), uniform(-r, r), uniform(-r, r), withW ? uniform(-r, r) : 0.0);
+}
+
+// The parameter-free power-2 formulas and the Menger sponge: params are the formula's own
+// (Menger: transf_scale_3)
+template <class F>
+void gen_simple(const char *tag, const double *params, int np, bool withW, double range)
+{
+	sFractal *f = fresh_fractal();
+	if (np > 0) f->transformCommon.scale3 = params[0];
+	for (int k = 0; k < 48; k++)
+	{
+		CVector4 z = point4(range, withW);
+		if (k == 0) z = CVector4(0.0, 0.0, 0.7, withW ? 0.3 : 0.0); // on the z axis
+		sExtendedAux aux;
+		fresh_aux(aux, z);
+		CVector4 zin = z;
+		sExtendedAux ain = aux;
+		call_formula<F>(z, f, aux);
+		print_record(tag, params, np, zin, ain, z, aux);
+	}
+}
+
+// Xenodreambuie params: power, alpha (deg), beta (deg)
+void gen_xenodreambuie()
+{
+	const double cases[][3] = {{8.0, 0.0, 0.0}, {3.0, 0.0, 0.0}, {5.5, 25.0, -40.0}};
+	for (const auto &c : cases)
+	{
+		sFractal *f = fresh_fractal();
+		f->bulb.power = c[0];
+		f->bulb.alphaAngleOffset = c[1] * M_PI_180;
+		f->bulb.betaAngleOffset = c[2] * M_PI_180;
+		for (int k = 0; k < 48; k++)
+		{
+			CVector4 z = point4(1.4, false);
+			if (k == 0) z = CVector4(0.0, 0.0, -0.7, 0.0);
+			sExtendedAux aux;
+			fresh_aux(aux, z);
+			CVector4 zin = z;
+			sExtendedAux ain = aux;
+			call_formula<cFractalXenodreambuie>(z, f, aux);
+			print_record("xenodreambuie", c, 3, zin, ain, z, aux);
+		}
+	}
+}
+
+// Box fold bulb pow 2 params: foldFactor, zFactor, analyticDE enabledFalse, scale1, offset1, offset2
+void gen_box_fold_bulb_pow2()
+{
+	const double cases[][6] = {{2.0, 5.0, 0, 1.0, 1.0, 1.0}, {1.0, 3.0, 1, 1.2, 0.5, 0.7}};
+	for (const auto &c : cases)
+	{
+		sFractal *f = fresh_fractal();
+		f->foldingIntPow.foldFactor = c[0];
+		f->foldingIntPow.zFactor = c[1];
+		f->analyticDE.enabledFalse = c[2] != 0.0;
+		f->analyticDE.scale1 = c[3];
+		f->analyticDE.offset1 = c[4];
+		f->analyticDE.offset2 = c[5];
+		for (int k = 0; k < 48; k++)
+		{
+			CVector4 z = point4(3.0, true);
+			sExtendedAux aux;
+			fresh_aux(aux, z);
+			CVector4 zin = z;
+			sExtendedAux ain = aux;
+			call_formula<cFractalBoxFoldBulbPow2>(z, f, aux);
+			print_record("boxfoldbulbpow2", c, 6, zin, ain, z, aux);
+		}
+	}
+}
+
+// Aexion params: cadd, iteration index, analyticDE enabled, scale1, offset1, then the input
+// const_c (xyzw); the record ends with the output const_c
+void gen_aexion()
+{
+	const double cases[][5] = {{-1.3, 0, 1, 1.0, 1.0}, {-1.3, 3, 1, 1.0, 1.0}, {0.4, 0, 0, 1.0, 1.0},
+		{-0.7, 5, 1, 0.8, 0.3}};
+	for (const auto &c : cases)
+	{
+		sFractal *f = fresh_fractal();
+		f->aexion.cadd = c[0];
+		f->analyticDE.enabled = c[2] != 0.0;
+		f->analyticDE.scale1 = c[3];
+		f->analyticDE.offset1 = c[4];
+		for (int k = 0; k < 48; k++)
+		{
+			CVector4 z = point4(1.5, true);
+			sExtendedAux aux;
+			fresh_aux(aux, z);
+			aux.i = (int)c[1];
+			aux.const_c = point4(1.5, true);
+			double params[9] = {c[0], c[1], c[2], c[3], c[4], aux.const_c.x, aux.const_c.y, aux.const_c.z,
+				aux.const_c.w};
+			CVector4 zin = z;
+			sExtendedAux ain = aux;
+			call_formula<cFractalAexion>(z, f, aux);
+			print_record("aexion", params, 9, zin, ain, z, aux);
+			std::printf("aexion_c %a %a %a %a\n", aux.const_c.x, aux.const_c.y, aux.const_c.z, aux.const_c.w);
+		}
+	}
+}
+
 void gen_random()
 {
 	gRandomSeed = 1;
@@ -195,6 +304,16 @@ int main()
 	gen_mandelbulb();
 	gen_mandelbox();
 	gen_kifs();
+	const double none[1] = {0.0};
+	gen_simple<cFractalMandelbulbPower2>("mandelbulbpower2", none, 0, false, 1.4);
+	gen_simple<cFractalHypercomplex>("hypercomplex", none, 0, true, 1.4);
+	gen_simple<cFractalQuaternion>("quaternion", none, 0, true, 1.4);
+	const double scales[2][1] = {{3.0}, {2.6}};
+	for (const auto &sc : scales)
+		gen_simple<cFractalMengerSponge>("mengersponge", sc, 1, true, 1.6);
+	gen_xenodreambuie();
+	gen_box_fold_bulb_pow2();
+	gen_aexion();
 	gen_random();
 	return 0;
 }

@@ -208,28 +208,14 @@ uint32_t mbxo_pixel_seed(uint32_t pixel_index, uint32_t frame_seed)
 /* ------------------------------------------------------------------------------------------ */
 /* scene: sParamRender + cHybridFractalSequences::sSequence subset, derived on the host        */
 /* ------------------------------------------------------------------------------------------ */
+/* One fractal container of the sequence: cHybridFractalSequences::sFractalData
+ * (hybrid_fractal_sequences.h:30-56) with its sFractal after RecalculateFractalParams
+ * (fractal.cpp:653-678). */
 typedef struct
 {
-	const mbx_params *p;
-	/* camera (render_worker.cpp:539-564, fractparams.cpp:145, render_job.cpp:477) */
-	rotm mRot;
-	v3 camera;
-	double fov;        /* CalcFOV: 2 tan(deg/360 pi) */
-	double resolution; /* 1 / imageHeight */
-	double aspectRatio;
-	/* sequence (hybrid_fractal_sequences.cpp:181-287, 377-503) */
 	int formula;
-	int formulaMaxiter;
-	int DEType;
-	int DEFunctionType;
-	int DEAnalyticFunction;
-	int addCConstant;
-	int juliaEnabled;
-	double bailout;
-	v3 juliaConstant;
-	v3 constantMultiplier;
-	double initialWAxis;
-	/* sFractal after RecalculateFractalParams (fractal.cpp:653-678) */
+	int addC, checkForBailout, iterations, startIteration, stopIteration;
+	double weight, bailout;
 	double bulbPower, bulbAlpha, bulbBeta;
 	double mbScale, mbFoldingLimit, mbFoldingValue, mbFR2, mbMR2, mbFactor1, mbSp1, mbSp2;
 	v4 mbOffset;
@@ -244,6 +230,39 @@ typedef struct
 	double ifsDistance[MBX_IFS_VECTOR_COUNT], ifsIntensity[MBX_IFS_VECTOR_COUNT];
 	int ifsEnabled[MBX_IFS_VECTOR_COUNT];
 	int ifsAbsX, ifsAbsY, ifsAbsZ, ifsMenger, ifsRotationEnabled, ifsEdgeEnabled;
+	double scale3, aexionCadd, fipFoldFactor, fipZFactor, adeScale1, adeOffset1, adeOffset2;
+	int adeEnabled, adeEnabledFalse;
+} slot_t;
+
+/* sequence length: maxN * 5 with maxN = 250 (hybrid_fractal_sequences.cpp:294-298) */
+#define SEQ_LENGTH 1250
+
+typedef struct
+{
+	const mbx_params *p;
+	/* camera (render_worker.cpp:539-564, fractparams.cpp:145, render_job.cpp:477) */
+	rotm mRot;
+	v3 camera;
+	double fov;        /* CalcFOV: 2 tan(deg/360 pi) */
+	double resolution; /* 1 / imageHeight */
+	double aspectRatio;
+	/* sequence (cHybridFractalSequences::sSequence, hybrid_fractal_sequences.h:58-103) */
+	int formula; /* the first container's formula */
+	int formulaMaxiter;
+	int DEType;
+	int DEFunctionType;
+	int DEAnalyticFunction;
+	int coloringFunction;
+	int isHybrid;
+	int juliaEnabled;
+	v3 juliaConstant;
+	v3 constantMultiplier;
+	double initialWAxis;
+	double linearDEOffset;
+	int nSlots;
+	int seqLength;
+	unsigned char seq[SEQ_LENGTH];
+	slot_t slot[MBX_MAX_FRACTALS];
 	/* light #1 (light.cpp:51-195) */
 	v3 lightDirection;
 	double lightIntensity, lightVisibility, lightSize, lightSoftShadowCone, lightContourSharpness;
@@ -289,9 +308,266 @@ static camtarget_t camera_target_top(v3 camera, v3 target, v3 top)
 
 static v3 v3_from(const double a[3]) { return V3(a[0], a[1], a[2]); }
 
+/* cAbstractFractal metadata of the formulas on the path (the constructors of
+ * formula/definition/fractal_<name>.cpp) */
+typedef struct
+{
+	int id, DEType, DEFunctionType, cpixel, DEAnalyticFunction, coloringFunction;
+	double defaultBailout;
+} formula_meta_t;
+enum { CPIXEL_ENABLED = 0, CPIXEL_DISABLED = 1, CPIXEL_ALREADY_HAS = 2 };
+enum { COLORING_DEFAULT = 0, COLORING_ABOX = 1, COLORING_IFS = 2 };
+enum { DE_FUNCTION_WITHOUT = 99, DE_FUNCTION_CUSTOM = 5, DE_FUNCTION_COUNT = 6 };
+
+static const formula_meta_t kFormulas[] = {
+	{MBX_FORMULA_MANDELBULB, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LOGARITHMIC, CPIXEL_ENABLED, MBX_ANALYTIC_LOGARITHMIC, COLORING_DEFAULT, 10.0},
+	{MBX_FORMULA_MANDELBULB_POWER2, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LOGARITHMIC, CPIXEL_ENABLED, MBX_ANALYTIC_LOGARITHMIC, COLORING_DEFAULT, 10.0},
+	{MBX_FORMULA_HYPERCOMPLEX, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LOGARITHMIC, CPIXEL_ENABLED, MBX_ANALYTIC_LOGARITHMIC, COLORING_DEFAULT, 10.0},
+	{MBX_FORMULA_QUATERNION, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LOGARITHMIC, CPIXEL_ENABLED, MBX_ANALYTIC_LOGARITHMIC, COLORING_DEFAULT, 10.0},
+	{MBX_FORMULA_MENGER_SPONGE, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LINEAR, CPIXEL_DISABLED, MBX_ANALYTIC_IFS, COLORING_IFS, 10.0},
+	{MBX_FORMULA_MANDELBOX, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LINEAR, CPIXEL_ENABLED, MBX_ANALYTIC_LINEAR, COLORING_ABOX, 100.0},
+	{MBX_FORMULA_KALEIDOSCOPIC_IFS, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LINEAR, CPIXEL_DISABLED, MBX_ANALYTIC_IFS, COLORING_IFS, 100.0},
+	{MBX_FORMULA_XENODREAMBUIE, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LOGARITHMIC, CPIXEL_ENABLED, MBX_ANALYTIC_LOGARITHMIC, COLORING_DEFAULT, 10.0},
+	{MBX_FORMULA_BOX_FOLD_BULB_POW2, MBX_DE_DELTA, MBX_DE_FUNCTION_LINEAR, CPIXEL_ENABLED, MBX_ANALYTIC_LINEAR, COLORING_DEFAULT, 10000.0},
+	{MBX_FORMULA_AEXION, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LOGARITHMIC, CPIXEL_ALREADY_HAS, MBX_ANALYTIC_LOGARITHMIC, COLORING_DEFAULT, 10000.0},
+};
+
+static const formula_meta_t *formula_meta(int id)
+{
+	for (size_t k = 0; k < sizeof(kFormulas) / sizeof(kFormulas[0]); k++)
+		if (kFormulas[k].id == id) return &kFormulas[k];
+	return NULL;
+}
+
+/* sFractal of one container after RecalculateFractalParams (fractal.cpp:41-120, 653-678) */
+static void slot_init(slot_t *sl, const mbx_fractal *f)
+{
+	sl->formula = f->formula;
+	sl->iterations = f->formulaIterations;
+	sl->startIteration = f->formulaStartIteration;
+	sl->stopIteration = f->formulaStopIteration;
+	sl->weight = f->formulaWeight;
+	sl->checkForBailout = f->checkForBailout;
+	sl->bulbPower = f->bulb.power;
+	sl->bulbAlpha = f->bulb.alphaAngleOffset * M_PI_180;
+	sl->bulbBeta = f->bulb.betaAngleOffset * M_PI_180;
+	const mbx_fractal_mandelbox *mb = &f->mandelbox;
+	sl->mbScale = mb->scale;
+	sl->mbFoldingLimit = mb->foldingLimit;
+	sl->mbFoldingValue = mb->foldingValue;
+	sl->mbFR2 = mb->foldingSphericalFixed * mb->foldingSphericalFixed;
+	sl->mbMR2 = mb->foldingSphericalMin * mb->foldingSphericalMin;
+	sl->mbFactor1 = sl->mbFR2 / sl->mbMR2;
+	sl->mbOffset = V4(mb->offset[0], mb->offset[1], mb->offset[2], 0.0);
+	sl->mbColorFactor = v3_from(mb->colorFactor);
+	sl->mbSp1 = mb->colorFactorSp1;
+	sl->mbSp2 = mb->colorFactorSp2;
+	sl->mbRotationsEnabled = mb->rotationsEnabled;
+	sl->mbMainRotationEnabled = mb->mainRotationEnabled;
+	rot_set2(&sl->mbMainRot, v3_mul(v3_from(mb->rotationMain), M_PI_180));
+	for (int fold = 0; fold < MBX_MANDELBOX_FOLDS; fold++)
+		for (int axis = 0; axis < 3; axis++)
+		{
+			rot_set2(&sl->mbRot[fold][axis], v3_mul(v3_from(mb->rotation[fold][axis]), M_PI_180));
+			sl->mbRotInv[fold][axis] = rot_transpose(&sl->mbRot[fold][axis]);
+		}
+	const mbx_fractal_ifs *ifs = &f->IFS;
+	sl->ifsScale = ifs->scale;
+	sl->ifsOffset = V4(ifs->offset[0], ifs->offset[1], ifs->offset[2], 0.0);
+	sl->ifsEdge = v3_from(ifs->edge);
+	rot_set3(&sl->ifsMainRot, v3_mul(v3_from(ifs->rotation), M_PI_180));
+	for (int i = 0; i < MBX_IFS_VECTOR_COUNT; i++)
+	{
+		rot_set3(&sl->ifsRot[i], v3_mul(v3_from(ifs->rotations[i]), M_PI_180));
+		/* CVector4::Normalize includes w (= 0) */
+		v4 d = V4(ifs->direction[i][0], ifs->direction[i][1], ifs->direction[i][2], 0.0);
+		double norm = 1.0 / v4_len(d);
+		sl->ifsDirection[i] = V4(d.x * norm, d.y * norm, d.z * norm, d.w * norm);
+		sl->ifsDistance[i] = ifs->distance[i];
+		sl->ifsIntensity[i] = ifs->intensity[i];
+		sl->ifsEnabled[i] = ifs->enabled[i];
+	}
+	sl->ifsAbsX = ifs->absX;
+	sl->ifsAbsY = ifs->absY;
+	sl->ifsAbsZ = ifs->absZ;
+	sl->ifsMenger = ifs->mengerSpongeMode;
+	sl->ifsRotationEnabled = ifs->rotationEnabled;
+	sl->ifsEdgeEnabled = ifs->edgeEnabled;
+	const mbx_fractal_transform *t = &f->transform;
+	sl->scale3 = t->scale3;
+	sl->aexionCadd = t->aexionCadd;
+	sl->fipFoldFactor = t->foldingIntPowFoldFactor;
+	sl->fipZFactor = t->foldingIntPowZFactor;
+	sl->adeScale1 = t->analyticDEScale1;
+	sl->adeOffset1 = t->analyticDEOffset1;
+	sl->adeOffset2 = t->analyticDEOffset2;
+	sl->adeEnabled = t->analyticDEEnabled;
+	sl->adeEnabledFalse = t->analyticDEEnabledFalse;
+}
+
+/* cHybridFractalSequences::CollectSequenceData and CreateSequence for the rendered node
+ * (hybrid_fractal_sequences.cpp:181-287, 290-503): one container, or the hybrid of
+ * p->fractal and p->hybridFractals[0 .. hybridCount - 2] */
+static int sequence_init(scene_t *s, const mbx_params *p)
+{
+	const mbx_fractal *node = &p->fractal;
+	const int isHybrid = p->hybridCount >= 2;
+	const int n = isHybrid ? p->hybridCount : 1;
+	if (n > MBX_MAX_FRACTALS) return MBX_ERR_INVALID;
+	const mbx_fractal *c[MBX_MAX_FRACTALS];
+	c[0] = &p->fractal;
+	for (int k = 1; k < n; k++) c[k] = &p->hybridFractals[k - 1];
+	s->isHybrid = isHybrid;
+	s->nSlots = n;
+	s->formula = c[0]->formula;
+	s->formulaMaxiter = node->formulaMaxiter;
+	s->juliaEnabled = node->juliaMode;
+	s->juliaConstant = v3_from(node->juliaC);
+	s->constantMultiplier = v3_from(node->constantFactor);
+	s->initialWAxis = node->initialWAxis;
+	s->linearDEOffset = p->linearDEOffset;
+
+	/* CollectSequenceData */
+	double maxBailout = 0.0;
+	for (int k = 0; k < n; k++)
+	{
+		slot_t *sl = &s->slot[k];
+		slot_init(sl, c[k]);
+		if (c[k]->formula == MBX_FORMULA_NONE) continue;
+		const formula_meta_t *m = formula_meta(c[k]->formula);
+		if (!m) return MBX_ERR_UNSUPPORTED;
+		if (!isHybrid) sl->checkForBailout = 1;
+		int addc;
+		if (m->cpixel == CPIXEL_ALREADY_HAS)
+			addc = 0;
+		else
+		{
+			addc = !c[k]->dontAddCConstant;
+			if (m->cpixel == CPIXEL_DISABLED) addc = !addc;
+		}
+		sl->addC = addc;
+		if (node->useDefaultBailout)
+		{
+			if (isHybrid)
+				maxBailout = fmax(maxBailout, m->defaultBailout);
+			else
+				sl->bailout = m->defaultBailout;
+		}
+		else
+			sl->bailout = node->bailout;
+	}
+	if (isHybrid && node->useDefaultBailout)
+		for (int k = 0; k < n; k++) s->slot[k].bailout = maxBailout;
+
+	/* CreateSequence: the formula of each iteration */
+	const int repeatFromIndex = (int)fmax(0.0, fmin((double)(p->repeatFrom - 1), (double)(n - 1)));
+	int fractalNo = 0, counter = 0, rapidEnd = 0, last = 0;
+	s->seqLength = SEQ_LENGTH;
+	for (int i = 0; i < SEQ_LENGTH; i++)
+	{
+		counter++;
+		int search = 0;
+		while ((c[fractalNo]->formula == MBX_FORMULA_NONE || i < s->slot[fractalNo].startIteration ||
+					 i > s->slot[fractalNo].stopIteration) && search < n)
+		{
+			fractalNo++;
+			if (fractalNo >= n) fractalNo = repeatFromIndex;
+			search++;
+		}
+		if (search >= n)
+		{
+			rapidEnd = 1;
+			break;
+		}
+		s->seq[i] = (unsigned char)fractalNo;
+		last = i;
+		if (counter >= s->slot[fractalNo].iterations)
+		{
+			counter = 0;
+			fractalNo++;
+			if (fractalNo >= n) fractalNo = repeatFromIndex;
+		}
+	}
+	if (rapidEnd) s->seqLength = last + 1;
+
+	/* DE type and function */
+	int forceDelta = node->deltaDEMethod == 1, forceAnalytic = node->deltaDEMethod == 2;
+	if (isHybrid)
+	{
+		s->DEType = MBX_DE_ANALYTIC;
+		if (node->deltaDEFunction == 0)
+		{
+			int count[DE_FUNCTION_COUNT + 1];
+			for (int i = 1; i <= DE_FUNCTION_COUNT; i++) count[i] = 0;
+			for (int k = 0; k < n; k++)
+			{
+				if (c[k]->formula == MBX_FORMULA_NONE) continue; /* the none formula: no DE function */
+				const formula_meta_t *m = formula_meta(c[k]->formula);
+				count[m->DEFunctionType] += s->slot[k].iterations;
+				if (!forceDelta && !forceAnalytic && m->DEType == MBX_DE_DELTA)
+				{
+					s->DEType = MBX_DE_DELTA;
+					forceDelta = 1;
+				}
+			}
+			if (count[DE_FUNCTION_CUSTOM] > 0)
+				s->DEFunctionType = DE_FUNCTION_CUSTOM;
+			else
+			{
+				int maxCount = -1;
+				for (int i = 1; i <= DE_FUNCTION_COUNT; i++)
+					if (count[i] > maxCount)
+					{
+						maxCount = count[i];
+						s->DEFunctionType = i;
+					}
+			}
+		}
+		else
+		{
+			s->DEFunctionType = node->deltaDEFunction;
+			for (int k = 0; k < n; k++)
+			{
+				const formula_meta_t *m = formula_meta(c[k]->formula);
+				if (m && m->DEType == MBX_DE_DELTA)
+				{
+					s->DEType = MBX_DE_DELTA;
+					break;
+				}
+			}
+		}
+		if (forceDelta) s->DEType = MBX_DE_DELTA;
+		if (forceAnalytic) s->DEType = MBX_DE_ANALYTIC;
+		s->coloringFunction = COLORING_DEFAULT;
+	}
+	else
+	{
+		const formula_meta_t *m = formula_meta(c[0]->formula);
+		if (!m) return MBX_ERR_UNSUPPORTED;
+		s->DEType = m->DEType;
+		s->DEFunctionType = m->DEFunctionType;
+		s->DEAnalyticFunction = m->DEAnalyticFunction;
+		s->coloringFunction = m->coloringFunction;
+		if (forceDelta) s->DEType = MBX_DE_DELTA;
+		if (forceAnalytic) s->DEType = MBX_DE_ANALYTIC;
+		if (node->deltaDEFunction != 0)
+		{
+			s->DEFunctionType = node->deltaDEFunction;
+			s->DEAnalyticFunction = node->deltaDEFunction == MBX_DE_FUNCTION_LOGARITHMIC ? MBX_ANALYTIC_LOGARITHMIC
+																							  : MBX_ANALYTIC_LINEAR;
+		}
+	}
+	if (s->DEFunctionType != MBX_DE_FUNCTION_LINEAR && s->DEFunctionType != MBX_DE_FUNCTION_LOGARITHMIC)
+		return MBX_ERR_UNSUPPORTED; /* pseudo-Kleinian, Jos-Kleinian, custom and max-axis finishes */
+	return MBX_OK;
+}
+
+static inline int seq_slot(const scene_t *s, int i) { return i < s->seqLength ? s->seq[i] : s->seq[0]; }
+
 /* Derive the per-frame constants.  Returns 0 on success, <0 on an unsupported setting. */
 static int scene_init(scene_t *s, const mbx_params *p)
 {
+	int rc;
 	memset(s, 0, sizeof(*s));
 	s->p = p;
 	if (p->imageWidth <= 0 || p->imageHeight <= 0) return MBX_ERR_INVALID;
@@ -310,104 +586,8 @@ static int scene_init(scene_t *s, const mbx_params *p)
 	s->resolution = 1.0 / p->imageHeight;      /* render_job.cpp:477 */
 	s->aspectRatio = (double)p->imageWidth / p->imageHeight;
 
-	/* sequence data for one formula (hybrid_fractal_sequences.cpp:181-287, 471-503) */
-	const mbx_fractal *f = &p->fractal;
-	s->formula = f->formula;
-	s->formulaMaxiter = f->formulaMaxiter;
-	double defaultBailout;
-	int cpixelDisabled = 0;
-	switch (f->formula)
-	{
-		case MBX_FORMULA_MANDELBULB: /* fractal_mandelbulb.cpp:16-27 */
-			s->DEType = MBX_DE_ANALYTIC;
-			s->DEFunctionType = MBX_DE_FUNCTION_LOGARITHMIC;
-			s->DEAnalyticFunction = MBX_ANALYTIC_LOGARITHMIC;
-			defaultBailout = 10.0;
-			break;
-		case MBX_FORMULA_MANDELBOX: /* fractal_mandelbox.cpp:17-29 */
-			s->DEType = MBX_DE_ANALYTIC;
-			s->DEFunctionType = MBX_DE_FUNCTION_LINEAR;
-			s->DEAnalyticFunction = MBX_ANALYTIC_LINEAR;
-			defaultBailout = 100.0;
-			break;
-		case MBX_FORMULA_KALEIDOSCOPIC_IFS: /* fractal_kaleidoscopic_ifs.cpp:17-28 */
-			s->DEType = MBX_DE_ANALYTIC;
-			s->DEFunctionType = MBX_DE_FUNCTION_LINEAR;
-			s->DEAnalyticFunction = MBX_ANALYTIC_IFS;
-			defaultBailout = 100.0;
-			cpixelDisabled = 1;
-			break;
-		default: return MBX_ERR_UNSUPPORTED;
-	}
-	if (f->deltaDEMethod == 1) s->DEType = MBX_DE_DELTA;
-	if (f->deltaDEMethod == 2) s->DEType = MBX_DE_ANALYTIC;
-	if (f->deltaDEFunction != 0)
-	{
-		s->DEFunctionType = f->deltaDEFunction;
-		if (f->deltaDEFunction == MBX_DE_FUNCTION_LOGARITHMIC)
-			s->DEAnalyticFunction = MBX_ANALYTIC_LOGARITHMIC;
-		else if (f->deltaDEFunction == MBX_DE_FUNCTION_LINEAR)
-			s->DEAnalyticFunction = MBX_ANALYTIC_LINEAR;
-		else
-			return MBX_ERR_UNSUPPORTED;
-	}
-	int addc = !f->dontAddCConstant;
-	if (cpixelDisabled) addc = !addc;
-	s->addCConstant = addc;
-	s->bailout = f->useDefaultBailout ? defaultBailout : f->bailout;
-	s->juliaEnabled = f->juliaMode;
-	s->juliaConstant = v3_from(f->juliaC);
-	s->constantMultiplier = v3_from(f->constantFactor);
-	s->initialWAxis = f->initialWAxis;
-
-	/* RecalculateFractalParams (fractal.cpp:653-678) */
-	s->bulbPower = f->bulb.power;
-	s->bulbAlpha = f->bulb.alphaAngleOffset * M_PI_180;
-	s->bulbBeta = f->bulb.betaAngleOffset * M_PI_180;
-
-	const mbx_fractal_mandelbox *mb = &f->mandelbox;
-	s->mbScale = mb->scale;
-	s->mbFoldingLimit = mb->foldingLimit;
-	s->mbFoldingValue = mb->foldingValue;
-	s->mbFR2 = mb->foldingSphericalFixed * mb->foldingSphericalFixed;
-	s->mbMR2 = mb->foldingSphericalMin * mb->foldingSphericalMin;
-	s->mbFactor1 = s->mbFR2 / s->mbMR2;
-	s->mbOffset = V4(mb->offset[0], mb->offset[1], mb->offset[2], 0.0);
-	s->mbColorFactor = v3_from(mb->colorFactor);
-	s->mbSp1 = mb->colorFactorSp1;
-	s->mbSp2 = mb->colorFactorSp2;
-	s->mbRotationsEnabled = mb->rotationsEnabled;
-	s->mbMainRotationEnabled = mb->mainRotationEnabled;
-	rot_set2(&s->mbMainRot, v3_mul(v3_from(mb->rotationMain), M_PI_180));
-	for (int fold = 0; fold < MBX_MANDELBOX_FOLDS; fold++)
-		for (int axis = 0; axis < 3; axis++)
-		{
-			rot_set2(&s->mbRot[fold][axis], v3_mul(v3_from(mb->rotation[fold][axis]), M_PI_180));
-			s->mbRotInv[fold][axis] = rot_transpose(&s->mbRot[fold][axis]);
-		}
-
-	const mbx_fractal_ifs *ifs = &f->IFS;
-	s->ifsScale = ifs->scale;
-	s->ifsOffset = V4(ifs->offset[0], ifs->offset[1], ifs->offset[2], 0.0);
-	s->ifsEdge = v3_from(ifs->edge);
-	rot_set3(&s->ifsMainRot, v3_mul(v3_from(ifs->rotation), M_PI_180));
-	for (int i = 0; i < MBX_IFS_VECTOR_COUNT; i++)
-	{
-		rot_set3(&s->ifsRot[i], v3_mul(v3_from(ifs->rotations[i]), M_PI_180));
-		/* CVector4::Normalize includes w (= 0) */
-		v4 d = V4(ifs->direction[i][0], ifs->direction[i][1], ifs->direction[i][2], 0.0);
-		double norm = 1.0 / v4_len(d);
-		s->ifsDirection[i] = V4(d.x * norm, d.y * norm, d.z * norm, d.w * norm);
-		s->ifsDistance[i] = ifs->distance[i];
-		s->ifsIntensity[i] = ifs->intensity[i];
-		s->ifsEnabled[i] = ifs->enabled[i];
-	}
-	s->ifsAbsX = ifs->absX;
-	s->ifsAbsY = ifs->absY;
-	s->ifsAbsZ = ifs->absZ;
-	s->ifsMenger = ifs->mengerSpongeMode;
-	s->ifsRotationEnabled = ifs->rotationEnabled;
-	s->ifsEdgeEnabled = ifs->edgeEnabled;
+	rc = sequence_init(s, p);
+	if (rc != MBX_OK) return rc;
 
 	/* cLight::setParameters for a directional light (light.cpp:80-195) */
 	const mbx_light *L = &p->light;
@@ -467,7 +647,7 @@ typedef struct
 } aux_t; /* sExtendedAux (mandelbulber2/src/fractal.h:77-104) */
 
 /* cFractalMandelbulb::FormulaCode (formula/definition/fractal_mandelbulb.cpp:29-43) */
-static void mandelbulb_iteration(const scene_t *s, v4 *z, aux_t *aux)
+static void mandelbulb_iteration(const slot_t *s, v4 *z, aux_t *aux)
 {
 	const double th0 = asin(z->z / aux->r) + s->bulbBeta;
 	const double ph0 = atan2(z->y, z->x) + s->bulbAlpha;
@@ -485,7 +665,7 @@ static void mandelbulb_iteration(const scene_t *s, v4 *z, aux_t *aux)
 static inline double sign_d(double v) { return (double)((0.0 < v) - (v < 0.0)); }
 
 /* cFractalMandelbox::FormulaCode (formula/definition/fractal_mandelbox.cpp:31-107) */
-static void mandelbox_iteration(const scene_t *s, v4 *zp, aux_t *aux)
+static void mandelbox_iteration(const slot_t *s, v4 *zp, aux_t *aux)
 {
 	v4 z = *zp;
 	if (s->mbRotationsEnabled)
@@ -555,7 +735,7 @@ static void mandelbox_iteration(const scene_t *s, v4 *zp, aux_t *aux)
 }
 
 /* cFractalKaleidoscopicIfs::FormulaCode (formula/definition/fractal_kaleidoscopic_ifs.cpp:30-74) */
-static void kifs_iteration(const scene_t *s, v4 *zp, aux_t *aux)
+static void kifs_iteration(const slot_t *s, v4 *zp, aux_t *aux)
 {
 	v4 z = *zp;
 	if (s->ifsAbsX) z.x = fabs(z.x);
@@ -597,19 +777,210 @@ static void kifs_iteration(const scene_t *s, v4 *zp, aux_t *aux)
 	*zp = z;
 }
 
-static void formula_code(const scene_t *s, v4 *z, aux_t *aux)
+/* cFractalMandelbulbPower2::FormulaCode (fractal_mandelbulb_power2.cpp:28-43) */
+static void mandelbulb_power2_iteration(v4 *z, aux_t *aux)
+{
+	aux->DE = aux->DE * 2.0 * aux->r;
+	const double x2 = z->x * z->x, y2 = z->y * z->y, z2 = z->z * z->z;
+	const double temp = 1.0 - z2 / (x2 + y2);
+	const double newx = (x2 - y2) * temp;
+	const double newy = 2.0 * z->x * z->y * temp;
+	const double newz = -2.0 * z->z * sqrt(x2 + y2);
+	z->x = newx;
+	z->y = newy;
+	z->z = newz;
+}
+
+/* cFractalHypercomplex::FormulaCode (fractal_hypercomplex.cpp:28-41) */
+static void hypercomplex_iteration(v4 *z, aux_t *aux)
+{
+	aux->DE = aux->DE * 2.0 * aux->r;
+	const double newx = z->x * z->x - z->y * z->y - z->z * z->z - z->w * z->w;
+	const double newy = 2.0 * z->x * z->y - 2.0 * z->w * z->z;
+	const double newz = 2.0 * z->x * z->z - 2.0 * z->y * z->w;
+	const double neww = 2.0 * z->x * z->w - 2.0 * z->y * z->z;
+	*z = V4(newx, newy, newz, neww);
+}
+
+/* cFractalQuaternion::FormulaCode (fractal_quaternion.cpp:28-39) */
+static void quaternion_iteration(v4 *z, aux_t *aux)
+{
+	aux->DE = aux->DE * 2.0 * aux->r;
+	const double newx = z->x * z->x - z->y * z->y - z->z * z->z;
+	const double newy = 2.0 * z->x * z->y;
+	const double newz = 2.0 * z->x * z->z;
+	z->x = newx;
+	z->y = newy;
+	z->z = newz;
+}
+
+/* cFractalMengerSponge::FormulaCode (fractal_menger_sponge.cpp:28-71) with the fold colour
+ * (fold_color_aux_color_enabled_false) off */
+static void menger_sponge_iteration(const slot_t *s, v4 *zp, aux_t *aux)
+{
+	v4 z = V4(fabs(zp->x), fabs(zp->y), fabs(zp->z), fabs(zp->w));
+	double temp;
+	if (z.x < z.y)
+	{
+		temp = z.y;
+		z.y = z.x;
+		z.x = temp;
+	}
+	if (z.x < z.z)
+	{
+		temp = z.z;
+		z.z = z.x;
+		z.x = temp;
+	}
+	if (z.y < z.z)
+	{
+		temp = z.z;
+		z.z = z.y;
+		z.y = temp;
+	}
+	z = v4_mul(z, s->scale3);
+	z.x -= 2.0;
+	z.y -= 2.0;
+	if (z.z > 1.0) z.z -= 2.0;
+	aux->DE *= s->scale3;
+	*zp = z;
+}
+
+/* cFractalXenodreambuie::FormulaCode (fractal_xenodreambuie.cpp:28-47) */
+static void xenodreambuie_iteration(const slot_t *s, v4 *z, aux_t *aux)
+{
+	double rp = pow(aux->r, s->bulbPower - 1.0);
+	aux->DE = rp * aux->DE * s->bulbPower + 1.0;
+	rp *= aux->r;
+	const double th = (atan2(z->y, z->x) + s->bulbBeta) * s->bulbPower;
+	double ph = acos(z->z / aux->r) + s->bulbAlpha;
+	if (fabs(ph) > 0.5 * M_PI) ph = sign_d(ph) * M_PI - ph;
+	ph *= s->bulbPower;
+	const double sph = sin(ph);
+	z->x = rp * cos(th) * sph;
+	z->y = rp * sin(th) * sph;
+	z->z = rp * cos(ph);
+}
+
+/* cFractalBoxFoldBulbPow2::FormulaCode (fractal_box_fold_bulb_pow2.cpp:28-78) */
+static void box_fold_bulb_pow2_iteration(const slot_t *s, v4 *zp, aux_t *aux)
+{
+	v4 z = *zp;
+	const double ff = s->fipFoldFactor;
+	if (fabs(z.x) > ff) z.x = sign_d(z.x) * ff * 2.0 - z.x;
+	if (fabs(z.y) > ff) z.y = sign_d(z.y) * ff * 2.0 - z.y;
+	if (fabs(z.z) > ff) z.z = sign_d(z.z) * ff * 2.0 - z.z;
+	const double fR2_2 = 1.0, mR2_2 = 0.25;
+	const double r2_2 = v4_dot(z, z);
+	const double tglad_factor1_2 = fR2_2 / mR2_2;
+	if (r2_2 < mR2_2)
+	{
+		z = v4_mul(z, tglad_factor1_2);
+		aux->DE *= tglad_factor1_2;
+	}
+	else if (r2_2 < fR2_2)
+	{
+		const double tglad_factor2_2 = fR2_2 / r2_2;
+		z = v4_mul(z, tglad_factor2_2);
+		aux->DE *= tglad_factor2_2;
+	}
+	z = v4_mul(z, 2.0);
+	const double x2 = z.x * z.x, y2 = z.y * z.y, z2 = z.z * z.z;
+	const double temp = 1.0 - z2 / (x2 + y2);
+	v4 zt;
+	zt.x = (x2 - y2) * temp;
+	zt.y = 2.0 * z.x * z.y * temp;
+	zt.z = -2.0 * z.z * sqrt(x2 + y2);
+	zt.w = z.w;
+	z = zt;
+	z.z *= s->fipZFactor;
+	if (s->adeEnabledFalse)
+		aux->DE = (aux->DE + 1.0) * 5.0 * aux->r * s->adeScale1 *
+				sqrt(s->fipZFactor * s->fipZFactor + 2.0 + s->adeOffset2) + s->adeOffset1;
+	*zp = z;
+}
+
+/* cFractalAexion::FormulaCode (fractal_aexion.cpp:28-64): at the first iteration it replaces
+ * the constant with a folded copy of the point */
+static void aexion_iteration(const slot_t *s, v4 *zp, aux_t *aux)
+{
+	v4 z = *zp;
+	v4 temp = z;
+	double t;
+	if (aux->i == 0)
+	{
+		t = s->aexionCadd;
+		temp.x = z.x + z.y + z.z;
+		temp.y = -z.x - z.y + z.z;
+		temp.z = -z.x + z.y - z.z;
+		temp.w = z.x - z.y - z.z;
+		z = V4(fabs(temp.x) + t, fabs(temp.y) + t, fabs(temp.z) + t, fabs(temp.w) + t);
+		aux->const_c = z;
+	}
+	t = 2.0 * z.w * z.z;
+	temp.x = z.x * z.x - z.y * z.y;
+	temp.y = t - temp.x;
+	temp.x += t;
+	t = 2.0 * z.x * z.y;
+	temp.z = z.z * z.z - z.w * z.w;
+	temp.w = t - temp.z;
+	temp.z += t;
+	z = v4_add(temp, aux->const_c);
+	if (s->adeEnabled)
+	{
+		double de1 = 1.1 * aux->r;
+		de1 = de1 + (v4_len(z) / aux->r - de1) * s->adeScale1;
+		aux->DE = de1 * 2.0 * aux->DE + s->adeOffset1;
+	}
+	*zp = z;
+}
+
+static void formula_code(const slot_t *s, v4 *z, aux_t *aux)
 {
 	switch (s->formula)
 	{
 		case MBX_FORMULA_MANDELBULB: mandelbulb_iteration(s, z, aux); break;
+		case MBX_FORMULA_MANDELBULB_POWER2: mandelbulb_power2_iteration(z, aux); break;
+		case MBX_FORMULA_HYPERCOMPLEX: hypercomplex_iteration(z, aux); break;
+		case MBX_FORMULA_QUATERNION: quaternion_iteration(z, aux); break;
+		case MBX_FORMULA_MENGER_SPONGE: menger_sponge_iteration(s, z, aux); break;
 		case MBX_FORMULA_MANDELBOX: mandelbox_iteration(s, z, aux); break;
 		case MBX_FORMULA_KALEIDOSCOPIC_IFS: kifs_iteration(s, z, aux); break;
+		case MBX_FORMULA_XENODREAMBUIE: xenodreambuie_iteration(s, z, aux); break;
+		case MBX_FORMULA_BOX_FOLD_BULB_POW2: box_fold_bulb_pow2_iteration(s, z, aux); break;
+		case MBX_FORMULA_AEXION: aexion_iteration(s, z, aux); break;
 		default: break;
 	}
 }
 
+/* SmoothCVector (common_math.cpp:132-163): the weighted blend of a hybrid step with the
+ * length interpolated separately from the direction */
+static v4 smooth_cvector(v4 v1, v4 v2, double k)
+{
+	if (k <= 0.0) return v1;
+	if (k >= 1.0) return v2;
+	const double nk = 1.0 - k;
+	const double lenInterp = v4_len(v1) * nk + v4_len(v2) * k;
+	const v4 vt = v4_add(v4_mul(v1, nk), v4_mul(v2, k));
+	const double lt = v4_len(vt);
+	if (lt > 0.0) return v4_mul(V4(vt.x / lt, vt.y / lt, vt.z / lt, vt.w / lt), lenInterp);
+	return v1;
+}
+
+/* z + c for the iteration's container (compute_fractal.cpp:232-281): the Julia constant, or
+ * the constant of the point (aux.const_c, which keeps its w) */
+static v4 add_constant(const scene_t *s, v4 z, const aux_t *aux)
+{
+	if (s->juliaEnabled)
+		return v4_add(z, V4(s->juliaConstant.x * s->constantMultiplier.x, s->juliaConstant.y * s->constantMultiplier.y,
+							 s->juliaConstant.z * s->constantMultiplier.z, 0.0));
+	/* CVector4 * CVector3 keeps w (algebra.hpp:340-343) */
+	return v4_add(z, V4(aux->const_c.x * s->constantMultiplier.x, aux->const_c.y * s->constantMultiplier.y,
+						 aux->const_c.z * s->constantMultiplier.z, aux->const_c.w));
+}
+
 /* ------------------------------------------------------------------------------------------ */
-/* Compute<Mode> (mandelbulber2/src/compute_fractal.cpp:67-654), single formula, no hybrid    */
+/* Compute<Mode> (mandelbulber2/src/compute_fractal.cpp:67-654): one container or a hybrid    */
 /* ------------------------------------------------------------------------------------------ */
 enum { MODE_NORMAL = 0, MODE_DELTA1 = 1 };
 
@@ -632,7 +1003,7 @@ static void compute(const scene_t *s, int mode, v3 point, int forcedMaxiter, int
 	aux.DE0 = 0.0;
 	aux.dist = 1000.0;
 	aux.pseudoKleinianDE = 1.0;
-	aux.actualScale = s->mbScale;
+	aux.actualScale = s->slot[s->seq[0]].mbScale;
 	aux.actualScaleA = 0.0;
 	aux.color = 1.0;
 	aux.colorHybrid = 0.0;
@@ -646,23 +1017,18 @@ static void compute(const scene_t *s, int mode, v3 point, int forcedMaxiter, int
 	{
 		lastGoodZ = lastZ;
 		lastZ = z;
+		const slot_t *f = &s->slot[seq_slot(s, i)];
+		const v4 tempZ = z;
+		const double tempAuxDE = aux.DE, tempAuxColor = aux.color;
 		aux.i = i;
-		formula_code(s, &z, &aux);
-		if (s->addCConstant)
+		if (f->weight > 0.0) formula_code(f, &z, &aux);
+		if (f->addC) z = add_constant(s, z, &aux);
+		if (s->isHybrid && f->weight < 1.0)
 		{
-			if (s->juliaEnabled)
-			{
-				z = v4_add(z, V4(s->juliaConstant.x * s->constantMultiplier.x,
-												s->juliaConstant.y * s->constantMultiplier.y,
-												s->juliaConstant.z * s->constantMultiplier.z, 0.0));
-			}
-			else
-			{
-				/* CVector4 * CVector3 keeps w (algebra.hpp:340-343) */
-				z = v4_add(z, V4(aux.const_c.x * s->constantMultiplier.x,
-												aux.const_c.y * s->constantMultiplier.y,
-												aux.const_c.z * s->constantMultiplier.z, aux.const_c.w));
-			}
+			const double k = f->weight, kn = 1.0 - k;
+			z = smooth_cvector(tempZ, z, k);
+			aux.DE = aux.DE * k + tempAuxDE * kn;
+			aux.color = aux.color * k + tempAuxColor * kn;
 		}
 		aux.r = v4_len(z);
 		if (v4_notfinite(z))
@@ -672,7 +1038,7 @@ static void compute(const scene_t *s, int mode, v3 point, int forcedMaxiter, int
 			out->maxiter = 1;
 			break;
 		}
-		if (aux.r > s->bailout)
+		if (f->checkForBailout && aux.r > f->bailout)
 		{
 			out->maxiter = 0;
 			break;
@@ -684,7 +1050,20 @@ static void compute(const scene_t *s, int mode, v3 point, int forcedMaxiter, int
 		}
 	}
 
-	if (mode == MODE_NORMAL)
+	if (mode == MODE_NORMAL && s->isHybrid)
+	{
+		/* hybrid finish by the sequence's DE function (compute_fractal.cpp:502-515) */
+		if (aux.DE > 0.0)
+		{
+			if (s->DEFunctionType == MBX_DE_FUNCTION_LINEAR)
+				out->distance = (aux.r - s->linearDEOffset) / aux.DE;
+			else
+				out->distance = (aux.r > 1.0) ? 0.5 * aux.r * log(aux.r) / aux.DE : 0.0;
+		}
+		else
+			out->distance = aux.r;
+	}
+	else if (mode == MODE_NORMAL)
 	{
 		if (aux.DE > 0.0)
 		{
@@ -728,11 +1107,10 @@ static double compute_colour_index(const scene_t *s, v3 point)
 	aux.DE = 1.0;
 	aux.dist = 1000.0;
 	aux.pseudoKleinianDE = 1.0;
-	aux.actualScale = s->mbScale;
+	aux.actualScale = s->slot[s->seq[0]].mbScale;
 	aux.color = 1.0;
 	aux.temp1000 = 1000.0;
 	double colorMin = 1000.0;
-	const int isBox = s->formula == MBX_FORMULA_MANDELBOX;
 	const int alg = p->coloringAlgorithm;
 	const v3 lineDir = v3_from(p->coloringLineDirection);
 	v4 lastZ = V4(0, 0, 0, 0);
@@ -740,18 +1118,18 @@ static double compute_colour_index(const scene_t *s, v3 point)
 	for (int i = 0; i < maxN; i++)
 	{
 		lastZ = z;
+		const slot_t *f = &s->slot[seq_slot(s, i)];
+		const v4 tempZ = z;
+		const double tempAuxDE = aux.DE, tempAuxColor = aux.color;
 		aux.i = i;
-		formula_code(s, &z, &aux);
-		if (s->addCConstant)
+		if (f->weight > 0.0) formula_code(f, &z, &aux);
+		if (f->addC) z = add_constant(s, z, &aux);
+		if (s->isHybrid && f->weight < 1.0)
 		{
-			if (s->juliaEnabled)
-				z = v4_add(z, V4(s->juliaConstant.x * s->constantMultiplier.x,
-												s->juliaConstant.y * s->constantMultiplier.y,
-												s->juliaConstant.z * s->constantMultiplier.z, 0.0));
-			else
-				z = v4_add(z, V4(aux.const_c.x * s->constantMultiplier.x,
-												aux.const_c.y * s->constantMultiplier.y,
-												aux.const_c.z * s->constantMultiplier.z, aux.const_c.w));
+			const double k = f->weight, kn = 1.0 - k;
+			z = smooth_cvector(tempZ, z, k);
+			aux.DE = aux.DE * k + tempAuxDE * kn;
+			aux.color = aux.color * k + tempAuxColor * kn;
 		}
 		aux.r = v4_len(z);
 		if (v4_notfinite(z))
@@ -779,12 +1157,14 @@ static double compute_colour_index(const scene_t *s, v3 point)
 			default: len = aux.r; break;
 		}
 		const double change = v4_len(v4_sub(z, lastZ)) / aux.r;
+		if (!f->checkForBailout) continue; /* the colouring test sits in the bailout block */
+		const int isBox = f->formula == MBX_FORMULA_MANDELBOX;
 		if (!p->coloringPreV215)
 		{
 			if (!isBox)
 			{
 				if (len < colorMin) colorMin = len;
-				if (aux.r > s->bailout) break;
+				if (aux.r > f->bailout) break;
 			}
 			else if (alg == 0)
 			{
@@ -793,7 +1173,7 @@ static double compute_colour_index(const scene_t *s, v3 point)
 			else
 			{
 				if (len < colorMin) colorMin = len;
-				if (aux.r > s->bailout || change < 1e-15) break;
+				if (aux.r > f->bailout || change < 1e-15) break;
 			}
 		}
 		else
@@ -807,13 +1187,20 @@ static double compute_colour_index(const scene_t *s, v3 point)
 				break;
 		}
 	}
-	switch (s->formula)
+	if (s->isHybrid)
 	{
-		case MBX_FORMULA_MANDELBOX: /* coloringFunctionABox */
+		/* hybrid colour index (fractal_coloring.cpp:292-316) */
+		colorMin = fmin(100.0, colorMin);
+		const double r2 = fmin(aux.r / fabs(aux.DE), 20.0);
+		return colorMin * 1000.0 + aux.color * 100.0 + r2 * 5000.0;
+	}
+	switch (s->coloringFunction)
+	{
+		case COLORING_ABOX:
 			return aux.color * 100.0 + aux.r * p->fractal.mandelbox.colorFactorR / 1e13 +
 				((alg != 0) ? colorMin * 1000.0 : 0.0);
-		case MBX_FORMULA_KALEIDOSCOPIC_IFS: return colorMin * 1000.0; /* coloringFunctionIFS */
-		default: return colorMin * 5000.0;                            /* coloringFunctionDefault */
+		case COLORING_IFS: return colorMin * 1000.0;
+		default: return colorMin * 5000.0; /* coloringFunctionDefault */
 	}
 }
 
@@ -1853,7 +2240,7 @@ int mbxo_formula_step(const mbx_params *p, int n, double *z4, double *aux3)
 		aux.r = aux3[3 * k];
 		aux.DE = aux3[3 * k + 1];
 		aux.color = aux3[3 * k + 2];
-		formula_code(&s, &z, &aux);
+		formula_code(&s.slot[0], &z, &aux);
 		z4[4 * k] = z.x;
 		z4[4 * k + 1] = z.y;
 		z4[4 * k + 2] = z.z;
@@ -1861,6 +2248,44 @@ int mbxo_formula_step(const mbx_params *p, int n, double *z4, double *aux3)
 		aux3[3 * k] = aux.r;
 		aux3[3 * k + 1] = aux.DE;
 		aux3[3 * k + 2] = aux.color;
+	}
+	return MBX_OK;
+}
+
+/* One FormulaCode step of container `slot` (0 = p->fractal) with the iteration index and the
+ * constant: z4 and constc4 (xyzw) and aux3 (r, DE, color) are updated in place. */
+int mbxo_formula_step_ex(const mbx_params *p, int slot, int n, double *z4, double *aux3,
+	const int32_t *iter, double *constc4)
+{
+	scene_t s;
+	int rc0 = scene_init(&s, p);
+	if (rc0) return rc0;
+	if (slot < 0 || slot >= s.nSlots) return MBX_ERR_INVALID;
+	for (int k = 0; k < n; k++)
+	{
+		v4 z = V4(z4[4 * k], z4[4 * k + 1], z4[4 * k + 2], z4[4 * k + 3]);
+		aux_t aux;
+		memset(&aux, 0, sizeof(aux));
+		aux.r = aux3[3 * k];
+		aux.DE = aux3[3 * k + 1];
+		aux.color = aux3[3 * k + 2];
+		aux.i = iter ? iter[k] : 0;
+		if (constc4) aux.const_c = V4(constc4[4 * k], constc4[4 * k + 1], constc4[4 * k + 2], constc4[4 * k + 3]);
+		formula_code(&s.slot[slot], &z, &aux);
+		z4[4 * k] = z.x;
+		z4[4 * k + 1] = z.y;
+		z4[4 * k + 2] = z.z;
+		z4[4 * k + 3] = z.w;
+		aux3[3 * k] = aux.r;
+		aux3[3 * k + 1] = aux.DE;
+		aux3[3 * k + 2] = aux.color;
+		if (constc4)
+		{
+			constc4[4 * k] = aux.const_c.x;
+			constc4[4 * k + 1] = aux.const_c.y;
+			constc4[4 * k + 2] = aux.const_c.z;
+			constc4[4 * k + 3] = aux.const_c.w;
+		}
 	}
 	return MBX_OK;
 }
@@ -2060,14 +2485,14 @@ static dl3 diff_de_o(const scene_t *s, const dl3 pw[3], int *iters, int *ok)
 	dl3 z[3] = {p[0], p[1], p[2]};
 	dl3 DE = dl_c(1.0);
 	dl3 r = dl_sqrt(dl_add(dl_add(dl_mul(z[0], z[0]), dl_mul(z[1], z[1])), dl_mul(z[2], z[2])));
-	const double P = s->bulbPower;
+	const double P = s->slot[0].bulbPower;
 	int i;
 	*ok = 1;
 	for (i = 0; i < s->formulaMaxiter; i++)
 	{
 		if (!(r.v > 0.0)) { *ok = 0; break; }
-		const dl3 th0 = dl_addc(dl_asin(dl_div(z[2], r)), s->bulbBeta);
-		const dl3 ph0 = dl_addc(dl_atan2(z[1], z[0]), s->bulbAlpha);
+		const dl3 th0 = dl_addc(dl_asin(dl_div(z[2], r)), s->slot[0].bulbBeta);
+		const dl3 ph0 = dl_addc(dl_atan2(z[1], z[0]), s->slot[0].bulbAlpha);
 		dl3 rp = dl_powc(r, P - 1.0);
 		const dl3 th = dl_scale(th0, P);
 		const dl3 ph = dl_scale(ph0, P);
@@ -2079,7 +2504,7 @@ static dl3 diff_de_o(const scene_t *s, const dl3 pw[3], int *iters, int *ok)
 		z[2] = dl_add(dl_mul(dl_sin(th), rp), dl_scale(p[2], s->constantMultiplier.z));
 		r = dl_sqrt(dl_add(dl_add(dl_mul(z[0], z[0]), dl_mul(z[1], z[1])), dl_mul(z[2], z[2])));
 		if (!isfinite(r.v)) { *ok = 0; break; }
-		if (r.v > s->bailout) break;
+		if (r.v > s->slot[0].bailout) break;
 	}
 	*iters = i + 1;
 	if (!(DE.v > 0.0) || !(r.v > 1.0)) { *ok = 0; return dl_c(0.0); }
@@ -2110,7 +2535,7 @@ int mbxo_render_diff(const mbx_params *p, int y0, int y1, int threads, double *r
 	int rc0 = scene_init(&s, p);
 	if (rc0) return rc0;
 	if (p->fractal.formula != MBX_FORMULA_MANDELBULB || s.DEType != MBX_DE_ANALYTIC ||
-		s.DEAnalyticFunction != MBX_ANALYTIC_LOGARITHMIC || !s.addCConstant || s.juliaEnabled)
+		s.DEAnalyticFunction != MBX_ANALYTIC_LOGARITHMIC || !s.slot[0].addC || s.juliaEnabled || s.isHybrid)
 		return MBX_ERR_UNSUPPORTED;
 	const int W = p->imageWidth, H = p->imageHeight;
 	const double dthr = p->constantDEThreshold ? 0.0 : s.resolution * s.fov / p->detailLevel;

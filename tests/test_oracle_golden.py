@@ -118,3 +118,87 @@ def test_park_miller_minimal_standard_kat(oracle_mod):
     for _ in range(50):
         x = (16807 * x) % 2147483647
     assert seq[49] == x
+
+
+def _run_ex(oracle_mod, p, recs, iters=None, const_c=None):
+    z_in = np.stack([_hex(r["z_in"]) for r in recs])
+    aux_in = np.stack([_hex(r["aux_in"]) for r in recs])
+    z, aux, cc = oracle_mod.formula_step_ex(p, z_in, aux_in, iters, const_c)
+    return z, aux, cc, np.stack([_hex(r["z_out"]) for r in recs]), np.stack([_hex(r["aux_out"]) for r in recs])
+
+
+@pytest.mark.parametrize("tag,formula", [("mandelbulbpower2", abi.MBX_FORMULA_MANDELBULB_POWER2),
+                                         ("hypercomplex", abi.MBX_FORMULA_HYPERCOMPLEX),
+                                         ("quaternion", abi.MBX_FORMULA_QUATERNION)])
+def test_power2_formulas_bitexact(oracle_mod, golden, tag, formula):
+    """z^2 in the bulb, hypercomplex and quaternion algebras, DE = 2 r DE (4D z)"""
+    recs = golden["records"][tag]
+    assert len(recs) == 48
+    p = scene.default_params()
+    p.fractal.formula = formula
+    z, aux, _, z_ref, aux_ref = _run_ex(oracle_mod, p, recs)
+    np.testing.assert_array_equal(z, z_ref)
+    np.testing.assert_array_equal(aux[:, 1], aux_ref[:, 1])
+
+
+def test_menger_sponge_formula_bitexact(oracle_mod, golden):
+    groups = _group(golden["records"]["mengersponge"])
+    assert len(groups) == 2
+    for params, recs in groups.items():
+        p = scene.default_params()
+        p.fractal.formula = abi.MBX_FORMULA_MENGER_SPONGE
+        p.fractal.transform.scale3 = float.fromhex(params[0])
+        z, aux, _, z_ref, aux_ref = _run_ex(oracle_mod, p, recs)
+        np.testing.assert_array_equal(z, z_ref)
+        np.testing.assert_array_equal(aux[:, 1], aux_ref[:, 1])
+
+
+def test_xenodreambuie_formula_bitexact(oracle_mod, golden):
+    groups = _group(golden["records"]["xenodreambuie"])
+    assert len(groups) == 3
+    for params, recs in groups.items():
+        power, alpha_deg, beta_deg = (float.fromhex(v) for v in params)
+        p = scene.default_params()
+        p.fractal.formula = abi.MBX_FORMULA_XENODREAMBUIE
+        p.fractal.bulb.power = power
+        p.fractal.bulb.alphaAngleOffset = alpha_deg
+        p.fractal.bulb.betaAngleOffset = beta_deg
+        z, aux, _, z_ref, aux_ref = _run_ex(oracle_mod, p, recs)
+        np.testing.assert_array_equal(z, z_ref)
+        np.testing.assert_array_equal(aux[:, 1], aux_ref[:, 1])
+
+
+def test_box_fold_bulb_pow2_formula_bitexact(oracle_mod, golden):
+    groups = _group(golden["records"]["boxfoldbulbpow2"])
+    assert len(groups) == 2
+    for params, recs in groups.items():
+        ff, zf, en, s1, o1, o2 = (float.fromhex(v) for v in params)
+        p = scene.default_params()
+        p.fractal.formula = abi.MBX_FORMULA_BOX_FOLD_BULB_POW2
+        t = p.fractal.transform
+        t.foldingIntPowFoldFactor, t.foldingIntPowZFactor = ff, zf
+        t.analyticDEEnabledFalse = int(en)
+        t.analyticDEScale1, t.analyticDEOffset1, t.analyticDEOffset2 = s1, o1, o2
+        z, aux, _, z_ref, aux_ref = _run_ex(oracle_mod, p, recs)
+        np.testing.assert_array_equal(z, z_ref)
+        np.testing.assert_array_equal(aux[:, 1], aux_ref[:, 1])
+
+
+def test_aexion_formula_bitexact(oracle_mod, golden):
+    """the first iteration (aux.i == 0) replaces the constant; later ones add it"""
+    recs = golden["records"]["aexion"]
+    groups = {}
+    for r in recs:
+        groups.setdefault(tuple(r["params"][:5]), []).append(r)
+    assert len(groups) == 4
+    for params, rs in groups.items():
+        cadd, it, en, s1, o1 = (float.fromhex(v) for v in params)
+        p = scene.default_params()
+        p.fractal.formula = abi.MBX_FORMULA_AEXION
+        t = p.fractal.transform
+        t.aexionCadd, t.analyticDEEnabled, t.analyticDEScale1, t.analyticDEOffset1 = cadd, int(en), s1, o1
+        cin = np.stack([_hex(r["params"][5:9]) for r in rs])
+        z, aux, cc, z_ref, aux_ref = _run_ex(oracle_mod, p, rs, np.full(len(rs), int(it)), cin)
+        np.testing.assert_array_equal(z, z_ref)
+        np.testing.assert_array_equal(aux[:, 1], aux_ref[:, 1])
+        np.testing.assert_array_equal(cc, np.stack([_hex(r["const_c_out"]) for r in rs]))
