@@ -297,12 +297,27 @@ typedef struct mbx_params {
 	double glowColor2[3];             /* "glow_color_2" */
 
 	/* hybrid sequence (cHybridFractalSequences::CreateSequence, hybrid_fractal_sequences.cpp:290-377):
-	 * hybridCount 0 or 1 renders `fractal` alone; 2 .. MBX_MAX_FRACTALS renders the hybrid of
-	 * `fractal` and hybridFractals[0 .. hybridCount - 2], in that order */
+	 * hybridCount 0 renders `fractal` alone; 1 .. MBX_MAX_FRACTALS renders the hybrid of `fractal`
+	 * and hybridFractals[0 .. hybridCount - 2], in that order.  A hybrid of one container is not
+	 * the container alone: it takes the hybrid DE finish, bailout and colour index. */
 	int32_t hybridCount;              /* fractal containers in the hybrid */
 	int32_t repeatFrom;               /* "repeat_from" (1-based) */
 	double linearDEOffset;            /* "linear_DE_offset" (linear DE finish of a hybrid) */
 	mbx_fractal hybridFractals[MBX_MAX_FRACTALS - 1];
+
+	/* "advanced quality" march limits: CalcDistThresh clamps the threshold to
+	 * [detailSizeMin, detailSizeMax] (render_worker.cpp:671-675), RayMarching clamps each step
+	 * (render_worker.cpp:817-834) and the delta DE offsets by deltaDERelativeDelta * detail
+	 * (calculate_distance.cpp:614-617) */
+	int32_t advancedQuality;          /* "advanced_quality" */
+	int32_t reserved3;
+	double absMaxMarchingStep;        /* "abs_max_marching_step" */
+	double absMinMarchingStep;        /* "abs_min_marching_step" */
+	double relMaxMarchingStep;        /* "rel_max_marching_step" */
+	double relMinMarchingStep;        /* "rel_min_marching_step" */
+	double detailSizeMax;             /* "detail_size_max" */
+	double detailSizeMin;             /* "detail_size_min" */
+	double deltaDERelativeDelta;      /* "deltade_relative_delta" */
 } mbx_params;
 
 /* Per-pixel output planes (device pointers; any may be NULL except where noted).

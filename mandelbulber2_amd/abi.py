@@ -220,6 +220,15 @@ class MbxParams(C.Structure):
         ("repeatFrom", C.c_int32),
         ("linearDEOffset", C.c_double),
         ("hybridFractals", MbxFractal * (MBX_MAX_FRACTALS - 1)),
+        ("advancedQuality", C.c_int32),
+        ("reserved3", C.c_int32),
+        ("absMaxMarchingStep", C.c_double),
+        ("absMinMarchingStep", C.c_double),
+        ("relMaxMarchingStep", C.c_double),
+        ("relMinMarchingStep", C.c_double),
+        ("detailSizeMax", C.c_double),
+        ("detailSizeMin", C.c_double),
+        ("deltaDERelativeDelta", C.c_double),
     ]
 
 

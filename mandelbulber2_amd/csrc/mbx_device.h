@@ -350,7 +350,7 @@ __device__ __forceinline__ double compute_box_d(DV3 p, const FrameConst &f, doub
 		double DE2 = DE;
 		MandelboxIterationD<false>(z2, DE2, nullptr, f);
 		z2 = add_c_d(z2, p, f);
-		const double delta = fmax(sqrt(ddot(p, p)) * 1e-14, 1e-5 * detailSize);
+		const double delta = fmax(sqrt(ddot(p, p)) * 1e-14, f.deltaRel * detailSize);
 		dr = 1.7320508075688772 * fabs(sqrt(ddot(z2, z2)) - r) / delta;
 	}
 	if (dr > 0.0)
@@ -393,7 +393,7 @@ __device__ __forceinline__ double compute_box_fd(DV3 p, const FrameConst &f, dou
 	int &itersLast, uint32_t &work)
 {
 #pragma clang fp contract(off)
-	const double deltaDE = fmax(sqrt(ddot(p, p)) * 1e-14, 1e-5 * detailSize);
+	const double deltaDE = fmax(sqrt(ddot(p, p)) * 1e-14, f.deltaRel * detailSize);
 	DV3 z;
 	iters = box_orbit_d(p, f, f.formulaMaxiter, z);
 	const double r = sqrt(ddot(z, z));
@@ -460,6 +460,13 @@ __device__ __forceinline__ double compute_bulb_d(DV3 p, const FrameConst &f, int
 	if (f.analyticFunction == MBX_ANALYTIC_IFS) return (r - 2.0) / DE;
 	return r / DE;
 }
+
+} // namespace mbx
+
+#include "mbx_generic.h"
+
+namespace mbx
+{
 
 // Kaleidoscopic IFS (fractal_kaleidoscopic_ifs.cpp:30-74)
 __device__ __forceinline__ void KaleidoscopicIfsIteration(V3 &z, float &DE, const FrameConst &f)
@@ -616,7 +623,7 @@ __device__ __forceinline__ float compute_analytic(V3 p, const FrameConst &f, con
 template <int KIND>
 struct PosPrecision
 {
-	static constexpr bool kFP64 = (KIND == kBoxAnalytic || KIND == kBoxDelta || KIND == kBulbD);
+	static constexpr bool kFP64 = (KIND == kBoxAnalytic || KIND == kBoxDelta || KIND == kBulbD || KIND == kGeneric);
 };
 
 template <bool D>
@@ -732,6 +739,11 @@ __device__ __forceinline__ typename Prec<PosPrecision<KIND>::kFP64>::R calculate
 	{
 		d = compute_bulb_d(p, f, iters);
 		w.iterations += (uint32_t)iters;
+		itersMinN = iters;
+	}
+	else if constexpr (KIND == kGeneric)
+	{
+		d = compute_generic(p, f, detailSize, &iters, &w.iterations);
 		itersMinN = iters;
 	}
 	else
@@ -902,7 +914,9 @@ template <int KIND>
 __device__ __forceinline__ double colour_index(typename Prec<PosPrecision<KIND>::kFP64>::P p,
 	const FrameConst &f, const Work &w)
 {
-	if constexpr (PosPrecision<KIND>::kFP64)
+	if constexpr (KIND == kGeneric)
+		return colour_index_generic(p, f);
+	else if constexpr (PosPrecision<KIND>::kFP64)
 		return colour_index_d<KIND>(p, f);
 	else
 		return colour_index_f<KIND>(p, f, w);

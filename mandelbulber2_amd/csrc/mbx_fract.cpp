@@ -320,7 +320,9 @@ bool is_fractal_param(const std::string &base)
 		"mandelbox_folding_fixed_radius", "mandelbox_sharpness", "mandelbox_offset", "mandelbox_rotation_main",
 		"mandelbox_color", "mandelbox_color_Sp1", "mandelbox_color_Sp2", "mandelbox_color_R", "mandelbox_main_rotation_enabled",
 		"mandelbox_rotations_enabled", "formula_position", "formula_rotation", "formula_repeat", "formula_scale",
-		"formula_material_id"};
+		"formula_material_id", "transf_scale_3", "cadd", "boxfold_bulbpow2_folding_factor",
+		"boxfold_bulbpow2_z_factor", "analyticDE_enabled", "analyticDE_enabled_false", "analyticDE_scale_1",
+		"analyticDE_offset_1", "analyticDE_offset_2", "fold_color_aux_color_enabled_false"};
 	return names.count(base) != 0;
 }
 
@@ -609,6 +611,15 @@ extern "C" void mbx_params_default(mbx_params *p)
 	p->hybridCount = 0;
 	p->repeatFrom = 1;
 	p->linearDEOffset = 0.0;
+	// advanced quality march limits :159-166
+	p->advancedQuality = 0;
+	p->absMaxMarchingStep = 3.0;
+	p->absMinMarchingStep = 1e-15;
+	p->relMaxMarchingStep = 1e4;
+	p->relMinMarchingStep = 1e-2;
+	p->detailSizeMax = 1.0;
+	p->detailSizeMin = 1e-12;
+	p->deltaDERelativeDelta = 0.01;
 	// palette colouring of material #1 (InitMaterialParams)
 	p->useColorsFromPalette = 1;
 	p->surfaceGradientEnable = 1;
@@ -740,6 +751,8 @@ extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, 
 	double juliaC[3] = {0, 0, 0}, constFactor[3] = {1, 1, 1};
 	double initialW = 0.0;
 	int maxiter = 250;
+	std::vector<int> hybridIds; // containers of a hybrid node, in sequence order
+	bool hybridNode = false;
 	if (ver < 2.35)
 	{
 		// MigrateToObjectsTree (settings.cpp:2551-2760) for the cases with one fractal node
@@ -802,8 +815,25 @@ extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, 
 		}
 		else
 		{
-			flags |= enabled.empty() ? MBX_FRACT_DISABLED : MBX_FRACT_HYBRID;
-			if (!enabled.empty()) objectId = enabled[0];
+			// MigrateToObjectsTree, hybrid mode (settings.cpp:2690-2760): a hybrid node with the
+			// legacy transform and sequence-wide values, and one child per container whose formula
+			// is not none; a child switched off by fractal_enable(_N) leaves the render list
+			// (objects_tree.cpp:234-270) and so the sequence
+			hybridNode = true;
+			for (int n : enabled)
+				if (F(n).b("fractal_enable", true)) hybridIds.push_back(n);
+			// the members' own transforms are not used: the distance code evaluates the sequence at
+			// the hybrid node and skips its fractal children (calculate_distance.cpp:414-436)
+			if (hybridIds.empty()) flags |= MBX_FRACT_DISABLED;
+			objectId = hybridIds.empty() ? (enabled.empty() ? 1 : enabled[0]) : hybridIds[0];
+			M.v3("fractal_rotation", rotation, 0, 0, 0);
+			M.v3("fractal_position", position, 0, 0, 0);
+			M.v3("repeat", repeat, 0, 0, 0);
+			juliaMode = M.b("julia_mode", false);
+			M.v3("julia_c", juliaC, 0, 0, 0);
+			M.v3("fractal_constant_factor", constFactor, 1, 1, 1);
+			initialW = M.d("initial_waxis", 0.0);
+			maxiter = (int)M.i("N", 250, 1, 99999);
 		}
 		if (doc.primitives) flags |= MBX_FRACT_PRIMITIVES;
 	}
@@ -814,6 +844,33 @@ extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, 
 		std::vector<int> ids(doc.nodeIds.begin(), doc.nodeIds.end());
 		if (ids.empty()) ids.push_back(1);
 		int fractalNodes = 0, otherNodes = 0, nodeId = -1;
+		int hybridId = -1, hybridNodes = 0, childrenOfHybrid = 0, hybridParent = 0;
+		std::vector<int> hybridChildren;
+		for (int id : ids)
+		{
+			char pfx[32];
+			std::snprintf(pfx, sizeof(pfx), "node_%04d_", id);
+			const std::string *def = M.raw(std::string(pfx) + "definition");
+			std::vector<std::string> parts = split(def ? *def : std::string("mandelbulb 1,1,1,0,1"), ',');
+			if (parts.size() >= 5 && to_int(parts[2]) == 10)
+			{
+				hybridNodes++;
+				hybridId = id;
+				hybridParent = (int)to_int(parts[3]);
+			}
+		}
+		for (int id : ids)
+		{
+			char pfx[32];
+			std::snprintf(pfx, sizeof(pfx), "node_%04d_", id);
+			const std::string *def = M.raw(std::string(pfx) + "definition");
+			std::vector<std::string> parts = split(def ? *def : std::string("mandelbulb 1,1,1,0,1"), ',');
+			if (parts.size() >= 5 && to_int(parts[2]) == 1 && hybridNodes == 1 && to_int(parts[3]) == hybridId)
+			{
+				childrenOfHybrid++;
+				if (M.b(std::string(pfx) + "enabled", true)) hybridChildren.push_back((int)to_int(parts[4]));
+			}
+		}
 		for (int id : ids)
 		{
 			char pfx[32];
@@ -836,7 +893,22 @@ extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, 
 			else if (type == 2) flags |= MBX_FRACT_PRIMITIVES;
 			else otherNodes++;
 		}
-		if (fractalNodes > 1 || otherNodes > 0) flags |= MBX_FRACT_HYBRID;
+		// a hybrid node at the root whose children are all the fractal nodes is a hybrid sequence
+		const bool hybridTree = hybridNodes == 1 && hybridParent == 0 && otherNodes == 1 &&
+			childrenOfHybrid == fractalNodes && fractalNodes > 0;
+		if (hybridTree)
+		{
+			hybridNode = true;
+			hybridIds = hybridChildren;
+			nodeId = hybridId;
+			if (hybridIds.empty()) flags |= MBX_FRACT_DISABLED;
+			else objectId = hybridIds[0];
+			char pfx[32];
+			std::snprintf(pfx, sizeof(pfx), "node_%04d_", hybridId);
+			nodeEnabled = M.b(std::string(pfx) + "enabled", true);
+		}
+		else if (fractalNodes > 1 || otherNodes > 0)
+			flags |= MBX_FRACT_HYBRID;
 		if (nodeId < 0) flags |= MBX_FRACT_DISABLED;
 		else
 		{
@@ -898,6 +970,14 @@ extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, 
 	p.interiorMode = M.b("interior_mode", false);
 	p.constantDEThreshold = M.b("constant_DE_threshold", false);
 	p.maxRaymarchingSteps = (int)M.i("max_raymarching_steps", 10000, 1, 100000000);
+	p.advancedQuality = M.b("advanced_quality", false);
+	p.absMaxMarchingStep = M.d("abs_max_marching_step", 3.0, 1e-15, 1e15);
+	p.absMinMarchingStep = M.d("abs_min_marching_step", 1e-15, 1e-15, 1e15);
+	p.relMaxMarchingStep = M.d("rel_max_marching_step", 1e4, 1e-15, 1e15);
+	p.relMinMarchingStep = M.d("rel_min_marching_step", 1e-2, 1e-15, 1e15);
+	p.detailSizeMax = M.d("detail_size_max", 1.0);
+	p.detailSizeMin = M.d("detail_size_min", 1e-12);
+	p.deltaDERelativeDelta = M.d("deltade_relative_delta", 0.01, 1e-15, 1e6);
 	if (p.iterThreshMode || p.interiorMode) flags |= MBX_FRACT_INTERIOR;
 
 	// ambient occlusion
@@ -1038,29 +1118,19 @@ extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, 
 	if (doc.animation) flags |= MBX_FRACT_ANIMATION;
 	if (M.b("antialiasing_enabled", false) && M.i("antialiasing_size", 2, 1, 10) > 1) flags |= MBX_FRACT_ANTIALIASING;
 
-	// ---- the fractal (sFractal, fractal.cpp:41-120; sequence-wide values, fractparams.cpp) ----
-	{
-		mbx_fractal &fr = p.fractal;
-		const View fv = F(objectId);
-		fr.formula = formula_of(objectId);
-		fr.formulaMaxiter = maxiter;
-		fr.deltaDEMethod = (int)M.i("delta_DE_method", 0, 0, 2);
-		fr.deltaDEFunction = (int)M.i("delta_DE_function", 0, 0, 6);
-		// settings.cpp:1500-1505: before 2.07 every non-linear DE function became logarithmic;
-		// settings.cpp:1596-1606: before 2.20 "preferred" method meant "preferred" function
-		if (ver <= 2.06 && fr.deltaDEFunction != MBX_DE_FUNCTION_LINEAR) fr.deltaDEFunction = MBX_DE_FUNCTION_LOGARITHMIC;
-		if (ver < 2.20 && fr.deltaDEMethod == 0) fr.deltaDEFunction = 0;
-		fr.useDefaultBailout = M.b("use_default_bailout", true);
-		fr.bailout = M.d("bailout", 100.0, 1.0, 1e15);
-		fr.juliaMode = juliaMode;
-		std::memcpy(fr.juliaC, juliaC, sizeof(juliaC));
-		std::memcpy(fr.constantFactor, constFactor, sizeof(constFactor));
-		fr.initialWAxis = initialW;
+	// ---- the fractal containers (sFractal, fractal.cpp:41-120) ---------------------------------
+	auto read_container = [&](int n, mbx_fractal &fr) {
+		const View fv = F(n);
+		fr.formula = formula_of(n);
+		fr.formulaIterations = (int)fv.i("formula_iterations", 1, 1, 65536);
+		fr.formulaWeight = fv.d("formula_weight", 1.0, 0.0, 1.0);
+		fr.formulaStartIteration = (int)fv.i("formula_start_iteration", 0, 0, 65536);
+		fr.formulaStopIteration = (int)fv.i("formula_stop_iteration", 250, 0, 65536);
+		fr.checkForBailout = fv.b("check_for_bailout", true);
 		fr.dontAddCConstant = fv.b("dont_add_c_constant", false);
 		// MigrateLegacyParamsToFractal (settings.cpp:2328-2340): a true main-section
 		// dont_add_c_constant_N sets fractal N's flag
-		if (ver < 2.35 && objectId <= 8 && M.b("dont_add_c_constant_" + std::to_string(objectId), false))
-			fr.dontAddCConstant = 1;
+		if (ver < 2.35 && n <= 8 && M.b("dont_add_c_constant_" + std::to_string(n), false)) fr.dontAddCConstant = 1;
 		fr.bulb.power = fv.d("power", 9.0);
 		fr.bulb.alphaAngleOffset = fv.d("alpha_angle_offset", 0.0);
 		fr.bulb.betaAngleOffset = fv.d("beta_angle_offset", 0.0);
@@ -1104,11 +1174,63 @@ extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, 
 		ifs.rotationEnabled = fv.b("IFS_rotation_enabled", false);
 		ifs.edgeEnabled = fv.b("IFS_edge_enabled", false);
 		// settings.cpp:1507-1513, 1629-1641
-		if (ver <= 2.06 && objectId <= 4) ifs.rotationEnabled = 1;
+		if (ver <= 2.06 && n <= 4) ifs.rotationEnabled = 1;
 		if (ver < 2.22 && (ifs.edge[0] != 0.0 || ifs.edge[1] != 0.0 || ifs.edge[2] != 0.0)) ifs.edgeEnabled = 1;
-		if (fr.formula != MBX_FORMULA_MANDELBULB && fr.formula != MBX_FORMULA_MANDELBOX &&
-			fr.formula != MBX_FORMULA_KALEIDOSCOPIC_IFS)
-			flags |= fr.formula == MBX_FORMULA_NONE ? MBX_FRACT_DISABLED : MBX_FRACT_FORMULA;
+		mbx_fractal_transform &t = fr.transform;
+		t.scale3 = fv.d("transf_scale_3", 3.0);
+		t.aexionCadd = fv.d("cadd", -1.3);
+		t.foldingIntPowFoldFactor = fv.d("boxfold_bulbpow2_folding_factor", 2.0);
+		t.foldingIntPowZFactor = fv.d("boxfold_bulbpow2_z_factor", 5.0);
+		t.analyticDEEnabled = fv.b("analyticDE_enabled", true);
+		t.analyticDEEnabledFalse = fv.b("analyticDE_enabled_false", false);
+		t.analyticDEScale1 = fv.d("analyticDE_scale_1", 1.0);
+		t.analyticDEOffset1 = fv.d("analyticDE_offset_1", 1.0);
+		t.analyticDEOffset2 = fv.d("analyticDE_offset_2", 1.0);
+		// what the device path does not render: formulas outside the table, rotated Mandelbox folds,
+		// the Menger sponge's fold colour with palette colouring
+		static const int kTable[] = {MBX_FORMULA_MANDELBULB, MBX_FORMULA_MANDELBULB_POWER2, MBX_FORMULA_HYPERCOMPLEX,
+			MBX_FORMULA_QUATERNION, MBX_FORMULA_MENGER_SPONGE, MBX_FORMULA_MANDELBOX, MBX_FORMULA_KALEIDOSCOPIC_IFS,
+			MBX_FORMULA_XENODREAMBUIE, MBX_FORMULA_BOX_FOLD_BULB_POW2, MBX_FORMULA_AEXION};
+		bool known = false;
+		for (int id : kTable) known |= fr.formula == id;
+		if (fr.formula != MBX_FORMULA_NONE && !known) flags |= MBX_FRACT_FORMULA;
+		if (fr.formula == MBX_FORMULA_MANDELBOX && (mb.rotationsEnabled || mb.mainRotationEnabled))
+			flags |= MBX_FRACT_FORMULA;
+		if (fr.formula == MBX_FORMULA_MENGER_SPONGE && p.useColorsFromPalette &&
+			fv.b("fold_color_aux_color_enabled_false", false))
+			flags |= MBX_FRACT_PALETTE;
+	};
+	{
+		// the rendered container, or the first of the hybrid's
+		mbx_fractal &fr = p.fractal;
+		read_container(objectId, fr);
+		if (fr.formula == MBX_FORMULA_NONE) flags |= MBX_FRACT_DISABLED;
+		if (hybridNode && !hybridIds.empty())
+		{
+			p.hybridCount = (int)hybridIds.size();
+			for (size_t k = 1; k < hybridIds.size(); k++) read_container(hybridIds[k], p.hybridFractals[k - 1]);
+			p.repeatFrom = (int)M.i("repeat_from", 1, 1, 9);
+			// a hybrid with palette colouring and the extra colour options uses scales this path
+			// does not apply (fractal_coloring.cpp:309-315)
+			if (p.useColorsFromPalette && M.b(std::string("mat") + std::to_string(material) +
+						"_fractal_coloring_extra_color_options_false", false))
+				flags |= MBX_FRACT_PALETTE;
+		}
+		p.linearDEOffset = M.d("linear_DE_offset", 0.0);
+		// sequence-wide values (fractparams.cpp, hybrid_fractal_sequences.cpp:181-287)
+		fr.formulaMaxiter = maxiter;
+		fr.deltaDEMethod = (int)M.i("delta_DE_method", 0, 0, 2);
+		fr.deltaDEFunction = (int)M.i("delta_DE_function", 0, 0, 6);
+		// settings.cpp:1500-1505: before 2.07 every non-linear DE function became logarithmic;
+		// settings.cpp:1596-1606: before 2.20 "preferred" method meant "preferred" function
+		if (ver <= 2.06 && fr.deltaDEFunction != MBX_DE_FUNCTION_LINEAR) fr.deltaDEFunction = MBX_DE_FUNCTION_LOGARITHMIC;
+		if (ver < 2.20 && fr.deltaDEMethod == 0) fr.deltaDEFunction = 0;
+		fr.useDefaultBailout = M.b("use_default_bailout", true);
+		fr.bailout = M.d("bailout", 100.0, 1.0, 1e15);
+		fr.juliaMode = juliaMode;
+		std::memcpy(fr.juliaC, juliaC, sizeof(juliaC));
+		std::memcpy(fr.constantFactor, constFactor, sizeof(constFactor));
+		fr.initialWAxis = initialW;
 		std::memcpy(p.formulaRotation, rotation, sizeof(rotation));
 	}
 

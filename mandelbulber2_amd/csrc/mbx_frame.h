@@ -23,8 +23,35 @@ enum DEKind : int32_t
 	kBoxDelta = 3,   // Mandelbox, delta DE (reference finite differences, or tangents)
 	kIfsAnalytic = 4,
 	kBulbD = 5,      // Mandelbulb any power, analytic DE, FP64 position and orbit (deep zooms)
-	kNumDEKinds = 6
+	kGeneric = 6,    // any formula of the table and hybrid sequences: FP64, 4D orbit, per-iteration
+	                 // formula from the sequence
+	kNumDEKinds = 7
 };
+
+// One container of the general formula path (kGeneric): cHybridFractalSequences::sFractalData
+// (hybrid_fractal_sequences.h:30-56) with its sFractal after RecalculateFractalParams
+// (fractal.cpp:653-678), in FP64.  An identity rotation is flagged: the reference skips the
+// multiply for it (CRotationMatrix's zero flag).
+struct GenSlot
+{
+	int32_t formula, addC, checkBailout, ifsMainRotId;
+	double weight, bailout;
+	double power, alpha, beta; // Mandelbulb, Xenodreambuie (radians)
+	double mbScale, mbAbsScale, mbFoldingLimit, mbFoldingValue, mbFR2, mbMR2, mbFactor1;
+	double mbOffset[3], mbColor[3], mbSp1, mbSp2;
+	double ifsScale, ifsOffset[3], ifsEdge[3];
+	double ifsMainRot[9];
+	double ifsRot[MBX_IFS_VECTOR_COUNT][9];
+	double ifsDirection[MBX_IFS_VECTOR_COUNT][4];
+	double ifsDistance[MBX_IFS_VECTOR_COUNT], ifsIntensity[MBX_IFS_VECTOR_COUNT];
+	int32_t ifsEnabled[MBX_IFS_VECTOR_COUNT], ifsRotId[MBX_IFS_VECTOR_COUNT];
+	int32_t ifsAbs[3], ifsMenger, ifsRotationEnabled, ifsEdgeEnabled;
+	int32_t adeEnabled, adeEnabledFalse;
+	double scale3, cadd, fipFoldFactor, fipZFactor, adeScale1, adeOffset1, adeOffset2;
+};
+
+// formula of each iteration: maxN * 5 entries with maxN = 250 (hybrid_fractal_sequences.cpp:294-298)
+constexpr int kGenSeqLength = 1250;
 
 struct FrameConst
 {
@@ -53,6 +80,11 @@ struct FrameConst
 	float smoothness;
 	int32_t maxRaymarchingSteps;
 	int32_t minN;
+	// advanced quality (render_worker.cpp:671-675, 817-834): threshold and step limits
+	int32_t advancedQuality;
+	float absMaxStep, absMinStep, relMaxStep, relMinStep, detailSizeMax, detailSizeMin;
+	double absMaxStepD, absMinStepD, relMaxStepD, relMinStepD, detailSizeMaxD, detailSizeMinD;
+	double deltaRel; // delta-DE offset per detail size: 1e-5, or deltade_relative_delta with advanced quality
 
 	// fractal sequence (hybrid_fractal_sequences.cpp:181-287, 377-503)
 	int32_t deKind;
@@ -158,6 +190,18 @@ struct FrameConst
 	int32_t nodeRotated;
 	float nodeRot[9];
 	double nodeRotD[9];
+
+	// general formula path (kGeneric): the sequence (cHybridFractalSequences::sSequence,
+	// hybrid_fractal_sequences.cpp:290-503)
+	int32_t genSlots, genHybrid, genSeqLength;
+	int32_t genDEType;     // MBX_DE_ANALYTIC / MBX_DE_DELTA
+	int32_t genDEFunction; // hybrid and delta finish: 1 linear, 2 logarithmic
+	int32_t genAnalytic;   // single-formula analytic finish: 1 linear, 2 log, 3 IFS
+	int32_t genColoring;   // single-formula colour index: 0 default, 1 Mandelbox, 2 IFS
+	int32_t genPad;
+	double genInitialW, genLinearDEOffset;
+	uint8_t genSeq[kGenSeqLength];
+	GenSlot gen[MBX_MAX_FRACTALS];
 };
 
 // Per-pixel seed of the Park-Miller stream (identical on host and device).

@@ -1,0 +1,524 @@
+// mbx_generic.h - the general formula path (DE kind kGeneric): every formula of the table, alone
+// or in a hybrid sequence, in FP64 with a 4D orbit and the reference's operation order
+// (no FMA contraction).  The formula of iteration i is FrameConst::genSeq[i]; its container
+// (parameters, weight, bailout, c addition) is FrameConst::gen[].  Included by mbx_device.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mbx_frame.h"
+
+namespace mbx
+{
+
+struct DV4
+{
+	double x, y, z, w;
+};
+__device__ __forceinline__ DV4 dv4(double x, double y, double z, double w) { return DV4{x, y, z, w}; }
+__device__ __forceinline__ double dot4(DV4 a, DV4 b)
+{
+#pragma clang fp contract(off)
+	return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
+}
+// CVector4::Length (algebra.hpp:401)
+__device__ __forceinline__ double len4(DV4 a) { return sqrt(dot4(a, a)); }
+__device__ __forceinline__ bool finite4(DV4 a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z) && isfinite(a.w); }
+
+// sExtendedAux members the formulas of the table use (fractal.h:77-104)
+struct GenAux
+{
+	DV4 c; // const_c
+	double r, DE, color;
+	int i;
+};
+
+// 3x3 rotation of xyz, w kept; identity matrices skip the multiply (algebra.cpp:320-344)
+__device__ __forceinline__ DV4 rot4(const double *m, int identity, DV4 v)
+{
+#pragma clang fp contract(off)
+	if (identity) return v;
+	return dv4(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
+		m[6] * v.x + m[7] * v.y + m[8] * v.z, v.w);
+}
+
+// ---- formulas -------------------------------------------------------------------------------
+
+// cFractalMandelbulb::FormulaCode (fractal_mandelbulb.cpp:29-43)
+__device__ __forceinline__ void gen_mandelbulb(const GenSlot &s, DV4 &z, GenAux &a)
+{
+#pragma clang fp contract(off)
+	const double th0 = asin(z.z / a.r) + s.beta;
+	const double ph0 = atan2(z.y, z.x) + s.alpha;
+	double rp = pow(a.r, s.power - 1.0);
+	const double th = th0 * s.power;
+	const double ph = ph0 * s.power;
+	const double cth = cos(th);
+	a.DE = (rp * a.DE) * s.power + 1.0;
+	rp *= a.r;
+	z.x = cth * cos(ph) * rp;
+	z.y = cth * sin(ph) * rp;
+	z.z = sin(th) * rp;
+}
+
+// cFractalMandelbulbPower2::FormulaCode (fractal_mandelbulb_power2.cpp:28-43)
+__device__ __forceinline__ void gen_mandelbulb_power2(DV4 &z, GenAux &a)
+{
+#pragma clang fp contract(off)
+	a.DE = a.DE * 2.0 * a.r;
+	const double x2 = z.x * z.x, y2 = z.y * z.y, z2 = z.z * z.z;
+	const double temp = 1.0 - z2 / (x2 + y2);
+	const double newx = (x2 - y2) * temp;
+	const double newy = 2.0 * z.x * z.y * temp;
+	const double newz = -2.0 * z.z * sqrt(x2 + y2);
+	z.x = newx;
+	z.y = newy;
+	z.z = newz;
+}
+
+// cFractalHypercomplex::FormulaCode (fractal_hypercomplex.cpp:28-41)
+__device__ __forceinline__ void gen_hypercomplex(DV4 &z, GenAux &a)
+{
+#pragma clang fp contract(off)
+	a.DE = a.DE * 2.0 * a.r;
+	const double newx = z.x * z.x - z.y * z.y - z.z * z.z - z.w * z.w;
+	const double newy = 2.0 * z.x * z.y - 2.0 * z.w * z.z;
+	const double newz = 2.0 * z.x * z.z - 2.0 * z.y * z.w;
+	const double neww = 2.0 * z.x * z.w - 2.0 * z.y * z.z;
+	z = dv4(newx, newy, newz, neww);
+}
+
+// cFractalQuaternion::FormulaCode (fractal_quaternion.cpp:28-39)
+__device__ __forceinline__ void gen_quaternion(DV4 &z, GenAux &a)
+{
+#pragma clang fp contract(off)
+	a.DE = a.DE * 2.0 * a.r;
+	const double newx = z.x * z.x - z.y * z.y - z.z * z.z;
+	const double newy = 2.0 * z.x * z.y;
+	const double newz = 2.0 * z.x * z.z;
+	z.x = newx;
+	z.y = newy;
+	z.z = newz;
+}
+
+// cFractalMengerSponge::FormulaCode (fractal_menger_sponge.cpp:28-71), fold colour off
+__device__ __forceinline__ void gen_menger_sponge(const GenSlot &s, DV4 &z, GenAux &a)
+{
+#pragma clang fp contract(off)
+	z = dv4(fabs(z.x), fabs(z.y), fabs(z.z), fabs(z.w));
+	double t;
+	if (z.x < z.y)
+	{
+		t = z.y;
+		z.y = z.x;
+		z.x = t;
+	}
+	if (z.x < z.z)
+	{
+		t = z.z;
+		z.z = z.x;
+		z.x = t;
+	}
+	if (z.y < z.z)
+	{
+		t = z.z;
+		z.z = z.y;
+		z.y = t;
+	}
+	z = dv4(z.x * s.scale3, z.y * s.scale3, z.z * s.scale3, z.w * s.scale3);
+	z.x -= 2.0;
+	z.y -= 2.0;
+	if (z.z > 1.0) z.z -= 2.0;
+	a.DE *= s.scale3;
+}
+
+// cFractalMandelbox::FormulaCode (fractal_mandelbox.cpp:31-107) without rotated folds, with the
+// fold colour of aux.color
+__device__ __forceinline__ void gen_mandelbox(const GenSlot &s, DV4 &z, GenAux &a)
+{
+#pragma clang fp contract(off)
+	const double L = s.mbFoldingLimit, V = s.mbFoldingValue;
+	if (fabs(z.x) > L)
+	{
+		z.x = dsgn(z.x) * V - z.x;
+		a.color += s.mbColor[0];
+	}
+	if (fabs(z.y) > L)
+	{
+		z.y = dsgn(z.y) * V - z.y;
+		a.color += s.mbColor[1];
+	}
+	if (fabs(z.z) > L)
+	{
+		z.z = dsgn(z.z) * V - z.z;
+		a.color += s.mbColor[2];
+	}
+	const double r2 = dot4(z, z);
+	z = dv4(z.x + s.mbOffset[0], z.y + s.mbOffset[1], z.z + s.mbOffset[2], z.w + 0.0);
+	if (r2 < s.mbMR2)
+	{
+		z = dv4(z.x * s.mbFactor1, z.y * s.mbFactor1, z.z * s.mbFactor1, z.w * s.mbFactor1);
+		a.DE *= s.mbFactor1;
+		a.color += s.mbSp1;
+	}
+	else if (r2 < s.mbFR2)
+	{
+		const double t = s.mbFR2 / r2;
+		z = dv4(z.x * t, z.y * t, z.z * t, z.w * t);
+		a.DE *= t;
+		a.color += s.mbSp2;
+	}
+	z = dv4(z.x - s.mbOffset[0], z.y - s.mbOffset[1], z.z - s.mbOffset[2], z.w - 0.0);
+	z = dv4(z.x * s.mbScale, z.y * s.mbScale, z.z * s.mbScale, z.w * s.mbScale);
+	a.DE = a.DE * s.mbAbsScale + 1.0;
+}
+
+// cFractalKaleidoscopicIfs::FormulaCode (fractal_kaleidoscopic_ifs.cpp:30-74)
+__device__ __forceinline__ void gen_kifs(const GenSlot &s, DV4 &z, GenAux &a)
+{
+#pragma clang fp contract(off)
+	if (s.ifsAbs[0]) z.x = fabs(z.x);
+	if (s.ifsAbs[1]) z.y = fabs(z.y);
+	if (s.ifsAbs[2]) z.z = fabs(z.z);
+	for (int i = 0; i < MBX_IFS_VECTOR_COUNT; i++)
+	{
+		if (!s.ifsEnabled[i]) continue;
+		z = rot4(s.ifsRot[i], s.ifsRotId[i], z);
+		const DV4 d = dv4(s.ifsDirection[i][0], s.ifsDirection[i][1], s.ifsDirection[i][2], s.ifsDirection[i][3]);
+		const double length = dot4(z, d);
+		if (length < s.ifsDistance[i])
+		{
+			const double k = 2.0 * (length - s.ifsDistance[i]) * s.ifsIntensity[i];
+			z = dv4(z.x - d.x * k, z.y - d.y * k, z.z - d.z * k, z.w - d.w * k);
+		}
+	}
+	const DV4 off = dv4(s.ifsOffset[0], s.ifsOffset[1], s.ifsOffset[2], 0.0);
+	if (s.ifsRotationEnabled)
+	{
+		DV4 t = rot4(s.ifsMainRot, s.ifsMainRotId, dv4(z.x - off.x, z.y - off.y, z.z - off.z, z.w - off.w));
+		z = dv4(t.x + off.x, t.y + off.y, t.z + off.z, t.w + off.w);
+	}
+	if (s.ifsEdgeEnabled)
+	{
+		if (s.ifsEdge[0] > 0.0) z.x = s.ifsEdge[0] - fabs(s.ifsEdge[0] - z.x);
+		if (s.ifsEdge[1] > 0.0) z.y = s.ifsEdge[1] - fabs(s.ifsEdge[1] - z.y);
+		if (s.ifsEdge[2] > 0.0) z.z = s.ifsEdge[2] - fabs(s.ifsEdge[2] - z.z);
+	}
+	z = dv4(z.x * s.ifsScale, z.y * s.ifsScale, z.z * s.ifsScale, z.w * s.ifsScale);
+	const double sm1 = s.ifsScale - 1.0;
+	if (s.ifsMenger)
+	{
+		z.x -= off.x * sm1;
+		z.y -= off.y * sm1;
+		if (z.z > 0.5 * off.z * sm1) z.z -= off.z * sm1;
+	}
+	else
+		z = dv4(z.x - off.x * sm1, z.y - off.y * sm1, z.z - off.z * sm1, z.w - off.w * sm1);
+	a.DE *= fabs(s.ifsScale);
+}
+
+// cFractalXenodreambuie::FormulaCode (fractal_xenodreambuie.cpp:28-47)
+__device__ __forceinline__ void gen_xenodreambuie(const GenSlot &s, DV4 &z, GenAux &a)
+{
+#pragma clang fp contract(off)
+	double rp = pow(a.r, s.power - 1.0);
+	a.DE = rp * a.DE * s.power + 1.0;
+	rp *= a.r;
+	const double th = (atan2(z.y, z.x) + s.beta) * s.power;
+	double ph = acos(z.z / a.r) + s.alpha;
+	if (fabs(ph) > 0.5 * M_PI) ph = dsgn(ph) * M_PI - ph;
+	ph *= s.power;
+	const double sph = sin(ph);
+	z.x = rp * cos(th) * sph;
+	z.y = rp * sin(th) * sph;
+	z.z = rp * cos(ph);
+}
+
+// cFractalBoxFoldBulbPow2::FormulaCode (fractal_box_fold_bulb_pow2.cpp:28-78)
+__device__ __forceinline__ void gen_box_fold_bulb_pow2(const GenSlot &s, DV4 &z, GenAux &a)
+{
+#pragma clang fp contract(off)
+	const double ff = s.fipFoldFactor;
+	if (fabs(z.x) > ff) z.x = dsgn(z.x) * ff * 2.0 - z.x;
+	if (fabs(z.y) > ff) z.y = dsgn(z.y) * ff * 2.0 - z.y;
+	if (fabs(z.z) > ff) z.z = dsgn(z.z) * ff * 2.0 - z.z;
+	const double fR2 = 1.0, mR2 = 0.25;
+	const double r2 = dot4(z, z);
+	const double factor1 = fR2 / mR2;
+	if (r2 < mR2)
+	{
+		z = dv4(z.x * factor1, z.y * factor1, z.z * factor1, z.w * factor1);
+		a.DE *= factor1;
+	}
+	else if (r2 < fR2)
+	{
+		const double factor2 = fR2 / r2;
+		z = dv4(z.x * factor2, z.y * factor2, z.z * factor2, z.w * factor2);
+		a.DE *= factor2;
+	}
+	z = dv4(z.x * 2.0, z.y * 2.0, z.z * 2.0, z.w * 2.0);
+	const double x2 = z.x * z.x, y2 = z.y * z.y, z2 = z.z * z.z;
+	const double temp = 1.0 - z2 / (x2 + y2);
+	DV4 t;
+	t.x = (x2 - y2) * temp;
+	t.y = 2.0 * z.x * z.y * temp;
+	t.z = -2.0 * z.z * sqrt(x2 + y2);
+	t.w = z.w;
+	z = t;
+	z.z *= s.fipZFactor;
+	if (s.adeEnabledFalse)
+		a.DE = (a.DE + 1.0) * 5.0 * a.r * s.adeScale1 * sqrt(s.fipZFactor * s.fipZFactor + 2.0 + s.adeOffset2) +
+			s.adeOffset1;
+}
+
+// cFractalAexion::FormulaCode (fractal_aexion.cpp:28-64): the first iteration replaces the
+// constant with a folded copy of the point
+__device__ __forceinline__ void gen_aexion(const GenSlot &s, DV4 &z, GenAux &a)
+{
+#pragma clang fp contract(off)
+	DV4 temp = z;
+	double t;
+	if (a.i == 0)
+	{
+		t = s.cadd;
+		temp.x = z.x + z.y + z.z;
+		temp.y = -z.x - z.y + z.z;
+		temp.z = -z.x + z.y - z.z;
+		temp.w = z.x - z.y - z.z;
+		z = dv4(fabs(temp.x) + t, fabs(temp.y) + t, fabs(temp.z) + t, fabs(temp.w) + t);
+		a.c = z;
+	}
+	t = 2.0 * z.w * z.z;
+	temp.x = z.x * z.x - z.y * z.y;
+	temp.y = t - temp.x;
+	temp.x += t;
+	t = 2.0 * z.x * z.y;
+	temp.z = z.z * z.z - z.w * z.w;
+	temp.w = t - temp.z;
+	temp.z += t;
+	z = dv4(temp.x + a.c.x, temp.y + a.c.y, temp.z + a.c.z, temp.w + a.c.w);
+	if (s.adeEnabled)
+	{
+		double de1 = 1.1 * a.r;
+		de1 = de1 + (len4(z) / a.r - de1) * s.adeScale1;
+		a.DE = de1 * 2.0 * a.DE + s.adeOffset1;
+	}
+}
+
+__device__ __forceinline__ void gen_formula(const GenSlot &s, DV4 &z, GenAux &a)
+{
+	switch (s.formula)
+	{
+		case MBX_FORMULA_MANDELBULB: gen_mandelbulb(s, z, a); break;
+		case MBX_FORMULA_MANDELBULB_POWER2: gen_mandelbulb_power2(z, a); break;
+		case MBX_FORMULA_HYPERCOMPLEX: gen_hypercomplex(z, a); break;
+		case MBX_FORMULA_QUATERNION: gen_quaternion(z, a); break;
+		case MBX_FORMULA_MENGER_SPONGE: gen_menger_sponge(s, z, a); break;
+		case MBX_FORMULA_MANDELBOX: gen_mandelbox(s, z, a); break;
+		case MBX_FORMULA_KALEIDOSCOPIC_IFS: gen_kifs(s, z, a); break;
+		case MBX_FORMULA_XENODREAMBUIE: gen_xenodreambuie(s, z, a); break;
+		case MBX_FORMULA_BOX_FOLD_BULB_POW2: gen_box_fold_bulb_pow2(s, z, a); break;
+		case MBX_FORMULA_AEXION: gen_aexion(s, z, a); break;
+		default: break;
+	}
+}
+
+// ---- one iteration of the sequence (compute_fractal.cpp:170-321) ------------------------------
+
+// formula, z + c, hybrid weight; returns the iteration's container
+__device__ __forceinline__ const GenSlot &gen_step(const FrameConst &f, int i, DV4 &z, GenAux &a)
+{
+#pragma clang fp contract(off)
+	const GenSlot &s = f.gen[i < f.genSeqLength ? f.genSeq[i] : f.genSeq[0]];
+	const DV4 tempZ = z;
+	const double tempDE = a.DE, tempColor = a.color;
+	a.i = i;
+	if (s.weight > 0.0) gen_formula(s, z, a);
+	if (s.addC)
+	{
+		if (f.julia)
+			z = dv4(z.x + f.juliaCD[0], z.y + f.juliaCD[1], z.z + f.juliaCD[2], z.w + 0.0);
+		else
+			z = dv4(z.x + a.c.x * f.constMulD[0], z.y + a.c.y * f.constMulD[1], z.z + a.c.z * f.constMulD[2],
+				z.w + a.c.w);
+	}
+	if (f.genHybrid && s.weight < 1.0)
+	{
+		// SmoothCVector (common_math.cpp:132-163) and the linear DE / colour blend
+		const double k = s.weight, kn = 1.0 - k;
+		if (k <= 0.0)
+			z = tempZ;
+		else
+		{
+			const double lenInterp = len4(tempZ) * kn + len4(z) * k;
+			const DV4 v = dv4(tempZ.x * kn + z.x * k, tempZ.y * kn + z.y * k, tempZ.z * kn + z.z * k,
+				tempZ.w * kn + z.w * k);
+			const double lv = len4(v);
+			z = (lv > 0.0) ? dv4(v.x / lv * lenInterp, v.y / lv * lenInterp, v.z / lv * lenInterp, v.w / lv * lenInterp)
+							: tempZ;
+		}
+		a.DE = a.DE * k + tempDE * kn;
+		a.color = a.color * k + tempColor * kn;
+	}
+	a.r = len4(z);
+	return s;
+}
+
+// Compute<calcModeNormal / calcModeDeltaDE1> (compute_fractal.cpp:67-654): the orbit of p with
+// at most maxN iterations; returns the iteration count (i + 1) and leaves z and aux
+__device__ __forceinline__ int gen_orbit(const FrameConst &f, DV3 p, int maxN, DV4 &zOut, GenAux &a)
+{
+	DV4 z = dv4(p.x, p.y, p.z, f.genInitialW);
+	a.c = z;
+	a.r = len4(z);
+	a.DE = 1.0;
+	a.color = 1.0;
+	a.i = 0;
+	int i;
+	for (i = 0; i < maxN; i++)
+	{
+		const DV4 lastZ = z;
+		const GenSlot &s = gen_step(f, i, z, a);
+		if (!finite4(z))
+		{
+			z = lastZ;
+			a.r = len4(z);
+			break;
+		}
+		if (s.checkBailout && a.r > s.bailout) break;
+	}
+	zOut = z;
+	return i + 1;
+}
+
+// The distance of the sequence at the fractal-local point p: the analytic finish
+// (compute_fractal.cpp:493-617) or the reference's 4-orbit delta DE (calculate_distance.cpp:609-711).
+// One out-of-line copy serves every call site of the kernels.
+__device__ __attribute__((noinline)) double compute_generic(DV3 p, const FrameConst &f, double detailSize,
+	int *itersOut, uint32_t *work)
+{
+#pragma clang fp contract(off)
+	GenAux a;
+	DV4 z;
+	const int iters = gen_orbit(f, p, f.formulaMaxiter, z, a);
+	*itersOut = iters;
+	if (f.genDEType == MBX_DE_ANALYTIC)
+	{
+		*work += (uint32_t)iters;
+		const double r = a.r;
+		if (!(a.DE > 0.0)) return r;
+		if (f.genHybrid)
+		{
+			if (f.genDEFunction == MBX_DE_FUNCTION_LINEAR) return (r - f.genLinearDEOffset) / a.DE;
+			return (r > 1.0) ? 0.5 * r * log(r) / a.DE : 0.0;
+		}
+		switch (f.genAnalytic)
+		{
+			case MBX_ANALYTIC_LOGARITHMIC: return (r > 1.0) ? 0.5 * r * log(r) / a.DE : 0.0;
+			case MBX_ANALYTIC_LINEAR: return r / a.DE;
+			case MBX_ANALYTIC_IFS: return (r - 2.0) / a.DE;
+			default: return r;
+		}
+	}
+	const double deltaDE = fmax(sqrt(ddot(p, p)) * 1e-14, f.deltaRel * detailSize);
+	const double r = sqrt(z.x * z.x + z.y * z.y + z.z * z.z);
+	GenAux b;
+	DV4 z1, z2, z3;
+	const int i1 = gen_orbit(f, dv3(p.x + deltaDE, p.y, p.z), iters, z1, b);
+	const int i2 = gen_orbit(f, dv3(p.x, p.y + deltaDE, p.z), iters, z2, b);
+	const int i3 = gen_orbit(f, dv3(p.x, p.y, p.z + deltaDE), iters, z3, b);
+	*work += (uint32_t)(iters + i1 + i2 + i3);
+	const double dr1 = fabs(sqrt(z1.x * z1.x + z1.y * z1.y + z1.z * z1.z) - r) / deltaDE;
+	const double dr2 = fabs(sqrt(z2.x * z2.x + z2.y * z2.y + z2.z * z2.z) - r) / deltaDE;
+	const double dr3 = fabs(sqrt(z3.x * z3.x + z3.y * z3.y + z3.z * z3.z) - r) / deltaDE;
+	const double dr = sqrt(dr1 * dr1 + dr2 * dr2 + dr3 * dr3);
+	if (dr > 0.0)
+		return (f.genDEFunction == MBX_DE_FUNCTION_LOGARITHMIC) ? 0.5 * r * log(r) / dr : 0.5 * r / dr;
+	return r;
+}
+
+// Compute<calcModeColouring> + CalculateColorIndex of the sequence (compute_fractal.cpp:343-436,
+// fractal_coloring.cpp:292-340) at the fractal-local point p
+__device__ __attribute__((noinline)) double colour_index_generic(DV3 p, const FrameConst &f)
+{
+#pragma clang fp contract(off)
+	DV4 z = dv4(p.x, p.y, p.z, f.genInitialW);
+	const DV4 start = z;
+	GenAux a;
+	a.c = z;
+	a.r = len4(z);
+	a.DE = 1.0;
+	a.color = 1.0;
+	a.i = 0;
+	double colorMin = 1000.0;
+	const int alg = f.colorAlgorithm;
+	const bool pre = f.colorPreV215 != 0;
+	const int maxN = 4 * f.formulaMaxiter;
+	for (int i = 0; i < maxN; i++)
+	{
+		const DV4 lastZ = z;
+		const GenSlot &s = gen_step(f, i, z, a);
+		if (!finite4(z))
+		{
+			z = lastZ;
+			a.r = len4(z);
+			break;
+		}
+		DV4 cz = z;
+		if (!f.color4D) cz.w = 0.0;
+		double len;
+		switch (alg)
+		{
+			case 0: len = len4(cz); break;
+			case 1: len = fabs(dot4(cz, start)); break;
+			case 2: len = fabs(len4(dv4(cz.x - start.x, cz.y - start.y, cz.z - start.z, cz.w - start.w)) - f.colorSphereRadiusD); break;
+			case 3:
+			{
+				const double ax = fabs(cz.x), ay = fabs(cz.y), az = fabs(cz.z);
+				len = (ax < ay) ? (ax < az ? ax : az) : (ay < az ? ay : az);
+				if (f.color4D && fabs(cz.w) < len) len = fabs(cz.w);
+				break;
+			}
+			default: len = fabs(cz.x * f.colorLineDirD[0] + cz.y * f.colorLineDirD[1] + cz.z * f.colorLineDirD[2]); break;
+		}
+		const DV4 dz = dv4(z.x - lastZ.x, z.y - lastZ.y, z.z - lastZ.z, z.w - lastZ.w);
+		const double change = len4(dz) / a.r;
+		if (!s.checkBailout) continue;
+		const bool box = s.formula == MBX_FORMULA_MANDELBOX;
+		if (!pre)
+		{
+			if (!box)
+			{
+				if (len < colorMin) colorMin = len;
+				if (a.r > s.bailout) break;
+			}
+			else if (alg == 0)
+			{
+				if (a.r > 1e15 || change < 1e-15) break;
+			}
+			else
+			{
+				if (len < colorMin) colorMin = len;
+				if (a.r > s.bailout || change < 1e-15) break;
+			}
+		}
+		else if (!box || alg != 0)
+		{
+			if (len < colorMin) colorMin = len;
+			if (a.r > 1e15 || change < 1e-15) break;
+		}
+		else if (a.r > 1e15 || change < 1e-15)
+			break;
+	}
+	if (f.genHybrid)
+	{
+		colorMin = fmin(100.0, colorMin);
+		const double r2 = fmin(a.r / fabs(a.DE), 20.0);
+		return colorMin * 1000.0 + a.color * 100.0 + r2 * 5000.0;
+	}
+	if (f.genColoring == 1) return a.color * 100.0 + a.r * f.mbColorRD / 1e13 + ((alg != 0) ? colorMin * 1000.0 : 0.0);
+	if (f.genColoring == 2) return colorMin * 1000.0;
+	return colorMin * 5000.0;
+}
+
+} // namespace mbx

@@ -413,6 +413,261 @@ double host_bulb_de(const double p[3], const mbx_fractal &fr, double bailout, in
 	return r / DE;
 }
 
+// cAbstractFractal metadata of the formulas of the general path (the constructors of
+// formula/definition/fractal_<name>.cpp): DE type, DE function, c addition (0 enabled by default,
+// 1 disabled by default, 2 the formula adds its own), default bailout, analytic finish, colour
+// index function (0 default, 1 Mandelbox, 2 IFS)
+struct FormulaMeta
+{
+	int id, deType, deFunction, cpixel;
+	double defaultBailout;
+	int analytic, coloring;
+};
+constexpr FormulaMeta kFormulaTable[] = {
+	{MBX_FORMULA_MANDELBULB, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LOGARITHMIC, 0, 10.0, MBX_ANALYTIC_LOGARITHMIC, 0},
+	{MBX_FORMULA_MANDELBULB_POWER2, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LOGARITHMIC, 0, 10.0, MBX_ANALYTIC_LOGARITHMIC, 0},
+	{MBX_FORMULA_HYPERCOMPLEX, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LOGARITHMIC, 0, 10.0, MBX_ANALYTIC_LOGARITHMIC, 0},
+	{MBX_FORMULA_QUATERNION, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LOGARITHMIC, 0, 10.0, MBX_ANALYTIC_LOGARITHMIC, 0},
+	{MBX_FORMULA_MENGER_SPONGE, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LINEAR, 1, 10.0, MBX_ANALYTIC_IFS, 2},
+	{MBX_FORMULA_MANDELBOX, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LINEAR, 0, 100.0, MBX_ANALYTIC_LINEAR, 1},
+	{MBX_FORMULA_KALEIDOSCOPIC_IFS, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LINEAR, 1, 100.0, MBX_ANALYTIC_IFS, 2},
+	{MBX_FORMULA_XENODREAMBUIE, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LOGARITHMIC, 0, 10.0, MBX_ANALYTIC_LOGARITHMIC, 0},
+	{MBX_FORMULA_BOX_FOLD_BULB_POW2, MBX_DE_DELTA, MBX_DE_FUNCTION_LINEAR, 0, 10000.0, MBX_ANALYTIC_LINEAR, 0},
+	{MBX_FORMULA_AEXION, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LOGARITHMIC, 2, 10000.0, MBX_ANALYTIC_LOGARITHMIC, 0},
+};
+
+const FormulaMeta *formula_meta(int id)
+{
+	for (const FormulaMeta &m : kFormulaTable)
+		if (m.id == id) return &m;
+	return nullptr;
+}
+
+// SetRotation3 of angles in degrees (algebra.cpp:305-311); identity when all are zero
+void rot3_deg(const double a[3], double out[9], int32_t &identity)
+{
+	Rot r;
+	r.z(a[0] * kPi180);
+	r.y(a[1] * kPi180);
+	r.x(a[2] * kPi180);
+	std::memcpy(out, r.m, sizeof(r.m));
+	identity = (a[0] == 0.0 && a[1] == 0.0 && a[2] == 0.0) ? 1 : 0;
+}
+
+// One container of the general path: sFractal after RecalculateFractalParams (fractal.cpp:41-120,
+// 653-678)
+void gen_slot(const mbx_fractal &f, mbx::GenSlot &s)
+{
+	std::memset(&s, 0, sizeof(s));
+	s.formula = f.formula;
+	s.weight = f.formulaWeight;
+	s.checkBailout = f.checkForBailout ? 1 : 0;
+	s.power = f.bulb.power;
+	s.alpha = f.bulb.alphaAngleOffset * kPi180;
+	s.beta = f.bulb.betaAngleOffset * kPi180;
+	const mbx_fractal_mandelbox &mb = f.mandelbox;
+	s.mbScale = mb.scale;
+	s.mbAbsScale = std::fabs(mb.scale);
+	s.mbFoldingLimit = mb.foldingLimit;
+	s.mbFoldingValue = mb.foldingValue;
+	s.mbFR2 = mb.foldingSphericalFixed * mb.foldingSphericalFixed;
+	s.mbMR2 = mb.foldingSphericalMin * mb.foldingSphericalMin;
+	s.mbFactor1 = s.mbFR2 / s.mbMR2;
+	for (int i = 0; i < 3; i++)
+	{
+		s.mbOffset[i] = mb.offset[i];
+		s.mbColor[i] = mb.colorFactor[i];
+		s.ifsOffset[i] = f.IFS.offset[i];
+		s.ifsEdge[i] = f.IFS.edge[i];
+	}
+	s.mbSp1 = mb.colorFactorSp1;
+	s.mbSp2 = mb.colorFactorSp2;
+	const mbx_fractal_ifs &ifs = f.IFS;
+	s.ifsScale = ifs.scale;
+	rot3_deg(ifs.rotation, s.ifsMainRot, s.ifsMainRotId);
+	for (int i = 0; i < MBX_IFS_VECTOR_COUNT; i++)
+	{
+		rot3_deg(ifs.rotations[i], s.ifsRot[i], s.ifsRotId[i]);
+		// CVector4::Normalize with w = 0 (fractal.cpp:653-660)
+		const double d[4] = {ifs.direction[i][0], ifs.direction[i][1], ifs.direction[i][2], 0.0};
+		const double norm = 1.0 / std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + d[3] * d[3]);
+		for (int k = 0; k < 4; k++) s.ifsDirection[i][k] = d[k] * norm;
+		s.ifsDistance[i] = ifs.distance[i];
+		s.ifsIntensity[i] = ifs.intensity[i];
+		s.ifsEnabled[i] = ifs.enabled[i] ? 1 : 0;
+	}
+	s.ifsAbs[0] = ifs.absX;
+	s.ifsAbs[1] = ifs.absY;
+	s.ifsAbs[2] = ifs.absZ;
+	s.ifsMenger = ifs.mengerSpongeMode;
+	s.ifsRotationEnabled = ifs.rotationEnabled;
+	s.ifsEdgeEnabled = ifs.edgeEnabled;
+	const mbx_fractal_transform &t = f.transform;
+	s.scale3 = t.scale3;
+	s.cadd = t.aexionCadd;
+	s.fipFoldFactor = t.foldingIntPowFoldFactor;
+	s.fipZFactor = t.foldingIntPowZFactor;
+	s.adeScale1 = t.analyticDEScale1;
+	s.adeOffset1 = t.analyticDEOffset1;
+	s.adeOffset2 = t.analyticDEOffset2;
+	s.adeEnabled = t.analyticDEEnabled ? 1 : 0;
+	s.adeEnabledFalse = t.analyticDEEnabledFalse ? 1 : 0;
+}
+
+// The sequence of the general path: cHybridFractalSequences::CollectSequenceData and
+// CreateSequence (hybrid_fractal_sequences.cpp:181-287, 290-503) for one container (hybridCount 0)
+// or the hybrid of p->fractal and p->hybridFractals[0 .. hybridCount - 2].
+int derive_generic(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
+{
+	const mbx_fractal &node = p->fractal;
+	const bool hybrid = p->hybridCount >= 1;
+	const int n = hybrid ? p->hybridCount : 1;
+	if (n > MBX_MAX_FRACTALS) return fail(ctx, MBX_ERR_INVALID, "hybridCount exceeds MBX_MAX_FRACTALS");
+	const mbx_fractal *c[MBX_MAX_FRACTALS];
+	c[0] = &p->fractal;
+	for (int k = 1; k < n; k++) c[k] = &p->hybridFractals[k - 1];
+	fc.genSlots = n;
+	fc.genHybrid = hybrid ? 1 : 0;
+	fc.genInitialW = node.initialWAxis;
+	fc.genLinearDEOffset = p->linearDEOffset;
+	double maxBailout = 0.0;
+	bool any = false;
+	for (int k = 0; k < n; k++)
+	{
+		mbx::GenSlot &s = fc.gen[k];
+		gen_slot(*c[k], s);
+		if (c[k]->formula == MBX_FORMULA_NONE) continue;
+		const FormulaMeta *m = formula_meta(c[k]->formula);
+		if (!m) return fail(ctx, MBX_ERR_UNSUPPORTED, "formula not supported");
+		if (c[k]->formula == MBX_FORMULA_MANDELBOX &&
+			(c[k]->mandelbox.rotationsEnabled || c[k]->mandelbox.mainRotationEnabled))
+			return fail(ctx, MBX_ERR_UNSUPPORTED, "Mandelbox rotated folds are not supported on the device");
+		if (c[k]->formulaIterations < 1) return fail(ctx, MBX_ERR_INVALID, "formula_iterations < 1");
+		any = true;
+		if (!hybrid) s.checkBailout = 1;
+		bool addc = false;
+		if (m->cpixel != 2)
+		{
+			addc = !c[k]->dontAddCConstant;
+			if (m->cpixel == 1) addc = !addc;
+		}
+		s.addC = addc ? 1 : 0;
+		if (node.useDefaultBailout)
+		{
+			if (hybrid)
+				maxBailout = std::max(maxBailout, m->defaultBailout);
+			else
+				s.bailout = m->defaultBailout;
+		}
+		else
+			s.bailout = node.bailout;
+	}
+	if (!any) return fail(ctx, MBX_ERR_INVALID, "every formula of the sequence is none");
+	if (hybrid && node.useDefaultBailout)
+		for (int k = 0; k < n; k++) fc.gen[k].bailout = maxBailout;
+
+	// the formula of each iteration: formula_iterations of each container in turn, repeating from
+	// repeat_from, skipping none formulas and containers outside their iteration range
+	const int repeatFromIndex = std::max(0, std::min(p->repeatFrom - 1, n - 1));
+	int fractalNo = 0, counter = 0, last = 0;
+	bool rapidEnd = false;
+	fc.genSeqLength = mbx::kGenSeqLength;
+	for (int i = 0; i < mbx::kGenSeqLength; i++)
+	{
+		counter++;
+		int search = 0;
+		while ((c[fractalNo]->formula == MBX_FORMULA_NONE || i < c[fractalNo]->formulaStartIteration ||
+				   i > c[fractalNo]->formulaStopIteration) && search < n)
+		{
+			fractalNo++;
+			if (fractalNo >= n) fractalNo = repeatFromIndex;
+			search++;
+		}
+		if (search >= n)
+		{
+			rapidEnd = true;
+			break;
+		}
+		fc.genSeq[i] = (uint8_t)fractalNo;
+		last = i;
+		if (counter >= c[fractalNo]->formulaIterations)
+		{
+			counter = 0;
+			fractalNo++;
+			if (fractalNo >= n) fractalNo = repeatFromIndex;
+		}
+	}
+	if (rapidEnd) fc.genSeqLength = last + 1;
+
+	// DE type and function (hybrid_fractal_sequences.cpp:379-503)
+	bool forceDelta = node.deltaDEMethod == 1;
+	const bool forceAnalytic = node.deltaDEMethod == 2;
+	int deType = MBX_DE_ANALYTIC, deFunction = 0, analytic = MBX_ANALYTIC_LINEAR, coloring = 0;
+	if (hybrid)
+	{
+		if (node.deltaDEFunction == 0)
+		{
+			// the DE function used for the most iterations; linear first on a tie
+			int count[7] = {0, 0, 0, 0, 0, 0, 0};
+			for (int k = 0; k < n; k++)
+			{
+				if (c[k]->formula == MBX_FORMULA_NONE) continue;
+				const FormulaMeta *m = formula_meta(c[k]->formula);
+				count[m->deFunction] += c[k]->formulaIterations;
+				if (!forceDelta && !forceAnalytic && m->deType == MBX_DE_DELTA)
+				{
+					deType = MBX_DE_DELTA;
+					forceDelta = true;
+				}
+			}
+			int maxCount = -1;
+			for (int i = 1; i <= 6; i++)
+				if (count[i] > maxCount)
+				{
+					maxCount = count[i];
+					deFunction = i;
+				}
+		}
+		else
+		{
+			deFunction = node.deltaDEFunction;
+			for (int k = 0; k < n; k++)
+			{
+				const FormulaMeta *m = formula_meta(c[k]->formula);
+				if (m && m->deType == MBX_DE_DELTA)
+				{
+					deType = MBX_DE_DELTA;
+					break;
+				}
+			}
+		}
+		if (forceDelta) deType = MBX_DE_DELTA;
+		if (forceAnalytic) deType = MBX_DE_ANALYTIC;
+	}
+	else
+	{
+		const FormulaMeta *m = formula_meta(c[0]->formula);
+		deType = m->deType;
+		deFunction = m->deFunction;
+		analytic = m->analytic;
+		coloring = m->coloring;
+		if (forceDelta) deType = MBX_DE_DELTA;
+		if (forceAnalytic) deType = MBX_DE_ANALYTIC;
+		if (node.deltaDEFunction != 0)
+		{
+			deFunction = node.deltaDEFunction;
+			analytic = deFunction == MBX_DE_FUNCTION_LOGARITHMIC ? MBX_ANALYTIC_LOGARITHMIC : MBX_ANALYTIC_LINEAR;
+		}
+	}
+	if (deFunction != MBX_DE_FUNCTION_LINEAR && deFunction != MBX_DE_FUNCTION_LOGARITHMIC)
+		return fail(ctx, MBX_ERR_UNSUPPORTED, "DE function not supported (pseudo-Kleinian, custom, max-axis)");
+	fc.genDEType = deType;
+	fc.genDEFunction = deFunction;
+	fc.genAnalytic = analytic;
+	fc.genColoring = coloring;
+	return MBX_OK;
+}
+
 // Derive the per-frame constants; returns MBX_OK or an error with ctx->err set.
 int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 {
@@ -426,8 +681,14 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 	if (p->maxRaymarchingSteps < 1) return fail(ctx, MBX_ERR_INVALID, "max_raymarching_steps < 1");
 	if (!(p->detailLevel > 0.0)) return fail(ctx, MBX_ERR_INVALID, "detail_level must be > 0");
 	const mbx_fractal &fr = p->fractal;
-	if (fr.initialWAxis != 0.0) return fail(ctx, MBX_ERR_UNSUPPORTED, "initial_waxis must be 0");
 	if (fr.formulaMaxiter < 1) return fail(ctx, MBX_ERR_INVALID, "formula maxiter < 1");
+	if (p->hybridCount < 0 || p->hybridCount > MBX_MAX_FRACTALS)
+		return fail(ctx, MBX_ERR_INVALID, "hybridCount out of range");
+	// the specialised kinds cover one Mandelbulb, Mandelbox or IFS with w = 0; everything else
+	// takes the general FP64 path
+	const bool generic = p->hybridCount >= 1 || fr.initialWAxis != 0.0 ||
+		(fr.formula != MBX_FORMULA_MANDELBULB && fr.formula != MBX_FORMULA_MANDELBOX &&
+			fr.formula != MBX_FORMULA_KALEIDOSCOPIC_IFS);
 
 	// camera: PrepareMainVectors (render_worker.cpp:539-564)
 	const CameraFrame cf = camera_frame(d3(p->camera), d3(p->target), d3(p->topVector));
@@ -473,6 +734,20 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 	fc.smoothness = (float)p->smoothness;
 	fc.maxRaymarchingSteps = p->maxRaymarchingSteps;
 	fc.minN = p->minN;
+	fc.advancedQuality = p->advancedQuality ? 1 : 0;
+	fc.absMaxStepD = p->absMaxMarchingStep;
+	fc.absMinStepD = p->absMinMarchingStep;
+	fc.relMaxStepD = p->relMaxMarchingStep;
+	fc.relMinStepD = p->relMinMarchingStep;
+	fc.detailSizeMaxD = p->detailSizeMax;
+	fc.detailSizeMinD = p->detailSizeMin;
+	fc.absMaxStep = (float)p->absMaxMarchingStep;
+	fc.absMinStep = (float)p->absMinMarchingStep;
+	fc.relMaxStep = (float)p->relMaxMarchingStep;
+	fc.relMinStep = (float)p->relMinMarchingStep;
+	fc.detailSizeMax = (float)p->detailSizeMax;
+	fc.detailSizeMin = (float)p->detailSizeMin;
+	fc.deltaRel = p->advancedQuality ? p->deltaDERelativeDelta : 1e-5;
 
 	// fractal sequence data (hybrid_fractal_sequences.cpp:181-287, 471-503)
 	int deType = 0, deFunction = 0, analyticFunction = 0;
@@ -496,7 +771,9 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 			defaultBailout = 100.0;
 			cpixelDisabled = true;
 			break;
-		default: return fail(ctx, MBX_ERR_UNSUPPORTED, "formula not supported");
+		default:
+			if (!generic) return fail(ctx, MBX_ERR_UNSUPPORTED, "formula not supported");
+			break;
 	}
 	if (fr.deltaDEMethod == 1) deType = MBX_DE_DELTA;
 	if (fr.deltaDEMethod == 2) deType = MBX_DE_ANALYTIC;
@@ -598,7 +875,13 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 
 	// kernel specialisation
 	int kind = -1;
-	if (fr.formula == MBX_FORMULA_MANDELBULB && deType == MBX_DE_ANALYTIC)
+	if (generic)
+	{
+		const int rc = derive_generic(ctx, p, fc);
+		if (rc != MBX_OK) return rc;
+		kind = mbx::kGeneric;
+	}
+	else if (fr.formula == MBX_FORMULA_MANDELBULB && deType == MBX_DE_ANALYTIC)
 	{
 		const bool pow8 = fr.bulb.power == 8.0 && fr.bulb.alphaAngleOffset == 0.0 &&
 			fr.bulb.betaAngleOffset == 0.0 && analyticFunction == MBX_ANALYTIC_LOGARITHMIC &&
@@ -630,7 +913,7 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 	if (kind < 0) return fail(ctx, MBX_ERR_UNSUPPORTED, "formula / DE mode combination not supported");
 	fc.deKind = kind;
 	fc.farR2 = std::numeric_limits<float>::infinity();
-	if (ctx->farField && (kind == mbx::kBulb8 || kind == mbx::kBulbTrig) && !fc.julia && fc.addC)
+	if (ctx->farField && (kind == mbx::kBulb8 || kind == mbx::kBulbTrig) && !fc.julia && fc.addC && !p->advancedQuality)
 	{
 		const double c = std::max(std::fabs(fr.constantFactor[0]),
 			std::max(std::fabs(fr.constantFactor[1]), std::fabs(fr.constantFactor[2])));

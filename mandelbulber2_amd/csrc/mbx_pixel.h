@@ -38,13 +38,48 @@ struct MarchOut
 	int stepCount; // buffCount: march iterations that ended without a hit (render_worker.cpp:838)
 };
 
+// Advanced quality: the threshold clamp of CalcDistThresh (render_worker.cpp:671-675) and the step
+// limits of RayMarching (render_worker.cpp:819-834; without advanced quality only the clamp at 3)
+__device__ __forceinline__ float clamp_thresh(const FrameConst &f, float t)
+{
+	if (!f.advancedQuality) return t;
+	t = t > f.detailSizeMax ? f.detailSizeMax : t;
+	return t < f.detailSizeMin ? f.detailSizeMin : t;
+}
+__device__ __forceinline__ double clamp_thresh(const FrameConst &f, double t)
+{
+	if (!f.advancedQuality) return t;
+	t = t > f.detailSizeMaxD ? f.detailSizeMaxD : t;
+	return t < f.detailSizeMinD ? f.detailSizeMinD : t;
+}
+__device__ __forceinline__ float limit_step(const FrameConst &f, float step, float thr)
+{
+#pragma clang fp contract(off)
+	if (!f.advancedQuality) return step > 3.0f ? 3.0f : step;
+	if (step > f.absMaxStep) step = f.absMaxStep;
+	if (step < f.absMinStep) step = f.absMinStep;
+	if (thr > f.absMinStep && step > f.relMaxStep * thr) step = f.relMaxStep * thr;
+	if (step < f.relMinStep * thr) step = f.relMinStep * thr;
+	return step;
+}
+__device__ __forceinline__ double limit_step(const FrameConst &f, double step, double thr)
+{
+#pragma clang fp contract(off)
+	if (!f.advancedQuality) return step > 3.0 ? 3.0 : step;
+	if (step > f.absMaxStepD) step = f.absMaxStepD;
+	if (step < f.absMinStepD) step = f.absMinStepD;
+	if (thr > f.absMinStepD && step > f.relMaxStepD * thr) step = f.relMaxStepD * thr;
+	if (step < f.relMinStepD * thr) step = f.relMinStepD * thr;
+	return step;
+}
+
 // CalcDistThresh (render_worker.cpp:653-679)
 template <int KIND>
 __device__ __forceinline__ typename PrecOf<KIND>::R dist_thresh(const FrameConst &f, typename PrecOf<KIND>::P p)
 {
 	using Q = PrecOf<KIND>;
-	if (f.constantDEThreshold) return Q::DEThresh(f);
-	return Q::thresh(f, p);
+	if (f.constantDEThreshold) return clamp_thresh(f, Q::DEThresh(f));
+	return clamp_thresh(f, Q::thresh(f, p));
 }
 
 // CalcDelta (render_worker.cpp:684-690)
@@ -115,7 +150,7 @@ __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start
 		if (kExact)
 			thr = dist_thresh<KIND>(f, point);
 		else
-			thr = constThr ? constThrValue : scan * thrScale;
+			thr = clamp_thresh(f, constThr ? constThrValue : scan * thrScale);
 		dist = calculate_distance<KIND>(point, thr, f, iters, w);
 		if (dist < thr)
 		{
@@ -126,13 +161,13 @@ __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start
 		if (kExact)
 		{
 			step = (dist - (R)0.5 * thr) * (R)f.DEFactorD * ((R)1 - (R)rnd / (R)10000);
-			step = step > (R)3 ? (R)3 : step;
+			step = limit_step(f, step, thr);
 			scan += step / dirLength;
 		}
 		else
 		{
 			step = (dist - (R)0.5 * thr) * (R)f.DEFactor * ((R)1 - (R)rnd * (R)1e-4);
-			step = step > (R)3 ? (R)3 : step;
+			step = limit_step(f, step, thr);
 			scan += step * invDirLength;
 		}
 		stepCount = i + 1;
@@ -154,7 +189,7 @@ __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start
 			if (kExact)
 				thr = dist_thresh<KIND>(f, point);
 			else
-				thr = constThr ? constThrValue : scan * thrScale;
+				thr = clamp_thresh(f, constThr ? constThrValue : scan * thrScale);
 			dist = calculate_distance<KIND>(point, thr, f, iters, w);
 			step *= (R)0.5;
 		}

@@ -214,6 +214,7 @@ hipError_t launch_render(int deKind, const FrameConst *frame, int width, int hei
 		case kIfsAnalytic:
 			return launch_render_t<kIfsAnalytic>(frame, width, height, out, counters, tileShift, full, stream);
 		case kBulbD: return launch_render_t<kBulbD>(frame, width, height, out, counters, tileShift, full, stream);
+		case kGeneric: return launch_render_t<kGeneric>(frame, width, height, out, counters, tileShift, full, stream);
 		default: return hipErrorInvalidValue;
 	}
 }
@@ -263,6 +264,7 @@ hipError_t launch_batch(int deKind, BatchArgs a, int numWaves, hipStream_t strea
 		case kBoxDelta: return launch_batch_t<kBoxDelta>(a, numWaves, a.full != 0, stream);
 		case kIfsAnalytic: return launch_batch_t<kIfsAnalytic>(a, numWaves, a.full != 0, stream);
 		case kBulbD: return launch_batch_t<kBulbD>(a, numWaves, a.full != 0, stream);
+		case kGeneric: return launch_batch_t<kGeneric>(a, numWaves, a.full != 0, stream);
 		default: return hipErrorInvalidValue;
 	}
 }
@@ -280,6 +282,7 @@ hipError_t launch_distance(int deKind, const FrameConst *frame, const float *pts
 		case kIfsAnalytic:
 			return launch_distance_t<kIfsAnalytic>(frame, pts, n, detail, dist, iters, stream);
 		case kBulbD: return launch_distance_t<kBulbD>(frame, pts, n, detail, dist, iters, stream);
+		case kGeneric: return launch_distance_t<kGeneric>(frame, pts, n, detail, dist, iters, stream);
 		default: return hipErrorInvalidValue;
 	}
 }

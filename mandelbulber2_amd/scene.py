@@ -142,6 +142,15 @@ def default_params(width=640, height=480):
     p.hybridCount = 0
     p.repeatFrom = 1
     p.linearDEOffset = 0.0
+    # advanced quality march limits (initparameters.cpp:159-166)
+    p.advancedQuality = 0
+    p.absMaxMarchingStep = 3.0
+    p.absMinMarchingStep = 1e-15
+    p.relMaxMarchingStep = 1e4
+    p.relMinMarchingStep = 1e-2
+    p.detailSizeMax = 1.0
+    p.detailSizeMin = 1e-12
+    p.deltaDERelativeDelta = 0.01
     # palette colouring of material #1 (InitMaterialParams, initparameters.cpp:1600-1810)
     p.useColorsFromPalette = 1
     p.surfaceGradientEnable = 1

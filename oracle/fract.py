@@ -129,11 +129,21 @@ def init_params():
     c.add("interior_mode", BOOL, False)                             # :147
     c.add("constant_DE_threshold", BOOL, False)                     # :148
     c.add("hybrid_fractal_enable", BOOL, False)                     # :149
+    c.add("repeat_from", INT, 1, 1, 9)                              # :151
+    c.add("linear_DE_offset", DOUBLE, 0.0)                          # :157
     c.add("bailout", DOUBLE, 100.0, 1.0, 1e15)                      # :150
     c.add("delta_DE_function", INT, 0, 0, 6)                        # :152-153
     c.add("delta_DE_method", INT, 0, 0, 2)                          # :154
     c.add("use_default_bailout", BOOL, True)                        # :155
     c.add("initial_waxis", DOUBLE, 0.0)                             # :156
+    c.add("advanced_quality", BOOL, False)                          # :159
+    c.add("abs_max_marching_step", DOUBLE, 3.0, 1e-15, 1e15)        # :160
+    c.add("abs_min_marching_step", DOUBLE, 1e-15, 1e-15, 1e15)      # :161
+    c.add("rel_max_marching_step", DOUBLE, 1e4, 1e-15, 1e15)        # :162
+    c.add("rel_min_marching_step", DOUBLE, 1e-2, 1e-15, 1e15)       # :163
+    c.add("detail_size_max", DOUBLE, 1.0)                           # :164
+    c.add("detail_size_min", DOUBLE, 1e-12)                         # :165
+    c.add("deltade_relative_delta", DOUBLE, 0.01, 1e-15, 1e6)       # :166
     c.add("max_raymarching_steps", INT, 10000, 1, 100000000)        # :167
     c.add("stereo_enabled", BOOL, False)                            # :170
     c.add("brightness", DOUBLE, 1.0, 0.0, 1e15)                     # :208
@@ -238,6 +248,16 @@ def init_fractal(index):
     c.add("mandelbox_color_Sp2", DOUBLE, 0.2)
     c.add("mandelbox_main_rotation_enabled", BOOL, False)           # :767
     c.add("mandelbox_rotations_enabled", BOOL, False)               # :768
+    c.add("cadd", DOUBLE, -1.3)                                     # :722
+    c.add("boxfold_bulbpow2_folding_factor", DOUBLE, 2.0)           # :784
+    c.add("boxfold_bulbpow2_z_factor", DOUBLE, 5.0)                 # :785
+    c.add("analyticDE_enabled", BOOL, True)                         # :904
+    c.add("analyticDE_enabled_false", BOOL, False)                  # :905
+    c.add("analyticDE_scale_1", DOUBLE, 1.0)                        # :906
+    c.add("analyticDE_offset_1", DOUBLE, 1.0)                       # :909
+    c.add("analyticDE_offset_2", DOUBLE, 1.0)                       # :910
+    c.add("fold_color_aux_color_enabled_false", BOOL, False)        # :917
+    c.add("transf_scale_3", DOUBLE, 3.0)                            # :1010
     return c
 
 
@@ -258,6 +278,7 @@ def init_material(c, m):
     c.add(p + "fractal_coloring_color_preV215_false", BOOL, False)
     c.add(p + "fractal_coloring_color_4D_enabled_false", BOOL, False)
     c.add(p + "fractal_coloring_extra_color_enabled_false", BOOL, False)
+    c.add(p + "fractal_coloring_extra_color_options_false", BOOL, False)
     c.add(p + "fractal_coloring_sphere_radius", DOUBLE, 1.0, 0.0, 1e20)
     c.add(p + "fractal_coloring_line_direction", VEC3, (1.0, 0.0, 0.0))
     for g in ("specular", "diffuse", "luminosity", "roughness", "reflectance", "transparency"):
@@ -679,7 +700,18 @@ def load(text):
                         initial_waxis=par.get("initial_waxis"), formula_maxiter=par.get("N"),
                         enabled=f.get("fractal_enable"))
         else:
-            flags |= FLAGS["disabled"] if not enabled else FLAGS["hybrid"]
+            # the hybrid node (settings.cpp:2690-2760) over the containers whose formula is not
+            # none; fractal_enable false takes a child out of the render list
+            members = [k for k in enabled if fract[k - 1].get("fractal_enable")]
+            if not members:
+                flags |= FLAGS["disabled"]
+            node = dict(object=members[0] if members else (enabled[0] if enabled else 1),
+                        rotation=par.get("fractal_rotation"), position=par.get("fractal_position"),
+                        repeat=par.get("repeat"), scale=1.0, material=1,
+                        julia_mode=par.get("julia_mode"), julia_c=par.get("julia_c"),
+                        fractal_constant_factor=par.get("fractal_constant_factor"),
+                        initial_waxis=par.get("initial_waxis"), formula_maxiter=par.get("N"),
+                        enabled=True, members=members)
         if node is None and enabled:
             node = _default_node(enabled[0])
         if primitives:
@@ -690,25 +722,35 @@ def load(text):
             p = "node_%04d_" % nid
             parts = par.get(p + "definition").split(",")
             typ = _qtoint(parts[2]) if len(parts) >= 5 else 0
+            parent = _qtoint(parts[3]) if len(parts) >= 5 else 0
             obj = _qtoint(parts[4]) if len(parts) >= 5 else 0
-            defs.append((nid, typ, obj, par.get(p + "enabled")))
+            defs.append((nid, typ, obj, par.get(p + "enabled"), parent))
         frac = [d for d in defs if d[1] == 1]
+        hyb = [d for d in defs if d[1] == 10]
         if any(d[1] == 2 for d in defs):
             flags |= FLAGS["primitives"]
-        if len(frac) > 1 or any(d[1] not in (1, 2) for d in defs):
+        # one hybrid node at the root whose children are all the fractal nodes: a hybrid sequence
+        hybrid_tree = (len(hyb) == 1 and hyb[0][4] == 0 and frac and
+                       all(d[4] == hyb[0][0] for d in frac) and
+                       not any(d[1] not in (1, 2, 10) for d in defs))
+        if not hybrid_tree and (len(frac) > 1 or any(d[1] not in (1, 2) for d in defs)):
             flags |= FLAGS["hybrid"]
-        pick = next((d for d in frac if d[3]), frac[0] if frac else None)
+        pick = hyb[0] if hybrid_tree else next((d for d in frac if d[3]), frac[0] if frac else None)
         if pick is None:
             flags |= FLAGS["disabled"]
         else:
+            members = [d[2] for d in frac if d[3]] if hybrid_tree else None
             p = "node_%04d_" % pick[0]
-            node = dict(object=pick[2], rotation=par.get(p + "rotation"),
+            node = dict(object=(members[0] if members else pick[2]), rotation=par.get(p + "rotation"),
                         position=par.get(p + "position"), repeat=par.get(p + "repeat"),
                         scale=par.get(p + "scale"), material=par.get(p + "material"),
                         julia_mode=par.get(p + "julia_mode"), julia_c=par.get(p + "julia_c"),
                         fractal_constant_factor=par.get(p + "fractal_constant_factor"),
                         initial_waxis=par.get(p + "initial_waxis"),
-                        formula_maxiter=par.get(p + "formula_maxiter"), enabled=pick[3])
+                        formula_maxiter=par.get(p + "formula_maxiter"), enabled=pick[3],
+                        members=members)
+            if hybrid_tree and not members:
+                flags |= FLAGS["disabled"]
     if node is None:
         node = _default_node(1)
     if not node["enabled"]:
@@ -734,6 +776,12 @@ def load(text):
     out["interiorMode"] = int(g("interior_mode"))
     out["constantDEThreshold"] = int(g("constant_DE_threshold"))
     out["maxRaymarchingSteps"] = g("max_raymarching_steps")
+    out["advancedQuality"] = int(g("advanced_quality"))
+    for k, v in (("absMaxMarchingStep", "abs_max_marching_step"), ("absMinMarchingStep", "abs_min_marching_step"),
+                 ("relMaxMarchingStep", "rel_max_marching_step"), ("relMinMarchingStep", "rel_min_marching_step"),
+                 ("detailSizeMax", "detail_size_max"), ("detailSizeMin", "detail_size_min"),
+                 ("deltaDERelativeDelta", "deltade_relative_delta")):
+        out[k] = g(v)
     if g("iteration_threshold_mode") or g("interior_mode"):
         flags |= FLAGS["interior"]
     out["ambientOcclusion"] = g("ambient_occlusion")
@@ -856,9 +904,78 @@ def load(text):
     if g("antialiasing_enabled") and g("antialiasing_size") > 1:
         flags |= FLAGS["antialiasing"]
 
+    TABLE = (2, 3, 4, 5, 7, 8, 10, 11, 16, 19)  # the formulas the device path renders
+
+    def container(prefix, f):
+        """sFractal of one container (fractal.cpp:41-120) as mbx_fractal fields"""
+        nonlocal flags
+        fo_ = f.get("formula")
+        out[prefix + "formula"] = fo_
+        out[prefix + "formulaIterations"] = f.get("formula_iterations")
+        out[prefix + "formulaWeight"] = f.get("formula_weight")
+        out[prefix + "formulaStartIteration"] = f.get("formula_start_iteration")
+        out[prefix + "formulaStopIteration"] = f.get("formula_stop_iteration")
+        out[prefix + "checkForBailout"] = int(f.get("check_for_bailout"))
+        out[prefix + "dontAddCConstant"] = int(f.get("dont_add_c_constant"))
+        out[prefix + "bulb.power"] = f.get("power")
+        out[prefix + "bulb.alphaAngleOffset"] = f.get("alpha_angle_offset")
+        out[prefix + "bulb.betaAngleOffset"] = f.get("beta_angle_offset")
+        mb = {"scale": "mandelbox_scale", "foldingLimit": "mandelbox_folding_limit",
+              "foldingValue": "mandelbox_folding_value",
+              "foldingSphericalMin": "mandelbox_folding_min_radius",
+              "foldingSphericalFixed": "mandelbox_folding_fixed_radius",
+              "offset": "mandelbox_offset", "rotationMain": "mandelbox_rotation_main",
+              "colorFactor": "mandelbox_color", "colorFactorSp1": "mandelbox_color_Sp1",
+              "colorFactorSp2": "mandelbox_color_Sp2"}
+        for k, v in mb.items():
+            out[prefix + "mandelbox." + k] = f.get(v)
+        out[prefix + "mandelbox.rotationsEnabled"] = int(f.get("mandelbox_rotations_enabled"))
+        out[prefix + "mandelbox.mainRotationEnabled"] = int(f.get("mandelbox_main_rotation_enabled"))
+        out[prefix + "mandelbox.colorFactorR"] = f.get("mandelbox_color_R")
+        out[prefix + "mandelbox.rotation"] = [[f.get("mandelbox_rotation_%s_%d" % (s_, i))
+                                               for i in (1, 2, 3)] for s_ in ("neg", "pos")]
+        out[prefix + "IFS.scale"] = f.get("IFS_scale")
+        out[prefix + "IFS.offset"] = f.get("IFS_offset")
+        out[prefix + "IFS.rotation"] = f.get("IFS_rotation")
+        out[prefix + "IFS.edge"] = f.get("IFS_edge")
+        out[prefix + "IFS.direction"] = [f.get("IFS_direction_%d" % i) for i in range(9)]
+        out[prefix + "IFS.rotations"] = [f.get("IFS_rotations_%d" % i) for i in range(9)]
+        out[prefix + "IFS.distance"] = [f.get("IFS_distance_%d" % i) for i in range(9)]
+        out[prefix + "IFS.intensity"] = [f.get("IFS_intensity_%d" % i) for i in range(9)]
+        out[prefix + "IFS.enabled"] = [int(f.get("IFS_enabled_%d" % i)) for i in range(9)]
+        for k, v in (("absX", "IFS_abs_x"), ("absY", "IFS_abs_y"), ("absZ", "IFS_abs_z"),
+                     ("mengerSpongeMode", "IFS_menger_sponge_mode"),
+                     ("rotationEnabled", "IFS_rotation_enabled"), ("edgeEnabled", "IFS_edge_enabled")):
+            out[prefix + "IFS." + k] = int(f.get(v))
+        for k, v in (("scale3", "transf_scale_3"), ("aexionCadd", "cadd"),
+                     ("foldingIntPowFoldFactor", "boxfold_bulbpow2_folding_factor"),
+                     ("foldingIntPowZFactor", "boxfold_bulbpow2_z_factor"),
+                     ("analyticDEScale1", "analyticDE_scale_1"), ("analyticDEOffset1", "analyticDE_offset_1"),
+                     ("analyticDEOffset2", "analyticDE_offset_2")):
+            out[prefix + "transform." + k] = f.get(v)
+        out[prefix + "transform.analyticDEEnabled"] = int(f.get("analyticDE_enabled"))
+        out[prefix + "transform.analyticDEEnabledFalse"] = int(f.get("analyticDE_enabled_false"))
+        if fo_ != 0 and fo_ not in TABLE:
+            flags |= FLAGS["formula"]
+        if fo_ == 8 and (f.get("mandelbox_rotations_enabled") or f.get("mandelbox_main_rotation_enabled")):
+            flags |= FLAGS["formula"]
+        if fo_ == 7 and g(mp + "use_colors_from_palette") and f.get("fold_color_aux_color_enabled_false"):
+            flags |= FLAGS["palette"]
+        return fo_
+
     f = fract[node["object"] - 1] if 1 <= node["object"] <= 9 else init_fractal(1)
-    fo = f.get("formula")
-    out["fractal.formula"] = fo
+    fo = container("fractal.", f)
+    members = node.get("members")
+    if members:
+        out["hybridCount"] = len(members)
+        for k, obj in enumerate(members[1:]):
+            container("hybridFractals.%d." % k, fract[obj - 1])
+        out["repeatFrom"] = g("repeat_from")
+        if g(mp + "use_colors_from_palette") and g(mp + "fractal_coloring_extra_color_options_false"):
+            flags |= FLAGS["palette"]
+    else:
+        out["hybridCount"] = 0
+    out["linearDEOffset"] = g("linear_DE_offset")
     out["fractal.formulaMaxiter"] = node["formula_maxiter"]
     out["fractal.deltaDEMethod"] = g("delta_DE_method")
     out["fractal.deltaDEFunction"] = g("delta_DE_function")
@@ -868,41 +985,8 @@ def load(text):
     out["fractal.juliaC"] = node["julia_c"]
     out["fractal.constantFactor"] = node["fractal_constant_factor"]
     out["fractal.initialWAxis"] = node["initial_waxis"]
-    out["fractal.dontAddCConstant"] = int(f.get("dont_add_c_constant"))
-    out["fractal.bulb.power"] = f.get("power")
-    out["fractal.bulb.alphaAngleOffset"] = f.get("alpha_angle_offset")
-    out["fractal.bulb.betaAngleOffset"] = f.get("beta_angle_offset")
-    mb = {"scale": "mandelbox_scale", "foldingLimit": "mandelbox_folding_limit",
-          "foldingValue": "mandelbox_folding_value",
-          "foldingSphericalMin": "mandelbox_folding_min_radius",
-          "foldingSphericalFixed": "mandelbox_folding_fixed_radius",
-          "offset": "mandelbox_offset", "rotationMain": "mandelbox_rotation_main",
-          "colorFactor": "mandelbox_color", "colorFactorSp1": "mandelbox_color_Sp1",
-          "colorFactorSp2": "mandelbox_color_Sp2"}
-    for k, v in mb.items():
-        out["fractal.mandelbox." + k] = f.get(v)
-    out["fractal.mandelbox.rotationsEnabled"] = int(f.get("mandelbox_rotations_enabled"))
-    out["fractal.mandelbox.mainRotationEnabled"] = int(f.get("mandelbox_main_rotation_enabled"))
-    out["fractal.mandelbox.colorFactorR"] = f.get("mandelbox_color_R")
-    out["fractal.mandelbox.rotation"] = [[f.get("mandelbox_rotation_%s_%d" % (s, i))
-                                          for i in (1, 2, 3)] for s in ("neg", "pos")]
-    out["fractal.IFS.scale"] = f.get("IFS_scale")
-    out["fractal.IFS.offset"] = f.get("IFS_offset")
-    out["fractal.IFS.rotation"] = f.get("IFS_rotation")
-    out["fractal.IFS.edge"] = f.get("IFS_edge")
-    out["fractal.IFS.direction"] = [f.get("IFS_direction_%d" % i) for i in range(9)]
-    out["fractal.IFS.rotations"] = [f.get("IFS_rotations_%d" % i) for i in range(9)]
-    out["fractal.IFS.distance"] = [f.get("IFS_distance_%d" % i) for i in range(9)]
-    out["fractal.IFS.intensity"] = [f.get("IFS_intensity_%d" % i) for i in range(9)]
-    out["fractal.IFS.enabled"] = [int(f.get("IFS_enabled_%d" % i)) for i in range(9)]
-    for k, v in (("absX", "IFS_abs_x"), ("absY", "IFS_abs_y"), ("absZ", "IFS_abs_z"),
-                 ("mengerSpongeMode", "IFS_menger_sponge_mode"),
-                 ("rotationEnabled", "IFS_rotation_enabled"), ("edgeEnabled", "IFS_edge_enabled")):
-        out["fractal.IFS." + k] = int(f.get(v))
     if fo == 0:
         flags |= FLAGS["disabled"]
-    elif fo not in (2, 8, 10):
-        flags |= FLAGS["formula"]
     out["formulaRotation"] = node["rotation"]
     info = dict(fileVersion=ver, format=fmt, lines=n_lines, errors=n_errors, formula=fo,
                 unsupported=flags)

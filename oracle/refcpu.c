@@ -406,12 +406,12 @@ static void slot_init(slot_t *sl, const mbx_fractal *f)
 }
 
 /* cHybridFractalSequences::CollectSequenceData and CreateSequence for the rendered node
- * (hybrid_fractal_sequences.cpp:181-287, 290-503): one container, or the hybrid of
- * p->fractal and p->hybridFractals[0 .. hybridCount - 2] */
+ * (hybrid_fractal_sequences.cpp:181-287, 290-503): one container (hybridCount 0), or the hybrid
+ * of p->fractal and p->hybridFractals[0 .. hybridCount - 2] */
 static int sequence_init(scene_t *s, const mbx_params *p)
 {
 	const mbx_fractal *node = &p->fractal;
-	const int isHybrid = p->hybridCount >= 2;
+	const int isHybrid = p->hybridCount >= 1;
 	const int n = isHybrid ? p->hybridCount : 1;
 	if (n > MBX_MAX_FRACTALS) return MBX_ERR_INVALID;
 	const mbx_fractal *c[MBX_MAX_FRACTALS];
@@ -1246,7 +1246,8 @@ static double calculate_distance(const scene_t *s, v3 point, double detailSize, 
 	}
 	else
 	{
-		const double deltaDE = fmax(v3_len(point) * 1e-14, 1e-5 * detailSize);
+		const double deltaDE = fmax(v3_len(point) * 1e-14,
+			(p->advancedQuality ? p->deltaDERelativeDelta : 1e-5) * detailSize);
 		compute(s, MODE_DELTA1, point, -1, maxiterMultiplier, &fo);
 		const double r = v3_len(fo.z);
 		v3 zFromIters = fo.z;
@@ -1336,6 +1337,11 @@ static double calc_dist_thresh(const scene_t *s, v3 point)
 		distThresh = p->DEThresh;
 	else
 		distThresh = v3_len(v3_sub(s->camera, point)) * s->resolution * s->fov / p->detailLevel;
+	if (p->advancedQuality)
+	{
+		if (distThresh > p->detailSizeMax) distThresh = p->detailSizeMax;
+		if (distThresh < p->detailSizeMin) distThresh = p->detailSizeMin;
+	}
 	distThresh /= 1.0; /* reduceDetail = 1 outside flight recording (render_job.cpp:1173-1184) */
 	return distThresh;
 }
@@ -1407,7 +1413,19 @@ static void ray_marching(ray_ctx *rc, v3 start, v3 direction, double minScan, do
 			step = (dist - 0.8 * distThresh) * p->DEFactor * (1.0 - mbxo_random(rc->seed, 1000) / 10000.0);
 		else
 			step = (dist - 0.5 * distThresh) * p->DEFactor * (1.0 - mbxo_random(rc->seed, 1000) / 10000.0);
-		if (step > 3.0) step = 3.0;
+		if (p->advancedQuality)
+		{
+			/* render_worker.cpp:819-830 */
+			if (step > p->absMaxMarchingStep) step = p->absMaxMarchingStep;
+			if (step < p->absMinMarchingStep) step = p->absMinMarchingStep;
+			if (distThresh > p->absMinMarchingStep)
+			{
+				if (step > p->relMaxMarchingStep * distThresh) step = p->relMaxMarchingStep * distThresh;
+			}
+			if (step < p->relMinMarchingStep * distThresh) step = p->relMinMarchingStep * distThresh;
+		}
+		else if (step > 3.0)
+			step = 3.0;
 		if (out->stepPoint) out->stepPoint[i] = point;
 		out->buffCount = i + 1;
 		scan += step / dirLength;

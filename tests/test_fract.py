@@ -47,7 +47,7 @@ class _CppResult:
     def __getitem__(self, path):
         obj = self.p
         for part in path.split("."):
-            obj = getattr(obj, part)
+            obj = obj[int(part)] if part.isdigit() else getattr(obj, part)
         return _tolist(obj)
 
     def flags(self):
@@ -229,13 +229,58 @@ def test_menger_sponge_and_mandelbox_22(load):
     assert not {"ao_lightmap", "ao_screen_space"} & r.flags()
 
 
-def test_boolean_and_node_hybrids_are_reported(load):
+def test_boolean_scene_is_reported(load):
     r = load(_read("boolean001.fract"))
     assert {"hybrid", "transform"} <= r.flags()
+
+
+def _containers(r):
+    n = r["hybridCount"]
+    return [(r["fractal.formula"], r["fractal.formulaIterations"])] + [
+        (r["hybridFractals.%d.formula" % k], r["hybridFractals.%d.formulaIterations" % k])
+        for k in range(n - 1)]
+
+
+def test_node_tree_hybrid(load):
+    # 2.35 object tree: a hybrid node (type 10) whose children are the fractal nodes 2-4;
+    # node n holds fractal n - 1 ([fractal_1] formula 8 iterations 2, [fractal_2] 7 / 3,
+    # [fractal_3] 3 with the default single iteration)
     r = load(_read("hybrid002.fract"))
     assert r.meta("fileVersion") == 2.35
-    assert "hybrid" in r.flags()
-    assert r["fractal.formula"] == abi.MBX_FORMULA_MANDELBOX   # node 2 -> fractal 1: formula 8
+    assert "hybrid" not in r.flags()
+    assert r["hybridCount"] == 3
+    assert _containers(r) == [(abi.MBX_FORMULA_MANDELBOX, 2), (abi.MBX_FORMULA_MENGER_SPONGE, 3),
+                              (abi.MBX_FORMULA_MANDELBULB_POWER2, 1)]
+    assert r["fractal.mandelbox.scale"] == -0.4
+    assert r["repeatFrom"] == 1
+
+
+def test_legacy_hybrids(load):
+    # formula_N / formula_iterations_N of the main section, [fractal_N] parameters; containers
+    # with formula none (hybrid14's fractal_3 and fractal_4) leave the sequence
+    r = load(_read("hybrid003.fract"))
+    assert "hybrid" not in r.flags()
+    assert _containers(r) == [(8, 2), (7, 3), (3, 1)]
+    r = load(_read("hybrid16.fract"))
+    assert _containers(r) == [(8, 3), (10, 3), (5, 1)]
+    assert r["fractal.juliaMode"] == 1
+    assert r["fractal.mandelbox.scale"] == -1.5 and r["hybridFractals.0.IFS.scale"] == 2.62
+    r = load(_read("hybrid14.fract"))
+    assert r.meta("fileVersion") == 2.05
+    assert _containers(r) == [(5, 20), (7, 250)]
+    assert r["hybridFractals.0.bulb.power"] == 8.0 and r["DEThresh"] == 0.03
+
+
+def test_advanced_quality_and_transform_parameters(load):
+    r = load(_read("hybrid22_foldingIntPow.fract"))
+    assert _containers(r) == [(8, 2), (8, 2), (abi.MBX_FORMULA_BOX_FOLD_BULB_POW2, 40)]
+    assert r["fractal.mandelbox.scale"] == 1.4 and r["hybridFractals.0.mandelbox.scale"] == 1.7
+    assert r["hybridFractals.1.transform.foldingIntPowZFactor"] == 10.0
+    assert r["hybridFractals.1.transform.foldingIntPowFoldFactor"] == 2.0   # default
+    assert r["fractal.deltaDEFunction"] == 1
+    assert r["advancedQuality"] == 1 and r["deltaDERelativeDelta"] == 0.005
+    assert r["constantDEThreshold"] == 1 and r["DEThresh"] == 5e-05
+    assert r["absMaxMarchingStep"] == 3.0 and r["relMinMarchingStep"] == 0.01   # defaults
 
 
 # ------------------------------------------------------------------------------------------
