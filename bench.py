@@ -9,7 +9,7 @@ launch's device time, HIP events on the launch stream), the plain per-frame kern
 same frame, and the CPU oracle timed on this host.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3]
-                    [--workload mandelbulb8|material|mandelbox|animation64|deepzoom|adjoint512]
+                    [--workload mandelbulb8|material|mandelbox|hybrid|animation64|deepzoom|adjoint512]
 
 `--workload material` times the same 1080p Mandelbulb-8 with the reference's default material:
 palette colouring from the default surface gradient, plastic and metallic highlights, glow.
@@ -17,6 +17,8 @@ palette colouring from the default surface gradient, plastic and metallic highli
 finite differences in FP64, hard shadow, 16-ray AO, seeded camera; 1920x1080).
 `--workload animation64` times a 64-frame batch (power 2 -> 12, mixed resolutions, 4 spp with
 GI depth 2) in one grid launch against one plain launch per frame.
+`--workload hybrid` times the reference's hybrid002 example scene (Mandelbox, Menger sponge and
+Mandelbulb power 2 in one hybrid sequence) at 1920x1080 on the general FP64 formula kernel.
 `--workload deepzoom` times the 1024x1024 deep-zoom Mandelbox (camera 1e-9 from the surface).
 `--workload adjoint512` times a differentiable render plus its adjoint (dL/dtheta) at 512x512.
 """
@@ -35,7 +37,10 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X published FP64 peak
 # Algorithmic flop counts (fixed, independent of the instruction mix): per DE evaluation
 # (ray point, threshold, DE finish, step) and per formula iteration.
 F_DE = 25
-F_IT = {"mandelbulb8": 48, "material": 48, "mandelbox": 29, "animation64": 32, "deepzoom": 29}
+F_IT = {"mandelbulb8": 48, "material": 48, "mandelbox": 29, "animation64": 32, "deepzoom": 29,
+        # the hybrid's cycle: 2 Mandelbox (29) + 3 Menger sponge (20) + 1 Mandelbulb power 2 (30)
+        # iterations, each with the 4D c addition and length: (2*29 + 3*20 + 30) / 6
+        "hybrid": 25}
 # A far-field step (no DE evaluation: the step is provably the clamped 3) does the ray point (6),
 # |p|^2 (5) and the scan update (2).
 F_FAR = 13
@@ -86,6 +91,17 @@ WORKLOADS = {
                    "camera": "seed 0, distance 2.6-3.4", "loss": "L2 against a target at "
                    "theta * (1 +- U(0, 0.05))", "parameters": "power, rotation xyz, camera xyz, "
                    "target xyz", "shading": "Lambert, analytic normals, FP64 Newton-refined hits"},
+    },
+    "hybrid": {
+        "metric": "Mrays/s DE raymarch of the hybrid002 example scene (Mandelbox x2, Menger sponge "
+                  "x3, Mandelbulb power 2 per cycle), 1080p, 1 GPU; % FP64 peak",
+        "kernel": "batch_kernel<kGeneric>", "dtype": "f64", "peak": FP64_PEAK_TFLOPS,
+        "config": {"formula": "hybrid sequence from tests/golden/fract/hybrid002.fract: mandelbox "
+                              "(scale -0.4) 2 iterations, menger sponge 3, mandelbulb power 2 1; "
+                              "analytic linear DE, maxiter 250, FP64 4D orbit",
+                   "camera": "the scene's own", "spp": 1,
+                   "shading": "the scene's material and main light; effects the path does not "
+                              "render (lightmap AO, aux lights, post effects) off"},
     },
     "mandelbox": {
         "metric": "Mrays/s DE raymarch with hard shadow + 16-ray AO, 1080p Mandelbox scale 2, "
@@ -152,6 +168,14 @@ def make_scene(name, w, h):
         return scene.animation64(seed=3)
     if name == "deepzoom":
         return scene.deep_zoom_mandelbox(1024, 1024, seed=5)
+    if name == "hybrid":
+        from mandelbulber2_amd import abi, runtime
+        with open(os.path.join(ROOT, "tests", "golden", "fract", "hybrid002.fract"), "rb") as f:
+            p, _ = runtime.load_fract(f.read())
+        p.imageWidth, p.imageHeight = w, h
+        if p.ambientOcclusionEnabled and p.ambientOcclusionMode != abi.MBX_AO_FAST:
+            p.ambientOcclusionEnabled = 0   # multi-ray AO weighs its rays by a lightmap texture
+        return p
     return scene.mandelbox_shadow_ao(w, h, seed=1, scale=2.0)
 
 

@@ -42,6 +42,21 @@ __device__ __forceinline__ DV4 rot4(const double *m, int identity, DV4 v)
 		m[6] * v.x + m[7] * v.y + m[8] * v.z, v.w);
 }
 
+// An out-of-line function sees every argument as possibly different per lane, so it would read
+// the frame constants with vector loads into VGPRs.  Every lane passes the same frame: taking the
+// address from the first active lane makes it provably uniform, and reading it through the
+// constant address space lets the compiler use scalar loads.
+template <class T>
+__device__ __forceinline__ const T &uniform_ref(const T &r)
+{
+	const uint64_t v = (uint64_t)(uintptr_t)&r;
+	const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+	const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+	typedef const __attribute__((address_space(4))) T ConstT;
+	ConstT *c = (ConstT *)(uintptr_t)(((uint64_t)hi << 32) | lo);
+	return *(const T *)c;
+}
+
 // ---- formulas -------------------------------------------------------------------------------
 
 // cFractalMandelbulb::FormulaCode (fractal_mandelbulb.cpp:29-43)
@@ -305,18 +320,57 @@ __device__ __forceinline__ void gen_aexion(const GenSlot &s, DV4 &z, GenAux &a)
 	}
 }
 
+// The formulas with FP64 libm calls (pow, asin / acos, atan2, sin, cos) run out of line: inlined,
+// their temporaries would set the register count of every caller of the orbit loop.  The orbit
+// state goes in and out by value, in registers.
+struct GenZ
+{
+	DV4 z;
+	double DE;
+};
+__device__ __attribute__((noinline)) GenZ gen_mandelbulb_call(const GenSlot &slot, DV4 z, double r, double DE)
+{
+	const GenSlot &s = uniform_ref(slot);
+	GenAux a;
+	a.r = r;
+	a.DE = DE;
+	gen_mandelbulb(s, z, a);
+	return GenZ{z, a.DE};
+}
+__device__ __attribute__((noinline)) GenZ gen_xenodreambuie_call(const GenSlot &slot, DV4 z, double r, double DE)
+{
+	const GenSlot &s = uniform_ref(slot);
+	GenAux a;
+	a.r = r;
+	a.DE = DE;
+	gen_xenodreambuie(s, z, a);
+	return GenZ{z, a.DE};
+}
+
 __device__ __forceinline__ void gen_formula(const GenSlot &s, DV4 &z, GenAux &a)
 {
 	switch (s.formula)
 	{
-		case MBX_FORMULA_MANDELBULB: gen_mandelbulb(s, z, a); break;
+		case MBX_FORMULA_MANDELBULB:
+		{
+			const GenZ o = gen_mandelbulb_call(s, z, a.r, a.DE);
+			z = o.z;
+			a.DE = o.DE;
+			break;
+		}
 		case MBX_FORMULA_MANDELBULB_POWER2: gen_mandelbulb_power2(z, a); break;
 		case MBX_FORMULA_HYPERCOMPLEX: gen_hypercomplex(z, a); break;
 		case MBX_FORMULA_QUATERNION: gen_quaternion(z, a); break;
 		case MBX_FORMULA_MENGER_SPONGE: gen_menger_sponge(s, z, a); break;
 		case MBX_FORMULA_MANDELBOX: gen_mandelbox(s, z, a); break;
 		case MBX_FORMULA_KALEIDOSCOPIC_IFS: gen_kifs(s, z, a); break;
-		case MBX_FORMULA_XENODREAMBUIE: gen_xenodreambuie(s, z, a); break;
+		case MBX_FORMULA_XENODREAMBUIE:
+		{
+			const GenZ o = gen_xenodreambuie_call(s, z, a.r, a.DE);
+			z = o.z;
+			a.DE = o.DE;
+			break;
+		}
 		case MBX_FORMULA_BOX_FOLD_BULB_POW2: gen_box_fold_bulb_pow2(s, z, a); break;
 		case MBX_FORMULA_AEXION: gen_aexion(s, z, a); break;
 		default: break;
@@ -393,55 +447,75 @@ __device__ __forceinline__ int gen_orbit(const FrameConst &f, DV3 p, int maxN, D
 
 // The distance of the sequence at the fractal-local point p: the analytic finish
 // (compute_fractal.cpp:493-617) or the reference's 4-orbit delta DE (calculate_distance.cpp:609-711).
-// One out-of-line copy serves every call site of the kernels.
-__device__ __attribute__((noinline)) double compute_generic(DV3 p, const FrameConst &f, double detailSize,
+// One out-of-line copy serves every call site of the kernels, and the delta DE's four orbits
+// (p, then p + delta along x, y, z with the first orbit's iteration count as their limit) run
+// through one copy of the orbit loop: the code and the registers of one orbit.
+__device__ __attribute__((noinline)) double compute_generic(DV3 p, const FrameConst &frame, double detailSize,
 	int *itersOut, uint32_t *work)
 {
 #pragma clang fp contract(off)
-	GenAux a;
-	DV4 z;
-	const int iters = gen_orbit(f, p, f.formulaMaxiter, z, a);
-	*itersOut = iters;
-	if (f.genDEType == MBX_DE_ANALYTIC)
+	const FrameConst &f = uniform_ref(frame);
+	const bool analytic = f.genDEType == MBX_DE_ANALYTIC;
+	const double deltaDE = analytic ? 0.0 : fmax(sqrt(ddot(p, p)) * 1e-14, f.deltaRel * detailSize);
+	const int orbits = analytic ? 1 : 4;
+	int iters = 0;
+	uint32_t total = 0;
+	double r = 0.0, DE = 0.0, r3 = 0.0, drSq = 0.0;
+#pragma unroll 1
+	for (int k = 0; k < orbits; k++)
 	{
-		*work += (uint32_t)iters;
-		const double r = a.r;
-		if (!(a.DE > 0.0)) return r;
+		DV3 q = p;
+		if (k == 1) q.x = p.x + deltaDE;
+		if (k == 2) q.y = p.y + deltaDE;
+		if (k == 3) q.z = p.z + deltaDE;
+		GenAux a;
+		DV4 z;
+		const int it = gen_orbit(f, q, k == 0 ? f.formulaMaxiter : iters, z, a);
+		total += (uint32_t)it;
+		const double len3 = sqrt(z.x * z.x + z.y * z.y + z.z * z.z);
+		if (k == 0)
+		{
+			iters = it;
+			r = a.r;
+			DE = a.DE;
+			r3 = len3;
+		}
+		else
+		{
+			const double dr = fabs(len3 - r3) / deltaDE;
+			drSq = drSq + dr * dr; // dr1^2 + dr2^2 + dr3^2, summed left to right
+		}
+	}
+	*itersOut = iters;
+	*work += total;
+	if (analytic)
+	{
+		if (!(DE > 0.0)) return r;
 		if (f.genHybrid)
 		{
-			if (f.genDEFunction == MBX_DE_FUNCTION_LINEAR) return (r - f.genLinearDEOffset) / a.DE;
-			return (r > 1.0) ? 0.5 * r * log(r) / a.DE : 0.0;
+			if (f.genDEFunction == MBX_DE_FUNCTION_LINEAR) return (r - f.genLinearDEOffset) / DE;
+			return (r > 1.0) ? 0.5 * r * log(r) / DE : 0.0;
 		}
 		switch (f.genAnalytic)
 		{
-			case MBX_ANALYTIC_LOGARITHMIC: return (r > 1.0) ? 0.5 * r * log(r) / a.DE : 0.0;
-			case MBX_ANALYTIC_LINEAR: return r / a.DE;
-			case MBX_ANALYTIC_IFS: return (r - 2.0) / a.DE;
+			case MBX_ANALYTIC_LOGARITHMIC: return (r > 1.0) ? 0.5 * r * log(r) / DE : 0.0;
+			case MBX_ANALYTIC_LINEAR: return r / DE;
+			case MBX_ANALYTIC_IFS: return (r - 2.0) / DE;
 			default: return r;
 		}
 	}
-	const double deltaDE = fmax(sqrt(ddot(p, p)) * 1e-14, f.deltaRel * detailSize);
-	const double r = sqrt(z.x * z.x + z.y * z.y + z.z * z.z);
-	GenAux b;
-	DV4 z1, z2, z3;
-	const int i1 = gen_orbit(f, dv3(p.x + deltaDE, p.y, p.z), iters, z1, b);
-	const int i2 = gen_orbit(f, dv3(p.x, p.y + deltaDE, p.z), iters, z2, b);
-	const int i3 = gen_orbit(f, dv3(p.x, p.y, p.z + deltaDE), iters, z3, b);
-	*work += (uint32_t)(iters + i1 + i2 + i3);
-	const double dr1 = fabs(sqrt(z1.x * z1.x + z1.y * z1.y + z1.z * z1.z) - r) / deltaDE;
-	const double dr2 = fabs(sqrt(z2.x * z2.x + z2.y * z2.y + z2.z * z2.z) - r) / deltaDE;
-	const double dr3 = fabs(sqrt(z3.x * z3.x + z3.y * z3.y + z3.z * z3.z) - r) / deltaDE;
-	const double dr = sqrt(dr1 * dr1 + dr2 * dr2 + dr3 * dr3);
+	const double dr = sqrt(drSq);
 	if (dr > 0.0)
-		return (f.genDEFunction == MBX_DE_FUNCTION_LOGARITHMIC) ? 0.5 * r * log(r) / dr : 0.5 * r / dr;
-	return r;
+		return (f.genDEFunction == MBX_DE_FUNCTION_LOGARITHMIC) ? 0.5 * r3 * log(r3) / dr : 0.5 * r3 / dr;
+	return r3;
 }
 
 // Compute<calcModeColouring> + CalculateColorIndex of the sequence (compute_fractal.cpp:343-436,
 // fractal_coloring.cpp:292-340) at the fractal-local point p
-__device__ __attribute__((noinline)) double colour_index_generic(DV3 p, const FrameConst &f)
+__device__ __attribute__((noinline)) double colour_index_generic(DV3 p, const FrameConst &frame)
 {
 #pragma clang fp contract(off)
+	const FrameConst &f = uniform_ref(frame);
 	DV4 z = dv4(p.x, p.y, p.z, f.genInitialW);
 	const DV4 start = z;
 	GenAux a;

@@ -40,32 +40,32 @@ struct MarchOut
 
 // Advanced quality: the threshold clamp of CalcDistThresh (render_worker.cpp:671-675) and the step
 // limits of RayMarching (render_worker.cpp:819-834; without advanced quality only the clamp at 3)
-__device__ __forceinline__ float clamp_thresh(const FrameConst &f, float t)
+__device__ __forceinline__ float clamp_thresh(const FrameConst &f, float t, bool adv)
 {
-	if (!f.advancedQuality) return t;
+	if (!adv) return t;
 	t = t > f.detailSizeMax ? f.detailSizeMax : t;
 	return t < f.detailSizeMin ? f.detailSizeMin : t;
 }
-__device__ __forceinline__ double clamp_thresh(const FrameConst &f, double t)
+__device__ __forceinline__ double clamp_thresh(const FrameConst &f, double t, bool adv)
 {
-	if (!f.advancedQuality) return t;
+	if (!adv) return t;
 	t = t > f.detailSizeMaxD ? f.detailSizeMaxD : t;
 	return t < f.detailSizeMinD ? f.detailSizeMinD : t;
 }
-__device__ __forceinline__ float limit_step(const FrameConst &f, float step, float thr)
+__device__ __forceinline__ float limit_step(const FrameConst &f, float step, float thr, bool adv)
 {
 #pragma clang fp contract(off)
-	if (!f.advancedQuality) return step > 3.0f ? 3.0f : step;
+	if (!adv) return step > 3.0f ? 3.0f : step;
 	if (step > f.absMaxStep) step = f.absMaxStep;
 	if (step < f.absMinStep) step = f.absMinStep;
 	if (thr > f.absMinStep && step > f.relMaxStep * thr) step = f.relMaxStep * thr;
 	if (step < f.relMinStep * thr) step = f.relMinStep * thr;
 	return step;
 }
-__device__ __forceinline__ double limit_step(const FrameConst &f, double step, double thr)
+__device__ __forceinline__ double limit_step(const FrameConst &f, double step, double thr, bool adv)
 {
 #pragma clang fp contract(off)
-	if (!f.advancedQuality) return step > 3.0 ? 3.0 : step;
+	if (!adv) return step > 3.0 ? 3.0 : step;
 	if (step > f.absMaxStepD) step = f.absMaxStepD;
 	if (step < f.absMinStepD) step = f.absMinStepD;
 	if (thr > f.absMinStepD && step > f.relMaxStepD * thr) step = f.relMaxStepD * thr;
@@ -78,8 +78,9 @@ template <int KIND>
 __device__ __forceinline__ typename PrecOf<KIND>::R dist_thresh(const FrameConst &f, typename PrecOf<KIND>::P p)
 {
 	using Q = PrecOf<KIND>;
-	if (f.constantDEThreshold) return clamp_thresh(f, Q::DEThresh(f));
-	return clamp_thresh(f, Q::thresh(f, p));
+	const bool adv = f.advancedQuality != 0;
+	if (f.constantDEThreshold) return clamp_thresh(f, Q::DEThresh(f), adv);
+	return clamp_thresh(f, Q::thresh(f, p), adv);
 }
 
 // CalcDelta (render_worker.cpp:684-690)
@@ -115,6 +116,7 @@ __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start
 	const R thrScale = kExact ? (R)0 : dirLength * (R)f.threshK;
 	const R constThrValue = Q::DEThresh(f);
 	const int maxSteps = f.maxRaymarchingSteps;
+	const bool adv = f.advancedQuality != 0; // advanced-quality threshold and step limits
 	// far field (FP32 Mandelbulb kinds): at |p|^2 >= farR2 the host has bounded the DE from the
 	// first formula iteration (|z_1| >= |p|^P - |c|, escape at i = 0, iters = 1) above the value
 	// that makes the step below clamp to exactly 3, so the step is taken without evaluating it;
@@ -150,7 +152,7 @@ __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start
 		if (kExact)
 			thr = dist_thresh<KIND>(f, point);
 		else
-			thr = clamp_thresh(f, constThr ? constThrValue : scan * thrScale);
+			thr = clamp_thresh(f, constThr ? constThrValue : scan * thrScale, adv);
 		dist = calculate_distance<KIND>(point, thr, f, iters, w);
 		if (dist < thr)
 		{
@@ -161,13 +163,13 @@ __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start
 		if (kExact)
 		{
 			step = (dist - (R)0.5 * thr) * (R)f.DEFactorD * ((R)1 - (R)rnd / (R)10000);
-			step = limit_step(f, step, thr);
+			step = limit_step(f, step, thr, adv);
 			scan += step / dirLength;
 		}
 		else
 		{
 			step = (dist - (R)0.5 * thr) * (R)f.DEFactor * ((R)1 - (R)rnd * (R)1e-4);
-			step = limit_step(f, step, thr);
+			step = limit_step(f, step, thr, adv);
 			scan += step * invDirLength;
 		}
 		stepCount = i + 1;
@@ -189,7 +191,7 @@ __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start
 			if (kExact)
 				thr = dist_thresh<KIND>(f, point);
 			else
-				thr = clamp_thresh(f, constThr ? constThrValue : scan * thrScale);
+				thr = clamp_thresh(f, constThr ? constThrValue : scan * thrScale, adv);
 			dist = calculate_distance<KIND>(point, thr, f, iters, w);
 			step *= (R)0.5;
 		}
