@@ -146,7 +146,8 @@ def measured_traffic(workload, w, h):
     workload (copy-calibrated), or None when there is none for this size."""
     name = {"mandelbulb8": "batch_kernel_bulb8_mandelbulb8_1080p.md",
             "material": "batch_kernel_bulb8_material_1080p.md",
-            "mandelbox": "batch_kernel_boxdelta_mandelbox_1080p.md"}.get(workload)
+            "mandelbox": "batch_kernel_boxdelta_mandelbox_1080p.md",
+            "hybrid": "batch_kernel_generic_hybrid002_1080p.md"}.get(workload)
     if not name or (w, h) != (1920, 1080):
         return None, None
     path = os.path.join(ROOT, "profiles", name)

@@ -76,7 +76,7 @@ __device__ __forceinline__ void gen_mandelbulb(const GenSlot &s, DV4 &z, GenAux 
 	z.z = sin(th) * rp;
 }
 
-// cFractalMandelbulbPower2::FormulaCode (fractal_mandelbulb_power2.cpp:28-43)
+// cFractalMandelbulbPower2::FormulaCode (fractal_mandelbulb_power2.cpp:29-44)
 __device__ __forceinline__ void gen_mandelbulb_power2(DV4 &z, GenAux &a)
 {
 #pragma clang fp contract(off)
@@ -91,7 +91,7 @@ __device__ __forceinline__ void gen_mandelbulb_power2(DV4 &z, GenAux &a)
 	z.z = newz;
 }
 
-// cFractalHypercomplex::FormulaCode (fractal_hypercomplex.cpp:28-41)
+// cFractalHypercomplex::FormulaCode (fractal_hypercomplex.cpp:31-44)
 __device__ __forceinline__ void gen_hypercomplex(DV4 &z, GenAux &a)
 {
 #pragma clang fp contract(off)
@@ -103,7 +103,7 @@ __device__ __forceinline__ void gen_hypercomplex(DV4 &z, GenAux &a)
 	z = dv4(newx, newy, newz, neww);
 }
 
-// cFractalQuaternion::FormulaCode (fractal_quaternion.cpp:28-39)
+// cFractalQuaternion::FormulaCode (fractal_quaternion.cpp:29-40)
 __device__ __forceinline__ void gen_quaternion(DV4 &z, GenAux &a)
 {
 #pragma clang fp contract(off)
@@ -116,7 +116,7 @@ __device__ __forceinline__ void gen_quaternion(DV4 &z, GenAux &a)
 	z.z = newz;
 }
 
-// cFractalMengerSponge::FormulaCode (fractal_menger_sponge.cpp:28-71), fold colour off
+// cFractalMengerSponge::FormulaCode (fractal_menger_sponge.cpp:30-71), fold colour off
 __device__ __forceinline__ void gen_menger_sponge(const GenSlot &s, DV4 &z, GenAux &a)
 {
 #pragma clang fp contract(off)
@@ -232,7 +232,7 @@ __device__ __forceinline__ void gen_kifs(const GenSlot &s, DV4 &z, GenAux &a)
 	a.DE *= fabs(s.ifsScale);
 }
 
-// cFractalXenodreambuie::FormulaCode (fractal_xenodreambuie.cpp:28-47)
+// cFractalXenodreambuie::FormulaCode (fractal_xenodreambuie.cpp:30-47)
 __device__ __forceinline__ void gen_xenodreambuie(const GenSlot &s, DV4 &z, GenAux &a)
 {
 #pragma clang fp contract(off)
@@ -249,7 +249,7 @@ __device__ __forceinline__ void gen_xenodreambuie(const GenSlot &s, DV4 &z, GenA
 	z.z = rp * cos(ph);
 }
 
-// cFractalBoxFoldBulbPow2::FormulaCode (fractal_box_fold_bulb_pow2.cpp:28-78)
+// cFractalBoxFoldBulbPow2::FormulaCode (fractal_box_fold_bulb_pow2.cpp:28-77)
 __device__ __forceinline__ void gen_box_fold_bulb_pow2(const GenSlot &s, DV4 &z, GenAux &a)
 {
 #pragma clang fp contract(off)
@@ -286,7 +286,7 @@ __device__ __forceinline__ void gen_box_fold_bulb_pow2(const GenSlot &s, DV4 &z,
 			s.adeOffset1;
 }
 
-// cFractalAexion::FormulaCode (fractal_aexion.cpp:28-64): the first iteration replaces the
+// cFractalAexion::FormulaCode (fractal_aexion.cpp:29-60): the first iteration replaces the
 // constant with a folded copy of the point
 __device__ __forceinline__ void gen_aexion(const GenSlot &s, DV4 &z, GenAux &a)
 {

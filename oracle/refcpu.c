@@ -777,7 +777,7 @@ static void kifs_iteration(const slot_t *s, v4 *zp, aux_t *aux)
 	*zp = z;
 }
 
-/* cFractalMandelbulbPower2::FormulaCode (fractal_mandelbulb_power2.cpp:28-43) */
+/* cFractalMandelbulbPower2::FormulaCode (fractal_mandelbulb_power2.cpp:29-44) */
 static void mandelbulb_power2_iteration(v4 *z, aux_t *aux)
 {
 	aux->DE = aux->DE * 2.0 * aux->r;
@@ -791,7 +791,7 @@ static void mandelbulb_power2_iteration(v4 *z, aux_t *aux)
 	z->z = newz;
 }
 
-/* cFractalHypercomplex::FormulaCode (fractal_hypercomplex.cpp:28-41) */
+/* cFractalHypercomplex::FormulaCode (fractal_hypercomplex.cpp:31-44) */
 static void hypercomplex_iteration(v4 *z, aux_t *aux)
 {
 	aux->DE = aux->DE * 2.0 * aux->r;
@@ -802,7 +802,7 @@ static void hypercomplex_iteration(v4 *z, aux_t *aux)
 	*z = V4(newx, newy, newz, neww);
 }
 
-/* cFractalQuaternion::FormulaCode (fractal_quaternion.cpp:28-39) */
+/* cFractalQuaternion::FormulaCode (fractal_quaternion.cpp:29-40) */
 static void quaternion_iteration(v4 *z, aux_t *aux)
 {
 	aux->DE = aux->DE * 2.0 * aux->r;
@@ -814,7 +814,7 @@ static void quaternion_iteration(v4 *z, aux_t *aux)
 	z->z = newz;
 }
 
-/* cFractalMengerSponge::FormulaCode (fractal_menger_sponge.cpp:28-71) with the fold colour
+/* cFractalMengerSponge::FormulaCode (fractal_menger_sponge.cpp:30-71) with the fold colour
  * (fold_color_aux_color_enabled_false) off */
 static void menger_sponge_iteration(const slot_t *s, v4 *zp, aux_t *aux)
 {
@@ -846,7 +846,7 @@ static void menger_sponge_iteration(const slot_t *s, v4 *zp, aux_t *aux)
 	*zp = z;
 }
 
-/* cFractalXenodreambuie::FormulaCode (fractal_xenodreambuie.cpp:28-47) */
+/* cFractalXenodreambuie::FormulaCode (fractal_xenodreambuie.cpp:30-47) */
 static void xenodreambuie_iteration(const slot_t *s, v4 *z, aux_t *aux)
 {
 	double rp = pow(aux->r, s->bulbPower - 1.0);
@@ -862,7 +862,7 @@ static void xenodreambuie_iteration(const slot_t *s, v4 *z, aux_t *aux)
 	z->z = rp * cos(ph);
 }
 
-/* cFractalBoxFoldBulbPow2::FormulaCode (fractal_box_fold_bulb_pow2.cpp:28-78) */
+/* cFractalBoxFoldBulbPow2::FormulaCode (fractal_box_fold_bulb_pow2.cpp:28-77) */
 static void box_fold_bulb_pow2_iteration(const slot_t *s, v4 *zp, aux_t *aux)
 {
 	v4 z = *zp;
@@ -900,7 +900,7 @@ static void box_fold_bulb_pow2_iteration(const slot_t *s, v4 *zp, aux_t *aux)
 	*zp = z;
 }
 
-/* cFractalAexion::FormulaCode (fractal_aexion.cpp:28-64): at the first iteration it replaces
+/* cFractalAexion::FormulaCode (fractal_aexion.cpp:29-60): at the first iteration it replaces
  * the constant with a folded copy of the point */
 static void aexion_iteration(const slot_t *s, v4 *zp, aux_t *aux)
 {
