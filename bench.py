@@ -429,6 +429,19 @@ def main():
         qwall, qms = time_kernel(torch, ctx, bufs, stream, psteps, 1)
         plain["persistent_kernel_ms_per_step"] = qwall * 1e3 / psteps
         ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_GRID)
+    image = None
+    if not args.no_plain:
+        # the image post-processing of the rendered frame (cImage::CompileImage +
+        # ConvertTo8bitChar): 12 bytes read, 6 + 3 written per pixel, HBM-bound
+        rgb16 = torch.empty_like(bufs["rgb"], dtype=torch.uint16)
+        rgb8 = torch.empty_like(bufs["rgb"], dtype=torch.uint8)
+        iwall, ims = time_kernel(torch, ctx, bufs, stream, args.steps, args.warmup,
+                                 lambda: ctx.compile_image(bufs["rgb"], p.image, rgb16, rgb8))
+        ibytes = 21 * p.imageWidth * p.imageHeight
+        iavg = sum(ims) / len(ims) / 1e3
+        image = {"kernel": "compile_image_kernel", "bound": "hbm", "avg_launch_ms": iavg * 1e3,
+                 "achieved": ibytes / iavg / 1e9, "peak": 8000.0, "unit": "GB/s",
+                 "frac": ibytes / iavg / 1e9 / 8000.0, "bytes_per_launch": ibytes}
     ctx.close()
     if args.calibrate_copy:
         a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
@@ -474,6 +487,8 @@ def main():
     if plain is not None:
         plain["batch_speedup"] = plain["ms_per_step"] / ms_per_step
         out["plain_kernel"] = plain
+    if image is not None:
+        out["image_compile"] = image
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(p, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MBX_ABI_VERSION 4
+#define MBX_ABI_VERSION 5
 
 /* array sizes as in mandelbulber2/src/fractal.h:52-54 */
 #define MBX_IFS_VECTOR_COUNT 9
@@ -47,6 +47,7 @@ extern "C" {
 #define MBX_ERR_UNSUPPORTED -2
 #define MBX_ERR_HIP -3
 #define MBX_ERR_NOMEM -4
+#define MBX_ERR_IO -5
 
 /* enumFractalFormula ids (mandelbulber2/formula/definition/all_fractal_list_enums.hpp:45-62) */
 enum mbx_formula {
@@ -187,6 +188,18 @@ typedef struct mbx_light {
 	double color[3];          /* "light1_color" as 0..1 */
 } mbx_light;
 
+/* sImageAdjustments (image_adjustments.h:38-47; fractparams.cpp:155-159): the post-processing
+ * cImage::CalculatePixel applies when it turns the float image into 16-bit RGB
+ * (cimage.cpp:317-361).  Defaults 1, 1, 1, 1, off (initparameters.cpp:208-212). */
+typedef struct mbx_image_adjustments {
+	float brightness;                 /* "brightness" */
+	float contrast;                   /* "contrast" */
+	float gamma;                      /* "gamma" */
+	float saturation;                 /* "saturation" */
+	int32_t hdr;                      /* "hdr": tanh tone mapping */
+	int32_t reserved;
+} mbx_image_adjustments;
+
 /* Render parameters: the sParamRender subset on the path
  * (mandelbulber2/src/fractparams.cpp:80-356), with material #1 and light #1. */
 typedef struct mbx_params {
@@ -318,6 +331,9 @@ typedef struct mbx_params {
 	double detailSizeMax;             /* "detail_size_max" */
 	double detailSizeMin;             /* "detail_size_min" */
 	double deltaDERelativeDelta;      /* "deltade_relative_delta" */
+
+	/* image adjustments for mbx_compile_image (not used by the render) */
+	mbx_image_adjustments image;
 } mbx_params;
 
 /* Per-pixel output planes (device pointers; any may be NULL except where noted).
@@ -539,6 +555,25 @@ typedef struct mbx_fract_info {
  * lines are malformed (the reference's limit, settings.cpp:888-901); info->message says which.
  * `info` may be NULL.  Host only; needs no context or device. */
 int mbx_load_fract(const char *text, size_t len, mbx_params *params, mbx_fract_info *info);
+
+/* cImage::CompileImage + ConvertTo8bitChar (mandelbulber2/src/cimage.cpp:317-397, 460-470): the
+ * float image `rgb` (3 floats per pixel, row-major, as mbx_outputs.rgb) of width x height pixels
+ * to 16-bit RGB `rgb16` through brightness, contrast about 0.5, the tanh tone map (hdr),
+ * saturation about the luma sqrt(0.299 R^2 + 0.587 G^2 + 0.114 B^2), truncation to 16 bits and the
+ * reference's 65536-entry gamma table int(powf(i / 65536, 1 / gamma) * 65535), and to 8-bit RGB
+ * `rgb8` = rgb16 / 256.  All three are device pointers; rgb16 or rgb8 may be NULL.  Asynchronous
+ * on the context stream (mbx_get_stats or a stream sync waits for it).  MBX_ERR_INVALID for a
+ * non-positive size or gamma, or both outputs NULL. */
+int mbx_compile_image(mbx_ctx *ctx, const mbx_image_adjustments *adj, const float *rgb, int32_t width,
+	int32_t height, uint16_t *rgb16, uint8_t *rgb8);
+
+/* ImageFileSavePNG::SavePNG (mandelbulber2/src/file_image.cpp:481-600): write `pixels` (host
+ * memory, row-major, `channels` samples per pixel: 3 = RGB as the colour image, 1 = grey as the
+ * z-buffer image; `bits` 8 = uint8_t samples, 16 = uint16_t samples, stored big-endian in the file
+ * as png_set_swap makes them) as a non-interlaced PNG file at `path`.  Host only.  Returns MBX_OK,
+ * MBX_ERR_INVALID for bad arguments, MBX_ERR_IO when the file cannot be written. */
+int mbx_write_png(const char *path, int32_t width, int32_t height, int32_t bits, int32_t channels,
+	const void *pixels);
 
 #ifdef __cplusplus
 }

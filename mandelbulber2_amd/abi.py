@@ -138,6 +138,17 @@ class MbxLight(C.Structure):
     ]
 
 
+class MbxImageAdjustments(C.Structure):
+    _fields_ = [
+        ("brightness", C.c_float),
+        ("contrast", C.c_float),
+        ("gamma", C.c_float),
+        ("saturation", C.c_float),
+        ("hdr", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
 class MbxParams(C.Structure):
     _fields_ = [
         ("imageWidth", C.c_int32),
@@ -229,6 +240,7 @@ class MbxParams(C.Structure):
         ("detailSizeMax", C.c_double),
         ("detailSizeMin", C.c_double),
         ("deltaDERelativeDelta", C.c_double),
+        ("image", MbxImageAdjustments),
     ]
 
 

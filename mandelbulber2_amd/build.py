@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmbx.so")
 
-SOURCES = ["mbx_render.hip", "mbx_diff.hip", "mbx_host.cpp", "mbx_fract.cpp"]
+SOURCES = ["mbx_render.hip", "mbx_diff.hip", "mbx_image.hip", "mbx_host.cpp", "mbx_fract.cpp", "mbx_png.cpp"]
 HEADERS = ["mbx_frame.h", "mbx_device.h", "mbx_launch.h", os.path.join("..", "..", "include", "mbx.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -34,7 +34,7 @@ def build(verbose=False, extra=()):
             print(" ".join(cmd), flush=True)
         subprocess.check_call(cmd)
         objs.append(obj)
-    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", LIB] + objs
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", LIB] + objs + ["-lz"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)

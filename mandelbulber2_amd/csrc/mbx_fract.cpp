@@ -620,6 +620,11 @@ extern "C" void mbx_params_default(mbx_params *p)
 	p->detailSizeMax = 1.0;
 	p->detailSizeMin = 1e-12;
 	p->deltaDERelativeDelta = 0.01;
+	p->image.brightness = 1.0f;
+	p->image.contrast = 1.0f;
+	p->image.gamma = 1.0f;
+	p->image.saturation = 1.0f;
+	p->image.hdr = 0;
 	// palette colouring of material #1 (InitMaterialParams)
 	p->useColorsFromPalette = 1;
 	p->surfaceGradientEnable = 1;
@@ -978,6 +983,12 @@ extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, 
 	p.detailSizeMax = M.d("detail_size_max", 1.0);
 	p.detailSizeMin = M.d("detail_size_min", 1e-12);
 	p.deltaDERelativeDelta = M.d("deltade_relative_delta", 0.01, 1e-15, 1e6);
+	// sImageAdjustments (fractparams.cpp:155-159): float members read from double parameters
+	p.image.brightness = (float)M.d("brightness", 1.0, 0.0, 1e15);
+	p.image.contrast = (float)M.d("contrast", 1.0, 0.0, 1e15);
+	p.image.gamma = (float)M.d("gamma", 1.0, 0.0, 1e15);
+	p.image.saturation = (float)M.d("saturation", 1.0, 0.0, 1e15);
+	p.image.hdr = M.b("hdr", false) ? 1 : 0;
 	if (p.iterThreshMode || p.interiorMode) flags |= MBX_FRACT_INTERIOR;
 
 	// ambient occlusion
@@ -1111,9 +1122,9 @@ extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, 
 		M.b("iteration_fog_enable", false) || M.b("clouds_enable", false))
 		flags |= MBX_FRACT_FOG;
 	if (M.b("textured_background", false) || M.b("env_mapping_enable", false)) flags |= MBX_FRACT_TEXTURES;
-	if (M.d("brightness", 1.0) != 1.0 || M.d("contrast", 1.0) != 1.0 || M.d("gamma", 1.0) != 1.0 ||
-		M.d("saturation", 1.0) != 1.0 || M.b("hdr", false) || M.b("hdr_blur_enabled", false) ||
-		M.b("DOF_enabled", false) || M.b("post_chromatic_aberration_enabled", false))
+	// brightness, contrast, gamma, saturation and hdr are read into params.image (mbx_compile_image)
+	if (M.b("hdr_blur_enabled", false) || M.b("DOF_enabled", false) ||
+		M.b("post_chromatic_aberration_enabled", false))
 		flags |= MBX_FRACT_POSTPROCESS;
 	if (doc.animation) flags |= MBX_FRACT_ANIMATION;
 	if (M.b("antialiasing_enabled", false) && M.i("antialiasing_size", 2, 1, 10) > 1) flags |= MBX_FRACT_ANTIALIASING;

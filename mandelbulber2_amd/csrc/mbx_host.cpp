@@ -68,6 +68,11 @@ struct mbx_ctx
 	mbx::DiffConst *d_diff = nullptr;
 	double *d_grad = nullptr;
 	bool diffUploaded = false;
+	// image post-processing: the gamma table of the last gamma (cImage::CalculateGammaTable)
+	uint16_t *d_gamma = nullptr;
+	float gammaOf = 0.0f;
+	bool gammaReady = false;
+	std::vector<uint16_t> gammaHost;
 };
 
 namespace
@@ -1435,6 +1440,7 @@ void mbx_ctx_destroy(mbx_ctx *ctx)
 	if (ctx->d_queue) (void)hipFree(ctx->d_queue);
 	if (ctx->d_diff) (void)hipFree(ctx->d_diff);
 	if (ctx->d_grad) (void)hipFree(ctx->d_grad);
+	if (ctx->d_gamma) (void)hipFree(ctx->d_gamma);
 	delete ctx;
 }
 
@@ -1755,6 +1761,39 @@ int mbx_distance(mbx_ctx *ctx, const double *points, int n, double detail_size, 
 	if (d_dist) (void)hipFree(d_dist);
 	if (d_it) (void)hipFree(d_it);
 	return rc;
+}
+
+int mbx_compile_image(mbx_ctx *ctx, const mbx_image_adjustments *adj, const float *rgb, int32_t width,
+	int32_t height, uint16_t *rgb16, uint8_t *rgb8)
+{
+	if (!ctx || !adj || !rgb || width <= 0 || height <= 0 || (!rgb16 && !rgb8))
+		return fail(ctx, MBX_ERR_INVALID, "bad argument");
+	if ((uint64_t)width * (uint64_t)height > 0xffffffffull) return fail(ctx, MBX_ERR_INVALID, "image too large");
+	if (!(adj->gamma >= 0.0f)) return fail(ctx, MBX_ERR_INVALID, "gamma must be >= 0");
+	hipError_t e = hipSetDevice(ctx->device);
+	if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+	if (!ctx->d_gamma && (e = hipMalloc((void **)&ctx->d_gamma, 65536 * sizeof(uint16_t))) != hipSuccess)
+		return hip_fail(ctx, e, "allocate gamma table");
+	if (!ctx->gammaReady || ctx->gammaOf != adj->gamma)
+	{
+		// cImage::CalculateGammaTable (cimage.cpp:363-376): int(powf(i / 65536, 1 / gamma) * 65535)
+		// for every 16-bit value, in float as the reference computes it
+		if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "sync");
+		ctx->gammaHost.resize(65536);
+		const float invGamma = 1.0f / adj->gamma;
+		for (uint32_t i = 0; i < 65536; i++)
+			ctx->gammaHost[i] = (uint16_t)(int)(powf((float)i / 65536.0f, invGamma) * 65535.0f);
+		if ((e = hipMemcpyAsync(ctx->d_gamma, ctx->gammaHost.data(), 65536 * sizeof(uint16_t),
+				 hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
+			return hip_fail(ctx, e, "copy gamma table");
+		ctx->gammaOf = adj->gamma;
+		ctx->gammaReady = true;
+	}
+	if ((e = mbx::launch_compile_image(rgb, (uint32_t)((uint64_t)width * (uint64_t)height), ctx->d_gamma,
+			 adj->brightness, adj->contrast, adj->saturation, adj->hdr ? 1 : 0, rgb16, rgb8, ctx->stream)) !=
+		hipSuccess)
+		return hip_fail(ctx, e, "compile image launch");
+	return MBX_OK;
 }
 
 size_t mbx_sizeof_params(void) { return sizeof(mbx_params); }

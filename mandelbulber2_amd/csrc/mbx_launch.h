@@ -29,6 +29,11 @@ hipError_t launch_render(int deKind, const FrameConst *frame, int width, int hei
 hipError_t launch_distance(int deKind, const FrameConst *frame, const float *pts, int n,
 	float detail, float *dist, int32_t *iters, hipStream_t stream);
 
+// Image post-processing of n pixels (3 floats each) through a 65536-entry gamma table (device);
+// out16 / out8 may be null.
+hipError_t launch_compile_image(const float *rgb, uint32_t n, const uint16_t *gammaTable, float brightness,
+	float contrast, float saturation, int hdr, uint16_t *out16, uint8_t *out8, hipStream_t stream);
+
 } // namespace mbx
 
 namespace mbx

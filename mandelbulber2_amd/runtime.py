@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(HERE, "libmbx.so")
 EXPORTS = ["mbx_version", "mbx_sizeof_params", "mbx_ctx_create", "mbx_ctx_destroy",
            "mbx_last_error", "mbx_scene_upload", "mbx_scene_info", "mbx_render", "mbx_distance",
            "mbx_get_stats", "mbx_render_batch", "mbx_tune", "mbx_get_tune", "mbx_autotune",
-           "mbx_render_diff", "mbx_adjoint", "mbx_params_default", "mbx_load_fract"]
+           "mbx_render_diff", "mbx_adjoint", "mbx_params_default", "mbx_load_fract",
+           "mbx_compile_image", "mbx_write_png"]
 
 _lib = None
 
@@ -56,6 +57,9 @@ def load():
     L.mbx_params_default.argtypes = [P(abi.MbxParams)]
     L.mbx_params_default.restype = None
     L.mbx_load_fract.argtypes = [C.c_char_p, C.c_size_t, P(abi.MbxParams), P(abi.MbxFractInfo)]
+    L.mbx_compile_image.argtypes = [C.c_void_p, P(abi.MbxImageAdjustments), C.c_void_p, C.c_int32,
+                                    C.c_int32, C.c_void_p, C.c_void_p]
+    L.mbx_write_png.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
     _lib = L
     return L
 
@@ -78,6 +82,21 @@ def load_fract(text):
     if rc != 0:
         raise MbxError("mbx_load_fract: %s (%d)" % (info.message.decode(errors="replace"), rc))
     return p, info
+
+
+def write_png(path, pixels):
+    """ImageFileSavePNG::SavePNG through mbx_write_png: `pixels` is a C-contiguous numpy array of
+    shape (H, W, 3) or (H, W), dtype uint8 or uint16."""
+    import numpy as np
+    a = np.ascontiguousarray(pixels)
+    if a.dtype not in (np.uint8, np.uint16) or a.ndim not in (2, 3) or (a.ndim == 3 and a.shape[2] != 3):
+        raise MbxError("write_png: (H, W, 3) or (H, W) uint8 / uint16 pixels")
+    bits = 8 if a.dtype == np.uint8 else 16
+    channels = 3 if a.ndim == 3 else 1
+    rc = load().mbx_write_png(str(path).encode(), a.shape[1], a.shape[0], bits, channels,
+                              a.ctypes.data_as(C.c_void_p))
+    if rc != 0:
+        raise MbxError("mbx_write_png failed (%d)" % rc)
 
 
 MBX_TUNE_KERNEL = 0
@@ -173,6 +192,25 @@ class Context:
         arr = (abi.MbxParams * n)(*params_list)
         outs = (abi.MbxOutputs * n)(*[self.outputs_struct(b) for b in bufs_list])
         self._check(self.L.mbx_render_batch(self.h, arr, outs, n), "mbx_render_batch")
+
+    def compile_image(self, rgb, adj=None, rgb16=None, rgb8=None):
+        """cImage::CompileImage + ConvertTo8bitChar on the device: `rgb` is an (H, W, 3) float32
+        CUDA tensor (the render's rgb plane); returns the (H, W, 3) uint16 and uint8 tensors.
+        `adj` defaults to the uploaded scene's image adjustments."""
+        torch = self.torch
+        if adj is None:
+            adj = self.params.image
+        H, W = rgb.shape[0], rgb.shape[1]
+        if rgb.dtype != torch.float32 or not rgb.is_contiguous() or tuple(rgb.shape) != (H, W, 3):
+            raise MbxError("compile_image: contiguous (H, W, 3) float32 image")
+        if rgb16 is None:
+            rgb16 = torch.empty((H, W, 3), dtype=torch.uint16, device=rgb.device)
+        if rgb8 is None:
+            rgb8 = torch.empty((H, W, 3), dtype=torch.uint8, device=rgb.device)
+        self._check(self.L.mbx_compile_image(self.h, C.byref(adj), C.c_void_p(rgb.data_ptr()), W, H,
+                                             C.c_void_p(rgb16.data_ptr()), C.c_void_p(rgb8.data_ptr())),
+                    "mbx_compile_image")
+        return rgb16, rgb8
 
     def tune(self, key, value):
         self._check(self.L.mbx_tune(self.h, key, value), "mbx_tune")

@@ -151,6 +151,8 @@ def default_params(width=640, height=480):
     p.detailSizeMax = 1.0
     p.detailSizeMin = 1e-12
     p.deltaDERelativeDelta = 0.01
+    p.image.brightness = p.image.contrast = p.image.gamma = p.image.saturation = 1.0
+    p.image.hdr = 0
     # palette colouring of material #1 (InitMaterialParams, initparameters.cpp:1600-1810)
     p.useColorsFromPalette = 1
     p.surfaceGradientEnable = 1

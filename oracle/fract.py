@@ -17,6 +17,7 @@ Every function cites the reference lines it follows.  Parity pins: tests/test_fr
 hand-read values of example files and the closed forms of the version migrations.
 """
 import math
+import struct
 
 # ------------------------------------------------------------------------------------------
 # typed values (cMultiVal)
@@ -782,6 +783,10 @@ def load(text):
                  ("detailSizeMax", "detail_size_max"), ("detailSizeMin", "detail_size_min"),
                  ("deltaDERelativeDelta", "deltade_relative_delta")):
         out[k] = g(v)
+    # sImageAdjustments (fractparams.cpp:155-159): floats
+    for k in ("brightness", "contrast", "gamma", "saturation"):
+        out["image." + k] = struct.unpack("f", struct.pack("f", g(k)))[0]
+    out["image.hdr"] = int(g("hdr"))
     if g("iteration_threshold_mode") or g("interior_mode"):
         flags |= FLAGS["interior"]
     out["ambientOcclusion"] = g("ambient_occlusion")
@@ -895,9 +900,7 @@ def load(text):
         flags |= FLAGS["fog"]
     if g("textured_background") or g("env_mapping_enable"):
         flags |= FLAGS["textures"]
-    if g("brightness") != 1.0 or g("contrast") != 1.0 or g("gamma") != 1.0 or \
-            g("saturation") != 1.0 or g("hdr") or g("hdr_blur_enabled") or g("DOF_enabled") or \
-            g("post_chromatic_aberration_enabled"):
+    if g("hdr_blur_enabled") or g("DOF_enabled") or g("post_chromatic_aberration_enabled"):
         flags |= FLAGS["postprocess"]
     if animation:
         flags |= FLAGS["animation"]

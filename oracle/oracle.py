@@ -52,6 +52,7 @@ def lib(fast=False):
     L.mbxo_colour_index.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_int, C.c_void_p]
     L.mbxo_gradient.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_int, C.c_void_p]
     L.mbxo_surface_colour.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_int, C.c_void_p]
+    L.mbxo_compile_image.argtypes = [P(abi.MbxImageAdjustments), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
     L.mbxo_random_int.argtypes = [P(C.c_uint32)]
     L.mbxo_random.argtypes = [P(C.c_uint32), C.c_int]
     L.mbxo_pixel_seed.argtypes = [C.c_uint32, C.c_uint32]
@@ -169,6 +170,18 @@ def surface_colour(p, pts):
     if rc != 0:
         raise RuntimeError("oracle surface_colour failed: %d" % rc)
     return rgb
+
+
+def compile_image(adj, rgb):
+    """CalculatePixel + ConvertTo8bitChar over an (..., 3) float32 image: (uint16, uint8) arrays."""
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    n = rgb.size // 3
+    out16 = np.zeros(rgb.shape, np.uint16)
+    out8 = np.zeros(rgb.shape, np.uint8)
+    rc = lib().mbxo_compile_image(C.byref(adj), _ptr(rgb), n, _ptr(out16), _ptr(out8))
+    if rc != 0:
+        raise RuntimeError("oracle compile_image failed: %d" % rc)
+    return out16, out8
 
 
 def frame_constants(p):

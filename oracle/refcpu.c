@@ -2678,3 +2678,59 @@ int mbxo_diff_de(const mbx_params *p, const double *pts, int n, double *de, doub
 	}
 	return MBX_OK;
 }
+
+/* cImage::CalculateGammaTable + CalculatePixel + ConvertTo8bitChar (cimage.cpp:317-376, 378-397,
+ * 460-470) over n pixels of a float RGB image, in float as the reference computes them:
+ * brightness, contrast about 0.5, the clamp at 0, tanhf with hdr, saturation about
+ * V = sqrtf(R R 0.299 + G G 0.587 + B B 0.114), the clamp to [0, 1], quint16(x * 65535) and the
+ * table int(powf(i / 65536, 1 / gamma) * 65535); 8-bit = 16-bit / 256.  out16 / out8 may be NULL. */
+int mbxo_compile_image(const mbx_image_adjustments *adj, const float *rgb, int n, uint16_t *out16,
+	uint8_t *out8)
+{
+	if (!adj || !rgb || n < 0 || !(adj->gamma >= 0.0f)) return MBX_ERR_INVALID;
+	static int table[65536];
+	const float invGamma = 1.0f / adj->gamma;
+	for (int i = 0; i < 65536; i++) table[i] = (int)(powf((float)i / 65536.0f, invGamma) * 65535.0f);
+	for (int i = 0; i < n; i++)
+	{
+		float R = rgb[3 * i] * adj->brightness;
+		float G = rgb[3 * i + 1] * adj->brightness;
+		float B = rgb[3 * i + 2] * adj->brightness;
+		R = (R - 0.5f) * adj->contrast + 0.5f;
+		G = (G - 0.5f) * adj->contrast + 0.5f;
+		B = (B - 0.5f) * adj->contrast + 0.5f;
+		if (R < 0.0f) R = 0.0f;
+		if (G < 0.0f) G = 0.0f;
+		if (B < 0.0f) B = 0.0f;
+		if (adj->hdr)
+		{
+			R = tanhf(R);
+			G = tanhf(G);
+			B = tanhf(B);
+		}
+		const float rFactor = 0.299f, gFactor = 0.587f, bFactor = 0.114f;
+		const float V = sqrtf(R * R * rFactor + G * G * gFactor + B * B * bFactor);
+		R = V + (R - V) * adj->saturation;
+		G = V + (G - V) * adj->saturation;
+		B = V + (B - V) * adj->saturation;
+		R = R < 0.0f ? 0.0f : (R > 1.0f ? 1.0f : R);
+		G = G < 0.0f ? 0.0f : (G > 1.0f ? 1.0f : G);
+		B = B < 0.0f ? 0.0f : (B > 1.0f ? 1.0f : B);
+		const uint16_t r16 = (uint16_t)table[(uint16_t)(R * 65535.0f)];
+		const uint16_t g16 = (uint16_t)table[(uint16_t)(G * 65535.0f)];
+		const uint16_t b16 = (uint16_t)table[(uint16_t)(B * 65535.0f)];
+		if (out16)
+		{
+			out16[3 * i] = r16;
+			out16[3 * i + 1] = g16;
+			out16[3 * i + 2] = b16;
+		}
+		if (out8)
+		{
+			out8[3 * i] = (uint8_t)(r16 / 256);
+			out8[3 * i + 1] = (uint8_t)(g16 / 256);
+			out8[3 * i + 2] = (uint8_t)(b16 / 256);
+		}
+	}
+	return MBX_OK;
+}

@@ -149,7 +149,11 @@ def test_mandelbulb001(load):
     assert r["light.enabled"] == 1 and r["light.rotation"] == [-45.0, 45.0, 0.0]
     assert r["light.intensity"] == 1.0
     assert r["ambientOcclusionEnabled"] == 1 and r["ambientOcclusionMode"] == abi.MBX_AO_SCREEN_SPACE
-    assert r.flags() == {"fog", "ao_screen_space", "postprocess"}
+    assert r.flags() == {"fog", "ao_screen_space"}
+    # image adjustments: float members of sImageAdjustments (fractparams.cpp:155-159)
+    assert r["image.brightness"] == float(np.float32(1.28))
+    assert r["image.gamma"] == float(np.float32(0.6899999999999999))
+    assert r["image.contrast"] == 1.0 and r["image.saturation"] == 1.0 and r["image.hdr"] == 0
     # mat1_surface_color_gradient "0 fd6029 999 698403 1999 fff59b ... 9000 aba53c": 10 stops and
     # the first colour again at 1.0, positions as float(n) / 10000
     assert r["surfaceGradientCount"] == 11
@@ -168,7 +172,8 @@ def test_mandelbox001(load):
     assert r["light.enabled"] == 1 and r["light.intensity"] == 4.0
     assert r["light.rotation"] == [-45.0, 45.0, 0.0]
     assert r["reflectionsMax"] == 3
-    assert r.flags() == {"fog", "reflections", "postprocess"}
+    assert r.flags() == {"fog", "reflections"}
+    assert r["image.brightness"] == 2.0 and r["image.hdr"] == 1
     assert r["useColorsFromPalette"] == 0 and r["specular"] == 1.0 and r["metallic"] == 0
 
 
@@ -185,7 +190,8 @@ def test_ifs001_pre_2_25_lights(load):
     assert r["light.enabled"] == 0
     assert r["ambientOcclusion"] == 1.04 and r["ambientOcclusionQuality"] == 3
     assert r["ambientOcclusionMode"] == abi.MBX_AO_MULTIPLE_RAYS
-    assert {"lights", "ao_lightmap", "postprocess"} <= r.flags()
+    assert {"lights", "ao_lightmap"} <= r.flags() and "postprocess" not in r.flags()
+    assert r["image.hdr"] == 1
 
 
 def test_pre_2_07_keyframe_file(load):
@@ -213,7 +219,8 @@ def test_mandelbulb_power_8_global_constant_factor(load):
     assert r["DEFactor"] == 0.3 and r["fov"] == 28.07 and r["detailLevel"] == 4.0
     assert r["light.rotation"] == [-45.0, -45.0, 0.0] and r["light.intensity"] == 0.3
     assert r["background3ColorsEnable"] == 0 and r["backgroundColor1"] == [0.0, 0.0, 0.0]
-    assert {"ao_lightmap", "antialiasing", "postprocess"} <= r.flags()
+    assert {"ao_lightmap", "antialiasing"} <= r.flags() and "postprocess" not in r.flags()
+    assert r["image.gamma"] == float(np.float32(0.7))
     assert "palette" not in r.flags()
 
 

@@ -1,0 +1,87 @@
+"""The oracle's image post-processing (cImage::CalculatePixel, CalculateGammaTable and
+ConvertTo8bitChar, mandelbulber2/src/cimage.cpp:317-376, 460-470) against what fixes it:
+
+  * gamma 1: the table int(powf(i / 65536, 1) * 65535) is i - 1 for every i >= 1 (i - i/65536 is
+    never rounded up to i in float), 0 at 0;
+  * saturation 0 makes every channel the luma V = sqrt(0.299 R^2 + 0.587 G^2 + 0.114 B^2);
+  * contrast pivots about 0.5, after brightness: a pixel at 0.5 / brightness is unchanged by any
+    contrast;
+  * values below 0 are cut before the tone map; hdr maps x to tanh(x);
+  * gamma g maps a 16-bit value v to about (v / 65536)^(1/g) * 65535;
+  * 8-bit = 16-bit / 256.
+"""
+import numpy as np
+
+from mandelbulber2_amd import abi
+
+
+def _adj(brightness=1.0, contrast=1.0, gamma=1.0, saturation=1.0, hdr=0):
+    a = abi.MbxImageAdjustments()
+    a.brightness, a.contrast, a.gamma, a.saturation, a.hdr = brightness, contrast, gamma, saturation, hdr
+    return a
+
+
+def _index(x):
+    """the 16-bit index of a channel value after the clamp to [0, 1]"""
+    return np.floor(np.clip(np.asarray(x, np.float64), 0.0, 1.0) * 65535.0).astype(np.int64)
+
+
+def test_gamma_one_table_is_i_minus_one(oracle_mod):
+    # saturation 0: every channel is V, whose 16-bit index is computed here in float64 (+-1)
+    rs = np.random.default_rng(1)
+    rgb = rs.uniform(0.0, 1.0, size=(4000, 3)).astype(np.float32)
+    out16, _ = oracle_mod.compile_image(_adj(saturation=0.0), rgb)
+    assert (out16[:, 0] == out16[:, 1]).all() and (out16[:, 1] == out16[:, 2]).all()
+    r, g, b = (rgb[:, k].astype(np.float64) for k in range(3))
+    v = np.sqrt(0.299 * r * r + 0.587 * g * g + 0.114 * b * b)
+    idx = _index(v)
+    want = np.maximum(idx - 1, 0)
+    assert np.abs(out16[:, 0].astype(np.int64) - want).max() <= 1
+    # exact on the values whose index is 1 away from a float boundary of V * 65535
+    frac = v * 65535.0 - np.floor(v * 65535.0)
+    safe = (frac > 0.01) & (frac < 0.99)
+    assert safe.mean() > 0.9
+    np.testing.assert_array_equal(out16[safe, 0], want[safe])
+
+
+def test_contrast_pivot_and_brightness_order(oracle_mod):
+    rgb = np.array([[0.5, 0.5, 0.5], [0.25, 0.25, 0.25]], np.float32)
+    for contrast in (0.3, 1.0, 2.7):
+        out16, _ = oracle_mod.compile_image(_adj(contrast=contrast, saturation=0.0), rgb[:1])
+        assert (out16 == 32766).all()                      # index 32767, gamma-1 table i - 1
+        out16, _ = oracle_mod.compile_image(_adj(brightness=2.0, contrast=contrast, saturation=0.0), rgb[1:])
+        assert (out16 == 32766).all()
+    out16, _ = oracle_mod.compile_image(_adj(contrast=2.0, saturation=0.0), rgb[1:])
+    assert (out16 == 0).all()                              # (0.25 - 0.5) * 2 + 0.5 = 0
+
+
+def test_negative_values_are_cut_before_saturation_and_hdr(oracle_mod):
+    rgb = np.array([[-0.7, 0.4, 0.4]], np.float32)
+    a, _ = oracle_mod.compile_image(_adj(saturation=0.0), rgb)
+    b, _ = oracle_mod.compile_image(_adj(saturation=0.0), np.array([[0.0, 0.4, 0.4]], np.float32))
+    np.testing.assert_array_equal(a, b)
+    grey = np.full((200, 3), 0.0, np.float32)
+    grey[:, :] = np.linspace(0.0, 3.0, 200, dtype=np.float32)[:, None]
+    out16, _ = oracle_mod.compile_image(_adj(hdr=1, saturation=0.0), grey)
+    want = np.maximum(_index(np.tanh(grey[:, 0].astype(np.float64))) - 1, 0)
+    assert np.abs(out16[:, 0].astype(np.int64) - want).max() <= 1
+
+
+def test_gamma_curve(oracle_mod):
+    grey = np.linspace(0.2, 1.0, 300, dtype=np.float32)
+    rgb = np.repeat(grey[:, None], 3, axis=1)
+    for gamma in (0.45, 2.2):
+        out16, _ = oracle_mod.compile_image(_adj(gamma=gamma, saturation=0.0), rgb)
+        idx = _index(grey.astype(np.float64))
+        want = np.floor((idx / 65536.0) ** (1.0 / gamma) * 65535.0)
+        assert np.abs(out16[:, 0] - want).max() <= 2, gamma
+    out16, _ = oracle_mod.compile_image(_adj(gamma=0.0, saturation=0.0), rgb)
+    assert (out16 == 0).all()                              # 1 / 0 = inf: every table entry 0
+
+
+def test_eight_bit_is_sixteen_bit_over_256(oracle_mod):
+    rs = np.random.default_rng(2)
+    rgb = rs.uniform(-0.2, 1.4, size=(3000, 3)).astype(np.float32)
+    out16, out8 = oracle_mod.compile_image(_adj(brightness=1.2, contrast=0.9, gamma=1.8, saturation=1.3), rgb)
+    np.testing.assert_array_equal(out8, (out16 // 256).astype(np.uint8))
+    assert out16.max() > 60000 and out16.min() == 0
