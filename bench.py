@@ -144,7 +144,8 @@ def parse():
 def measured_traffic(workload, w, h):
     """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this
     workload (copy-calibrated), or None when there is none for this size."""
-    name = {"mandelbulb8": "batch_kernel_bulb8_mandelbulb8_1080p.md",
+    name = {"image": "compile_image_kernel_1080p.md",
+            "mandelbulb8": "batch_kernel_bulb8_mandelbulb8_1080p.md",
             "material": "batch_kernel_bulb8_material_1080p.md",
             "mandelbox": "batch_kernel_boxdelta_mandelbox_1080p.md",
             "hybrid": "batch_kernel_generic_hybrid002_1080p.md"}.get(workload)
@@ -439,9 +440,11 @@ def main():
                                  lambda: ctx.compile_image(bufs["rgb"], p.image, rgb16, rgb8))
         ibytes = 21 * p.imageWidth * p.imageHeight
         iavg = sum(ims) / len(ims) / 1e3
+        itraffic, isrc = measured_traffic("image", p.imageWidth, p.imageHeight)
         image = {"kernel": "compile_image_kernel", "bound": "hbm", "avg_launch_ms": iavg * 1e3,
                  "achieved": ibytes / iavg / 1e9, "peak": 8000.0, "unit": "GB/s",
-                 "frac": ibytes / iavg / 1e9 / 8000.0, "bytes_per_launch": ibytes}
+                 "frac": ibytes / iavg / 1e9 / 8000.0, "bytes_per_launch": ibytes,
+                 "traffic": itraffic, "traffic_source": isrc}
     ctx.close()
     if args.calibrate_copy:
         a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")

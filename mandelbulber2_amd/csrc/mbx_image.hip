@@ -11,36 +11,72 @@
 namespace mbx
 {
 
-__global__ __launch_bounds__(256) void compile_image_kernel(const float *__restrict__ rgb, uint32_t n,
-	const uint16_t *__restrict__ gammaTable, float brightness, float contrast, float saturation, int hdr,
-	uint16_t *__restrict__ out16, uint8_t *__restrict__ out8)
+// cImage::CalculatePixel of one pixel: 16-bit R, G, B
+__device__ __forceinline__ void calculate_pixel(float r, float g, float b, const uint16_t *__restrict__ gammaTable,
+	float brightness, float contrast, float saturation, int hdr, uint16_t o[3])
 {
 #pragma clang fp contract(off)
+	float c[3] = {r, g, b};
+	for (int j = 0; j < 3; j++)
+	{
+		c[j] = c[j] * brightness;
+		c[j] = (c[j] - 0.5f) * contrast + 0.5f;
+		if (c[j] < 0.0f) c[j] = 0.0f;
+		if (hdr) c[j] = tanhf(c[j]);
+	}
+	// saturation about the luma (rFactor 0.299, gFactor 0.587, bFactor 0.114).  sqrtf must be
+	// correctly rounded as the host's is; v_sqrt_f32 is not (1 ulp), so the root is taken in
+	// FP64 (correctly rounded) and rounded to float, which is the correctly rounded float root
+	// (53 >= 2 * 24 + 2 bits).
+	const float s = c[0] * c[0] * 0.299f + c[1] * c[1] * 0.587f + c[2] * c[2] * 0.114f;
+	const float V = (float)sqrt((double)s);
+	for (int j = 0; j < 3; j++)
+	{
+		float x = V + (c[j] - V) * saturation;
+		x = x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x);
+		o[j] = gammaTable[(uint16_t)(x * 65535.0f)];
+	}
+}
+
+// Four pixels per thread: three 16-byte loads (12 floats), three 8-byte stores of 16-bit samples
+// and three 4-byte stores of 8-bit samples; rgb 16-byte, out16 8-byte and out8 4-byte aligned (the
+// host checks).  Pixels past the last whole group of four go through the one-pixel path.
+__global__ __launch_bounds__(256) void compile_image_kernel(const float *__restrict__ rgb, uint32_t n,
+	const uint16_t *__restrict__ gammaTable, float brightness, float contrast, float saturation, int hdr,
+	uint16_t *__restrict__ out16, uint8_t *__restrict__ out8, int vec4)
+{
 	const uint32_t stride = gridDim.x * blockDim.x;
-	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+	const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+	const uint32_t quads = vec4 ? n / 4 : 0;
+	for (uint32_t q = tid; q < quads; q += stride)
+	{
+		const float4 *in = (const float4 *)rgb + 3 * (size_t)q;
+		const float4 a = in[0], b = in[1], c = in[2];
+		const float f[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
+		uint16_t o[12];
+		for (int k = 0; k < 4; k++)
+			calculate_pixel(f[3 * k], f[3 * k + 1], f[3 * k + 2], gammaTable, brightness, contrast, saturation, hdr,
+				o + 3 * k);
+		if (out16)
+		{
+			uint2 *d = (uint2 *)out16 + 3 * (size_t)q;
+			for (int w = 0; w < 3; w++)
+				d[w] = make_uint2((uint32_t)o[4 * w] | ((uint32_t)o[4 * w + 1] << 16),
+					(uint32_t)o[4 * w + 2] | ((uint32_t)o[4 * w + 3] << 16));
+		}
+		if (out8)
+		{
+			uint32_t *d = (uint32_t *)out8 + 3 * (size_t)q;
+			for (int w = 0; w < 3; w++)
+				d[w] = (uint32_t)(o[4 * w] >> 8) | ((uint32_t)(o[4 * w + 1] >> 8) << 8) |
+					((uint32_t)(o[4 * w + 2] >> 8) << 16) | ((uint32_t)(o[4 * w + 3] >> 8) << 24);
+		}
+	}
+	for (uint32_t i = 4 * quads + tid; i < n; i += stride)
 	{
 		const size_t k = 3 * (size_t)i;
-		float c[3] = {rgb[k], rgb[k + 1], rgb[k + 2]};
-		for (int j = 0; j < 3; j++)
-		{
-			c[j] = c[j] * brightness;
-			c[j] = (c[j] - 0.5f) * contrast + 0.5f;
-			if (c[j] < 0.0f) c[j] = 0.0f;
-			if (hdr) c[j] = tanhf(c[j]);
-		}
-		// saturation about the luma (rFactor 0.299, gFactor 0.587, bFactor 0.114).  sqrtf must be
-		// correctly rounded as the host's is; v_sqrt_f32 is not (1 ulp), so the root is taken in
-		// FP64 (correctly rounded) and rounded to float, which is the correctly rounded float root
-		// (53 >= 2 * 24 + 2 bits).
-		const float s = c[0] * c[0] * 0.299f + c[1] * c[1] * 0.587f + c[2] * c[2] * 0.114f;
-		const float V = (float)sqrt((double)s);
 		uint16_t o[3];
-		for (int j = 0; j < 3; j++)
-		{
-			float x = V + (c[j] - V) * saturation;
-			x = x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x);
-			o[j] = gammaTable[(uint16_t)(x * 65535.0f)];
-		}
+		calculate_pixel(rgb[k], rgb[k + 1], rgb[k + 2], gammaTable, brightness, contrast, saturation, hdr, o);
 		if (out16)
 		{
 			out16[k] = o[0];
@@ -59,10 +95,12 @@ __global__ __launch_bounds__(256) void compile_image_kernel(const float *__restr
 hipError_t launch_compile_image(const float *rgb, uint32_t n, const uint16_t *gammaTable, float brightness,
 	float contrast, float saturation, int hdr, uint16_t *out16, uint8_t *out8, hipStream_t stream)
 {
-	const uint32_t blocks = (n + 255u) / 256u;
-	const uint32_t grid = blocks < 65536u ? blocks : 65536u;
+	const bool vec4 = ((uintptr_t)rgb % 16 == 0) && ((uintptr_t)out16 % 8 == 0) && ((uintptr_t)out8 % 4 == 0);
+	const uint32_t items = vec4 ? (n / 4 + (n % 4)) : n;
+	const uint32_t blocks = (items + 255u) / 256u;
+	const uint32_t grid = blocks < 65536u ? (blocks ? blocks : 1u) : 65536u;
 	hipLaunchKernelGGL(compile_image_kernel, dim3(grid), dim3(256), 0, stream, rgb, n, gammaTable, brightness,
-		contrast, saturation, hdr, out16, out8);
+		contrast, saturation, hdr, out16, out8, vec4 ? 1 : 0);
 	return hipGetLastError();
 }
 

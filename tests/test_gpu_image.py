@@ -58,6 +58,29 @@ def test_compile_image_parity(rt, oracle_mod, name):
         assert np.abs(g8.astype(int) - o8.astype(int)).max() <= 1
 
 
+def test_unaligned_buffers_take_the_one_pixel_path(rt, oracle_mod):
+    """Buffers offset by one pixel (12, 6 and 3 bytes) are not 16/8/4-byte aligned: the kernel's
+    one-pixel path must give the same samples as the four-pixel path."""
+    import torch
+    adj = CASES["adjusted"]
+    rs = np.random.default_rng(12)
+    H, W = 37, 61
+    img = rs.uniform(-0.3, 1.6, size=(H * W + 1, 3)).astype(np.float32)
+    flat = torch.from_numpy(img).cuda()
+    o16 = torch.zeros((H * W + 1, 3), dtype=torch.uint16, device="cuda")
+    o8 = torch.zeros((H * W + 1, 3), dtype=torch.uint8, device="cuda")
+    ctx = rt.Context(0)
+    ctx.params = scene.default_params(8, 8)
+    print("[case] compile_image unaligned %dx%d" % (W, H), flush=True)
+    ctx.compile_image(flat[1:].view(H, W, 3), adj, o16[1:].view(H, W, 3), o8[1:].view(H, W, 3))
+    torch.cuda.synchronize()
+    w16, w8 = oracle_mod.compile_image(adj, img[1:])
+    np.testing.assert_array_equal(o16[1:].cpu().numpy(), w16)
+    np.testing.assert_array_equal(o8[1:].cpu().numpy(), w8)
+    assert (o16[0].cpu().numpy() == 0).all() and (o8[0].cpu().numpy() == 0).all()
+    ctx.close()
+
+
 def test_rendered_frame_with_the_scene_adjustments(rt, oracle_mod):
     """A rendered Mandelbulb frame compiled with the image adjustments of its parameter block, and
     a second compile with another gamma (the gamma table is rebuilt) on the same context."""
