@@ -309,7 +309,7 @@ __global__ __launch_bounds__(64) void render_diff_kernel(const FrameConst *__res
 	const R ox = (Q::invWidth(f) * (R)xs - (R)0.5) * Q::aspect(f);
 	const R oy = (R)0.5 - Q::invHeight(f) * (R)ys;
 	MarchOut<KIND> m;
-	ray_marching<KIND>(f, Q::cam(f), Q::view(f, ox, oy), (R)f.viewDistanceMax, seed, m, w);
+	ray_marching<KIND, 2>(f, Q::cam(f), Q::view(f, ox, oy), (R)f.viewDistanceMax, seed, m, w);
 
 	DV3 vv, d;
 	diff_view(c, xs, ys, vv, d);

@@ -219,4 +219,23 @@ inline uint32_t pixel_seed(uint32_t pixel_index, uint32_t frame_seed)
 	return 1u + h % 2147483646u;
 }
 
+#if defined(__HIP__) // HIP sources only (the host .cpp files include this header as C++)
+// An out-of-line function (ours, or one the compiler chose not to inline) sees every argument as
+// possibly different per lane, so it would read the frame constants with vector loads into VGPRs.
+// Every lane passes the same frame: taking the address from the first active lane makes it
+// provably uniform, and reading it through the constant address space lets the compiler use
+// scalar loads.
+template <class T>
+__device__ __forceinline__ const T &uniform_ref(const T &r)
+{
+	const uint64_t v = (uint64_t)(uintptr_t)&r;
+	const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+	const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+	typedef const __attribute__((address_space(4))) T ConstT;
+	ConstT *c = (ConstT *)(uintptr_t)(((uint64_t)hi << 32) | lo);
+	return *(const T *)c;
+}
+
+#endif
+
 } // namespace mbx

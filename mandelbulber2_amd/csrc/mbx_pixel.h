@@ -94,7 +94,9 @@ __device__ __forceinline__ typename PrecOf<KIND>::R calc_delta(const FrameConst 
 // RayMarching (render_worker.cpp:708-951), exterior mode, binary search on.  In the FP64
 // position precision the arithmetic follows the reference's operation order without FMA
 // contraction, so that march decisions at the DE's discontinuities come out the same.
-template <int KIND>
+// ADV: 0 = advanced quality off, 1 = on (both compiled into the FP32 kinds, so the default march
+// carries no test of the flag), 2 = read from the frame (FP64 kinds: one copy of the march)
+template <int KIND, int ADV>
 __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start, typename PrecOf<KIND>::P dir,
 	typename PrecOf<KIND>::R maxScan, uint32_t &seed, MarchOut<KIND> &out, Work &w)
 {
@@ -116,7 +118,7 @@ __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start
 	const R thrScale = kExact ? (R)0 : dirLength * (R)f.threshK;
 	const R constThrValue = Q::DEThresh(f);
 	const int maxSteps = f.maxRaymarchingSteps;
-	const bool adv = f.advancedQuality != 0; // advanced-quality threshold and step limits
+	const bool adv = ADV == 2 ? f.advancedQuality != 0 : ADV == 1; // advanced-quality threshold and step limits
 	// far field (FP32 Mandelbulb kinds): at |p|^2 >= farR2 the host has bounded the DE from the
 	// first formula iteration (|z_1| >= |p|^P - |c|, escape at i = 0, iters = 1) above the value
 	// that makes the step below clamp to exactly 3, so the step is taken without evaluating it;
@@ -208,10 +210,11 @@ __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start
 
 // CalculateNormals, fast mode: central differences of the DE (shader_calculate_normals.cpp:44-78)
 template <int KIND>
-__device__ V3 calculate_normals(const FrameConst &f, typename PrecOf<KIND>::P p, typename PrecOf<KIND>::R thr,
+__device__ V3 calculate_normals(const FrameConst &frame, typename PrecOf<KIND>::P p, typename PrecOf<KIND>::R thr,
 	Work &w)
 {
 #pragma clang fp contract(off)
+	const FrameConst &f = uniform_ref(frame);
 	using Q = PrecOf<KIND>;
 	using R = typename Q::R;
 	using P = typename Q::P;
@@ -357,8 +360,9 @@ __device__ float fast_ambient_occlusion(const FrameConst &f, typename PrecOf<KIN
 }
 
 // BackgroundShader: 3-colour gradient and the directional light's disk (shader_background.cpp)
-__device__ V3 background_shader(const FrameConst &f, V3 viewVector)
+__device__ V3 background_shader(const FrameConst &frame, V3 viewVector)
 {
+	const FrameConst &f = uniform_ref(frame);
 	const V3 vn = normalize(viewVector);
 	float c[3];
 	if (f.background3Colors)
@@ -400,9 +404,10 @@ __device__ V3 background_shader(const FrameConst &f, V3 viewVector)
 // SurfaceColour of the fractal (shader_surface_color.cpp:39-108): the gradient colour at the
 // palette position of the fractal-local point's colour index, or the material colour
 template <int KIND>
-__device__ V3 surface_colour(const FrameConst &f, typename PrecOf<KIND>::P p, const Work &w)
+__device__ V3 surface_colour(const FrameConst &frame, typename PrecOf<KIND>::P p, const Work &w)
 {
 #pragma clang fp contract(off)
+	const FrameConst &f = uniform_ref(frame);
 	using Q = PrecOf<KIND>;
 	if (!f.palette || !f.paletteGradient) return v3(f.surfaceColor[0], f.surfaceColor[1], f.surfaceColor[2]);
 	const typename Q::P local = w.nodeRotated ? Q::node_local(f, p) : p;
@@ -503,9 +508,10 @@ __device__ V3 object_shader(const FrameConst &f, typename PrecOf<KIND>::P p, V3 
 // GlobalIlumination (shader_global_illumination.cpp:41-209).  colour0 is the primary hit's
 // surface colour; each bounce is tinted by the colour of the surface it left.
 template <int KIND>
-__device__ V3 global_illumination(const FrameConst &f, typename PrecOf<KIND>::P p0, V3 n0,
+__device__ V3 global_illumination(const FrameConst &frame, typename PrecOf<KIND>::P p0, V3 n0,
 	typename PrecOf<KIND>::R thr0, V3 colour0, uint32_t &seed, Work &w)
 {
+	const FrameConst &f = uniform_ref(frame);
 	using Q = PrecOf<KIND>;
 	using R = typename Q::R;
 	using P = typename Q::P;
@@ -607,9 +613,10 @@ struct PixelOut
 // One pixel: the sample loop of doWork and RayRecursion at depth 0.  FULL = false is the
 // variant for frames without Monte Carlo sampling, GI, shadows and AO.
 template <int KIND, bool FULL>
-__device__ void render_pixel(const FrameConst &f, int xs, int ys, PixelOut &po, Work &w)
+__device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &po, Work &w)
 {
 #pragma clang fp contract(off)
+	const FrameConst &f = uniform_ref(frame);
 	using Q = PrecOf<KIND>;
 	using R = typename Q::R;
 	using P = typename Q::P;
@@ -632,7 +639,12 @@ __device__ void render_pixel(const FrameConst &f, int xs, int ys, PixelOut &po, 
 		}
 		const P dir = Q::view(f, ix, iy);
 		MarchOut<KIND> m;
-		ray_marching<KIND>(f, cam, dir, (R)f.viewDistanceMax, seed, m, w);
+		if constexpr (PosPrecision<KIND>::kFP64)
+			ray_marching<KIND, 2>(f, cam, dir, (R)f.viewDistanceMax, seed, m, w);
+		else if (f.advancedQuality)
+			ray_marching<KIND, 1>(f, cam, dir, (R)f.viewDistanceMax, seed, m, w);
+		else
+			ray_marching<KIND, 0>(f, cam, dir, (R)f.viewDistanceMax, seed, m, w);
 		V3 rgb;
 		V3 n = v3(0.0f, 0.0f, 0.0f);
 		if (m.found)

@@ -7,16 +7,24 @@
 namespace mbx
 {
 
-// Occupancy targets (min waves per SIMD in __launch_bounds__): the FP64 kinds 2 (256 VGPRs); the
-// general formula kind 3 (168 VGPRs), which its out-of-line DE fits without spilling and which
-// measured 1.32x faster than 2 on the hybrid002 scene although the full variant then spills
-// ~200 B per lane (4 waves gained 2.5% more at 50x the spill traffic).  The distance query
-// shares the out-of-line DE, so it carries the same target.
+// Occupancy targets (min waves per SIMD in __launch_bounds__).  The general formula kind and the
+// two Mandelbox kinds take 3 (168 VGPRs): the general kind's out-of-line DE fits that without
+// spilling, and 3 measured 1.32x faster than 2 on the hybrid002 scene although its full variant
+// then spills ~200 B per lane (4 waves gained 2.5% more at 50x the spill traffic); the Mandelbox
+// delta-DE full variant (shadow + AO scene) went from 256 VGPRs and 256 B of spills at 2 waves to
+// 168 VGPRs, 1.22x faster.  The IFS and FP64 Mandelbulb kinds take 2 (256 VGPRs), the FP32
+// Mandelbulb kinds 1 (no cap needed).  The distance query shares the general kind's out-of-line
+// DE, so it carries the same target for that kind.
+template <int KIND>
+constexpr int occupancy_target()
+{
+	return (KIND == kGeneric || KIND == kBoxDelta || KIND == kBoxAnalytic) ? 3 : (KIND >= kBoxAnalytic ? 2 : 1);
+}
 
 // Plain per-frame kernel: 64 lanes = one tile of (1 << tileShift) x (64 >> tileShift) pixels
 // (8x8 by default), one workgroup per tile, one launch per frame.
 template <int KIND, bool FULL>
-__global__ __launch_bounds__(64, KIND == kGeneric ? 3 : KIND >= kBoxAnalytic ? 2 : 1) void render_kernel(const FrameConst *__restrict__ frame,
+__global__ __launch_bounds__(64, occupancy_target<KIND>()) void render_kernel(const FrameConst *__restrict__ frame,
 	Outputs out, unsigned long long *__restrict__ counters, int tileShift)
 {
 	const FrameConst &f = *frame;
@@ -52,7 +60,7 @@ static_assert(kCounterSlots == 64, "workgroup 0 of batch_kernel clears one count
 // When clearNext is set, workgroup 0 zeroes those counter slots (another bank than `counters`):
 // the next launch accumulates there without a clear kernel in front of it.
 template <int KIND, bool FULL>
-__global__ __launch_bounds__(64, KIND == kGeneric ? 3 : KIND >= kBoxAnalytic ? 2 : 1) void batch_kernel(const FrameConst *__restrict__ frames,
+__global__ __launch_bounds__(64, occupancy_target<KIND>()) void batch_kernel(const FrameConst *__restrict__ frames,
 	const FrameJob *__restrict__ jobs, int numFrames, unsigned long long *__restrict__ counters, int tileShift,
 	unsigned long long *__restrict__ clearNext)
 {
@@ -106,7 +114,7 @@ __global__ __launch_bounds__(64, KIND == kGeneric ? 3 : KIND >= kBoxAnalytic ? 2
 // (frames and jobs are separate __restrict__ kernel arguments so that the compiler may read
 // them with scalar loads)
 template <int KIND, bool FULL>
-__global__ __launch_bounds__(64, KIND == kGeneric ? 3 : KIND >= kBoxAnalytic ? 2 : 1) void persistent_kernel(const FrameConst *__restrict__ frames,
+__global__ __launch_bounds__(64, occupancy_target<KIND>()) void persistent_kernel(const FrameConst *__restrict__ frames,
 	const FrameJob *__restrict__ jobs, int numFrames, unsigned int *__restrict__ queue,
 	unsigned long long *__restrict__ counters, int tileShift)
 {
