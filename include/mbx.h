@@ -567,6 +567,17 @@ int mbx_load_fract(const char *text, size_t len, mbx_params *params, mbx_fract_i
 int mbx_compile_image(mbx_ctx *ctx, const mbx_image_adjustments *adj, const float *rgb, int32_t width,
 	int32_t height, uint16_t *rgb16, uint8_t *rgb8);
 
+/* The z-buffer image of ImageFileSavePNG::SavePNG (mandelbulber2/src/file_image.cpp:628-742):
+ * `depth` (device, 1 float per pixel, as mbx_outputs.depth) to 8- or 16-bit grey in `out` (device,
+ * uint8_t or uint16_t per pixel).  The range [minZ, maxZ] is the frame's (its smallest depth and
+ * its largest depth below 1e19) or, with constant_range, [min_depth, max_depth] as floats; then
+ * z1 = logf(z / minZ) / logf(maxZ / minZ) (logarithmic) or (z - minZ) / (maxZ - minZ), clamped to
+ * [0, 1], int(z1 * 254) or int(z1 * 65534); a miss (z > 1e19) is 255 or 65535; invert subtracts
+ * from 255 or 65535.  The reference's defaults (zbuffer_logarithmic true, zbuffer_invert false,
+ * zbuffer_constant_range false, initparameters.cpp:618-622).  Asynchronous on the context stream. */
+int mbx_compile_zbuffer(mbx_ctx *ctx, const float *depth, int32_t width, int32_t height, int32_t bits,
+	int32_t logarithmic, int32_t invert, int32_t constant_range, double min_depth, double max_depth, void *out);
+
 /* ImageFileSavePNG::SavePNG (mandelbulber2/src/file_image.cpp:481-600): write `pixels` (host
  * memory, row-major, `channels` samples per pixel: 3 = RGB as the colour image, 1 = grey as the
  * z-buffer image; `bits` 8 = uint8_t samples, 16 = uint16_t samples, stored big-endian in the file

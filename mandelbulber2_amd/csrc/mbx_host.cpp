@@ -73,6 +73,7 @@ struct mbx_ctx
 	float gammaOf = 0.0f;
 	bool gammaReady = false;
 	std::vector<uint16_t> gammaHost;
+	uint32_t *d_zrange = nullptr; // z-buffer image: the frame's depth range
 };
 
 namespace
@@ -1441,6 +1442,7 @@ void mbx_ctx_destroy(mbx_ctx *ctx)
 	if (ctx->d_diff) (void)hipFree(ctx->d_diff);
 	if (ctx->d_grad) (void)hipFree(ctx->d_grad);
 	if (ctx->d_gamma) (void)hipFree(ctx->d_gamma);
+	if (ctx->d_zrange) (void)hipFree(ctx->d_zrange);
 	delete ctx;
 }
 
@@ -1793,6 +1795,23 @@ int mbx_compile_image(mbx_ctx *ctx, const mbx_image_adjustments *adj, const floa
 			 adj->brightness, adj->contrast, adj->saturation, adj->hdr ? 1 : 0, rgb16, rgb8, ctx->stream)) !=
 		hipSuccess)
 		return hip_fail(ctx, e, "compile image launch");
+	return MBX_OK;
+}
+
+int mbx_compile_zbuffer(mbx_ctx *ctx, const float *depth, int32_t width, int32_t height, int32_t bits,
+	int32_t logarithmic, int32_t invert, int32_t constant_range, double min_depth, double max_depth, void *out)
+{
+	if (!ctx || !depth || !out || width <= 0 || height <= 0 || (bits != 8 && bits != 16))
+		return fail(ctx, MBX_ERR_INVALID, "bad argument");
+	if ((uint64_t)width * (uint64_t)height > 0xffffffffull) return fail(ctx, MBX_ERR_INVALID, "image too large");
+	hipError_t e = hipSetDevice(ctx->device);
+	if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+	if (!ctx->d_zrange && (e = hipMalloc((void **)&ctx->d_zrange, 2 * sizeof(uint32_t))) != hipSuccess)
+		return hip_fail(ctx, e, "allocate z range");
+	if ((e = mbx::launch_zbuffer(depth, (uint32_t)((uint64_t)width * (uint64_t)height), ctx->d_zrange,
+			 constant_range ? 0 : 1, (float)min_depth, (float)max_depth, bits, logarithmic ? 1 : 0, invert ? 1 : 0, out,
+			 ctx->stream)) != hipSuccess)
+		return hip_fail(ctx, e, "z-buffer launch");
 	return MBX_OK;
 }
 

@@ -105,3 +105,94 @@ hipError_t launch_compile_image(const float *rgb, uint32_t n, const uint16_t *ga
 }
 
 } // namespace mbx
+
+namespace mbx
+{
+
+// z-buffer range (file_image.cpp:631-651): the smallest depth and the largest depth below 1e19
+// (misses are 1e20), as order-preserving bit patterns of the non-negative depths.  range[0] starts
+// at +inf (the reference's float(1e50)), range[1] at 0.
+__global__ __launch_bounds__(64) void zrange_init_kernel(uint32_t *range)
+{
+	if (threadIdx.x == 0)
+	{
+		range[0] = 0x7f800000u;
+		range[1] = 0u;
+	}
+}
+
+__global__ __launch_bounds__(256) void zrange_kernel(const float *__restrict__ depth, uint32_t n, uint32_t *range)
+{
+	__shared__ uint32_t sMin[256], sMax[256];
+	uint32_t lo = 0x7f800000u, hi = 0u;
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+	{
+		const float z = depth[i];
+		const uint32_t b = __float_as_uint(z);
+		if (z < __uint_as_float(lo)) lo = b;
+		if (z > __uint_as_float(hi) && z < 1e19f) hi = b;
+	}
+	sMin[threadIdx.x] = lo;
+	sMax[threadIdx.x] = hi;
+	__syncthreads();
+	for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1)
+	{
+		if (threadIdx.x < s)
+		{
+			sMin[threadIdx.x] = min(sMin[threadIdx.x], sMin[threadIdx.x + s]);
+			sMax[threadIdx.x] = max(sMax[threadIdx.x], sMax[threadIdx.x + s]);
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+	{
+		atomicMin(&range[0], sMin[0]);
+		atomicMax(&range[1], sMax[0]);
+	}
+}
+
+// the grey value of each pixel (file_image.cpp:652-742)
+__global__ __launch_bounds__(256) void zbuffer_kernel(const float *__restrict__ depth, uint32_t n,
+	const uint32_t *__restrict__ range, float minDepth, float maxDepth, int useRange, int bits, int logarithmic,
+	int invert, void *__restrict__ out)
+{
+#pragma clang fp contract(off)
+	const float minZ = useRange ? __uint_as_float(range[0]) : minDepth;
+	const float maxZ = useRange ? __uint_as_float(range[1]) : maxDepth;
+	// logf as the host's: the FP64 log rounded to float (glibc's logf is correctly rounded in
+	// practice; the device's differs in the last bit now and then)
+	const float kZ = (float)log((double)(maxZ / minZ));
+	const int top = bits == 16 ? 65534 : 254;
+	const int miss = bits == 16 ? 65535 : 255;
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+	{
+		const float z = depth[i];
+		float z1 = logarithmic ? (float)log((double)(z / minZ)) / kZ : (z - minZ) / (maxZ - minZ);
+		if (z1 < 0) z1 = 0.0f;
+		if (z1 > 1.0f) z1 = 1.0f;
+		int intZ = (int)(z1 * (float)top);
+		if (z > 1e19f) intZ = miss;
+		if (invert) intZ = miss - intZ;
+		if (bits == 16)
+			((uint16_t *)out)[i] = (uint16_t)intZ;
+		else
+			((uint8_t *)out)[i] = (uint8_t)intZ;
+	}
+}
+
+hipError_t launch_zbuffer(const float *depth, uint32_t n, uint32_t *range, int useRange, float minDepth,
+	float maxDepth, int bits, int logarithmic, int invert, void *out, hipStream_t stream)
+{
+	const uint32_t blocks = (n + 255u) / 256u;
+	const uint32_t grid = blocks < 4096u ? (blocks ? blocks : 1u) : 4096u;
+	if (useRange)
+	{
+		hipLaunchKernelGGL(zrange_init_kernel, dim3(1), dim3(64), 0, stream, range);
+		hipLaunchKernelGGL(zrange_kernel, dim3(grid), dim3(256), 0, stream, depth, n, range);
+	}
+	hipLaunchKernelGGL(zbuffer_kernel, dim3(grid), dim3(256), 0, stream, depth, n, (const uint32_t *)range, minDepth,
+		maxDepth, useRange, bits, logarithmic, invert, out);
+	return hipGetLastError();
+}
+
+} // namespace mbx

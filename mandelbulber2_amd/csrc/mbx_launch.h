@@ -34,6 +34,11 @@ hipError_t launch_distance(int deKind, const FrameConst *frame, const float *pts
 hipError_t launch_compile_image(const float *rgb, uint32_t n, const uint16_t *gammaTable, float brightness,
 	float contrast, float saturation, int hdr, uint16_t *out16, uint8_t *out8, hipStream_t stream);
 
+// z-buffer grey image of n depths; `range` (2 uint32, device) receives the frame's depth range
+// when useRange is set.
+hipError_t launch_zbuffer(const float *depth, uint32_t n, uint32_t *range, int useRange, float minDepth,
+	float maxDepth, int bits, int logarithmic, int invert, void *out, hipStream_t stream);
+
 } // namespace mbx
 
 namespace mbx

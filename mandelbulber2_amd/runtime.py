@@ -17,7 +17,7 @@ EXPORTS = ["mbx_version", "mbx_sizeof_params", "mbx_ctx_create", "mbx_ctx_destro
            "mbx_last_error", "mbx_scene_upload", "mbx_scene_info", "mbx_render", "mbx_distance",
            "mbx_get_stats", "mbx_render_batch", "mbx_tune", "mbx_get_tune", "mbx_autotune",
            "mbx_render_diff", "mbx_adjoint", "mbx_params_default", "mbx_load_fract",
-           "mbx_compile_image", "mbx_write_png"]
+           "mbx_compile_image", "mbx_write_png", "mbx_compile_zbuffer"]
 
 _lib = None
 
@@ -59,6 +59,8 @@ def load():
     L.mbx_load_fract.argtypes = [C.c_char_p, C.c_size_t, P(abi.MbxParams), P(abi.MbxFractInfo)]
     L.mbx_compile_image.argtypes = [C.c_void_p, P(abi.MbxImageAdjustments), C.c_void_p, C.c_int32,
                                     C.c_int32, C.c_void_p, C.c_void_p]
+    L.mbx_compile_zbuffer.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                      C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_void_p]
     L.mbx_write_png.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
     _lib = L
     return L
@@ -211,6 +213,19 @@ class Context:
                                              C.c_void_p(rgb16.data_ptr()), C.c_void_p(rgb8.data_ptr())),
                     "mbx_compile_image")
         return rgb16, rgb8
+
+    def compile_zbuffer(self, depth, bits=16, logarithmic=True, invert=False, constant_range=False,
+                        min_depth=1e-4, max_depth=10.0):
+        """The z-buffer grey image of an (H, W) float32 depth plane: (H, W) uint16 or uint8."""
+        torch = self.torch
+        if depth.dtype != torch.float32 or not depth.is_contiguous() or depth.dim() != 2:
+            raise MbxError("compile_zbuffer: contiguous (H, W) float32 depth")
+        out = torch.empty(depth.shape, dtype=torch.uint16 if bits == 16 else torch.uint8, device=depth.device)
+        self._check(self.L.mbx_compile_zbuffer(self.h, C.c_void_p(depth.data_ptr()), depth.shape[1], depth.shape[0],
+                                               bits, int(logarithmic), int(invert), int(constant_range),
+                                               min_depth, max_depth, C.c_void_p(out.data_ptr())),
+                    "mbx_compile_zbuffer")
+        return out
 
     def tune(self, key, value):
         self._check(self.L.mbx_tune(self.h, key, value), "mbx_tune")

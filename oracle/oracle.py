@@ -52,6 +52,8 @@ def lib(fast=False):
     L.mbxo_colour_index.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_int, C.c_void_p]
     L.mbxo_gradient.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_int, C.c_void_p]
     L.mbxo_surface_colour.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_int, C.c_void_p]
+    L.mbxo_compile_zbuffer.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                       C.c_double, C.c_void_p]
     L.mbxo_compile_image.argtypes = [P(abi.MbxImageAdjustments), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
     L.mbxo_random_int.argtypes = [P(C.c_uint32)]
     L.mbxo_random.argtypes = [P(C.c_uint32), C.c_int]
@@ -182,6 +184,18 @@ def compile_image(adj, rgb):
     if rc != 0:
         raise RuntimeError("oracle compile_image failed: %d" % rc)
     return out16, out8
+
+
+def compile_zbuffer(depth, bits=16, logarithmic=True, invert=False, constant_range=False, min_depth=1e-4,
+                    max_depth=10.0):
+    """The z-buffer grey image of a float32 depth array (same shape, uint16 or uint8)."""
+    depth = np.ascontiguousarray(depth, np.float32)
+    out = np.zeros(depth.shape, np.uint16 if bits == 16 else np.uint8)
+    rc = lib().mbxo_compile_zbuffer(_ptr(depth), depth.size, bits, int(logarithmic), int(invert),
+                                    int(constant_range), min_depth, max_depth, _ptr(out))
+    if rc != 0:
+        raise RuntimeError("oracle compile_zbuffer failed: %d" % rc)
+    return out
 
 
 def frame_constants(p):

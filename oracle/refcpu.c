@@ -2734,3 +2734,50 @@ int mbxo_compile_image(const mbx_image_adjustments *adj, const float *rgb, int n
 	}
 	return MBX_OK;
 }
+
+/* The z-buffer image of ImageFileSavePNG::SavePNG (file_image.cpp:628-742) for n depths: the
+ * range (smallest depth, largest depth below 1e19; or the constant one), the logarithmic or
+ * linear scale in float, int(z1 * 254 or 65534), misses 255 / 65535, optional inversion.
+ * `out` holds uint8_t (bits 8) or uint16_t (bits 16) samples. */
+int mbxo_compile_zbuffer(const float *depth, int n, int bits, int logarithmic, int invert, int constantRange,
+	double minDepth, double maxDepth, void *out)
+{
+	if (!depth || !out || n < 0 || (bits != 8 && bits != 16)) return MBX_ERR_INVALID;
+	float minZ = (float)1.0e50;
+	float maxZ = 0.0f;
+	if (constantRange)
+	{
+		minZ = (float)minDepth;
+		maxZ = (float)maxDepth;
+	}
+	else
+		for (int i = 0; i < n; i++)
+		{
+			const float z = depth[i];
+			if (z > maxZ && z < 1e19f) maxZ = z;
+			if (z < minZ) minZ = z;
+		}
+	const float kZ = logf(maxZ / minZ);
+	for (int i = 0; i < n; i++)
+	{
+		const float z = depth[i];
+		float z1 = logarithmic ? logf(z / minZ) / kZ : (z - minZ) / (maxZ - minZ);
+		if (z1 < 0) z1 = 0.0f;
+		if (z1 > 1.0f) z1 = 1.0f;
+		if (bits == 16)
+		{
+			int intZ = (int)(z1 * 65534);
+			if (z > 1e19f) intZ = 65535;
+			if (invert) intZ = 65535 - intZ;
+			((uint16_t *)out)[i] = (uint16_t)intZ;
+		}
+		else
+		{
+			int intZ = (int)(z1 * 254);
+			if (z > 1e19) intZ = 255;
+			if (invert) intZ = 255 - intZ;
+			((uint8_t *)out)[i] = (uint8_t)intZ;
+		}
+	}
+	return MBX_OK;
+}

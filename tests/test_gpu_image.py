@@ -115,3 +115,34 @@ def test_compile_image_validation(rt):
     with pytest.raises(rt.MbxError):
         ctx.compile_image(img.to(torch.float64))
     ctx.close()
+
+
+@pytest.mark.parametrize("bits", [16, 8])
+@pytest.mark.parametrize("mode", ["log", "linear", "log_inverted", "constant_range"])
+def test_zbuffer_image_parity(rt, oracle_mod, mode, bits):
+    """mbx_compile_zbuffer on a rendered depth plane (hits and misses) against the oracle: the
+    frame's range by the device reduction, then the grey values.  Linear scale bit-exact; the
+    logarithmic scale calls the device's logf, whose last bit may differ from the host's: >= 99.9%
+    exact and within 1."""
+    import torch
+    p = scene.mandelbulb8(160, 120)
+    ctx = rt.Context(0)
+    ctx.upload(p)
+    bufs = ctx.alloc_outputs()
+    ctx.render(bufs)
+    depth = bufs["depth"].reshape(p.imageHeight, p.imageWidth).contiguous()
+    kw = {"bits": bits, "logarithmic": mode != "linear", "invert": mode == "log_inverted",
+          "constant_range": mode == "constant_range", "min_depth": 1.5, "max_depth": 4.0}
+    print("[case] zbuffer %s %d-bit" % (mode, bits), flush=True)
+    g = ctx.compile_zbuffer(depth, **kw)
+    torch.cuda.synchronize()
+    host = depth.cpu().numpy()
+    want = oracle_mod.compile_zbuffer(host, **kw)
+    got = g.cpu().numpy()
+    ctx.close()
+    assert (host > 1e19).any() and (host < 1e19).any()
+    if mode == "linear":
+        np.testing.assert_array_equal(got, want)
+    else:
+        assert (got == want).mean() >= 0.999
+        assert np.abs(got.astype(int) - want.astype(int)).max() <= 1

@@ -85,3 +85,21 @@ def test_eight_bit_is_sixteen_bit_over_256(oracle_mod):
     out16, out8 = oracle_mod.compile_image(_adj(brightness=1.2, contrast=0.9, gamma=1.8, saturation=1.3), rgb)
     np.testing.assert_array_equal(out8, (out16 // 256).astype(np.uint8))
     assert out16.max() > 60000 and out16.min() == 0
+
+
+def test_zbuffer_image_closed_forms(oracle_mod):
+    """file_image.cpp:628-742: linear scale (z - min) / (max - min) * 65534 on depths where it is
+    exact, the logarithmic midpoint sqrt(min * max) at half scale, misses at full scale, the
+    largest hit at 65534, inversion, the constant range and 8-bit."""
+    d = np.array([1.0, 2.0, 3.0, 1.5, 1e20], np.float32)
+    lin = oracle_mod.compile_zbuffer(d, logarithmic=False)
+    np.testing.assert_array_equal(lin, [0, 32767, 65534, 16383, 65535])
+    inv = oracle_mod.compile_zbuffer(d, logarithmic=False, invert=True)
+    np.testing.assert_array_equal(inv, 65535 - lin)
+    lg = oracle_mod.compile_zbuffer(np.array([0.5, 2.0, 8.0, 1e20], np.float32))
+    assert lg[0] == 0 and lg[2] == 65534 and lg[3] == 65535
+    assert abs(int(lg[1]) - 32767) <= 1
+    const = oracle_mod.compile_zbuffer(d, logarithmic=False, constant_range=True, min_depth=0.0, max_depth=4.0)
+    np.testing.assert_array_equal(const, [16383, 32767, 49150, 24575, 65535])
+    eight = oracle_mod.compile_zbuffer(d, bits=8, logarithmic=False)
+    np.testing.assert_array_equal(eight, [0, 127, 254, 63, 255])
