@@ -9,7 +9,7 @@ launch's device time, HIP events on the launch stream), the plain per-frame kern
 same frame, and the CPU oracle timed on this host.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3]
-                    [--workload mandelbulb8|material|mandelbox|hybrid|animation64|deepzoom|adjoint512]
+                    [--workload mandelbulb8|material|mandelbox|menger|hybrid|animation64|deepzoom|adjoint512]
 
 `--workload material` times the same 1080p Mandelbulb-8 with the reference's default material:
 palette colouring from the default surface gradient, plastic and metallic highlights, glow.
@@ -40,7 +40,10 @@ F_DE = 25
 F_IT = {"mandelbulb8": 48, "material": 48, "mandelbox": 29, "animation64": 32, "deepzoom": 29,
         # the hybrid's cycle: 2 Mandelbox (29) + 3 Menger sponge (20) + 1 Mandelbulb power 2 (30)
         # iterations, each with the 4D c addition and length: (2*29 + 3*20 + 30) / 6
-        "hybrid": 25}
+        "hybrid": 25,
+        # Menger-mode KIFS iteration: scale and offset (10), DE scale (1), |z|^2 and sqrt (6),
+        # the three compare-swaps and abs folds counted as 3
+        "menger": 20}
 # A far-field step (no DE evaluation: the step is provably the clamped 3) does the ray point (6),
 # |p|^2 (5) and the scan update (2).
 F_FAR = 13
@@ -91,6 +94,16 @@ WORKLOADS = {
                    "camera": "seed 0, distance 2.6-3.4", "loss": "L2 against a target at "
                    "theta * (1 +- U(0, 0.05))", "parameters": "power, rotation xyz, camera xyz, "
                    "target xyz", "shading": "Lambert, analytic normals, FP64 Newton-refined hits"},
+    },
+    "menger": {
+        "metric": "Mrays/s DE raymarch of the Menger-mode kaleidoscopic IFS with the default material "
+                  "and a hard shadow, 1080p, 1 GPU; % FP32 VALU peak",
+        "kernel": "batch_kernel<kIfsAnalytic> (full variant)", "dtype": "f32", "peak": FP32_PEAK_TFLOPS,
+        "config": {"formula": "kaleidoscopic IFS, Menger mode, scale 3, offset (1,1,1), abs folds, "
+                              "maxiter 30, analytic IFS DE (as IFS_001.fract)",
+                   "camera": "(2.2, -2.6, 1.7) to the origin, fov 53 deg", "spp": 1,
+                   "shading": "default material (palette, plastic + metallic highlights, glow), "
+                              "hard shadow from the main light"},
     },
     "hybrid": {
         "metric": "Mrays/s DE raymarch of the hybrid002 example scene (Mandelbox x2, Menger sponge "
@@ -148,7 +161,8 @@ def measured_traffic(workload, w, h):
             "mandelbulb8": "batch_kernel_bulb8_mandelbulb8_1080p.md",
             "material": "batch_kernel_bulb8_material_1080p.md",
             "mandelbox": "batch_kernel_boxdelta_mandelbox_1080p.md",
-            "hybrid": "batch_kernel_generic_hybrid002_1080p.md"}.get(workload)
+            "hybrid": "batch_kernel_generic_hybrid002_1080p.md",
+            "menger": "batch_kernel_ifs_menger_1080p.md"}.get(workload)
     if not name or (w, h) != (1920, 1080):
         return None, None
     path = os.path.join(ROOT, "profiles", name)
@@ -170,6 +184,10 @@ def make_scene(name, w, h):
         return scene.animation64(seed=3)
     if name == "deepzoom":
         return scene.deep_zoom_mandelbox(1024, 1024, seed=5)
+    if name == "menger":
+        p = scene.reference_material(scene.menger_ifs(w, h))
+        p.light.castShadows = 1
+        return p
     if name == "hybrid":
         from mandelbulber2_amd import abi, runtime
         with open(os.path.join(ROOT, "tests", "golden", "fract", "hybrid002.fract"), "rb") as f:
