@@ -201,6 +201,8 @@ class Context:
         `adj` defaults to the uploaded scene's image adjustments."""
         torch = self.torch
         if adj is None:
+            if self.params is None:
+                raise MbxError("compile_image: no image adjustments given and no scene uploaded")
             adj = self.params.image
         H, W = rgb.shape[0], rgb.shape[1]
         if rgb.dtype != torch.float32 or not rgb.is_contiguous() or tuple(rgb.shape) != (H, W, 3):
