@@ -37,6 +37,12 @@ SCENES = [
     "hybrid14.fract",
     "hybrid16.fract",
     "hybrid22_foldingIntPow.fract",
+    "hybrid001.fract",
+    # single formulas on the general kernel
+    "aexion01.fract",
+    "xenodreambuie2.fract",
+    "hypercomplex_03.fract",
+    "box_fold_bulb_pow2_001.fract",
 ]
 
 
