@@ -194,6 +194,15 @@ __device__ __forceinline__ double dsgn(double v) { return (double)((0.0 < v) - (
 
 // With COLOR, aux.color gains the fold colour factors in the reference's order: x, y, z box
 // folds, then the inner (Sp1) or outer (Sp2) sphere fold.
+// mandelbox.mainRot.RotateVector (algebra.cpp:320-344) in the reference's operation order
+__device__ __forceinline__ DV3 rot_main_d(const FrameConst &f, DV3 v)
+{
+#pragma clang fp contract(off)
+	const double *m = f.mbMainRotD;
+	return dv3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
+		m[6] * v.x + m[7] * v.y + m[8] * v.z);
+}
+
 template <bool TANGENT, bool COLOR = false>
 __device__ __forceinline__ void MandelboxIterationD(DV3 &z, double &DE, DV3 *J, const FrameConst &f,
 	double *color = nullptr)
@@ -243,10 +252,15 @@ __device__ __forceinline__ void MandelboxIterationD(DV3 &z, double &DE, DV3 *J, 
 		for (int c = 0; c < 3; c++)
 		{
 			const double zd = ddot(z, J[c]) * g;
-			J[c] = (J[c] * k + zo * zd) * f.mbScaleD;
+			J[c] = J[c] * k + zo * zd;
+			if (f.mbMainRotOn) J[c] = rot_main_d(f, J[c]);
+			J[c] = J[c] * f.mbScaleD;
 		}
 	}
-	z = (zo * k - o) * f.mbScaleD;
+	// fractal_mandelbox.cpp:101-105: offset back, main rotation, scale
+	DV3 zr = zo * k - o;
+	if (f.mbMainRotOn) zr = rot_main_d(f, zr);
+	z = zr * f.mbScaleD;
 	DE = DE * k * f.mbAbsScaleD + 1.0;
 }
 

@@ -35,6 +35,8 @@ enum DEKind : int32_t
 struct GenSlot
 {
 	int32_t formula, addC, checkBailout, ifsMainRotId;
+	int32_t mbMainRotOn, mbPad;
+	double mbMainRot[9]; // Mandelbox main rotation (SetRotation2)
 	double weight, bailout;
 	double power, alpha, beta; // Mandelbulb, Xenodreambuie (radians)
 	double mbScale, mbAbsScale, mbFoldingLimit, mbFoldingValue, mbFR2, mbMR2, mbFactor1;
@@ -105,6 +107,9 @@ struct FrameConst
 	// the Mandelbox orbit runs in FP64 (full-rate FMA on gfx950): its constants in FP64
 	double mbScaleD, mbAbsScaleD, mbFoldingLimitD, mbFoldingValueD, mbFR2D, mbMR2D, mbFactor1D;
 	double mbOffsetD[3];
+	double mbMainRotD[9];  // Mandelbox main rotation (SetRotation2 of mandelbox_rotation_main)
+	int32_t mbMainRotOn;   // mandelbox_main_rotation_enabled
+	int32_t mbPad;
 	double juliaCD[3], constMulD[3];
 	double bailoutD;
 	double powerD, alphaD, betaD; // Mandelbulb in FP64 (radians)

@@ -169,6 +169,7 @@ __device__ __forceinline__ void gen_mandelbox(const GenSlot &s, DV4 &z, GenAux &
 		a.color += s.mbSp2;
 	}
 	z = dv4(z.x - s.mbOffset[0], z.y - s.mbOffset[1], z.z - s.mbOffset[2], z.w - 0.0);
+	if (s.mbMainRotOn) z = rot4(s.mbMainRot, 0, z);
 	z = dv4(z.x * s.mbScale, z.y * s.mbScale, z.z * s.mbScale, z.w * s.mbScale);
 	a.DE = a.DE * s.mbAbsScale + 1.0;
 }

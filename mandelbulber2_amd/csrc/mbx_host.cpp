@@ -449,6 +449,19 @@ const FormulaMeta *formula_meta(int id)
 	return nullptr;
 }
 
+// SetRotation2 of angles in degrees (algebra.cpp:297-303: Z(z), Y(y), X(x)), as
+// RecalculateFractalParams sets mandelbox.mainRot (fractal.cpp:663); returns false for the
+// identity (all angles zero), which RotateVector skips
+bool rot2_deg(const double a[3], double out[9])
+{
+	Rot r;
+	r.z(a[2] * kPi180);
+	r.y(a[1] * kPi180);
+	r.x(a[0] * kPi180);
+	std::memcpy(out, r.m, sizeof(r.m));
+	return !(a[0] == 0.0 && a[1] == 0.0 && a[2] == 0.0);
+}
+
 // SetRotation3 of angles in degrees (algebra.cpp:305-311); identity when all are zero
 void rot3_deg(const double a[3], double out[9], int32_t &identity)
 {
@@ -488,6 +501,7 @@ void gen_slot(const mbx_fractal &f, mbx::GenSlot &s)
 	}
 	s.mbSp1 = mb.colorFactorSp1;
 	s.mbSp2 = mb.colorFactorSp2;
+	s.mbMainRotOn = (rot2_deg(mb.rotationMain, s.mbMainRot) && mb.mainRotationEnabled) ? 1 : 0;
 	const mbx_fractal_ifs &ifs = f.IFS;
 	s.ifsScale = ifs.scale;
 	rot3_deg(ifs.rotation, s.ifsMainRot, s.ifsMainRotId);
@@ -545,8 +559,7 @@ int derive_generic(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 		if (c[k]->formula == MBX_FORMULA_NONE) continue;
 		const FormulaMeta *m = formula_meta(c[k]->formula);
 		if (!m) return fail(ctx, MBX_ERR_UNSUPPORTED, "formula not supported");
-		if (c[k]->formula == MBX_FORMULA_MANDELBOX &&
-			(c[k]->mandelbox.rotationsEnabled || c[k]->mandelbox.mainRotationEnabled))
+		if (c[k]->formula == MBX_FORMULA_MANDELBOX && c[k]->mandelbox.rotationsEnabled)
 			return fail(ctx, MBX_ERR_UNSUPPORTED, "Mandelbox rotated folds are not supported on the device");
 		if (c[k]->formulaIterations < 1) return fail(ctx, MBX_ERR_INVALID, "formula_iterations < 1");
 		any = true;
@@ -829,6 +842,7 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 	set3f(fc.mbOffset, mb.offset);
 	fc.deltaTangent = ctx->deltaTangent;
 	fc.mbScaleD = mb.scale;
+	fc.mbMainRotOn = (rot2_deg(mb.rotationMain, fc.mbMainRotD) && mb.mainRotationEnabled) ? 1 : 0;
 	fc.mbAbsScaleD = std::fabs(mb.scale);
 	fc.mbFoldingLimitD = mb.foldingLimit;
 	fc.mbFoldingValueD = mb.foldingValue;
@@ -910,7 +924,7 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 	}
 	else if (fr.formula == MBX_FORMULA_MANDELBOX)
 	{
-		if (mb.rotationsEnabled || mb.mainRotationEnabled)
+		if (mb.rotationsEnabled)
 			return fail(ctx, MBX_ERR_UNSUPPORTED, "Mandelbox rotated folds are not supported on the device");
 		kind = deType == MBX_DE_DELTA ? mbx::kBoxDelta : mbx::kBoxAnalytic;
 	}

@@ -43,6 +43,9 @@ SCENES = [
     "xenodreambuie2.fract",
     "hypercomplex_03.fract",
     "box_fold_bulb_pow2_001.fract",
+    # Mandelbox main rotation: the specialised FP64 kernel and a hybrid
+    "mandelbox_rot_001.fract",
+    "FoldIntPow2_3.fract",
 ]
 
 

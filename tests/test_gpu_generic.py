@@ -179,6 +179,33 @@ def test_generic_settings_are_validated(rt):
         rt.render_to_host(bad)
 
 
+@pytest.mark.parametrize("kind", ["box_delta", "box_analytic", "hybrid"])
+def test_mandelbox_main_rotation_parity(rt, oracle_mod, kind):
+    """mandelbox_main_rotation_enabled (fractal_mandelbox.cpp:103: SetRotation2 of
+    mandelbox_rotation_main between the offset and the scale) on the specialised FP64 Mandelbox
+    kernels and in a hybrid on the general kernel."""
+    if kind == "hybrid":
+        p = _hybrid([(abi.MBX_FORMULA_MANDELBOX, 2, 1.0, None), (abi.MBX_FORMULA_MENGER_SPONGE, 1, 1.0, None)],
+                    camera=(3.0, -6.0, 2.0))
+        mb = p.fractal.mandelbox
+    else:
+        p = scene.mandelbox_shadow_ao(96, 54, seed=1)
+        p.fractal.deltaDEMethod = 1 if kind == "box_delta" else 2
+        mb = p.fractal.mandelbox
+    mb.mainRotationEnabled = 1
+    abi.set3(mb.rotationMain, (9.98, 3.995, 4.98))
+    print("[case] mandelbox main rotation %s" % kind, flush=True)
+    g = rt.render_to_host(p)
+    r = oracle_mod.render(p)
+    assert r["mask"].mean() > 0.05
+    st = compare(g, r)
+    _report("mandelbox main rotation %s" % kind, st)
+    _check(st, False)
+    plain = scene.copy_params(p)
+    plain.fractal.mandelbox.mainRotationEnabled = 0
+    assert (oracle_mod.render(plain)["iters"] != r["iters"]).mean() > 0.05, "the rotation changes the fractal"
+
+
 def _advanced(p, variant):
     p.advancedQuality = 1
     if variant == "fixed_step":      # every step 0.02, horizon 133 steps away
