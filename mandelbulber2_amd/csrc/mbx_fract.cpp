@@ -1205,8 +1205,6 @@ extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, 
 		bool known = false;
 		for (int id : kTable) known |= fr.formula == id;
 		if (fr.formula != MBX_FORMULA_NONE && !known) flags |= MBX_FRACT_FORMULA;
-		if (fr.formula == MBX_FORMULA_MANDELBOX && mb.rotationsEnabled)
-			flags |= MBX_FRACT_FORMULA;
 		if (fr.formula == MBX_FORMULA_MENGER_SPONGE && p.useColorsFromPalette &&
 			fv.b("fold_color_aux_color_enabled_false", false))
 			flags |= MBX_FRACT_PALETTE;

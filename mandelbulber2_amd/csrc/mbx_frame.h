@@ -35,8 +35,11 @@ enum DEKind : int32_t
 struct GenSlot
 {
 	int32_t formula, addC, checkBailout, ifsMainRotId;
-	int32_t mbMainRotOn, mbPad;
-	double mbMainRot[9]; // Mandelbox main rotation (SetRotation2)
+	int32_t mbMainRotOn, mbRotOn; // Mandelbox main rotation, rotated folds
+	int32_t mbRotId[2][3];        // rotated folds: identity (skipped) matrices
+	double mbMainRot[9];          // Mandelbox main rotation (SetRotation2)
+	double mbRot[2][3][9];        // rotated folds: mandelbox.rot[fold][axis] (SetRotation2)
+	double mbRotInv[2][3][9];     // and their transposes (rotinv)
 	double weight, bailout;
 	double power, alpha, beta; // Mandelbulb, Xenodreambuie (radians)
 	double mbScale, mbAbsScale, mbFoldingLimit, mbFoldingValue, mbFR2, mbMR2, mbFactor1;

@@ -132,12 +132,46 @@ __device__ __forceinline__ void gen_menger_sponge(const GenSlot &s, DV4 &z, GenA
 	a.DE *= s.scale3;
 }
 
-// cFractalMandelbox::FormulaCode (fractal_mandelbox.cpp:31-107) without rotated folds, with the
-// fold colour of aux.color
+// cFractalMandelbox::FormulaCode (fractal_mandelbox.cpp:31-107) with the fold colour of aux.color
 __device__ __forceinline__ void gen_mandelbox(const GenSlot &s, DV4 &z, GenAux &a)
 {
 #pragma clang fp contract(off)
 	const double L = s.mbFoldingLimit, V = s.mbFoldingValue;
+	if (s.mbRotOn)
+	{
+		// rotated folds (fractal_mandelbox.cpp:33-63): for each axis, the fold of the axis rotated
+		// by rot[0] (positive side) or else rot[1] (negative side), rotated back by its transpose
+		for (int dim = 0; dim < 3; dim++)
+		{
+			DV4 zRot = rot4(s.mbRot[0][dim], s.mbRotId[0][dim], z);
+			double zr = dim == 0 ? zRot.x : (dim == 1 ? zRot.y : zRot.z);
+			if (zr > L)
+			{
+				zr = V - zr;
+				if (dim == 0) zRot.x = zr;
+				else if (dim == 1) zRot.y = zr;
+				else zRot.z = zr;
+				z = rot4(s.mbRotInv[0][dim], 0, zRot);
+				a.color += s.mbColor[dim];
+			}
+			else
+			{
+				zRot = rot4(s.mbRot[1][dim], s.mbRotId[1][dim], z);
+				zr = dim == 0 ? zRot.x : (dim == 1 ? zRot.y : zRot.z);
+				if (zr < -L)
+				{
+					zr = -V - zr;
+					if (dim == 0) zRot.x = zr;
+					else if (dim == 1) zRot.y = zr;
+					else zRot.z = zr;
+					z = rot4(s.mbRotInv[1][dim], 0, zRot);
+					a.color += s.mbColor[dim];
+				}
+			}
+		}
+	}
+	else
+	{
 	if (fabs(z.x) > L)
 	{
 		z.x = dsgn(z.x) * V - z.x;
@@ -152,6 +186,7 @@ __device__ __forceinline__ void gen_mandelbox(const GenSlot &s, DV4 &z, GenAux &
 	{
 		z.z = dsgn(z.z) * V - z.z;
 		a.color += s.mbColor[2];
+	}
 	}
 	const double r2 = dot4(z, z);
 	z = dv4(z.x + s.mbOffset[0], z.y + s.mbOffset[1], z.z + s.mbOffset[2], z.w + 0.0);

@@ -502,6 +502,17 @@ void gen_slot(const mbx_fractal &f, mbx::GenSlot &s)
 	s.mbSp1 = mb.colorFactorSp1;
 	s.mbSp2 = mb.colorFactorSp2;
 	s.mbMainRotOn = (rot2_deg(mb.rotationMain, s.mbMainRot) && mb.mainRotationEnabled) ? 1 : 0;
+	// rotated folds: rot[fold][axis] = SetRotation2, rotinv = its transpose (fractal.cpp:665-672)
+	s.mbRotOn = mb.rotationsEnabled ? 1 : 0;
+	for (int fold = 0; fold < MBX_MANDELBOX_FOLDS; fold++)
+		for (int axis = 0; axis < 3; axis++)
+		{
+			double *m = s.mbRot[fold][axis];
+			s.mbRotId[fold][axis] = rot2_deg(mb.rotation[fold][axis], m) ? 0 : 1;
+			double *t = s.mbRotInv[fold][axis];
+			for (int r = 0; r < 3; r++)
+				for (int c = 0; c < 3; c++) t[3 * r + c] = m[3 * c + r];
+		}
 	const mbx_fractal_ifs &ifs = f.IFS;
 	s.ifsScale = ifs.scale;
 	rot3_deg(ifs.rotation, s.ifsMainRot, s.ifsMainRotId);
@@ -559,8 +570,6 @@ int derive_generic(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 		if (c[k]->formula == MBX_FORMULA_NONE) continue;
 		const FormulaMeta *m = formula_meta(c[k]->formula);
 		if (!m) return fail(ctx, MBX_ERR_UNSUPPORTED, "formula not supported");
-		if (c[k]->formula == MBX_FORMULA_MANDELBOX && c[k]->mandelbox.rotationsEnabled)
-			return fail(ctx, MBX_ERR_UNSUPPORTED, "Mandelbox rotated folds are not supported on the device");
 		if (c[k]->formulaIterations < 1) return fail(ctx, MBX_ERR_INVALID, "formula_iterations < 1");
 		any = true;
 		if (!hybrid) s.checkBailout = 1;
@@ -705,7 +714,9 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 		return fail(ctx, MBX_ERR_INVALID, "hybridCount out of range");
 	// the specialised kinds cover one Mandelbulb, Mandelbox or IFS with w = 0; everything else
 	// takes the general FP64 path
+	// the general kernel also takes the Mandelbox with rotated folds
 	const bool generic = p->hybridCount >= 1 || fr.initialWAxis != 0.0 ||
+		(fr.formula == MBX_FORMULA_MANDELBOX && fr.mandelbox.rotationsEnabled) ||
 		(fr.formula != MBX_FORMULA_MANDELBULB && fr.formula != MBX_FORMULA_MANDELBOX &&
 			fr.formula != MBX_FORMULA_KALEIDOSCOPIC_IFS);
 

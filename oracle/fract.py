@@ -960,8 +960,6 @@ def load(text):
         out[prefix + "transform.analyticDEEnabledFalse"] = int(f.get("analyticDE_enabled_false"))
         if fo_ != 0 and fo_ not in TABLE:
             flags |= FLAGS["formula"]
-        if fo_ == 8 and f.get("mandelbox_rotations_enabled"):
-            flags |= FLAGS["formula"]
         if fo_ == 7 and g(mp + "use_colors_from_palette") and f.get("fold_color_aux_color_enabled_false"):
             flags |= FLAGS["palette"]
         return fo_

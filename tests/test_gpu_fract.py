@@ -46,6 +46,8 @@ SCENES = [
     # Mandelbox main rotation: the specialised FP64 kernel and a hybrid
     "mandelbox_rot_001.fract",
     "FoldIntPow2_3.fract",
+    # rotated Mandelbox folds (general kernel)
+    "mandelbox62.fract",
 ]
 
 

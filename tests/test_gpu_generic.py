@@ -179,7 +179,7 @@ def test_generic_settings_are_validated(rt):
         rt.render_to_host(bad)
 
 
-@pytest.mark.parametrize("kind", ["box_delta", "box_analytic", "hybrid"])
+@pytest.mark.parametrize("kind", ["box_delta", "box_analytic", "hybrid", "rotated_folds"])
 def test_mandelbox_main_rotation_parity(rt, oracle_mod, kind):
     """mandelbox_main_rotation_enabled (fractal_mandelbox.cpp:103: SetRotation2 of
     mandelbox_rotation_main between the offset and the scale) on the specialised FP64 Mandelbox
@@ -194,6 +194,12 @@ def test_mandelbox_main_rotation_parity(rt, oracle_mod, kind):
         mb = p.fractal.mandelbox
     mb.mainRotationEnabled = 1
     abi.set3(mb.rotationMain, (9.98, 3.995, 4.98))
+    if kind == "rotated_folds":
+        mb.rotationsEnabled = 1
+        for fold, axis, angles in ((0, 0, (5.0, -7.0, 3.0)), (1, 0, (-4.0, 2.0, 6.0)), (0, 2, (3.0, 8.0, -2.0)),
+                                   (1, 1, (0.0, 0.0, 0.0))):
+            for k in range(3):
+                mb.rotation[fold][axis][k] = angles[k]
     print("[case] mandelbox main rotation %s" % kind, flush=True)
     g = rt.render_to_host(p)
     r = oracle_mod.render(p)
@@ -201,8 +207,11 @@ def test_mandelbox_main_rotation_parity(rt, oracle_mod, kind):
     st = compare(g, r)
     _report("mandelbox main rotation %s" % kind, st)
     _check(st, False)
+    if kind == "rotated_folds":
+        assert g["de_kind"] == GENERIC
     plain = scene.copy_params(p)
     plain.fractal.mandelbox.mainRotationEnabled = 0
+    plain.fractal.mandelbox.rotationsEnabled = 0
     assert (oracle_mod.render(plain)["iters"] != r["iters"]).mean() > 0.05, "the rotation changes the fractal"
 
 
