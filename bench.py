@@ -199,11 +199,45 @@ def make_scene(name, w, h):
     return scene.mandelbox_shadow_ao(w, h, seed=1, scale=2.0)
 
 
+def host_cpu():
+    """Threads for the CPU baseline and a description of the host.  The baseline runs one
+    thread per core this process may use: the CPUs of its affinity mask, capped by the cgroup's
+    CPU quota and by OMP_NUM_THREADS when either is set (a shared host grants a job a share of
+    its cores)."""
+    affinity = len(os.sched_getaffinity(0))
+    total = os.cpu_count() or affinity
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+        if q != "max":
+            quota = max(1, int(-(-int(q) // int(period))))
+    except (OSError, ValueError):
+        pass
+    omp = None
+    try:
+        omp = int(os.environ.get("OMP_NUM_THREADS", "")) or None
+    except ValueError:
+        pass
+    threads = min(v for v in (affinity, quota, omp) if v)
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, {"host_cores": total, "affinity_cores": affinity, "cgroup_cpu_quota": quota,
+                     "omp_num_threads": omp, "cpu_model": model}
+
+
 def cpu_baseline(p, budget_s):
     """The FP64 oracle (built with the reference's release flags) on this host's cores, on
     rows of the same frame until about budget_s has elapsed."""
     from oracle import oracle
-    threads = min(16, os.cpu_count() or 1)
+    threads, host = host_cpu()
     H = p.imageHeight
     band = max(8, H // 16)
     rays = 0
@@ -217,10 +251,10 @@ def cpu_baseline(p, budget_s):
         if time.time() - t0 >= budget_s or rays >= 64 * H * p.imageWidth:
             break
     dt = time.time() - t0
-    return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": "%d rays (row bands of the same %dx%d frame), FP64 oracle built -O3 "
-                      "-ffast-math -fopenmp, %d threads, %.1f s"
-                      % (rays, p.imageWidth, H, threads, dt)}
+    return dict({"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+                 "sample": "%d rays (row bands of the same %dx%d frame), FP64 oracle built -O3 "
+                           "-ffast-math -fopenmp, %d threads, %.1f s"
+                           % (rays, p.imageWidth, H, threads, dt)}, **host)
 
 
 def time_kernel(torch, ctx, bufs, stream, steps, warmup, step_fn=None):
@@ -246,7 +280,7 @@ def cpu_baseline_batch(frames, budget_s):
     """The FP64 oracle on row bands of the batch's first and last frames (power 2 and 12) until
     about budget_s has elapsed; Mrays/s counts samples."""
     from oracle import oracle
-    threads = min(16, os.cpu_count() or 1)
+    threads, host = host_cpu()
     rays = 0
     t0 = time.time()
     k = 0
@@ -260,10 +294,10 @@ def cpu_baseline_batch(frames, budget_s):
         if time.time() - t0 >= budget_s:
             break
     dt = time.time() - t0
-    return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": "%d samples (8-row bands of frames 0 and 63, 4 spp, GI depth 2), FP64 "
-                      "oracle built -O3 -ffast-math -fopenmp, %d threads, %.1f s"
-                      % (rays, threads, dt)}
+    return dict({"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+                 "sample": "%d samples (8-row bands of frames 0 and 63, 4 spp, GI depth 2), FP64 "
+                           "oracle built -O3 -ffast-math -fopenmp, %d threads, %.1f s"
+                           % (rays, threads, dt)}, **host)
 
 
 def cpu_baseline_gradient(p, budget_s):
@@ -272,7 +306,7 @@ def cpu_baseline_gradient(p, budget_s):
     scaled down to about budget_s; Mpixels/s per gradient."""
     from oracle import oracle
     from mandelbulber2_amd import diff
-    threads = min(16, os.cpu_count() or 1)
+    threads, host = host_cpu()
     theta = diff.theta_of(p)
     rows = 8
     y0 = p.imageHeight // 2 - rows // 2
@@ -290,9 +324,10 @@ def cpu_baseline_gradient(p, budget_s):
             break
     dt = time.time() - t0
     px = grads * rows * p.imageWidth
-    return {"value": px / dt / 1e6, "unit": "Mpixels/s", "cores": threads, "kind": "port",
-            "sample": "%d gradient(s) of the middle %d rows of the 512x512 frame by 21 FP64 oracle "
-                      "renders each (central differences), %d threads, %.1f s" % (grads, rows, threads, dt)}
+    return dict({"value": px / dt / 1e6, "unit": "Mpixels/s", "cores": threads, "kind": "port",
+                 "sample": "%d gradient(s) of the middle %d rows of the 512x512 frame by 21 FP64 oracle "
+                           "renders each (central differences), %d threads, %.1f s"
+                           % (grads, rows, threads, dt)}, **host)
 
 
 def bench_adjoint(args, torch, runtime, wl):

@@ -406,9 +406,12 @@ int mbx_render(mbx_ctx *ctx, const mbx_outputs *out);
 int mbx_distance(mbx_ctx *ctx, const double *points, int n, double detail_size,
 	float *dist_out, int32_t *iters_out);
 
-/* Render n frames (each its own parameter block and output planes) in ONE persistent,
- * work-stealing launch replayed from a captured HIP graph.  The frames may differ in
- * resolution, camera and formula parameters but must share the fractal formula family.
+/* Render n frames (each its own parameter block and output planes) in one launch.  By default
+ * that launch is the grid kernel (one 64-lane workgroup per 64-pixel tile of every frame, launched
+ * directly); MBX_TUNE_KERNEL selects one plain launch per frame or the persistent kernel, and
+ * MBX_TUNE_GRAPH replays the launch from a captured HIP graph.  The frames may differ in
+ * resolution, camera and formula parameters but must share one DE kernel kind (Mandelbulb frames
+ * of different powers share the general-power kernel).
  * Replaces the per-frame job loop of cKeyframeAnimation::RenderKeyframes
  * (mandelbulber2/src/animation_keyframes.cpp:1058-1111), which runs one full render (and one
  * OpenCL launch) per frame.  Asynchronous; mbx_get_stats synchronises. */

@@ -696,8 +696,9 @@ int derive_generic(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 	return MBX_OK;
 }
 
-// Derive the per-frame constants; returns MBX_OK or an error with ctx->err set.
-int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
+// Derive the per-frame constants; returns MBX_OK or an error with ctx->err set.  forceGeneric
+// routes the frame to the general formula kernel whatever its formula (batches that mix kinds).
+int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc, bool forceGeneric)
 {
 	std::memset(&fc, 0, sizeof(fc));
 	if (p->imageWidth <= 0 || p->imageHeight <= 0 || p->imageWidth > 65536 || p->imageHeight > 65536)
@@ -715,7 +716,7 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 	// the specialised kinds cover one Mandelbulb, Mandelbox or IFS with w = 0; everything else
 	// takes the general FP64 path
 	// the general kernel also takes the Mandelbox with rotated folds
-	const bool generic = p->hybridCount >= 1 || fr.initialWAxis != 0.0 ||
+	const bool generic = forceGeneric || p->hybridCount >= 1 || fr.initialWAxis != 0.0 ||
 		(fr.formula == MBX_FORMULA_MANDELBOX && fr.mandelbox.rotationsEnabled) ||
 		(fr.formula != MBX_FORMULA_MANDELBULB && fr.formula != MBX_FORMULA_MANDELBOX &&
 			fr.formula != MBX_FORMULA_KALEIDOSCOPIC_IFS);
@@ -1330,6 +1331,44 @@ int render_frames(mbx_ctx *ctx, const std::vector<FrameConst> &frames, const mbx
 	return MBX_OK;
 }
 
+// Derive the frames of a batch with one kernel specialisation per launch: Mandelbulb frames of
+// mixed powers use the general-power (trigonometric) iteration, and all of them the FP64 one if
+// any frame needs it; any other mix of kinds (a Mandelbox animation that turns its rotated folds
+// on, a hybrid among single formulas) renders every frame on the general formula kernel.
+int derive_batch(mbx_ctx *ctx, const mbx_params *params, FrameConst *frames, int n)
+{
+	for (int i = 0; i < n; i++)
+	{
+		int rc = derive_frame(ctx, &params[i], frames[i], false);
+		if (rc != MBX_OK) return rc;
+	}
+	int kind = frames[0].deKind;
+	bool generic = false;
+	auto bulb = [](int k) { return k == mbx::kBulb8 || k == mbx::kBulbTrig || k == mbx::kBulbD; };
+	for (int i = 1; i < n; i++)
+	{
+		const int k = frames[i].deKind;
+		if (k == kind) continue;
+		if (!bulb(k) || !bulb(kind))
+		{
+			generic = true;
+			break;
+		}
+		kind = (k == mbx::kBulbD || kind == mbx::kBulbD) ? mbx::kBulbD : mbx::kBulbTrig;
+	}
+	if (generic)
+	{
+		for (int i = 0; i < n; i++)
+		{
+			int rc = derive_frame(ctx, &params[i], frames[i], true);
+			if (rc != MBX_OK) return rc;
+		}
+		return MBX_OK;
+	}
+	for (int i = 0; i < n; i++) frames[i].deKind = kind;
+	return MBX_OK;
+}
+
 // Constants of the differentiable render, or the reason the scene is not differentiable.
 std::string derive_diff(const mbx_params *p, const FrameConst &fc, const double lightDir[3], mbx::DiffConst &dc_)
 {
@@ -1481,7 +1520,7 @@ int mbx_scene_upload(mbx_ctx *ctx, const mbx_params *params)
 {
 	if (!ctx || !params) return fail(ctx, MBX_ERR_INVALID, "null argument");
 	FrameConst fc;
-	int rc = derive_frame(ctx, params, fc);
+	int rc = derive_frame(ctx, params, fc, false);
 	if (rc != MBX_OK) return rc;
 	hipError_t e = hipSetDevice(ctx->device);
 	if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
@@ -1586,24 +1625,8 @@ int mbx_render_batch(mbx_ctx *ctx, const mbx_params *params, const mbx_outputs *
 {
 	if (!ctx || !params || !outs || n < 1) return fail(ctx, MBX_ERR_INVALID, "bad argument");
 	std::vector<FrameConst> frames((size_t)n);
-	for (int i = 0; i < n; i++)
-	{
-		int rc = derive_frame(ctx, &params[i], frames[i]);
-		if (rc != MBX_OK) return rc;
-	}
-	// one kernel specialisation per launch: Mandelbulb frames of mixed powers use the general
-	// (trigonometric) iteration, and all of them the FP64 one if any frame needs it
-	int kind = frames[0].deKind;
-	auto bulb = [](int k) { return k == mbx::kBulb8 || k == mbx::kBulbTrig || k == mbx::kBulbD; };
-	for (int i = 1; i < n; i++)
-	{
-		int k = frames[i].deKind;
-		if (k == kind) continue;
-		if (!bulb(k) || !bulb(kind))
-			return fail(ctx, MBX_ERR_UNSUPPORTED, "a batch must share one fractal formula and DE mode");
-		kind = (k == mbx::kBulbD || kind == mbx::kBulbD) ? mbx::kBulbD : mbx::kBulbTrig;
-	}
-	for (auto &f : frames) f.deKind = kind;
+	int rc = derive_batch(ctx, params, frames.data(), n);
+	if (rc != MBX_OK) return rc;
 	return render_frames(ctx, frames, outs);
 }
 

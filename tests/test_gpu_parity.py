@@ -3,8 +3,9 @@
 Bars (the oracle restates the reference CPU renderer in FP64):
   * hit/miss mask and per-pixel iteration count: equal on >= 99.9% of pixels (integers,
     compared exactly);
-  * depth: within 1e-3 relative on >= 99.9% of the pixels both renders hit (FP32 march);
-    Mandelbox scenes march in FP64 in the reference's operation order: depth within 1e-6;
+  * depth: FP32 marches within 1e-4 relative on >= 99.9% of the pixels both renders hit (the
+    headline bar, asserted on the 1920x1080 Mandelbulb-8 frame itself); FP64 marches (Mandelbox,
+    deep zoom) within 1e-6; a comparison without common hits fails;
   * RGB: within 1e-2 absolute on >= 99.9% of pixels.
 """
 import numpy as np
@@ -33,13 +34,12 @@ def _report(name, st):
                                       for k, v in st.items()), flush=True)
 
 
-def _check(st, depth_tol=1e-3):
+def _check(st, depth_tol=1e-4, iters=0.999):
     assert st["mask_agree"] >= 0.999, st
-    assert st["iters_agree"] >= 0.999, st
-    if depth_tol == 1e-3:
-        assert st.get("depth_within_1e-3", 1.0) >= 0.999, st
-    else:
-        assert st.get("depth_rel_p999", 0.0) <= depth_tol, st
+    assert st["iters_agree"] >= iters, st
+    # ">= 99.9% of common hits within depth_tol": the 0.999 quantile of the relative error
+    assert "depth_rel_p999" in st, ("no pixel hit in both renders", st)
+    assert st["depth_rel_p999"] <= depth_tol, st
     assert st["rgb_within_1e-2"] >= 0.999, st
 
 
@@ -95,15 +95,28 @@ def test_mandelbulb8_640x480_parity(rt, oracle_mod):
     assert abs(g["iterations"] + g["far_steps"] - r["iterations"]) / r["iterations"] < 0.001
 
 
+def test_mandelbulb8_1080p_parity(rt, oracle_mod):
+    """The benchmark's own frame: 1920x1080 Mandelbulb-8 from the default launch.  Mask and
+    iteration counts equal on >= 99.9% of pixels, depth within 1e-4 relative on >= 99.9% of the
+    common hits, RGB within 1e-2."""
+    p = scene.mandelbulb8(1920, 1080)
+    g = rt.render_to_host(p)
+    r = oracle_mod.render(p)
+    st = compare(g, r)
+    _report("mandelbulb8 1920x1080", st)
+    _check(st, depth_tol=1e-4)
+    assert st["depth_within_1e-4"] >= 0.999, st
+
+
 @pytest.mark.parametrize("scale", [2.0, -1.5])
 def test_mandelbox_shadow_ao_parity(rt, oracle_mod, scale):
     """Mandelbox, delta DE (the reference's 4-orbit finite difference in FP64), hard shadow
     from the penetrating directional light and 16-ray AO."""
-    p = scene.mandelbox_shadow_ao(192, 108, seed=1, scale=scale)
+    p = scene.mandelbox_shadow_ao(960, 540, seed=1, scale=scale)
     g = rt.render_to_host(p)
     r = oracle_mod.render(p)
     st = compare(g, r)
-    _report("mandelbox %g shadow+AO 192x108" % scale, st)
+    _report("mandelbox %g shadow+AO 960x540" % scale, st)
     _check(st, depth_tol=1e-6)
     assert g["de_evals"] == pytest.approx(r["de_evals"], rel=1e-4)
 
@@ -129,7 +142,7 @@ def test_rotated_node_parity(rt, oracle_mod, which):
     r = oracle_mod.render(p)
     st = compare(g, r)
     _report("rotated %s" % which, st)
-    _check(st, depth_tol=1e-3 if which == "mandelbulb8" else 1e-6)
+    _check(st, depth_tol=1e-4 if which == "mandelbulb8" else 1e-6)
 
 
 def test_mandelbulb_trig_power_parity(rt, oracle_mod):
@@ -189,24 +202,55 @@ def test_batch_equals_single_frames(rt, powers, kernel):
 
 
 def test_animation_batch_parity(rt, oracle_mod):
-    """The batched-animation workload (power sweep, random-walk camera, mixed resolutions, 4 spp
-    Monte Carlo GI of depth 2) at 1/16 resolution, six frames in one persistent launch, each
-    frame against the oracle."""
+    """The batched-animation workload (power sweep 2 -> 12, random-walk camera, mixed
+    resolutions, 4 spp Monte Carlo GI of depth 2): all 64 frames at a quarter of their
+    resolution (480x270 / 320x180 / 160x120) in one launch, each frame against the oracle."""
     import torch
-    frames = scene.animation64(seed=3, frames=6, downscale=16)
+    frames = scene.animation64(seed=3, frames=64, downscale=4)
     ctx = rt.Context(0)
     bufs = [ctx.alloc_outputs(p) for p in frames]
     ctx.render_batch(frames, bufs)
     torch.cuda.synchronize()
+    worst = {"mask_agree": 1.0, "iters_agree": 1.0, "rgb_within_1e-2": 1.0, "depth_rel_p999": 0.0}
     for k, (p, b) in enumerate(zip(frames, bufs)):
         g = {key: t.cpu().numpy() for key, t in b.items() if not key.startswith("_")}
         r = oracle_mod.render(p)
         st = compare(g, r)
-        _report("animation frame %d power %.2f %dx%d" % (k, p.fractal.bulb.power, p.imageWidth,
-                                                         p.imageHeight), st)
-        assert st["mask_agree"] >= 0.999, st
-        assert st["iters_agree"] >= 0.995, st  # general power: asin/atan2/pow in FP32
-        assert st["rgb_within_1e-2"] >= 0.995, st
+        if k % 8 == 0:
+            _report("animation frame %d power %.2f %dx%d" % (k, p.fractal.bulb.power, p.imageWidth,
+                                                             p.imageHeight), st)
+        assert st["mask_agree"] >= 0.999, (k, st)
+        assert st["iters_agree"] >= 0.995, (k, st)  # general power: asin/atan2/pow in FP32
+        assert st["rgb_within_1e-2"] >= 0.995, (k, st)
+        assert st["depth_rel_p999"] <= 1e-4, (k, st)
+        for key in ("mask_agree", "iters_agree", "rgb_within_1e-2"):
+            worst[key] = min(worst[key], st[key])
+        worst["depth_rel_p999"] = max(worst["depth_rel_p999"], st["depth_rel_p999"])
+    _report("animation 64 frames, worst frame", worst)
+    ctx.close()
+
+
+def test_mixed_kind_batch_parity(rt, oracle_mod):
+    """A Mandelbox batch mixing rotated folds (general kernel) with plain frames (delta-DE
+    kernel) renders every frame on the general kernel; each frame against the oracle."""
+    import torch
+    frames = []
+    for k in range(3):
+        p = scene.mandelbox_shadow_ao(96, 54, seed=k + 1)
+        if k == 1:
+            p.fractal.mandelbox.rotationsEnabled = 1
+            p.fractal.mandelbox.rotation[0][0][0] = 11.0
+            p.fractal.mandelbox.rotation[1][2][1] = -23.0
+        frames.append(p)
+    ctx = rt.Context(0)
+    out = [ctx.alloc_outputs(p) for p in frames]
+    ctx.render_batch(frames, out)
+    torch.cuda.synchronize()
+    for k, (p, b) in enumerate(zip(frames, out)):
+        g = {key: t.cpu().numpy() for key, t in b.items() if not key.startswith("_")}
+        st = compare(g, oracle_mod.render(p))
+        _report("mixed mandelbox batch frame %d" % k, st)
+        _check(st, depth_tol=1e-6)
     ctx.close()
 
 
@@ -223,14 +267,14 @@ def _histogram(values, edges):
 
 def test_deep_zoom_mandelbox_parity(rt, oracle_mod):
     """Deep zoom: camera 1e-9 from a Mandelbox surface point found by a seeded FP64 surface
-    search, constant DE threshold 1e-11, 256x256.  Ray positions, scan and orbits run in FP64 on
+    search, constant DE threshold 1e-11, at the configuration's 1024x1024.  Ray positions, scan and orbits run in FP64 on
     the device, so against the all-FP64 oracle the depth (about 1e-9) agrees to 1e-6 relative;
     the per-pixel depth and RGB error histograms are printed."""
-    p = scene.deep_zoom_mandelbox(256, 256, seed=5)
+    p = scene.deep_zoom_mandelbox(1024, 1024, seed=5)
     g = rt.render_to_host(p)
     r = oracle_mod.render(p)
     st = compare(g, r)
-    _report("deep zoom mandelbox 256x256", st)
+    _report("deep zoom mandelbox 1024x1024", st)
     both = (g["mask"] > 0) & (r["mask"] > 0)
     rel = np.abs(g["depth"][both].astype(np.float64) - r["depth"][both]) / r["depth"][both]
     rgb = np.abs(g["rgb"].astype(np.float64) - r["rgb"]).max(axis=-1).ravel()
