@@ -40,6 +40,8 @@ extern "C" {
 #define MBX_MAX_GRADIENT_STOPS 64
 /* fractal containers of a hybrid sequence (NUMBER_OF_FRACTALS, mandelbulber2/src/fractal.h:52) */
 #define MBX_MAX_FRACTALS 9
+/* most frames of one mbx_render_batch_noalloc call (the size of its static device scratch) */
+#define MBX_NOALLOC_MAX_FRAMES 64
 
 /* status codes */
 #define MBX_OK 0
@@ -417,6 +419,17 @@ int mbx_distance(mbx_ctx *ctx, const double *points, int n, double detail_size,
  * OpenCL launch) per frame.  Asynchronous; mbx_get_stats synchronises. */
 int mbx_render_batch(mbx_ctx *ctx, const mbx_params *frames, const mbx_outputs *outs, int n);
 
+/* mbx_render_batch without any device allocation: the same kernels and results, with the
+ * frame constants, job table, tile queues and work counters in a static device array of
+ * about 1.8 MiB per GPU (MBX_NOALLOC_MAX_FRAMES frame blocks), which each GPU the library runs on
+ * holds until the process ends.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc); the context is created beforehand.  n must be 1 .. MBX_NOALLOC_MAX_FRAMES
+ * (MBX_ERR_INVALID otherwise).  All calls on one GPU share that GPU's array, which keeps the
+ * last call's contents: calls that overlap on one GPU (two contexts on different streams)
+ * race.  Graph replay (MBX_TUNE_GRAPH) does not apply; the kernel mode does.  Stats of the call
+ * are read with mbx_get_stats. */
+int mbx_render_batch_noalloc(mbx_ctx *ctx, const mbx_params *frames, const mbx_outputs *outs, int n);
 /* Launch tuning keys for mbx_tune. */
 #define MBX_TUNE_KERNEL 0    /* 0 = one grid launch over every tile of every frame of a render or batch
                                 (default), 1 = one plain launch per frame, 2 = persistent waves

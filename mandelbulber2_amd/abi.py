@@ -12,6 +12,7 @@ MBX_MANDELBOX_FOLDS = 2
 MBX_MAX_AO_VECTORS = 64
 MBX_MAX_GRADIENT_STOPS = 64
 MBX_MAX_FRACTALS = 9
+MBX_NOALLOC_MAX_FRAMES = 64  # frames of one mbx_render_batch_noalloc call
 
 MBX_FORMULA_NONE = 0
 MBX_FORMULA_MANDELBULB = 2

@@ -74,6 +74,14 @@ struct mbx_ctx
 	bool gammaReady = false;
 	std::vector<uint16_t> gammaHost;
 	uint32_t *d_zrange = nullptr; // z-buffer image: the frame's depth range
+	// allocation-free batches (mbx_render_batch_noalloc): the device's static scratch, host staging
+	// sized at creation, and an event marking the end of the last copy out of the staging
+	unsigned char *d_scratch = nullptr;
+	std::vector<FrameConst> naFrames;
+	std::vector<mbx::FrameJob> naJobs;
+	hipEvent_t naCopied = nullptr;
+	bool naCopyPending = false;
+	unsigned long long *statsCounters = nullptr; // counters of the last launch when not a bank
 };
 
 namespace
@@ -1328,6 +1336,7 @@ int render_frames(mbx_ctx *ctx, const std::vector<FrameConst> &frames, const mbx
 	ctx->launched = true;
 	ctx->timedLast = ctx->timing != 0;
 	ctx->rays = rays;
+	ctx->statsCounters = nullptr;
 	return MBX_OK;
 }
 
@@ -1476,11 +1485,23 @@ int mbx_ctx_create(int device, void *stream, mbx_ctx **out)
 	if ((e = hipMalloc((void **)&ctx->d_frame, sizeof(FrameConst))) != hipSuccess ||
 		(e = hipMalloc((void **)&ctx->d_counters, 2 * mbx::kCounterBytes)) != hipSuccess ||
 		(e = hipMalloc((void **)&ctx->d_queue, mbx::kTileQueueBytes)) != hipSuccess ||
-		(e = hipEventCreate(&ctx->ev0)) != hipSuccess || (e = hipEventCreate(&ctx->ev1)) != hipSuccess)
+		(e = hipEventCreate(&ctx->ev0)) != hipSuccess || (e = hipEventCreate(&ctx->ev1)) != hipSuccess ||
+		(e = hipEventCreateWithFlags(&ctx->naCopied, hipEventDisableTiming)) != hipSuccess)
 	{
 		mbx_ctx_destroy(ctx);
 		return MBX_ERR_HIP;
 	}
+	// the allocation-free batch path: this device's static scratch (its address is taken once per
+	// device, with the device current) and host staging for its largest batch
+	void *scratchAddr = nullptr;
+	if ((e = mbx::noalloc_scratch_address(device, &scratchAddr)) != hipSuccess)
+	{
+		mbx_ctx_destroy(ctx);
+		return MBX_ERR_HIP;
+	}
+	ctx->d_scratch = (unsigned char *)scratchAddr;
+	ctx->naFrames.resize(mbx::kNoAllocMaxFrames);
+	ctx->naJobs.resize(mbx::kNoAllocMaxFrames);
 	int cus = 0;
 	if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
 		ctx->numCUs = cus;
@@ -1499,6 +1520,7 @@ void mbx_ctx_destroy(mbx_ctx *ctx)
 	if (ctx->d_counters) (void)hipFree(ctx->d_counters);
 	if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
 	if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+	if (ctx->naCopied) (void)hipEventDestroy(ctx->naCopied);
 	if (ctx->graphExec) (void)hipGraphExecDestroy(ctx->graphExec);
 	if (ctx->d_frames) (void)hipFree(ctx->d_frames);
 	if (ctx->d_jobs) (void)hipFree(ctx->d_jobs);
@@ -1628,6 +1650,89 @@ int mbx_render_batch(mbx_ctx *ctx, const mbx_params *params, const mbx_outputs *
 	int rc = derive_batch(ctx, params, frames.data(), n);
 	if (rc != MBX_OK) return rc;
 	return render_frames(ctx, frames, outs);
+}
+
+int mbx_render_batch_noalloc(mbx_ctx *ctx, const mbx_params *params, const mbx_outputs *outs, int n)
+{
+	if (!ctx || !params || !outs) return fail(ctx, MBX_ERR_INVALID, "null argument");
+	if (n < 1 || n > mbx::kNoAllocMaxFrames)
+		return fail(ctx, MBX_ERR_INVALID,
+			"mbx_render_batch_noalloc: " + std::to_string(n) + " frames; the static scratch holds 1 to " +
+				std::to_string(mbx::kNoAllocMaxFrames));
+	if (!ctx->d_scratch) return fail(ctx, MBX_ERR_HIP, "no static scratch on this device");
+	hipError_t e = hipSetDevice(ctx->device);
+	if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+	// the host staging is overwritten below: the previous call's copy out of it must have ended
+	if (ctx->naCopyPending && (e = hipEventSynchronize(ctx->naCopied)) != hipSuccess)
+		return hip_fail(ctx, e, "staging copy");
+	ctx->naCopyPending = false;
+	FrameConst *frames = ctx->naFrames.data();
+	int rc = derive_batch(ctx, params, frames, n);
+	if (rc != MBX_OK) return rc;
+	const int kind = frames[0].deKind;
+	uint64_t rays = 0;
+	int64_t tiles = 0;
+	size_t pixels = 0;
+	int full = 0;
+	for (int i = 0; i < n; i++)
+	{
+		ctx->naJobs[i] = make_job(frames[i], outs[i], (int)tiles, (int)pixels, ctx->tileShift);
+		tiles = ctx->naJobs[i].tileEnd;
+		pixels += (size_t)frames[i].width * frames[i].height;
+		rays += (uint64_t)frames[i].width * frames[i].height * (uint64_t)frames[i].samples;
+		full |= frame_needs_full(frames[i]) ? 1 : 0;
+		if (tiles > (int64_t)1 << 30) return fail(ctx, MBX_ERR_INVALID, "batch too large");
+	}
+	unsigned char *base = ctx->d_scratch;
+	auto *dCounters = (unsigned long long *)(base + mbx::kNoAllocCountersOff);
+	auto *dQueue = (unsigned int *)(base + mbx::kNoAllocQueueOff);
+	auto *dJobs = (mbx::FrameJob *)(base + mbx::kNoAllocJobsOff);
+	auto *dFrames = (FrameConst *)(base + mbx::kNoAllocFramesOff);
+	// copies on the context stream, ordered after any launch still reading the scratch
+	if ((e = hipMemcpyAsync(dJobs, ctx->naJobs.data(), (size_t)n * sizeof(mbx::FrameJob), hipMemcpyHostToDevice,
+			 ctx->stream)) != hipSuccess ||
+		(e = hipMemcpyAsync(dFrames, frames, (size_t)n * sizeof(FrameConst), hipMemcpyHostToDevice, ctx->stream)) !=
+			hipSuccess)
+		return hip_fail(ctx, e, "stage frames");
+	if ((e = hipEventRecord(ctx->naCopied, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "event");
+	ctx->naCopyPending = true;
+	if ((e = mark(ctx, ctx->ev0)) != hipSuccess) return hip_fail(ctx, e, "event");
+	if (ctx->kernelMode == 1)
+	{
+		if ((e = mbx::launch_clear(dQueue, dCounters, ctx->stream)) != hipSuccess)
+			return hip_fail(ctx, e, "clear counters");
+		for (int i = 0; i < n; i++)
+		{
+			const mbx::Outputs o{outs[i].rgb, outs[i].depth, outs[i].iters, outs[i].mask, outs[i].normal,
+				outs[i].steps};
+			e = mbx::launch_render(kind, dFrames + i, frames[i].width, frames[i].height, o, dCounters,
+				ctx->tileShift, frame_needs_full(frames[i]), ctx->stream);
+			if (e != hipSuccess) return hip_fail(ctx, e, "render launch");
+		}
+	}
+	else
+	{
+		mbx::BatchArgs a{};
+		a.frames = dFrames;
+		a.jobs = dJobs;
+		a.numFrames = n;
+		a.queue = dQueue;
+		a.tileShift = ctx->tileShift;
+		a.counters = dCounters;
+		a.clearFirst = 1; // a clear kernel, not a memset: the call may sit in a captured stream
+		a.clearNext = nullptr;
+		a.full = full;
+		a.mode = ctx->kernelMode == 2 ? mbx::kBatchPersistent : mbx::kBatchGrid;
+		a.totalTiles = (unsigned int)tiles;
+		const int waves = ctx->waves > 0 ? ctx->waves : default_waves(ctx);
+		if ((e = mbx::launch_batch(kind, a, waves, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "batch launch");
+	}
+	if ((e = mark(ctx, ctx->ev1)) != hipSuccess) return hip_fail(ctx, e, "event");
+	ctx->launched = true;
+	ctx->timedLast = ctx->timing != 0;
+	ctx->rays = rays;
+	ctx->statsCounters = dCounters;
+	return MBX_OK;
 }
 
 int mbx_tune(mbx_ctx *ctx, int key, int value)
@@ -1761,8 +1866,8 @@ int mbx_get_stats(mbx_ctx *ctx, mbx_stats *stats)
 	else if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
 		return hip_fail(ctx, e, "render");
 	std::vector<unsigned long long> slots((size_t)mbx::kCounterSlots * mbx::kCounterStride);
-	if ((e = hipMemcpy(slots.data(), counter_bank(ctx, ctx->statsBank), mbx::kCounterBytes, hipMemcpyDeviceToHost)) !=
-		hipSuccess)
+	const unsigned long long *src = ctx->statsCounters ? ctx->statsCounters : counter_bank(ctx, ctx->statsBank);
+	if ((e = hipMemcpy(slots.data(), src, mbx::kCounterBytes, hipMemcpyDeviceToHost)) != hipSuccess)
 		return hip_fail(ctx, e, "read counters");
 	unsigned long long c[mbx::kCounters] = {0, 0, 0};
 	for (unsigned int sl = 0; sl < mbx::kCounterSlots; sl++)

@@ -99,3 +99,24 @@ hipError_t launch_batch(int deKind, BatchArgs a, int numWaves, hipStream_t strea
 hipError_t launch_clear(unsigned int *queue, unsigned long long *counters, hipStream_t stream);
 
 } // namespace mbx
+
+namespace mbx
+{
+
+// Static scratch of the allocation-free batch entry point (mbx_render_batch_noalloc): one
+// file-scope __device__ array per GPU, laid out as the work-counter slots, the tile-queue heads,
+// the job table and the frame constants of at most kNoAllocMaxFrames frames.
+constexpr int kNoAllocMaxFrames = MBX_NOALLOC_MAX_FRAMES;
+constexpr size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+constexpr size_t kNoAllocCountersOff = 0;
+constexpr size_t kNoAllocQueueOff = align256(kNoAllocCountersOff + kCounterBytes);
+constexpr size_t kNoAllocJobsOff = align256(kNoAllocQueueOff + kTileQueueBytes);
+constexpr size_t kNoAllocFramesOff = align256(kNoAllocJobsOff + (size_t)kNoAllocMaxFrames * sizeof(FrameJob));
+constexpr size_t kNoAllocScratchBytes = align256(kNoAllocFramesOff + (size_t)kNoAllocMaxFrames * sizeof(FrameConst));
+
+// Device address of the static scratch on `device`, which the caller has made current: taken
+// with hipGetSymbolAddress on the first call for the device and kept per device.  Host code uses
+// this address, never the array's name, for kernel arguments and copies.
+hipError_t noalloc_scratch_address(int device, void **ptr);
+
+} // namespace mbx

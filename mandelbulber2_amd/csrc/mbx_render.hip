@@ -2,6 +2,8 @@
 // distance query) and their launchers; the per-pixel path is in mbx_pixel.h.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "mbx_pixel.h"
 
 namespace mbx
@@ -164,6 +166,31 @@ __global__ __launch_bounds__(64, occupancy_target<KIND>()) void persistent_kerne
 		wave_add_u64(&slot[1], w.iterations);
 		wave_add_u64(&slot[2], w.far_steps);
 	}
+}
+
+// Static scratch of mbx_render_batch_noalloc (layout in mbx_launch.h): every call on a GPU
+// shares this GPU's copy, which keeps the last call's contents.  Host code never passes the
+// array's name to a kernel: the name is a host-side stand-in, and the device address comes from
+// hipGetSymbolAddress (noalloc_scratch_address) once per device.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+hipError_t noalloc_scratch_address(int device, void **ptr)
+{
+	constexpr int kMaxDevices = 64;
+	static std::mutex lock;
+	static void *address[kMaxDevices] = {};
+	*ptr = nullptr;
+	if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+	std::lock_guard<std::mutex> guard(lock);
+	if (!address[device])
+	{
+		void *p = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (e != hipSuccess) return e;
+		address[device] = p;
+	}
+	*ptr = address[device];
+	return hipSuccess;
 }
 
 // Zero the tile queue heads and the work counters (a kernel, so that it can be captured in a

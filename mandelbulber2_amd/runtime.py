@@ -17,7 +17,7 @@ EXPORTS = ["mbx_version", "mbx_sizeof_params", "mbx_ctx_create", "mbx_ctx_destro
            "mbx_last_error", "mbx_scene_upload", "mbx_scene_info", "mbx_render", "mbx_distance",
            "mbx_get_stats", "mbx_render_batch", "mbx_tune", "mbx_get_tune", "mbx_autotune",
            "mbx_render_diff", "mbx_adjoint", "mbx_params_default", "mbx_load_fract",
-           "mbx_compile_image", "mbx_write_png", "mbx_compile_zbuffer"]
+           "mbx_compile_image", "mbx_write_png", "mbx_compile_zbuffer", "mbx_render_batch_noalloc"]
 
 _lib = None
 
@@ -49,6 +49,7 @@ def load():
     L.mbx_distance.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_void_p]
     L.mbx_get_stats.argtypes = [C.c_void_p, P(abi.MbxStats)]
     L.mbx_render_batch.argtypes = [C.c_void_p, P(abi.MbxParams), P(abi.MbxOutputs), C.c_int]
+    L.mbx_render_batch_noalloc.argtypes = [C.c_void_p, P(abi.MbxParams), P(abi.MbxOutputs), C.c_int]
     L.mbx_tune.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.mbx_get_tune.argtypes = [C.c_void_p, C.c_int, P(C.c_int)]
     L.mbx_render_diff.argtypes = [C.c_void_p, P(abi.MbxOutputs), C.c_void_p]
@@ -194,6 +195,14 @@ class Context:
         arr = (abi.MbxParams * n)(*params_list)
         outs = (abi.MbxOutputs * n)(*[self.outputs_struct(b) for b in bufs_list])
         self._check(self.L.mbx_render_batch(self.h, arr, outs, n), "mbx_render_batch")
+
+    def render_batch_noalloc(self, params_list, bufs_list):
+        """mbx_render_batch_noalloc: the batch render with the library's static device scratch
+        (at most MBX_NOALLOC_MAX_FRAMES frames, no device allocation)."""
+        n = len(params_list)
+        arr = (abi.MbxParams * max(n, 1))(*params_list)
+        outs = (abi.MbxOutputs * max(n, 1))(*[self.outputs_struct(b) for b in bufs_list])
+        self._check(self.L.mbx_render_batch_noalloc(self.h, arr, outs, n), "mbx_render_batch_noalloc")
 
     def compile_image(self, rgb, adj=None, rgb16=None, rgb8=None):
         """cImage::CompileImage + ConvertTo8bitChar on the device: `rgb` is an (H, W, 3) float32
