@@ -1,14 +1,17 @@
 """Benchmark: Mrays/s of the primary DE raymarch on 1080p Mandelbulb-8, one MI355X.
 
-A step is one full frame (1920x1080, 1 ray per pixel: march + binary search + 6-tap normal +
-Lambert shading) rendered by libmbx.so from constants already resident in HBM, launched as the
-batch grid kernel (one 64-lane workgroup per tile; a counter-clear kernel then the render kernel,
-launched directly: a captured graph of the same two kernels costs ~10 us more per frame).  The JSON line
-also carries the FP32 VALU roofline of that kernel (algorithmic flops per launch over the
-launch's device time, HIP events on the launch stream), the plain per-frame kernel timed on the
-same frame, and the CPU oracle timed on this host.
+A step renders 8 frames of 1920x1080 Mandelbulb-8 (1 ray per pixel: march + binary search +
+6-tap normal + Lambert shading) along a camera orbit, 3 degrees per frame, in one grid launch of
+libmbx.so (one 64-lane workgroup per 8x8 tile of every frame), from constants already resident
+in HBM; `--frames 1` makes the step the single-frame launch.  A single frame's launch is bounded
+by its few longest rays, which graze the surface for hundreds of march steps and finish nearly
+alone; the tiles of the other frames of a batch fill the GPU meanwhile.  The JSON line also
+carries the FP32 VALU roofline of the batch launch (algorithmic flops over the launch's device
+time, HIP events on the launch stream), the same frames rendered one plain launch per frame (the
+reference engine's job-per-frame shape: `plain_kernel.batch_speedup`), the persistent tile-queue
+kernel, the single-frame launch, and the CPU oracle timed on this host.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--frames 8]
                     [--workload mandelbulb8|material|mandelbox|menger|hybrid|animation64|deepzoom|adjoint512]
 
 `--workload material` times the same 1080p Mandelbulb-8 with the reference's default material:
@@ -134,6 +137,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="mandelbulb8")
+    ap.add_argument("--frames", type=int, default=8,
+                    help="mandelbulb8: frames of the 1080p orbit rendered per step in one launch "
+                         "(1 = the single-frame launch)")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
@@ -148,6 +154,9 @@ def parse():
                     help="run the launch-shape autotuner (tile shape x wave count) before timing "
                          "the mandelbulb8 workload instead of the library's default tile shape; "
                          "its choice follows timing noise between shapes within ~3%% of each other")
+    ap.add_argument("--fma-probe", action="store_true",
+                    help="measure the unpacked FP32 FMA rate (v_fma_f32, the instructions this "
+                         "build's kernels may use) with a probe kernel compiled by hipcc, and exit")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 256 MiB device-to-device copy, whose FETCH_SIZE/WRITE_SIZE "
                          "counters calibrate the profiler's byte counts")
@@ -159,6 +168,7 @@ def measured_traffic(workload, w, h):
     workload (copy-calibrated), or None when there is none for this size."""
     name = {"image": "compile_image_kernel_1080p.md",
             "mandelbulb8": "batch_kernel_bulb8_mandelbulb8_1080p.md",
+            "mandelbulb8_orbit8": "batch_kernel_bulb8_orbit8_1080p.md",
             "material": "batch_kernel_bulb8_material_1080p.md",
             "mandelbox": "batch_kernel_boxdelta_mandelbox_1080p.md",
             "hybrid": "batch_kernel_generic_hybrid002_1080p.md",
@@ -377,6 +387,103 @@ def bench_adjoint(args, torch, runtime, wl):
     print(json.dumps(out), flush=True)
 
 
+def bench_orbit(args, torch, runtime, wl):
+    """One step = `args.frames` 1920x1080 Mandelbulb-8 frames along a camera orbit (3 degrees
+    per frame) rendered in one grid launch.  The same frames are also timed one plain launch per
+    frame (the reference engine's job-per-frame shape) and in the persistent kernel, and the
+    orbit's first frame alone (the single-frame launch)."""
+    from mandelbulber2_amd import scene
+    n = args.frames
+    frames = scene.mandelbulb8_orbit(args.width, args.height, n)
+    stream = torch.cuda.current_stream(0)
+    ctx = runtime.Context(0, stream)
+    if args.far_field is not None:
+        ctx.tune(runtime.MBX_TUNE_FAR_FIELD, args.far_field)
+    ctx.tune(runtime.MBX_TUNE_TIMING, 0)
+    bufs = [ctx.alloc_outputs(p) for p in frames]
+    ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_GRID)
+    wall, launch_ms = time_kernel(torch, ctx, None, stream, args.steps, args.warmup,
+                                  lambda: ctx.render_batch(frames, bufs))
+    st = ctx.stats()
+    legs = {}
+    if not args.no_plain:
+        psteps = max(3, args.steps // 2)
+        ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_PLAIN)
+        pwall, pms = time_kernel(torch, ctx, None, stream, psteps, 1, lambda: ctx.render_batch(frames, bufs))
+        legs["plain_kernel"] = {"kernel": "render_kernel<kBulb8>, one launch per frame",
+                                "ms_per_step": pwall * 1e3 / psteps,
+                                "mrays_per_s": n * args.width * args.height / (pwall / psteps) / 1e6}
+        ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_PERSISTENT)
+        qwall, _ = time_kernel(torch, ctx, None, stream, psteps, 1, lambda: ctx.render_batch(frames, bufs))
+        legs["persistent_kernel"] = {"kernel": "persistent_kernel<kBulb8>, tile queues",
+                                     "ms_per_step": qwall * 1e3 / psteps}
+        ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_GRID)
+        ctx.upload(frames[0])
+        swall, sms = time_kernel(torch, ctx, bufs[0], stream, args.steps, args.warmup)
+        s1 = ctx.stats()
+        sflops = F_DE * s1["de_evals"] + F_IT["mandelbulb8"] * s1["iterations"] + F_FAR * s1["far_steps"]
+        savg = sum(sms) / len(sms) / 1e3
+        legs["single_frame"] = {"kernel": "batch_kernel<kBulb8>, one frame per launch",
+                                "ms_per_step": swall * 1e3 / args.steps,
+                                "mrays_per_s": args.width * args.height / (swall / args.steps) / 1e6,
+                                "avg_launch_ms": savg * 1e3, "flops_per_launch": sflops,
+                                "roofline_frac": sflops / savg / 1e12 / wl["peak"]}
+    image = None
+    if not args.no_plain:
+        p = frames[0]
+        rgb16 = torch.empty_like(bufs[0]["rgb"], dtype=torch.uint16)
+        rgb8 = torch.empty_like(bufs[0]["rgb"], dtype=torch.uint8)
+        iwall, ims = time_kernel(torch, ctx, None, stream, args.steps, args.warmup,
+                                 lambda: ctx.compile_image(bufs[0]["rgb"], p.image, rgb16, rgb8))
+        ibytes = 21 * p.imageWidth * p.imageHeight
+        iavg = sum(ims) / len(ims) / 1e3
+        itraffic, isrc = measured_traffic("image", p.imageWidth, p.imageHeight)
+        image = {"kernel": "compile_image_kernel", "bound": "hbm", "avg_launch_ms": iavg * 1e3,
+                 "achieved": ibytes / iavg / 1e9, "peak": 8000.0, "unit": "GB/s",
+                 "frac": ibytes / iavg / 1e9 / 8000.0, "bytes_per_launch": ibytes,
+                 "traffic": itraffic, "traffic_source": isrc}
+    ctx.close()
+    if args.calibrate_copy:
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+    rays = n * args.width * args.height
+    ms_per_step = wall * 1e3 / args.steps
+    avg_launch_s = sum(launch_ms) / len(launch_ms) / 1e3
+    flops = F_DE * st["de_evals"] + F_IT["mandelbulb8"] * st["iterations"] + F_FAR * st["far_steps"]
+    achieved = flops / avg_launch_s / 1e12
+    traffic, traffic_src = measured_traffic("mandelbulb8_orbit%d" % n, args.width, args.height)
+    out = {
+        "metric": wl["metric"], "value": rays / (ms_per_step / 1e3) / 1e6, "unit": "Mrays/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": wl["dtype"],
+        "data": "synthetic",
+        "config": dict({"workload": "mandelbulb8_%dx%d_orbit%d" % (args.width, args.height, n),
+                        "frames_per_step": n,
+                        "orbit": "camera at distance 3.5 in the z = 0 plane, 3 degrees per frame",
+                        "launch": "one grid launch per step (8x8 tiles of every frame)"}, **wl["config"]),
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": wl["peak"], "unit": "TFLOP/s",
+                     "frac": achieved / wl["peak"], "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel": wl["kernel"], "avg_launch_ms": avg_launch_s * 1e3,
+                     "median_launch_ms": statistics.median(launch_ms), "flops_per_launch": flops},
+        "de_evals_per_s": st["de_evals"] / avg_launch_s,
+        "iterations_per_s": st["iterations"] / avg_launch_s,
+        "de_evals_per_step": st["de_evals"], "iterations_per_step": st["iterations"],
+        "far_steps_per_step": st["far_steps"],
+    }
+    out.update(legs)
+    if "plain_kernel" in legs:
+        out["plain_kernel"]["batch_speedup"] = legs["plain_kernel"]["ms_per_step"] / ms_per_step
+    if image is not None:
+        out["image_compile"] = image
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(frames[0], args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
 def bench_batch(args, torch, runtime, wl):
     """One step = the whole 64-frame batch in one grid launch; the
     comparison renders the same frames one plain launch per frame."""
@@ -430,8 +537,68 @@ def bench_batch(args, torch, runtime, wl):
     print(json.dumps(out), flush=True)
 
 
+FMA_PROBE_SRC = r"""
+#include <hip/hip_runtime.h>
+#include <cstdio>
+// eight independent FMA chains per lane; 8192 workgroups of 256 lanes (8 waves per SIMD)
+__global__ __launch_bounds__(256) void fma_kernel(float *out, int iters, float a, float b)
+{
+	float x0 = threadIdx.x, x1 = x0 + 1, x2 = x0 + 2, x3 = x0 + 3, x4 = x0 + 4, x5 = x0 + 5, x6 = x0 + 6, x7 = x0 + 7;
+	for (int i = 0; i < iters; i++)
+	{
+		x0 = __builtin_fmaf(x0, a, b); x1 = __builtin_fmaf(x1, a, b); x2 = __builtin_fmaf(x2, a, b);
+		x3 = __builtin_fmaf(x3, a, b); x4 = __builtin_fmaf(x4, a, b); x5 = __builtin_fmaf(x5, a, b);
+		x6 = __builtin_fmaf(x6, a, b); x7 = __builtin_fmaf(x7, a, b);
+	}
+	out[blockIdx.x * 256 + threadIdx.x] = x0 + x1 + x2 + x3 + x4 + x5 + x6 + x7;
+}
+int main()
+{
+	const int blocks = 8192, iters = 20000;
+	float *out = nullptr;
+	if (hipMalloc(&out, sizeof(float) * blocks * 256) != hipSuccess) return 1;
+	hipEvent_t e0, e1;
+	if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return 1;
+	fma_kernel<<<blocks, 256>>>(out, 100, 0.999f, 0.001f);
+	if (hipDeviceSynchronize() != hipSuccess) return 1;
+	float best = 1e30f;
+	for (int rep = 0; rep < 5; rep++)
+	{
+		(void)hipEventRecord(e0);
+		fma_kernel<<<blocks, 256>>>(out, iters, 0.999f, 0.001f);
+		(void)hipEventRecord(e1);
+		if (hipEventSynchronize(e1) != hipSuccess) return 1;
+		float ms = 0.0f;
+		(void)hipEventElapsedTime(&ms, e0, e1);
+		best = ms < best ? ms : best;
+	}
+	printf("%.6f\n", 2.0 * 8.0 * iters * (double)blocks * 256 / best / 1e9);
+	return 0;
+}
+"""
+
+
+def fma_probe():
+    """Unpacked FP32 FMA throughput of the GPU (TFLOP/s), from a probe kernel built with the same
+    flags as the library (packed FP32 instructions off)."""
+    import subprocess
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "fma.hip"), os.path.join(d, "fma")
+        with open(src, "w") as f:
+            f.write(FMA_PROBE_SRC)
+        subprocess.check_call(["/opt/rocm/bin/hipcc", "-O3", "--offload-arch=gfx950", "-Xclang", "-target-feature",
+                               "-Xclang", "-packed-fp32-ops", src, "-o", exe], stderr=subprocess.DEVNULL)
+        tflops = float(subprocess.check_output([exe], timeout=120).decode().split()[-1])
+    print(json.dumps({"metric": "unpacked FP32 FMA rate (v_fma_f32)", "value": tflops, "unit": "TFLOP/s",
+                      "published_fp32_peak": FP32_PEAK_TFLOPS, "fraction_of_published": tflops / FP32_PEAK_TFLOPS}),
+          flush=True)
+
+
 def main():
     args = parse()
+    if args.fma_probe:
+        return fma_probe()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (rays are independent; the multi-GPU "
               "sharding is a design note, not a measured path). Run with --gpus 1.",
@@ -445,6 +612,8 @@ def main():
         return bench_batch(args, torch, runtime, wl)
     if args.workload == "adjoint512":
         return bench_adjoint(args, torch, runtime, wl)
+    if args.workload == "mandelbulb8" and args.frames > 1:
+        return bench_orbit(args, torch, runtime, wl)
     p = make_scene(args.workload, args.width, args.height)
     stream = torch.cuda.current_stream(0)
     ctx = runtime.Context(0, stream)

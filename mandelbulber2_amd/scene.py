@@ -245,6 +245,19 @@ def mandelbulb8(width=640, height=480, seed=0):
     return p
 
 
+def mandelbulb8_orbit(width=1920, height=1080, frames=8, step_deg=3.0):
+    """Frames of the Mandelbulb-8 scene along a camera orbit: frame k has the camera at distance
+    3.5 in the z = 0 plane, k * step_deg degrees round the z axis from (0, -3.5, 0), looking at
+    the origin (an animation's consecutive frames)."""
+    out = []
+    for k in range(frames):
+        p = mandelbulb8(width, height, seed=k)
+        a = math.radians(step_deg * k)
+        abi.set3(p.camera, (3.5 * math.sin(a), -3.5 * math.cos(a), 0.0))
+        out.append(p)
+    return out
+
+
 def splitmix64(state):
     """SplitMix64 step: returns (value, new_state)."""
     state = (state + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
