@@ -55,6 +55,8 @@ def lib(fast=False):
     L.mbxo_compile_zbuffer.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                        C.c_double, C.c_void_p]
     L.mbxo_compile_image.argtypes = [P(abi.MbxImageAdjustments), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    L.mbxo_shader_probe.argtypes = [P(abi.MbxParams), C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_uint32,
+                                    C.c_void_p, C.c_void_p]
     L.mbxo_random_int.argtypes = [P(C.c_uint32)]
     L.mbxo_random.argtypes = [P(C.c_uint32), C.c_int]
     L.mbxo_pixel_seed.argtypes = [C.c_uint32, C.c_uint32]
@@ -233,6 +235,23 @@ def normal(p, pt, dist_thresh):
     if rc != 0:
         raise RuntimeError("oracle normal failed: %d" % rc)
     return n
+
+
+SHADOW, AO_RAYS, AO_FAST, GI = 0, 1, 2, 3
+
+
+def shader_probe(p, which, point, normal, dist_thresh, seed=1, colour=(1.0, 1.0, 1.0)):
+    """A secondary-ray shader of the oracle at one surface point (mbxo_shader_probe): the shadow
+    factor (SHADOW), the AO colour (AO_RAYS, AO_FAST) or the GI radiance (GI) as 3 floats."""
+    pt = np.ascontiguousarray(point, np.float64)
+    n = np.ascontiguousarray(normal, np.float64)
+    col = np.ascontiguousarray(colour, np.float32)
+    out = np.zeros(3, np.float32)
+    rc = lib().mbxo_shader_probe(C.byref(p), which, _ptr(pt), _ptr(n), float(dist_thresh), seed, _ptr(col),
+                                 _ptr(out))
+    if rc != 0:
+        raise RuntimeError("oracle shader_probe failed: %d" % rc)
+    return out
 
 
 def random_ints(seed, count, maxv=None):

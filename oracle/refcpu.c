@@ -20,8 +20,11 @@
  * sources (oracle/_ref); Park-Miller against the reference's common_math.cpp; Compute against
  * the 2D Multibrot closed form on the z=0 plane and the Mandelbox linear regime; the march
  * against dense FP64 ray sampling; normals against a high-order finite difference of the DE;
- * camera against the closed-form orthonormal frame.  Shading functions (light, shadow, AO,
- * background, GI) are restatements checked by invariants only: parity unpinned beyond them.
+ * camera against the closed-form orthonormal frame; AuxShadow and multi-ray AO against dense
+ * brute-force walks along their rays; fast AO against its closed form without the tune; GI by
+ * depth 0 leaving pixels unchanged and missed bounces adding the background; the delta DE against
+ * an independent forward-mode Jacobian on affine orbits; the Mandelbox, IFS and light rotations
+ * against quarter turns derived from the reference's rotation definitions (tests/test_oracle_pins.py).
  */
 #include <math.h>
 #include <stddef.h>
@@ -2370,6 +2373,54 @@ int mbxo_normal(const mbx_params *p, const double *pt3, double distThresh, doubl
 	n3[1] = n.y;
 	n3[2] = n.z;
 	return MBX_OK;
+}
+
+/* Secondary-ray shaders at one surface point, for tests that pin them: `which` 0 = AuxShadow of
+ * light #1 over its distance (delta / resolution for the penetrating light, view_distance_max
+ * otherwise; out3[0]), 1 = AmbientOcclusion with the scene's AO vectors, 2 = FastAmbientOcclusion,
+ * 3 = GlobalIlumination with the surface colour col3 (RGB).  The shader input is the point, the
+ * normal, the distance threshold and delta = CalcDelta at the point; the random stream starts at
+ * `seed`. */
+int mbxo_shader_probe(const mbx_params *p, int which, const double *pt3, const double *n3, double distThresh,
+	uint32_t seed, const float *col3, float *out3)
+{
+	scene_t s;
+	int rc0 = scene_init(&s, p);
+	if (rc0) return rc0;
+	uint32_t sd = seed;
+	ray_ctx rc = {&s, &sd, 0, 0};
+	shin_t in;
+	memset(&in, 0, sizeof(in));
+	in.point = V3(pt3[0], pt3[1], pt3[2]);
+	in.normal = V3(n3[0], n3[1], n3[2]);
+	in.distThresh = distThresh;
+	in.delta = calc_delta(&s, in.point);
+	out3[0] = out3[1] = out3[2] = 0.0f;
+	switch (which)
+	{
+		case 0:
+		{
+			const double distance = p->light.penetrating ? in.delta / s.resolution : p->viewDistanceMax;
+			out3[0] = aux_shadow(&rc, &in, distance, s.lightDirection);
+			return MBX_OK;
+		}
+		case 1:
+		case 2:
+		{
+			const rgba a = which == 1 ? ambient_occlusion(&rc, &in) : fast_ambient_occlusion(&rc, &in);
+			out3[0] = a.R;
+			out3[1] = a.G;
+			out3[2] = a.B;
+			return MBX_OK;
+		}
+		case 3:
+		{
+			const rgba c = RGBA(col3[0], col3[1], col3[2], 1.0f);
+			global_illumination(&rc, &in, c, out3);
+			return MBX_OK;
+		}
+		default: return MBX_ERR_INVALID;
+	}
 }
 
 /* Colour index (Compute<calcModeColouring> + CalculateColorIndex) of fractal-local points. */
