@@ -423,3 +423,28 @@ def test_light_vector_absolute_quarter_turns(angles, expect):
     abi.set3(p.light.rotation, angles)
     _, _, light = oracle.frame_constants(p)
     np.testing.assert_allclose(light, expect, rtol=0, atol=1e-14)
+
+
+# ------------------------------------------------------------------------------------------
+# Lambert light (shader_light_shading.cpp:40-122, shader_object.cpp:40-229)
+# ------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("shading", [0.0, 1.0, 0.4])
+def test_lambert_light_closed_form(shading):
+    """No fill light, AO, luminosity or highlights: a hit's colour is
+    surface * light colour * intensity * (1 - shading + shading * max(n . l, 0)), with n the
+    render's normal and l the light vector; shading 0 removes the normal altogether."""
+    p = scene.mandelbulb8(96, 72)
+    p.shading = shading
+    p.light.intensity = 1.7
+    abi.set3(p.light.color, (0.9, 0.8, 0.5))
+    abi.set3(p.surfaceColor, (0.6, 0.3, 0.2))
+    abi.set3(p.fillLightColor, (0.0, 0.0, 0.0))
+    r = oracle.render(p)
+    _, _, light = oracle.frame_constants(p)
+    hit = r["mask"].ravel() > 0
+    assert hit.sum() > 500
+    n = r["normal"].reshape(-1, 3)[hit].astype(np.float64)
+    lam = np.maximum(n @ np.asarray(light), 0.0)
+    shade = (1.0 - shading + shading * lam) * 1.7
+    expect = shade[:, None] * np.array([0.9 * 0.6, 0.8 * 0.3, 0.5 * 0.2])[None, :]
+    np.testing.assert_allclose(r["rgb"].reshape(-1, 3)[hit], expect, rtol=2e-6, atol=2e-7)
