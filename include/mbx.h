@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MBX_ABI_VERSION 5
+#define MBX_ABI_VERSION 6
 
 /* array sizes as in mandelbulber2/src/fractal.h:52-54 */
 #define MBX_IFS_VECTOR_COUNT 9
@@ -336,6 +336,20 @@ typedef struct mbx_params {
 
 	/* image adjustments for mbx_compile_image (not used by the render) */
 	mbx_image_adjustments image;
+	/* global foldings before every formula iteration of Compute (compute_fractal.cpp:192-202):
+	 * BoxFolding and SphericalFolding (legacy_fractal_transforrms.cpp:43-101), read as
+	 * sCommonParams::foldings (fractparams.cpp:346-351); either one takes the general kernel */
+	int32_t boxFolding;             /* "box_folding" */
+	int32_t sphericalFolding;       /* "spherical_folding" */
+	double boxFoldingLimit;         /* "box_folding_limit" */
+	double boxFoldingValue;         /* "box_folding_value" */
+	double sphericalFoldingOuter;   /* "spherical_folding_outer" */
+	double sphericalFoldingInner;   /* "spherical_folding_inner" */
+	/* normals from the iteration counts of an 11 x 11 x 11 grid of orbits round the hit
+	 * (CalculateNormals, shader_calculate_normals.cpp:81-109) instead of the DE's central
+	 * differences */
+	int32_t slowShading;            /* "slow_shading" */
+	int32_t reserved4;
 } mbx_params;
 
 /* Per-pixel output planes (device pointers; any may be NULL except where noted).

@@ -242,6 +242,14 @@ class MbxParams(C.Structure):
         ("detailSizeMin", C.c_double),
         ("deltaDERelativeDelta", C.c_double),
         ("image", MbxImageAdjustments),
+        ("boxFolding", C.c_int32),
+        ("sphericalFolding", C.c_int32),
+        ("boxFoldingLimit", C.c_double),
+        ("boxFoldingValue", C.c_double),
+        ("sphericalFoldingOuter", C.c_double),
+        ("sphericalFoldingInner", C.c_double),
+        ("slowShading", C.c_int32),
+        ("reserved4", C.c_int32),
     ]
 
 

@@ -781,6 +781,35 @@ __device__ __forceinline__ typename Prec<PosPrecision<KIND>::kFP64>::R calculate
 	return d;
 }
 
+// Iteration count of Compute<calcModeNormal> at p (compute_fractal.cpp:67-654, iters = i + 1):
+// the orbit alone, without CalculateDistance's node transform (slow-shading normals call Compute
+// on the grid points directly, shader_calculate_normals.cpp:96-101)
+template <int KIND>
+__device__ __forceinline__ int orbit_iterations(typename Prec<PosPrecision<KIND>::kFP64>::P p, const FrameConst &f,
+	Work &w)
+{
+	int iters;
+	if constexpr (KIND == kBoxDelta)
+	{
+		DV3 z;
+		iters = box_orbit_d(p, f, f.formulaMaxiter, z);
+	}
+	else if constexpr (KIND == kBoxAnalytic)
+		(void)compute_box_d<false>(p, f, 0.0, iters);
+	else if constexpr (KIND == kBulbD)
+		(void)compute_bulb_d(p, f, iters);
+	else if constexpr (KIND == kGeneric)
+	{
+		GenAux a;
+		DV4 z;
+		iters = gen_orbit(f, p, f.formulaMaxiter, z, a);
+	}
+	else
+		(void)compute_analytic<KIND>(p, f, w, iters);
+	w.iterations += (uint32_t)iters;
+	return iters;
+}
+
 // ------------------------------------------------------------------------------------------
 // Compute<calcModeColouring> (compute_fractal.cpp:67-654, colouring branch :343-436) and
 // CalculateColorIndex for a single formula (fractal_coloring.cpp:318-340).  SurfaceColour runs it

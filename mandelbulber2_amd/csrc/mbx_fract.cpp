@@ -625,6 +625,14 @@ extern "C" void mbx_params_default(mbx_params *p)
 	p->image.gamma = 1.0f;
 	p->image.saturation = 1.0f;
 	p->image.hdr = 0;
+	// global foldings (initparameters.cpp:199-204) and slow shading (:140)
+	p->boxFolding = 0;
+	p->boxFoldingLimit = 1.0;
+	p->boxFoldingValue = 2.0;
+	p->sphericalFolding = 0;
+	p->sphericalFoldingOuter = 1.0;
+	p->sphericalFoldingInner = 0.5;
+	p->slowShading = 0;
 	// palette colouring of material #1 (InitMaterialParams)
 	p->useColorsFromPalette = 1;
 	p->surfaceGradientEnable = 1;
@@ -990,6 +998,14 @@ extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, 
 	p.image.saturation = (float)M.d("saturation", 1.0, 0.0, 1e15);
 	p.image.hdr = M.b("hdr", false) ? 1 : 0;
 	if (p.iterThreshMode || p.interiorMode) flags |= MBX_FRACT_INTERIOR;
+	// sCommonParams::foldings (fractparams.cpp:346-351) and slow shading (fractparams.cpp:218)
+	p.boxFolding = M.b("box_folding", false) ? 1 : 0;
+	p.boxFoldingLimit = M.d("box_folding_limit", 1.0);
+	p.boxFoldingValue = M.d("box_folding_value", 2.0);
+	p.sphericalFolding = M.b("spherical_folding", false) ? 1 : 0;
+	p.sphericalFoldingOuter = M.d("spherical_folding_outer", 1.0);
+	p.sphericalFoldingInner = M.d("spherical_folding_inner", 0.5);
+	p.slowShading = M.b("slow_shading", false) ? 1 : 0;
 
 	// ambient occlusion
 	p.ambientOcclusion = M.d("ambient_occlusion", 1.0, 0.0, 1e15);

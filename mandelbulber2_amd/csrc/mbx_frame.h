@@ -208,6 +208,12 @@ struct FrameConst
 	int32_t genColoring;   // single-formula colour index: 0 default, 1 Mandelbox, 2 IFS
 	int32_t genPad;
 	double genInitialW, genLinearDEOffset;
+	// global foldings before every formula iteration (compute_fractal.cpp:192-202): general kernel
+	int32_t boxFold, sphFold;
+	double boxFoldLimitD, boxFoldValueD, sphFR2D, sphMR2D, sphFactor1D;
+	// normals from the iteration counts of an 11^3 grid (shader_calculate_normals.cpp:81-109)
+	int32_t slowShading;
+	int32_t slowShadingN; // params->N of the pseudo-distance 1 + N - iterations
 	uint8_t genSeq[kGenSeqLength];
 	GenSlot gen[MBX_MAX_FRACTALS];
 };

@@ -400,11 +400,49 @@ __device__ __forceinline__ void gen_formula(const GenSlot &s, DV4 &z, GenAux &a)
 
 // ---- one iteration of the sequence (compute_fractal.cpp:170-321) ------------------------------
 
+// BoxFolding and SphericalFolding (legacy_fractal_transforrms.cpp:43-101) before the formula, each
+// followed by aux.r = |z| (compute_fractal.cpp:192-202)
+__device__ __forceinline__ void gen_global_foldings(const FrameConst &f, DV4 &z, GenAux &a)
+{
+#pragma clang fp contract(off)
+	if (f.boxFold)
+	{
+		const double L = f.boxFoldLimitD, V = f.boxFoldValueD;
+		if (z.x > L) { z.x = V - z.x; a.color *= 0.9; }
+		else if (z.x < -L) { z.x = -V - z.x; a.color *= 0.9; }
+		if (z.y > L) { z.y = V - z.y; a.color *= 0.9; }
+		else if (z.y < -L) { z.y = -V - z.y; a.color *= 0.9; }
+		if (z.z > L) { z.z = V - z.z; a.color *= 0.9; }
+		else if (z.z < -L) { z.z = -V - z.z; a.color *= 0.9; }
+		a.r = len4(z);
+	}
+	if (f.sphFold)
+	{
+		const double r2 = a.r * a.r;
+		if (r2 < f.sphMR2D)
+		{
+			const double k = f.sphFactor1D;
+			z = dv4(z.x * k, z.y * k, z.z * k, z.w * k);
+			a.DE *= k;
+			a.color *= 0.9;
+		}
+		else if (r2 < f.sphFR2D)
+		{
+			const double k = f.sphFR2D / r2;
+			z = dv4(z.x * k, z.y * k, z.z * k, z.w * k);
+			a.DE *= k;
+			a.color *= 0.9;
+		}
+		a.r = len4(z);
+	}
+}
+
 // formula, z + c, hybrid weight; returns the iteration's container
 __device__ __forceinline__ const GenSlot &gen_step(const FrameConst &f, int i, DV4 &z, GenAux &a)
 {
 #pragma clang fp contract(off)
 	const GenSlot &s = f.gen[i < f.genSeqLength ? f.genSeq[i] : f.genSeq[0]];
+	if (f.boxFold || f.sphFold) gen_global_foldings(f, z, a);
 	const DV4 tempZ = z;
 	const double tempDE = a.DE, tempColor = a.color;
 	a.i = i;

@@ -724,7 +724,8 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc, bool forceGe
 	// the specialised kinds cover one Mandelbulb, Mandelbox or IFS with w = 0; everything else
 	// takes the general FP64 path
 	// the general kernel also takes the Mandelbox with rotated folds
-	const bool generic = forceGeneric || p->hybridCount >= 1 || fr.initialWAxis != 0.0 ||
+	const bool generic = forceGeneric || p->hybridCount >= 1 || fr.initialWAxis != 0.0 || p->boxFolding ||
+		p->sphericalFolding ||
 		(fr.formula == MBX_FORMULA_MANDELBOX && fr.mandelbox.rotationsEnabled) ||
 		(fr.formula != MBX_FORMULA_MANDELBULB && fr.formula != MBX_FORMULA_MANDELBOX &&
 			fr.formula != MBX_FORMULA_KALEIDOSCOPIC_IFS);
@@ -1107,6 +1108,16 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc, bool forceGe
 	if (p->rngMode != MBX_RNG_PER_PIXEL)
 		return fail(ctx, MBX_ERR_UNSUPPORTED, "the device uses per-pixel random streams only");
 	fc.randomSeed = p->randomSeed;
+	// global foldings (fractparams.cpp:346-351; the general kernel applies them) and slow shading
+	fc.boxFold = p->boxFolding ? 1 : 0;
+	fc.sphFold = p->sphericalFolding ? 1 : 0;
+	fc.boxFoldLimitD = p->boxFoldingLimit;
+	fc.boxFoldValueD = p->boxFoldingValue;
+	fc.sphFR2D = p->sphericalFoldingOuter * p->sphericalFoldingOuter;
+	fc.sphMR2D = p->sphericalFoldingInner * p->sphericalFoldingInner;
+	fc.sphFactor1D = fc.sphFR2D / fc.sphMR2D;
+	fc.slowShading = p->slowShading ? 1 : 0;
+	fc.slowShadingN = fr.formulaMaxiter;
 
 	// fractal node rotation (objects_tree.cpp:292-382): local-to-world is R^T with
 	// R = SetRotation2(rotation * pi/180), so world-to-local (calculate_distance.cpp:296) is R
@@ -1177,7 +1188,7 @@ bool same_frames(const std::vector<FrameConst> &a, const std::vector<FrameConst>
 bool frame_needs_full(const FrameConst &f)
 {
 	return f.monteCarlo || f.aoEnabled || (f.lightEnabled && (f.castShadows || f.specPlastic || f.specMetal)) ||
-		(f.palette && f.paletteGradient) || f.glow;
+		(f.palette && f.paletteGradient) || f.glow || f.slowShading;
 }
 
 // Upload frames and jobs when they differ from what the device holds.

@@ -248,6 +248,40 @@ __device__ V3 calculate_normals(const FrameConst &frame, typename PrecOf<KIND>::
 	return n;
 }
 
+// CalculateNormals with slow_shading (shader_calculate_normals.cpp:81-123): the sum of
+// q * (1 + N - iterations) over the 11 x 11 x 11 grid q in {-1, -0.8, .., 1}^3 (the loop counters
+// accumulate 0.2 in FP64 as the reference's do) of orbits at p + q delta, normalised, with the
+// reference's fallback (1, 0, 0).  Out of line: 1331 orbits per hit, only for scenes asking for it.
+template <int KIND>
+__device__ __attribute__((noinline)) V3 slow_normals(const FrameConst &frame, typename PrecOf<KIND>::P p,
+	double delta, Work &w)
+{
+#pragma clang fp contract(off)
+	const FrameConst &f = uniform_ref(frame);
+	using Q = PrecOf<KIND>;
+	using R = typename Q::R;
+	using P = typename Q::P;
+	const double d = delta * (double)f.smoothness;
+	double nx = 0.0, ny = 0.0, nz = 0.0;
+	for (double qx = -1.0; qx <= 1.0; qx += 0.2)
+		for (double qy = -1.0; qy <= 1.0; qy += 0.2)
+			for (double qz = -1.0; qz <= 1.0; qz += 0.2)
+			{
+				P q = p;
+				q.x = p.x + (R)(qx * d);
+				q.y = p.y + (R)(qy * d);
+				q.z = p.z + (R)(qz * d);
+				const double pseudo = (double)(1 + f.slowShadingN - orbit_iterations<KIND>(q, f, w));
+				nx = nx + qx * pseudo;
+				ny = ny + qy * pseudo;
+				nz = nz + qz * pseudo;
+			}
+	if (fabs(nx) < 1e-50 && fabs(ny) < 1e-50 && fabs(nz) < 1e-50) return v3(1.0f, 0.0f, 0.0f);
+	const double inv = 1.0 / sqrt(nx * nx + ny * ny + nz * nz);
+	const V3 n = v3((float)(nx * inv), (float)(ny * inv), (float)(nz * inv));
+	return finite3(n) ? n : v3(1.0f, 0.0f, 0.0f);
+}
+
 // AuxShadow for the directional light (shader_aux_shadow.cpp:39-323): hard, or the soft
 // cone variant when soft_shadow_cone > 0
 template <int KIND>
@@ -554,7 +588,8 @@ __device__ V3 global_illumination(const FrameConst &frame, typename PrecOf<KIND>
 		if (found)
 		{
 			p = hit;
-			n = calculate_normals<KIND>(f, p, thrCur, w);
+			// the bounce's shader input has delta = distThresh (shader_global_illumination.cpp:86)
+			n = f.slowShading ? slow_normals<KIND>(f, p, (double)thrCur, w) : calculate_normals<KIND>(f, p, thrCur, w);
 			const V3 colour = surface_colour<KIND>(f, p, w);
 			V3 spec;
 			result = object_shader<KIND>(f, p, n, thrCur, thrCur, dirf, colour, spec, seed, w);
@@ -649,8 +684,11 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 		V3 n = v3(0.0f, 0.0f, 0.0f);
 		if (m.found)
 		{
-			n = calculate_normals<KIND>(f, m.point, m.distThresh, w);
 			const R delta = calc_delta<KIND>(f, m.point);
+			if (FULL && f.slowShading)
+				n = slow_normals<KIND>(f, m.point, (double)delta, w);
+			else
+				n = calculate_normals<KIND>(f, m.point, m.distThresh, w);
 			const V3 colour = FULL ? surface_colour<KIND>(f, m.point, w)
 								   : v3(f.surfaceColor[0], f.surfaceColor[1], f.surfaceColor[2]);
 			V3 spec;

@@ -153,6 +153,14 @@ def default_params(width=640, height=480):
     p.deltaDERelativeDelta = 0.01
     p.image.brightness = p.image.contrast = p.image.gamma = p.image.saturation = 1.0
     p.image.hdr = 0
+    # global foldings (initparameters.cpp:199-204) and slow shading (:140)
+    p.boxFolding = 0
+    p.boxFoldingLimit = 1.0
+    p.boxFoldingValue = 2.0
+    p.sphericalFolding = 0
+    p.sphericalFoldingOuter = 1.0
+    p.sphericalFoldingInner = 0.5
+    p.slowShading = 0
     # palette colouring of material #1 (InitMaterialParams, initparameters.cpp:1600-1810)
     p.useColorsFromPalette = 1
     p.surfaceGradientEnable = 1

@@ -151,6 +151,13 @@ def init_params():
     c.add("contrast", DOUBLE, 1.0, 0.0, 1e15)                       # :209
     c.add("gamma", DOUBLE, 1.0, 0.0, 1e15)                          # :210
     c.add("hdr", BOOL, False)                                       # :211
+    c.add("slow_shading", BOOL, False)                              # :140
+    c.add("box_folding", BOOL, False)                               # :199
+    c.add("box_folding_limit", DOUBLE, 1.0)                         # :200
+    c.add("box_folding_value", DOUBLE, 2.0)                         # :201
+    c.add("spherical_folding", BOOL, False)                         # :202
+    c.add("spherical_folding_outer", DOUBLE, 1.0)                   # :203
+    c.add("spherical_folding_inner", DOUBLE, 0.5)                   # :204
     c.add("saturation", DOUBLE, 1.0, 0.0, 1e15)                     # :212
     c.add("ambient_occlusion", DOUBLE, 1.0, 0.0, 1e15)              # :213
     c.add("ambient_occlusion_quality", INT, 4, 1, 10)               # :214
@@ -789,6 +796,14 @@ def load(text):
     out["image.hdr"] = int(g("hdr"))
     if g("iteration_threshold_mode") or g("interior_mode"):
         flags |= FLAGS["interior"]
+    # sCommonParams::foldings (fractparams.cpp:346-351), slow shading (fractparams.cpp:218)
+    out["boxFolding"] = int(g("box_folding"))
+    out["boxFoldingLimit"] = g("box_folding_limit")
+    out["boxFoldingValue"] = g("box_folding_value")
+    out["sphericalFolding"] = int(g("spherical_folding"))
+    out["sphericalFoldingOuter"] = g("spherical_folding_outer")
+    out["sphericalFoldingInner"] = g("spherical_folding_inner")
+    out["slowShading"] = int(g("slow_shading"))
     out["ambientOcclusion"] = g("ambient_occlusion")
     out["ambientOcclusionQuality"] = g("ambient_occlusion_quality")
     out["ambientOcclusionFastTune"] = g("ambient_occlusion_fast_tune")

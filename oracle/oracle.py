@@ -37,6 +37,7 @@ def lib(fast=False):
     L.mbxo_compute.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                C.c_void_p, C.c_void_p]
     L.mbxo_formula_step.argtypes = [P(abi.MbxParams), C.c_int, C.c_void_p, C.c_void_p]
+    L.mbxo_global_foldings.argtypes = [P(abi.MbxParams), C.c_int, C.c_void_p, C.c_void_p]
     L.mbxo_frame_constants.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_void_p, C.c_void_p]
     L.mbxo_march_pixel.argtypes = [P(abi.MbxParams), C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -130,6 +131,16 @@ def formula_step(p, z4, aux3):
     rc = L.mbxo_formula_step(C.byref(p), len(z4), _ptr(z4), _ptr(aux3))
     if rc != 0:
         raise RuntimeError("oracle formula_step failed: %d" % rc)
+    return z4, aux3
+
+
+def global_foldings(p, z4, aux3):
+    """BoxFolding then SphericalFolding (as p enables them) on z4 and aux3 (r, DE, color)."""
+    z4 = np.array(z4, np.float64).reshape(-1, 4).copy()
+    aux3 = np.array(aux3, np.float64).reshape(-1, 3).copy()
+    rc = lib().mbxo_global_foldings(C.byref(p), len(z4), _ptr(z4), _ptr(aux3))
+    if rc != 0:
+        raise RuntimeError("oracle global_foldings failed: %d" % rc)
     return z4, aux3
 
 

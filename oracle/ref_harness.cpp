@@ -7,6 +7,7 @@
 //   mandelbulber2/formula/definition/fractal_kaleidoscopic_ifs.cpp
 //   mandelbulber2/formula/definition/fractal_{mandelbulb_power2,hypercomplex,quaternion,
 //     menger_sponge,xenodreambuie,box_fold_bulb_pow2,aexion}.cpp
+//   mandelbulber2/formula/definition/legacy_fractal_transforrms.cpp (BoxFolding, SphericalFolding)
 //   mandelbulber2/src/common_math.cpp
 // compiled where they lie under the reference checkout.  abstract_fractal.cpp and algebra.cpp
 // need Qt and are not built: the link leaves their symbols (the cAbstractFractal base
@@ -22,6 +23,8 @@
 #include <cstdint>
 
 #include "formula/definition/all_fractal_definitions.h"
+#include "formula/definition/legacy_fractal_transforms.hpp"
+#include "src/common_params.hpp"
 #include "src/fractal.h"
 #include "src/common_math.h"
 
@@ -283,6 +286,38 @@ void gen_aexion()
 	}
 }
 
+// Global foldings (compute_fractal.cpp:192-202).  Params: boxLimit, boxValue, sphericalOuter,
+// sphericalInner.  Each record is one call on a fresh point (aux.r = |z| as Compute leaves it).
+void gen_foldings()
+{
+	const double cases[][4] = {{1.0, 2.0, 1.0, 0.5}, {0.7, 1.5, 1.3, 0.4}, {1.2, 2.6, 0.9, 0.6}};
+	for (const auto &c : cases)
+	{
+		sFractalFoldings fold;
+		std::memset(&fold, 0, sizeof(fold));
+		fold.boxLimit = c[0];
+		fold.boxValue = c[1];
+		fold.sphericalOuter = c[2];
+		fold.sphericalInner = c[3];
+		for (int k = 0; k < 48; k++)
+		{
+			const double range = (k < 24) ? 2.0 : 1.2;
+			CVector4 z(uniform(-range, range), uniform(-range, range), uniform(-range, range),
+				(k % 3 == 0) ? uniform(-0.5, 0.5) : 0.0);
+			sExtendedAux aux;
+			fresh_aux(aux, z);
+			CVector4 zin = z;
+			sExtendedAux ain = aux;
+			BoxFolding(z, &fold, aux);
+			print_record("boxfolding", c, 4, zin, ain, z, aux);
+			z = zin;
+			aux = ain;
+			SphericalFolding(z, &fold, aux);
+			print_record("sphericalfolding", c, 4, zin, ain, z, aux);
+		}
+	}
+}
+
 void gen_random()
 {
 	gRandomSeed = 1;
@@ -314,6 +349,7 @@ int main()
 	gen_xenodreambuie();
 	gen_box_fold_bulb_pow2();
 	gen_aexion();
+	gen_foldings();
 	gen_random();
 	return 0;
 }

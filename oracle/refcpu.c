@@ -982,6 +982,46 @@ static v4 add_constant(const scene_t *s, v4 z, const aux_t *aux)
 						 aux->const_c.z * s->constantMultiplier.z, aux->const_c.w));
 }
 
+/* BoxFolding and SphericalFolding (legacy_fractal_transforrms.cpp:43-101), applied before every
+ * formula iteration when enabled (compute_fractal.cpp:192-202), each followed by aux.r = |z| */
+static void global_foldings(const mbx_params *p, v4 *zp, aux_t *aux)
+{
+	v4 z = *zp;
+	if (p->boxFolding)
+	{
+		const double L = p->boxFoldingLimit, V = p->boxFoldingValue;
+		if (z.x > L) { z.x = V - z.x; aux->color *= 0.9; }
+		else if (z.x < -L) { z.x = -V - z.x; aux->color *= 0.9; }
+		if (z.y > L) { z.y = V - z.y; aux->color *= 0.9; }
+		else if (z.y < -L) { z.y = -V - z.y; aux->color *= 0.9; }
+		if (z.z > L) { z.z = V - z.z; aux->color *= 0.9; }
+		else if (z.z < -L) { z.z = -V - z.z; aux->color *= 0.9; }
+		aux->r = v4_len(z);
+	}
+	if (p->sphericalFolding)
+	{
+		const double fR2 = p->sphericalFoldingOuter * p->sphericalFoldingOuter;
+		const double mR2 = p->sphericalFoldingInner * p->sphericalFoldingInner;
+		const double r2 = aux->r * aux->r;
+		const double factor1 = fR2 / mR2;
+		if (r2 < mR2)
+		{
+			z = v4_mul(z, factor1);
+			aux->DE *= factor1;
+			aux->color *= 0.9;
+		}
+		else if (r2 < fR2)
+		{
+			const double factor2 = fR2 / r2;
+			z = v4_mul(z, factor2);
+			aux->DE *= factor2;
+			aux->color *= 0.9;
+		}
+		aux->r = v4_len(z);
+	}
+	*zp = z;
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* Compute<Mode> (mandelbulber2/src/compute_fractal.cpp:67-654): one container or a hybrid    */
 /* ------------------------------------------------------------------------------------------ */
@@ -1021,6 +1061,7 @@ static void compute(const scene_t *s, int mode, v3 point, int forcedMaxiter, int
 		lastGoodZ = lastZ;
 		lastZ = z;
 		const slot_t *f = &s->slot[seq_slot(s, i)];
+		global_foldings(s->p, &z, &aux);
 		const v4 tempZ = z;
 		const double tempAuxDE = aux.DE, tempAuxColor = aux.color;
 		aux.i = i;
@@ -1122,6 +1163,7 @@ static double compute_colour_index(const scene_t *s, v3 point)
 	{
 		lastZ = z;
 		const slot_t *f = &s->slot[seq_slot(s, i)];
+		global_foldings(p, &z, &aux);
 		const v4 tempZ = z;
 		const double tempAuxDE = aux.DE, tempAuxColor = aux.color;
 		aux.i = i;
@@ -1490,19 +1532,43 @@ typedef struct
 	double distThresh, delta, depth;
 } shin_t; /* sShaderInputData subset (render_worker.hpp:57-88) */
 
-/* CalculateNormals, fast mode (shader_calculate_normals.cpp:40-130) */
+/* CalculateNormals (shader_calculate_normals.cpp:40-130): central differences of the DE, or with
+ * slow_shading the iteration-count gradient over an 11 x 11 x 11 grid of orbits round the point
+ * (:81-109; Compute on the grid points themselves, without the node's transform) */
 static v3 calculate_normals(ray_ctx *rc, const shin_t *in)
 {
 	const mbx_params *p = rc->s->p;
-	double delta = in->distThresh * p->smoothness;
-	if (p->interiorMode) delta = in->distThresh * 0.2 * p->smoothness;
-	double sx1 = de(rc, v3_add(in->point, V3(delta, 0.0, 0.0)), in->distThresh, 1, NULL);
-	double sx2 = de(rc, v3_sub(in->point, V3(delta, 0.0, 0.0)), in->distThresh, 1, NULL);
-	double sy1 = de(rc, v3_add(in->point, V3(0.0, delta, 0.0)), in->distThresh, 1, NULL);
-	double sy2 = de(rc, v3_sub(in->point, V3(0.0, delta, 0.0)), in->distThresh, 1, NULL);
-	double sz1 = de(rc, v3_add(in->point, V3(0.0, 0.0, delta)), in->distThresh, 1, NULL);
-	double sz2 = de(rc, v3_sub(in->point, V3(0.0, 0.0, delta)), in->distThresh, 1, NULL);
-	v3 normal = V3(sx1 - sx2, sy1 - sy2, sz1 - sz2);
+	v3 normal;
+	if (!p->slowShading)
+	{
+		double delta = in->distThresh * p->smoothness;
+		if (p->interiorMode) delta = in->distThresh * 0.2 * p->smoothness;
+		double sx1 = de(rc, v3_add(in->point, V3(delta, 0.0, 0.0)), in->distThresh, 1, NULL);
+		double sx2 = de(rc, v3_sub(in->point, V3(delta, 0.0, 0.0)), in->distThresh, 1, NULL);
+		double sy1 = de(rc, v3_add(in->point, V3(0.0, delta, 0.0)), in->distThresh, 1, NULL);
+		double sy2 = de(rc, v3_sub(in->point, V3(0.0, delta, 0.0)), in->distThresh, 1, NULL);
+		double sz1 = de(rc, v3_add(in->point, V3(0.0, 0.0, delta)), in->distThresh, 1, NULL);
+		double sz2 = de(rc, v3_sub(in->point, V3(0.0, 0.0, delta)), in->distThresh, 1, NULL);
+		normal = V3(sx1 - sx2, sy1 - sy2, sz1 - sz2);
+	}
+	else
+	{
+		double delta = in->delta * p->smoothness;
+		if (p->interiorMode) delta = in->distThresh * 0.2 * p->smoothness;
+		normal = V3(0.0, 0.0, 0.0);
+		v3 q;
+		for (q.x = -1.0; q.x <= 1.0; q.x += 0.2)
+			for (q.y = -1.0; q.y <= 1.0; q.y += 0.2)
+				for (q.z = -1.0; q.z <= 1.0; q.z += 0.2)
+				{
+					const v3 point3 = v3_add(in->point, v3_mul(q, delta));
+					fout_t fo;
+					compute(rc->s, MODE_NORMAL, point3, -1, 1, &fo);
+					const double pseudoDistance = 1 + p->fractal.formulaMaxiter - fo.iters;
+					rc->iterations += (uint64_t)fo.iters;
+					normal = v3_add(normal, v3_mul(q, pseudoDistance));
+				}
+	}
 	if (fabs(normal.x) < 1e-50 && fabs(normal.y) < 1e-50 && fabs(normal.z) < 1e-50)
 		normal = V3(1.0, 0.0, 0.0);
 	else
@@ -2311,6 +2377,30 @@ int mbxo_formula_step_ex(const mbx_params *p, int slot, int n, double *z4, doubl
 	return MBX_OK;
 }
 
+/* The global foldings alone (BoxFolding then SphericalFolding as the params enable them) on z4
+ * and aux3 (r, DE, color), in place. */
+int mbxo_global_foldings(const mbx_params *p, int n, double *z4, double *aux3)
+{
+	for (int k = 0; k < n; k++)
+	{
+		v4 z = V4(z4[4 * k], z4[4 * k + 1], z4[4 * k + 2], z4[4 * k + 3]);
+		aux_t aux;
+		memset(&aux, 0, sizeof(aux));
+		aux.r = aux3[3 * k];
+		aux.DE = aux3[3 * k + 1];
+		aux.color = aux3[3 * k + 2];
+		global_foldings(p, &z, &aux);
+		z4[4 * k] = z.x;
+		z4[4 * k + 1] = z.y;
+		z4[4 * k + 2] = z.z;
+		z4[4 * k + 3] = z.w;
+		aux3[3 * k] = aux.r;
+		aux3[3 * k + 1] = aux.DE;
+		aux3[3 * k + 2] = aux.color;
+	}
+	return MBX_OK;
+}
+
 /* Camera/light derivation for host-code parity: mRot (9), fov, lightDirection (3). */
 int mbxo_frame_constants(const mbx_params *p, double *mrot9, double *fov, double *light3)
 {
@@ -2368,6 +2458,7 @@ int mbxo_normal(const mbx_params *p, const double *pt3, double distThresh, doubl
 	memset(&in, 0, sizeof(in));
 	in.point = V3(pt3[0], pt3[1], pt3[2]);
 	in.distThresh = distThresh;
+	in.delta = calc_delta(&s, in.point);
 	v3 n = calculate_normals(&rc, &in);
 	n3[0] = n.x;
 	n3[1] = n.y;

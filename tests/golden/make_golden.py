@@ -3,7 +3,8 @@
 Runs oracle/_ref/formula_ref, which `make -C oracle ref` compiles from
 mandelbulber2/formula/definition/fractal_{mandelbulb,mandelbox,kaleidoscopic_ifs,
 mandelbulb_power2,hypercomplex,quaternion,menger_sponge,xenodreambuie,box_fold_bulb_pow2,
-aexion}.cpp and mandelbulber2/src/common_math.cpp in the reference checkout (see
+aexion}.cpp, legacy_fractal_transforrms.cpp (the global foldings) and
+mandelbulber2/src/common_math.cpp in the reference checkout (see
 oracle/ref_harness.cpp).
 The fixture holds inputs and outputs only, every double as an exact hex string.
 
@@ -25,7 +26,7 @@ def main():
     text = subprocess.check_output([exe]).decode()
     records = {"mandelbulb": [], "mandelbox": [], "kifs": [], "mandelbulbpower2": [],
                "hypercomplex": [], "quaternion": [], "mengersponge": [], "xenodreambuie": [],
-               "boxfoldbulbpow2": [], "aexion": []}
+               "boxfoldbulbpow2": [], "aexion": [], "boxfolding": [], "sphericalfolding": []}
     rng = {}
     for line in text.splitlines():
         tag, _, rest = line.partition(" ")
@@ -51,6 +52,7 @@ def main():
         "xenodreambuie_params": ["power", "alphaAngleOffset_deg", "betaAngleOffset_deg"],
         "boxfoldbulbpow2_params": ["foldFactor", "zFactor", "analyticDE.enabledFalse",
                                    "analyticDE.scale1", "analyticDE.offset1", "analyticDE.offset2"],
+        "foldings_params": ["boxLimit", "boxValue", "sphericalOuter", "sphericalInner"],
         "aexion_params": ["cadd", "aux.i", "analyticDE.enabled", "analyticDE.scale1",
                           "analyticDE.offset1", "const_c.x", "const_c.y", "const_c.z", "const_c.w"],
         "records": records,

@@ -498,3 +498,19 @@ def test_readers_agree_on_every_example():
         with open(f, "rb") as fh:
             loaded += _agree(fh.read())
     assert loaded >= len(files) - 2     # the 1.x-format defaults file is rejected by both
+
+
+def test_global_foldings_and_slow_shading(load):
+    """folded mender sponge.fract: box_folding true, box_folding_limit 0,1, box_folding_value 0,2,
+    spherical_folding true (outer / inner radii at their defaults 1 and 0.5, initparameters.cpp:199-204);
+    aexion05.fract: slow_shading true with N 47."""
+    r = load(_read("folded_mender_sponge.fract"))
+    assert r["boxFolding"] == 1 and r["sphericalFolding"] == 1
+    assert r["boxFoldingLimit"] == 0.1 and r["boxFoldingValue"] == 0.2
+    assert r["sphericalFoldingOuter"] == 1.0 and r["sphericalFoldingInner"] == 0.5
+    assert r["slowShading"] == 0
+    r = load(_read("aexion05.fract"))
+    assert r["slowShading"] == 1 and r["fractal.formulaMaxiter"] == 47
+    assert r["boxFolding"] == 0 and r["sphericalFolding"] == 0
+    r = load(_hdr("2.33", "spherical_folding true;\nspherical_folding_outer 1,5;\nspherical_folding_inner 0.25;\n"))
+    assert r["sphericalFolding"] == 1 and r["sphericalFoldingOuter"] == 1.5 and r["sphericalFoldingInner"] == 0.25

@@ -37,9 +37,9 @@ def _check(st, trig):
     assert st["mask_agree"] >= 0.999, st
     assert st["iters_agree"] >= (0.995 if trig else 0.999), st
     if trig:
-        assert st.get("depth_within_1e-3", 1.0) >= 0.999, st
+        assert st["depth_within_1e-3"] >= 0.999, st
     else:
-        assert st.get("depth_rel_p999", 0.0) <= 1e-6, st
+        assert st["depth_rel_p999"] <= 1e-6, st
     assert st["rgb_within_1e-2"] >= 0.999, st
 
 
@@ -252,7 +252,7 @@ def test_advanced_quality_parity(rt, oracle_mod, kind, variant):
     _report("advanced quality %s %s" % (kind, variant), st)
     if kind == "bulb8":   # FP32 march: depth within 1e-3
         assert st["mask_agree"] >= 0.999 and st["iters_agree"] >= 0.999, st
-        assert st.get("depth_within_1e-3", 1.0) >= 0.999, st
+        assert st["depth_within_1e-3"] >= 0.999, st
         assert st["rgb_within_1e-2"] >= 0.999, st
     else:
         _check(st, False)

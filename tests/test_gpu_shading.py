@@ -33,9 +33,9 @@ def _check(st, depth_tol=1e-3, iters=0.999):
     assert st["mask_agree"] >= 0.999, st
     assert st["iters_agree"] >= iters, st
     if depth_tol == 1e-3:
-        assert st.get("depth_within_1e-3", 1.0) >= 0.999, st
+        assert st["depth_within_1e-3"] >= 0.999, st
     else:
-        assert st.get("depth_rel_p999", 0.0) <= depth_tol, st
+        assert st["depth_rel_p999"] <= depth_tol, st
     assert st["rgb_within_1e-2"] >= 0.999, st
 
 

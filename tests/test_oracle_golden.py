@@ -202,3 +202,28 @@ def test_aexion_formula_bitexact(oracle_mod, golden):
         np.testing.assert_array_equal(z, z_ref)
         np.testing.assert_array_equal(aux[:, 1], aux_ref[:, 1])
         np.testing.assert_array_equal(cc, np.stack([_hex(r["const_c_out"]) for r in rs]))
+
+
+@pytest.mark.parametrize("which", ["boxfolding", "sphericalfolding"])
+def test_global_foldings_bitexact(oracle_mod, golden, which):
+    """BoxFolding / SphericalFolding (legacy_fractal_transforrms.cpp:43-101) against the
+    reference's own functions: z, DE and colour bit for bit; the oracle also refreshes aux.r to
+    the 4D length of the folded z, as Compute does after each folding (compute_fractal.cpp:192-202)."""
+    for params, recs in _group(golden["records"][which]).items():
+        limit, value, outer, inner = _hex(params)
+        p = scene.mandelbulb8(8, 8)
+        p.boxFolding = int(which == "boxfolding")
+        p.sphericalFolding = int(which == "sphericalfolding")
+        p.boxFoldingLimit, p.boxFoldingValue = limit, value
+        p.sphericalFoldingOuter, p.sphericalFoldingInner = outer, inner
+        z_in = np.stack([_hex(r["z_in"]) for r in recs])
+        aux_in = np.stack([_hex(r["aux_in"]) for r in recs])
+        z_out = np.stack([_hex(r["z_out"]) for r in recs])
+        aux_out = np.stack([_hex(r["aux_out"]) for r in recs])
+        z, aux = oracle_mod.global_foldings(p, z_in, aux_in)
+        np.testing.assert_array_equal(z, z_out)
+        np.testing.assert_array_equal(aux[:, 1:], aux_out[:, 1:])
+        np.testing.assert_array_equal(aux[:, 0], np.sqrt((z_out * z_out).sum(1)))
+        # the records exercise every branch
+        folded = (z != z_in).any(1)
+        assert folded.any() and (~folded).any()

@@ -448,3 +448,33 @@ def test_lambert_light_closed_form(shading):
     shade = (1.0 - shading + shading * lam) * 1.7
     expect = shade[:, None] * np.array([0.9 * 0.6, 0.8 * 0.3, 0.5 * 0.2])[None, :]
     np.testing.assert_allclose(r["rgb"].reshape(-1, 3)[hit], expect, rtol=2e-6, atol=2e-7)
+
+
+# ------------------------------------------------------------------------------------------
+# slow-shading normals (shader_calculate_normals.cpp:81-109)
+# ------------------------------------------------------------------------------------------
+def test_slow_shading_normal_follows_the_iteration_gradient():
+    """A Mandelbox that never folds (folding limit 1e6, sphere-fold radii 1e-6) and adds no c
+    maps z -> 2 z, so the iteration count is the radial step function ceil(log2(100 / |p|)): the
+    slow-shading normal, sum q (1 + N - iterations(p + q delta)) over the 11^3 grid, points
+    radially outwards (fewer iterations) at points on a step's sphere |p| = 100 / 2^7; without
+    slow shading the same points give the DE's central-difference normal, also radial."""
+    p = scene.mandelbox_shadow_ao(64, 36, seed=1)
+    f = p.fractal
+    f.deltaDEMethod = 2                      # analytic DE: r / DE, radial as well
+    f.dontAddCConstant = 1
+    f.mandelbox.foldingLimit = 1e6
+    f.mandelbox.foldingSphericalMin = 1e-6
+    f.mandelbox.foldingSphericalFixed = 1e-6
+    abi.set3(p.camera, (0.0, -2.5, 0.0))
+    rs = np.random.default_rng(9)
+    dirs = rs.normal(size=(24, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    R = 100.0 / 2.0 ** 7
+    for slow in (1, 0):
+        p.slowShading = slow
+        cos = [float(np.dot(oracle.normal(p, R * d, 0.01), d)) for d in dirs]
+        assert min(cos) > 0.97, (slow, min(cos))
+    # iterations step across the sphere (the grid straddles it): inside 7 + 1, outside 7
+    _, it = oracle.distance(p, np.array([0.99 * R * dirs[0], 1.01 * R * dirs[0]]), 0.01)
+    assert list(it) == [8, 7]
