@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MBX_ABI_VERSION 6
+#define MBX_ABI_VERSION 7
 
 /* array sizes as in mandelbulber2/src/fractal.h:52-54 */
 #define MBX_IFS_VECTOR_COUNT 9
@@ -349,7 +349,9 @@ typedef struct mbx_params {
 	 * (CalculateNormals, shader_calculate_normals.cpp:81-109) instead of the DE's central
 	 * differences */
 	int32_t slowShading;            /* "slow_shading" */
-	int32_t reserved4;
+	/* params->N (fractparams.cpp:189; "N", default 250): the 1 + N - iterations pseudo-distance
+	 * of slow shading (shader_calculate_normals.cpp:103), apart from the node's maxiter */
+	int32_t N;
 } mbx_params;
 
 /* Per-pixel output planes (device pointers; any may be NULL except where noted).
@@ -553,13 +555,15 @@ void mbx_params_default(mbx_params *p);
 #define MBX_FRACT_POSTPROCESS 0x1000u  /* brightness, contrast, gamma, saturation, HDR, blur, post DOF */
 #define MBX_FRACT_MC_ADAPTIVE 0x2000u  /* Monte Carlo stops early (DOF_min_samples < DOF_samples) */
 #define MBX_FRACT_PERSPECTIVE 0x4000u  /* fish-eye or equirectangular projection, stereo */
-#define MBX_FRACT_FORMULA 0x8000u      /* a formula outside Mandelbulb, Mandelbox, Kaleidoscopic IFS */
+#define MBX_FRACT_FORMULA 0x8000u      /* a formula outside the device's table (the MBX_FORMULA_* ids) */
 #define MBX_FRACT_ANIMATION 0x10000u   /* [frames] / [keyframes] sections (not read) */
 #define MBX_FRACT_MATERIAL 0x20000u    /* other material effects: iridescence, roughness, subsurface,
                                           perlin noise, luminosity gradient */
 #define MBX_FRACT_INTERIOR 0x40000u    /* interior mode or iteration-threshold mode */
 #define MBX_FRACT_DISABLED 0x80000u    /* the fractal node is disabled or absent: nothing to render */
 #define MBX_FRACT_ANTIALIASING 0x100000u /* antialiasing repeats (antialiasing_size^2 per pixel) */
+#define MBX_FRACT_MC_EFFECTS 0x200000u /* Monte Carlo soft shadows or chromatic aberration */
+#define MBX_FRACT_NEBULA 0x400000u     /* nebula mode (the OpenCL nebula engine renders the scene) */
 
 /* What mbx_load_fract found besides the parameter block. */
 typedef struct mbx_fract_info {

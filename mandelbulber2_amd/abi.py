@@ -249,7 +249,7 @@ class MbxParams(C.Structure):
         ("sphericalFoldingOuter", C.c_double),
         ("sphericalFoldingInner", C.c_double),
         ("slowShading", C.c_int32),
-        ("reserved4", C.c_int32),
+        ("N", C.c_int32),
     ]
 
 
@@ -290,7 +290,8 @@ FRACT_FLAGS = {
     "glow": 0x20, "fog": 0x40, "reflections": 0x80, "lights": 0x100, "textures": 0x200,
     "ao_screen_space": 0x400, "ao_lightmap": 0x800, "postprocess": 0x1000, "mc_adaptive": 0x2000,
     "perspective": 0x4000, "formula": 0x8000, "animation": 0x10000, "material": 0x20000,
-    "interior": 0x40000, "disabled": 0x80000, "antialiasing": 0x100000,
+    "interior": 0x40000, "disabled": 0x80000, "antialiasing": 0x100000, "mc_effects": 0x200000,
+    "nebula": 0x400000,
 }
 
 

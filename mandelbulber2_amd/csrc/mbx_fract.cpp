@@ -633,6 +633,7 @@ extern "C" void mbx_params_default(mbx_params *p)
 	p->sphericalFoldingOuter = 1.0;
 	p->sphericalFoldingInner = 0.5;
 	p->slowShading = 0;
+	p->N = 250;
 	// palette colouring of material #1 (InitMaterialParams)
 	p->useColorsFromPalette = 1;
 	p->surfaceGradientEnable = 1;
@@ -1031,6 +1032,13 @@ extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, 
 	p.DOFMonteCarloGlobalIllumination = M.b("DOF_MC_global_illumination", false);
 	p.monteCarloGIRadianceLimit = M.d("MC_GI_radiance_limit", 10.0, 0.001, 1e10);
 	if (p.DOFMonteCarlo && M.i("DOF_min_samples", 10) < p.DOFSamples) flags |= MBX_FRACT_MC_ADAPTIVE;
+	// Monte Carlo soft shadows jitter the light vector (shader_aux_shadow.cpp:76-87); chromatic
+	// aberration shifts the view vector by a random hue (render_worker.cpp:255-264)
+	if (p.DOFMonteCarlo && (M.b("MC_soft_shadows_enable", false) || M.b("DOF_MC_CA_enable", false)))
+		flags |= MBX_FRACT_MC_EFFECTS;
+	// the OpenCL nebula engine renders these scenes (render_job.cpp:664-673)
+	if (M.b("nebula_mode", false)) flags |= MBX_FRACT_NEBULA;
+	p.N = (int32_t)M.i("N", 250, 1, 99999); // params->N: slow shading's pseudo-distance (fractparams.cpp:189)
 
 	// material of the node (InitMaterialParams :1600-1810, material.cpp:330-349); a file that names
 	// no material at all gets material 1 with its defaults (CheckIfMaterialsAreDefined)
@@ -1116,6 +1124,10 @@ extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, 
 				if (ver <= 2.12) boost = 100.0; // settings.cpp:1547-1551
 				L.intensity /= boost;
 			}
+			// cLight (light.cpp:77-87): intensity, visibility and size scale by the all_lights_ factors
+			L.intensity *= M.d("all_lights_intensity", 1.0, 0.0, 100000.0);
+			L.visibility *= M.d("all_lights_visibility", 1.0, 0.0, 100000.0);
+			L.size *= M.d("all_lights_size", 1.0, 0.0, 100000.0);
 			const long long type = M.e("light1_type", 0, {"directional", "point", "spot", "projection", "beam"});
 			if (L.enabled && (type != 0 || M.b("light1_volumetric", false))) flags |= MBX_FRACT_LIGHTS;
 		}

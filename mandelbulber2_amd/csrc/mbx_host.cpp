@@ -1117,7 +1117,7 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc, bool forceGe
 	fc.sphMR2D = p->sphericalFoldingInner * p->sphericalFoldingInner;
 	fc.sphFactor1D = fc.sphFR2D / fc.sphMR2D;
 	fc.slowShading = p->slowShading ? 1 : 0;
-	fc.slowShadingN = fr.formulaMaxiter;
+	fc.slowShadingN = p->N;
 
 	// fractal node rotation (objects_tree.cpp:292-382): local-to-world is R^T with
 	// R = SetRotation2(rotation * pi/180), so world-to-local (calculate_distance.cpp:296) is R

@@ -86,10 +86,14 @@ class Param:
 
 
 class Container(dict):
+    """A parameter container; `read` collects the names the reader looked up, which the
+    key-coverage test compares with the keys a scene sets."""
+
     def add(self, name, *spec, **kw):
         self[name] = Param(*spec, **kw)
 
     def get(self, name, default=None):
+        self.__dict__.setdefault("read", set()).add(name)
         p = dict.get(self, name)
         return default if p is None else p.value
 
@@ -186,6 +190,9 @@ def init_params():
     c.add("hdr_blur_enabled", BOOL, False)                          # :299
     c.add("post_chromatic_aberration_enabled", BOOL, False)         # :303
     c.add("fill_light_color", RGB, (0, 0, 0))                       # :309
+    c.add("all_lights_intensity", DOUBLE, 1.0, 0.0, 100000.0)       # :310
+    c.add("all_lights_visibility", DOUBLE, 1.0, 0.0, 100000.0)      # :311
+    c.add("all_lights_size", DOUBLE, 1.0, 0.0, 100000.0)            # :312
     c.add("basic_fog_enabled", BOOL, False)                         # :315
     c.add("DOF_enabled", BOOL, False)                               # :322
     c.add("DOF_monte_carlo", BOOL, False)                           # :329
@@ -193,10 +200,13 @@ def init_params():
     c.add("DOF_min_samples", INT, 10)                               # :331
     c.add("DOF_MC_global_illumination", BOOL, False)                # :333
     c.add("MC_global_illumination_volumetric", BOOL, False)         # :334
+    c.add("DOF_MC_CA_enable", BOOL, False)                          # :336
+    c.add("MC_soft_shadows_enable", BOOL, False)                    # :339
     c.add("MC_GI_radiance_limit", DOUBLE, 10.0, 0.001, 1e10)        # :340
     c.add("volumetric_light_DE_Factor", DOUBLE, 1.0, 1e-15, 1e15)   # :350
     c.add("random_lights_group", BOOL, False)                       # :360
     c.add("fake_lights_enabled", BOOL, False)                       # :374
+    c.add("nebula_mode", BOOL, False)                               # :404
     c.add("fractal_position", VEC3, (0.0, 0.0, 0.0))                # :400
     c.add("fractal_rotation", VEC3, (0.0, 0.0, 0.0))                # :401
     c.add("repeat", VEC3, (0.0, 0.0, 0.0))                          # :402
@@ -474,16 +484,41 @@ FLAGS = {
     "glow": 0x20, "fog": 0x40, "reflections": 0x80, "lights": 0x100, "textures": 0x200,
     "ao_screen_space": 0x400, "ao_lightmap": 0x800, "postprocess": 0x1000, "mc_adaptive": 0x2000,
     "perspective": 0x4000, "formula": 0x8000, "animation": 0x10000, "material": 0x20000,
-    "interior": 0x40000, "disabled": 0x80000, "antialiasing": 0x100000,
+    "interior": 0x40000, "disabled": 0x80000, "antialiasing": 0x100000, "mc_effects": 0x200000,
+    "nebula": 0x400000,
 }
 
 _LEGACY_MAIN_INDEXED = ("julia_mode", "julia_c", "fractal_constant_factor", "formula_maxiter",
                         "initial_waxis")
 
 
-def load(text):
+class _LegacyMain(dict):
+    """The name_N temporaries of a pre-2.35 file, recording which ones the migration reads."""
+
+    def __init__(self):
+        super().__init__()
+        self.read = set()
+
+    def get(self, name, default=None):
+        self.read.add(name)
+        return dict.get(self, name, default)
+
+    def __contains__(self, name):
+        self.read.add(name)
+        return dict.__contains__(self, name)
+
+
+def load(text, trace=None):
     """Returns (fields, info): fields maps mbx_params field paths to values, info the header and
-    the effect flags.  Raises FractError where mbx_load_fract returns MBX_ERR_INVALID."""
+    the effect flags.  Raises FractError where mbx_load_fract returns MBX_ERR_INVALID.
+
+    With a dict as `trace`, also records where every parameter line went: trace["lines"] holds
+    (section, name as written, target) with target one of ("main", name), ("fractal", index,
+    name), ("legacy", name), ("skip", name) or ("dropped", name) for a name no container
+    registers; trace["containers"] holds the main container, the nine fractal containers and
+    the legacy temporaries, whose `read` sets name what the reader looked up; trace["node"] is
+    the fractal node the scene resolved to and trace["boolean"] whether the file is a pre-2.35
+    boolean scene."""
     lines = text.strip().replace("\r\n", "\n").replace("\r", "\n").split("\n")
     # DecodeHeader (settings.cpp:540-601)
     if len(lines) < 3:
@@ -510,6 +545,8 @@ def load(text):
 
     par = init_params()
     fract = [init_fractal(i) for i in range(9)]
+    if trace is not None:
+        trace["lines"] = []
     if ver < 2.35:
         # temporaries the reference injects for pre-2.35 files (settings.cpp:2128-2214)
         par.add("boolean_operators", BOOL, False)
@@ -520,7 +557,7 @@ def load(text):
             f.add("formula_repeat", VEC3, (0.0, 0.0, 0.0))
             f.add("formula_scale", DOUBLE, 1.0)
     materials, lights, nodes = set(), set(), set()
-    legacy_main = {}        # name_N temporaries of settings.cpp:2128-2172 that a file set
+    legacy_main = _LegacyMain()   # name_N temporaries of settings.cpp:2128-2172 that a file set
     primitives = animation = False
     n_lines = n_errors = 0
 
@@ -534,8 +571,12 @@ def load(text):
     for nid in (sorted(nodes) or [1]):
         init_node(par, nid)
 
+    def note(target):
+        if trace is not None:
+            trace["lines"].append((section, raw_name, target))
+
     def decode(container, line, main):
-        nonlocal n_lines, primitives
+        nonlocal n_lines, primitives, raw_name
         line = line.strip()
         if not line:
             return True
@@ -550,6 +591,7 @@ def load(text):
             if len(value) >= 2 and value[0] == q and value[-1] == q:
                 value = value[1:-1].strip()
         n_lines += 1
+        raw_name = name
         if main and ver < 2.35:
             # TryResolveLegacyFractalParam (settings.cpp:2268-2322)
             us = name.rfind("_")
@@ -561,14 +603,19 @@ def load(text):
                     temp = base in _LEGACY_MAIN_INDEXED or (base == "dont_add_c_constant" and k <= 8)
                     if temp:
                         legacy_main[name] = value
+                        note(("legacy", name))
                         return True
                     if base != "formula_maxiter" and k <= 9 and base in fract[k - 1]:
                         n2, v2 = compatibility(ver, base, value)
                         if n2 != "skip" and n2 in fract[k - 1]:
                             fract[k - 1][n2].set_text(v2)
+                            note(("fractal", k - 1, n2))
+                        else:
+                            note(("skip", n2))
                         return True
         name, value = compatibility(ver, name, value)
         if name == "skip":
+            note(("skip", raw_name))
             return True
         if main:
             if name.startswith("primitive_"):
@@ -590,9 +637,14 @@ def load(text):
                     lights.add(int(idx))
         if name in container:
             container[name].set_text(value)
+            note(("main", name) if main else
+                 ("fractal", [id(f) for f in fract].index(id(container)), name))
+        else:
+            note(("dropped", name))
         return True
 
     section = ""
+    raw_name = ""
     for l in range(3, len(lines)):
         line = lines[l]
         if len(line) >= 2 and line[0] == "[" and line[-1] == "]":
@@ -828,6 +880,13 @@ def load(text):
     out["monteCarloGIRadianceLimit"] = g("MC_GI_radiance_limit")
     if g("DOF_monte_carlo") and g("DOF_min_samples") < g("DOF_samples"):
         flags |= FLAGS["mc_adaptive"]
+    # Monte Carlo soft shadows jitter the light vector (shader_aux_shadow.cpp:76-87);
+    # chromatic aberration shifts the view vector by a random hue (render_worker.cpp:255-264)
+    if g("DOF_monte_carlo") and (g("MC_soft_shadows_enable") or g("DOF_MC_CA_enable")):
+        flags |= FLAGS["mc_effects"]
+    if g("nebula_mode"):                    # the OpenCL nebula engine renders these (render_job.cpp:664-673)
+        flags |= FLAGS["nebula"]
+    out["N"] = g("N")                       # params->N: slow shading's pseudo-distance (fractparams.cpp:189)
 
     m = node["material"]
     mp = "mat%d_" % m
@@ -883,11 +942,14 @@ def load(text):
 
     L = {}
     if 1 in lights:
+        # cLight (light.cpp:77-87): intensity, visibility and size scale by the all_lights_ factors
         L = dict(enabled=int(g("light1_enabled")), castShadows=int(g("light1_cast_shadows")),
                  penetrating=int(g("light1_penetrating")),
                  relativePosition=int(g("light1_relative_position")), rotation=g("light1_rotation"),
-                 intensity=g("light1_intensity"), visibility=g("light1_visibility"),
-                 size=g("light1_size"), softShadowCone=g("light1_soft_shadow_cone"),
+                 intensity=g("light1_intensity") * g("all_lights_intensity"),
+                 visibility=g("light1_visibility") * g("all_lights_visibility"),
+                 size=g("light1_size") * g("all_lights_size"),
+                 softShadowCone=g("light1_soft_shadow_cone"),
                  contourSharpness=g("light1_contour_sharpness"), color=_rgbf(g("light1_color")))
         if g("light1_enabled") and (g("light1_type") != 0 or g("light1_volumetric")):
             flags |= FLAGS["lights"]
@@ -1006,6 +1068,9 @@ def load(text):
     out["formulaRotation"] = node["rotation"]
     info = dict(fileVersion=ver, format=fmt, lines=n_lines, errors=n_errors, formula=fo,
                 unsupported=flags)
+    if trace is not None:
+        trace.update(containers=(par, fract, legacy_main), node=node,
+                     boolean=ver < 2.35 and bool(par.get("boolean_operators")))
     return out, info
 
 

@@ -1564,7 +1564,7 @@ static v3 calculate_normals(ray_ctx *rc, const shin_t *in)
 					const v3 point3 = v3_add(in->point, v3_mul(q, delta));
 					fout_t fo;
 					compute(rc->s, MODE_NORMAL, point3, -1, 1, &fo);
-					const double pseudoDistance = 1 + p->fractal.formulaMaxiter - fo.iters;
+					const double pseudoDistance = 1 + p->N - fo.iters;
 					rc->iterations += (uint64_t)fo.iters;
 					normal = v3_add(normal, v3_mul(q, pseudoDistance));
 				}

@@ -514,3 +514,32 @@ def test_global_foldings_and_slow_shading(load):
     assert r["boxFolding"] == 0 and r["sphericalFolding"] == 0
     r = load(_hdr("2.33", "spherical_folding true;\nspherical_folding_outer 1,5;\nspherical_folding_inner 0.25;\n"))
     assert r["sphericalFolding"] == 1 and r["sphericalFoldingOuter"] == 1.5 and r["sphericalFoldingInner"] == 0.25
+
+
+def test_all_lights_factors_and_main_n(load):
+    """cLight (light.cpp:77-87) scales light 1's intensity, visibility and size by the
+    all_lights_ factors; the main section's N (initparameters.cpp:130, default 250) is kept apart
+    from the node's maxiter for slow shading (shader_calculate_normals.cpp:103)."""
+    body = ("light1_is_defined true;\nlight1_intensity 1,5;\nlight1_visibility 2;\nlight1_size 0,25;\n"
+            "all_lights_intensity 2;\nall_lights_visibility 0,5;\nall_lights_size 4;\nN 47;\n")
+    r = load(_hdr("2.33", body))
+    assert r["light.intensity"] == 3.0 and r["light.visibility"] == 1.0 and r["light.size"] == 1.0
+    assert r["N"] == 47 and r["fractal.formulaMaxiter"] == 47      # pre-2.35: N is the maxiter
+    r = load(_hdr("2.35", body))
+    assert r["N"] == 47 and r["fractal.formulaMaxiter"] == 250     # 2.35: the node's own maxiter
+    r = load(_hdr("2.33", "light1_is_defined true;\n"))
+    assert r["light.intensity"] == 1.0 and r["N"] == 250
+    r = load(_hdr("2.27", "iteration_fog_enable true;\niteration_fog_brightness_boost 4;\n"
+                          "light1_is_defined true;\nlight1_intensity 2;\nall_lights_intensity 3;\n"))
+    assert r["light.intensity"] == 1.5       # (2 / 4) * 3: settings.cpp:1700-1712, then cLight
+
+
+@pytest.mark.parametrize("body,flag", [
+    ("DOF_monte_carlo true;\nMC_soft_shadows_enable true;\n", "mc_effects"),
+    ("DOF_monte_carlo true;\nDOF_MC_CA_enable true;\n", "mc_effects"),
+    ("MC_soft_shadows_enable true;\n", None),        # without Monte Carlo the switch is inert
+    ("nebula_mode true;\n", "nebula"),
+])
+def test_monte_carlo_effect_and_nebula_flags(load, body, flag):
+    got = load(_hdr("2.33", body)).flags()
+    assert ({"mc_effects", "nebula"} & got) == ({flag} if flag else set())
