@@ -79,6 +79,22 @@ enum mbx_analytic_function {
 	MBX_ANALYTIC_IFS = 3
 };
 
+/* enumCPixelAddition (fractal_enums.h:62-68): whether the formula adds c by default */
+enum mbx_cpixel_addition {
+	MBX_CPIXEL_ENABLED_BY_DEFAULT = 0,
+	MBX_CPIXEL_DISABLED_BY_DEFAULT = 1,
+	MBX_CPIXEL_ALREADY_HAS = 2
+};
+
+/* enumColoringFunction (fractal_enums.h:90-99): the colour index of a single formula */
+enum mbx_coloring_function {
+	MBX_COLORING_DEFAULT = 0,
+	MBX_COLORING_ABOX = 1,
+	MBX_COLORING_IFS = 2,
+	MBX_COLORING_AMAZING_SURF = 3,
+	MBX_COLORING_DONUT = 4
+};
+
 /* enumAOMode (mandelbulber2/src/ao_modes.h:42-44); screen-space AO is a post effect and is not
  * rendered by this library */
 enum mbx_ao_mode { MBX_AO_FAST = 0, MBX_AO_MULTIPLE_RAYS = 1, MBX_AO_SCREEN_SPACE = 2 };
@@ -423,6 +439,66 @@ int mbx_render(mbx_ctx *ctx, const mbx_outputs *out);
  * cRenderWorker-free CalculateDistance (calculate_distance.cpp:60-64). */
 int mbx_distance(mbx_ctx *ctx, const double *points, int n, double detail_size,
 	float *dist_out, int32_t *iters_out);
+
+/* ---- formula plugins --------------------------------------------------------------------
+ * The reference's formula contract is cAbstractFractal (formula/definition/abstract_fractal.h:
+ * 74-109): FormulaCode(CVector4 &z, const sFractal *fractal, sExtendedAux &aux) plus metadata
+ * each formula's constructor sets.  The device holds one <Name>Iteration(z, fractal, aux) per
+ * formula of its table (csrc/mbx_generic.h); these calls expose the table's metadata and run one
+ * iteration of any of its formulas. */
+
+/* The metadata of one formula: cAbstractFractal's fields (abstract_fractal.h:85-93) as the
+ * formula's constructor sets them. */
+typedef struct mbx_formula_info {
+	int32_t internalID;         /* enumFractalFormula (MBX_FORMULA_*) */
+	int32_t DEType;             /* mbx_de_type */
+	int32_t DEFunctionType;     /* mbx_de_function */
+	int32_t cpixelAddition;     /* mbx_cpixel_addition */
+	double defaultBailout;
+	int32_t DEAnalyticFunction; /* mbx_analytic_function */
+	int32_t coloringFunction;   /* mbx_coloring_function */
+	char internalName[64];      /* internalName, e.g. "mandelbox" */
+	char nameInComboBox[64];    /* nameInComboBox, e.g. "Mandelbox" */
+} mbx_formula_info;
+
+/* Number of formulas the device renders (the rows of mbx_formula_get).  Host only.  The
+ * reference's counterpart is newFractalList (formula/definition/all_fractal_list.cpp:39). */
+int mbx_formula_count(void);
+
+/* Metadata of formula `index` (0 <= index < mbx_formula_count()), replacing the getters of
+ * cAbstractFractal (abstract_fractal.h:96-104).  Host only.  MBX_ERR_INVALID for an index out
+ * of range or a NULL info. */
+int mbx_formula_get(int index, mbx_formula_info *info);
+
+/* sExtendedAux (src/fractal.h:77-104), field for field. */
+typedef struct mbx_extended_aux {
+	double c[4];       /* variable c */
+	double const_c[4]; /* constant c */
+	double old_z[4];   /* hybrid colour 2 */
+	double pos_neg;
+	double r;
+	double DE;
+	double DE0;
+	double dist;
+	double pseudoKleinianDE;
+	double actualScale;
+	double actualScaleA;
+	double color;
+	double colorHybrid;
+	double temp1000;
+	int32_t i;
+	int32_t reserved;
+} mbx_extended_aux;
+
+/* One FormulaCode call (abstract_fractal.h:82) of fractal->formula per point, on the device of
+ * ctx: z (4 doubles per point) and aux as given, the formula's parameters from `fractal` after
+ * RecalculateFractalParams (fractal.cpp:653-678); the updated z and aux are written to z_out and
+ * aux_out (host memory, n entries; they may alias the inputs).  The formula-level known-answer
+ * hook: the reference's own formula sources give the expected outputs.  Weight, c addition and
+ * bailout belong to the sequence, not to the formula, and are not applied.  MBX_ERR_UNSUPPORTED
+ * for a formula outside the table; needs no uploaded scene. */
+int mbx_formula_iterate(mbx_ctx *ctx, const mbx_fractal *fractal, int n, const double *z_in,
+	const mbx_extended_aux *aux_in, double *z_out, mbx_extended_aux *aux_out);
 
 /* Render n frames (each its own parameter block and output planes) in one launch.  By default
  * that launch is the grid kernel (one 64-lane workgroup per 64-pixel tile of every frame, launched

@@ -308,6 +308,53 @@ class MbxFractInfo(C.Structure):
     ]
 
 
+class MbxFormulaInfo(C.Structure):
+    """mbx_formula_info: cAbstractFractal's metadata (abstract_fractal.h:85-93)."""
+    _fields_ = [
+        ("internalID", C.c_int32),
+        ("DEType", C.c_int32),
+        ("DEFunctionType", C.c_int32),
+        ("cpixelAddition", C.c_int32),
+        ("defaultBailout", C.c_double),
+        ("DEAnalyticFunction", C.c_int32),
+        ("coloringFunction", C.c_int32),
+        ("internalName", C.c_char * 64),
+        ("nameInComboBox", C.c_char * 64),
+    ]
+
+
+class MbxExtendedAux(C.Structure):
+    """mbx_extended_aux: sExtendedAux (fractal.h:77-104), field for field."""
+    _fields_ = [
+        ("c", C.c_double * 4),
+        ("const_c", C.c_double * 4),
+        ("old_z", C.c_double * 4),
+        ("pos_neg", C.c_double),
+        ("r", C.c_double),
+        ("DE", C.c_double),
+        ("DE0", C.c_double),
+        ("dist", C.c_double),
+        ("pseudoKleinianDE", C.c_double),
+        ("actualScale", C.c_double),
+        ("actualScaleA", C.c_double),
+        ("color", C.c_double),
+        ("colorHybrid", C.c_double),
+        ("temp1000", C.c_double),
+        ("i", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+MBX_CPIXEL_ENABLED_BY_DEFAULT = 0
+MBX_CPIXEL_DISABLED_BY_DEFAULT = 1
+MBX_CPIXEL_ALREADY_HAS = 2
+MBX_COLORING_DEFAULT = 0
+MBX_COLORING_ABOX = 1
+MBX_COLORING_IFS = 2
+MBX_COLORING_AMAZING_SURF = 3
+MBX_COLORING_DONUT = 4
+
+
 def fract_flag_names(mask):
     return [k for k, v in FRACT_FLAGS.items() if mask & v]
 

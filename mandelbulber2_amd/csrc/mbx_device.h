@@ -800,7 +800,7 @@ __device__ __forceinline__ int orbit_iterations(typename Prec<PosPrecision<KIND>
 		(void)compute_bulb_d(p, f, iters);
 	else if constexpr (KIND == kGeneric)
 	{
-		GenAux a;
+		ExtendedAux a;
 		DV4 z;
 		iters = gen_orbit(f, p, f.formulaMaxiter, z, a);
 	}

@@ -28,11 +28,12 @@ enum DEKind : int32_t
 	kNumDEKinds = 7
 };
 
-// One container of the general formula path (kGeneric): cHybridFractalSequences::sFractalData
+// One container of the general formula path (kGeneric), the `const sFractal *fractal` that each
+// formula's <Name>Iteration receives: cHybridFractalSequences::sFractalData
 // (hybrid_fractal_sequences.h:30-56) with its sFractal after RecalculateFractalParams
 // (fractal.cpp:653-678), in FP64.  An identity rotation is flagged: the reference skips the
 // multiply for it (CRotationMatrix's zero flag).
-struct GenSlot
+struct FractalParams
 {
 	int32_t formula, addC, checkBailout, ifsMainRotId;
 	int32_t mbMainRotOn, mbRotOn; // Mandelbox main rotation, rotated folds
@@ -215,7 +216,7 @@ struct FrameConst
 	int32_t slowShading;
 	int32_t slowShadingN; // params->N of the pseudo-distance 1 + N - iterations
 	uint8_t genSeq[kGenSeqLength];
-	GenSlot gen[MBX_MAX_FRACTALS];
+	FractalParams gen[MBX_MAX_FRACTALS];
 };
 
 // Per-pixel seed of the Park-Miller stream (identical on host and device).

@@ -25,13 +25,48 @@ __device__ __forceinline__ double dot4(DV4 a, DV4 b)
 __device__ __forceinline__ double len4(DV4 a) { return sqrt(dot4(a, a)); }
 __device__ __forceinline__ bool finite4(DV4 a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z) && isfinite(a.w); }
 
-// sExtendedAux members the formulas of the table use (fractal.h:77-104)
-struct GenAux
+// sExtendedAux (fractal.h:77-104) with the reference's field names: the state one formula
+// iteration reads and updates.  Fields that no formula of the table reads cost nothing: the
+// compiler drops the stores to them.
+struct ExtendedAux
 {
-	DV4 c; // const_c
-	double r, DE, color;
-	int i;
+	DV4 c;       // variable c
+	DV4 const_c; // constant c
+	DV4 old_z;   // hybrid colour 2
+	double pos_neg;
+	double r;    // |z|, refreshed after every iteration
+	double DE;   // running derivative
+	double DE0;
+	double dist;
+	double pseudoKleinianDE;
+	double actualScale; // Mandelbox vary scale
+	double actualScaleA;
+	double color;
+	double colorHybrid;
+	double temp1000;
+	int i;       // iteration index
 };
+
+// Compute's initial aux for the orbit of z (compute_fractal.cpp:107-132); actualScale is the
+// Mandelbox scale of the sequence's first container
+__device__ __forceinline__ void init_aux(ExtendedAux &aux, DV4 z, double actualScale)
+{
+	aux.c = z;
+	aux.const_c = z;
+	aux.old_z = z;
+	aux.pos_neg = 1.0;
+	aux.r = len4(z);
+	aux.DE = 1.0;
+	aux.DE0 = 0.0;
+	aux.dist = 1000.0;
+	aux.pseudoKleinianDE = 1.0;
+	aux.actualScale = actualScale;
+	aux.actualScaleA = 0.0;
+	aux.color = 1.0;
+	aux.colorHybrid = 0.0;
+	aux.temp1000 = 1000.0;
+	aux.i = 0;
+}
 
 // 3x3 rotation of xyz, w kept; identity matrices skip the multiply (algebra.cpp:320-344)
 __device__ __forceinline__ DV4 rot4(const double *m, int identity, DV4 v)
@@ -45,27 +80,28 @@ __device__ __forceinline__ DV4 rot4(const double *m, int identity, DV4 v)
 // ---- formulas -------------------------------------------------------------------------------
 
 // cFractalMandelbulb::FormulaCode (fractal_mandelbulb.cpp:29-43)
-__device__ __forceinline__ void gen_mandelbulb(const GenSlot &s, DV4 &z, GenAux &a)
+__device__ __forceinline__ void MandelbulbIteration(DV4 &z, const FractalParams *fractal, ExtendedAux &aux)
 {
 #pragma clang fp contract(off)
-	const double th0 = asin(z.z / a.r) + s.beta;
-	const double ph0 = atan2(z.y, z.x) + s.alpha;
-	double rp = pow(a.r, s.power - 1.0);
-	const double th = th0 * s.power;
-	const double ph = ph0 * s.power;
+	const double th0 = asin(z.z / aux.r) + fractal->beta;
+	const double ph0 = atan2(z.y, z.x) + fractal->alpha;
+	double rp = pow(aux.r, fractal->power - 1.0);
+	const double th = th0 * fractal->power;
+	const double ph = ph0 * fractal->power;
 	const double cth = cos(th);
-	a.DE = (rp * a.DE) * s.power + 1.0;
-	rp *= a.r;
+	aux.DE = (rp * aux.DE) * fractal->power + 1.0;
+	rp *= aux.r;
 	z.x = cth * cos(ph) * rp;
 	z.y = cth * sin(ph) * rp;
 	z.z = sin(th) * rp;
 }
 
 // cFractalMandelbulbPower2::FormulaCode (fractal_mandelbulb_power2.cpp:29-44)
-__device__ __forceinline__ void gen_mandelbulb_power2(DV4 &z, GenAux &a)
+__device__ __forceinline__ void MandelbulbPower2Iteration(DV4 &z, const FractalParams *fractal, ExtendedAux &aux)
 {
 #pragma clang fp contract(off)
-	a.DE = a.DE * 2.0 * a.r;
+	(void)fractal; // no parameters
+	aux.DE = aux.DE * 2.0 * aux.r;
 	const double x2 = z.x * z.x, y2 = z.y * z.y, z2 = z.z * z.z;
 	const double temp = 1.0 - z2 / (x2 + y2);
 	const double newx = (x2 - y2) * temp;
@@ -77,10 +113,11 @@ __device__ __forceinline__ void gen_mandelbulb_power2(DV4 &z, GenAux &a)
 }
 
 // cFractalHypercomplex::FormulaCode (fractal_hypercomplex.cpp:31-44)
-__device__ __forceinline__ void gen_hypercomplex(DV4 &z, GenAux &a)
+__device__ __forceinline__ void HypercomplexIteration(DV4 &z, const FractalParams *fractal, ExtendedAux &aux)
 {
 #pragma clang fp contract(off)
-	a.DE = a.DE * 2.0 * a.r;
+	(void)fractal; // no parameters
+	aux.DE = aux.DE * 2.0 * aux.r;
 	const double newx = z.x * z.x - z.y * z.y - z.z * z.z - z.w * z.w;
 	const double newy = 2.0 * z.x * z.y - 2.0 * z.w * z.z;
 	const double newz = 2.0 * z.x * z.z - 2.0 * z.y * z.w;
@@ -89,10 +126,11 @@ __device__ __forceinline__ void gen_hypercomplex(DV4 &z, GenAux &a)
 }
 
 // cFractalQuaternion::FormulaCode (fractal_quaternion.cpp:29-40)
-__device__ __forceinline__ void gen_quaternion(DV4 &z, GenAux &a)
+__device__ __forceinline__ void QuaternionIteration(DV4 &z, const FractalParams *fractal, ExtendedAux &aux)
 {
 #pragma clang fp contract(off)
-	a.DE = a.DE * 2.0 * a.r;
+	(void)fractal; // no parameters
+	aux.DE = aux.DE * 2.0 * aux.r;
 	const double newx = z.x * z.x - z.y * z.y - z.z * z.z;
 	const double newy = 2.0 * z.x * z.y;
 	const double newz = 2.0 * z.x * z.z;
@@ -102,7 +140,7 @@ __device__ __forceinline__ void gen_quaternion(DV4 &z, GenAux &a)
 }
 
 // cFractalMengerSponge::FormulaCode (fractal_menger_sponge.cpp:30-71), fold colour off
-__device__ __forceinline__ void gen_menger_sponge(const GenSlot &s, DV4 &z, GenAux &a)
+__device__ __forceinline__ void MengerSpongeIteration(DV4 &z, const FractalParams *fractal, ExtendedAux &aux)
 {
 #pragma clang fp contract(off)
 	z = dv4(fabs(z.x), fabs(z.y), fabs(z.z), fabs(z.w));
@@ -125,25 +163,25 @@ __device__ __forceinline__ void gen_menger_sponge(const GenSlot &s, DV4 &z, GenA
 		z.z = z.y;
 		z.y = t;
 	}
-	z = dv4(z.x * s.scale3, z.y * s.scale3, z.z * s.scale3, z.w * s.scale3);
+	z = dv4(z.x * fractal->scale3, z.y * fractal->scale3, z.z * fractal->scale3, z.w * fractal->scale3);
 	z.x -= 2.0;
 	z.y -= 2.0;
 	if (z.z > 1.0) z.z -= 2.0;
-	a.DE *= s.scale3;
+	aux.DE *= fractal->scale3;
 }
 
 // cFractalMandelbox::FormulaCode (fractal_mandelbox.cpp:31-107) with the fold colour of aux.color
-__device__ __forceinline__ void gen_mandelbox(const GenSlot &s, DV4 &z, GenAux &a)
+__device__ __forceinline__ void MandelboxIteration(DV4 &z, const FractalParams *fractal, ExtendedAux &aux)
 {
 #pragma clang fp contract(off)
-	const double L = s.mbFoldingLimit, V = s.mbFoldingValue;
-	if (s.mbRotOn)
+	const double L = fractal->mbFoldingLimit, V = fractal->mbFoldingValue;
+	if (fractal->mbRotOn)
 	{
 		// rotated folds (fractal_mandelbox.cpp:33-63): for each axis, the fold of the axis rotated
 		// by rot[0] (positive side) or else rot[1] (negative side), rotated back by its transpose
 		for (int dim = 0; dim < 3; dim++)
 		{
-			DV4 zRot = rot4(s.mbRot[0][dim], s.mbRotId[0][dim], z);
+			DV4 zRot = rot4(fractal->mbRot[0][dim], fractal->mbRotId[0][dim], z);
 			double zr = dim == 0 ? zRot.x : (dim == 1 ? zRot.y : zRot.z);
 			if (zr > L)
 			{
@@ -151,12 +189,12 @@ __device__ __forceinline__ void gen_mandelbox(const GenSlot &s, DV4 &z, GenAux &
 				if (dim == 0) zRot.x = zr;
 				else if (dim == 1) zRot.y = zr;
 				else zRot.z = zr;
-				z = rot4(s.mbRotInv[0][dim], 0, zRot);
-				a.color += s.mbColor[dim];
+				z = rot4(fractal->mbRotInv[0][dim], 0, zRot);
+				aux.color += fractal->mbColor[dim];
 			}
 			else
 			{
-				zRot = rot4(s.mbRot[1][dim], s.mbRotId[1][dim], z);
+				zRot = rot4(fractal->mbRot[1][dim], fractal->mbRotId[1][dim], z);
 				zr = dim == 0 ? zRot.x : (dim == 1 ? zRot.y : zRot.z);
 				if (zr < -L)
 				{
@@ -164,8 +202,8 @@ __device__ __forceinline__ void gen_mandelbox(const GenSlot &s, DV4 &z, GenAux &
 					if (dim == 0) zRot.x = zr;
 					else if (dim == 1) zRot.y = zr;
 					else zRot.z = zr;
-					z = rot4(s.mbRotInv[1][dim], 0, zRot);
-					a.color += s.mbColor[dim];
+					z = rot4(fractal->mbRotInv[1][dim], 0, zRot);
+					aux.color += fractal->mbColor[dim];
 				}
 			}
 		}
@@ -175,74 +213,74 @@ __device__ __forceinline__ void gen_mandelbox(const GenSlot &s, DV4 &z, GenAux &
 	if (fabs(z.x) > L)
 	{
 		z.x = dsgn(z.x) * V - z.x;
-		a.color += s.mbColor[0];
+		aux.color += fractal->mbColor[0];
 	}
 	if (fabs(z.y) > L)
 	{
 		z.y = dsgn(z.y) * V - z.y;
-		a.color += s.mbColor[1];
+		aux.color += fractal->mbColor[1];
 	}
 	if (fabs(z.z) > L)
 	{
 		z.z = dsgn(z.z) * V - z.z;
-		a.color += s.mbColor[2];
+		aux.color += fractal->mbColor[2];
 	}
 	}
 	const double r2 = dot4(z, z);
-	z = dv4(z.x + s.mbOffset[0], z.y + s.mbOffset[1], z.z + s.mbOffset[2], z.w + 0.0);
-	if (r2 < s.mbMR2)
+	z = dv4(z.x + fractal->mbOffset[0], z.y + fractal->mbOffset[1], z.z + fractal->mbOffset[2], z.w + 0.0);
+	if (r2 < fractal->mbMR2)
 	{
-		z = dv4(z.x * s.mbFactor1, z.y * s.mbFactor1, z.z * s.mbFactor1, z.w * s.mbFactor1);
-		a.DE *= s.mbFactor1;
-		a.color += s.mbSp1;
+		z = dv4(z.x * fractal->mbFactor1, z.y * fractal->mbFactor1, z.z * fractal->mbFactor1, z.w * fractal->mbFactor1);
+		aux.DE *= fractal->mbFactor1;
+		aux.color += fractal->mbSp1;
 	}
-	else if (r2 < s.mbFR2)
+	else if (r2 < fractal->mbFR2)
 	{
-		const double t = s.mbFR2 / r2;
+		const double t = fractal->mbFR2 / r2;
 		z = dv4(z.x * t, z.y * t, z.z * t, z.w * t);
-		a.DE *= t;
-		a.color += s.mbSp2;
+		aux.DE *= t;
+		aux.color += fractal->mbSp2;
 	}
-	z = dv4(z.x - s.mbOffset[0], z.y - s.mbOffset[1], z.z - s.mbOffset[2], z.w - 0.0);
-	if (s.mbMainRotOn) z = rot4(s.mbMainRot, 0, z);
-	z = dv4(z.x * s.mbScale, z.y * s.mbScale, z.z * s.mbScale, z.w * s.mbScale);
-	a.DE = a.DE * s.mbAbsScale + 1.0;
+	z = dv4(z.x - fractal->mbOffset[0], z.y - fractal->mbOffset[1], z.z - fractal->mbOffset[2], z.w - 0.0);
+	if (fractal->mbMainRotOn) z = rot4(fractal->mbMainRot, 0, z);
+	z = dv4(z.x * fractal->mbScale, z.y * fractal->mbScale, z.z * fractal->mbScale, z.w * fractal->mbScale);
+	aux.DE = aux.DE * fractal->mbAbsScale + 1.0;
 }
 
 // cFractalKaleidoscopicIfs::FormulaCode (fractal_kaleidoscopic_ifs.cpp:30-74)
-__device__ __forceinline__ void gen_kifs(const GenSlot &s, DV4 &z, GenAux &a)
+__device__ __forceinline__ void KaleidoscopicIfsIteration(DV4 &z, const FractalParams *fractal, ExtendedAux &aux)
 {
 #pragma clang fp contract(off)
-	if (s.ifsAbs[0]) z.x = fabs(z.x);
-	if (s.ifsAbs[1]) z.y = fabs(z.y);
-	if (s.ifsAbs[2]) z.z = fabs(z.z);
+	if (fractal->ifsAbs[0]) z.x = fabs(z.x);
+	if (fractal->ifsAbs[1]) z.y = fabs(z.y);
+	if (fractal->ifsAbs[2]) z.z = fabs(z.z);
 	for (int i = 0; i < MBX_IFS_VECTOR_COUNT; i++)
 	{
-		if (!s.ifsEnabled[i]) continue;
-		z = rot4(s.ifsRot[i], s.ifsRotId[i], z);
-		const DV4 d = dv4(s.ifsDirection[i][0], s.ifsDirection[i][1], s.ifsDirection[i][2], s.ifsDirection[i][3]);
+		if (!fractal->ifsEnabled[i]) continue;
+		z = rot4(fractal->ifsRot[i], fractal->ifsRotId[i], z);
+		const DV4 d = dv4(fractal->ifsDirection[i][0], fractal->ifsDirection[i][1], fractal->ifsDirection[i][2], fractal->ifsDirection[i][3]);
 		const double length = dot4(z, d);
-		if (length < s.ifsDistance[i])
+		if (length < fractal->ifsDistance[i])
 		{
-			const double k = 2.0 * (length - s.ifsDistance[i]) * s.ifsIntensity[i];
+			const double k = 2.0 * (length - fractal->ifsDistance[i]) * fractal->ifsIntensity[i];
 			z = dv4(z.x - d.x * k, z.y - d.y * k, z.z - d.z * k, z.w - d.w * k);
 		}
 	}
-	const DV4 off = dv4(s.ifsOffset[0], s.ifsOffset[1], s.ifsOffset[2], 0.0);
-	if (s.ifsRotationEnabled)
+	const DV4 off = dv4(fractal->ifsOffset[0], fractal->ifsOffset[1], fractal->ifsOffset[2], 0.0);
+	if (fractal->ifsRotationEnabled)
 	{
-		DV4 t = rot4(s.ifsMainRot, s.ifsMainRotId, dv4(z.x - off.x, z.y - off.y, z.z - off.z, z.w - off.w));
+		DV4 t = rot4(fractal->ifsMainRot, fractal->ifsMainRotId, dv4(z.x - off.x, z.y - off.y, z.z - off.z, z.w - off.w));
 		z = dv4(t.x + off.x, t.y + off.y, t.z + off.z, t.w + off.w);
 	}
-	if (s.ifsEdgeEnabled)
+	if (fractal->ifsEdgeEnabled)
 	{
-		if (s.ifsEdge[0] > 0.0) z.x = s.ifsEdge[0] - fabs(s.ifsEdge[0] - z.x);
-		if (s.ifsEdge[1] > 0.0) z.y = s.ifsEdge[1] - fabs(s.ifsEdge[1] - z.y);
-		if (s.ifsEdge[2] > 0.0) z.z = s.ifsEdge[2] - fabs(s.ifsEdge[2] - z.z);
+		if (fractal->ifsEdge[0] > 0.0) z.x = fractal->ifsEdge[0] - fabs(fractal->ifsEdge[0] - z.x);
+		if (fractal->ifsEdge[1] > 0.0) z.y = fractal->ifsEdge[1] - fabs(fractal->ifsEdge[1] - z.y);
+		if (fractal->ifsEdge[2] > 0.0) z.z = fractal->ifsEdge[2] - fabs(fractal->ifsEdge[2] - z.z);
 	}
-	z = dv4(z.x * s.ifsScale, z.y * s.ifsScale, z.z * s.ifsScale, z.w * s.ifsScale);
-	const double sm1 = s.ifsScale - 1.0;
-	if (s.ifsMenger)
+	z = dv4(z.x * fractal->ifsScale, z.y * fractal->ifsScale, z.z * fractal->ifsScale, z.w * fractal->ifsScale);
+	const double sm1 = fractal->ifsScale - 1.0;
+	if (fractal->ifsMenger)
 	{
 		z.x -= off.x * sm1;
 		z.y -= off.y * sm1;
@@ -250,20 +288,20 @@ __device__ __forceinline__ void gen_kifs(const GenSlot &s, DV4 &z, GenAux &a)
 	}
 	else
 		z = dv4(z.x - off.x * sm1, z.y - off.y * sm1, z.z - off.z * sm1, z.w - off.w * sm1);
-	a.DE *= fabs(s.ifsScale);
+	aux.DE *= fabs(fractal->ifsScale);
 }
 
 // cFractalXenodreambuie::FormulaCode (fractal_xenodreambuie.cpp:30-47)
-__device__ __forceinline__ void gen_xenodreambuie(const GenSlot &s, DV4 &z, GenAux &a)
+__device__ __forceinline__ void XenodreambuieIteration(DV4 &z, const FractalParams *fractal, ExtendedAux &aux)
 {
 #pragma clang fp contract(off)
-	double rp = pow(a.r, s.power - 1.0);
-	a.DE = rp * a.DE * s.power + 1.0;
-	rp *= a.r;
-	const double th = (atan2(z.y, z.x) + s.beta) * s.power;
-	double ph = acos(z.z / a.r) + s.alpha;
+	double rp = pow(aux.r, fractal->power - 1.0);
+	aux.DE = rp * aux.DE * fractal->power + 1.0;
+	rp *= aux.r;
+	const double th = (atan2(z.y, z.x) + fractal->beta) * fractal->power;
+	double ph = acos(z.z / aux.r) + fractal->alpha;
 	if (fabs(ph) > 0.5 * M_PI) ph = dsgn(ph) * M_PI - ph;
-	ph *= s.power;
+	ph *= fractal->power;
 	const double sph = sin(ph);
 	z.x = rp * cos(th) * sph;
 	z.y = rp * sin(th) * sph;
@@ -271,10 +309,10 @@ __device__ __forceinline__ void gen_xenodreambuie(const GenSlot &s, DV4 &z, GenA
 }
 
 // cFractalBoxFoldBulbPow2::FormulaCode (fractal_box_fold_bulb_pow2.cpp:28-77)
-__device__ __forceinline__ void gen_box_fold_bulb_pow2(const GenSlot &s, DV4 &z, GenAux &a)
+__device__ __forceinline__ void BoxFoldBulbPow2Iteration(DV4 &z, const FractalParams *fractal, ExtendedAux &aux)
 {
 #pragma clang fp contract(off)
-	const double ff = s.fipFoldFactor;
+	const double ff = fractal->fipFoldFactor;
 	if (fabs(z.x) > ff) z.x = dsgn(z.x) * ff * 2.0 - z.x;
 	if (fabs(z.y) > ff) z.y = dsgn(z.y) * ff * 2.0 - z.y;
 	if (fabs(z.z) > ff) z.z = dsgn(z.z) * ff * 2.0 - z.z;
@@ -284,13 +322,13 @@ __device__ __forceinline__ void gen_box_fold_bulb_pow2(const GenSlot &s, DV4 &z,
 	if (r2 < mR2)
 	{
 		z = dv4(z.x * factor1, z.y * factor1, z.z * factor1, z.w * factor1);
-		a.DE *= factor1;
+		aux.DE *= factor1;
 	}
 	else if (r2 < fR2)
 	{
 		const double factor2 = fR2 / r2;
 		z = dv4(z.x * factor2, z.y * factor2, z.z * factor2, z.w * factor2);
-		a.DE *= factor2;
+		aux.DE *= factor2;
 	}
 	z = dv4(z.x * 2.0, z.y * 2.0, z.z * 2.0, z.w * 2.0);
 	const double x2 = z.x * z.x, y2 = z.y * z.y, z2 = z.z * z.z;
@@ -301,28 +339,28 @@ __device__ __forceinline__ void gen_box_fold_bulb_pow2(const GenSlot &s, DV4 &z,
 	t.z = -2.0 * z.z * sqrt(x2 + y2);
 	t.w = z.w;
 	z = t;
-	z.z *= s.fipZFactor;
-	if (s.adeEnabledFalse)
-		a.DE = (a.DE + 1.0) * 5.0 * a.r * s.adeScale1 * sqrt(s.fipZFactor * s.fipZFactor + 2.0 + s.adeOffset2) +
-			s.adeOffset1;
+	z.z *= fractal->fipZFactor;
+	if (fractal->adeEnabledFalse)
+		aux.DE = (aux.DE + 1.0) * 5.0 * aux.r * fractal->adeScale1 * sqrt(fractal->fipZFactor * fractal->fipZFactor + 2.0 + fractal->adeOffset2) +
+			fractal->adeOffset1;
 }
 
 // cFractalAexion::FormulaCode (fractal_aexion.cpp:29-60): the first iteration replaces the
-// constant with a folded copy of the point
-__device__ __forceinline__ void gen_aexion(const GenSlot &s, DV4 &z, GenAux &a)
+// constant (aux.const_c) with a folded copy of the point
+__device__ __forceinline__ void AexionIteration(DV4 &z, const FractalParams *fractal, ExtendedAux &aux)
 {
 #pragma clang fp contract(off)
 	DV4 temp = z;
 	double t;
-	if (a.i == 0)
+	if (aux.i == 0)
 	{
-		t = s.cadd;
+		t = fractal->cadd;
 		temp.x = z.x + z.y + z.z;
 		temp.y = -z.x - z.y + z.z;
 		temp.z = -z.x + z.y - z.z;
 		temp.w = z.x - z.y - z.z;
 		z = dv4(fabs(temp.x) + t, fabs(temp.y) + t, fabs(temp.z) + t, fabs(temp.w) + t);
-		a.c = z;
+		aux.const_c = z;
 	}
 	t = 2.0 * z.w * z.z;
 	temp.x = z.x * z.x - z.y * z.y;
@@ -332,12 +370,12 @@ __device__ __forceinline__ void gen_aexion(const GenSlot &s, DV4 &z, GenAux &a)
 	temp.z = z.z * z.z - z.w * z.w;
 	temp.w = t - temp.z;
 	temp.z += t;
-	z = dv4(temp.x + a.c.x, temp.y + a.c.y, temp.z + a.c.z, temp.w + a.c.w);
-	if (s.adeEnabled)
+	z = dv4(temp.x + aux.const_c.x, temp.y + aux.const_c.y, temp.z + aux.const_c.z, temp.w + aux.const_c.w);
+	if (fractal->adeEnabled)
 	{
-		double de1 = 1.1 * a.r;
-		de1 = de1 + (len4(z) / a.r - de1) * s.adeScale1;
-		a.DE = de1 * 2.0 * a.DE + s.adeOffset1;
+		double de1 = 1.1 * aux.r;
+		de1 = de1 + (len4(z) / aux.r - de1) * fractal->adeScale1;
+		aux.DE = de1 * 2.0 * aux.DE + fractal->adeOffset1;
 	}
 }
 
@@ -349,26 +387,29 @@ struct GenZ
 	DV4 z;
 	double DE;
 };
-__device__ __attribute__((noinline)) GenZ gen_mandelbulb_call(const GenSlot &slot, DV4 z, double r, double DE)
+__device__ __attribute__((noinline)) GenZ gen_mandelbulb_call(const FractalParams &slot, DV4 z, double r, double DE)
 {
-	const GenSlot &s = uniform_ref(slot);
-	GenAux a;
+	const FractalParams &s = uniform_ref(slot);
+	ExtendedAux a;
 	a.r = r;
 	a.DE = DE;
-	gen_mandelbulb(s, z, a);
+	MandelbulbIteration(z, &s, a);
 	return GenZ{z, a.DE};
 }
-__device__ __attribute__((noinline)) GenZ gen_xenodreambuie_call(const GenSlot &slot, DV4 z, double r, double DE)
+__device__ __attribute__((noinline)) GenZ gen_xenodreambuie_call(const FractalParams &slot, DV4 z, double r, double DE)
 {
-	const GenSlot &s = uniform_ref(slot);
-	GenAux a;
+	const FractalParams &s = uniform_ref(slot);
+	ExtendedAux a;
 	a.r = r;
 	a.DE = DE;
-	gen_xenodreambuie(s, z, a);
+	XenodreambuieIteration(z, &s, a);
 	return GenZ{z, a.DE};
 }
 
-__device__ __forceinline__ void gen_formula(const GenSlot &s, DV4 &z, GenAux &a)
+// cAbstractFractal::FormulaCode of the container's formula (abstract_fractal.h:82): the formula
+// table of the device.  A formula added here also needs its metadata row in the host's table
+// (mbx_formula_get) and its parameters in FractalParams.
+__device__ __forceinline__ void formula_iteration(const FractalParams &s, DV4 &z, ExtendedAux &a)
 {
 	switch (s.formula)
 	{
@@ -379,12 +420,12 @@ __device__ __forceinline__ void gen_formula(const GenSlot &s, DV4 &z, GenAux &a)
 			a.DE = o.DE;
 			break;
 		}
-		case MBX_FORMULA_MANDELBULB_POWER2: gen_mandelbulb_power2(z, a); break;
-		case MBX_FORMULA_HYPERCOMPLEX: gen_hypercomplex(z, a); break;
-		case MBX_FORMULA_QUATERNION: gen_quaternion(z, a); break;
-		case MBX_FORMULA_MENGER_SPONGE: gen_menger_sponge(s, z, a); break;
-		case MBX_FORMULA_MANDELBOX: gen_mandelbox(s, z, a); break;
-		case MBX_FORMULA_KALEIDOSCOPIC_IFS: gen_kifs(s, z, a); break;
+		case MBX_FORMULA_MANDELBULB_POWER2: MandelbulbPower2Iteration(z, &s, a); break;
+		case MBX_FORMULA_HYPERCOMPLEX: HypercomplexIteration(z, &s, a); break;
+		case MBX_FORMULA_QUATERNION: QuaternionIteration(z, &s, a); break;
+		case MBX_FORMULA_MENGER_SPONGE: MengerSpongeIteration(z, &s, a); break;
+		case MBX_FORMULA_MANDELBOX: MandelboxIteration(z, &s, a); break;
+		case MBX_FORMULA_KALEIDOSCOPIC_IFS: KaleidoscopicIfsIteration(z, &s, a); break;
 		case MBX_FORMULA_XENODREAMBUIE:
 		{
 			const GenZ o = gen_xenodreambuie_call(s, z, a.r, a.DE);
@@ -392,8 +433,8 @@ __device__ __forceinline__ void gen_formula(const GenSlot &s, DV4 &z, GenAux &a)
 			a.DE = o.DE;
 			break;
 		}
-		case MBX_FORMULA_BOX_FOLD_BULB_POW2: gen_box_fold_bulb_pow2(s, z, a); break;
-		case MBX_FORMULA_AEXION: gen_aexion(s, z, a); break;
+		case MBX_FORMULA_BOX_FOLD_BULB_POW2: BoxFoldBulbPow2Iteration(z, &s, a); break;
+		case MBX_FORMULA_AEXION: AexionIteration(z, &s, a); break;
 		default: break;
 	}
 }
@@ -402,7 +443,7 @@ __device__ __forceinline__ void gen_formula(const GenSlot &s, DV4 &z, GenAux &a)
 
 // BoxFolding and SphericalFolding (legacy_fractal_transforrms.cpp:43-101) before the formula, each
 // followed by aux.r = |z| (compute_fractal.cpp:192-202)
-__device__ __forceinline__ void gen_global_foldings(const FrameConst &f, DV4 &z, GenAux &a)
+__device__ __forceinline__ void gen_global_foldings(const FrameConst &f, DV4 &z, ExtendedAux &a)
 {
 #pragma clang fp contract(off)
 	if (f.boxFold)
@@ -438,22 +479,22 @@ __device__ __forceinline__ void gen_global_foldings(const FrameConst &f, DV4 &z,
 }
 
 // formula, z + c, hybrid weight; returns the iteration's container
-__device__ __forceinline__ const GenSlot &gen_step(const FrameConst &f, int i, DV4 &z, GenAux &a)
+__device__ __forceinline__ const FractalParams &gen_step(const FrameConst &f, int i, DV4 &z, ExtendedAux &a)
 {
 #pragma clang fp contract(off)
-	const GenSlot &s = f.gen[i < f.genSeqLength ? f.genSeq[i] : f.genSeq[0]];
+	const FractalParams &s = f.gen[i < f.genSeqLength ? f.genSeq[i] : f.genSeq[0]];
 	if (f.boxFold || f.sphFold) gen_global_foldings(f, z, a);
 	const DV4 tempZ = z;
 	const double tempDE = a.DE, tempColor = a.color;
 	a.i = i;
-	if (s.weight > 0.0) gen_formula(s, z, a);
+	if (s.weight > 0.0) formula_iteration(s, z, a);
 	if (s.addC)
 	{
 		if (f.julia)
 			z = dv4(z.x + f.juliaCD[0], z.y + f.juliaCD[1], z.z + f.juliaCD[2], z.w + 0.0);
 		else
-			z = dv4(z.x + a.c.x * f.constMulD[0], z.y + a.c.y * f.constMulD[1], z.z + a.c.z * f.constMulD[2],
-				z.w + a.c.w);
+			z = dv4(z.x + a.const_c.x * f.constMulD[0], z.y + a.const_c.y * f.constMulD[1],
+				z.z + a.const_c.z * f.constMulD[2], z.w + a.const_c.w);
 	}
 	if (f.genHybrid && s.weight < 1.0)
 	{
@@ -479,19 +520,15 @@ __device__ __forceinline__ const GenSlot &gen_step(const FrameConst &f, int i, D
 
 // Compute<calcModeNormal / calcModeDeltaDE1> (compute_fractal.cpp:67-654): the orbit of p with
 // at most maxN iterations; returns the iteration count (i + 1) and leaves z and aux
-__device__ __forceinline__ int gen_orbit(const FrameConst &f, DV3 p, int maxN, DV4 &zOut, GenAux &a)
+__device__ __forceinline__ int gen_orbit(const FrameConst &f, DV3 p, int maxN, DV4 &zOut, ExtendedAux &a)
 {
 	DV4 z = dv4(p.x, p.y, p.z, f.genInitialW);
-	a.c = z;
-	a.r = len4(z);
-	a.DE = 1.0;
-	a.color = 1.0;
-	a.i = 0;
+	init_aux(a, z, f.gen[f.genSeq[0]].mbScale);
 	int i;
 	for (i = 0; i < maxN; i++)
 	{
 		const DV4 lastZ = z;
-		const GenSlot &s = gen_step(f, i, z, a);
+		const FractalParams &s = gen_step(f, i, z, a);
 		if (!finite4(z))
 		{
 			z = lastZ;
@@ -527,7 +564,7 @@ __device__ __attribute__((noinline)) double compute_generic(DV3 p, const FrameCo
 		if (k == 1) q.x = p.x + deltaDE;
 		if (k == 2) q.y = p.y + deltaDE;
 		if (k == 3) q.z = p.z + deltaDE;
-		GenAux a;
+		ExtendedAux a;
 		DV4 z;
 		const int it = gen_orbit(f, q, k == 0 ? f.formulaMaxiter : iters, z, a);
 		total += (uint32_t)it;
@@ -577,12 +614,8 @@ __device__ __attribute__((noinline)) double colour_index_generic(DV3 p, const Fr
 	const FrameConst &f = uniform_ref(frame);
 	DV4 z = dv4(p.x, p.y, p.z, f.genInitialW);
 	const DV4 start = z;
-	GenAux a;
-	a.c = z;
-	a.r = len4(z);
-	a.DE = 1.0;
-	a.color = 1.0;
-	a.i = 0;
+	ExtendedAux a;
+	init_aux(a, z, f.gen[f.genSeq[0]].mbScale);
 	double colorMin = 1000.0;
 	const int alg = f.colorAlgorithm;
 	const bool pre = f.colorPreV215 != 0;
@@ -590,7 +623,7 @@ __device__ __attribute__((noinline)) double colour_index_generic(DV3 p, const Fr
 	for (int i = 0; i < maxN; i++)
 	{
 		const DV4 lastZ = z;
-		const GenSlot &s = gen_step(f, i, z, a);
+		const FractalParams &s = gen_step(f, i, z, a);
 		if (!finite4(z))
 		{
 			z = lastZ;

@@ -427,28 +427,39 @@ double host_bulb_de(const double p[3], const mbx_fractal &fr, double bailout, in
 	return r / DE;
 }
 
-// cAbstractFractal metadata of the formulas of the general path (the constructors of
-// formula/definition/fractal_<name>.cpp): DE type, DE function, c addition (0 enabled by default,
-// 1 disabled by default, 2 the formula adds its own), default bailout, analytic finish, colour
-// index function (0 default, 1 Mandelbox, 2 IFS)
+// cAbstractFractal metadata of the formulas of the general path, as the constructors of
+// formula/definition/fractal_<name>.cpp set it (abstract_fractal.h:85-93): DE type, DE
+// function, c addition, default bailout, analytic finish, colour index function, names
 struct FormulaMeta
 {
 	int id, deType, deFunction, cpixel;
 	double defaultBailout;
 	int analytic, coloring;
+	const char *internalName, *nameInComboBox;
 };
 constexpr FormulaMeta kFormulaTable[] = {
-	{MBX_FORMULA_MANDELBULB, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LOGARITHMIC, 0, 10.0, MBX_ANALYTIC_LOGARITHMIC, 0},
-	{MBX_FORMULA_MANDELBULB_POWER2, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LOGARITHMIC, 0, 10.0, MBX_ANALYTIC_LOGARITHMIC, 0},
-	{MBX_FORMULA_HYPERCOMPLEX, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LOGARITHMIC, 0, 10.0, MBX_ANALYTIC_LOGARITHMIC, 0},
-	{MBX_FORMULA_QUATERNION, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LOGARITHMIC, 0, 10.0, MBX_ANALYTIC_LOGARITHMIC, 0},
-	{MBX_FORMULA_MENGER_SPONGE, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LINEAR, 1, 10.0, MBX_ANALYTIC_IFS, 2},
-	{MBX_FORMULA_MANDELBOX, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LINEAR, 0, 100.0, MBX_ANALYTIC_LINEAR, 1},
-	{MBX_FORMULA_KALEIDOSCOPIC_IFS, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LINEAR, 1, 100.0, MBX_ANALYTIC_IFS, 2},
-	{MBX_FORMULA_XENODREAMBUIE, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LOGARITHMIC, 0, 10.0, MBX_ANALYTIC_LOGARITHMIC, 0},
-	{MBX_FORMULA_BOX_FOLD_BULB_POW2, MBX_DE_DELTA, MBX_DE_FUNCTION_LINEAR, 0, 10000.0, MBX_ANALYTIC_LINEAR, 0},
-	{MBX_FORMULA_AEXION, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LOGARITHMIC, 2, 10000.0, MBX_ANALYTIC_LOGARITHMIC, 0},
+	{MBX_FORMULA_MANDELBULB, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LOGARITHMIC, MBX_CPIXEL_ENABLED_BY_DEFAULT, 10.0,
+		MBX_ANALYTIC_LOGARITHMIC, MBX_COLORING_DEFAULT, "mandelbulb", "Mandelbulb"},
+	{MBX_FORMULA_MANDELBULB_POWER2, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LOGARITHMIC, MBX_CPIXEL_ENABLED_BY_DEFAULT,
+		10.0, MBX_ANALYTIC_LOGARITHMIC, MBX_COLORING_DEFAULT, "mandelbulb_power2", "Mandelbulb - Power 2"},
+	{MBX_FORMULA_HYPERCOMPLEX, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LOGARITHMIC, MBX_CPIXEL_ENABLED_BY_DEFAULT, 10.0,
+		MBX_ANALYTIC_LOGARITHMIC, MBX_COLORING_DEFAULT, "hypercomplex", "Hypercomplex"},
+	{MBX_FORMULA_QUATERNION, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LOGARITHMIC, MBX_CPIXEL_ENABLED_BY_DEFAULT, 10.0,
+		MBX_ANALYTIC_LOGARITHMIC, MBX_COLORING_DEFAULT, "quaternion", "Quaternion"},
+	{MBX_FORMULA_MENGER_SPONGE, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LINEAR, MBX_CPIXEL_DISABLED_BY_DEFAULT, 10.0,
+		MBX_ANALYTIC_IFS, MBX_COLORING_IFS, "menger_sponge", "Menger sponge"},
+	{MBX_FORMULA_MANDELBOX, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LINEAR, MBX_CPIXEL_ENABLED_BY_DEFAULT, 100.0,
+		MBX_ANALYTIC_LINEAR, MBX_COLORING_ABOX, "mandelbox", "Mandelbox"},
+	{MBX_FORMULA_KALEIDOSCOPIC_IFS, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LINEAR, MBX_CPIXEL_DISABLED_BY_DEFAULT, 100.0,
+		MBX_ANALYTIC_IFS, MBX_COLORING_IFS, "kaleidoscopic_ifs", "Kaleidoscopic IFS"},
+	{MBX_FORMULA_XENODREAMBUIE, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LOGARITHMIC, MBX_CPIXEL_ENABLED_BY_DEFAULT, 10.0,
+		MBX_ANALYTIC_LOGARITHMIC, MBX_COLORING_DEFAULT, "xenodreambuie", "Xenodreambuie"},
+	{MBX_FORMULA_BOX_FOLD_BULB_POW2, MBX_DE_DELTA, MBX_DE_FUNCTION_LINEAR, MBX_CPIXEL_ENABLED_BY_DEFAULT, 10000.0,
+		MBX_ANALYTIC_LINEAR, MBX_COLORING_DEFAULT, "box_fold_bulb_pow2", "Box Fold Bulb Pow 2"},
+	{MBX_FORMULA_AEXION, MBX_DE_ANALYTIC, MBX_DE_FUNCTION_LOGARITHMIC, MBX_CPIXEL_ALREADY_HAS, 10000.0,
+		MBX_ANALYTIC_LOGARITHMIC, MBX_COLORING_DEFAULT, "aexion", "Aexion"},
 };
+constexpr int kFormulaCount = (int)(sizeof(kFormulaTable) / sizeof(kFormulaTable[0]));
 
 const FormulaMeta *formula_meta(int id)
 {
@@ -483,7 +494,7 @@ void rot3_deg(const double a[3], double out[9], int32_t &identity)
 
 // One container of the general path: sFractal after RecalculateFractalParams (fractal.cpp:41-120,
 // 653-678)
-void gen_slot(const mbx_fractal &f, mbx::GenSlot &s)
+void gen_slot(const mbx_fractal &f, mbx::FractalParams &s)
 {
 	std::memset(&s, 0, sizeof(s));
 	s.formula = f.formula;
@@ -573,7 +584,7 @@ int derive_generic(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc)
 	bool any = false;
 	for (int k = 0; k < n; k++)
 	{
-		mbx::GenSlot &s = fc.gen[k];
+		mbx::FractalParams &s = fc.gen[k];
 		gen_slot(*c[k], s);
 		if (c[k]->formula == MBX_FORMULA_NONE) continue;
 		const FormulaMeta *m = formula_meta(c[k]->formula);
@@ -1889,6 +1900,57 @@ int mbx_get_stats(mbx_ctx *ctx, mbx_stats *stats)
 	stats->kernel_ms = ms;
 	stats->far_steps = c[2];
 	return MBX_OK;
+}
+
+int mbx_formula_count(void) { return kFormulaCount; }
+
+int mbx_formula_get(int index, mbx_formula_info *info)
+{
+	if (!info || index < 0 || index >= kFormulaCount) return MBX_ERR_INVALID;
+	const FormulaMeta &m = kFormulaTable[index];
+	std::memset(info, 0, sizeof(*info));
+	info->internalID = m.id;
+	info->DEType = m.deType;
+	info->DEFunctionType = m.deFunction;
+	info->cpixelAddition = m.cpixel;
+	info->defaultBailout = m.defaultBailout;
+	info->DEAnalyticFunction = m.analytic;
+	info->coloringFunction = m.coloring;
+	std::snprintf(info->internalName, sizeof(info->internalName), "%s", m.internalName);
+	std::snprintf(info->nameInComboBox, sizeof(info->nameInComboBox), "%s", m.nameInComboBox);
+	return MBX_OK;
+}
+
+int mbx_formula_iterate(mbx_ctx *ctx, const mbx_fractal *fractal, int n, const double *z_in,
+	const mbx_extended_aux *aux_in, double *z_out, mbx_extended_aux *aux_out)
+{
+	if (!ctx || !fractal || n < 0 || (n > 0 && (!z_in || !aux_in || !z_out || !aux_out)))
+		return fail(ctx, MBX_ERR_INVALID, "bad argument");
+	if (!formula_meta(fractal->formula)) return fail(ctx, MBX_ERR_UNSUPPORTED, "formula not in the table");
+	if (n == 0) return MBX_OK;
+	hipError_t e = hipSetDevice(ctx->device);
+	if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+	mbx::FractalParams slot;
+	gen_slot(*fractal, slot);
+	const size_t zBytes = 4 * sizeof(double) * (size_t)n, auxBytes = sizeof(mbx_extended_aux) * (size_t)n;
+	const size_t slotBytes = (sizeof(slot) + 255) / 256 * 256, zOff = slotBytes, auxOff = zOff + (zBytes + 255) / 256 * 256;
+	unsigned char *d = nullptr;
+	int rc = MBX_OK;
+	if ((e = hipMalloc((void **)&d, auxOff + auxBytes)) != hipSuccess) return hip_fail(ctx, e, "allocate");
+	double *dz = (double *)(d + zOff);
+	mbx_extended_aux *daux = (mbx_extended_aux *)(d + auxOff);
+	if ((e = hipMemcpyAsync(d, &slot, sizeof(slot), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess ||
+		(e = hipMemcpyAsync(dz, z_in, zBytes, hipMemcpyHostToDevice, ctx->stream)) != hipSuccess ||
+		(e = hipMemcpyAsync(daux, aux_in, auxBytes, hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
+		rc = hip_fail(ctx, e, "copy inputs");
+	if (rc == MBX_OK && (e = mbx::launch_formula((const mbx::FractalParams *)d, dz, daux, n, ctx->stream)) != hipSuccess)
+		rc = hip_fail(ctx, e, "formula launch");
+	if (rc == MBX_OK && (e = hipStreamSynchronize(ctx->stream)) != hipSuccess) rc = hip_fail(ctx, e, "formula");
+	if (rc == MBX_OK && ((e = hipMemcpy(z_out, dz, zBytes, hipMemcpyDeviceToHost)) != hipSuccess ||
+							(e = hipMemcpy(aux_out, daux, auxBytes, hipMemcpyDeviceToHost)) != hipSuccess))
+		rc = hip_fail(ctx, e, "copy outputs");
+	(void)hipFree(d);
+	return rc;
 }
 
 int mbx_distance(mbx_ctx *ctx, const double *points, int n, double detail_size, float *dist_out,

@@ -29,6 +29,10 @@ hipError_t launch_render(int deKind, const FrameConst *frame, int width, int hei
 hipError_t launch_distance(int deKind, const FrameConst *frame, const float *pts, int n,
 	float detail, float *dist, int32_t *iters, hipStream_t stream);
 
+// One formula iteration per point (device memory): z (4 doubles) and aux updated in place.
+hipError_t launch_formula(const FractalParams *fractal, double *z, mbx_extended_aux *aux, int n,
+	hipStream_t stream);
+
 // Image post-processing of n pixels (3 floats each) through a 65536-entry gamma table (device);
 // out16 / out8 may be null.
 hipError_t launch_compile_image(const float *rgb, uint32_t n, const uint16_t *gammaTable, float brightness,

@@ -310,6 +310,66 @@ hipError_t launch_batch(int deKind, BatchArgs a, int numWaves, hipStream_t strea
 	}
 }
 
+// One FormulaCode call per point: the table's <Name>Iteration on z and the full sExtendedAux,
+// in the layout of mbx_extended_aux
+__global__ __launch_bounds__(64) void formula_kernel(const FractalParams *__restrict__ fractal,
+	double *__restrict__ zs, mbx_extended_aux *__restrict__ auxs, int n)
+{
+	const int k = blockIdx.x * 64 + threadIdx.x;
+	if (k >= n) return;
+	DV4 z = dv4(zs[4 * k], zs[4 * k + 1], zs[4 * k + 2], zs[4 * k + 3]);
+	mbx_extended_aux &e = auxs[k];
+	ExtendedAux a;
+	a.c = dv4(e.c[0], e.c[1], e.c[2], e.c[3]);
+	a.const_c = dv4(e.const_c[0], e.const_c[1], e.const_c[2], e.const_c[3]);
+	a.old_z = dv4(e.old_z[0], e.old_z[1], e.old_z[2], e.old_z[3]);
+	a.pos_neg = e.pos_neg;
+	a.r = e.r;
+	a.DE = e.DE;
+	a.DE0 = e.DE0;
+	a.dist = e.dist;
+	a.pseudoKleinianDE = e.pseudoKleinianDE;
+	a.actualScale = e.actualScale;
+	a.actualScaleA = e.actualScaleA;
+	a.color = e.color;
+	a.colorHybrid = e.colorHybrid;
+	a.temp1000 = e.temp1000;
+	a.i = e.i;
+	formula_iteration(*fractal, z, a);
+	zs[4 * k] = z.x;
+	zs[4 * k + 1] = z.y;
+	zs[4 * k + 2] = z.z;
+	zs[4 * k + 3] = z.w;
+	const double v4[3][4] = {{a.c.x, a.c.y, a.c.z, a.c.w}, {a.const_c.x, a.const_c.y, a.const_c.z, a.const_c.w},
+		{a.old_z.x, a.old_z.y, a.old_z.z, a.old_z.w}};
+	for (int j = 0; j < 4; j++)
+	{
+		e.c[j] = v4[0][j];
+		e.const_c[j] = v4[1][j];
+		e.old_z[j] = v4[2][j];
+	}
+	e.pos_neg = a.pos_neg;
+	e.r = a.r;
+	e.DE = a.DE;
+	e.DE0 = a.DE0;
+	e.dist = a.dist;
+	e.pseudoKleinianDE = a.pseudoKleinianDE;
+	e.actualScale = a.actualScale;
+	e.actualScaleA = a.actualScaleA;
+	e.color = a.color;
+	e.colorHybrid = a.colorHybrid;
+	e.temp1000 = a.temp1000;
+	e.i = a.i;
+}
+
+hipError_t launch_formula(const FractalParams *fractal, double *z, mbx_extended_aux *aux, int n,
+	hipStream_t stream)
+{
+	if (n <= 0) return hipSuccess;
+	hipLaunchKernelGGL(formula_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, fractal, z, aux, n);
+	return hipGetLastError();
+}
+
 hipError_t launch_distance(int deKind, const FrameConst *frame, const float *pts, int n,
 	float detail, float *dist, int32_t *iters, hipStream_t stream)
 {

@@ -17,7 +17,8 @@ EXPORTS = ["mbx_version", "mbx_sizeof_params", "mbx_ctx_create", "mbx_ctx_destro
            "mbx_last_error", "mbx_scene_upload", "mbx_scene_info", "mbx_render", "mbx_distance",
            "mbx_get_stats", "mbx_render_batch", "mbx_tune", "mbx_get_tune", "mbx_autotune",
            "mbx_render_diff", "mbx_adjoint", "mbx_params_default", "mbx_load_fract",
-           "mbx_compile_image", "mbx_write_png", "mbx_compile_zbuffer", "mbx_render_batch_noalloc"]
+           "mbx_compile_image", "mbx_write_png", "mbx_compile_zbuffer", "mbx_render_batch_noalloc",
+           "mbx_formula_count", "mbx_formula_get", "mbx_formula_iterate"]
 
 _lib = None
 
@@ -63,6 +64,11 @@ def load():
     L.mbx_compile_zbuffer.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                       C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_void_p]
     L.mbx_write_png.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
+    L.mbx_formula_count.argtypes = []
+    L.mbx_formula_count.restype = C.c_int
+    L.mbx_formula_get.argtypes = [C.c_int, P(abi.MbxFormulaInfo)]
+    L.mbx_formula_iterate.argtypes = [C.c_void_p, P(abi.MbxFractal), C.c_int, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p]
     _lib = L
     return L
 
@@ -85,6 +91,19 @@ def load_fract(text):
     if rc != 0:
         raise MbxError("mbx_load_fract: %s (%d)" % (info.message.decode(errors="replace"), rc))
     return p, info
+
+
+def formulas():
+    """The device's formula table with its metadata (mbx_formula_count / mbx_formula_get): a list
+    of abi.MbxFormulaInfo."""
+    L = load()
+    out = []
+    for k in range(L.mbx_formula_count()):
+        info = abi.MbxFormulaInfo()
+        if L.mbx_formula_get(k, C.byref(info)) != 0:
+            raise MbxError("mbx_formula_get(%d) failed" % k)
+        out.append(info)
+    return out
 
 
 def write_png(path, pixels):
@@ -281,6 +300,18 @@ class Context:
         self._check(self.L.mbx_get_stats(self.h, C.byref(s)), "mbx_get_stats")
         return {"rays": s.rays, "de_evals": s.de_evals, "iterations": s.iterations,
                 "kernel_ms": s.kernel_ms, "far_steps": s.far_steps}
+
+    def formula_iterate(self, fractal, z, aux):
+        """One FormulaCode call of fractal.formula per point on the device (mbx_formula_iterate):
+        z is (n, 4) float64, aux an array of n abi.MbxExtendedAux; returns the updated copies."""
+        import numpy as np
+        z = np.ascontiguousarray(z, np.float64).reshape(-1, 4).copy()
+        n = len(z)
+        a = (abi.MbxExtendedAux * n)(*aux)
+        self._check(self.L.mbx_formula_iterate(self.h, C.byref(fractal), n, z.ctypes.data_as(C.c_void_p),
+                                               C.cast(a, C.c_void_p), z.ctypes.data_as(C.c_void_p),
+                                               C.cast(a, C.c_void_p)), "mbx_formula_iterate")
+        return z, list(a)
 
     def distance(self, pts, detail_size):
         import numpy as np

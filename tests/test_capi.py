@@ -45,3 +45,33 @@ def test_product_does_not_import_oracle():
             if fn.endswith((".py", ".cpp", ".hip", ".h")):
                 text = open(os.path.join(dirpath, fn)).read()
                 assert "oracle" not in text.replace("oracle.", "").lower() or fn == "__init__.py", fn
+
+
+def test_formula_structs_match_c_layout(tmp_path):
+    """mbx_formula_info and mbx_extended_aux: ctypes sizes and field offsets equal the C
+    compiler's (a probe built from include/mbx.h with gcc)."""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("no C compiler")
+    fields = {"mbx_formula_info": abi.MbxFormulaInfo, "mbx_extended_aux": abi.MbxExtendedAux}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mbx.h"', "int main(void) {"]
+    for cname, cls in fields.items():
+        lines.append('printf("%%s sizeof %%zu\\n", "%s", sizeof(%s));' % (cname, cname))
+        for f, _ in cls._fields_:
+            lines.append('printf("%%s %%s %%zu\\n", "%s", "%s", offsetof(%s, %s));' % (cname, f, cname, f))
+    lines.append("return 0; }")
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for ln in out:
+        parts = ln.split()
+        if len(parts) == 3:
+            got[(parts[0], parts[1])] = int(parts[2])
+    for cname, cls in fields.items():
+        assert got[(cname, "sizeof")] == ctypes.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert got[(cname, f)] == getattr(cls, f).offset, (cname, f)
