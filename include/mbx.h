@@ -368,6 +368,12 @@ typedef struct mbx_params {
 	/* params->N (fractparams.cpp:189; "N", default 250): the 1 + N - iterations pseudo-distance
 	 * of slow shading (shader_calculate_normals.cpp:103), apart from the node's maxiter */
 	int32_t N;
+	/* adaptive Monte Carlo (render_worker.cpp:436-449): after each sample past the first
+	 * DOFMinSamples + 1, the pixel stops once MonteCarloDOFNoiseEstimation (:1871-1901) falls
+	 * below DOFMaxNoise * 0.01 */
+	int32_t DOFMinSamples;          /* "DOF_min_samples" (default 10) */
+	int32_t reserved5;
+	double DOFMaxNoise;             /* "DOF_max_noise" (default 1) */
 } mbx_params;
 
 /* Per-pixel output planes (device pointers; any may be NULL except where noted).
@@ -629,7 +635,7 @@ void mbx_params_default(mbx_params *p);
 #define MBX_FRACT_AO_LIGHTMAP 0x800u   /* multi-ray AO: its directions and colours come from the
                                           lightmap texture (aoVectorsCount is left 0) */
 #define MBX_FRACT_POSTPROCESS 0x1000u  /* brightness, contrast, gamma, saturation, HDR, blur, post DOF */
-#define MBX_FRACT_MC_ADAPTIVE 0x2000u  /* Monte Carlo stops early (DOF_min_samples < DOF_samples) */
+#define MBX_FRACT_MC_ADAPTIVE 0x2000u  /* (unused: the adaptive Monte Carlo stop is rendered) */
 #define MBX_FRACT_PERSPECTIVE 0x4000u  /* fish-eye or equirectangular projection, stereo */
 #define MBX_FRACT_FORMULA 0x8000u      /* a formula outside the device's table (the MBX_FORMULA_* ids) */
 #define MBX_FRACT_ANIMATION 0x10000u   /* [frames] / [keyframes] sections (not read) */

@@ -250,6 +250,9 @@ class MbxParams(C.Structure):
         ("sphericalFoldingInner", C.c_double),
         ("slowShading", C.c_int32),
         ("N", C.c_int32),
+        ("DOFMinSamples", C.c_int32),
+        ("reserved5", C.c_int32),
+        ("DOFMaxNoise", C.c_double),
     ]
 
 

@@ -192,6 +192,8 @@ struct FrameConst
 
 	// Monte Carlo
 	int32_t monteCarlo, samples, globalIllumination, reflectionsMax;
+	int32_t DOFMinSamples;  // adaptive stop: past this many samples ...
+	double DOFMaxNoise01;   // ... once the noise estimate is below DOF_max_noise * 0.01
 	float giRadianceLimit;
 
 	uint32_t randomSeed;

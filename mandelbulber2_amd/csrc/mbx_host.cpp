@@ -1112,6 +1112,8 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc, bool forceGe
 	// Monte Carlo
 	fc.monteCarlo = p->DOFMonteCarlo;
 	fc.samples = p->DOFMonteCarlo ? p->DOFSamples : 1;
+	fc.DOFMinSamples = p->DOFMinSamples;
+	fc.DOFMaxNoise01 = p->DOFMaxNoise * 0.01;
 	if (fc.samples < 1) return fail(ctx, MBX_ERR_INVALID, "DOF_samples < 1");
 	fc.globalIllumination = p->DOFMonteCarloGlobalIllumination;
 	fc.reflectionsMax = p->reflectionsMax;

@@ -645,6 +645,23 @@ struct PixelOut
 	int steps;
 };
 
+// MonteCarloDOFNoiseEstimation (render_worker.cpp:1871-1901) after sample `rep` with the running
+// sum: channels in float as sRGBFloat holds them, the noise in double
+__device__ __forceinline__ double mc_noise(V3 pixel, int rep, V3 sum, V3 &stdDevSum)
+{
+#pragma clang fp contract(off)
+	const float k = (float)(rep + 1);
+	const float aR = sum.x / k, aG = sum.y / k, aB = sum.z / k;
+	stdDevSum.x += (pixel.x - aR) * (pixel.x - aR);
+	stdDevSum.y += (pixel.y - aG) * (pixel.y - aG);
+	stdDevSum.z += (pixel.z - aB) * (pixel.z - aB);
+	double total = sqrt((double)(stdDevSum.x + stdDevSum.y + stdDevSum.z) / ((double)rep + 1.0));
+	total /= sqrt((double)(rep + 1));
+	const double brightness = (double)(sum.x + sum.y + sum.z) / ((double)rep + 1.0);
+	if (brightness > 1.0) total /= brightness;
+	return total;
+}
+
 // One pixel: the sample loop of doWork and RayRecursion at depth 0.  FULL = false is the
 // variant for frames without Monte Carlo sampling, GI, shadows and AO.
 template <int KIND, bool FULL>
@@ -661,8 +678,9 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 	const R ox = (Q::invWidth(f) * (R)xs - (R)0.5) * aspect;
 	const R oy = (R)0.5 - Q::invHeight(f) * (R)ys;
 	const P cam = Q::cam(f);
-	const int repeats = (FULL && f.monteCarlo) ? f.samples : 1;
+	int repeats = (FULL && f.monteCarlo) ? f.samples : 1;
 	V3 acc = v3(0.0f, 0.0f, 0.0f);
+	V3 stdDevSum = v3(0.0f, 0.0f, 0.0f);
 	po.steps = 0;
 	for (int rep = 0; rep < repeats; rep++)
 	{
@@ -710,6 +728,11 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 		}
 		po.depth = m.found ? (float)m.depth : 1e20f;
 		po.steps += m.steps;
+		if (FULL && f.monteCarlo && mc_noise(rgb, rep, acc, stdDevSum) < f.DOFMaxNoise01 && rep > f.DOFMinSamples)
+		{
+			repeats = rep + 1; // adaptive stop (render_worker.cpp:436-449)
+			break;
+		}
 	}
 	po.rgb = repeats > 1 ? v3(acc.x / (float)repeats, acc.y / (float)repeats, acc.z / (float)repeats) : acc;
 }

@@ -198,6 +198,7 @@ def init_params():
     c.add("DOF_monte_carlo", BOOL, False)                           # :329
     c.add("DOF_samples", INT, 100)                                  # :330
     c.add("DOF_min_samples", INT, 10)                               # :331
+    c.add("DOF_max_noise", DOUBLE, 1.0, 0.00001, 100.0)             # :332
     c.add("DOF_MC_global_illumination", BOOL, False)                # :333
     c.add("MC_global_illumination_volumetric", BOOL, False)         # :334
     c.add("DOF_MC_CA_enable", BOOL, False)                          # :336
@@ -878,8 +879,8 @@ def load(text, trace=None):
     out["DOFSamples"] = g("DOF_samples")
     out["DOFMonteCarloGlobalIllumination"] = int(g("DOF_MC_global_illumination"))
     out["monteCarloGIRadianceLimit"] = g("MC_GI_radiance_limit")
-    if g("DOF_monte_carlo") and g("DOF_min_samples") < g("DOF_samples"):
-        flags |= FLAGS["mc_adaptive"]
+    out["DOFMinSamples"] = g("DOF_min_samples")         # the adaptive stop (render_worker.cpp:436-449)
+    out["DOFMaxNoise"] = g("DOF_max_noise")
     # Monte Carlo soft shadows jitter the light vector (shader_aux_shadow.cpp:76-87);
     # chromatic aberration shifts the view vector by a random hue (render_worker.cpp:255-264)
     if g("DOF_monte_carlo") and (g("MC_soft_shadows_enable") or g("DOF_MC_CA_enable")):

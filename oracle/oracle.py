@@ -59,6 +59,8 @@ def lib(fast=False):
     L.mbxo_shader_probe.argtypes = [P(abi.MbxParams), C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_uint32,
                                     C.c_void_p, C.c_void_p]
     L.mbxo_random_int.argtypes = [P(C.c_uint32)]
+    L.mbxo_mc_noise.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+    L.mbxo_mc_noise.restype = None
     L.mbxo_random.argtypes = [P(C.c_uint32), C.c_int]
     L.mbxo_pixel_seed.argtypes = [C.c_uint32, C.c_uint32]
     L.mbxo_pixel_seed.restype = C.c_uint32
@@ -122,6 +124,14 @@ def compute(p, pts):
     if rc != 0:
         raise RuntimeError("oracle compute failed: %d" % rc)
     return d, it, mx, z
+
+
+def mc_noise(samples):
+    """MonteCarloDOFNoiseEstimation after each sample of one pixel: samples (n, 3) float32."""
+    x = np.ascontiguousarray(samples, np.float32).reshape(-1, 3)
+    out = np.zeros(len(x), np.float64)
+    lib().mbxo_mc_noise(_ptr(x), len(x), _ptr(out))
+    return out
 
 
 def formula_step(p, z4, aux3):

@@ -2148,6 +2148,36 @@ static v3 view_vector(const scene_t *s, double ix, double iy)
 	return rot_apply3(&s->mRot, v);
 }
 
+/* MonteCarloDOFNoiseEstimation (render_worker.cpp:1871-1901): the standard error of the pixel's
+ * samples so far, relative to its mean brightness when that exceeds 1; sRGBFloat channels are
+ * float, the estimate double */
+static double mc_noise(const float pixel[3], int repeat, const float sum[3], float stdDevSum[3])
+{
+	for (int c = 0; c < 3; c++)
+	{
+		const float avg = sum[c] / (float)(repeat + 1);
+		const float d = (pixel[c] - avg) * (pixel[c] - avg);
+		stdDevSum[c] += d;
+	}
+	double total = sqrt((double)(stdDevSum[0] + stdDevSum[1] + stdDevSum[2]) / (repeat + 1.0));
+	total /= sqrt((double)(repeat + 1));
+	const double brightness = (double)(sum[0] + sum[1] + sum[2]) / (repeat + 1.0);
+	if (brightness > 1.0) total /= brightness;
+	return total;
+}
+
+/* the noise estimate after each of n samples (3 floats each) of one pixel, accumulated as the
+ * pixel loop does (test probe) */
+void mbxo_mc_noise(const float *pixels, int n, double *noiseOut)
+{
+	float acc[3] = {0, 0, 0}, stdDevSum[3] = {0, 0, 0};
+	for (int k = 0; k < n; k++)
+	{
+		for (int c = 0; c < 3; c++) acc[c] += pixels[3 * k + c];
+		noiseOut[k] = mc_noise(pixels + 3 * k, k, acc, stdDevSum);
+	}
+}
+
 /* pixel loop body of cRenderWorker::doWork (render_worker.cpp:155-525) for one pixel */
 static void render_pixel(ray_ctx *rc, int xs, int ys, ray_result *out)
 {
@@ -2163,6 +2193,7 @@ static void render_pixel(ray_ctx *rc, int xs, int ys, ray_result *out)
 	int repeats = p->DOFMonteCarlo ? p->DOFSamples : 1;
 	if (repeats < 1) repeats = 1;
 	float acc[3] = {0, 0, 0};
+	float stdDevSum[3] = {0, 0, 0};
 	ray_result r;
 	memset(out, 0, sizeof(*out));
 	for (int repeat = 0; repeat < repeats; repeat++)
@@ -2187,6 +2218,16 @@ static void render_pixel(ray_ctx *rc, int xs, int ys, ray_result *out)
 		out->steps += r.steps;
 		/* depth of the last sample is what the reference writes (render_worker.cpp:346-348) */
 		out->depth = r.depth;
+		/* adaptive Monte Carlo: stop once the noise estimate is low (render_worker.cpp:436-449) */
+		if (p->DOFMonteCarlo)
+		{
+			const double noise = mc_noise(r.rgb, repeat, acc, stdDevSum);
+			if (repeat > p->DOFMinSamples && noise < p->DOFMaxNoise * 0.01)
+			{
+				repeats = repeat + 1;
+				break;
+			}
+		}
 	}
 	if (p->DOFMonteCarlo)
 	{

@@ -126,8 +126,6 @@ MAIN_RULES = [
     (r"post_chromatic_aberration_.*", _gate("postprocess", _on("post_chromatic_aberration_enabled"))),
     (r"DOF_(focus|radius|blur_opacity|number_of_passes|HDR|enabled)",
      _gate("postprocess", _on("DOF_enabled"))),              # post DOF, or the lens of MC DOF
-    (r"DOF_(max_noise|min_samples)", _gate("mc_adaptive", lambda p, m: _mc(p, m) and
-                                           _v(p, "DOF_min_samples") < _v(p, "DOF_samples"))),
     (r"DOF_MC_CA_.*", _gate("mc_effects", lambda p, m: _mc(p, m) and _v(p, "DOF_MC_CA_enable"))),
     (r"MC_soft_shadows_enable", _gate("mc_effects", _mc)),
     (r"MC_global_illumination_volumetric|volumetric_light_DE_Factor",

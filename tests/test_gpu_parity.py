@@ -173,6 +173,31 @@ def test_monte_carlo_gi_parity(rt, oracle_mod):
     _check(st)
 
 
+def test_adaptive_monte_carlo_parity(rt, oracle_mod):
+    """Adaptive Monte Carlo (render_worker.cpp:436-449): at most 24 samples, the stop allowed
+    from the fifth; at a noise threshold of 0.02 the pixels of this scene stop anywhere from 5 to
+    24 samples.  Against the oracle, and on the device alone: with the largest threshold every
+    pixel stops at DOF_min_samples + 2, so the image equals a fixed 5-sample render bit for bit."""
+    p = scene.mandelbox_shadow_ao(96, 64, seed=3)
+    p.DOFMonteCarlo = 1
+    p.DOFSamples = 24
+    p.DOFMinSamples = 3
+    p.DOFMaxNoise = 2.0
+    g = rt.render_to_host(p)
+    r = oracle_mod.render(p)
+    st = compare(g, r)
+    _report("mandelbox adaptive MC 96x64", st)
+    _check(st, depth_tol=1e-6)
+    p.DOFMaxNoise = 100.0
+    fixed = scene.mandelbox_shadow_ao(96, 64, seed=3)
+    fixed.DOFMonteCarlo = 1
+    fixed.DOFSamples = 5
+    fixed.DOFMinSamples = 10
+    a, b = rt.render_to_host(p), rt.render_to_host(fixed)
+    np.testing.assert_array_equal(a["rgb"], b["rgb"])
+    np.testing.assert_array_equal(a["depth"], b["depth"])
+
+
 @pytest.mark.parametrize("powers", [(6.0, 7.0, 9.5), (8.0, 8.0, 8.0)])
 @pytest.mark.parametrize("kernel", [GRID, PLAIN, PERSISTENT])
 def test_batch_equals_single_frames(rt, powers, kernel):

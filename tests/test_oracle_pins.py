@@ -478,3 +478,67 @@ def test_slow_shading_normal_follows_the_iteration_gradient():
     # iterations step across the sphere (the grid straddles it): inside 7 + 1, outside 7
     _, it = oracle.distance(p, np.array([0.99 * R * dirs[0], 1.01 * R * dirs[0]]), 0.01)
     assert list(it) == [8, 7]
+
+
+# ------------------------------------------------------------------------------------------
+# adaptive Monte Carlo (render_worker.cpp:436-449, 1871-1901)
+# ------------------------------------------------------------------------------------------
+def _alternating_noise(a, b, n):
+    """Noise after each of n samples alternating a, b (grey, 3 equal channels), from the running
+    mean: sample 2j sits j (a - b) / (2j + 1) from the mean of the first 2j + 1 samples, sample
+    2j + 1 half of (b - a) from the mean (a + b) / 2.  Standard error = sqrt(3 D / k) / sqrt(k)
+    for the summed squared deviations D of one channel, divided by the mean brightness
+    3 (sum / k) when that exceeds 1."""
+    out, D, s = [], 0.0, 0.0
+    for k in range(n):
+        x = a if k % 2 == 0 else b
+        j = k // 2
+        dev = j * (a - b) / (2 * j + 1) if k % 2 == 0 else (b - a) / 2
+        D += dev * dev
+        s += x
+        e = math.sqrt(3 * D / (k + 1)) / math.sqrt(k + 1)
+        bright = 3 * s / (k + 1)
+        out.append(e / bright if bright > 1.0 else e)
+    return np.array(out)
+
+
+def test_mc_noise_follows_the_running_mean_closed_form():
+    a, b = 0.25, 0.75                         # dyadic: the float sums are exact
+    x = np.array([[a] * 3 if k % 2 == 0 else [b] * 3 for k in range(40)], np.float32)
+    np.testing.assert_allclose(oracle.mc_noise(x), _alternating_noise(a, b, 40), rtol=1e-6)
+    # brighter than 1: the estimate is relative to the brightness, so scaling by 4 (exact in
+    # float) leaves it unchanged once the mean brightness exceeds 1 (a wrong power of the
+    # brightness, or the absolute error, would scale it)
+    n1, n4 = oracle.mc_noise(x), oracle.mc_noise(4 * x)
+    bright1 = np.cumsum(x.sum(1)) / np.arange(1, 41)
+    assert (bright1[1:] > 1).all() and n1[0] == 0.0
+    np.testing.assert_array_equal(n4, n1)
+    np.testing.assert_allclose(n4, _alternating_noise(4 * a, 4 * b, 40), rtol=1e-6)
+    # constant samples: no noise at all
+    assert (oracle.mc_noise(np.full((12, 3), 0.5, np.float32)) == 0.0).all()
+
+
+def test_adaptive_stop_takes_min_samples_plus_two():
+    """With DOF_max_noise at its maximum (threshold 1.0) every pixel's estimate is below the
+    threshold once the loop may stop, at repeat DOF_min_samples + 1: the image equals a fixed
+    render of DOF_min_samples + 2 samples (same per-pixel streams), bit for bit.  With threshold 0
+    (no estimate is below it; the strict comparison of render_worker.cpp:444) the image equals the
+    full fixed render."""
+    p = scene.mandelbox_shadow_ao(48, 32, seed=3)
+    p.DOFMonteCarlo = 1
+    p.DOFSamples = 24
+    p.DOFMinSamples = 3
+    p.DOFMaxNoise = 100.0
+    adaptive = oracle.render(p)
+    fixed = scene.mandelbox_shadow_ao(48, 32, seed=3)
+    fixed.DOFMonteCarlo = 1
+    fixed.DOFSamples = 5
+    fixed.DOFMinSamples = 10                  # >= samples: never stops early
+    ref5 = oracle.render(fixed)
+    np.testing.assert_array_equal(adaptive["rgb"], ref5["rgb"])
+    np.testing.assert_array_equal(adaptive["depth"], ref5["depth"])
+    p.DOFMaxNoise = 0.0                       # even a pixel of identical samples does not stop
+    tight = oracle.render(p)
+    fixed.DOFSamples = 24
+    fixed.DOFMinSamples = 30
+    np.testing.assert_array_equal(tight["rgb"], oracle.render(fixed)["rgb"])
