@@ -641,7 +641,7 @@ void mbx_params_default(mbx_params *p);
 #define MBX_FRACT_ANIMATION 0x10000u   /* [frames] / [keyframes] sections (not read) */
 #define MBX_FRACT_MATERIAL 0x20000u    /* other material effects: iridescence, roughness, subsurface,
                                           perlin noise, luminosity gradient */
-#define MBX_FRACT_INTERIOR 0x40000u    /* interior mode or iteration-threshold mode */
+#define MBX_FRACT_INTERIOR 0x40000u    /* (unused: interior and iteration-threshold modes are rendered) */
 #define MBX_FRACT_DISABLED 0x80000u    /* the fractal node is disabled or absent: nothing to render */
 #define MBX_FRACT_ANTIALIASING 0x100000u /* antialiasing repeats (antialiasing_size^2 per pixel) */
 #define MBX_FRACT_MC_EFFECTS 0x200000u /* Monte Carlo soft shadows or chromatic aberration */

@@ -652,12 +652,20 @@ struct Prec<false>
 	static __device__ __forceinline__ R threshK(const FrameConst &f) { return f.threshK; }
 	static __device__ __forceinline__ R deltaK(const FrameConst &f) { return f.deltaK; }
 	static __device__ __forceinline__ R DEThresh(const FrameConst &f) { return f.DEThresh; }
+	static __device__ __forceinline__ R DEFactor(const FrameConst &f) { return f.DEFactor; }
+	static __device__ __forceinline__ R smoothness(const FrameConst &f) { return f.smoothness; }
 	static __device__ __forceinline__ R len(P a) { return fsqrt(dot(a, a)); }
 	static __device__ __forceinline__ R rcp(R x) { return frcp(x); }
 	static __device__ __forceinline__ R thresh(const FrameConst &f, P p) { return len(cam(f) - p) * f.threshK; }
 	static __device__ __forceinline__ P along(P start, P dir, R scan) { return start + dir * scan; }
 	static __device__ __forceinline__ P from(V3 a) { return a; }
 	static __device__ __forceinline__ V3 tof(P a) { return a; }
+	static __device__ __forceinline__ P fromD(double x, double y, double z) { return v3((float)x, (float)y, (float)z); }
+	static __device__ __forceinline__ P lightDir(const FrameConst &f) { return v3(f.lightDir[0], f.lightDir[1], f.lightDir[2]); }
+	static __device__ __forceinline__ P aoVector(const FrameConst &f, int i)
+	{
+		return v3(f.aoVectors[i][0], f.aoVectors[i][1], f.aoVectors[i][2]);
+	}
 	// world -> local point of the rotated fractal node (calculate_distance.cpp:296)
 	static __device__ __forceinline__ P node_local(const FrameConst &f, P p) { return rot(f.nodeRot, p); }
 	static __device__ __forceinline__ R invWidth(const FrameConst &f) { return f.invWidth; }
@@ -680,13 +688,17 @@ struct Prec<true>
 	static __device__ __forceinline__ R threshK(const FrameConst &f) { return f.threshKD; }
 	static __device__ __forceinline__ R deltaK(const FrameConst &f) { return f.deltaKD; }
 	static __device__ __forceinline__ R DEThresh(const FrameConst &f) { return f.DEThreshD; }
+	static __device__ __forceinline__ R DEFactor(const FrameConst &f) { return f.DEFactorD; }
+	static __device__ __forceinline__ R smoothness(const FrameConst &f) { return f.smoothnessD; }
 	static __device__ __forceinline__ R len(P a) { return sqrt(ddot(a, a)); }
 	static __device__ __forceinline__ R rcp(R x) { return 1.0 / x; }
-	// CalcDistThresh in the reference's operation order: |camera - p| * resolution * fov / detail
+	// CalcDistThresh in the reference's operation order: |camera - p| * resolution * fov / detail,
+	// without the detail level in iteration-threshold mode (render_worker.cpp:656-661)
 	static __device__ __forceinline__ R thresh(const FrameConst &f, P p)
 	{
 #pragma clang fp contract(off)
-		return len(cam(f) - p) * f.resolutionD * f.fovD / f.detailLevelD;
+		const R t = len(cam(f) - p) * f.resolutionD * f.fovD;
+		return f.iterThreshMode ? t : t / f.detailLevelD;
 	}
 	// start + dir * scan without FMA contraction (CVector3 operator* then operator+)
 	static __device__ __forceinline__ P along(P start, P dir, R scan)
@@ -696,6 +708,14 @@ struct Prec<true>
 	}
 	static __device__ __forceinline__ P from(V3 a) { return dv3(a.x, a.y, a.z); }
 	static __device__ __forceinline__ V3 tof(P a) { return v3((float)a.x, (float)a.y, (float)a.z); }
+	static __device__ __forceinline__ P fromD(double x, double y, double z) { return dv3(x, y, z); }
+	// secondary-ray directions in FP64, as the reference holds them: sample positions along them
+	// must match where the DE is discontinuous (interior and iteration-threshold modes)
+	static __device__ __forceinline__ P lightDir(const FrameConst &f) { return dv3(f.lightDirD[0], f.lightDirD[1], f.lightDirD[2]); }
+	static __device__ __forceinline__ P aoVector(const FrameConst &f, int i)
+	{
+		return dv3(f.aoVectorsD[i][0], f.aoVectorsD[i][1], f.aoVectorsD[i][2]);
+	}
 	// world -> local point of the rotated fractal node (calculate_distance.cpp:296; CMatrix44
 	// TransformPoint sums left to right)
 	static __device__ __forceinline__ P node_local(const FrameConst &f, P p)
@@ -720,13 +740,40 @@ struct Prec<true>
 	}
 };
 
+// The tail of CalculateDistanceSimple with interior or iteration-threshold mode on
+// (calculate_distance.cpp:561-605, 722-759): maxiter is the first orbit's Compute flag.  The
+// analytic branch tests that raw flag for the interior; the delta branch a copy kept only in
+// iteration-threshold mode ("limits disabled").
+__device__ __forceinline__ double distance_modes(const FrameConst &f, double d, double detailSize, int itersMinN,
+	int minN, bool maxiter, bool normalCalc)
+{
+#pragma clang fp contract(off)
+	if (f.iterThreshMode && !normalCalc && maxiter) d = 0.0; // out->maxiter
+	if (itersMinN < minN && d < detailSize) d = detailSize;
+	const bool interiorMaxiter = (f.genDEType == MBX_DE_ANALYTIC) ? maxiter : (f.iterThreshMode && maxiter);
+	if (f.interiorMode && !normalCalc)
+	{
+		if (d < 0.5 * detailSize || interiorMaxiter) d = detailSize;
+	}
+	else if (f.interiorMode && normalCalc)
+	{
+		if (d < 0.9 * detailSize) d = detailSize - d;
+	}
+	if (f.iterThreshMode && !normalCalc && !maxiter && d < detailSize) d = detailSize * 1.01;
+	d = (d < 0.0) ? 0.0 : d;
+	d = (d > 10.0) ? 10.0 : d;
+	if (!(d < 1e20)) d = 1e20;
+	return d;
+}
+
 // CalculateDistance for the single fractal node (calculate_distance.cpp:225-516 reduces to
-// CalculateDistanceSimple for one node with identity transform and scale 1).  iterThresh and
-// interior modes are off on the device path.
+// CalculateDistanceSimple for one node with identity transform and scale 1).  normalCalc marks the
+// normal-vector taps (sDistanceIn's normalCalculationMode).  Interior and iteration-threshold
+// modes run on the general kernel only (the host routes such scenes there).
 template <int KIND>
 __device__ __forceinline__ typename Prec<PosPrecision<KIND>::kFP64>::R calculate_distance(
 	typename Prec<PosPrecision<KIND>::kFP64>::P p, typename Prec<PosPrecision<KIND>::kFP64>::R detailSize,
-	const FrameConst &f, int &iters, Work &w)
+	const FrameConst &f, int &iters, Work &w, bool normalCalc = false)
 {
 	using R = typename Prec<PosPrecision<KIND>::kFP64>::R;
 	if (w.nodeRotated) p = Prec<PosPrecision<KIND>::kFP64>::node_local(f, p);
@@ -757,8 +804,16 @@ __device__ __forceinline__ typename Prec<PosPrecision<KIND>::kFP64>::R calculate
 	}
 	else if constexpr (KIND == kGeneric)
 	{
-		d = compute_generic(p, f, detailSize, &iters, &w.iterations);
-		itersMinN = iters;
+		const GenDistance g = compute_generic(p, f, detailSize, normalCalc ? 1 : 0);
+		d = g.d;
+		iters = g.iters;
+		itersMinN = g.itersLast;
+		w.iterations += g.work;
+		if (f.interiorMode || f.iterThreshMode)
+		{
+			w.de_evals++;
+			return distance_modes(f, d, detailSize, itersMinN, w.minN, g.maxiter != 0, normalCalc);
+		}
 	}
 	else
 	{

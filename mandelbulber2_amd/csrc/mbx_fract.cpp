@@ -1000,7 +1000,6 @@ extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, 
 	p.image.gamma = (float)M.d("gamma", 1.0, 0.0, 1e15);
 	p.image.saturation = (float)M.d("saturation", 1.0, 0.0, 1e15);
 	p.image.hdr = M.b("hdr", false) ? 1 : 0;
-	if (p.iterThreshMode || p.interiorMode) flags |= MBX_FRACT_INTERIOR;
 	// sCommonParams::foldings (fractparams.cpp:346-351) and slow shading (fractparams.cpp:218)
 	p.boxFolding = M.b("box_folding", false) ? 1 : 0;
 	p.boxFoldingLimit = M.d("box_folding_limit", 1.0);

@@ -84,6 +84,10 @@ struct FrameConst
 	float farR2;               // march points with |p|^2 >= farR2 take the clamped step 3 without a
 	                           // DE evaluation (+inf: never)
 	float smoothness;
+	double smoothnessD;
+	// interior mode and iteration-threshold mode (general kernel only; calculate_distance.cpp:561-600,
+	// render_worker.cpp:653-679, 809-812, 913-933)
+	int32_t interiorMode, iterThreshMode;
 	int32_t maxRaymarchingSteps;
 	int32_t minN;
 	// advanced quality (render_worker.cpp:671-675, 817-834): threshold and step limits
@@ -138,6 +142,7 @@ struct FrameConst
 	// light #1, directional (light.cpp:80-195, shader_light_shading.cpp:40-122)
 	int32_t lightEnabled, castShadows, penetrating, softShadows;
 	float lightDir[3];
+	double lightDirD[3];
 	float lightIntensity, lightVisibility, lightSize, lightContourSharpness, softRange;
 	float lightColor[3];
 	float viewDistanceMaxLight;
@@ -183,6 +188,7 @@ struct FrameConst
 	float aoColor[3];
 	float aoFastTune;
 	float aoVectors[MBX_MAX_AO_VECTORS][3];
+	double aoVectorsD[MBX_MAX_AO_VECTORS][3]; // the same in FP64 for the FP64 kinds
 	float aoColors[MBX_MAX_AO_VECTORS][3];
 
 	// background (shader_background.cpp:39-200)

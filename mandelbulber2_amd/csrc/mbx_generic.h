@@ -519,11 +519,14 @@ __device__ __forceinline__ const FractalParams &gen_step(const FrameConst &f, in
 }
 
 // Compute<calcModeNormal / calcModeDeltaDE1> (compute_fractal.cpp:67-654): the orbit of p with
-// at most maxN iterations; returns the iteration count (i + 1) and leaves z and aux
-__device__ __forceinline__ int gen_orbit(const FrameConst &f, DV3 p, int maxN, DV4 &zOut, ExtendedAux &a)
+// at most maxN iterations; returns the iteration count (i + 1) and leaves z and aux.  maxiter is
+// Compute's flag: set unless the orbit bailed out (a non-finite z also leaves it set).
+__device__ __forceinline__ int gen_orbit(const FrameConst &f, DV3 p, int maxN, DV4 &zOut, ExtendedAux &a,
+	bool &maxiter)
 {
 	DV4 z = dv4(p.x, p.y, p.z, f.genInitialW);
 	init_aux(a, z, f.gen[f.genSeq[0]].mbScale);
+	maxiter = true;
 	int i;
 	for (i = 0; i < maxN; i++)
 	{
@@ -535,26 +538,48 @@ __device__ __forceinline__ int gen_orbit(const FrameConst &f, DV3 p, int maxN, D
 			a.r = len4(z);
 			break;
 		}
-		if (s.checkBailout && a.r > s.bailout) break;
+		if (s.checkBailout && a.r > s.bailout)
+		{
+			maxiter = false;
+			break;
+		}
 	}
 	zOut = z;
 	return i + 1;
 }
+__device__ __forceinline__ int gen_orbit(const FrameConst &f, DV3 p, int maxN, DV4 &zOut, ExtendedAux &a)
+{
+	bool maxiter;
+	return gen_orbit(f, p, maxN, zOut, a, maxiter);
+}
+
+// What CalculateDistanceSimple needs of the general kernel's orbits: the DE, the first orbit's
+// iteration count and maxiter flag, the last orbit's count (the delta DE's minN rule reads the
+// z-offset orbit, calculate_distance.cpp:727) and the iterations done
+struct GenDistance
+{
+	double d;
+	int iters, itersLast, maxiter;
+	uint32_t work;
+};
 
 // The distance of the sequence at the fractal-local point p: the analytic finish
 // (compute_fractal.cpp:493-617) or the reference's 4-orbit delta DE (calculate_distance.cpp:609-711).
 // One out-of-line copy serves every call site of the kernels, and the delta DE's four orbits
 // (p, then p + delta along x, y, z with the first orbit's iteration count as their limit) run
-// through one copy of the orbit loop: the code and the registers of one orbit.
-__device__ __attribute__((noinline)) double compute_generic(DV3 p, const FrameConst &frame, double detailSize,
-	int *itersOut, uint32_t *work)
+// through one copy of the orbit loop: the code and the registers of one orbit.  normalCalc: the
+// normal taps of iteration-threshold mode run 5 x maxiter (calculate_distance.cpp:541).
+__device__ __attribute__((noinline)) GenDistance compute_generic(DV3 p, const FrameConst &frame, double detailSize,
+	int normalCalc)
 {
 #pragma clang fp contract(off)
 	const FrameConst &f = uniform_ref(frame);
 	const bool analytic = f.genDEType == MBX_DE_ANALYTIC;
 	const double deltaDE = analytic ? 0.0 : fmax(sqrt(ddot(p, p)) * 1e-14, f.deltaRel * detailSize);
 	const int orbits = analytic ? 1 : 4;
-	int iters = 0;
+	const int maxN = f.formulaMaxiter * ((normalCalc && f.iterThreshMode) ? 5 : 1);
+	int iters = 0, itersLast = 0;
+	bool maxiterFirst = true;
 	uint32_t total = 0;
 	double r = 0.0, DE = 0.0, r3 = 0.0, drSq = 0.0;
 #pragma unroll 1
@@ -566,12 +591,15 @@ __device__ __attribute__((noinline)) double compute_generic(DV3 p, const FrameCo
 		if (k == 3) q.z = p.z + deltaDE;
 		ExtendedAux a;
 		DV4 z;
-		const int it = gen_orbit(f, q, k == 0 ? f.formulaMaxiter : iters, z, a);
+		bool maxiter;
+		const int it = gen_orbit(f, q, k == 0 ? maxN : iters, z, a, maxiter);
 		total += (uint32_t)it;
+		itersLast = it;
 		const double len3 = sqrt(z.x * z.x + z.y * z.y + z.z * z.z);
 		if (k == 0)
 		{
 			iters = it;
+			maxiterFirst = maxiter;
 			r = a.r;
 			DE = a.DE;
 			r3 = len3;
@@ -582,28 +610,40 @@ __device__ __attribute__((noinline)) double compute_generic(DV3 p, const FrameCo
 			drSq = drSq + dr * dr; // dr1^2 + dr2^2 + dr3^2, summed left to right
 		}
 	}
-	*itersOut = iters;
-	*work += total;
+	GenDistance o;
+	o.iters = iters;
+	o.itersLast = analytic ? iters : itersLast;
+	o.maxiter = maxiterFirst ? 1 : 0;
+	o.work = total;
 	if (analytic)
 	{
-		if (!(DE > 0.0)) return r;
-		if (f.genHybrid)
+		if (!(DE > 0.0))
+			o.d = r;
+		else if (f.genHybrid)
 		{
-			if (f.genDEFunction == MBX_DE_FUNCTION_LINEAR) return (r - f.genLinearDEOffset) / DE;
-			return (r > 1.0) ? 0.5 * r * log(r) / DE : 0.0;
+			if (f.genDEFunction == MBX_DE_FUNCTION_LINEAR)
+				o.d = (r - f.genLinearDEOffset) / DE;
+			else
+				o.d = (r > 1.0) ? 0.5 * r * log(r) / DE : 0.0;
 		}
-		switch (f.genAnalytic)
+		else
 		{
-			case MBX_ANALYTIC_LOGARITHMIC: return (r > 1.0) ? 0.5 * r * log(r) / DE : 0.0;
-			case MBX_ANALYTIC_LINEAR: return r / DE;
-			case MBX_ANALYTIC_IFS: return (r - 2.0) / DE;
-			default: return r;
+			switch (f.genAnalytic)
+			{
+				case MBX_ANALYTIC_LOGARITHMIC: o.d = (r > 1.0) ? 0.5 * r * log(r) / DE : 0.0; break;
+				case MBX_ANALYTIC_LINEAR: o.d = r / DE; break;
+				case MBX_ANALYTIC_IFS: o.d = (r - 2.0) / DE; break;
+				default: o.d = r; break;
+			}
 		}
+		return o;
 	}
 	const double dr = sqrt(drSq);
 	if (dr > 0.0)
-		return (f.genDEFunction == MBX_DE_FUNCTION_LOGARITHMIC) ? 0.5 * r3 * log(r3) / dr : 0.5 * r3 / dr;
-	return r3;
+		o.d = (f.genDEFunction == MBX_DE_FUNCTION_LOGARITHMIC) ? 0.5 * r3 * log(r3) / dr : 0.5 * r3 / dr;
+	else
+		o.d = r3;
+	return o;
 }
 
 // Compute<calcModeColouring> + CalculateColorIndex of the sequence (compute_fractal.cpp:343-436,

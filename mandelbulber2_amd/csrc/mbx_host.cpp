@@ -724,8 +724,6 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc, bool forceGe
 		return fail(ctx, MBX_ERR_INVALID, "image size out of range");
 	if (p->perspectiveType != 0)
 		return fail(ctx, MBX_ERR_UNSUPPORTED, "only the three-point perspective is supported");
-	if (p->iterThreshMode || p->interiorMode)
-		return fail(ctx, MBX_ERR_UNSUPPORTED, "iteration-threshold and interior modes are not supported");
 	if (p->maxRaymarchingSteps < 1) return fail(ctx, MBX_ERR_INVALID, "max_raymarching_steps < 1");
 	if (!(p->detailLevel > 0.0)) return fail(ctx, MBX_ERR_INVALID, "detail_level must be > 0");
 	const mbx_fractal &fr = p->fractal;
@@ -735,8 +733,9 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc, bool forceGe
 	// the specialised kinds cover one Mandelbulb, Mandelbox or IFS with w = 0; everything else
 	// takes the general FP64 path
 	// the general kernel also takes the Mandelbox with rotated folds
+	// and interior / iteration-threshold modes
 	const bool generic = forceGeneric || p->hybridCount >= 1 || fr.initialWAxis != 0.0 || p->boxFolding ||
-		p->sphericalFolding ||
+		p->sphericalFolding || p->interiorMode || p->iterThreshMode ||
 		(fr.formula == MBX_FORMULA_MANDELBOX && fr.mandelbox.rotationsEnabled) ||
 		(fr.formula != MBX_FORMULA_MANDELBULB && fr.formula != MBX_FORMULA_MANDELBOX &&
 			fr.formula != MBX_FORMULA_KALEIDOSCOPIC_IFS);
@@ -776,13 +775,17 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc, bool forceGe
 	fc.deltaKD = resolution * fov;
 	fc.threshK = (float)fc.threshKD;
 	fc.deltaK = (float)fc.deltaKD;
-	fc.constantDEThreshold = p->constantDEThreshold;
+	// iteration-threshold mode takes precedence over the constant threshold (render_worker.cpp:656-665)
+	fc.constantDEThreshold = (p->constantDEThreshold && !p->iterThreshMode) ? 1 : 0;
+	fc.interiorMode = p->interiorMode ? 1 : 0;
+	fc.iterThreshMode = p->iterThreshMode ? 1 : 0;
 	fc.DEThresh = (float)p->DEThresh;
 	fc.DEThreshD = p->DEThresh;
 	fc.DEFactor = (float)p->DEFactor;
 	fc.searchLimit = (float)(1.0 - 0.001 * p->detailLevel);
 	fc.viewDistanceMax = (float)p->viewDistanceMax;
 	fc.smoothness = (float)p->smoothness;
+	fc.smoothnessD = p->smoothness;
 	fc.maxRaymarchingSteps = p->maxRaymarchingSteps;
 	fc.minN = p->minN;
 	fc.advancedQuality = p->advancedQuality ? 1 : 0;
@@ -1002,13 +1005,17 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc, bool forceGe
 	fc.castShadows = L.castShadows;
 	fc.penetrating = L.penetrating;
 	set3f(fc.lightDir, ld);
+	fc.lightDirD[0] = ld.x;
+	fc.lightDirD[1] = ld.y;
+	fc.lightDirD[2] = ld.z;
 	fc.lightIntensity = (float)L.intensity;
 	fc.lightVisibility = (float)L.visibility;
 	fc.lightSize = (float)L.size;
 	fc.lightContourSharpness = (float)L.contourSharpness;
 	const double softRange = std::tan(L.softShadowCone / 180.0 * M_PI);
 	fc.softRange = (float)softRange;
-	fc.softShadows = softRange > 0.0;
+	// no soft shadows in iteration-threshold or interior mode (shader_aux_shadow.cpp:73-76)
+	fc.softShadows = softRange > 0.0 && !p->iterThreshMode && !p->interiorMode;
 	set3f(fc.lightColor, L.color);
 	fc.viewDistanceMaxLight = (float)p->viewDistanceMax;
 
@@ -1098,6 +1105,7 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc, bool forceGe
 	for (int i = 0; i < MBX_MAX_AO_VECTORS; i++)
 	{
 		set3f(fc.aoVectors[i], p->aoVectors[i]);
+		for (int k = 0; k < 3; k++) fc.aoVectorsD[i][k] = p->aoVectors[i][k];
 		set3f(fc.aoColors[i], p->aoColors[i]);
 	}
 

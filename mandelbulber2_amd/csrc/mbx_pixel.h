@@ -164,7 +164,9 @@ __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start
 		const int rnd = random_mod(seed, 1000);
 		if (kExact)
 		{
-			step = (dist - (R)0.5 * thr) * (R)f.DEFactorD * ((R)1 - (R)rnd / (R)10000);
+			// interior mode keeps a wider margin to the surface (render_worker.cpp:809-812)
+			const R margin = (KIND == kGeneric && f.interiorMode) ? (R)0.8 : (R)0.5;
+			step = (dist - margin * thr) * (R)f.DEFactorD * ((R)1 - (R)rnd / (R)10000);
 			step = limit_step(f, step, thr, adv);
 			scan += step / dirLength;
 		}
@@ -198,6 +200,16 @@ __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start
 			step *= (R)0.5;
 		}
 	}
+	if constexpr (KIND == kGeneric)
+	{
+		// iteration-threshold mode backs off by one threshold (render_worker.cpp:913-933)
+		if (f.iterThreshMode)
+		{
+			scan -= thr;
+			point = Q::along(start, dir, scan);
+			if (found) dist = calculate_distance<KIND>(point, thr, f, iters, w);
+		}
+	}
 	out.point = point;
 	out.depth = scan;
 	out.distThresh = thr;
@@ -208,43 +220,45 @@ __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start
 	out.stepCount = stepCount;
 }
 
-// CalculateNormals, fast mode: central differences of the DE (shader_calculate_normals.cpp:44-78)
+// CalculateNormals, fast mode: central differences of the DE (shader_calculate_normals.cpp:44-78),
+// normalised in the kind's precision
 template <int KIND>
-__device__ V3 calculate_normals(const FrameConst &frame, typename PrecOf<KIND>::P p, typename PrecOf<KIND>::R thr,
-	Work &w)
+__device__ typename PrecOf<KIND>::P calculate_normals(const FrameConst &frame, typename PrecOf<KIND>::P p,
+	typename PrecOf<KIND>::R thr, Work &w)
 {
 #pragma clang fp contract(off)
 	const FrameConst &f = uniform_ref(frame);
 	using Q = PrecOf<KIND>;
 	using R = typename Q::R;
 	using P = typename Q::P;
-	const R delta = thr * (R)f.smoothness;
+	// delta = distThresh * smoothness, a fifth of it in interior mode (shader_calculate_normals.cpp:46-47)
+	const R delta = (KIND == kGeneric && f.interiorMode) ? thr * (R)0.2 * Q::smoothness(f) : thr * Q::smoothness(f);
 	int it;
 	P e = p;
 	e.x = p.x + delta;
-	const R sx1 = calculate_distance<KIND>(e, thr, f, it, w);
+	const R sx1 = calculate_distance<KIND>(e, thr, f, it, w, true);
 	e.x = p.x - delta;
-	const R sx2 = calculate_distance<KIND>(e, thr, f, it, w);
+	const R sx2 = calculate_distance<KIND>(e, thr, f, it, w, true);
 	e = p;
 	e.y = p.y + delta;
-	const R sy1 = calculate_distance<KIND>(e, thr, f, it, w);
+	const R sy1 = calculate_distance<KIND>(e, thr, f, it, w, true);
 	e.y = p.y - delta;
-	const R sy2 = calculate_distance<KIND>(e, thr, f, it, w);
+	const R sy2 = calculate_distance<KIND>(e, thr, f, it, w, true);
 	e = p;
 	e.z = p.z + delta;
-	const R sz1 = calculate_distance<KIND>(e, thr, f, it, w);
+	const R sz1 = calculate_distance<KIND>(e, thr, f, it, w, true);
 	e.z = p.z - delta;
-	const R sz2 = calculate_distance<KIND>(e, thr, f, it, w);
+	const R sz2 = calculate_distance<KIND>(e, thr, f, it, w, true);
 	const R nx = sx1 - sx2, ny = sy1 - sy2, nz = sz1 - sz2;
-	V3 n;
+	P n;
 	if (nx == (R)0 && ny == (R)0 && nz == (R)0)
-		n = v3(1.0f, 0.0f, 0.0f);
+		n = Q::from(v3(1.0f, 0.0f, 0.0f));
 	else
 	{
-		const R k = (R)1 / Q::len(Q::from(v3(0.0f, 0.0f, 0.0f)) + P{nx, ny, nz});
-		n = v3((float)(nx * k), (float)(ny * k), (float)(nz * k));
+		const R k = (R)1 / Q::len(P{nx, ny, nz});
+		n = P{nx * k, ny * k, nz * k};
 	}
-	if (!finite3(n)) n = v3(1.0f, 0.0f, 0.0f);
+	if (!finite3(Q::tof(n))) n = Q::from(v3(1.0f, 0.0f, 0.0f));
 	return n;
 }
 
@@ -253,15 +267,15 @@ __device__ V3 calculate_normals(const FrameConst &frame, typename PrecOf<KIND>::
 // accumulate 0.2 in FP64 as the reference's do) of orbits at p + q delta, normalised, with the
 // reference's fallback (1, 0, 0).  Out of line: 1331 orbits per hit, only for scenes asking for it.
 template <int KIND>
-__device__ __attribute__((noinline)) V3 slow_normals(const FrameConst &frame, typename PrecOf<KIND>::P p,
-	double delta, Work &w)
+__device__ __attribute__((noinline)) typename PrecOf<KIND>::P slow_normals(const FrameConst &frame,
+	typename PrecOf<KIND>::P p, double delta, Work &w)
 {
 #pragma clang fp contract(off)
 	const FrameConst &f = uniform_ref(frame);
 	using Q = PrecOf<KIND>;
 	using R = typename Q::R;
 	using P = typename Q::P;
-	const double d = delta * (double)f.smoothness;
+	const double d = delta * f.smoothnessD;
 	double nx = 0.0, ny = 0.0, nz = 0.0;
 	for (double qx = -1.0; qx <= 1.0; qx += 0.2)
 		for (double qy = -1.0; qy <= 1.0; qy += 0.2)
@@ -276,42 +290,47 @@ __device__ __attribute__((noinline)) V3 slow_normals(const FrameConst &frame, ty
 				ny = ny + qy * pseudo;
 				nz = nz + qz * pseudo;
 			}
-	if (fabs(nx) < 1e-50 && fabs(ny) < 1e-50 && fabs(nz) < 1e-50) return v3(1.0f, 0.0f, 0.0f);
+	if (fabs(nx) < 1e-50 && fabs(ny) < 1e-50 && fabs(nz) < 1e-50) return Q::from(v3(1.0f, 0.0f, 0.0f));
 	const double inv = 1.0 / sqrt(nx * nx + ny * ny + nz * nz);
-	const V3 n = v3((float)(nx * inv), (float)(ny * inv), (float)(nz * inv));
-	return finite3(n) ? n : v3(1.0f, 0.0f, 0.0f);
+	const P n = Q::fromD(nx * inv, ny * inv, nz * inv);
+	return finite3(Q::tof(n)) ? n : Q::from(v3(1.0f, 0.0f, 0.0f));
 }
 
 // AuxShadow for the directional light (shader_aux_shadow.cpp:39-323): hard, or the soft
 // cone variant when soft_shadow_cone > 0
 template <int KIND>
 __device__ float aux_shadow(const FrameConst &f, typename PrecOf<KIND>::P p, typename PrecOf<KIND>::R thr,
-	typename PrecOf<KIND>::R distance, V3 lvf, Work &w)
+	typename PrecOf<KIND>::R distance, Work &w)
 {
 	using Q = PrecOf<KIND>;
 	using R = typename Q::R;
 	using P = typename Q::P;
-	const P lv = Q::from(lvf);
+	const P lv = Q::lightDir(f);
 	R shadowTemp = 1;
 	R maxSoft = 0;
 	R step = thr;
 	int count = 0;
 	int it;
-	for (R i = thr; i < distance; i += step)
+	// interior mode starts one DE-factor threshold out (shader_aux_shadow.cpp:53-54)
+	const R start = (KIND == kGeneric && f.interiorMode) ? thr * Q::DEFactor(f) : thr;
+	// the loop compares with the threshold as a float (shader_aux_shadow.cpp:98-105); it
+	// matters where the DE returns the threshold itself (interior mode)
+	const R thrF = (R)(float)thr;
+	for (R i = start; i < distance; i += step)
 	{
 		const P p2 = p + lv * i;
 		const R dist = calculate_distance<KIND>(p2, thr, f, it, w);
 		if (f.softShadows)
 		{
-			R angle = (dist - thr) / i;
-			if (angle < (R)0 || dist < thr) angle = 0;
+			R angle = (dist - thrF) / i;
+			if (angle < (R)0 || dist < thrF) angle = 0;
 			R soft = (R)1 - angle / (R)f.softRange;
 			if (f.penetrating) soft *= (distance - i) / distance;
 			soft = soft < (R)0 ? (R)0 : soft;
 			maxSoft = soft > maxSoft ? soft : maxSoft;
 		}
 		shadowTemp = 1;
-		if (dist < thr)
+		if (dist < thrF)
 		{
 			if (f.penetrating)
 			{
@@ -322,7 +341,7 @@ __device__ float aux_shadow(const FrameConst &f, typename PrecOf<KIND>::P p, typ
 				shadowTemp = 0;
 			break;
 		}
-		step = (dist < (R)1e6 ? dist : (R)1e6) * (R)f.DEFactor;
+		step = (dist < (R)1e6 ? dist : (R)1e6) * Q::DEFactor(f);
 		step = step > (R)1e-15 ? step : (R)1e-15;
 		count++;
 		if (count > f.maxRaymarchingSteps) break;
@@ -350,7 +369,7 @@ __device__ V3 ambient_occlusion(const FrameConst &f, typename PrecOf<KIND>::P p,
 	int it;
 	for (int i = start; i <= end; i++)
 	{
-		const P v = Q::from(v3(f.aoVectors[i][0], f.aoVectors[i][1], f.aoVectors[i][2]));
+		const P v = Q::aoVector(f, i);
 		R shadowTemp = 1;
 		R dist;
 		for (R r = thr; r < end_dist; r += dist * (R)2)
@@ -371,13 +390,11 @@ __device__ V3 ambient_occlusion(const FrameConst &f, typename PrecOf<KIND>::P p,
 
 // FastAmbientOcclusion (shader_fast_ambient_occlusion.cpp:40-66)
 template <int KIND>
-__device__ float fast_ambient_occlusion(const FrameConst &f, typename PrecOf<KIND>::P p, V3 nf,
-	typename PrecOf<KIND>::R thr, Work &w)
+__device__ float fast_ambient_occlusion(const FrameConst &f, typename PrecOf<KIND>::P p,
+	typename PrecOf<KIND>::P n, typename PrecOf<KIND>::R thr, Work &w)
 {
 	using Q = PrecOf<KIND>;
 	using R = typename Q::R;
-	using P = typename Q::P;
-	const P n = Q::from(nf);
 	float aoTemp = 0.0f;
 	R lastDist = (R)1e20;
 	const int q2 = f.aoQuality * f.aoQuality;
@@ -472,17 +489,19 @@ __device__ __forceinline__ float specular_shade(V3 n, V3 view, V3 lv, float expo
 // direction; the highlight (already shadowed) goes to spec.  FULL = false compiles the shadow,
 // AO and specular paths out (frames that use none of them), which lowers the kernel's registers.
 template <int KIND, bool FULL = true>
-__device__ V3 object_shader(const FrameConst &f, typename PrecOf<KIND>::P p, V3 n, typename PrecOf<KIND>::R thr,
-	typename PrecOf<KIND>::R delta, V3 view, V3 colour, V3 &spec, uint32_t &seed, Work &w)
+__device__ V3 object_shader(const FrameConst &f, typename PrecOf<KIND>::P p, typename PrecOf<KIND>::P nP,
+	typename PrecOf<KIND>::R thr, typename PrecOf<KIND>::R delta, V3 view, V3 colour, V3 &spec, uint32_t &seed,
+	Work &w)
 {
 	using R = typename PrecOf<KIND>::R;
+	const V3 n = PrecOf<KIND>::tof(nP);
 	spec = v3(0.0f, 0.0f, 0.0f);
 	V3 ambient = v3(0.0f, 0.0f, 0.0f);
 	if (FULL && f.aoEnabled)
 	{
 		if (f.aoMode == MBX_AO_FAST)
 		{
-			const float a = fast_ambient_occlusion<KIND>(f, p, n, thr, w);
+			const float a = fast_ambient_occlusion<KIND>(f, p, nP, thr, w);
 			ambient = v3(a, a, a);
 		}
 		else
@@ -520,7 +539,7 @@ __device__ V3 object_shader(const FrameConst &f, typename PrecOf<KIND>::P p, V3 
 			if (shade > 0.001f || specMax > 0.001f)
 			{
 				const R distance = f.penetrating ? delta * (R)f.height : (R)f.viewDistanceMaxLight;
-				shadow = aux_shadow<KIND>(f, p, thr, distance, lv, w);
+				shadow = aux_shadow<KIND>(f, p, thr, distance, w);
 				specular = specular * shadow;
 			}
 			else
@@ -542,7 +561,7 @@ __device__ V3 object_shader(const FrameConst &f, typename PrecOf<KIND>::P p, V3 
 // GlobalIlumination (shader_global_illumination.cpp:41-209).  colour0 is the primary hit's
 // surface colour; each bounce is tinted by the colour of the surface it left.
 template <int KIND>
-__device__ V3 global_illumination(const FrameConst &frame, typename PrecOf<KIND>::P p0, V3 n0,
+__device__ V3 global_illumination(const FrameConst &frame, typename PrecOf<KIND>::P p0, typename PrecOf<KIND>::P n0,
 	typename PrecOf<KIND>::R thr0, V3 colour0, uint32_t &seed, Work &w)
 {
 	const FrameConst &f = uniform_ref(frame);
@@ -551,23 +570,38 @@ __device__ V3 global_illumination(const FrameConst &frame, typename PrecOf<KIND>
 	using P = typename Q::P;
 	V3 out = v3(0.0f, 0.0f, 0.0f);
 	P p = p0;
-	V3 n = n0;
+	P n = n0;
 	R thrCur = thr0;
 	V3 colourTemp = colour0;
 	V3 newColour = colour0;
 	for (int depth = 0; depth < f.reflectionsMax; depth++)
 	{
-		const float rx = (float)(random_mod(seed, 20000) - 10000) * 1e-4f;
-		const float ry = (float)(random_mod(seed, 20000) - 10000) * 1e-4f;
-		const float rz = (float)(random_mod(seed, 20000) - 10000) * 1e-4f;
-		const V3 dirf = normalize(n + v3(rx * 1.2f, ry * 1.2f, rz * 1.2f));
-		const P dir = Q::from(dirf);
+		P dir;
+		if constexpr (PosPrecision<KIND>::kFP64)
+		{
+			// the reference's FP64 bounce direction (shader_global_illumination.cpp:63-71)
+#pragma clang fp contract(off)
+			const double rx = (double)(random_mod(seed, 20000) - 10000) / 10000.0;
+			const double ry = (double)(random_mod(seed, 20000) - 10000) / 10000.0;
+			const double rz = (double)(random_mod(seed, 20000) - 10000) / 10000.0;
+			const P v = P{n.x + rx * 1.2, n.y + ry * 1.2, n.z + rz * 1.2};
+			const double norm = 1.0 / Q::len(v);
+			dir = P{v.x * norm, v.y * norm, v.z * norm};
+		}
+		else
+		{
+			const float rx = (float)(random_mod(seed, 20000) - 10000) * 1e-4f;
+			const float ry = (float)(random_mod(seed, 20000) - 10000) * 1e-4f;
+			const float rz = (float)(random_mod(seed, 20000) - 10000) * 1e-4f;
+			dir = Q::from(normalize(n + v3(rx * 1.2f, ry * 1.2f, rz * 1.2f)));
+		}
+		const V3 dirf = Q::tof(dir);
 		R dist = 0;
 		bool found = false;
 		int stepCount = 0;
 		int it;
 		P hit = p;
-		for (R scan = thrCur; scan < (R)f.viewDistanceMax; scan += dist * (R)f.DEFactor)
+		for (R scan = thrCur; scan < (R)f.viewDistanceMax; scan += dist * Q::DEFactor(f))
 		{
 			if (stepCount >= f.maxRaymarchingSteps) break;
 			const P q = p + dir * scan;
@@ -589,7 +623,9 @@ __device__ V3 global_illumination(const FrameConst &frame, typename PrecOf<KIND>
 		{
 			p = hit;
 			// the bounce's shader input has delta = distThresh (shader_global_illumination.cpp:86)
-			n = f.slowShading ? slow_normals<KIND>(f, p, (double)thrCur, w) : calculate_normals<KIND>(f, p, thrCur, w);
+			n = f.slowShading ? slow_normals<KIND>(f, p, (KIND == kGeneric && f.interiorMode) ? (double)thrCur * 0.2
+																								 : (double)thrCur, w)
+							  : calculate_normals<KIND>(f, p, thrCur, w);
 			const V3 colour = surface_colour<KIND>(f, p, w);
 			V3 spec;
 			result = object_shader<KIND>(f, p, n, thrCur, thrCur, dirf, colour, spec, seed, w);
@@ -646,8 +682,9 @@ struct PixelOut
 };
 
 // MonteCarloDOFNoiseEstimation (render_worker.cpp:1871-1901) after sample `rep` with the running
-// sum: channels in float as sRGBFloat holds them, the noise in double
-__device__ __forceinline__ double mc_noise(V3 pixel, int rep, V3 sum, V3 &stdDevSum)
+// sum: channels in float as sRGBFloat holds them.  mc_deviation accumulates every sample;
+// mc_noise, the estimate in double, is only needed once the pixel may stop.
+__device__ __forceinline__ void mc_deviation(V3 pixel, int rep, V3 sum, V3 &stdDevSum)
 {
 #pragma clang fp contract(off)
 	const float k = (float)(rep + 1);
@@ -655,6 +692,10 @@ __device__ __forceinline__ double mc_noise(V3 pixel, int rep, V3 sum, V3 &stdDev
 	stdDevSum.x += (pixel.x - aR) * (pixel.x - aR);
 	stdDevSum.y += (pixel.y - aG) * (pixel.y - aG);
 	stdDevSum.z += (pixel.z - aB) * (pixel.z - aB);
+}
+__device__ __attribute__((noinline)) double mc_noise(int rep, V3 sum, V3 stdDevSum)
+{
+#pragma clang fp contract(off)
 	double total = sqrt((double)(stdDevSum.x + stdDevSum.y + stdDevSum.z) / ((double)rep + 1.0));
 	total /= sqrt((double)(rep + 1));
 	const double brightness = (double)(sum.x + sum.y + sum.z) / ((double)rep + 1.0);
@@ -703,16 +744,19 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 		if (m.found)
 		{
 			const R delta = calc_delta<KIND>(f, m.point);
+			P nP;
 			if (FULL && f.slowShading)
-				n = slow_normals<KIND>(f, m.point, (double)delta, w);
+				nP = slow_normals<KIND>(f, m.point,
+					(KIND == kGeneric && f.interiorMode) ? (double)m.distThresh * 0.2 : (double)delta, w);
 			else
-				n = calculate_normals<KIND>(f, m.point, m.distThresh, w);
+				nP = calculate_normals<KIND>(f, m.point, m.distThresh, w);
+			n = Q::tof(nP);
 			const V3 colour = FULL ? surface_colour<KIND>(f, m.point, w)
 								   : v3(f.surfaceColor[0], f.surfaceColor[1], f.surfaceColor[2]);
 			V3 spec;
-			rgb = object_shader<KIND, FULL>(f, m.point, n, m.distThresh, delta, Q::tof(dir), colour, spec, seed, w);
+			rgb = object_shader<KIND, FULL>(f, m.point, nP, m.distThresh, delta, Q::tof(dir), colour, spec, seed, w);
 			if (FULL && f.monteCarlo && f.globalIllumination)
-				rgb = rgb + global_illumination<KIND>(f, m.point, n, m.distThresh, colour, seed, w);
+				rgb = rgb + global_illumination<KIND>(f, m.point, nP, m.distThresh, colour, seed, w);
 			if (FULL) rgb = rgb + spec; // resultShader = objectShader + specular (render_worker.cpp:1458-1462)
 			rgb = v3(fmaxf(rgb.x, 0.0f), fmaxf(rgb.y, 0.0f), fmaxf(rgb.z, 0.0f));
 		}
@@ -728,10 +772,14 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 		}
 		po.depth = m.found ? (float)m.depth : 1e20f;
 		po.steps += m.steps;
-		if (FULL && f.monteCarlo && mc_noise(rgb, rep, acc, stdDevSum) < f.DOFMaxNoise01 && rep > f.DOFMinSamples)
+		if (FULL && f.monteCarlo)
 		{
-			repeats = rep + 1; // adaptive stop (render_worker.cpp:436-449)
-			break;
+			mc_deviation(rgb, rep, acc, stdDevSum);
+			if (rep > f.DOFMinSamples && mc_noise(rep, acc, stdDevSum) < f.DOFMaxNoise01)
+			{
+				repeats = rep + 1; // adaptive stop (render_worker.cpp:436-449)
+				break;
+			}
 		}
 	}
 	po.rgb = repeats > 1 ? v3(acc.x / (float)repeats, acc.y / (float)repeats, acc.z / (float)repeats) : acc;

@@ -847,8 +847,6 @@ def load(text, trace=None):
     for k in ("brightness", "contrast", "gamma", "saturation"):
         out["image." + k] = struct.unpack("f", struct.pack("f", g(k)))[0]
     out["image.hdr"] = int(g("hdr"))
-    if g("iteration_threshold_mode") or g("interior_mode"):
-        flags |= FLAGS["interior"]
     # sCommonParams::foldings (fractparams.cpp:346-351), slow shading (fractparams.cpp:218)
     out["boxFolding"] = int(g("box_folding"))
     out["boxFoldingLimit"] = g("box_folding_limit")

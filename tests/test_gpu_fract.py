@@ -52,6 +52,9 @@ SCENES = [
     "folded_mender_sponge.fract",
     # slow-shading normals (iteration counts of an 11^3 grid of orbits)
     "aexion05.fract",
+    # iteration-threshold mode: the surface is where orbits reach maxiter (general kernel)
+    "mandelbox8_maxiter.fract",
+    "glow.fract",
 ]
 
 

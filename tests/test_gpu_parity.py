@@ -374,12 +374,54 @@ def test_far_field_steps_leave_pixels_unchanged(rt, case):
     ctx.close()
 
 
+def _unit_ball(w, h, camera):
+    """The power-2 Mandelbulb in Julia mode with c = 0: its filled set is the unit ball."""
+    p = scene.mandelbulb8(w, h)
+    p.fractal.bulb.power = 2.0
+    p.fractal.juliaMode = 1
+    abi.set3(p.fractal.juliaC, (0.0, 0.0, 0.0))
+    abi.set3(p.camera, camera)
+    abi.set3(p.target, (0.0, 0.0, 0.0))
+    return p
+
+
+@pytest.mark.parametrize("mode", ["iteration_threshold", "interior"])
+def test_interior_and_iteration_threshold_parity(rt, oracle_mod, mode):
+    """Iteration-threshold mode (the march stops where orbits reach maxiter, then backs off one
+    threshold; the threshold drops the detail level) and interior mode (maxiter points measure
+    the detail size, the march keeps 0.8 thresholds, normal taps invert the DE, shadows start one
+    DE-factor threshold out) on the general kernel, with the hard shadow and fast AO on: the unit
+    ball of the z^2 Julia set, then the Mandelbox of the shadow scene.
+
+    Mask, iterations and depth: the FP64 bars.  Colour: in these modes the DE jumps between 0
+    (or the detail size) and 1.01 thresholds at the maxiter set, and a shadow ray crawls along
+    the surface in steps of 1.01 thresholds, so whether one of its samples lands inside the set
+    changes under perturbations of the ray at the last bit.  The oracle shows it on itself: turning
+    the light by 1e-13 degrees changes a few percent of its pixels (none in the ordinary mode).
+    The kernel may differ from the oracle on at most twice the oracle's own fraction."""
+    for p, name in ((_unit_ball(160, 120, (0.4, -2.6, 0.9)), "unit ball"),
+                    (scene.mandelbox_shadow_ao(160, 120, seed=2), "mandelbox")):
+        p.iterThreshMode = int(mode == "iteration_threshold")
+        p.interiorMode = int(mode == "interior")
+        p.detailLevel = 1.5
+        p.light.castShadows = 1
+        p.ambientOcclusionEnabled = 1
+        p.ambientOcclusionMode = abi.MBX_AO_FAST
+        g = rt.render_to_host(p)
+        assert g["de_kind"] == 6
+        r = oracle_mod.render(p)
+        st = compare(g, r)
+        p.light.rotation[0] += 1e-13
+        r2 = oracle_mod.render(p)
+        chaos = float((np.abs(r["rgb"] - r2["rgb"]).reshape(-1, 3).max(1) > 1e-2).mean())
+        _report("%s %s 160x120 (oracle self-sensitivity %.4f)" % (name, mode, chaos), st)
+        assert st["mask_agree"] >= 0.999 and st["iters_agree"] >= 0.995, st
+        assert "depth_rel_p999" in st and st["depth_rel_p999"] <= 1e-6, st
+        assert st["rgb_within_1e-2"] >= 1.0 - max(2.0 * chaos, 0.001), (st, chaos)
+
+
 def test_unsupported_settings_are_rejected(rt):
     ctx = rt.Context(0)
-    p = scene.mandelbulb8(64, 48)
-    p.interiorMode = 1
-    with pytest.raises(rt.MbxError, match="interior"):
-        ctx.upload(p)
     p = scene.mandelbulb8(64, 48)
     p.perspectiveType = 1
     with pytest.raises(rt.MbxError, match="perspective"):

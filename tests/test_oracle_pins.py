@@ -542,3 +542,96 @@ def test_adaptive_stop_takes_min_samples_plus_two():
     fixed.DOFSamples = 24
     fixed.DOFMinSamples = 30
     np.testing.assert_array_equal(tight["rgb"], oracle.render(fixed)["rgb"])
+
+
+# ------------------------------------------------------------------------------------------
+# iteration-threshold and interior modes (calculate_distance.cpp:561-600, render_worker.cpp:
+# 653-679, 809-812, 913-933, shader_calculate_normals.cpp:46-47)
+# ------------------------------------------------------------------------------------------
+def _unit_ball(camera, target):
+    """The power-2 Mandelbulb in Julia mode with c = 0 squares |z| at every iteration, so its
+    filled set (the points that never bail out) is the closed unit ball."""
+    p = scene.mandelbulb8(48, 32)
+    p.fractal.bulb.power = 2.0
+    p.fractal.juliaMode = 1
+    abi.set3(p.fractal.juliaC, (0.0, 0.0, 0.0))
+    abi.set3(p.camera, camera)
+    abi.set3(p.target, target)
+    return p
+
+
+def _sphere_entry(c, d):
+    d = d / np.linalg.norm(d)
+    b = float(np.dot(c, d))
+    disc = b * b - (float(np.dot(c, c)) - 1.0)
+    return None if disc < 0 else -b - math.sqrt(disc)
+
+
+def test_iteration_threshold_surface_is_the_unit_ball():
+    """In iteration-threshold mode the surface is where orbits reach maxiter: the march stops
+    on entering the unit ball, then backs off by one threshold, so depth + threshold is the
+    ray's entry distance to the ball; the threshold has no detail-level divisor."""
+    cam = np.array([0.0, -3.0, 0.0])
+    p = _unit_ball(cam, (0.0, 0.0, 0.0))
+    p.iterThreshMode = 1
+    p.detailLevel = 3.0
+    _, fov, _ = oracle.frame_constants(p)
+    hits = 0
+    for x, y in [(24, 16), (30, 20), (18, 12), (27, 9), (10, 5), (40, 28), (2, 2)]:
+        m = oracle.march_pixel(p, x, y)
+        t = _sphere_entry(cam, m["dir"])
+        assert m["found"] == (t is not None), (x, y)
+        if t is None:
+            continue
+        hits += 1
+        dl = float(np.linalg.norm(m["dir"]))
+        assert abs(m["depth"] * dl + m["thr"] - t) < 1e-6, (x, y, m, t)
+        # CalcDistThresh in this mode: |camera - point| * resolution * fov, detail level unused,
+        # taken at the point before the back-off (the surface, at distance t)
+        want = (m["depth"] * dl + m["thr"]) * (1.0 / p.imageHeight) * fov
+        assert m["thr"] == pytest.approx(want, rel=1e-12)
+        n = oracle.normal(p, m["point"], m["thr"])
+        assert float(np.dot(n, m["point"])) / float(np.linalg.norm(m["point"])) > 0.95
+    assert hits >= 4
+
+
+def test_interior_mode_keeps_interior_points_open_and_inverts_normals():
+    """Interior mode: a point inside the ball (an orbit that reaches maxiter) measures the
+    detail size instead of 0, and the normal-vector taps invert the DE near the surface
+    (detailSize - d), so the normal at a hit on the ball points into it."""
+    cam = np.array([0.0, -3.0, 0.0])
+    p = _unit_ball(cam, (0.0, 0.0, 0.0))
+    inside = np.array([[0.0, 0.2, 0.1], [0.5, -0.3, 0.2]])
+    d0, _ = oracle.distance(p, inside, 0.01)
+    assert (d0 == 0.0).all()
+    p.interiorMode = 1
+    d1, _ = oracle.distance(p, inside, 0.01)
+    assert (d1 == 0.01).all()
+    # an orbit cut short by maxiter counts as inside whatever its DE: with 2 iterations the point
+    # at radius 1.5 reaches |z| = 1.5^4 < the bailout 10, and its positive DE becomes 1e-3
+    p.fractal.formulaMaxiter = 2
+    _, _, mx, _ = oracle.compute(p, np.array([[0.0, 1.5, 0.0]]))
+    assert mx[0] == 1
+    p.interiorMode = 0
+    d2, _ = oracle.distance(p, np.array([[0.0, 1.5, 0.0]]), 1e-3)
+    p.interiorMode = 1
+    d3, _ = oracle.distance(p, np.array([[0.0, 1.5, 0.0]]), 1e-3)
+    assert d2[0] > 1e-3 and d3[0] == 1e-3
+    p.fractal.formulaMaxiter = 250
+    for x, y in [(24, 16), (30, 20), (18, 12)]:
+        m = oracle.march_pixel(p, x, y)
+        t = _sphere_entry(cam, m["dir"])
+        # the hit lies in the exterior band where the DE drops below the threshold, a few
+        # thresholds before the ball (the Julia set's log DE underestimates |p| - 1 by ~2)
+        assert m["found"] and 0.0 < t - m["depth"] * float(np.linalg.norm(m["dir"])) < 5 * m["thr"]
+    # 2% outside the ball every tap's DE is well below 0.9 x the threshold 0.05, so all six are
+    # inverted and the normal turns inwards; without interior mode it points outwards
+    rs = np.random.default_rng(4)
+    for _ in range(6):
+        u = rs.normal(size=3)
+        u /= np.linalg.norm(u)
+        p.interiorMode = 1
+        assert float(np.dot(oracle.normal(p, 1.02 * u, 0.05), u)) < -0.95
+        p.interiorMode = 0
+        assert float(np.dot(oracle.normal(p, 1.02 * u, 0.05), u)) > 0.95
+    p.interiorMode = 1
