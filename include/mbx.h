@@ -414,7 +414,8 @@ size_t mbx_sizeof_params(void);
 int mbx_ctx_create(int device, void *stream, mbx_ctx **out);
 void mbx_ctx_destroy(mbx_ctx *ctx);
 
-/* Last error message of the context (never NULL). */
+/* Last error message of the context (never NULL).  With a NULL context: why the last
+ * mbx_ctx_create on the calling thread failed. */
 const char *mbx_last_error(const mbx_ctx *ctx);
 
 /* Validate a parameter block and derive the per-frame constants (camera rotation, light

@@ -1500,6 +1500,16 @@ extern "C" {
 
 int mbx_version(void) { return MBX_ABI_VERSION; }
 
+// why the last mbx_ctx_create on this thread failed (mbx_last_error with a NULL context)
+static thread_local std::string g_createError = "null context";
+
+static int create_fail(mbx_ctx *ctx, int rc, const char *what, hipError_t e)
+{
+	g_createError = std::string("mbx_ctx_create: ") + what + ": " + hipGetErrorString(e);
+	if (ctx) mbx_ctx_destroy(ctx);
+	return rc;
+}
+
 int mbx_ctx_create(int device, void *stream, mbx_ctx **out)
 {
 	if (!out) return MBX_ERR_INVALID;
@@ -1511,26 +1521,21 @@ int mbx_ctx_create(int device, void *stream, mbx_ctx **out)
 	if (e != hipSuccess)
 	{
 		delete ctx;
-		return MBX_ERR_HIP;
+		return create_fail(nullptr, MBX_ERR_HIP, "hipSetDevice", e);
 	}
 	ctx->stream = (hipStream_t)stream;
 	if ((e = hipMalloc((void **)&ctx->d_frame, sizeof(FrameConst))) != hipSuccess ||
 		(e = hipMalloc((void **)&ctx->d_counters, 2 * mbx::kCounterBytes)) != hipSuccess ||
-		(e = hipMalloc((void **)&ctx->d_queue, mbx::kTileQueueBytes)) != hipSuccess ||
-		(e = hipEventCreate(&ctx->ev0)) != hipSuccess || (e = hipEventCreate(&ctx->ev1)) != hipSuccess ||
+		(e = hipMalloc((void **)&ctx->d_queue, mbx::kTileQueueBytes)) != hipSuccess)
+		return create_fail(ctx, MBX_ERR_HIP, "hipMalloc", e);
+	if ((e = hipEventCreate(&ctx->ev0)) != hipSuccess || (e = hipEventCreate(&ctx->ev1)) != hipSuccess ||
 		(e = hipEventCreateWithFlags(&ctx->naCopied, hipEventDisableTiming)) != hipSuccess)
-	{
-		mbx_ctx_destroy(ctx);
-		return MBX_ERR_HIP;
-	}
+		return create_fail(ctx, MBX_ERR_HIP, "hipEventCreate", e);
 	// the allocation-free batch path: this device's static scratch (its address is taken once per
 	// device, with the device current) and host staging for its largest batch
 	void *scratchAddr = nullptr;
 	if ((e = mbx::noalloc_scratch_address(device, &scratchAddr)) != hipSuccess)
-	{
-		mbx_ctx_destroy(ctx);
-		return MBX_ERR_HIP;
-	}
+		return create_fail(ctx, MBX_ERR_HIP, "hipGetSymbolAddress(scratch)", e);
 	ctx->d_scratch = (unsigned char *)scratchAddr;
 	ctx->naFrames.resize(mbx::kNoAllocMaxFrames);
 	ctx->naJobs.resize(mbx::kNoAllocMaxFrames);
@@ -1566,7 +1571,7 @@ void mbx_ctx_destroy(mbx_ctx *ctx)
 
 const char *mbx_last_error(const mbx_ctx *ctx)
 {
-	if (!ctx) return "null context";
+	if (!ctx) return g_createError.c_str();
 	return ctx->err.c_str();
 }
 

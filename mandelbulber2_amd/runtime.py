@@ -148,7 +148,7 @@ class Context:
         handle = C.c_void_p()
         rc = self.L.mbx_ctx_create(device, C.c_void_p(stream.cuda_stream), C.byref(handle))
         if rc != 0:
-            raise MbxError("mbx_ctx_create failed: %d" % rc)
+            raise MbxError("mbx_ctx_create failed: %d (%s)" % (rc, self.L.mbx_last_error(None).decode()))
         self.h = handle
         self.params = None
 
