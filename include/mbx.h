@@ -34,7 +34,7 @@ extern "C" {
 #define MBX_IFS_VECTOR_COUNT 9
 #define MBX_MANDELBOX_FOLDS 2
 /* upper bound of explicitly supplied ambient-occlusion directions */
-#define MBX_MAX_AO_VECTORS 64
+#define MBX_MAX_AO_VECTORS 1024 /* PrepareAOVectors keeps at most ~660 at the largest quality, 10 */
 /* upper bound of the stops of a colour gradient (the first colour counts twice: at its position
  * and again at 1.0) */
 #define MBX_MAX_GRADIENT_STOPS 64
@@ -374,6 +374,9 @@ typedef struct mbx_params {
 	int32_t DOFMinSamples;          /* "DOF_min_samples" (default 10) */
 	int32_t reserved5;
 	double DOFMaxNoise;             /* "DOF_max_noise" (default 1) */
+	/* "ao_light_map_rotation" (degrees, a single angle: fractparams.cpp:62-64 reads it as the x
+	 * of a vector, SetRotation turns it into a rotation about z) for mbx_prepare_ao_vectors */
+	double aoLightMapRotation;
 } mbx_params;
 
 /* Per-pixel output planes (device pointers; any may be NULL except where noted).
@@ -446,6 +449,15 @@ int mbx_render(mbx_ctx *ctx, const mbx_outputs *out);
  * cRenderWorker-free CalculateDistance (calculate_distance.cpp:60-64). */
 int mbx_distance(mbx_ctx *ctx, const double *points, int n, double detail_size,
 	float *dist_out, int32_t *iters_out);
+
+/* PrepareAOVectors (render_worker.cpp:599-639): the directions and colours of multi-ray AO from
+ * the lightmap texture ("file_lightmap", which the caller decodes: width x height RGB floats,
+ * row-major, each channel as cTexture holds it, the 8-bit value / 256).  NULL stands for the
+ * texture cTexture substitutes when the file does not load (5 x 5 white, texture.cpp:150-160).
+ * Fills params->aoVectors, aoColors and aoVectorsCount from params->ambientOcclusionQuality and
+ * params->aoLightMapRotation.  Host only.  MBX_ERR_UNSUPPORTED when more than
+ * MBX_MAX_AO_VECTORS directions would be kept, MBX_ERR_INVALID for bad arguments. */
+int mbx_prepare_ao_vectors(mbx_params *params, const float *lightmap, int width, int height);
 
 /* ---- formula plugins --------------------------------------------------------------------
  * The reference's formula contract is cAbstractFractal (formula/definition/abstract_fractal.h:
@@ -634,7 +646,9 @@ void mbx_params_default(mbx_params *p);
 #define MBX_FRACT_TEXTURES 0x200u      /* textured background, environment map, material textures */
 #define MBX_FRACT_AO_SCREEN_SPACE 0x400u /* screen-space AO (a post effect) */
 #define MBX_FRACT_AO_LIGHTMAP 0x800u   /* multi-ray AO: its directions and colours come from the
-                                          lightmap texture (aoVectorsCount is left 0) */
+                                          lightmap texture, which the reader cannot decode
+                                          (aoVectorsCount is left 0): pass the decoded
+                                          file_lightmap to mbx_prepare_ao_vectors */
 #define MBX_FRACT_POSTPROCESS 0x1000u  /* brightness, contrast, gamma, saturation, HDR, blur, post DOF */
 #define MBX_FRACT_MC_ADAPTIVE 0x2000u  /* (unused: the adaptive Monte Carlo stop is rendered) */
 #define MBX_FRACT_PERSPECTIVE 0x4000u  /* fish-eye or equirectangular projection, stereo */

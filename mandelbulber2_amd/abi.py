@@ -9,7 +9,7 @@ import ctypes as C
 
 MBX_IFS_VECTOR_COUNT = 9
 MBX_MANDELBOX_FOLDS = 2
-MBX_MAX_AO_VECTORS = 64
+MBX_MAX_AO_VECTORS = 1024
 MBX_MAX_GRADIENT_STOPS = 64
 MBX_MAX_FRACTALS = 9
 MBX_NOALLOC_MAX_FRAMES = 64  # frames of one mbx_render_batch_noalloc call
@@ -253,6 +253,7 @@ class MbxParams(C.Structure):
         ("DOFMinSamples", C.c_int32),
         ("reserved5", C.c_int32),
         ("DOFMaxNoise", C.c_double),
+        ("aoLightMapRotation", C.c_double),
     ]
 
 

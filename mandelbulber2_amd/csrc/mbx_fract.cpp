@@ -636,6 +636,7 @@ extern "C" void mbx_params_default(mbx_params *p)
 	p->N = 250;
 	p->DOFMinSamples = 10;
 	p->DOFMaxNoise = 1.0;
+	p->aoLightMapRotation = 0.0;
 	// palette colouring of material #1 (InitMaterialParams)
 	p->useColorsFromPalette = 1;
 	p->surfaceGradientEnable = 1;
@@ -1034,6 +1035,7 @@ extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, 
 	p.monteCarloGIRadianceLimit = M.d("MC_GI_radiance_limit", 10.0, 0.001, 1e10);
 	p.DOFMinSamples = (int32_t)M.i("DOF_min_samples", 10);
 	p.DOFMaxNoise = M.d("DOF_max_noise", 1.0, 0.00001, 100.0);
+	p.aoLightMapRotation = M.d("ao_light_map_rotation", 0.0, -180.0, 180.0); // initparameters.cpp:220
 	// Monte Carlo soft shadows jitter the light vector (shader_aux_shadow.cpp:76-87); chromatic
 	// aberration shifts the view vector by a random hue (render_worker.cpp:255-264)
 	if (p.DOFMonteCarlo && (M.b("MC_soft_shadows_enable", false) || M.b("DOF_MC_CA_enable", false)))

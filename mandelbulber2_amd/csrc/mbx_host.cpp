@@ -1917,6 +1917,56 @@ int mbx_get_stats(mbx_ctx *ctx, mbx_stats *stats)
 	return MBX_OK;
 }
 
+int mbx_prepare_ao_vectors(mbx_params *p, const float *lightmap, int width, int height)
+{
+	if (!p || (lightmap && (width <= 0 || height <= 0)) || p->ambientOcclusionQuality < 1) return MBX_ERR_INVALID;
+	static const float kWhite[5 * 5 * 3] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1,
+		1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1,
+		1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+	const float *tex = lightmap ? lightmap : kWhite;
+	const int W = lightmap ? width : 5, H = lightmap ? height : 5;
+	// mRotAmbientOcclusionLightMapRotation: SetRotation of (angle, 0, 0) in radians, a turn about z
+	Rot rot;
+	rot.z(p->aoLightMapRotation / 180.0 * M_PI);
+	const double q = p->ambientOcclusionQuality;
+	int count = 0;
+	// rows of constant latitude b, each sampled in longitude a with a step widened by 1 / cos(b);
+	// the lightmap pixel of (a, b) sets the colour and directions with a black pixel are dropped
+	// sin and cos of b, and of a + b, each from one sincos call, as the reference's build folds
+	// them (glibc's sincos may differ from sin / cos in the last bit)
+	for (double b = -0.49 * M_PI; b < 0.49 * M_PI; b += 1.0 / q)
+	{
+		double sb, cb;
+		::sincos(b, &sb, &cb);
+		for (double a = 0.0; a < 2.0 * M_PI; a += ((2.0 / q) / cb))
+		{
+			double sab, cab;
+			::sincos(a + b, &sab, &cab);
+			const D3 d = rot.apply(D3{cab * cb, sab * cb, sb});
+			const int X = int((a + b) / (2.0 * M_PI) * W + W * 8.5) % W;
+			const int Y = int(b / (M_PI)*H + H * 8.5) % H;
+			const float *c = tex + 3 * ((size_t)X + (size_t)Y * W);
+			if (c[0] > 0.001 || c[1] > 0.001 || c[2] > 0.001)
+			{
+				if (count >= MBX_MAX_AO_VECTORS) return MBX_ERR_UNSUPPORTED;
+				p->aoVectors[count][0] = d.x;
+				p->aoVectors[count][1] = d.y;
+				p->aoVectors[count][2] = d.z;
+				for (int k = 0; k < 3; k++) p->aoColors[count][k] = c[k];
+				count++;
+			}
+		}
+	}
+	if (count == 0)
+	{
+		// no lit direction: one null direction with no colour
+		for (int k = 0; k < 3; k++) p->aoVectors[0][k] = p->aoColors[0][k] = 0.0;
+		count = 1;
+	}
+	p->aoVectorsCount = count;
+	return MBX_OK;
+}
+
 int mbx_formula_count(void) { return kFormulaCount; }
 
 int mbx_formula_get(int index, mbx_formula_info *info)

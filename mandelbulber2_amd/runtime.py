@@ -18,7 +18,7 @@ EXPORTS = ["mbx_version", "mbx_sizeof_params", "mbx_ctx_create", "mbx_ctx_destro
            "mbx_get_stats", "mbx_render_batch", "mbx_tune", "mbx_get_tune", "mbx_autotune",
            "mbx_render_diff", "mbx_adjoint", "mbx_params_default", "mbx_load_fract",
            "mbx_compile_image", "mbx_write_png", "mbx_compile_zbuffer", "mbx_render_batch_noalloc",
-           "mbx_formula_count", "mbx_formula_get", "mbx_formula_iterate"]
+           "mbx_formula_count", "mbx_formula_get", "mbx_formula_iterate", "mbx_prepare_ao_vectors"]
 
 _lib = None
 
@@ -64,6 +64,7 @@ def load():
     L.mbx_compile_zbuffer.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                       C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_void_p]
     L.mbx_write_png.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
+    L.mbx_prepare_ao_vectors.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_int, C.c_int]
     L.mbx_formula_count.argtypes = []
     L.mbx_formula_count.restype = C.c_int
     L.mbx_formula_get.argtypes = [C.c_int, P(abi.MbxFormulaInfo)]
@@ -91,6 +92,24 @@ def load_fract(text):
     if rc != 0:
         raise MbxError("mbx_load_fract: %s (%d)" % (info.message.decode(errors="replace"), rc))
     return p, info
+
+
+def prepare_ao_vectors(params, lightmap=None):
+    """Multi-ray AO directions and colours from a decoded lightmap ((H, W, 3) floats, the 8-bit
+    value / 256 per channel), or from the white substitute with None (mbx_prepare_ao_vectors);
+    fills params in place and returns the direction count."""
+    L = load()
+    if lightmap is None:
+        rc = L.mbx_prepare_ao_vectors(C.byref(params), None, 0, 0)
+    else:
+        import numpy as np
+        t = np.ascontiguousarray(lightmap, np.float32)
+        if t.ndim != 3 or t.shape[2] != 3:
+            raise MbxError("prepare_ao_vectors: (H, W, 3) lightmap")
+        rc = L.mbx_prepare_ao_vectors(C.byref(params), t.ctypes.data_as(C.c_void_p), t.shape[1], t.shape[0])
+    if rc != 0:
+        raise MbxError("mbx_prepare_ao_vectors failed (%d)" % rc)
+    return params.aoVectorsCount
 
 
 def formulas():

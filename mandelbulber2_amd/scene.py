@@ -164,6 +164,7 @@ def default_params(width=640, height=480):
     p.N = 250
     p.DOFMinSamples = 10
     p.DOFMaxNoise = 1.0
+    p.aoLightMapRotation = 0.0
     # palette colouring of material #1 (InitMaterialParams, initparameters.cpp:1600-1810)
     p.useColorsFromPalette = 1
     p.surfaceGradientEnable = 1

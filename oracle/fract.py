@@ -169,6 +169,7 @@ def init_params():
     c.add("ambient_occlusion_enabled", BOOL, False)                 # :216
     c.add("ambient_occlusion_mode", INT, 2)                         # :217-218 (AOModeScreenSpace)
     c.add("ambient_occlusion_color", RGB, (65535, 65535, 65535))    # :219
+    c.add("ao_light_map_rotation", DOUBLE, 0.0, -180.0, 180.0)      # :220
     c.add("glow_enabled", BOOL, True)                               # :222
     c.add("glow_intensity", DOUBLE, 0.2, 0.0, 1e15)                 # :223
     c.add("glow_color_1", RGB, (40984, 44713, 49490))               # :240
@@ -879,6 +880,7 @@ def load(text, trace=None):
     out["monteCarloGIRadianceLimit"] = g("MC_GI_radiance_limit")
     out["DOFMinSamples"] = g("DOF_min_samples")         # the adaptive stop (render_worker.cpp:436-449)
     out["DOFMaxNoise"] = g("DOF_max_noise")
+    out["aoLightMapRotation"] = g("ao_light_map_rotation")
     # Monte Carlo soft shadows jitter the light vector (shader_aux_shadow.cpp:76-87);
     # chromatic aberration shifts the view vector by a random hue (render_worker.cpp:255-264)
     if g("DOF_monte_carlo") and (g("MC_soft_shadows_enable") or g("DOF_MC_CA_enable")):

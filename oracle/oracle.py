@@ -61,6 +61,7 @@ def lib(fast=False):
     L.mbxo_random_int.argtypes = [P(C.c_uint32)]
     L.mbxo_mc_noise.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
     L.mbxo_mc_noise.restype = None
+    L.mbxo_prepare_ao_vectors.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_int, C.c_int]
     L.mbxo_random.argtypes = [P(C.c_uint32), C.c_int]
     L.mbxo_pixel_seed.argtypes = [C.c_uint32, C.c_uint32]
     L.mbxo_pixel_seed.restype = C.c_uint32
@@ -124,6 +125,19 @@ def compute(p, pts):
     if rc != 0:
         raise RuntimeError("oracle compute failed: %d" % rc)
     return d, it, mx, z
+
+
+def prepare_ao_vectors(p, lightmap=None):
+    """PrepareAOVectors into p.aoVectors / aoColors / aoVectorsCount; lightmap (H, W, 3) float32
+    or None for the white substitute."""
+    if lightmap is None:
+        rc = lib().mbxo_prepare_ao_vectors(C.byref(p), None, 0, 0)
+    else:
+        t = np.ascontiguousarray(lightmap, np.float32)
+        rc = lib().mbxo_prepare_ao_vectors(C.byref(p), _ptr(t), t.shape[1], t.shape[0])
+    if rc != 0:
+        raise RuntimeError("oracle prepare_ao_vectors failed: %d" % rc)
+    return p.aoVectorsCount
 
 
 def mc_noise(samples):

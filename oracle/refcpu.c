@@ -28,6 +28,8 @@
  */
 #include <math.h>
 #include <stddef.h>
+
+void sincos(double x, double *s, double *c); /* GNU libm */
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -2164,6 +2166,59 @@ static double mc_noise(const float pixel[3], int repeat, const float sum[3], flo
 	const double brightness = (double)(sum[0] + sum[1] + sum[2]) / (repeat + 1.0);
 	if (brightness > 1.0) total /= brightness;
 	return total;
+}
+
+/* PrepareAOVectors (render_worker.cpp:599-639): latitude rows b in steps of 1/quality, longitude
+ * a in steps of (2/quality)/cos(b); direction (cos(a+b) cos b, sin(a+b) cos b, sin b) turned by
+ * mRotAmbientOcclusionLightMapRotation (SetRotation of (angle, 0, 0): about z,
+ * fractparams.cpp:62-64); colour from lightmap pixel X = int((a+b)/2pi W + 8.5 W) % W,
+ * Y = int(b/pi H + 8.5 H) % H; directions whose colour is at most 0.001 in every channel are
+ * dropped; none left: one zero direction with no colour.  tex NULL: cTexture's 5 x 5 white
+ * substitute (texture.cpp:150-160). */
+int mbxo_prepare_ao_vectors(mbx_params *p, const float *tex, int w, int h)
+{
+	static const float white[3] = {1.0f, 1.0f, 1.0f};
+	const int W = tex ? w : 5, H = tex ? h : 5;
+	rotm r;
+	v3 angles = V3(p->aoLightMapRotation / 180.0 * M_PI, 0.0, 0.0);
+	rot_set1(&r, angles);
+	int n = 0;
+	const double quality = p->ambientOcclusionQuality;
+	/* sin and cos of b, and of a + b, each from one sincos call: GCC compiles the reference's
+	 * loop so (-O2 and the release -O3 -ffast-math alike), and glibc's sincos may differ from
+	 * sin / cos in the last bit */
+	for (double b = -0.49 * M_PI; b < 0.49 * M_PI; b += 1.0 / quality)
+	{
+		double sb, cb;
+		sincos(b, &sb, &cb);
+		for (double a = 0.0; a < 2.0 * M_PI; a += ((2.0 / quality) / cb))
+		{
+			double sab, cab;
+			sincos(a + b, &sab, &cab);
+			v3 d = V3(cab * cb, sab * cb, sb);
+			d = rot_apply3(&r, d);
+			const int X = (int)((a + b) / (2.0 * M_PI) * W + W * 8.5) % W;
+			const int Y = (int)(b / (M_PI)*H + H * 8.5) % H;
+			const float *c = tex ? tex + 3 * ((size_t)X + (size_t)Y * W) : white;
+			if (!(c[0] > 0.001 || c[1] > 0.001 || c[2] > 0.001)) continue;
+			if (n >= MBX_MAX_AO_VECTORS) return MBX_ERR_UNSUPPORTED;
+			p->aoVectors[n][0] = d.x;
+			p->aoVectors[n][1] = d.y;
+			p->aoVectors[n][2] = d.z;
+			p->aoColors[n][0] = c[0];
+			p->aoColors[n][1] = c[1];
+			p->aoColors[n][2] = c[2];
+			n++;
+		}
+	}
+	if (n == 0)
+	{
+		memset(p->aoVectors[0], 0, sizeof(p->aoVectors[0]));
+		memset(p->aoColors[0], 0, sizeof(p->aoColors[0]));
+		n = 1;
+	}
+	p->aoVectorsCount = n;
+	return 0;
 }
 
 /* the noise estimate after each of n samples (3 floats each) of one pixel, accumulated as the
