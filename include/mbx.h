@@ -532,7 +532,7 @@ int mbx_render_batch(mbx_ctx *ctx, const mbx_params *frames, const mbx_outputs *
 
 /* mbx_render_batch without any device allocation: the same kernels and results, with the
  * frame constants, job table, tile queues and work counters in a static device array of
- * about 1.8 MiB per GPU (MBX_NOALLOC_MAX_FRAMES frame blocks), which each GPU the library runs on
+ * about 4.7 MiB per GPU (MBX_NOALLOC_MAX_FRAMES frame blocks), which each GPU the library runs on
  * holds until the process ends.  No counterpart in the reference: it is for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc); the context is created beforehand.  n must be 1 .. MBX_NOALLOC_MAX_FRAMES
