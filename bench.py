@@ -37,6 +37,8 @@ sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak (packed)
 FP64_PEAK_TFLOPS = 78.6   # MI355X published FP64 peak
+# FP64 FLOPs of the adjoint kernel per pixel gradient (issued lanes; 9.28e9 per 512x512 launch)
+ADJOINT_FP64_FLOP_PER_PIXEL = 35400.0
 # Algorithmic flop counts (fixed, independent of the instruction mix): per DE evaluation
 # (ray point, threshold, DE finish, step) and per formula iteration.
 F_DE = 25
@@ -378,7 +380,15 @@ def bench_adjoint(args, torch, runtime, wl):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": wl["dtype"],
         "data": "synthetic",
         "config": dict({"workload": "adjoint512", "width": 512, "height": 512}, **wl["config"]),
-        "roofline": None,
+        # the adjoint kernel against the FP64 peak: its FP64 FLOPs per pixel gradient counted by
+        # SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes (the counter's unit, calibrated on an FMA chain of
+        # known size); the render half is a march whose time is set by its longest rays
+        "roofline": {"bound": "valu", "kernel": "adjoint_kernel",
+                     "achieved": ADJOINT_FP64_FLOP_PER_PIXEL * n / (statistics.median(adj_ms) * 1e-3) / 1e12,
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": ADJOINT_FP64_FLOP_PER_PIXEL * n / (statistics.median(adj_ms) * 1e-3) / 1e12
+                     / FP64_PEAK_TFLOPS, "traffic": None,
+                     "flop_per_pixel": ADJOINT_FP64_FLOP_PER_PIXEL},
         "kernels_ms": {"render_diff": statistics.median(fwd_ms), "adjoint": statistics.median(adj_ms)},
         "gradient": [float(v) for v in g],
     }
