@@ -377,6 +377,19 @@ typedef struct mbx_params {
 	/* "ao_light_map_rotation" (degrees, a single angle: fractparams.cpp:62-64 reads it as the x
 	 * of a vector, SetRotation turns it into a rotation about z) for mbx_prepare_ao_vectors */
 	double aoLightMapRotation;
+	/* Monte Carlo lens, chromatic aberration and soft shadows, each only with DOFMonteCarlo:
+	 * MonteCarloDOF (render_worker.cpp:239-242, 1832-1844) moves the ray's origin on the lens and
+	 * aims it at the focal plane; the aberration (:255-264, 374-380) bends the view vector by a
+	 * random hue and tints the sample; soft shadows jitter the light vector inside the shadow
+	 * cone (shader_aux_shadow.cpp:56-87) */
+	int32_t DOFEnabled;                        /* "DOF_enabled" */
+	int32_t DOFMonteCarloChromaticAberration;  /* "DOF_MC_CA_enable" */
+	double DOFFocus;                           /* "DOF_focus" (default 6) */
+	double DOFRadius;                          /* "DOF_radius" (default 10) */
+	double DOFMonteCarloCACameraDispersion;    /* "DOF_MC_CA_camera_dispersion" (default 1; a float in
+	                                              sParamRender) */
+	int32_t monteCarloSoftShadows;             /* "MC_soft_shadows_enable" */
+	int32_t reserved6;
 } mbx_params;
 
 /* Per-pixel output planes (device pointers; any may be NULL except where noted).
@@ -659,7 +672,7 @@ void mbx_params_default(mbx_params *p);
 #define MBX_FRACT_INTERIOR 0x40000u    /* (unused: interior and iteration-threshold modes are rendered) */
 #define MBX_FRACT_DISABLED 0x80000u    /* the fractal node is disabled or absent: nothing to render */
 #define MBX_FRACT_ANTIALIASING 0x100000u /* antialiasing repeats (antialiasing_size^2 per pixel) */
-#define MBX_FRACT_MC_EFFECTS 0x200000u /* Monte Carlo soft shadows or chromatic aberration */
+#define MBX_FRACT_MC_EFFECTS 0x200000u /* (unused: Monte Carlo soft shadows and chromatic aberration are rendered) */
 #define MBX_FRACT_NEBULA 0x400000u     /* nebula mode (the OpenCL nebula engine renders the scene) */
 
 /* What mbx_load_fract found besides the parameter block. */

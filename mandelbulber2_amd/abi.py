@@ -254,6 +254,13 @@ class MbxParams(C.Structure):
         ("reserved5", C.c_int32),
         ("DOFMaxNoise", C.c_double),
         ("aoLightMapRotation", C.c_double),
+        ("DOFEnabled", C.c_int32),
+        ("DOFMonteCarloChromaticAberration", C.c_int32),
+        ("DOFFocus", C.c_double),
+        ("DOFRadius", C.c_double),
+        ("DOFMonteCarloCACameraDispersion", C.c_double),
+        ("monteCarloSoftShadows", C.c_int32),
+        ("reserved6", C.c_int32),
     ]
 
 

@@ -677,6 +677,14 @@ struct Prec<false>
 		const V3 vv = normalize(v3(ix * f.fov, 1.0f, iy * f.fov));
 		return normalize(rot(f.mRot, vv));
 	}
+	// CalculateViewVector before the march normalises it again, and the camera rotation of a vector
+	static __device__ __forceinline__ P view_raw(const FrameConst &f, R ix, R iy)
+	{
+		return rot(f.mRot, normalize(v3(ix * f.fov, 1.0f, iy * f.fov)));
+	}
+	static __device__ __forceinline__ P cam_rot(const FrameConst &f, P v) { return rot(f.mRot, v); }
+	static __device__ __forceinline__ P normalize_p(P v) { return normalize(v); }
+	static __device__ __forceinline__ R softRange(const FrameConst &f) { return f.softRange; }
 };
 
 template <>
@@ -738,6 +746,27 @@ struct Prec<true>
 			m[6] * vv.x + m[7] * vv.y + m[8] * vv.z);
 		return d * (1.0 / sqrt(ddot(d, d)));
 	}
+	static __device__ __forceinline__ P view_raw(const FrameConst &f, R ix, R iy)
+	{
+#pragma clang fp contract(off)
+		DV3 vv = dv3(ix * f.fovD, 1.0, iy * f.fovD);
+		vv = vv * (1.0 / sqrt(ddot(vv, vv)));
+		return cam_rot(f, vv);
+	}
+	// mRot.RotateVector (CMatrix33 times the vector, each row summed left to right)
+	static __device__ __forceinline__ P cam_rot(const FrameConst &f, P v)
+	{
+#pragma clang fp contract(off)
+		const double *m = f.mRotD;
+		return dv3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
+			m[6] * v.x + m[7] * v.y + m[8] * v.z);
+	}
+	static __device__ __forceinline__ P normalize_p(P v)
+	{
+#pragma clang fp contract(off)
+		return v * (1.0 / sqrt(ddot(v, v)));
+	}
+	static __device__ __forceinline__ R softRange(const FrameConst &f) { return f.softRangeD; }
 };
 
 // The tail of CalculateDistanceSimple with interior or iteration-threshold mode on

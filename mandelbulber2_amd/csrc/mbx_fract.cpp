@@ -637,6 +637,9 @@ extern "C" void mbx_params_default(mbx_params *p)
 	p->DOFMinSamples = 10;
 	p->DOFMaxNoise = 1.0;
 	p->aoLightMapRotation = 0.0;
+	p->DOFFocus = 6.0;
+	p->DOFRadius = 10.0;
+	p->DOFMonteCarloCACameraDispersion = 1.0;
 	// palette colouring of material #1 (InitMaterialParams)
 	p->useColorsFromPalette = 1;
 	p->surfaceGradientEnable = 1;
@@ -1036,10 +1039,13 @@ extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, 
 	p.DOFMinSamples = (int32_t)M.i("DOF_min_samples", 10);
 	p.DOFMaxNoise = M.d("DOF_max_noise", 1.0, 0.00001, 100.0);
 	p.aoLightMapRotation = M.d("ao_light_map_rotation", 0.0, -180.0, 180.0); // initparameters.cpp:220
-	// Monte Carlo soft shadows jitter the light vector (shader_aux_shadow.cpp:76-87); chromatic
-	// aberration shifts the view vector by a random hue (render_worker.cpp:255-264)
-	if (p.DOFMonteCarlo && (M.b("MC_soft_shadows_enable", false) || M.b("DOF_MC_CA_enable", false)))
-		flags |= MBX_FRACT_MC_EFFECTS;
+	// the Monte Carlo lens, chromatic aberration and soft shadows (fractparams.cpp:116-130, 180)
+	p.DOFEnabled = M.b("DOF_enabled", false) ? 1 : 0;
+	p.DOFFocus = M.d("DOF_focus", 6.0, 0.0, 10000.0);
+	p.DOFRadius = M.d("DOF_radius", 10.0, 0.0, 200.0);
+	p.DOFMonteCarloChromaticAberration = M.b("DOF_MC_CA_enable", false) ? 1 : 0;
+	p.DOFMonteCarloCACameraDispersion = M.d("DOF_MC_CA_camera_dispersion", 1.0, 1e-15, 1000.0);
+	p.monteCarloSoftShadows = M.b("MC_soft_shadows_enable", false) ? 1 : 0;
 	// the OpenCL nebula engine renders these scenes (render_job.cpp:664-673)
 	if (M.b("nebula_mode", false)) flags |= MBX_FRACT_NEBULA;
 	p.N = (int32_t)M.i("N", 250, 1, 99999); // params->N: slow shading's pseudo-distance (fractparams.cpp:189)
@@ -1155,7 +1161,8 @@ extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, 
 		flags |= MBX_FRACT_FOG;
 	if (M.b("textured_background", false) || M.b("env_mapping_enable", false)) flags |= MBX_FRACT_TEXTURES;
 	// brightness, contrast, gamma, saturation and hdr are read into params.image (mbx_compile_image)
-	if (M.b("hdr_blur_enabled", false) || M.b("DOF_enabled", false) ||
+	// DOF without Monte Carlo is the post-processing blur; with it, the lens of the pixel loop
+	if (M.b("hdr_blur_enabled", false) || (M.b("DOF_enabled", false) && !p.DOFMonteCarlo) ||
 		M.b("post_chromatic_aberration_enabled", false))
 		flags |= MBX_FRACT_POSTPROCESS;
 	if (doc.animation) flags |= MBX_FRACT_ANIMATION;

@@ -59,6 +59,12 @@ struct FractalParams
 // formula of each iteration: maxN * 5 entries with maxN = 250 (hybrid_fractal_sequences.cpp:294-298)
 constexpr int kGenSeqLength = 1250;
 
+// Random(max) draws 0 .. max inclusive: Monte Carlo lens angles are whole radians 0 .. 65536
+// (Random(65536), render_worker.cpp:1837); soft-shadow ball radii are the cube roots of
+// Random(10000) / 10000 (shader_aux_shadow.cpp:83), 10001 values
+constexpr int kLensAngles = 65536 + 1;
+constexpr int kSoftRadii = 10000 + 1;
+
 struct FrameConst
 {
 	// image and camera (render_worker.cpp:539-564, projection_3d.cpp:77-89)
@@ -199,6 +205,15 @@ struct FrameConst
 	// Monte Carlo
 	int32_t monteCarlo, samples, globalIllumination, reflectionsMax;
 	int32_t DOFMinSamples;  // adaptive stop: past this many samples ...
+	// the Monte Carlo lens (MonteCarloDOF), chromatic aberration and soft shadows
+	int32_t dofEnabled, chromaticAberration, mcSoftShadows, reservedMc;
+	double dofFocusD, dofRadiusD, softRangeD;
+	float caCameraDispersion;
+	// the context's tables of the random draws' transcendental images: lensSinCos[2k], [2k + 1] =
+	// sin k, cos k (k < kLensAngles radians) and softCbrt[k] = pow(k / 10000, 1/3) (k < kSoftRadii),
+	// indexed by the draws themselves
+	const double *lensSinCos;
+	const double *softCbrt;
 	double DOFMaxNoise01;   // ... once the noise estimate is below DOF_max_noise * 0.01
 	float giRadianceLimit;
 

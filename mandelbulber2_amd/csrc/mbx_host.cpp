@@ -61,7 +61,7 @@ struct mbx_ctx
 	int precision = 0; // Mandelbulb march precision (MBX_TUNE_PRECISION): 0 auto, 1 FP32, 2 FP64
 	int timing = 1;     // time renders with events (MBX_TUNE_TIMING)
 	bool timedLast = false; // the last launch was bracketed by ev0/ev1
-	int batchFull = 1; // the staged frames need the Monte Carlo / GI / shadow / AO code
+	int batchFull = 1; // kernel variant the staged frames need (mbx::kVariantLite .. kVariantLens)
 	// differentiable render
 	mbx::DiffConst diff{};
 	std::string diffWhy = "no scene uploaded"; // empty when the scene is differentiable
@@ -71,6 +71,8 @@ struct mbx_ctx
 	// image post-processing: the gamma table of the last gamma (cImage::CalculateGammaTable)
 	uint16_t *d_gamma = nullptr;
 	float gammaOf = 0.0f;
+	// sin / cos of the lens angles and the soft-shadow cube roots, computed once on the host
+	double *d_mcTables = nullptr;
 	bool gammaReady = false;
 	std::vector<uint16_t> gammaHost;
 	uint32_t *d_zrange = nullptr; // z-buffer image: the frame's depth range
@@ -86,6 +88,8 @@ struct mbx_ctx
 
 namespace
 {
+
+constexpr size_t kMcTableDoubles = 2 * (size_t)mbx::kLensAngles + (size_t)mbx::kSoftRadii;
 
 // M_PI_180 of mandelbulber2/src/fractal.h:46, the factor RecalculateFractalParams uses
 constexpr double kPi180 = 0.01745329251994329576923690768489;
@@ -968,7 +972,9 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc, bool forceGe
 	if (kind < 0) return fail(ctx, MBX_ERR_UNSUPPORTED, "formula / DE mode combination not supported");
 	fc.deKind = kind;
 	fc.farR2 = std::numeric_limits<float>::infinity();
-	if (ctx->farField && (kind == mbx::kBulb8 || kind == mbx::kBulbTrig) && !fc.julia && fc.addC && !p->advancedQuality)
+	// (not with the Monte Carlo lens: its rays start off the camera, outside the bound's premise)
+	if (ctx->farField && (kind == mbx::kBulb8 || kind == mbx::kBulbTrig) && !fc.julia && fc.addC && !p->advancedQuality &&
+		!(p->DOFMonteCarlo && p->DOFEnabled))
 	{
 		const double c = std::max(std::fabs(fr.constantFactor[0]),
 			std::max(std::fabs(fr.constantFactor[1]), std::fabs(fr.constantFactor[2])));
@@ -1015,7 +1021,10 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc, bool forceGe
 	const double softRange = std::tan(L.softShadowCone / 180.0 * M_PI);
 	fc.softRange = (float)softRange;
 	// no soft shadows in iteration-threshold or interior mode (shader_aux_shadow.cpp:73-76)
-	fc.softShadows = softRange > 0.0 && !p->iterThreshMode && !p->interiorMode;
+	// and none with Monte Carlo soft shadows, which jitter the light instead (:76-87)
+	fc.softShadows = softRange > 0.0 && !p->iterThreshMode && !p->interiorMode &&
+		!(p->DOFMonteCarlo && p->monteCarloSoftShadows);
+	fc.softRangeD = softRange;
 	set3f(fc.lightColor, L.color);
 	fc.viewDistanceMaxLight = (float)p->viewDistanceMax;
 
@@ -1121,6 +1130,15 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc, bool forceGe
 	fc.monteCarlo = p->DOFMonteCarlo;
 	fc.samples = p->DOFMonteCarlo ? p->DOFSamples : 1;
 	fc.DOFMinSamples = p->DOFMinSamples;
+	fc.dofEnabled = (p->DOFMonteCarlo && p->DOFEnabled) ? 1 : 0;
+	fc.chromaticAberration = (p->DOFMonteCarlo && p->DOFMonteCarloChromaticAberration) ? 1 : 0;
+	fc.mcSoftShadows = (p->DOFMonteCarlo && p->monteCarloSoftShadows) ? 1 : 0;
+	fc.dofFocusD = p->DOFFocus;
+	fc.dofRadiusD = p->DOFRadius;
+	fc.caCameraDispersion = (float)p->DOFMonteCarloCACameraDispersion;
+	fc.lensSinCos = ctx->d_mcTables;
+	fc.softCbrt = ctx->d_mcTables ? ctx->d_mcTables + 2 * mbx::kLensAngles : nullptr;
+	if (fc.dofEnabled && !(p->DOFFocus > 0.0)) return fail(ctx, MBX_ERR_INVALID, "DOF_focus must be > 0");
 	fc.DOFMaxNoise01 = p->DOFMaxNoise * 0.01;
 	if (fc.samples < 1) return fail(ctx, MBX_ERR_INVALID, "DOF_samples < 1");
 	fc.globalIllumination = p->DOFMonteCarloGlobalIllumination;
@@ -1204,12 +1222,15 @@ bool same_frames(const std::vector<FrameConst> &a, const std::vector<FrameConst>
 		(a.empty() || std::memcmp(a.data(), b.data(), a.size() * sizeof(FrameConst)) == 0);
 }
 
-// Whether a frame needs the kernels' Monte Carlo / GI / shadow / AO / palette / specular / glow
-// code.
-bool frame_needs_full(const FrameConst &f)
+// The kernel variant a frame needs: the Monte Carlo / GI / shadow / AO / palette / specular / glow
+// code (full), and the Monte Carlo lens or chromatic aberration besides (lens).
+int frame_variant(const FrameConst &f)
 {
-	return f.monteCarlo || f.aoEnabled || (f.lightEnabled && (f.castShadows || f.specPlastic || f.specMetal)) ||
-		(f.palette && f.paletteGradient) || f.glow || f.slowShading;
+	if (f.dofEnabled || f.chromaticAberration) return mbx::kVariantLens;
+	return (f.monteCarlo || f.aoEnabled || (f.lightEnabled && (f.castShadows || f.specPlastic || f.specMetal)) ||
+			   (f.palette && f.paletteGradient) || f.glow || f.slowShading)
+		? mbx::kVariantFull
+		: mbx::kVariantLite;
 }
 
 // Upload frames and jobs when they differ from what the device holds.
@@ -1342,7 +1363,7 @@ int render_frames(mbx_ctx *ctx, const std::vector<FrameConst> &frames, const mbx
 	int rc = stage_batch(ctx, frames, jobs);
 	if (rc != MBX_OK) return rc;
 	ctx->batchFull = 0;
-	for (const auto &f : frames) ctx->batchFull |= frame_needs_full(f) ? 1 : 0;
+	for (const auto &f : frames) ctx->batchFull = std::max(ctx->batchFull, frame_variant(f));
 	if (ctx->kernelMode == 1)
 	{
 		// one plain launch per frame (the reference engine's job-per-frame shape)
@@ -1355,7 +1376,7 @@ int render_frames(mbx_ctx *ctx, const std::vector<FrameConst> &frames, const mbx
 			const mbx::Outputs o{outs[i].rgb, outs[i].depth, outs[i].iters, outs[i].mask, outs[i].normal,
 				outs[i].steps};
 			e = mbx::launch_render(kind, ctx->d_frames + i, frames[i].width, frames[i].height, o, counters,
-				ctx->tileShift, frame_needs_full(frames[i]), ctx->stream);
+				ctx->tileShift, frame_variant(frames[i]), ctx->stream);
 			if (e != hipSuccess) return hip_fail(ctx, e, "render launch");
 		}
 	}
@@ -1526,8 +1547,20 @@ int mbx_ctx_create(int device, void *stream, mbx_ctx **out)
 	ctx->stream = (hipStream_t)stream;
 	if ((e = hipMalloc((void **)&ctx->d_frame, sizeof(FrameConst))) != hipSuccess ||
 		(e = hipMalloc((void **)&ctx->d_counters, 2 * mbx::kCounterBytes)) != hipSuccess ||
-		(e = hipMalloc((void **)&ctx->d_queue, mbx::kTileQueueBytes)) != hipSuccess)
+		(e = hipMalloc((void **)&ctx->d_queue, mbx::kTileQueueBytes)) != hipSuccess ||
+		(e = hipMalloc((void **)&ctx->d_mcTables, kMcTableDoubles * sizeof(double))) != hipSuccess)
 		return create_fail(ctx, MBX_ERR_HIP, "hipMalloc", e);
+	{
+		// the host's sincos and pow: a few thousand cycles once per context instead of the FP64
+		// argument reductions' registers in every Monte Carlo kernel
+		std::vector<double> t(kMcTableDoubles);
+		for (int k = 0; k < mbx::kLensAngles; k++) ::sincos((double)k, &t[2 * k], &t[2 * k + 1]);
+		for (int k = 0; k < mbx::kSoftRadii; k++)
+			t[2 * mbx::kLensAngles + k] = std::pow((double)k / 10000.0, 1.0 / 3.0);
+		if ((e = hipMemcpy(ctx->d_mcTables, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice)) !=
+			hipSuccess)
+			return create_fail(ctx, MBX_ERR_HIP, "hipMemcpy(tables)", e);
+	}
 	if ((e = hipEventCreate(&ctx->ev0)) != hipSuccess || (e = hipEventCreate(&ctx->ev1)) != hipSuccess ||
 		(e = hipEventCreateWithFlags(&ctx->naCopied, hipEventDisableTiming)) != hipSuccess)
 		return create_fail(ctx, MBX_ERR_HIP, "hipEventCreate", e);
@@ -1565,6 +1598,7 @@ void mbx_ctx_destroy(mbx_ctx *ctx)
 	if (ctx->d_diff) (void)hipFree(ctx->d_diff);
 	if (ctx->d_grad) (void)hipFree(ctx->d_grad);
 	if (ctx->d_gamma) (void)hipFree(ctx->d_gamma);
+	if (ctx->d_mcTables) (void)hipFree(ctx->d_mcTables);
 	if (ctx->d_zrange) (void)hipFree(ctx->d_zrange);
 	delete ctx;
 }
@@ -1717,7 +1751,7 @@ int mbx_render_batch_noalloc(mbx_ctx *ctx, const mbx_params *params, const mbx_o
 		tiles = ctx->naJobs[i].tileEnd;
 		pixels += (size_t)frames[i].width * frames[i].height;
 		rays += (uint64_t)frames[i].width * frames[i].height * (uint64_t)frames[i].samples;
-		full |= frame_needs_full(frames[i]) ? 1 : 0;
+		full = std::max(full, frame_variant(frames[i]));
 		if (tiles > (int64_t)1 << 30) return fail(ctx, MBX_ERR_INVALID, "batch too large");
 	}
 	unsigned char *base = ctx->d_scratch;
@@ -1743,7 +1777,7 @@ int mbx_render_batch_noalloc(mbx_ctx *ctx, const mbx_params *params, const mbx_o
 			const mbx::Outputs o{outs[i].rgb, outs[i].depth, outs[i].iters, outs[i].mask, outs[i].normal,
 				outs[i].steps};
 			e = mbx::launch_render(kind, dFrames + i, frames[i].width, frames[i].height, o, dCounters,
-				ctx->tileShift, frame_needs_full(frames[i]), ctx->stream);
+				ctx->tileShift, frame_variant(frames[i]), ctx->stream);
 			if (e != hipSuccess) return hip_fail(ctx, e, "render launch");
 		}
 	}

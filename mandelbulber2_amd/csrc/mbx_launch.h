@@ -19,11 +19,18 @@ struct Outputs
 	int32_t *steps;
 };
 
+// Kernel variants: without Monte Carlo, GI, shadow and AO code; with it; and with it plus the
+// Monte Carlo lens and chromatic aberration, whose rays start off the camera (a march with
+// per-point thresholds that the other variants do not carry).
+constexpr int kVariantLite = 0;
+constexpr int kVariantFull = 1;
+constexpr int kVariantLens = 2;
+
 // Plain per-frame launch: one 64-lane workgroup per tile of (1 << tileShift) x (64 >> tileShift)
-// pixels.  full = false launches the variant without Monte Carlo, GI, shadow and AO code.  The
-// work counters are added into `counters` (kCounterBytes, slot layout above).
+// pixels, in the given kernel variant.  The work counters are added into `counters`
+// (kCounterBytes, slot layout above).
 hipError_t launch_render(int deKind, const FrameConst *frame, int width, int height,
-	const Outputs &out, unsigned long long *counters, int tileShift, bool full, hipStream_t stream);
+	const Outputs &out, unsigned long long *counters, int tileShift, int variant, hipStream_t stream);
 
 // Point DE query at a fixed detail size.
 hipError_t launch_distance(int deKind, const FrameConst *frame, const float *pts, int n,
@@ -82,7 +89,7 @@ struct BatchArgs
 	const FrameJob *jobs;
 	int numFrames;
 	int tileShift; // tiles are (1 << tileShift) x (64 >> tileShift) pixels, tileShift in 2..5
-	int full;      // 0: every frame renders without Monte Carlo, GI, shadows and AO
+	int full;      // kernel variant (kVariantLite, kVariantFull, kVariantLens) that every frame fits
 	int mode;      // kBatchGrid or kBatchPersistent
 	unsigned int totalTiles;
 	unsigned int *queue;           // kTileQueues tile counters (kTileQueueBytes)

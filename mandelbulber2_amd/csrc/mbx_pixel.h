@@ -98,7 +98,7 @@ __device__ __forceinline__ typename PrecOf<KIND>::R calc_delta(const FrameConst 
 // carries no test of the flag), 2 = read from the frame (FP64 kinds: one copy of the march)
 template <int KIND, int ADV>
 __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start, typename PrecOf<KIND>::P dir,
-	typename PrecOf<KIND>::R maxScan, uint32_t &seed, MarchOut<KIND> &out, Work &w)
+	typename PrecOf<KIND>::R maxScan, uint32_t &seed, MarchOut<KIND> &out, Work &w, bool fromCamera = true)
 {
 #pragma clang fp contract(off)
 	using Q = PrecOf<KIND>;
@@ -151,7 +151,7 @@ __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start
 			if (scan > maxScan) break;
 			continue;
 		}
-		if (kExact)
+		if (kExact || !fromCamera)
 			thr = dist_thresh<KIND>(f, point);
 		else
 			thr = clamp_thresh(f, constThr ? constThrValue : scan * thrScale, adv);
@@ -192,7 +192,7 @@ __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start
 			else if (dist < thr * searchLimit)
 				scan -= step;
 			point = Q::along(start, dir, scan);
-			if (kExact)
+			if (kExact || !fromCamera)
 				thr = dist_thresh<KIND>(f, point);
 			else
 				thr = clamp_thresh(f, constThr ? constThrValue : scan * thrScale, adv);
@@ -300,12 +300,25 @@ __device__ __attribute__((noinline)) typename PrecOf<KIND>::P slow_normals(const
 // cone variant when soft_shadow_cone > 0
 template <int KIND>
 __device__ float aux_shadow(const FrameConst &f, typename PrecOf<KIND>::P p, typename PrecOf<KIND>::R thr,
-	typename PrecOf<KIND>::R distance, Work &w)
+	typename PrecOf<KIND>::R distance, uint32_t &seed, Work &w)
 {
+#pragma clang fp contract(off)
 	using Q = PrecOf<KIND>;
 	using R = typename Q::R;
 	using P = typename Q::P;
-	const P lv = Q::lightDir(f);
+	P lv = Q::lightDir(f);
+	if (f.mcSoftShadows)
+	{
+		// a random point of the ball of radius softRange (uniform in volume through the cube root
+		// of the radius) added to the light vector (shader_aux_shadow.cpp:78-87)
+		P rv;
+		rv.x = (R)((double)random_mod(seed, 10000) / 5000.0 - 1.0);
+		rv.y = (R)((double)random_mod(seed, 10000) / 5000.0 - 1.0);
+		rv.z = (R)((double)random_mod(seed, 10000) / 5000.0 - 1.0);
+		const R radius = (R)f.softCbrt[random_mod(seed, kSoftRadii - 1)];
+		const R k = Q::softRange(f) * radius / Q::len(rv);
+		lv = Q::normalize_p(P{lv.x + rv.x * k, lv.y + rv.y * k, lv.z + rv.z * k});
+	}
 	R shadowTemp = 1;
 	R maxSoft = 0;
 	R step = thr;
@@ -539,7 +552,7 @@ __device__ V3 object_shader(const FrameConst &f, typename PrecOf<KIND>::P p, typ
 			if (shade > 0.001f || specMax > 0.001f)
 			{
 				const R distance = f.penetrating ? delta * (R)f.height : (R)f.viewDistanceMaxLight;
-				shadow = aux_shadow<KIND>(f, p, thr, distance, w);
+				shadow = aux_shadow<KIND>(f, p, thr, distance, seed, w);
 				specular = specular * shadow;
 			}
 			else
@@ -703,12 +716,37 @@ __device__ __attribute__((noinline)) double mc_noise(int rep, V3 sum, V3 stdDevS
 	return total;
 }
 
-// One pixel: the sample loop of doWork and RayRecursion at depth 0.  FULL = false is the
-// variant for frames without Monte Carlo sampling, GI, shadows and AO.
-template <int KIND, bool FULL>
+// Hsv2rgb (hsv2rgb.cpp:40-60) in float: hue in degrees, six sectors
+__device__ __forceinline__ V3 hsv2rgb(float hue, float sat, float val)
+{
+#pragma clang fp contract(off)
+	const float h = hue / 60.0f;
+	const int i = (int)floorf(hue / 60.0f);
+	const float fr = h - (float)i;
+	const float pp = val * (1.0f - sat);
+	const float q = val * (1.0f - sat * fr);
+	const float t = val * (1.0f - sat * (1.0f - fr));
+	switch (i)
+	{
+		case 0: return v3(val, t, pp);
+		case 1: return v3(q, val, pp);
+		case 2: return v3(pp, val, t);
+		case 3: return v3(pp, q, val);
+		case 4: return v3(t, pp, val);
+		case 5: return v3(val, pp, q);
+		default: return v3(0.0f, 0.0f, 0.0f);
+	}
+}
+
+// One pixel: the sample loop of doWork and RayRecursion at depth 0.  VAR = 0 is the variant for
+// frames without Monte Carlo sampling, GI, shadows and AO; 1 carries them; 2 adds the Monte Carlo
+// lens and chromatic aberration.
+template <int KIND, int VAR>
 __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &po, Work &w)
 {
 #pragma clang fp contract(off)
+	constexpr bool FULL = VAR >= 1;
+	constexpr bool LENS = VAR == 2;
 	const FrameConst &f = uniform_ref(frame);
 	using Q = PrecOf<KIND>;
 	using R = typename Q::R;
@@ -731,14 +769,46 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 			ix = ox + ((R)random_mod(seed, 1000) * (R)1e-3 - (R)0.5) * Q::invWidth(f) * aspect;
 			iy = oy + ((R)random_mod(seed, 1000) * (R)1e-3 - (R)0.5) * Q::invHeight(f);
 		}
-		const P dir = Q::view(f, ix, iy);
+		P dir, start = cam;
+		float hue = 0.0f; // chromatic aberration: the sample's hue (its tint is applied after shading)
+		const bool lens = LENS && (f.dofEnabled || f.chromaticAberration);
+		if (lens)
+		{
+			P v = Q::view_raw(f, ix, iy);
+			if (f.dofEnabled)
+			{
+				// MonteCarloDOF (render_worker.cpp:1832-1844): a random point of the lens, turned by
+				// the camera rotation, becomes the ray origin; the view vector loses 1/focus of it so
+				// that the ray still crosses the undisturbed one at the focal distance
+				const double randR =
+					0.0015 * f.dofRadiusD * f.dofFocusD * sqrt((double)random_mod(seed, 65536) / 65536.0);
+				// the angle is a whole number of radians: its sine and cosine come from the table
+				const int randAngle = random_mod(seed, kLensAngles - 1);
+				const double sa = f.lensSinCos[2 * randAngle], ca = f.lensSinCos[2 * randAngle + 1];
+				const P rr = Q::cam_rot(f, P{(R)(randR * sa), (R)0, (R)(randR * ca)});
+				const R invFocus = (R)(1.0 / f.dofFocusD);
+				v = P{v.x - rr.x * invFocus, v.y - rr.y * invFocus, v.z - rr.z * invFocus};
+				start = P{cam.x + rr.x, cam.y + rr.y, cam.z + rr.z};
+			}
+			if (f.chromaticAberration)
+			{
+				// a hue per sample bends the view vector and tints the sample
+				// (render_worker.cpp:255-264, 374-380)
+				hue = (float)((double)random_mod(seed, 3600) / 10.0);
+				const P cv = Q::cam_rot(f, P{(R)0, (R)(double)(hue / 20000.0f * f.caCameraDispersion), (R)0});
+				v = Q::normalize_p(P{v.x - cv.x, v.y - cv.y, v.z - cv.z});
+			}
+			dir = Q::normalize_p(v);
+		}
+		else
+			dir = Q::view(f, ix, iy);
 		MarchOut<KIND> m;
 		if constexpr (PosPrecision<KIND>::kFP64)
-			ray_marching<KIND, 2>(f, cam, dir, (R)f.viewDistanceMax, seed, m, w);
+			ray_marching<KIND, 2>(f, start, dir, (R)f.viewDistanceMax, seed, m, w, !lens);
 		else if (f.advancedQuality)
-			ray_marching<KIND, 1>(f, cam, dir, (R)f.viewDistanceMax, seed, m, w);
+			ray_marching<KIND, 1>(f, start, dir, (R)f.viewDistanceMax, seed, m, w, !lens);
 		else
-			ray_marching<KIND, 0>(f, cam, dir, (R)f.viewDistanceMax, seed, m, w);
+			ray_marching<KIND, 0>(f, start, dir, (R)f.viewDistanceMax, seed, m, w, !lens);
 		V3 rgb;
 		V3 n = v3(0.0f, 0.0f, 0.0f);
 		if (m.found)
@@ -763,6 +833,11 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 		else
 			rgb = background_shader(f, Q::tof(dir));
 		if (FULL && f.glow) rgb = glow_shader(f, m.stepCount, rgb);
+		if (LENS && f.chromaticAberration)
+		{
+			const V3 tint = hsv2rgb(fmodf(360.0f + hue - 60.0f, 360.0f), 1.0f, 2.0f);
+			rgb = v3(rgb.x * tint.x, rgb.y * tint.y, rgb.z * tint.z);
+		}
 		acc = acc + rgb;
 		if (rep == 0)
 		{

@@ -25,7 +25,7 @@ constexpr int occupancy_target()
 
 // Plain per-frame kernel: 64 lanes = one tile of (1 << tileShift) x (64 >> tileShift) pixels
 // (8x8 by default), one workgroup per tile, one launch per frame.
-template <int KIND, bool FULL>
+template <int KIND, int VAR>
 __global__ __launch_bounds__(64, occupancy_target<KIND>()) void render_kernel(const FrameConst *__restrict__ frame,
 	Outputs out, unsigned long long *__restrict__ counters, int tileShift)
 {
@@ -39,7 +39,7 @@ __global__ __launch_bounds__(64, occupancy_target<KIND>()) void render_kernel(co
 	if (xs < f.width && ys < f.height)
 	{
 		PixelOut po;
-		render_pixel<KIND, FULL>(f, xs, ys, po, w);
+		render_pixel<KIND, VAR>(f, xs, ys, po, w);
 		write_pixel(out, (size_t)ys * (size_t)f.width + (size_t)xs, po);
 	}
 	if (counters)
@@ -61,7 +61,7 @@ static_assert(kCounterSlots == 64, "workgroup 0 of batch_kernel clears one count
 
 // When clearNext is set, workgroup 0 zeroes those counter slots (another bank than `counters`):
 // the next launch accumulates there without a clear kernel in front of it.
-template <int KIND, bool FULL>
+template <int KIND, int VAR>
 __global__ __launch_bounds__(64, occupancy_target<KIND>()) void batch_kernel(const FrameConst *__restrict__ frames,
 	const FrameJob *__restrict__ jobs, int numFrames, unsigned long long *__restrict__ counters, int tileShift,
 	unsigned long long *__restrict__ clearNext)
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(64, occupancy_target<KIND>()) void batch_kernel(con
 	if (xs < f.width && ys < f.height)
 	{
 		PixelOut po;
-		render_pixel<KIND, FULL>(f, xs, ys, po, w);
+		render_pixel<KIND, VAR>(f, xs, ys, po, w);
 		const Outputs out{job.rgb, job.depth, job.iters, job.mask, job.normal, job.steps};
 		write_pixel(out, (size_t)ys * (size_t)f.width + (size_t)xs, po);
 	}
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(64, occupancy_target<KIND>()) void batch_kernel(con
 // (block mod kTileQueues) and, when that range is exhausted, moves on to the next queue.
 // (frames and jobs are separate __restrict__ kernel arguments so that the compiler may read
 // them with scalar loads)
-template <int KIND, bool FULL>
+template <int KIND, int VAR>
 __global__ __launch_bounds__(64, occupancy_target<KIND>()) void persistent_kernel(const FrameConst *__restrict__ frames,
 	const FrameJob *__restrict__ jobs, int numFrames, unsigned int *__restrict__ queue,
 	unsigned long long *__restrict__ counters, int tileShift)
@@ -153,7 +153,7 @@ __global__ __launch_bounds__(64, occupancy_target<KIND>()) void persistent_kerne
 			if (xs < f.width && ys < f.height)
 			{
 				PixelOut po;
-				render_pixel<KIND, FULL>(f, xs, ys, po, w);
+				render_pixel<KIND, VAR>(f, xs, ys, po, w);
 				const Outputs out{job.rgb, job.depth, job.iters, job.mask, job.normal, job.steps};
 				write_pixel(out, (size_t)ys * (size_t)f.width + (size_t)xs, po);
 			}
@@ -220,16 +220,19 @@ __global__ __launch_bounds__(64, KIND == kGeneric ? 3 : 1) void distance_kernel(
 
 template <int KIND>
 static hipError_t launch_render_t(const FrameConst *frame, int width, int height, const Outputs &out,
-	unsigned long long *counters, int tileShift, bool full, hipStream_t stream)
+	unsigned long long *counters, int tileShift, int variant, hipStream_t stream)
 {
 	const int tw = 1 << tileShift, th = 64 >> tileShift;
 	const int tiles = ((width + tw - 1) / tw) * ((height + th - 1) / th);
-	if (full)
-		hipLaunchKernelGGL((render_kernel<KIND, true>), dim3(tiles), dim3(64), 0, stream, frame, out, counters,
-			tileShift);
+	if (variant == kVariantLens)
+		hipLaunchKernelGGL((render_kernel<KIND, kVariantLens>), dim3(tiles), dim3(64), 0, stream, frame, out,
+			counters, tileShift);
+	else if (variant == kVariantFull)
+		hipLaunchKernelGGL((render_kernel<KIND, kVariantFull>), dim3(tiles), dim3(64), 0, stream, frame, out,
+			counters, tileShift);
 	else
-		hipLaunchKernelGGL((render_kernel<KIND, false>), dim3(tiles), dim3(64), 0, stream, frame, out, counters,
-			tileShift);
+		hipLaunchKernelGGL((render_kernel<KIND, kVariantLite>), dim3(tiles), dim3(64), 0, stream, frame, out,
+			counters, tileShift);
 	return hipGetLastError();
 }
 
@@ -243,19 +246,19 @@ static hipError_t launch_distance_t(const FrameConst *frame, const float *pts, i
 }
 
 hipError_t launch_render(int deKind, const FrameConst *frame, int width, int height,
-	const Outputs &out, unsigned long long *counters, int tileShift, bool full, hipStream_t stream)
+	const Outputs &out, unsigned long long *counters, int tileShift, int variant, hipStream_t stream)
 {
 	switch (deKind)
 	{
-		case kBulb8: return launch_render_t<kBulb8>(frame, width, height, out, counters, tileShift, full, stream);
-		case kBulbTrig: return launch_render_t<kBulbTrig>(frame, width, height, out, counters, tileShift, full, stream);
+		case kBulb8: return launch_render_t<kBulb8>(frame, width, height, out, counters, tileShift, variant, stream);
+		case kBulbTrig: return launch_render_t<kBulbTrig>(frame, width, height, out, counters, tileShift, variant, stream);
 		case kBoxAnalytic:
-			return launch_render_t<kBoxAnalytic>(frame, width, height, out, counters, tileShift, full, stream);
-		case kBoxDelta: return launch_render_t<kBoxDelta>(frame, width, height, out, counters, tileShift, full, stream);
+			return launch_render_t<kBoxAnalytic>(frame, width, height, out, counters, tileShift, variant, stream);
+		case kBoxDelta: return launch_render_t<kBoxDelta>(frame, width, height, out, counters, tileShift, variant, stream);
 		case kIfsAnalytic:
-			return launch_render_t<kIfsAnalytic>(frame, width, height, out, counters, tileShift, full, stream);
-		case kBulbD: return launch_render_t<kBulbD>(frame, width, height, out, counters, tileShift, full, stream);
-		case kGeneric: return launch_render_t<kGeneric>(frame, width, height, out, counters, tileShift, full, stream);
+			return launch_render_t<kIfsAnalytic>(frame, width, height, out, counters, tileShift, variant, stream);
+		case kBulbD: return launch_render_t<kBulbD>(frame, width, height, out, counters, tileShift, variant, stream);
+		case kGeneric: return launch_render_t<kGeneric>(frame, width, height, out, counters, tileShift, variant, stream);
 		default: return hipErrorInvalidValue;
 	}
 }
@@ -266,27 +269,26 @@ hipError_t launch_clear(unsigned int *queue, unsigned long long *counters, hipSt
 	return hipGetLastError();
 }
 
-template <int KIND>
-static hipError_t launch_batch_t(const BatchArgs &a, int numWaves, bool full, hipStream_t stream)
+template <int KIND, int VAR>
+static void launch_batch_v(const BatchArgs &a, int numWaves, hipStream_t stream)
 {
 	if (a.mode == kBatchPersistent)
-	{
-		if (full)
-			hipLaunchKernelGGL((persistent_kernel<KIND, true>), dim3(numWaves), dim3(64), 0, stream, a.frames,
-				a.jobs, a.numFrames, a.queue, a.counters, a.tileShift);
-		else
-			hipLaunchKernelGGL((persistent_kernel<KIND, false>), dim3(numWaves), dim3(64), 0, stream, a.frames,
-				a.jobs, a.numFrames, a.queue, a.counters, a.tileShift);
-	}
+		hipLaunchKernelGGL((persistent_kernel<KIND, VAR>), dim3(numWaves), dim3(64), 0, stream, a.frames, a.jobs,
+			a.numFrames, a.queue, a.counters, a.tileShift);
 	else
-	{
-		if (full)
-			hipLaunchKernelGGL((batch_kernel<KIND, true>), dim3(a.totalTiles), dim3(64), 0, stream, a.frames, a.jobs,
-				a.numFrames, a.counters, a.tileShift, a.clearNext);
-		else
-			hipLaunchKernelGGL((batch_kernel<KIND, false>), dim3(a.totalTiles), dim3(64), 0, stream, a.frames, a.jobs,
-				a.numFrames, a.counters, a.tileShift, a.clearNext);
-	}
+		hipLaunchKernelGGL((batch_kernel<KIND, VAR>), dim3(a.totalTiles), dim3(64), 0, stream, a.frames, a.jobs,
+			a.numFrames, a.counters, a.tileShift, a.clearNext);
+}
+
+template <int KIND>
+static hipError_t launch_batch_t(const BatchArgs &a, int numWaves, int variant, hipStream_t stream)
+{
+	if (variant == kVariantLens)
+		launch_batch_v<KIND, kVariantLens>(a, numWaves, stream);
+	else if (variant == kVariantFull)
+		launch_batch_v<KIND, kVariantFull>(a, numWaves, stream);
+	else
+		launch_batch_v<KIND, kVariantLite>(a, numWaves, stream);
 	return hipGetLastError();
 }
 
@@ -299,13 +301,13 @@ hipError_t launch_batch(int deKind, BatchArgs a, int numWaves, hipStream_t strea
 	}
 	switch (deKind)
 	{
-		case kBulb8: return launch_batch_t<kBulb8>(a, numWaves, a.full != 0, stream);
-		case kBulbTrig: return launch_batch_t<kBulbTrig>(a, numWaves, a.full != 0, stream);
-		case kBoxAnalytic: return launch_batch_t<kBoxAnalytic>(a, numWaves, a.full != 0, stream);
-		case kBoxDelta: return launch_batch_t<kBoxDelta>(a, numWaves, a.full != 0, stream);
-		case kIfsAnalytic: return launch_batch_t<kIfsAnalytic>(a, numWaves, a.full != 0, stream);
-		case kBulbD: return launch_batch_t<kBulbD>(a, numWaves, a.full != 0, stream);
-		case kGeneric: return launch_batch_t<kGeneric>(a, numWaves, a.full != 0, stream);
+		case kBulb8: return launch_batch_t<kBulb8>(a, numWaves, a.full, stream);
+		case kBulbTrig: return launch_batch_t<kBulbTrig>(a, numWaves, a.full, stream);
+		case kBoxAnalytic: return launch_batch_t<kBoxAnalytic>(a, numWaves, a.full, stream);
+		case kBoxDelta: return launch_batch_t<kBoxDelta>(a, numWaves, a.full, stream);
+		case kIfsAnalytic: return launch_batch_t<kIfsAnalytic>(a, numWaves, a.full, stream);
+		case kBulbD: return launch_batch_t<kBulbD>(a, numWaves, a.full, stream);
+		case kGeneric: return launch_batch_t<kGeneric>(a, numWaves, a.full, stream);
 		default: return hipErrorInvalidValue;
 	}
 }

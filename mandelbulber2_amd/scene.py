@@ -165,6 +165,12 @@ def default_params(width=640, height=480):
     p.DOFMinSamples = 10
     p.DOFMaxNoise = 1.0
     p.aoLightMapRotation = 0.0
+    p.DOFEnabled = 0
+    p.DOFMonteCarloChromaticAberration = 0
+    p.DOFFocus = 6.0
+    p.DOFRadius = 10.0
+    p.DOFMonteCarloCACameraDispersion = 1.0
+    p.monteCarloSoftShadows = 0
     # palette colouring of material #1 (InitMaterialParams, initparameters.cpp:1600-1810)
     p.useColorsFromPalette = 1
     p.surfaceGradientEnable = 1

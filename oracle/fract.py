@@ -196,6 +196,9 @@ def init_params():
     c.add("all_lights_size", DOUBLE, 1.0, 0.0, 100000.0)            # :312
     c.add("basic_fog_enabled", BOOL, False)                         # :315
     c.add("DOF_enabled", BOOL, False)                               # :322
+    c.add("DOF_focus", DOUBLE, 6.0, 0.0, 10000.0)                   # :323
+    c.add("DOF_radius", DOUBLE, 10.0, 0.0, 200.0)                   # :324
+    c.add("DOF_MC_CA_camera_dispersion", DOUBLE, 1.0, 1e-15, 1000.0)  # :338
     c.add("DOF_monte_carlo", BOOL, False)                           # :329
     c.add("DOF_samples", INT, 100)                                  # :330
     c.add("DOF_min_samples", INT, 10)                               # :331
@@ -882,9 +885,13 @@ def load(text, trace=None):
     out["DOFMaxNoise"] = g("DOF_max_noise")
     out["aoLightMapRotation"] = g("ao_light_map_rotation")
     # Monte Carlo soft shadows jitter the light vector (shader_aux_shadow.cpp:76-87);
-    # chromatic aberration shifts the view vector by a random hue (render_worker.cpp:255-264)
-    if g("DOF_monte_carlo") and (g("MC_soft_shadows_enable") or g("DOF_MC_CA_enable")):
-        flags |= FLAGS["mc_effects"]
+    # the Monte Carlo lens, chromatic aberration and soft shadows (fractparams.cpp:116-130, 180)
+    out["DOFEnabled"] = int(g("DOF_enabled"))
+    out["DOFFocus"] = g("DOF_focus")
+    out["DOFRadius"] = g("DOF_radius")
+    out["DOFMonteCarloChromaticAberration"] = int(g("DOF_MC_CA_enable"))
+    out["DOFMonteCarloCACameraDispersion"] = g("DOF_MC_CA_camera_dispersion")
+    out["monteCarloSoftShadows"] = int(g("MC_soft_shadows_enable"))
     if g("nebula_mode"):                    # the OpenCL nebula engine renders these (render_job.cpp:664-673)
         flags |= FLAGS["nebula"]
     out["N"] = g("N")                       # params->N: slow shading's pseudo-distance (fractparams.cpp:189)
@@ -978,7 +985,9 @@ def load(text, trace=None):
         flags |= FLAGS["fog"]
     if g("textured_background") or g("env_mapping_enable"):
         flags |= FLAGS["textures"]
-    if g("hdr_blur_enabled") or g("DOF_enabled") or g("post_chromatic_aberration_enabled"):
+    # DOF without Monte Carlo is the post-processing blur; with it, the lens of the pixel loop
+    if g("hdr_blur_enabled") or (g("DOF_enabled") and not g("DOF_monte_carlo")) or \
+            g("post_chromatic_aberration_enabled"):
         flags |= FLAGS["postprocess"]
     if animation:
         flags |= FLAGS["animation"]

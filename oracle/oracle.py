@@ -61,6 +61,12 @@ def lib(fast=False):
     L.mbxo_random_int.argtypes = [P(C.c_uint32)]
     L.mbxo_mc_noise.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
     L.mbxo_mc_noise.restype = None
+    L.mbxo_sample_ray.argtypes = [P(abi.MbxParams), C.c_int, C.c_int, P(C.c_uint32), C.c_void_p, C.c_void_p,
+                                  C.c_void_p]
+    L.mbxo_soft_light.argtypes = [C.c_void_p, C.c_double, P(C.c_uint32), C.c_int, C.c_void_p]
+    L.mbxo_soft_light.restype = None
+    L.mbxo_hsv2rgb.argtypes = [C.c_float, C.c_float, C.c_float, C.c_void_p]
+    L.mbxo_hsv2rgb.restype = None
     L.mbxo_prepare_ao_vectors.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_int, C.c_int]
     L.mbxo_random.argtypes = [P(C.c_uint32), C.c_int]
     L.mbxo_pixel_seed.argtypes = [C.c_uint32, C.c_uint32]
@@ -145,6 +151,33 @@ def mc_noise(samples):
     x = np.ascontiguousarray(samples, np.float32).reshape(-1, 3)
     out = np.zeros(len(x), np.float64)
     lib().mbxo_mc_noise(_ptr(x), len(x), _ptr(out))
+    return out
+
+
+def sample_ray(p, xs, ys, seed):
+    """The first sample's ray of pixel (xs, ys) from random state `seed`: (start, view vector
+    before normalisation, tint, seed after the sample's draws)."""
+    s = C.c_uint32(seed)
+    st = np.zeros(3, np.float64)
+    vv = np.zeros(3, np.float64)
+    tint = np.zeros(3, np.float32)
+    rc = lib().mbxo_sample_ray(C.byref(p), xs, ys, C.byref(s), _ptr(st), _ptr(vv), _ptr(tint))
+    if rc != 0:
+        raise RuntimeError("oracle sample_ray failed: %d" % rc)
+    return st, vv, tint, s.value
+
+
+def soft_light(light, soft_range, seed, n):
+    """n successive Monte Carlo soft-shadow light vectors around the unit vector `light`."""
+    l3 = np.ascontiguousarray(light, np.float64)
+    out = np.zeros((n, 3), np.float64)
+    lib().mbxo_soft_light(_ptr(l3), soft_range, C.byref(C.c_uint32(seed)), n, _ptr(out))
+    return out
+
+
+def hsv2rgb(hue, sat, val):
+    out = np.zeros(3, np.float32)
+    lib().mbxo_hsv2rgb(hue, sat, val, _ptr(out))
     return out
 
 

@@ -1579,6 +1579,19 @@ static v3 calculate_normals(ray_ctx *rc, const shin_t *in)
 	return normal;
 }
 
+/* Monte Carlo soft shadows (shader_aux_shadow.cpp:78-87): a random point of the ball of radius
+ * softRange added to the light vector, uniform in volume by the cube root of the radius */
+static v3 mc_soft_light(uint32_t *seed, v3 lightVector, double softRange)
+{
+	v3 randomVector;
+	randomVector.x = mbxo_random(seed, 10000) / 5000.0 - 1.0;
+	randomVector.y = mbxo_random(seed, 10000) / 5000.0 - 1.0;
+	randomVector.z = mbxo_random(seed, 10000) / 5000.0 - 1.0;
+	const double randomSphereRadius = pow(mbxo_random(seed, 10000) / 10000.0, 1.0 / 3.0);
+	const v3 randomSphere = v3_mul(randomVector, softRange * randomSphereRadius / v3_len(randomVector));
+	return v3_normalize(v3_add(lightVector, randomSphere));
+}
+
 /* AuxShadow, hard variant with one directional light (shader_aux_shadow.cpp:39-323) */
 static float aux_shadow(ray_ctx *rc, const shin_t *in, double distance, v3 lightVector)
 {
@@ -1590,7 +1603,9 @@ static float aux_shadow(ray_ctx *rc, const shin_t *in, double distance, v3 light
 	double start = in->distThresh;
 	if (p->interiorMode) start = in->distThresh * DEFactor;
 	const double softRange = tan(s->lightSoftShadowCone);
-	const int bSoft = !p->iterThreshMode && !p->interiorMode && softRange > 0.0;
+	const int mcSoft = p->DOFMonteCarlo && p->monteCarloSoftShadows;
+	const int bSoft = !p->iterThreshMode && !p->interiorMode && softRange > 0.0 && !mcSoft;
+	if (mcSoft) lightVector = mc_soft_light(rc->seed, lightVector, softRange);
 	double maxSoft = 0.0;
 	int count = 0;
 	double step = in->distThresh;
@@ -2233,6 +2248,108 @@ void mbxo_mc_noise(const float *pixels, int n, double *noiseOut)
 	}
 }
 
+/* MonteCarloDOF, three-point perspective (render_worker.cpp:1832-1844): a random point of the
+ * lens (radius 0.0015 DOF_radius DOF_focus sqrt(u), angle an integer number of radians) turned
+ * by the camera rotation becomes the ray's origin, and the view vector loses 1/focus of it so
+ * that the ray still meets the undisturbed one at the focal distance */
+static void monte_carlo_dof(ray_ctx *rc, v3 *startRay, v3 *viewVector)
+{
+	const scene_t *s = rc->s;
+	const mbx_params *p = s->p;
+	const double randR = 0.0015 * p->DOFRadius * p->DOFFocus * sqrt(mbxo_random(rc->seed, 65536) / 65536.0);
+	const double randAngle = mbxo_random(rc->seed, 65536);
+	double sa, ca; /* one sincos call, as GCC compiles the reference's sin / cos pair */
+	sincos(randAngle, &sa, &ca);
+	const v3 randVector = V3(randR * sa, 0.0, randR * ca);
+	const v3 randVectorRot = rot_apply3(&s->mRot, randVector);
+	*viewVector = v3_sub(*viewVector, v3_mul(randVectorRot, 1.0 / p->DOFFocus));
+	*startRay = v3_add(s->camera, randVectorRot);
+}
+
+/* Hsv2rgb (hsv2rgb.cpp:40-60) in float */
+static void hsv2rgb(float hue, float sat, float val, float rgb[3])
+{
+	const float h = hue / 60.0f;
+	const int i = (int)floorf(hue / 60.0f);
+	const float f = h - i;
+	const float pp = val * (1 - sat);
+	const float q = val * (1 - (sat * f));
+	const float t = val * (1 - (sat * (1 - f)));
+	float r = 0, g = 0, b = 0;
+	switch (i)
+	{
+		case 0: r = val, g = t, b = pp; break;
+		case 1: r = q, g = val, b = pp; break;
+		case 2: r = pp, g = val, b = t; break;
+		case 3: r = pp, g = q, b = val; break;
+		case 4: r = t, g = pp, b = val; break;
+		case 5: r = val, g = pp, b = q; break;
+		default: break;
+	}
+	rgb[0] = r;
+	rgb[1] = g;
+	rgb[2] = b;
+}
+
+void mbxo_hsv2rgb(float hue, float sat, float val, float *rgb) { hsv2rgb(hue, sat, val, rgb); }
+
+/* the ray of one sample of a pixel at image-plane point (ox, oy) (render_worker.cpp:226-264):
+ * Monte Carlo anti-aliasing jitter, the lens, chromatic aberration; tint is the sample's colour
+ * factor (1 without chromatic aberration) */
+static void sample_ray(ray_ctx *rc, double ox, double oy, v3 *startRay, v3 *viewVector, float tint[3])
+{
+	const scene_t *s = rc->s;
+	const mbx_params *p = s->p;
+	const int W = p->imageWidth, H = p->imageHeight;
+	double ix = ox, iy = oy;
+	if (p->DOFMonteCarlo)
+	{
+		ix = ox + ((double)mbxo_random(rc->seed, 1000) / 1000.0 - 0.5) / W * s->aspectRatio;
+		iy = oy + ((double)mbxo_random(rc->seed, 1000) / 1000.0 - 0.5) / H;
+	}
+	*viewVector = view_vector(s, ix, iy);
+	*startRay = s->camera;
+	if (p->DOFMonteCarlo && p->DOFEnabled) monte_carlo_dof(rc, startRay, viewVector);
+	/* chromatic aberration (render_worker.cpp:255-264): a hue per sample bends the view
+	 * vector and tints the sample (:374-380) */
+	tint[0] = tint[1] = tint[2] = 1.0f;
+	if (p->DOFMonteCarlo && p->DOFMonteCarloChromaticAberration)
+	{
+		const float actualHue = (float)(mbxo_random(rc->seed, 3600) / 10.0);
+		hsv2rgb(fmodf(360.0f + actualHue - 60.0f, 360.0f), 1.0f, 2.0f, tint);
+		const v3 randVector = V3(0.0, (double)(actualHue / 20000.0f * (float)p->DOFMonteCarloCACameraDispersion), 0.0);
+		*viewVector = v3_normalize(v3_sub(*viewVector, rot_apply3(&s->mRot, randVector)));
+	}
+}
+
+/* the first sample's ray of pixel (xs, ys) from the random state *seed: origin, view vector
+ * (before the march normalises it) and tint; *seed advances past the sample's draws */
+int mbxo_sample_ray(const mbx_params *p, int xs, int ys, uint32_t *seed, double *start3, double *view3, float *tint3)
+{
+	scene_t s;
+	int rc0 = scene_init(&s, p);
+	if (rc0) return rc0;
+	ray_ctx rc = {&s, seed, 0, 0};
+	const double ox = ((1.0 / p->imageWidth) * xs + (-0.5)) * s.aspectRatio;
+	const double oy = (-1.0 / p->imageHeight) * ys + 0.5;
+	v3 st, vv;
+	sample_ray(&rc, ox, oy, &st, &vv, tint3);
+	start3[0] = st.x, start3[1] = st.y, start3[2] = st.z;
+	view3[0] = vv.x, view3[1] = vv.y, view3[2] = vv.z;
+	return 0;
+}
+
+/* n successive Monte Carlo soft-shadow light vectors around the unit vector light3 for a cone of
+ * tangent softRange, from the random state *seed */
+void mbxo_soft_light(const double *light3, double softRange, uint32_t *seed, int n, double *out3)
+{
+	for (int i = 0; i < n; i++)
+	{
+		const v3 l = mc_soft_light(seed, V3(light3[0], light3[1], light3[2]), softRange);
+		out3[3 * i] = l.x, out3[3 * i + 1] = l.y, out3[3 * i + 2] = l.z;
+	}
+}
+
 /* pixel loop body of cRenderWorker::doWork (render_worker.cpp:155-525) for one pixel */
 static void render_pixel(ray_ctx *rc, int xs, int ys, ray_result *out)
 {
@@ -2253,13 +2370,13 @@ static void render_pixel(ray_ctx *rc, int xs, int ys, ray_result *out)
 	memset(out, 0, sizeof(*out));
 	for (int repeat = 0; repeat < repeats; repeat++)
 	{
-		if (p->DOFMonteCarlo)
-		{
-			ix = ox + ((double)mbxo_random(rc->seed, 1000) / 1000.0 - 0.5) / W * s->aspectRatio;
-			iy = oy + ((double)mbxo_random(rc->seed, 1000) / 1000.0 - 0.5) / H;
-		}
-		v3 viewVector = view_vector(s, ix, iy);
-		trace_primary(rc, s->camera, viewVector, &r);
+		v3 viewVector, startRay;
+		float tint[3];
+		sample_ray(rc, ox, oy, &startRay, &viewVector, tint);
+		trace_primary(rc, startRay, viewVector, &r);
+		r.rgb[0] *= tint[0];
+		r.rgb[1] *= tint[1];
+		r.rgb[2] *= tint[2];
 		acc[0] += r.rgb[0];
 		acc[1] += r.rgb[1];
 		acc[2] += r.rgb[2];

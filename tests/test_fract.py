@@ -535,11 +535,24 @@ def test_all_lights_factors_and_main_n(load):
 
 
 @pytest.mark.parametrize("body,flag", [
-    ("DOF_monte_carlo true;\nMC_soft_shadows_enable true;\n", "mc_effects"),
-    ("DOF_monte_carlo true;\nDOF_MC_CA_enable true;\n", "mc_effects"),
-    ("MC_soft_shadows_enable true;\n", None),        # without Monte Carlo the switch is inert
+    ("DOF_monte_carlo true;\nMC_soft_shadows_enable true;\n", None),
+    ("DOF_monte_carlo true;\nDOF_MC_CA_enable true;\n", None),
+    ("DOF_monte_carlo true;\nDOF_enabled true;\n", None),     # the Monte Carlo lens: rendered
+    ("DOF_enabled true;\n", "postprocess"),                     # the post-processing blur
     ("nebula_mode true;\n", "nebula"),
 ])
 def test_monte_carlo_effect_and_nebula_flags(load, body, flag):
     got = load(_hdr("2.33", body)).flags()
-    assert ({"mc_effects", "nebula"} & got) == ({flag} if flag else set())
+    assert ({"mc_effects", "nebula", "postprocess"} & got) == ({flag} if flag else set())
+
+
+def test_monte_carlo_lens_aberration_and_soft_shadows(load):
+    """DOF_focus / DOF_radius / DOF_MC_CA_camera_dispersion with their limits
+    (initparameters.cpp:323-338) and the two switches."""
+    r = load(_hdr("2.33", "DOF_monte_carlo true;\nDOF_enabled true;\nDOF_focus 2,5;\nDOF_radius 300;\n"
+                          "DOF_MC_CA_enable true;\nDOF_MC_CA_camera_dispersion 0,5;\nMC_soft_shadows_enable true;\n"))
+    assert r["DOFEnabled"] == 1 and r["DOFFocus"] == 2.5 and r["DOFRadius"] == 200.0
+    assert r["DOFMonteCarloChromaticAberration"] == 1 and r["DOFMonteCarloCACameraDispersion"] == 0.5
+    assert r["monteCarloSoftShadows"] == 1
+    r = load(_hdr("2.33", ""))
+    assert (r["DOFEnabled"], r["DOFFocus"], r["DOFRadius"], r["DOFMonteCarloCACameraDispersion"]) == (0, 6.0, 10.0, 1.0)

@@ -73,7 +73,8 @@ INERT = "inert"
 
 
 def _gate(flag, switch=None):
-    """Gated by `switch(par, match)` (True when the key can act); the reader must set `flag`."""
+    """Gated by `switch(par, node)` (True when the key can act; node holds the rendered node's
+    material); the reader must set `flag`."""
     return ("gate", flag, switch)
 
 
@@ -124,10 +125,14 @@ MAIN_RULES = [
                        _v(p, "ambient_occlusion_mode") == 2)),
     (r"hdr_blur_.*", _gate("postprocess", _on("hdr_blur_enabled"))),
     (r"post_chromatic_aberration_.*", _gate("postprocess", _on("post_chromatic_aberration_enabled"))),
-    (r"DOF_(focus|radius|blur_opacity|number_of_passes|HDR|enabled)",
-     _gate("postprocess", _on("DOF_enabled"))),              # post DOF, or the lens of MC DOF
-    (r"DOF_MC_CA_.*", _gate("mc_effects", lambda p, m: _mc(p, m) and _v(p, "DOF_MC_CA_enable"))),
-    (r"MC_soft_shadows_enable", _gate("mc_effects", _mc)),
+    (r"DOF_(blur_opacity|number_of_passes|HDR|max_radius)",  # the post-processing DOF only
+     _gate("postprocess", lambda p, m: _v(p, "DOF_enabled") and not _v(p, "DOF_monte_carlo"))),
+    # the aberration's dispersion gain bends refraction only (render_worker.cpp:1105-1110)
+    # (of the rendered node's material; the switch's second argument holds the node)
+    (r"DOF_MC_CA_dispersion_gain", _gate("reflections", lambda p, node: _mc(p, node) and
+                                         _v(p, "DOF_MC_CA_enable") and _v(p, "raytraced_reflections") and
+                                         (_v(p, "mat%d_reflectance" % node["material"], 0.0) > 0.0 or
+                                          _v(p, "mat%d_transparency_of_surface" % node["material"], 0.0) > 0.0))),
     (r"MC_global_illumination_volumetric|volumetric_light_DE_Factor",
      _gate("glow", lambda p, m: _v(p, "glow_enabled") and _v(p, "glow_intensity") > 0.0)),
     (r"antialiasing_size", _gate("antialiasing", _on("antialiasing_enabled"))),
@@ -264,7 +269,7 @@ def check_scene(text):
                 if rule == INERT:
                     break
                 _, flag, switch = rule
-                if flag in flags or (switch is not None and not switch(par, None)):
+                if flag in flags or (switch is not None and not switch(par, {"material": material})):
                     break
                 problems.append("%s acts without the %s flag" % (name, flag))
                 break

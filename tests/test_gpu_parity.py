@@ -198,6 +198,98 @@ def test_adaptive_monte_carlo_parity(rt, oracle_mod):
     np.testing.assert_array_equal(a["depth"], b["depth"])
 
 
+def _lens(p, dof=True, ca=True, soft=True):
+    """Monte Carlo with the lens focused at the target (DOF_radius 10: a lens of 0.0015 x 10 x
+    focus), chromatic aberration, and soft shadows through a 5 degree cone."""
+    cam = np.array([p.camera[i] for i in range(3)])
+    tgt = np.array([p.target[i] for i in range(3)])
+    p.DOFMonteCarlo = 1
+    p.DOFSamples = 8
+    p.DOFEnabled = int(dof)
+    p.DOFFocus = float(np.linalg.norm(tgt - cam)) * 0.9
+    p.DOFRadius = 10.0
+    p.DOFMonteCarloChromaticAberration = int(ca)
+    p.DOFMonteCarloCACameraDispersion = 2.0
+    p.monteCarloSoftShadows = int(soft)
+    p.light.castShadows = 1
+    p.light.softShadowCone = 5.0
+    return p
+
+
+@pytest.mark.parametrize("which", ["mandelbulb8", "mandelbulb_trig", "mandelbulb_fp64", "mandelbox", "generic"])
+def test_monte_carlo_lens_parity(rt, oracle_mod, which):
+    """Lens rays start off the camera (the march takes its thresholds from the distance to the
+    camera point by point), chromatic aberration tints and bends each sample, and the soft-shadow
+    light vector is jittered per sample: 8 samples per pixel against the oracle on the FP32
+    Mandelbulb kinds, the FP64 Mandelbulb and Mandelbox, and the general kernel (Xenodreambuie)."""
+    if which == "mandelbox":
+        p = scene.mandelbox_shadow_ao(96, 64, seed=1)   # surface about 3 units from the lens
+    else:
+        p = scene.mandelbulb8(96, 72)
+        if which == "mandelbulb_trig":
+            p.fractal.bulb.power = 5.5
+        if which == "generic":
+            p.fractal.formula = abi.MBX_FORMULA_XENODREAMBUIE
+    _lens(p)
+    tune = {rt.MBX_TUNE_PRECISION: 2} if which == "mandelbulb_fp64" else None
+    g = rt.render_to_host(p, tune=tune)
+    r = oracle_mod.render(p)
+    st = compare(g, r)
+    _report("%s lens + CA + MC soft shadows" % which, st)
+    assert g["de_kind"] in {"mandelbulb8": (0,), "mandelbulb_trig": (1,), "mandelbulb_fp64": (5,),
+                            "mandelbox": (2, 3), "generic": (6,)}[which]
+    if which == "mandelbulb_trig":
+        assert st["mask_agree"] >= 0.999 and st["iters_agree"] >= 0.995, st
+        assert st["rgb_within_1e-2"] >= 0.999, st
+    else:
+        _check(st, depth_tol=1e-4 if which == "mandelbulb8" else 1e-6)
+
+
+def test_lens_angle_draw_at_its_maximum(rt, oracle_mod):
+    """Random(65536) draws 0 .. 65536 inclusive: a pixel whose lens angle draw is 65536 (frame seed
+    and pixel found from the oracle's random stream) renders as the oracle's, FP64 march, 1 sample."""
+    p = scene.mandelbulb8(64, 48)
+    hits = np.flatnonzero(oracle_mod.render(p)["mask"].ravel())
+    found = None
+    for fs in range(1, 400):
+        for pix in hits:
+            _, s1 = oracle_mod.random_ints(oracle_mod.pixel_seed(int(pix), fs), 2, 1000)
+            if oracle_mod.random_ints(s1, 2, 65536)[0][1] == 65536:
+                found = (fs, int(pix))
+                break
+        if found:
+            break
+    assert found
+    p.randomSeed = found[0]
+    _lens(p, ca=False, soft=False)
+    p.DOFSamples = 1
+    g = rt.render_to_host(p, tune={rt.MBX_TUNE_PRECISION: 2})
+    r = oracle_mod.render(p)
+    y, x = divmod(found[1], p.imageWidth)
+    assert r["mask"][y, x] and g["mask"][y, x]
+    assert g["depth"][y, x] == r["depth"][y, x]
+    np.testing.assert_array_equal(g["depth"], r["depth"])
+
+
+def test_lens_frames_in_a_batch_equal_single_frames(rt):
+    """A batch holding a lens frame runs every frame through the lens variant of the kernel; each
+    frame equals its single-frame render (plain frames: the lite or full variant) bit for bit."""
+    import torch
+    frames = [_lens(scene.mandelbulb8(64, 48, seed=1)), scene.mandelbulb8(80, 40, seed=2),
+              _lens(scene.mandelbulb8(48, 48, seed=3), dof=False, soft=False)]
+    for kernel in (GRID, PERSISTENT):
+        ctx = rt.Context(0)
+        ctx.tune(rt.MBX_TUNE_KERNEL, kernel)
+        bufs = [ctx.alloc_outputs(p) for p in frames]
+        ctx.render_batch(frames, bufs)
+        torch.cuda.synchronize()
+        for p, b in zip(frames, bufs):
+            single = rt.render_to_host(p)
+            for key in ("rgb", "depth", "iters", "mask", "steps"):
+                np.testing.assert_array_equal(b[key].cpu().numpy(), single[key], err_msg=key)
+        ctx.close()
+
+
 @pytest.mark.parametrize("powers", [(6.0, 7.0, 9.5), (8.0, 8.0, 8.0)])
 @pytest.mark.parametrize("kernel", [GRID, PLAIN, PERSISTENT])
 def test_batch_equals_single_frames(rt, powers, kernel):

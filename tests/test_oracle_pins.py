@@ -635,3 +635,95 @@ def test_interior_mode_keeps_interior_points_open_and_inverts_normals():
         p.interiorMode = 0
         assert float(np.dot(oracle.normal(p, 1.02 * u, 0.05), u)) > 0.95
     p.interiorMode = 1
+
+
+# ------------------------------------------------------------------------------------------
+# Monte Carlo lens, chromatic aberration and soft shadows (render_worker.cpp:226-264,
+# 1832-1844, shader_aux_shadow.cpp:78-87, hsv2rgb.cpp:40-60)
+# ------------------------------------------------------------------------------------------
+def _lens_scene(dof=True, ca=False):
+    p = scene.mandelbulb8(64, 48, seed=2)
+    p.DOFMonteCarlo = 1
+    p.DOFSamples = 4
+    p.DOFEnabled = int(dof)
+    p.DOFFocus = 2.5
+    p.DOFRadius = 10.0
+    p.DOFMonteCarloChromaticAberration = int(ca)
+    p.DOFMonteCarloCACameraDispersion = 3.0
+    return p
+
+
+def test_lens_rays_cross_the_undisturbed_ray_at_the_focus():
+    """Every lens ray starts on the lens disc (radius 0.0015 DOF_radius DOF_focus, in the plane
+    through the camera perpendicular to its forward axis) and meets the ray of the same
+    anti-aliasing draws without the lens at distance DOF_focus along the view vector; the lens
+    takes exactly two more random draws; the radii fill the disc."""
+    p, q = _lens_scene(), _lens_scene(dof=False)
+    m, _, _ = oracle.frame_constants(p)
+    forward = m[:, 1]
+    cam = np.array([p.camera[i] for i in range(3)])
+    rmax = 0.0015 * p.DOFRadius * p.DOFFocus
+    radii = []
+    for seed in range(1, 4000, 13):
+        st, vv, tint, s1 = oracle.sample_ray(p, 17, 29, seed)
+        st0, v0, _, s0 = oracle.sample_ray(q, 17, 29, seed)
+        np.testing.assert_array_equal(st0, cam)
+        np.testing.assert_allclose(st + p.DOFFocus * vv, cam + p.DOFFocus * v0, rtol=0, atol=1e-13)
+        r = st - cam
+        assert abs(np.dot(r, forward)) < 1e-15
+        radii.append(np.linalg.norm(r))
+        assert oracle.random_ints(s0, 2, 65536)[1] == s1
+        np.testing.assert_array_equal(tint, [1.0, 1.0, 1.0])
+    radii = np.array(radii) / rmax
+    assert radii.max() <= 1.0 + 1e-12 and radii.max() > 0.98
+    # sqrt of a uniform number: the radius is uniform over the disc's area, P(r < 1/2) = 1/4
+    assert abs(np.mean(radii < 0.5) - 0.25) < 0.06
+
+
+@pytest.mark.parametrize("hue, expect", [
+    (0.0, (2, 0, 0)), (60.0, (2, 2, 0)), (120.0, (0, 2, 0)), (180.0, (0, 2, 2)), (240.0, (0, 0, 2)),
+    (300.0, (2, 0, 2)), (30.0, (2, 1, 0)), (90.0, (1, 2, 0)), (210.0, (0, 1, 2)), (330.0, (2, 0, 1))])
+def test_hsv2rgb_sectors(hue, expect):
+    """Full saturation at value 2: the primaries and secondaries at multiples of 60 degrees, the
+    halfway colours between them."""
+    np.testing.assert_array_equal(oracle.hsv2rgb(hue, 1.0, 2.0), np.array(expect, np.float32))
+
+
+def test_chromatic_aberration_tints_average_to_white_and_bend_radially():
+    """The tints of the 3600 hues average to white (each channel is 2 over a third of the circle and
+    ramps over two sixths); each sample's view vector is the undisturbed one pulled back along the
+    camera's forward axis, so it stays in their common plane and turns away from that axis."""
+    tints = np.array([oracle.hsv2rgb(np.float32(math.fmod(360.0 + h / 10.0 - 60.0, 360.0)), 1.0, 2.0)
+                      for h in range(3600)], np.float64)
+    np.testing.assert_allclose(tints.mean(axis=0), [1.0, 1.0, 1.0], atol=2e-3)
+    p, q = _lens_scene(dof=False, ca=True), _lens_scene(dof=False)
+    m, _, _ = oracle.frame_constants(p)
+    forward = m[:, 1]
+    seen = set()
+    for seed in range(1, 3000, 7):
+        _, vv, tint, _ = oracle.sample_ray(p, 5, 40, seed)
+        _, v0, _, _ = oracle.sample_ray(q, 5, 40, seed)
+        seen.add(tuple(np.round(tint, 3)))
+        assert abs(np.linalg.norm(vv) - 1.0) < 1e-15
+        assert abs(np.linalg.det(np.stack([vv, v0, forward]))) < 1e-15
+        a = math.acos(min(1.0, np.dot(vv, forward)))
+        a0 = math.acos(min(1.0, np.dot(v0, forward) / np.linalg.norm(v0)))
+        assert a >= a0 - 1e-15
+    assert len(seen) > 100
+
+
+def test_soft_shadow_light_vectors_fill_the_cone():
+    """Monte Carlo soft shadows: the unit light vector plus a point of the ball of radius R =
+    tan(cone): every jittered vector is a unit vector within asin(R) of the light, the extreme is
+    reached, and the jitter is symmetric about the light."""
+    light = np.array([0.48, -0.6, 0.64])
+    for cone_deg in (3.0, 12.0):
+        R = math.tan(math.radians(cone_deg))
+        out = oracle.soft_light(light, R, 11, 20000)
+        np.testing.assert_allclose(np.linalg.norm(out, axis=1), 1.0, atol=1e-15)
+        ang = np.arccos(np.clip(out @ light, -1.0, 1.0))
+        assert ang.max() <= math.asin(R) + 1e-12
+        assert ang.max() > 0.97 * math.asin(R)
+        mean = out.mean(axis=0)
+        np.testing.assert_allclose(mean / np.linalg.norm(mean), light, atol=3e-3 * R / 0.05)
+    np.testing.assert_allclose(oracle.soft_light(light, 0.0, 5, 3), np.tile(light, (3, 1)), atol=1e-15)
