@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MBX_ABI_VERSION 7
+#define MBX_ABI_VERSION 8
 
 /* array sizes as in mandelbulber2/src/fractal.h:52-54 */
 #define MBX_IFS_VECTOR_COUNT 9
@@ -95,8 +95,8 @@ enum mbx_coloring_function {
 	MBX_COLORING_DONUT = 4
 };
 
-/* enumAOMode (mandelbulber2/src/ao_modes.h:42-44); screen-space AO is a post effect and is not
- * rendered by this library */
+/* enumAOMode (mandelbulber2/src/ao_modes.h:42-44); screen-space AO is a post effect over the
+ * finished planes (mbx_render_ssao), the render itself takes no AO term in that mode */
 enum mbx_ao_mode { MBX_AO_FAST = 0, MBX_AO_MULTIPLE_RAYS = 1, MBX_AO_SCREEN_SPACE = 2 };
 
 /* RNG stream layout.  PER_PIXEL: one Park-Miller stream per pixel, seeded from
@@ -389,7 +389,30 @@ typedef struct mbx_params {
 	double DOFMonteCarloCACameraDispersion;    /* "DOF_MC_CA_camera_dispersion" (default 1; a float in
 	                                              sParamRender) */
 	int32_t monteCarloSoftShadows;             /* "MC_soft_shadows_enable" */
-	int32_t reserved6;
+	/* screen-space AO (mbx_render_ssao; cSSAOWorker::doWork, ssao_worker.cpp:63-261): random
+	 * angles and radii per pixel instead of the fixed angle set */
+	int32_t SSAORandomMode;                    /* "SSAO_random_mode" */
+
+	/* fog of VolumetricShader (shader_volumetric.cpp:70-700) along the primary ray's march steps:
+	 * basic fog (opacity step / visibility per step, :228-241, 575-590) and the volumetric
+	 * distance fog (DistanceFogOpacity, shader_distance_fog_opacity.cpp:37-52, colours :325-356,
+	 * blend :556-571).  Iteration fog, clouds, shadows in fog and a volumetric DE factor other
+	 * than 1 are not rendered (the render refuses them). */
+	int32_t basicFogEnabled;                   /* "basic_fog_enabled" */
+	int32_t basicFogCastShadows;               /* "basic_fog_cast_shadows" */
+	double basicFogVisibility;                 /* "basic_fog_visibility" (default 20) */
+	double basicFogColor[3];                   /* "basic_fog_color" */
+	int32_t volFogEnabled;                     /* "volumetric_fog_enabled" */
+	int32_t distanceFogShadows;                /* "distance_fog_shadows" */
+	double volFogDensity;                      /* "volumetric_fog_density" (a float in sParamRender) */
+	double volFogColour1Distance;              /* "volumetric_fog_colour_1_distance" */
+	double volFogColour2Distance;              /* "volumetric_fog_colour_2_distance" */
+	double volFogDistanceFactor;               /* "volumetric_fog_distance_factor" */
+	double volFogDistanceFromSurface;          /* "volumetric_fog_distance_from_surface" */
+	double volFogColour[3][3];                 /* "fog_color_1", "fog_color_2", "fog_color_3" */
+	int32_t iterFogEnabled;                    /* "iteration_fog_enable" (not rendered) */
+	int32_t cloudsEnabled;                     /* "clouds_enable" (not rendered) */
+	double volumetricLightDEFactor;            /* "volumetric_light_DE_Factor" (default 1) */
 } mbx_params;
 
 /* Per-pixel output planes (device pointers; any may be NULL except where noted).
@@ -401,6 +424,12 @@ typedef struct mbx_outputs {
 	uint8_t *mask;   /* 1 = surface hit */
 	float *normal;   /* 3 floats per pixel, world-space unit normal (0 on a miss) */
 	int32_t *steps;  /* raymarching step count per pixel (coarse + binary search) */
+	uint8_t *colour; /* 3 bytes per pixel: the surface colour of the hit as the reference's colour
+	                    buffer holds it, uchar(c * 255) per sample and the integer mean over Monte
+	                    Carlo samples (render_worker.cpp:402-404, 433-435, 470-472); 0 on a miss.
+	                    The input of mbx_render_ssao */
+	uint16_t *opacity; /* fog opacity, ushort(opacity * 65535) per sample and the integer mean
+	                      (render_worker.cpp:400, 432, 469); 0 without fog */
 } mbx_outputs;
 
 /* Work statistics of the last launch on a context (cStatistics analogue,
@@ -570,11 +599,12 @@ int mbx_render_batch_noalloc(mbx_ctx *ctx, const mbx_params *frames, const mbx_o
                                  where the first formula iteration bounds the DE above the value at
                                  which the reference's step clamps to 3, take that step without
                                  evaluating the DE (pixel results are identical); 0 = evaluate */
-#define MBX_TUNE_PRECISION 6  /* Mandelbulb march precision: 0 (default) = FP32, or FP64 position and
-                                 orbit when FP32 rounding of a coordinate near the nearest hit exceeds
-                                 1/256 of the view's smallest distance threshold (deep zooms);
-                                 1 = always FP32; 2 = always FP64.  Mandelbox scenes always march in
-                                 FP64, Kaleidoscopic IFS scenes in FP32 */
+#define MBX_TUNE_PRECISION 6  /* Mandelbulb and Kaleidoscopic IFS march precision: 0 (default) = FP32,
+                                 or FP64 position and orbit when FP32 rounding of a coordinate near
+                                 the nearest hit exceeds 1/256 of the view's smallest distance
+                                 threshold (deep zooms and close-ups; the IFS then takes the general
+                                 FP64 kernel); 1 = always FP32; 2 = always FP64.  Mandelbox scenes
+                                 always march in FP64 */
 #define MBX_TUNE_TIMING 7     /* 1 (default) = bracket each mbx_render / mbx_render_batch with events so
                                  that mbx_get_stats reports its device time; 0 = no events (each event
                                  marker costs the stream ~10 us between launches; kernel_ms reads 0) */
@@ -652,12 +682,14 @@ void mbx_params_default(mbx_params *p);
 #define MBX_FRACT_SPECULAR 0x10u       /* (unused: specular highlights are rendered) */
 #define MBX_FRACT_GLOW 0x20u           /* glow with a volumetric DE factor other than 1, or on GI
                                           bounces (MC_global_illumination_volumetric) */
-#define MBX_FRACT_FOG 0x40u            /* basic, volumetric, iteration or distance fog; clouds */
+#define MBX_FRACT_FOG 0x40u            /* iteration fog, clouds, shadows in fog, or basic / distance
+                                          fog with a volumetric DE factor other than 1 (basic and
+                                          distance fog themselves are rendered) */
 #define MBX_FRACT_REFLECTIONS 0x80u    /* ray-traced reflections or transparency */
 #define MBX_FRACT_LIGHTS 0x100u        /* auxiliary, random, fake or volumetric lights; light #1 not
                                           directional */
 #define MBX_FRACT_TEXTURES 0x200u      /* textured background, environment map, material textures */
-#define MBX_FRACT_AO_SCREEN_SPACE 0x400u /* screen-space AO (a post effect) */
+#define MBX_FRACT_AO_SCREEN_SPACE 0x400u /* (unused: screen-space AO is rendered by mbx_render_ssao) */
 #define MBX_FRACT_AO_LIGHTMAP 0x800u   /* multi-ray AO: its directions and colours come from the
                                           lightmap texture, which the reader cannot decode
                                           (aoVectorsCount is left 0): pass the decoded
@@ -721,6 +753,26 @@ int mbx_compile_image(mbx_ctx *ctx, const mbx_image_adjustments *adj, const floa
  * zbuffer_constant_range false, initparameters.cpp:618-622).  Asynchronous on the context stream. */
 int mbx_compile_zbuffer(mbx_ctx *ctx, const float *depth, int32_t width, int32_t height, int32_t bits,
 	int32_t logarithmic, int32_t invert, int32_t constant_range, double min_depth, double max_depth, void *out);
+
+/* Screen-space ambient occlusion, the post effect cRenderer runs after the render when
+ * ambient_occlusion_mode is 2 (render_image.cpp:498-508; cRenderSSAO::RenderSSAO,
+ * render_ssao.cpp:76-200, over the whole image with no progressive step; per pixel
+ * cSSAOWorker::doWork, ssao_worker.cpp:63-261).  For each pixel with depth below 1e19 the depth
+ * plane is scanned along quality = max(3, ambientOcclusionQuality^2) angles at radii r^2 * W /
+ * quality^2 / 2 pixels (r = 1, 2, ... < quality), the steepest rise of the surface towards the
+ * camera per angle gives atan(max rise), and the mean of 0.5 - atan / pi over the angles that
+ * found a sample is the ambient term; the pixel gains colour / 256 * ambient * ambientOcclusion *
+ * (1 - opacity / 65535) * ambientOcclusionColor.  The image's last column is left as it is, as the
+ * reference's worker leaves it.  params gives the image size, perspectiveType and fov (CalcFOV,
+ * projection_3d.cpp:140-159), the quality, intensity, colour and SSAORandomMode; in random mode
+ * each pixel takes its angle offsets and radius step from its own Park-Miller stream, seeded by
+ * (pixel index, randomSeed ^ 0x55A0A055), where the reference draws them from its one shared
+ * Random() stream.  depth (float), colour (3 x uint8), opacity (uint16, NULL = 0), rgb_in and
+ * rgb_out (3 x float; may be the same buffer) are device planes of imageWidth x imageHeight pixels
+ * as mbx_render writes them.  FP64 arithmetic in the reference's operation order.  Asynchronous on
+ * the context stream.  MBX_ERR_INVALID for a bad size or a NULL plane other than opacity. */
+int mbx_render_ssao(mbx_ctx *ctx, const mbx_params *params, const float *depth, const uint8_t *colour,
+	const uint16_t *opacity, const float *rgb_in, float *rgb_out);
 
 /* ImageFileSavePNG::SavePNG (mandelbulber2/src/file_image.cpp:481-600): write `pixels` (host
  * memory, row-major, `channels` samples per pixel: 3 = RGB as the colour image, 1 = grey as the

@@ -260,7 +260,22 @@ class MbxParams(C.Structure):
         ("DOFRadius", C.c_double),
         ("DOFMonteCarloCACameraDispersion", C.c_double),
         ("monteCarloSoftShadows", C.c_int32),
-        ("reserved6", C.c_int32),
+        ("SSAORandomMode", C.c_int32),
+        ("basicFogEnabled", C.c_int32),
+        ("basicFogCastShadows", C.c_int32),
+        ("basicFogVisibility", C.c_double),
+        ("basicFogColor", D3),
+        ("volFogEnabled", C.c_int32),
+        ("distanceFogShadows", C.c_int32),
+        ("volFogDensity", C.c_double),
+        ("volFogColour1Distance", C.c_double),
+        ("volFogColour2Distance", C.c_double),
+        ("volFogDistanceFactor", C.c_double),
+        ("volFogDistanceFromSurface", C.c_double),
+        ("volFogColour", D3 * 3),
+        ("iterFogEnabled", C.c_int32),
+        ("cloudsEnabled", C.c_int32),
+        ("volumetricLightDEFactor", C.c_double),
     ]
 
 
@@ -282,6 +297,8 @@ class MbxOutputs(C.Structure):
         ("mask", C.c_void_p),
         ("normal", C.c_void_p),
         ("steps", C.c_void_p),
+        ("colour", C.c_void_p),
+        ("opacity", C.c_void_p),
     ]
 
 

@@ -666,6 +666,25 @@ extern "C" void mbx_params_default(mbx_params *p)
 	p->glowIntensity = 0.2;
 	rgb16(p->glowColor1, 40984, 44713, 49490);
 	rgb16(p->glowColor2, 57192, 60888, 62408);
+	// screen-space AO :221; fog :247-272, 315-319; volumetric light DE factor :350
+	p->SSAORandomMode = 0;
+	p->basicFogEnabled = 0;
+	p->basicFogCastShadows = 0;
+	p->basicFogVisibility = 20.0;
+	rgb16(p->basicFogColor, 59399, 61202, 65535);
+	p->volFogEnabled = 0;
+	p->distanceFogShadows = 0;
+	p->volFogDensity = 0.5;
+	p->volFogColour1Distance = 1.0;
+	p->volFogColour2Distance = 2.0;
+	p->volFogDistanceFactor = 1.0;
+	p->volFogDistanceFromSurface = 1e-15;
+	rgb16(p->volFogColour[0], 30000, 30000, 30000);
+	rgb16(p->volFogColour[1], 0, 30000, 65535);
+	rgb16(p->volFogColour[2], 65535, 65535, 65535);
+	p->iterFogEnabled = 0;
+	p->cloudsEnabled = 0;
+	p->volumetricLightDEFactor = 1.0;
 }
 
 extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, mbx_fract_info *info)
@@ -1020,7 +1039,8 @@ extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, 
 	p.ambientOcclusionEnabled = M.b("ambient_occlusion_enabled", false);
 	p.ambientOcclusionMode = (int)M.i("ambient_occlusion_mode", MBX_AO_SCREEN_SPACE);
 	M.rgb("ambient_occlusion_color", p.ambientOcclusionColor, 65535, 65535, 65535);
-	if (p.ambientOcclusionEnabled && p.ambientOcclusionMode == MBX_AO_SCREEN_SPACE) flags |= MBX_FRACT_AO_SCREEN_SPACE;
+	// screen-space AO (mode 2) is rendered by mbx_render_ssao after the render
+	p.SSAORandomMode = M.b("SSAO_random_mode", false);
 	if (p.ambientOcclusionEnabled && p.ambientOcclusionMode == MBX_AO_MULTIPLE_RAYS) flags |= MBX_FRACT_AO_LIGHTMAP;
 
 	// background, fill light, Monte Carlo
@@ -1155,10 +1175,33 @@ extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, 
 		(M.d("volumetric_light_DE_Factor", 1.0, 1e-15, 1e15) != 1.0 ||
 			(p.DOFMonteCarlo && p.DOFMonteCarloGlobalIllumination && M.b("MC_global_illumination_volumetric", false))))
 		flags |= MBX_FRACT_GLOW;
+	// fog (initparameters.cpp:247-272, 315-319): basic and distance fog are rendered, lit by
+	// (1, 1, 1); iteration fog, clouds, shadows in fog and the volumetric shader's own march (a DE
+	// factor other than 1) are not
+	p.basicFogEnabled = M.b("basic_fog_enabled", false);
+	p.basicFogVisibility = M.d("basic_fog_visibility", 20.0, 0.0, 1e15);
+	M.rgb("basic_fog_color", p.basicFogColor, 59399, 61202, 65535);
+	p.basicFogCastShadows = M.b("basic_fog_cast_shadows", false);
+	p.volFogEnabled = M.b("volumetric_fog_enabled", false);
+	p.volFogDensity = M.d("volumetric_fog_density", 0.5, 0.0, 1e15);
+	p.volFogColour1Distance = M.d("volumetric_fog_colour_1_distance", 1.0, 0.0, 1e15);
+	p.volFogColour2Distance = M.d("volumetric_fog_colour_2_distance", 2.0, 0.0, 1e15);
+	p.volFogDistanceFactor = M.d("volumetric_fog_distance_factor", 1.0, 0.0, 1e15);
+	p.volFogDistanceFromSurface = M.d("volumetric_fog_distance_from_surface", 1e-15, 0.0, 1e15);
+	M.rgb("fog_color_1", p.volFogColour[0], 30000, 30000, 30000);
+	M.rgb("fog_color_2", p.volFogColour[1], 0, 30000, 65535);
+	M.rgb("fog_color_3", p.volFogColour[2], 65535, 65535, 65535);
+	p.distanceFogShadows = M.b("distance_fog_shadows", false);
+	p.iterFogEnabled = M.b("iteration_fog_enable", false);
+	p.cloudsEnabled = M.b("clouds_enable", false);
+	p.volumetricLightDEFactor = M.d("volumetric_light_DE_Factor", 1.0, 1e-15, 1e15);
+	{
+		const bool fogOn = p.basicFogEnabled || p.volFogEnabled;
+		if (p.iterFogEnabled || p.cloudsEnabled || (p.basicFogEnabled && p.basicFogCastShadows) ||
+			(p.volFogEnabled && p.distanceFogShadows) || (fogOn && p.volumetricLightDEFactor != 1.0))
+			flags |= MBX_FRACT_FOG;
+	}
 	// other effects the path does not render
-	if (M.b("basic_fog_enabled", false) || M.b("volumetric_fog_enabled", false) ||
-		M.b("iteration_fog_enable", false) || M.b("clouds_enable", false))
-		flags |= MBX_FRACT_FOG;
 	if (M.b("textured_background", false) || M.b("env_mapping_enable", false)) flags |= MBX_FRACT_TEXTURES;
 	// brightness, contrast, gamma, saturation and hdr are read into params.image (mbx_compile_image)
 	// DOF without Monte Carlo is the post-processing blur; with it, the lens of the pixel loop

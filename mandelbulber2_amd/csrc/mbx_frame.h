@@ -187,6 +187,16 @@ struct FrameConst
 	int32_t glow;
 	float glowIntensity;
 	float glowColor1[3], glowColor2[3];
+	// fog along the primary march (shader_volumetric.cpp:228-241, 325-356, 556-590): basic fog
+	// and the volumetric distance fog, lit by (1, 1, 1)
+	int32_t fog;           // either fog is on: the kernel's fog variant accumulates per march step
+	int32_t basicFog, volFog;
+	float glowStepOpacity; // a glow blend's opacity, glowIntensity / 512 * DEFactor clamped to 1
+	double basicFogVisibilityD;
+	float basicFogColor[3];
+	float volFogDensity;
+	double volFogFactorD, volFogFromSurfaceD, volFogColour1DistD, volFogColour2DistD;
+	float volFogColour[3][3];
 
 	// ambient occlusion
 	int32_t aoEnabled, aoMode, aoQuality, aoCount;

@@ -431,6 +431,82 @@ double host_bulb_de(const double p[3], const mbx_fractal &fr, double bailout, in
 	return r / DE;
 }
 
+// Kaleidoscopic IFS DE at one point in FP64 (FormulaCode of fractal_kaleidoscopic_ifs.cpp:30-74
+// inside the Compute loop, IFS finish (r - 2) / DE, compute_fractal.cpp:606-610), for the same
+// precision choice as host_bulb_de.
+double host_ifs_de(const double p[3], const mbx_fractal &fr, double bailout, bool addc, bool julia)
+{
+	const mbx_fractal_ifs &ifs = fr.IFS;
+	Rot rots[MBX_IFS_VECTOR_COUNT], mainRot;
+	D3 dirs[MBX_IFS_VECTOR_COUNT];
+	for (int i = 0; i < MBX_IFS_VECTOR_COUNT; i++)
+	{
+		rots[i].z(ifs.rotations[i][0] * M_PI / 180.0);
+		rots[i].y(ifs.rotations[i][1] * M_PI / 180.0);
+		rots[i].x(ifs.rotations[i][2] * M_PI / 180.0);
+		const D3 d = d3(ifs.direction[i]);
+		const double l = len(d);
+		dirs[i] = l > 0.0 ? mul(d, 1.0 / l) : d;
+	}
+	mainRot.z(ifs.rotation[0] * M_PI / 180.0);
+	mainRot.y(ifs.rotation[1] * M_PI / 180.0);
+	mainRot.x(ifs.rotation[2] * M_PI / 180.0);
+	const D3 off = d3(ifs.offset);
+	D3 z = d3(p);
+	double DE = 1.0, r = len(z);
+	const double sm1 = ifs.scale - 1.0;
+	for (int it = 0; it < fr.formulaMaxiter; it++)
+	{
+		if (ifs.absX) z.x = std::fabs(z.x);
+		if (ifs.absY) z.y = std::fabs(z.y);
+		if (ifs.absZ) z.z = std::fabs(z.z);
+		for (int i = 0; i < MBX_IFS_VECTOR_COUNT; i++)
+		{
+			if (!ifs.enabled[i]) continue;
+			z = rots[i].apply(z);
+			const double length = dot(z, dirs[i]);
+			if (length < ifs.distance[i]) z = sub(z, mul(dirs[i], 2.0 * (length - ifs.distance[i]) * ifs.intensity[i]));
+		}
+		if (ifs.rotationEnabled) z = add(mainRot.apply(sub(z, off)), off);
+		if (ifs.edgeEnabled)
+		{
+			if (ifs.edge[0] > 0.0) z.x = ifs.edge[0] - std::fabs(ifs.edge[0] - z.x);
+			if (ifs.edge[1] > 0.0) z.y = ifs.edge[1] - std::fabs(ifs.edge[1] - z.y);
+			if (ifs.edge[2] > 0.0) z.z = ifs.edge[2] - std::fabs(ifs.edge[2] - z.z);
+		}
+		z = mul(z, ifs.scale);
+		if (ifs.mengerSpongeMode)
+		{
+			z.x -= off.x * sm1;
+			z.y -= off.y * sm1;
+			if (z.z > 0.5 * off.z * sm1) z.z -= off.z * sm1;
+		}
+		else
+			z = sub(z, mul(off, sm1));
+		DE *= std::fabs(ifs.scale);
+		if (addc)
+		{
+			const double *c = julia ? fr.juliaC : p;
+			z = D3{z.x + c[0] * fr.constantFactor[0], z.y + c[1] * fr.constantFactor[1], z.z + c[2] * fr.constantFactor[2]};
+		}
+		r = len(z);
+		if (!std::isfinite(r) || r > bailout) break;
+	}
+	if (!std::isfinite(r) || !(DE > 0.0)) return 0.0;
+	return (r - 2.0) / DE;
+}
+
+// Whether FP32 rounding of a coordinate near the nearest hit (|camera| + DE at the camera, the DE a
+// bound on the distance to the nearest surface) exceeds 1/256 of the view's smallest distance
+// threshold: the FP32 kinds then lose hits and iteration counts, and the view takes an FP64 kernel.
+bool view_needs_fp64(const mbx_params *p, double de0, double threshK)
+{
+	de0 = std::fabs(de0);
+	const double thrMin = p->constantDEThreshold ? p->DEThresh : de0 * threshK;
+	const double mag = std::max(std::fabs(p->camera[0]), std::max(std::fabs(p->camera[1]), std::fabs(p->camera[2]))) + de0;
+	return mag * 0x1p-24 > thrMin / 256.0;
+}
+
 // cAbstractFractal metadata of the formulas of the general path, as the constructors of
 // formula/definition/fractal_<name>.cpp set it (abstract_fractal.h:85-93): DE type, DE
 // function, c addition, default bailout, analytic finish, colour index function, names
@@ -738,7 +814,7 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc, bool forceGe
 	// takes the general FP64 path
 	// the general kernel also takes the Mandelbox with rotated folds
 	// and interior / iteration-threshold modes
-	const bool generic = forceGeneric || p->hybridCount >= 1 || fr.initialWAxis != 0.0 || p->boxFolding ||
+	bool generic = forceGeneric || p->hybridCount >= 1 || fr.initialWAxis != 0.0 || p->boxFolding ||
 		p->sphericalFolding || p->interiorMode || p->iterThreshMode ||
 		(fr.formula == MBX_FORMULA_MANDELBOX && fr.mandelbox.rotationsEnabled) ||
 		(fr.formula != MBX_FORMULA_MANDELBULB && fr.formula != MBX_FORMULA_MANDELBOX &&
@@ -932,6 +1008,15 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc, bool forceGe
 	}
 	fc.ifsEnabledCount = enabledCount;
 
+	// the IFS kind marches in FP32; a close-up that FP32 cannot resolve takes the general FP64
+	// kernel (MBX_TUNE_PRECISION forces either)
+	if (!generic && fr.formula == MBX_FORMULA_KALEIDOSCOPIC_IFS && deType == MBX_DE_ANALYTIC && ctx->precision != 1)
+	{
+		bool fp64 = ctx->precision == 2;
+		if (!fp64) fp64 = view_needs_fp64(p, host_ifs_de(p->camera, fr, fc.bailoutD, fc.addC, fc.julia), fc.threshKD);
+		generic = fp64;
+	}
+
 	// kernel specialisation
 	int kind = -1;
 	if (generic)
@@ -950,13 +1035,8 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc, bool forceGe
 		// camera) exceeds 1/256 of the smallest distance threshold of the view
 		bool fp64 = ctx->precision == 2;
 		if (ctx->precision == 0)
-		{
-			const double de0 = host_bulb_de(p->camera, fr, fc.bailoutD, analyticFunction, fc.addC, fc.julia);
-			const double thrMin = p->constantDEThreshold ? p->DEThresh : de0 * fc.threshKD;
-			const double mag = std::max(std::fabs(p->camera[0]), std::max(std::fabs(p->camera[1]),
-								   std::fabs(p->camera[2]))) + de0;
-			fp64 = mag * 0x1p-24 > thrMin / 256.0;
-		}
+			fp64 = view_needs_fp64(p, host_bulb_de(p->camera, fr, fc.bailoutD, analyticFunction, fc.addC, fc.julia),
+				fc.threshKD);
 		if (fp64) kind = mbx::kBulbD;
 		if (analyticFunction != MBX_ANALYTIC_LOGARITHMIC && analyticFunction != MBX_ANALYTIC_LINEAR)
 			kind = -1;
@@ -1089,16 +1169,47 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc, bool forceGe
 	fc.specRoughness = (float)p->specularMetallicRoughness;
 	set3f(fc.specColor, p->specularColor);
 
-	// glow
+	// glow and fog: VolumetricShader over the primary march's steps (shader_volumetric.cpp:70-700)
 	fc.glow = p->glowEnabled ? 1 : 0;
-	if (fc.glow && !(p->DEFactor >= 1e-9))
-		return fail(ctx, MBX_ERR_UNSUPPORTED, "glow needs DEFactor >= 1e-9 (the step-merging case is not covered)");
+	fc.basicFog = p->basicFogEnabled ? 1 : 0;
+	fc.volFog = p->volFogEnabled ? 1 : 0;
+	fc.fog = (fc.basicFog || fc.volFog) ? 1 : 0;
+	if (fc.glow || fc.fog)
+	{
+		// every march step is at least 0.2 thresholds * DEFactor (0.45 outside interior mode), or
+		// relMinMarchingStep thresholds with advanced quality: with these bounds no step falls
+		// below the 1e-10 thresholds at which the reference merges steps (shader_volumetric.cpp:178)
+		if (!(p->DEFactor >= 1e-9) || (p->advancedQuality && !(p->relMinMarchingStep >= 1e-9)))
+			return fail(ctx, MBX_ERR_UNSUPPORTED,
+				"glow and fog need DEFactor >= 1e-9 and, with advanced quality, relMinMarchingStep >= 1e-9 "
+				"(the step-merging case is not covered)");
+		if (p->volumetricLightDEFactor != 1.0)
+			return fail(ctx, MBX_ERR_UNSUPPORTED,
+				"a volumetric light DE factor other than 1 (the volumetric shader's own march) is not rendered");
+	}
+	if (p->iterFogEnabled || p->cloudsEnabled)
+		return fail(ctx, MBX_ERR_UNSUPPORTED, "iteration fog and clouds are not rendered");
+	if ((fc.basicFog && p->basicFogCastShadows) || (fc.volFog && p->distanceFogShadows))
+		return fail(ctx, MBX_ERR_UNSUPPORTED, "shadows in fog are not rendered");
 	fc.glowIntensity = (float)p->glowIntensity;
 	set3f(fc.glowColor1, p->glowColor1);
 	set3f(fc.glowColor2, p->glowColor2);
+	{
+		const float go = (float)p->glowIntensity / 512.0f * (float)p->DEFactor;
+		fc.glowStepOpacity = go > 1.0f ? 1.0f : go;
+	}
+	fc.basicFogVisibilityD = p->basicFogVisibility;
+	set3f(fc.basicFogColor, p->basicFogColor);
+	fc.volFogDensity = (float)p->volFogDensity;
+	fc.volFogFactorD = p->volFogDistanceFactor;
+	fc.volFogFromSurfaceD = p->volFogDistanceFromSurface;
+	fc.volFogColour1DistD = p->volFogColour1Distance;
+	fc.volFogColour2DistD = p->volFogColour2Distance;
+	for (int k = 0; k < 3; k++) set3f(fc.volFogColour[k], p->volFogColour[k]);
 
-	// ambient occlusion
-	fc.aoEnabled = p->ambientOcclusionEnabled;
+	// ambient occlusion; in screen-space mode the object shader takes no AO term
+	// (shader_object.cpp:108-121) and mbx_render_ssao adds it to the finished image
+	fc.aoEnabled = p->ambientOcclusionEnabled && p->ambientOcclusionMode != MBX_AO_SCREEN_SPACE;
 	fc.aoMode = p->ambientOcclusionMode;
 	fc.aoQuality = p->ambientOcclusionQuality;
 	fc.aoCount = p->aoVectorsCount;
@@ -1109,8 +1220,8 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc, bool forceGe
 		(p->aoVectorsCount < 1 || p->aoVectorsCount > MBX_MAX_AO_VECTORS))
 		return fail(ctx, MBX_ERR_INVALID, "aoVectorsCount out of range");
 	if (p->ambientOcclusionEnabled && p->ambientOcclusionMode != MBX_AO_MULTIPLE_RAYS &&
-		p->ambientOcclusionMode != MBX_AO_FAST)
-		return fail(ctx, MBX_ERR_UNSUPPORTED, "screen-space AO is a post effect, not on this path");
+		p->ambientOcclusionMode != MBX_AO_FAST && p->ambientOcclusionMode != MBX_AO_SCREEN_SPACE)
+		return fail(ctx, MBX_ERR_INVALID, "ambientOcclusionMode must be 0, 1 or 2");
 	for (int i = 0; i < MBX_MAX_AO_VECTORS; i++)
 	{
 		set3f(fc.aoVectors[i], p->aoVectors[i]);
@@ -1207,6 +1318,8 @@ mbx::FrameJob make_job(const FrameConst &fc, const mbx_outputs &o, int tileBase,
 	j.mask = o.mask;
 	j.normal = o.normal;
 	j.steps = o.steps;
+	j.colour = o.colour;
+	j.opacity = o.opacity;
 	return j;
 }
 
@@ -1226,7 +1339,7 @@ bool same_frames(const std::vector<FrameConst> &a, const std::vector<FrameConst>
 // code (full), and the Monte Carlo lens or chromatic aberration besides (lens).
 int frame_variant(const FrameConst &f)
 {
-	if (f.dofEnabled || f.chromaticAberration) return mbx::kVariantLens;
+	if (f.dofEnabled || f.chromaticAberration || f.fog) return mbx::kVariantLens;
 	return (f.monteCarlo || f.aoEnabled || (f.lightEnabled && (f.castShadows || f.specPlastic || f.specMetal)) ||
 			   (f.palette && f.paletteGradient) || f.glow || f.slowShading)
 		? mbx::kVariantFull
@@ -1374,7 +1487,7 @@ int render_frames(mbx_ctx *ctx, const std::vector<FrameConst> &frames, const mbx
 		for (size_t i = 0; i < frames.size(); i++)
 		{
 			const mbx::Outputs o{outs[i].rgb, outs[i].depth, outs[i].iters, outs[i].mask, outs[i].normal,
-				outs[i].steps};
+				outs[i].steps, outs[i].colour, outs[i].opacity};
 			e = mbx::launch_render(kind, ctx->d_frames + i, frames[i].width, frames[i].height, o, counters,
 				ctx->tileShift, frame_variant(frames[i]), ctx->stream);
 			if (e != hipSuccess) return hip_fail(ctx, e, "render launch");
@@ -1446,6 +1559,7 @@ std::string derive_diff(const mbx_params *p, const FrameConst &fc, const double 
 	if ((p->useColorsFromPalette && p->surfaceGradientEnable) || p->glowEnabled ||
 		(p->light.enabled && (p->specularPlasticEnable || p->metallic)))
 		return "the differentiable render has no palette colouring, specular highlights or glow";
+	if (p->basicFogEnabled || p->volFogEnabled) return "the differentiable render has no fog";
 	if (p->backgroundGamma != 1.0) return "the differentiable render needs background gamma 1";
 	if (p->light.enabled && p->light.relativePosition)
 		return "the differentiable render needs the light at an absolute angle (relativePosition 0)";
@@ -1775,7 +1889,7 @@ int mbx_render_batch_noalloc(mbx_ctx *ctx, const mbx_params *params, const mbx_o
 		for (int i = 0; i < n; i++)
 		{
 			const mbx::Outputs o{outs[i].rgb, outs[i].depth, outs[i].iters, outs[i].mask, outs[i].normal,
-				outs[i].steps};
+				outs[i].steps, outs[i].colour, outs[i].opacity};
 			e = mbx::launch_render(kind, dFrames + i, frames[i].width, frames[i].height, o, dCounters,
 				ctx->tileShift, frame_variant(frames[i]), ctx->stream);
 			if (e != hipSuccess) return hip_fail(ctx, e, "render launch");
@@ -2137,6 +2251,51 @@ int mbx_compile_zbuffer(mbx_ctx *ctx, const float *depth, int32_t width, int32_t
 			 constant_range ? 0 : 1, (float)min_depth, (float)max_depth, bits, logarithmic ? 1 : 0, invert ? 1 : 0, out,
 			 ctx->stream)) != hipSuccess)
 		return hip_fail(ctx, e, "z-buffer launch");
+	return MBX_OK;
+}
+
+int mbx_render_ssao(mbx_ctx *ctx, const mbx_params *params, const float *depth, const uint8_t *colour,
+	const uint16_t *opacity, const float *rgb_in, float *rgb_out)
+{
+	if (!ctx || !params || !depth || !colour || !rgb_in || !rgb_out) return fail(ctx, MBX_ERR_INVALID, "bad argument");
+	const int W = params->imageWidth, H = params->imageHeight;
+	if (W <= 0 || H <= 0 || (uint64_t)W * (uint64_t)H > 0xffffffffull)
+		return fail(ctx, MBX_ERR_INVALID, "bad image size");
+	if (params->ambientOcclusionQuality < 1 || params->ambientOcclusionQuality > 10)
+		return fail(ctx, MBX_ERR_INVALID, "ambientOcclusionQuality must be 1 .. 10");
+	if (params->perspectiveType < 0 || params->perspectiveType > 3)
+		return fail(ctx, MBX_ERR_INVALID, "perspectiveType must be 0 .. 3");
+	mbx::SsaoArgs a{};
+	a.depth = depth;
+	a.colour = colour;
+	a.opacity = opacity;
+	a.rgbIn = rgb_in;
+	a.rgbOut = rgb_out;
+	a.width = W;
+	a.height = H;
+	// render_ssao.cpp:126-136 with no progressive pass: quality = int(q^2 * 1.0), at least 3
+	int quality = (int)(params->ambientOcclusionQuality * params->ambientOcclusionQuality * 1.0);
+	if (quality < 3) quality = 3;
+	a.quality = quality;
+	a.perspective = params->perspectiveType;
+	a.randomMode = params->SSAORandomMode ? 1 : 0;
+	a.seed = params->randomSeed ^ 0x55A0A055u;
+	// CalcFOV (projection_3d.cpp:140-159)
+	a.fov = params->perspectiveType == 0 ? 2.0 * std::tan(params->fov / 360.0 * M_PI) : params->fov / 180.0 * M_PI;
+	a.aspectRatio = params->perspectiveType == 2 ? 2.0 : (double)W / H;
+	a.scaleFactor = (double)W / (quality * quality) / 2.0;
+	a.intensity = (float)params->ambientOcclusion;
+	// ambientOcclusionColor as toRGBFloat gives it (a float per channel)
+	for (int k = 0; k < 3; k++) a.aoColor[k] = (float)params->ambientOcclusionColor[k];
+	// the worker's angle table (ssao_worker.cpp:75-81), with the host's libm
+	for (int i = 0; i < quality; i++)
+	{
+		a.sinT[i] = std::sin(double(i) / quality * 2.0 * M_PI);
+		a.cosT[i] = std::cos(double(i) / quality * 2.0 * M_PI);
+	}
+	hipError_t e = hipSetDevice(ctx->device);
+	if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+	if ((e = mbx::launch_ssao(a, ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "SSAO launch");
 	return MBX_OK;
 }
 

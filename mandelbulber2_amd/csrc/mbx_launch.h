@@ -17,11 +17,13 @@ struct Outputs
 	uint8_t *mask;
 	float *normal;
 	int32_t *steps;
+	uint8_t *colour;
+	uint16_t *opacity;
 };
 
 // Kernel variants: without Monte Carlo, GI, shadow and AO code; with it; and with it plus the
 // Monte Carlo lens and chromatic aberration, whose rays start off the camera (a march with
-// per-point thresholds that the other variants do not carry).
+// per-point thresholds that the other variants do not carry), and the fog walk of the march.
 constexpr int kVariantLite = 0;
 constexpr int kVariantFull = 1;
 constexpr int kVariantLens = 2;
@@ -50,6 +52,31 @@ hipError_t launch_compile_image(const float *rgb, uint32_t n, const uint16_t *ga
 hipError_t launch_zbuffer(const float *depth, uint32_t n, uint32_t *range, int useRange, float minDepth,
 	float maxDepth, int bits, int logarithmic, int invert, void *out, hipStream_t stream);
 
+// Screen-space AO over a frame's planes (passed by value: the angle table rides in the kernel
+// arguments).  quality = max(3, ambient_occlusion_quality^2) <= kSsaoMaxQuality angles.
+constexpr int kSsaoMaxQuality = 100;
+struct SsaoArgs
+{
+	const float *depth;
+	const uint8_t *colour;
+	const uint16_t *opacity; // may be null (opacity 0)
+	const float *rgbIn;
+	float *rgbOut;
+	int width, height;
+	int quality;
+	int perspective; // enumPerspectiveType: 0 three-point, 1 fish-eye, 2 equirectangular, 3 fish-eye cut
+	int randomMode;
+	uint32_t seed;   // frame seed of the per-pixel streams (random mode)
+	double fov;      // CalcFOV of the perspective
+	double aspectRatio;
+	double scaleFactor; // width / quality^2 / 2
+	float intensity;
+	float aoColor[3];
+	double cosT[kSsaoMaxQuality]; // cos and sin of 2 pi i / quality, as the host's libm gives them
+	double sinT[kSsaoMaxQuality];
+};
+hipError_t launch_ssao(const SsaoArgs &a, hipStream_t stream);
+
 } // namespace mbx
 
 namespace mbx
@@ -66,6 +93,8 @@ struct FrameJob
 	uint8_t *mask;
 	float *normal;
 	int32_t *steps;
+	uint8_t *colour;
+	uint16_t *opacity;
 };
 
 // Work counters: kCounterSlots slots of kCounterStride u64, each slot on its own 256-byte line;

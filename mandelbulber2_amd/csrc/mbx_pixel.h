@@ -91,14 +91,85 @@ __device__ __forceinline__ typename PrecOf<KIND>::R calc_delta(const FrameConst 
 	return Q::len(Q::cam(f) - p) * Q::deltaK(f);
 }
 
+// Fog of VolumetricShader (shader_volumetric.cpp:70-700) as the march goes.  The reference stores
+// every march step and walks them back from the last to the second, blending per step the glow
+// (:196-204), the distance fog (:556-571) and the basic fog (:575-590) into the shaded colour.
+// Each step's blend is affine in the colour, so the walk equals the composition of those affine
+// maps, which the march accumulates front to back (nearest step first) without a step buffer:
+// colour = A * shaded + gc * G + B with A the product of the steps' transmittances, gc the
+// weight of the glow colour G (known only once the step count is) and B the fogs' own
+// contributions; the fog opacity (opacityOut) is 1 - T with T the product of the fogs'
+// transmittances.  The reference's merge of steps below 1e-10 of the threshold never happens:
+// the host requires DEFactor >= 1e-9 (each step is at least 0.2 thresholds * DEFactor).
+struct FogAcc
+{
+	float A, T, gc;
+	V3 B;
+};
+
+__device__ __forceinline__ void fog_init(FogAcc &acc)
+{
+	acc.A = 1.0f;
+	acc.T = 1.0f;
+	acc.gc = 0.0f;
+	acc.B = v3(0.0f, 0.0f, 0.0f);
+}
+
+// One step of the walk: `step` is the march step that reached the point, `distance` the DE there
+// (the step buffer's entry, render_worker.cpp:788-838).
+__device__ __forceinline__ void fog_step(const FrameConst &f, FogAcc &acc, double step, double distance)
+{
+#pragma clang fp contract(off)
+	const float og = f.glow ? f.glowStepOpacity : 0.0f;
+	float od = 0.0f;
+	V3 cd = v3(0.0f, 0.0f, 0.0f);
+	if (f.volFog && f.volFogDensity > 0.0f)
+	{
+		// DistanceFogOpacity (shader_distance_fog_opacity.cpp:37-52)
+		const double factor = f.volFogFactorD;
+		const double shifted = fabs(distance - f.volFogFromSurfaceD) + 0.1 * f.volFogFromSurfaceD;
+		const float densityTemp = (float)(step * factor / (shifted * shifted + factor * factor));
+		double o = 0.3f * f.volFogDensity * densityTemp / (1.0f + f.volFogDensity * densityTemp) - 0.001;
+		o = o < 0.0 ? 0.0 : (o > 1.0 ? 1.0 : o);
+		od = (float)o;
+		// its colour from the shifted distance (shader_volumetric.cpp:344-355)
+		float k = (float)(shifted / f.volFogColour1DistD);
+		if (k > 1.0f) k = 1.0f;
+		float kn = 1.0f - k;
+		const V3 t = v3(f.volFogColour[0][0] * kn + f.volFogColour[1][0] * k,
+			f.volFogColour[0][1] * kn + f.volFogColour[1][1] * k, f.volFogColour[0][2] * kn + f.volFogColour[1][2] * k);
+		float k2 = (float)(shifted / f.volFogColour2DistD * k);
+		if (k2 > 1.0f) k2 = 1.0f;
+		kn = 1.0f - k2;
+		cd = v3(t.x * kn + f.volFogColour[2][0] * k2, t.y * kn + f.volFogColour[2][1] * k2,
+			t.z * kn + f.volFogColour[2][2] * k2);
+	}
+	float ob = 0.0f;
+	if (f.basicFog)
+	{
+		ob = (float)(step / f.basicFogVisibilityD);
+		if (ob > 1.0f) ob = 1.0f;
+	}
+	// the step's map: x -> (1 - ob) [(1 - od) ((1 - og) x + og G) + od cd] + ob cb
+	const float kb = 1.0f - ob, kd = 1.0f - od;
+	acc.gc = acc.gc + acc.A * (kb * kd * og);
+	const float wd = acc.A * (kb * od), wb = acc.A * ob;
+	acc.B = v3(acc.B.x + wd * cd.x + wb * f.basicFogColor[0], acc.B.y + wd * cd.y + wb * f.basicFogColor[1],
+		acc.B.z + wd * cd.z + wb * f.basicFogColor[2]);
+	acc.A = acc.A * (kb * kd * (1.0f - og));
+	acc.T = acc.T * (kb * kd);
+}
+
 // RayMarching (render_worker.cpp:708-951), exterior mode, binary search on.  In the FP64
 // position precision the arithmetic follows the reference's operation order without FMA
 // contraction, so that march decisions at the DE's discontinuities come out the same.
 // ADV: 0 = advanced quality off, 1 = on (both compiled into the FP32 kinds, so the default march
 // carries no test of the flag), 2 = read from the frame (FP64 kinds: one copy of the march)
-template <int KIND, int ADV>
+// FOG compiles the fog walk in: with fogOn the steps are accumulated into *fog (non-null).
+template <int KIND, int ADV, bool FOG = false>
 __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start, typename PrecOf<KIND>::P dir,
-	typename PrecOf<KIND>::R maxScan, uint32_t &seed, MarchOut<KIND> &out, Work &w, bool fromCamera = true)
+	typename PrecOf<KIND>::R maxScan, uint32_t &seed, MarchOut<KIND> &out, Work &w, bool fromCamera = true,
+	FogAcc *fog = nullptr, bool fogOn = false)
 {
 #pragma clang fp contract(off)
 	using Q = PrecOf<KIND>;
@@ -123,7 +194,7 @@ __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start
 	// first formula iteration (|z_1| >= |p|^P - |c|, escape at i = 0, iters = 1) above the value
 	// that makes the step below clamp to exactly 3, so the step is taken without evaluating it;
 	// the jitter's random number is still drawn, so every later value is unchanged
-	const float farR2 = kExact ? __builtin_inff() : f.farR2;
+	const float farR2 = (kExact || (FOG && fogOn)) ? __builtin_inff() : f.farR2; // fog needs every DE value
 	for (int i = 0; i < maxSteps; i++)
 	{
 		const P last = point;
@@ -161,6 +232,9 @@ __device__ void ray_marching(const FrameConst &f, typename PrecOf<KIND>::P start
 			found = true;
 			break;
 		}
+		// the step buffer's entry i (step that reached the point, DE there); the walk takes 1 ..
+		if constexpr (FOG)
+			if (fogOn && i > 0) fog_step(f, *fog, (double)step, (double)dist);
 		const int rnd = random_mod(seed, 1000);
 		if (kExact)
 		{
@@ -667,15 +741,21 @@ __device__ V3 global_illumination(const FrameConst &frame, typename PrecOf<KIND>
 // effects off: stepCount - 1 identical blends towards the glow colour.  The reference skips a
 // blend only when a march step is below 1e-10 of the distance threshold, which needs a DE factor
 // below 1e-9 (the host rejects glow there).
+__device__ __forceinline__ V3 glow_colour(const FrameConst &f, int stepCount)
+{
+#pragma clang fp contract(off)
+	const float glow = (float)stepCount * f.glowIntensity / 512.0f * f.DEFactor;
+	float glowN = 1.0f - glow;
+	if (glowN < 0.0f) glowN = 0.0f;
+	return v3(f.glowColor1[0] * glowN + f.glowColor2[0] * glow, f.glowColor1[1] * glowN + f.glowColor2[1] * glow,
+		f.glowColor1[2] * glowN + f.glowColor2[2] * glow);
+}
 __device__ __forceinline__ V3 glow_shader(const FrameConst &f, int stepCount, V3 rgb)
 {
 #pragma clang fp contract(off)
 	if (stepCount <= 0) return rgb;
 	const float glow = (float)stepCount * f.glowIntensity / 512.0f * f.DEFactor;
-	float glowN = 1.0f - glow;
-	if (glowN < 0.0f) glowN = 0.0f;
-	const V3 g = v3(f.glowColor1[0] * glowN + f.glowColor2[0] * glow, f.glowColor1[1] * glowN + f.glowColor2[1] * glow,
-		f.glowColor1[2] * glowN + f.glowColor2[2] * glow);
+	const V3 g = glow_colour(f, stepCount);
 	float opacity = glow / (float)stepCount;
 	if (opacity > 1.0f) opacity = 1.0f;
 	const float keep = 1.0f - opacity;
@@ -692,7 +772,17 @@ struct PixelOut
 	int found;
 	int iters;
 	int steps;
+	uint32_t colour[3]; // the colour buffer's bytes (integer mean over the samples)
+	uint32_t opacity;   // the opacity buffer's ushort (integer mean over the samples)
 };
+
+// uchar(c * 255) of the reference's colour buffer (render_worker.cpp:402-404): a float product
+// truncated to an integer whose low byte is kept
+__device__ __forceinline__ uint32_t colour_byte(float c)
+{
+#pragma clang fp contract(off)
+	return (uint32_t)(uint8_t)(int)(c * 255.0f);
+}
 
 // MonteCarloDOFNoiseEstimation (render_worker.cpp:1871-1901) after sample `rep` with the running
 // sum: channels in float as sRGBFloat holds them.  mc_deviation accumulates every sample;
@@ -740,7 +830,7 @@ __device__ __forceinline__ V3 hsv2rgb(float hue, float sat, float val)
 
 // One pixel: the sample loop of doWork and RayRecursion at depth 0.  VAR = 0 is the variant for
 // frames without Monte Carlo sampling, GI, shadows and AO; 1 carries them; 2 adds the Monte Carlo
-// lens and chromatic aberration.
+// lens, chromatic aberration and fog.
 template <int KIND, int VAR>
 __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &po, Work &w)
 {
@@ -761,6 +851,7 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 	V3 acc = v3(0.0f, 0.0f, 0.0f);
 	V3 stdDevSum = v3(0.0f, 0.0f, 0.0f);
 	po.steps = 0;
+	uint32_t colR = 0, colG = 0, colB = 0, opSum = 0;
 	for (int rep = 0; rep < repeats; rep++)
 	{
 		R ix = ox, iy = oy;
@@ -803,12 +894,15 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 		else
 			dir = Q::view(f, ix, iy);
 		MarchOut<KIND> m;
+		FogAcc fogAcc;
+		fog_init(fogAcc);
+		const bool fogOn = LENS && f.fog;
 		if constexpr (PosPrecision<KIND>::kFP64)
-			ray_marching<KIND, 2>(f, start, dir, (R)f.viewDistanceMax, seed, m, w, !lens);
+			ray_marching<KIND, 2, LENS>(f, start, dir, (R)f.viewDistanceMax, seed, m, w, !lens, &fogAcc, fogOn);
 		else if (f.advancedQuality)
-			ray_marching<KIND, 1>(f, start, dir, (R)f.viewDistanceMax, seed, m, w, !lens);
+			ray_marching<KIND, 1, LENS>(f, start, dir, (R)f.viewDistanceMax, seed, m, w, !lens, &fogAcc, fogOn);
 		else
-			ray_marching<KIND, 0>(f, start, dir, (R)f.viewDistanceMax, seed, m, w, !lens);
+			ray_marching<KIND, 0, LENS>(f, start, dir, (R)f.viewDistanceMax, seed, m, w, !lens, &fogAcc, fogOn);
 		V3 rgb;
 		V3 n = v3(0.0f, 0.0f, 0.0f);
 		if (m.found)
@@ -829,10 +923,25 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 				rgb = rgb + global_illumination<KIND>(f, m.point, nP, m.distThresh, colour, seed, w);
 			if (FULL) rgb = rgb + spec; // resultShader = objectShader + specular (render_worker.cpp:1458-1462)
 			rgb = v3(fmaxf(rgb.x, 0.0f), fmaxf(rgb.y, 0.0f), fmaxf(rgb.z, 0.0f));
+			// the colour buffer holds the hit's surface colour; a miss keeps (0, 0, 0)
+			colR += colour_byte(colour.x);
+			colG += colour_byte(colour.y);
+			colB += colour_byte(colour.z);
 		}
 		else
 			rgb = background_shader(f, Q::tof(dir));
-		if (FULL && f.glow) rgb = glow_shader(f, m.stepCount, rgb);
+		if (LENS && f.fog)
+		{
+			// the volumetric walk as accumulated by the march; the glow colour needs the step count
+			const V3 g = glow_colour(f, m.stepCount);
+			rgb = v3(fogAcc.A * rgb.x + fogAcc.gc * g.x + fogAcc.B.x, fogAcc.A * rgb.y + fogAcc.gc * g.y + fogAcc.B.y,
+				fogAcc.A * rgb.z + fogAcc.gc * g.z + fogAcc.B.z);
+			float op = 1.0f - fogAcc.T;
+			op = op > 1.0f ? 1.0f : op;
+			opSum += (uint32_t)(uint16_t)(int)((double)op * 65535.0);
+		}
+		else if (FULL && f.glow)
+			rgb = glow_shader(f, m.stepCount, rgb);
 		if (LENS && f.chromaticAberration)
 		{
 			const V3 tint = hsv2rgb(fmodf(360.0f + hue - 60.0f, 360.0f), 1.0f, 2.0f);
@@ -858,6 +967,12 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 		}
 	}
 	po.rgb = repeats > 1 ? v3(acc.x / (float)repeats, acc.y / (float)repeats, acc.z / (float)repeats) : acc;
+	// integer means of the colour and opacity buffers (render_worker.cpp:469-472)
+	const uint32_t reps = (uint32_t)repeats;
+	po.colour[0] = colR / reps;
+	po.colour[1] = colG / reps;
+	po.colour[2] = colB / reps;
+	po.opacity = opSum / reps;
 }
 
 __device__ __forceinline__ void wave_add_u64(unsigned long long *dst, uint32_t v)
@@ -886,6 +1001,13 @@ __device__ __forceinline__ void write_pixel(const Outputs &out, size_t i, const 
 		out.normal[3 * i + 2] = po.normal.z;
 	}
 	if (out.steps) out.steps[i] = po.steps;
+	if (out.colour)
+	{
+		out.colour[3 * i + 0] = (uint8_t)po.colour[0];
+		out.colour[3 * i + 1] = (uint8_t)po.colour[1];
+		out.colour[3 * i + 2] = (uint8_t)po.colour[2];
+	}
+	if (out.opacity) out.opacity[i] = (uint16_t)po.opacity;
 }
 
 } // namespace mbx

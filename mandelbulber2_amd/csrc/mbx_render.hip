@@ -92,7 +92,7 @@ __global__ __launch_bounds__(64, occupancy_target<KIND>()) void batch_kernel(con
 	{
 		PixelOut po;
 		render_pixel<KIND, VAR>(f, xs, ys, po, w);
-		const Outputs out{job.rgb, job.depth, job.iters, job.mask, job.normal, job.steps};
+		const Outputs out{job.rgb, job.depth, job.iters, job.mask, job.normal, job.steps, job.colour, job.opacity};
 		write_pixel(out, (size_t)ys * (size_t)f.width + (size_t)xs, po);
 	}
 	if (counters)
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(64, occupancy_target<KIND>()) void persistent_kerne
 			{
 				PixelOut po;
 				render_pixel<KIND, VAR>(f, xs, ys, po, w);
-				const Outputs out{job.rgb, job.depth, job.iters, job.mask, job.normal, job.steps};
+				const Outputs out{job.rgb, job.depth, job.iters, job.mask, job.normal, job.steps, job.colour, job.opacity};
 				write_pixel(out, (size_t)ys * (size_t)f.width + (size_t)xs, po);
 			}
 		}

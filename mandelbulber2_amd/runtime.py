@@ -18,7 +18,8 @@ EXPORTS = ["mbx_version", "mbx_sizeof_params", "mbx_ctx_create", "mbx_ctx_destro
            "mbx_get_stats", "mbx_render_batch", "mbx_tune", "mbx_get_tune", "mbx_autotune",
            "mbx_render_diff", "mbx_adjoint", "mbx_params_default", "mbx_load_fract",
            "mbx_compile_image", "mbx_write_png", "mbx_compile_zbuffer", "mbx_render_batch_noalloc",
-           "mbx_formula_count", "mbx_formula_get", "mbx_formula_iterate", "mbx_prepare_ao_vectors"]
+           "mbx_formula_count", "mbx_formula_get", "mbx_formula_iterate", "mbx_prepare_ao_vectors",
+           "mbx_render_ssao"]
 
 _lib = None
 
@@ -64,6 +65,8 @@ def load():
     L.mbx_compile_zbuffer.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                       C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_void_p]
     L.mbx_write_png.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
+    L.mbx_render_ssao.argtypes = [C.c_void_p, P(abi.MbxParams), C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_void_p]
     L.mbx_prepare_ao_vectors.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_int, C.c_int]
     L.mbx_formula_count.argtypes = []
     L.mbx_formula_count.restype = C.c_int
@@ -214,12 +217,14 @@ class Context:
                 "mask": torch.empty((p.imageHeight, p.imageWidth), dtype=torch.uint8, device=dev),
                 "normal": torch.empty((p.imageHeight, p.imageWidth, 3), dtype=torch.float32, device=dev),
                 "steps": torch.empty((p.imageHeight, p.imageWidth), dtype=torch.int32, device=dev),
+                "colour": torch.empty((p.imageHeight, p.imageWidth, 3), dtype=torch.uint8, device=dev),
+                "opacity": torch.empty((p.imageHeight, p.imageWidth), dtype=torch.uint16, device=dev),
                 "_n": n}
 
     @staticmethod
     def outputs_struct(bufs):
         o = abi.MbxOutputs()
-        for name in ("rgb", "depth", "iters", "mask", "normal", "steps"):
+        for name in ("rgb", "depth", "iters", "mask", "normal", "steps", "colour", "opacity"):
             t = bufs.get(name)
             setattr(o, name, t.data_ptr() if t is not None else None)
         return o
@@ -275,6 +280,32 @@ class Context:
                                                min_depth, max_depth, C.c_void_p(out.data_ptr())),
                     "mbx_compile_zbuffer")
         return out
+
+    def render_ssao(self, bufs, params=None, rgb_out=None):
+        """Screen-space AO (mbx_render_ssao) over the planes of a render: bufs holds the (H, W)
+        float32 depth, (H, W, 3) uint8 colour, (H, W) uint16 opacity (or None) and (H, W, 3)
+        float32 rgb tensors; returns the post-effect image (a new tensor unless rgb_out is given,
+        which may be bufs["rgb"] itself)."""
+        torch = self.torch
+        p = params or self.params
+        H, W = p.imageHeight, p.imageWidth
+        checks = (("depth", torch.float32, (H, W)), ("colour", torch.uint8, (H, W, 3)),
+                  ("rgb", torch.float32, (H, W, 3)))
+        for name, dt, shape in checks:
+            t = bufs.get(name)
+            if t is None or t.dtype != dt or not t.is_contiguous() or tuple(t.shape) != shape:
+                raise MbxError("render_ssao: contiguous %s %s plane %s" % (str(dt), name, shape))
+        op = bufs.get("opacity")
+        if op is not None and (op.dtype != torch.uint16 or not op.is_contiguous() or tuple(op.shape) != (H, W)):
+            raise MbxError("render_ssao: contiguous uint16 opacity plane (H, W)")
+        if rgb_out is None:
+            rgb_out = torch.empty_like(bufs["rgb"])
+        self._check(self.L.mbx_render_ssao(self.h, C.byref(p), C.c_void_p(bufs["depth"].data_ptr()),
+                                           C.c_void_p(bufs["colour"].data_ptr()),
+                                           C.c_void_p(op.data_ptr()) if op is not None else None,
+                                           C.c_void_p(bufs["rgb"].data_ptr()),
+                                           C.c_void_p(rgb_out.data_ptr())), "mbx_render_ssao")
+        return rgb_out
 
     def tune(self, key, value):
         self._check(self.L.mbx_tune(self.h, key, value), "mbx_tune")
@@ -357,6 +388,10 @@ def render_to_host(params, device=0, kernel=KERNEL_GRID, tune=None):
         bufs = ctx.alloc_outputs()
         ctx.render(bufs)
         st = ctx.stats()
+        # the post effects cRenderer runs after the render (render_image.cpp:498-508): with
+        # screen-space AO the post image is the render's image plus the SSAO term
+        if params.ambientOcclusionEnabled and params.ambientOcclusionMode == abi.MBX_AO_SCREEN_SPACE:
+            bufs["post_rgb"] = ctx.render_ssao(bufs)
         torch.cuda.synchronize(device)
         out = {k: v.cpu().numpy() for k, v in bufs.items() if not k.startswith("_")}
         out.update(st)

@@ -195,6 +195,25 @@ def default_params(width=640, height=480):
     p.glowIntensity = 0.2
     abi.set3(p.glowColor1, _u16((40984, 44713, 49490)))
     abi.set3(p.glowColor2, _u16((57192, 60888, 62408)))
+    # screen-space AO (initparameters.cpp:221), fog (:247-272, 315-319), volumetric light DE
+    # factor (:350)
+    p.SSAORandomMode = 0
+    p.basicFogEnabled = 0
+    p.basicFogCastShadows = 0
+    p.basicFogVisibility = 20.0
+    abi.set3(p.basicFogColor, _u16((59399, 61202, 65535)))
+    p.volFogEnabled = 0
+    p.distanceFogShadows = 0
+    p.volFogDensity = 0.5
+    p.volFogColour1Distance = 1.0
+    p.volFogColour2Distance = 2.0
+    p.volFogDistanceFactor = 1.0
+    p.volFogDistanceFromSurface = 1e-15
+    for k, c in enumerate(((30000, 30000, 30000), (0, 30000, 65535), (65535, 65535, 65535))):
+        abi.set3(p.volFogColour[k], _u16(c))
+    p.iterFogEnabled = 0
+    p.cloudsEnabled = 0
+    p.volumetricLightDEFactor = 1.0
     return p
 
 

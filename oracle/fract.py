@@ -170,6 +170,7 @@ def init_params():
     c.add("ambient_occlusion_mode", INT, 2)                         # :217-218 (AOModeScreenSpace)
     c.add("ambient_occlusion_color", RGB, (65535, 65535, 65535))    # :219
     c.add("ao_light_map_rotation", DOUBLE, 0.0, -180.0, 180.0)      # :220
+    c.add("SSAO_random_mode", BOOL, False)                          # :221
     c.add("glow_enabled", BOOL, True)                               # :222
     c.add("glow_intensity", DOUBLE, 0.2, 0.0, 1e15)                 # :223
     c.add("glow_color_1", RGB, (40984, 44713, 49490))               # :240
@@ -184,7 +185,16 @@ def init_params():
     c.add("background_color_2", RGB, (65535, 65535, 65535))         # :243
     c.add("background_color_3", RGB, (0, 10000, 500))               # :244
     c.add("background_3_colors_enable", BOOL, True)                 # :245
+    c.add("fog_color_1", RGB, (30000, 30000, 30000))                # :247
+    c.add("fog_color_2", RGB, (0, 30000, 65535))                    # :248
+    c.add("fog_color_3", RGB, (65535, 65535, 65535))                # :249
     c.add("volumetric_fog_enabled", BOOL, False)                    # :251
+    c.add("volumetric_fog_density", DOUBLE, 0.5, 0.0, 1e15)         # :252
+    c.add("volumetric_fog_colour_1_distance", DOUBLE, 1.0, 0.0, 1e15)  # :253
+    c.add("volumetric_fog_colour_2_distance", DOUBLE, 2.0, 0.0, 1e15)  # :254
+    c.add("volumetric_fog_distance_factor", DOUBLE, 1.0, 0.0, 1e15)    # :255
+    c.add("volumetric_fog_distance_from_surface", DOUBLE, 1e-15, 0.0, 1e15)  # :256-257
+    c.add("distance_fog_shadows", BOOL, False)                      # :258
     c.add("iteration_fog_enable", BOOL, False)                      # :261
     c.add("iteration_fog_brightness_boost", DOUBLE, 1.0, 0.0, 1e6)  # :270
     c.add("clouds_enable", BOOL, False)                             # :274
@@ -195,6 +205,9 @@ def init_params():
     c.add("all_lights_visibility", DOUBLE, 1.0, 0.0, 100000.0)      # :311
     c.add("all_lights_size", DOUBLE, 1.0, 0.0, 100000.0)            # :312
     c.add("basic_fog_enabled", BOOL, False)                         # :315
+    c.add("basic_fog_visibility", DOUBLE, 20.0, 0.0, 1e15)          # :316
+    c.add("basic_fog_color", RGB, (59399, 61202, 65535))            # :317
+    c.add("basic_fog_cast_shadows", BOOL, False)                    # :319
     c.add("DOF_enabled", BOOL, False)                               # :322
     c.add("DOF_focus", DOUBLE, 6.0, 0.0, 10000.0)                   # :323
     c.add("DOF_radius", DOUBLE, 10.0, 0.0, 200.0)                   # :324
@@ -865,11 +878,10 @@ def load(text, trace=None):
     out["ambientOcclusionEnabled"] = int(g("ambient_occlusion_enabled"))
     out["ambientOcclusionMode"] = g("ambient_occlusion_mode")
     out["ambientOcclusionColor"] = _rgbf(g("ambient_occlusion_color"))
-    if g("ambient_occlusion_enabled"):
-        if g("ambient_occlusion_mode") == 2:
-            flags |= FLAGS["ao_screen_space"]
-        elif g("ambient_occlusion_mode") == 1:
-            flags |= FLAGS["ao_lightmap"]
+    out["SSAORandomMode"] = int(g("SSAO_random_mode"))
+    # screen-space AO (mode 2) is rendered by mbx_render_ssao after the render
+    if g("ambient_occlusion_enabled") and g("ambient_occlusion_mode") == 1:
+        flags |= FLAGS["ao_lightmap"]
     out["backgroundBrightness"] = g("background_brightness")
     out["backgroundGamma"] = g("background_gamma")
     for i in (1, 2, 3):
@@ -980,8 +992,28 @@ def load(text, trace=None):
             (g("DOF_monte_carlo") and g("DOF_MC_global_illumination") and
              g("MC_global_illumination_volumetric"))):
         flags |= FLAGS["glow"]
-    if g("basic_fog_enabled") or g("volumetric_fog_enabled") or g("iteration_fog_enable") or \
-            g("clouds_enable"):
+    # basic and distance fog are rendered (lit by (1, 1, 1)); iteration fog, clouds, shadows in
+    # fog and the volumetric shader's own march (a DE factor other than 1) are not
+    out["basicFogEnabled"] = int(g("basic_fog_enabled"))
+    out["basicFogVisibility"] = g("basic_fog_visibility")
+    out["basicFogColor"] = _rgbf(g("basic_fog_color"))
+    out["basicFogCastShadows"] = int(g("basic_fog_cast_shadows"))
+    out["volFogEnabled"] = int(g("volumetric_fog_enabled"))
+    out["volFogDensity"] = g("volumetric_fog_density")
+    out["volFogColour1Distance"] = g("volumetric_fog_colour_1_distance")
+    out["volFogColour2Distance"] = g("volumetric_fog_colour_2_distance")
+    out["volFogDistanceFactor"] = g("volumetric_fog_distance_factor")
+    out["volFogDistanceFromSurface"] = g("volumetric_fog_distance_from_surface")
+    out["volFogColour"] = [_rgbf(g("fog_color_%d" % k)) for k in (1, 2, 3)]
+    out["distanceFogShadows"] = int(g("distance_fog_shadows"))
+    out["iterFogEnabled"] = int(g("iteration_fog_enable"))
+    out["cloudsEnabled"] = int(g("clouds_enable"))
+    out["volumetricLightDEFactor"] = g("volumetric_light_DE_Factor")
+    fog_on = g("basic_fog_enabled") or g("volumetric_fog_enabled")
+    if g("iteration_fog_enable") or g("clouds_enable") or \
+            (g("basic_fog_enabled") and g("basic_fog_cast_shadows")) or \
+            (g("volumetric_fog_enabled") and g("distance_fog_shadows")) or \
+            (fog_on and g("volumetric_light_DE_Factor") != 1.0):
         flags |= FLAGS["fog"]
     if g("textured_background") or g("env_mapping_enable"):
         flags |= FLAGS["textures"]

@@ -30,7 +30,10 @@ def lib(fast=False):
     L = C.CDLL(path)
     P = C.POINTER
     L.mbxo_render.argtypes = [P(abi.MbxParams), C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
-                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                              C.c_void_p]
+    L.mbxo_ssao.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                            C.c_int]
     L.mbxo_render.restype = C.c_int
     L.mbxo_distance.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_int, C.c_double, C.c_int,
                                 C.c_void_p, C.c_void_p]
@@ -93,18 +96,43 @@ def render(p, threads=None, rows=None, fast=False):
     mask = np.zeros(n, np.uint8)
     normal = np.zeros((n, 3), np.float32)
     steps = np.zeros(n, np.int32)
+    colour = np.zeros((n, 3), np.uint8)
+    opacity = np.zeros(n, np.uint16)
     stats = np.zeros(3, np.uint64)
     if threads is None:
         threads = os.cpu_count() or 1
     rc = L.mbxo_render(C.byref(p), y0, y1, threads, _ptr(rgb), _ptr(depth), _ptr(iters), _ptr(mask),
-                       _ptr(normal), _ptr(steps), _ptr(stats))
+                       _ptr(normal), _ptr(steps), _ptr(colour), _ptr(opacity), _ptr(stats))
     if rc != 0:
         raise RuntimeError("oracle render failed: %d" % rc)
     sl = slice(y0 * W, y1 * W)
-    return {"rgb": rgb[sl].reshape(-1, W, 3), "depth": depth[sl].reshape(-1, W),
-            "iters": iters[sl].reshape(-1, W), "mask": mask[sl].reshape(-1, W),
-            "normal": normal[sl].reshape(-1, W, 3), "steps": steps[sl].reshape(-1, W),
-            "rays": int(stats[0]), "de_evals": int(stats[1]), "iterations": int(stats[2])}
+    out = {"rgb": rgb[sl].reshape(-1, W, 3), "depth": depth[sl].reshape(-1, W),
+           "iters": iters[sl].reshape(-1, W), "mask": mask[sl].reshape(-1, W),
+           "normal": normal[sl].reshape(-1, W, 3), "steps": steps[sl].reshape(-1, W),
+           "colour": colour[sl].reshape(-1, W, 3), "opacity": opacity[sl].reshape(-1, W),
+           "rays": int(stats[0]), "de_evals": int(stats[1]), "iterations": int(stats[2])}
+    # the post effect cRenderer runs after a whole image (render_image.cpp:498-508)
+    if rows is None and p.ambientOcclusionEnabled and p.ambientOcclusionMode == abi.MBX_AO_SCREEN_SPACE:
+        out["post_rgb"] = ssao(p, out["depth"], out["colour"], out["opacity"], out["rgb"], threads=threads)
+    return out
+
+
+def ssao(p, depth, colour, opacity, rgb, threads=None):
+    """Screen-space AO of cSSAOWorker::doWork over one frame's planes (numpy, as render returns
+    them); returns the post-effect image."""
+    L = lib()
+    W, H = p.imageWidth, p.imageHeight
+    depth = np.ascontiguousarray(depth, np.float32).reshape(H, W)
+    colour = np.ascontiguousarray(colour, np.uint8).reshape(H, W, 3)
+    rgb = np.ascontiguousarray(rgb, np.float32).reshape(H, W, 3)
+    if opacity is not None:
+        opacity = np.ascontiguousarray(opacity, np.uint16).reshape(H, W)
+    out = np.zeros((H, W, 3), np.float32)
+    rc = L.mbxo_ssao(C.byref(p), _ptr(depth), _ptr(colour), _ptr(opacity), _ptr(rgb), _ptr(out),
+                     threads or os.cpu_count() or 1)
+    if rc != 0:
+        raise RuntimeError("oracle ssao failed: %d" % rc)
+    return out
 
 
 def distance(p, pts, detail_size, normal_calc=False):

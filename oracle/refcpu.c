@@ -1409,11 +1409,15 @@ typedef struct
 	v3 point;
 	int steps;
 	int iters;
-	/* sRayMarchingInOut: buffCount and stepBuff[i].step / .point (render_worker.cpp:725-838);
-	 * stepStep / stepPoint may be NULL when the caller needs no step buffer */
+	/* sRayMarchingInOut: buffCount and stepBuff[i].step / .point / .distance / .iters /
+	 * .distThresh (render_worker.cpp:725-838); the arrays may be NULL when the caller needs no
+	 * step buffer */
 	int buffCount;
 	double *stepStep;
 	v3 *stepPoint;
+	double *stepDist;
+	int *stepIters;
+	double *stepThresh;
 } march_t;
 
 static void ray_marching(ray_ctx *rc, v3 start, v3 direction, double minScan, double maxScan,
@@ -1450,6 +1454,13 @@ static void ray_marching(ray_ctx *rc, v3 start, v3 direction, double minScan, do
 		}
 		distThresh = calc_dist_thresh(s, point);
 		dist = de(rc, point, distThresh, 0, &iters);
+		/* the step data the volumetric shader reads (render_worker.cpp:788-790) */
+		if (out->stepDist)
+		{
+			out->stepDist[i] = dist;
+			out->stepIters[i] = iters;
+			out->stepThresh[i] = distThresh;
+		}
 		if (dist < distThresh)
 		{
 			found = 1;
@@ -2066,32 +2077,109 @@ typedef struct
 	int iters;
 	int steps;
 	v3 normal;
+	unsigned colour[3]; /* uchar(objectColour * 255) (render_worker.cpp:402-404) */
+	float opacity;      /* recursionOut.fogOpacity (render_worker.cpp:354, 1788) */
 } ray_result;
 
 /* The glow part of VolumetricShader (shader_volumetric.cpp:70-204) with every other volumetric
  * effect off and volumetric_light_DE_Factor 1: stepCount - 1 blends towards the glow colour,
  * walking the step buffer from the end, each blend taking the steps skipped before it. */
-static void glow_shader(const scene_t *s, const march_t *m, float rgb[3])
+/* DistanceFogOpacity (shader_distance_fog_opacity.cpp:37-52): the float and double
+ * intermediates of the reference */
+static double distance_fog_opacity(double step, double distance, double fromSurface, double factor,
+	float density, double *distanceShifted)
+{
+	*distanceShifted = fabs(distance - fromSurface) + 0.1 * fromSurface;
+	const float densityTemp =
+		(float)(step * factor / (*distanceShifted * *distanceShifted + factor * factor));
+	double o = 0.3f * density * densityTemp / (1.0f + density * densityTemp) - 0.001;
+	if (o < 0.0) o = 0.0;
+	return o;
+}
+
+/* VolumetricShader (shader_volumetric.cpp:70-700) with the step buffer of the primary march,
+ * walked from its last entry back to entry 1: glow (:82-92, 196-204), the volumetric distance
+ * fog (:325-356, 556-571) and the basic fog (:228-241, 575-590), lit by (1, 1, 1) as without
+ * shadows in fog.  Iteration fog, clouds, volumetric and visible lights, fake lights and the
+ * recalculated steps of a volumetric DE factor other than 1 are outside the path.  `opacity`
+ * receives opacityOut.R: the fogs' total opacity, or 0 when the walk has no entry. */
+static void volumetric_shader(const scene_t *s, const march_t *m, float rgb[3], float *opacity)
 {
 	const mbx_params *p = s->p;
 	const int stepCount = m->buffCount;
-	if (!p->glowEnabled || stepCount <= 0) return;
+	*opacity = 0.0f;
+	if (!p->glowEnabled && !p->basicFogEnabled && !p->volFogEnabled) return;
+	float out[3] = {rgb[0], rgb[1], rgb[2]};
+	float totalOpacity = 0.0f;
 	const float glow = stepCount * (float)p->glowIntensity / 512.0f * (float)p->DEFactor;
 	float glowN = 1.0f - glow;
 	if (glowN < 0.0f) glowN = 0.0f;
 	const float g[3] = {(float)p->glowColor1[0] * glowN + (float)p->glowColor2[0] * glow,
 		(float)p->glowColor1[1] * glowN + (float)p->glowColor2[1] * glow,
 		(float)p->glowColor1[2] * glowN + (float)p->glowColor2[2] * glow};
+	const float fogVolDensity = (float)p->volFogDensity;
+	float fc[3][3], basicColour[3];
+	for (int k = 0; k < 3; k++)
+	{
+		basicColour[k] = (float)p->basicFogColor[k];
+		for (int j = 0; j < 3; j++) fc[j][k] = (float)p->volFogColour[j][k];
+	}
 	double totalStep = 0.0;
 	for (int index = stepCount - 1; index > 0; index--)
 	{
-		totalStep += m->stepStep[index];
+		double step = m->stepStep[index];
+		const double distance = m->stepDist[index];
+		totalStep += step;
 		if (totalStep < 1e-10 * calc_dist_thresh(s, m->stepPoint[index])) continue;
+		step = totalStep;
 		totalStep = 0.0;
-		float glowOpacity = (float)(glow / stepCount * 1.0);
-		if (glowOpacity > 1.0f) glowOpacity = 1.0f;
-		for (int k = 0; k < 3; k++) rgb[k] = glowOpacity * g[k] + (1.0f - glowOpacity) * rgb[k];
+		if (p->glowEnabled && stepCount > 0)
+		{
+			float glowOpacity = (float)(glow / stepCount * p->volumetricLightDEFactor);
+			if (glowOpacity > 1.0f) glowOpacity = 1.0f;
+			for (int k = 0; k < 3; k++) out[k] = glowOpacity * g[k] + (1.0f - glowOpacity) * out[k];
+		}
+		float basicFogOpacity = 0.0f;
+		if (p->basicFogEnabled) basicFogOpacity = (float)(step / p->basicFogVisibility);
+		double distFogOpacity = 0.0;
+		float distFogColor[3] = {0.0f, 0.0f, 0.0f};
+		if (fogVolDensity > 0.0f && p->volFogEnabled)
+		{
+			double shifted;
+			distFogOpacity = distance_fog_opacity(step, distance, p->volFogDistanceFromSurface,
+				p->volFogDistanceFactor, fogVolDensity, &shifted);
+			float k = (float)(shifted / p->volFogColour1Distance);
+			if (k > 1) k = 1.0f;
+			float kn = 1.0f - k;
+			float temp[3];
+			for (int c = 0; c < 3; c++) temp[c] = fc[0][c] * kn + fc[1][c] * k;
+			float k2 = (float)(shifted / p->volFogColour2Distance * k);
+			if (k2 > 1) k2 = 1.0;
+			kn = 1.0f - k2;
+			for (int c = 0; c < 3; c++) distFogColor[c] = temp[c] * kn + fc[2][c] * k2;
+		}
+		if (p->volFogEnabled && distFogOpacity > 0.0)
+		{
+			const float light = 1.0f, ao = 0.0f;
+			if (distFogOpacity > 1.0) distFogOpacity = 1.0;
+			for (int c = 0; c < 3; c++)
+				out[c] = (float)(distFogOpacity * distFogColor[c] * (light + ao) + (1.0f - distFogOpacity) * out[c]);
+			totalOpacity = (float)(distFogOpacity + (1.0f - distFogOpacity) * totalOpacity);
+		}
+		if (p->basicFogEnabled && basicFogOpacity > 0.0)
+		{
+			const float light = 1.0f, ao = 0.0f;
+			if (basicFogOpacity > 1.0f) basicFogOpacity = 1.0f;
+			for (int c = 0; c < 3; c++)
+				out[c] = basicFogOpacity * basicColour[c] * (light + ao) + (1.0f - basicFogOpacity) * out[c];
+			totalOpacity = basicFogOpacity + (1.0f - basicFogOpacity) * totalOpacity;
+		}
+		if (totalOpacity > 1.0f) totalOpacity = 1.0f;
+		*opacity = totalOpacity;
 	}
+	rgb[0] = out[0];
+	rgb[1] = out[1];
+	rgb[2] = out[2];
 }
 
 static void trace_primary(ray_ctx *rc, v3 start, v3 viewVector, ray_result *res)
@@ -2100,11 +2188,14 @@ static void trace_primary(ray_ctx *rc, v3 start, v3 viewVector, ray_result *res)
 	const mbx_params *p = s->p;
 	v3 direction = v3_normalize(viewVector);
 	march_t m = {0};
-	if (p->glowEnabled)
+	if (p->glowEnabled || p->basicFogEnabled || p->volFogEnabled)
 	{
 		const size_t n = (size_t)(p->maxRaymarchingSteps > 0 ? p->maxRaymarchingSteps : 0) + 2;
 		m.stepStep = (double *)calloc(n, sizeof(double));
 		m.stepPoint = (v3 *)calloc(n, sizeof(v3));
+		m.stepDist = (double *)calloc(n, sizeof(double));
+		m.stepIters = (int *)calloc(n, sizeof(int));
+		m.stepThresh = (double *)calloc(n, sizeof(double));
 	}
 	ray_marching(rc, start, direction, 0.0, p->viewDistanceMax, 1, &m);
 	shin_t in;
@@ -2141,6 +2232,11 @@ static void trace_primary(ray_ctx *rc, v3 start, v3 viewVector, ray_result *res)
 		res->rgb[2] = B;
 		res->depth = m.depth;
 		res->normal = in.normal;
+		/* the colour buffer: uchar(objectColour * 255), a float product converted to an integer
+		 * whose low byte is kept */
+		res->colour[0] = (unsigned char)(int)(objectColour.R * 255);
+		res->colour[1] = (unsigned char)(int)(objectColour.G * 255);
+		res->colour[2] = (unsigned char)(int)(objectColour.B * 255);
 	}
 	else
 	{
@@ -2150,11 +2246,16 @@ static void trace_primary(ray_ctx *rc, v3 start, v3 viewVector, ray_result *res)
 		res->rgb[2] = bg.B;
 		res->depth = 1e20;
 		res->normal = V3(0, 0, 0);
+		/* a miss keeps the colour the sample started with, (0, 0, 0) (render_worker.cpp:312) */
+		res->colour[0] = res->colour[1] = res->colour[2] = 0;
 	}
 	/* VolumetricShader on the result, outside any object (render_worker.cpp:1775-1779) */
-	glow_shader(s, &m, res->rgb);
+	volumetric_shader(s, &m, res->rgb, &res->opacity);
 	free(m.stepStep);
 	free(m.stepPoint);
+	free(m.stepDist);
+	free(m.stepIters);
+	free(m.stepThresh);
 }
 
 /* CalculateViewVector, three-point perspective (projection_3d.cpp:41-89) */
@@ -2366,6 +2467,7 @@ static void render_pixel(ray_ctx *rc, int xs, int ys, ray_result *out)
 	if (repeats < 1) repeats = 1;
 	float acc[3] = {0, 0, 0};
 	float stdDevSum[3] = {0, 0, 0};
+	unsigned colourSum[3] = {0, 0, 0}, opacitySum = 0;
 	ray_result r;
 	memset(out, 0, sizeof(*out));
 	for (int repeat = 0; repeat < repeats; repeat++)
@@ -2390,6 +2492,10 @@ static void render_pixel(ray_ctx *rc, int xs, int ys, ray_result *out)
 		out->steps += r.steps;
 		/* depth of the last sample is what the reference writes (render_worker.cpp:346-348) */
 		out->depth = r.depth;
+		/* colour and opacity buffers: per-sample bytes and ushorts summed (render_worker.cpp:
+		 * 400-404, 432-435) */
+		for (int k = 0; k < 3; k++) colourSum[k] += r.colour[k];
+		opacitySum += (unsigned short)(r.opacity * 65535.0);
 		/* adaptive Monte Carlo: stop once the noise estimate is low (render_worker.cpp:436-449) */
 		if (p->DOFMonteCarlo)
 		{
@@ -2413,6 +2519,9 @@ static void render_pixel(ray_ctx *rc, int xs, int ys, ray_result *out)
 		out->rgb[1] = acc[1];
 		out->rgb[2] = acc[2];
 	}
+	/* integer means over the samples (render_worker.cpp:469-472) */
+	for (int k = 0; k < 3; k++) out->colour[k] = colourSum[k] / (unsigned)repeats;
+	out->opacity = (float)(opacitySum / (unsigned)repeats);
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -2424,7 +2533,8 @@ static void render_pixel(ray_ctx *rc, int xs, int ys, ray_result *out)
  * (mandelbulber2/src/scheduler.hpp:51-88).  Counters (rays, DE evals, iterations) are
  * accumulated into stats[0..2] when stats is non-NULL. */
 int mbxo_render(const mbx_params *p, int y0, int y1, int threads, float *rgb, float *depth,
-	int32_t *iters, uint8_t *mask, float *normal, int32_t *steps, uint64_t *stats)
+	int32_t *iters, uint8_t *mask, float *normal, int32_t *steps, uint8_t *colour, uint16_t *opacity,
+	uint64_t *stats)
 {
 	scene_t s;
 	int rc0 = scene_init(&s, p);
@@ -2449,6 +2559,8 @@ int mbxo_render(const mbx_params *p, int y0, int y1, int threads, float *rgb, fl
 				if (mask) mask[i] = (uint8_t)r.found;
 				if (normal) { normal[3 * i] = (float)r.normal.x; normal[3 * i + 1] = (float)r.normal.y; normal[3 * i + 2] = (float)r.normal.z; }
 				if (steps) steps[i] = r.steps;
+				if (colour) { colour[3 * i] = (uint8_t)r.colour[0]; colour[3 * i + 1] = (uint8_t)r.colour[1]; colour[3 * i + 2] = (uint8_t)r.colour[2]; }
+				if (opacity) opacity[i] = (uint16_t)r.opacity;
 				n_rays += (uint64_t)(p->DOFMonteCarlo ? p->DOFSamples : 1);
 			}
 		n_de = rc.de_evals;
@@ -2474,6 +2586,8 @@ int mbxo_render(const mbx_params *p, int y0, int y1, int threads, float *rgb, fl
 				if (mask) mask[i] = (uint8_t)r.found;
 				if (normal) { normal[3 * i] = (float)r.normal.x; normal[3 * i + 1] = (float)r.normal.y; normal[3 * i + 2] = (float)r.normal.z; }
 				if (steps) steps[i] = r.steps;
+				if (colour) { colour[3 * i] = (uint8_t)r.colour[0]; colour[3 * i + 1] = (uint8_t)r.colour[1]; colour[3 * i + 2] = (uint8_t)r.colour[2]; }
+				if (opacity) opacity[i] = (uint16_t)r.opacity;
 				n_de += rc.de_evals;
 				n_it += rc.iterations;
 				n_rays += (uint64_t)(p->DOFMonteCarlo ? p->DOFSamples : 1);
@@ -2486,6 +2600,179 @@ int mbxo_render(const mbx_params *p, int y0, int y1, int threads, float *rgb, fl
 		stats[1] += n_de;
 		stats[2] += n_it;
 	}
+	return MBX_OK;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* screen-space AO: cRenderSSAO::RenderSSAO (render_ssao.cpp:76-200) over the whole image with   */
+/* no progressive step, per pixel cSSAOWorker::doWork (ssao_worker.cpp:63-261)                  */
+/* ------------------------------------------------------------------------------------------ */
+
+/* The pixel's projected point and a sample's (ssao_worker.cpp:125-150, 192-218), in the
+ * reference's operation order; `sample` selects the sample branch, whose fish-eye and
+ * equirectangular forms carry the factor pi and whose fish-eye test is on the radius loop's r. */
+static void ssao_point(int persp, double fov, double aspectRatio, int width, int height, double px, double py,
+	double z, int sample, double r, double *ox, double *oy)
+{
+	double x2, y2;
+	if (persp == 1 || persp == 3)
+	{
+		if (sample)
+		{
+			x2 = M_PI * (px / width - 0.5) * aspectRatio;
+			y2 = M_PI * (py / height - 0.5);
+			const double r2 = sqrt(x2 * x2 + y2 * y2);
+			if (r != 0.0)
+			{
+				x2 = x2 / r2 * sin(r2 * fov) * z;
+				y2 = y2 / r2 * sin(r2 * fov) * z;
+			}
+		}
+		else
+		{
+			x2 = (px / width - 0.5) * aspectRatio;
+			y2 = (py / height - 0.5);
+			const double rr = sqrt(x2 * x2 + y2 * y2);
+			if (rr != 0.0)
+			{
+				x2 = x2 / rr * sin(rr * fov) * z;
+				y2 = y2 / rr * sin(rr * fov) * z;
+			}
+		}
+	}
+	else if (persp == 2)
+	{
+		x2 = M_PI * (px / width - 0.5) * aspectRatio;
+		y2 = M_PI * (py / height - 0.5);
+		x2 = sin(fov * x2) * cos(fov * y2) * z;
+		y2 = sin(fov * y2) * z;
+	}
+	else if (sample)
+	{
+		x2 = (px / width - 0.5) * aspectRatio;
+		y2 = py / height - 0.5;
+		x2 = x2 * (z * fov);
+		y2 = y2 * (z * fov);
+	}
+	else
+	{
+		x2 = (px / width - 0.5) * aspectRatio;
+		y2 = py / height - 0.5;
+		x2 = x2 * z * fov;
+		y2 = y2 * z * fov;
+	}
+	*ox = x2;
+	*oy = y2;
+}
+
+/* rgbOut = rgbIn plus the SSAO term, for the planes of one rendered frame (host memory).
+ * opacity may be NULL (0).  In random mode each pixel draws from its own stream seeded by
+ * (pixel index, randomSeed ^ 0x55A0A055). */
+int mbxo_ssao(const mbx_params *p, const float *depth, const uint8_t *colour, const uint16_t *opacity,
+	const float *rgbIn, float *rgbOut, int threads)
+{
+	const int width = p->imageWidth, height = p->imageHeight;
+	if (width <= 0 || height <= 0 || p->ambientOcclusionQuality < 1 || p->ambientOcclusionQuality > 10 ||
+		p->perspectiveType < 0 || p->perspectiveType > 3)
+		return MBX_ERR_INVALID;
+	int quality = (int)(p->ambientOcclusionQuality * p->ambientOcclusionQuality * 1.0);
+	if (quality < 3) quality = 3;
+	double *cosine = (double *)malloc(sizeof(double) * quality);
+	double *sine = (double *)malloc(sizeof(double) * quality);
+	for (int i = 0; i < quality; i++)
+	{
+		sine[i] = sin((double)i / quality * 2.0 * M_PI);
+		cosine[i] = cos((double)i / quality * 2.0 * M_PI);
+	}
+	const int persp = p->perspectiveType;
+	const double scale_factor = (double)width / (quality * quality) / 2.0;
+	double aspectRatio = (double)width / height;
+	if (persp == 2) aspectRatio = 2.0;
+	const double fov = persp == 0 ? 2.0 * tan(p->fov / 360.0 * M_PI) : p->fov / 180.0 * M_PI;
+	const float intensity = (float)p->ambientOcclusion;
+	const float aoColor[3] = {(float)p->ambientOcclusionColor[0], (float)p->ambientOcclusionColor[1],
+		(float)p->ambientOcclusionColor[2]};
+	const int startX = 0, endX = width, startLine = 0, endLine = height;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
+	for (int y = 0; y < height; y++)
+	{
+		for (int x = 0; x < width; x++)
+		{
+			const size_t i = (size_t)y * width + x;
+			const double z = (double)depth[i];
+			const float op = opacity ? opacity[i] / 65535.0f : 0.0f;
+			float total_ambient = 0.0f;
+			if (z < 1e19)
+			{
+				double x2, y2;
+				ssao_point(persp, fov, aspectRatio, width, height, (double)(x - startX), (double)(y - startLine), z, 0,
+					0.0, &x2, &y2);
+				float ambient = 0.0f;
+				const double angleStep = M_PI * 2.0 / (double)quality;
+				const int maxRandom = 62831 / quality;
+				double rRandom = 1.0;
+				uint32_t seed = mbxo_pixel_seed((uint32_t)i, p->randomSeed ^ 0x55A0A055u);
+				if (p->SSAORandomMode) rRandom = 0.5 + mbxo_random(&seed, 65536) / 65536.0;
+				int rayCount = 0;
+				for (int angleIndex = 0; angleIndex < quality; angleIndex++)
+				{
+					double ca, sa;
+					double angle = angleIndex;
+					if (p->SSAORandomMode)
+					{
+						angle = angleStep * angleIndex + mbxo_random(&seed, maxRandom) / 10000.0;
+						ca = cos(angle);
+						sa = sin(angle);
+					}
+					else
+					{
+						ca = cosine[(int)angle];
+						sa = sine[(int)angle];
+					}
+					double max_diff = -1e50;
+					int wasRay = 0;
+					for (double r = 1.0; r < quality; r += rRandom)
+					{
+						const double rr = r * r * scale_factor;
+						const double xx = x + rr * ca;
+						const double yy = y + rr * sa;
+						if ((int)xx == x && (int)yy == y) continue;
+						if (xx < startX || xx > endX - 1 || yy < startLine || yy > endLine - 1) continue;
+						const double z2 = (double)depth[(size_t)(int)yy * width + (int)xx];
+						wasRay = 1;
+						double xx2, yy2;
+						ssao_point(persp, fov, aspectRatio, width, height, xx - startX, yy - startLine, z2, 1, r, &xx2,
+							&yy2);
+						const double dx = xx2 - x2;
+						const double dy = yy2 - y2;
+						const double dz = z2 - z;
+						const double dr = sqrt(dx * dx + dy * dy);
+						const double diff = -dz / dr;
+						if (diff > max_diff) max_diff = diff;
+					}
+					const double max_angle = atan(max_diff);
+					if (wasRay)
+					{
+						ambient += -max_angle / M_PI + 0.5;
+						rayCount++;
+					}
+				}
+				total_ambient = ambient / rayCount;
+				if (total_ambient < 0) total_ambient = 0;
+			}
+			float px[3] = {rgbIn[3 * i], rgbIn[3 * i + 1], rgbIn[3 * i + 2]};
+			if (!(x >= endX - 1))
+			{
+				const float shadeFactor = 1.0f / 256.0f * total_ambient * intensity * (1.0f - op);
+				for (int k = 0; k < 3; k++) px[k] = px[k] + colour[3 * i + k] * shadeFactor * aoColor[k];
+			}
+			rgbOut[3 * i] = px[0];
+			rgbOut[3 * i + 1] = px[1];
+			rgbOut[3 * i + 2] = px[2];
+		}
+	}
+	free(cosine);
+	free(sine);
 	return MBX_OK;
 }
 

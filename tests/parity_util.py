@@ -22,4 +22,16 @@ def compare(gpu, ref):
     out["rgb_abs_median"] = float(np.median(err))
     out["rgb_within_1e-2"] = float((err <= 1e-2).mean())
     out["rgb_within_1e-3"] = float((err <= 1e-3).mean())
+    if "colour" in gpu and "colour" in ref:
+        dc = np.abs(gpu["colour"].astype(np.int32) - ref["colour"].astype(np.int32)).max(axis=-1)
+        out["colour_equal"] = float((dc == 0).mean())
+        out["colour_within_1"] = float((dc <= 1).mean())
+    if "opacity" in gpu and "opacity" in ref:
+        do = np.abs(gpu["opacity"].astype(np.int64) - ref["opacity"].astype(np.int64))
+        out["opacity_max_diff"] = int(do.max())
+        out["opacity_within_2"] = float((do <= 2).mean())
+    if "post_rgb" in gpu and "post_rgb" in ref:
+        e = np.abs(gpu["post_rgb"].astype(np.float64) - ref["post_rgb"].astype(np.float64)).max(axis=-1)
+        out["post_within_1e-2"] = float((e <= 1e-2).mean())
+        out["post_within_1e-3"] = float((e <= 1e-3).mean())
     return out

@@ -33,7 +33,7 @@ def test_library_exports_declared_symbols():
 
 def test_params_layout_matches_library(oracle_mod):
     lib = runtime.load()
-    assert lib.mbx_version() == 7
+    assert lib.mbx_version() == 8
     assert lib.mbx_sizeof_params() == ctypes.sizeof(abi.MbxParams)
     assert oracle_mod.lib().mbxo_sizeof_params() == ctypes.sizeof(abi.MbxParams)
 
@@ -48,13 +48,14 @@ def test_product_does_not_import_oracle():
 
 
 def test_formula_structs_match_c_layout(tmp_path):
-    """mbx_formula_info and mbx_extended_aux: ctypes sizes and field offsets equal the C
-    compiler's (a probe built from include/mbx.h with gcc)."""
+    """mbx_formula_info, mbx_extended_aux, mbx_outputs and mbx_params: ctypes sizes and field
+    offsets equal the C compiler's (a probe built from include/mbx.h with gcc)."""
     import shutil
     import subprocess
     if not shutil.which("gcc"):
         pytest.skip("no C compiler")
-    fields = {"mbx_formula_info": abi.MbxFormulaInfo, "mbx_extended_aux": abi.MbxExtendedAux}
+    fields = {"mbx_formula_info": abi.MbxFormulaInfo, "mbx_extended_aux": abi.MbxExtendedAux,
+              "mbx_outputs": abi.MbxOutputs, "mbx_params": abi.MbxParams}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mbx.h"', "int main(void) {"]
     for cname, cls in fields.items():
         lines.append('printf("%%s sizeof %%zu\\n", "%s", sizeof(%s));' % (cname, cname))

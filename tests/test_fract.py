@@ -149,7 +149,10 @@ def test_mandelbulb001(load):
     assert r["light.enabled"] == 1 and r["light.rotation"] == [-45.0, 45.0, 0.0]
     assert r["light.intensity"] == 1.0
     assert r["ambientOcclusionEnabled"] == 1 and r["ambientOcclusionMode"] == abi.MBX_AO_SCREEN_SPACE
-    assert r.flags() == {"fog", "ao_screen_space"}
+    assert r["ambientOcclusionQuality"] == 7 and r["SSAORandomMode"] == 0
+    # screen-space AO is rendered (mbx_render_ssao); iteration fog is not
+    assert r["iterFogEnabled"] == 1 and r["basicFogEnabled"] == 0 and r["volFogEnabled"] == 0
+    assert r.flags() == {"fog"}
     # image adjustments: float members of sImageAdjustments (fractparams.cpp:155-159)
     assert r["image.brightness"] == float(np.float32(1.28))
     assert r["image.gamma"] == float(np.float32(0.6899999999999999))
@@ -172,7 +175,17 @@ def test_mandelbox001(load):
     assert r["light.enabled"] == 1 and r["light.intensity"] == 4.0
     assert r["light.rotation"] == [-45.0, 45.0, 0.0]
     assert r["reflectionsMax"] == 3
-    assert r.flags() == {"fog", "reflections"}
+    # the volumetric distance fog is rendered: its values as the file writes them, fog colour 1
+    # at its default (initparameters.cpp:247)
+    assert r["volFogEnabled"] == 1 and r["volFogDensity"] == 0.125
+    assert r["volFogColour1Distance"] == 0.00192060288419113
+    assert r["volFogColour2Distance"] == 0.00384120576838226
+    assert r["volFogDistanceFactor"] == 0.00384120576838226
+    assert r["volFogDistanceFromSurface"] == 7.8125e-18
+    assert r["volFogColour"] == [list(_rgb(30000, 30000, 30000)), list(_rgb(0xff00, 0, 0)),
+                                 list(_rgb(0x5500, 0, 0))]
+    assert r["distanceFogShadows"] == 0 and r["volumetricLightDEFactor"] == 1.0
+    assert r.flags() == {"reflections"}
     assert r["image.brightness"] == 2.0 and r["image.hdr"] == 1
     assert r["useColorsFromPalette"] == 0 and r["specular"] == 1.0 and r["metallic"] == 0
 
@@ -207,7 +220,10 @@ def test_pre_2_07_keyframe_file(load):
     assert r["fractal.IFS.rotationEnabled"] == 1      # settings.cpp:1507-1513
     assert r["backgroundColor3"] == _rgb(0x3d00, 0x1c00, 0x500)
     assert r["ambientOcclusion"] == 0.45
-    assert {"animation", "fog", "postprocess", "ao_screen_space"} <= r.flags()
+    assert {"animation", "postprocess"} <= r.flags()
+    assert not {"fog", "ao_screen_space"} & r.flags()
+    assert r["volFogEnabled"] == 1 and r["volFogColour1Distance"] == 0.7762942609158674
+    assert r["volFogColour"][2] == list(_rgb(0xbc00, 0x9700, 0x5b00))
 
 
 def test_mandelbulb_power_8_global_constant_factor(load):

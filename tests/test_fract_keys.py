@@ -104,12 +104,18 @@ MAIN_RULES = [
     # the boolean path takes its node from the name_N temporaries, not from these
     (r"formula_\d+|hybrid_fractal_enable|julia_c", INERT),
     # feature switches
-    (r"(basic_fog|volumetric_fog|iteration_fog|fog_color|clouds|distance_fog|rayleigh_scattering)"
-     r"_?.*", _gate("fog", _on("basic_fog_enabled", "volumetric_fog_enabled",
-                               "iteration_fog_enable", "clouds_enable", "distance_fog_shadows"))),
-    (r"MC_fog_illumination", _gate("fog", lambda p, m: _mc(p, m) and any(
-        _v(p, n) for n in ("basic_fog_enabled", "volumetric_fog_enabled", "iteration_fog_enable",
-                           "clouds_enable")))),
+    # the fog-limiting primitives are commented out in the reference
+    # (shader_volumetric.cpp:229-238, 250-259, 336-342)
+    (r"(basic_fog|distance_fog|iteration_fog)_primitive", INERT),
+    # basic and distance fog are read; iteration fog and clouds act only when switched on
+    (r"(iteration_fog|clouds)_.*", _gate("fog", _on("iteration_fog_enable", "clouds_enable"))),
+    # the scattering colours and the GI of volumetrics light only iteration fog, clouds, shadowed
+    # fog and volumetric lights (shader_volumetric.cpp:364-468; the reader flags the first three,
+    # and volumetric lights raise the lights flag)
+    (r"rayleigh_scattering_.*|MC_fog_illumination", _gate("fog", lambda p, m: _v(p, "iteration_fog_enable") or
+                                                          _v(p, "clouds_enable") or
+                                                          (_v(p, "basic_fog_enabled") and _v(p, "basic_fog_cast_shadows")) or
+                                                          (_v(p, "volumetric_fog_enabled") and _v(p, "distance_fog_shadows")))),
     (r"(fake_lights)_.*", _gate("lights", _on("fake_lights_enabled"))),
     (r"(random_lights)_.*", _gate("lights", _on("random_lights_group"))),
     (r"primitive_.*", _gate("primitives")),
@@ -121,8 +127,6 @@ MAIN_RULES = [
     (r"file_envmap|env_mapping_.*", _gate("textures", _on("env_mapping_enable"))),
     (r"file_lightmap", _gate("ao_lightmap", lambda p, m: _v(p, "ambient_occlusion_enabled") and
                              _v(p, "ambient_occlusion_mode") == 1)),
-    (r"SSAO_.*", _gate("ao_screen_space", lambda p, m: _v(p, "ambient_occlusion_enabled") and
-                       _v(p, "ambient_occlusion_mode") == 2)),
     (r"hdr_blur_.*", _gate("postprocess", _on("hdr_blur_enabled"))),
     (r"post_chromatic_aberration_.*", _gate("postprocess", _on("post_chromatic_aberration_enabled"))),
     (r"DOF_(blur_opacity|number_of_passes|HDR|max_radius)",  # the post-processing DOF only

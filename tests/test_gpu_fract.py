@@ -4,9 +4,11 @@ with the FP64 oracle's render of the same parameter block.
 This mirrors the reference's example test (mandelbulber2/src/test.cpp:88-247), which loads every
 example scene, overrides its image size and renders it: here the fixtures under
 tests/golden/fract are loaded, shrunk to 192 pixels wide (aspect kept), and rendered.  Effects the
-path does not render (mbx_fract_info.unsupported) are absent from both renders; screen-space AO
-is switched off (a post effect); multi-ray AO takes its directions and colours from the default
-lightmap texture (mbx_prepare_ao_vectors on the decoded fixture).  Bars as
+path does not render (mbx_fract_info.unsupported) are absent from both renders (iteration fog,
+clouds and shadows in fog, which the render refuses, are switched off); screen-space AO is the
+post effect of mbx_render_ssao over the rendered planes, checked on the post image; multi-ray AO
+takes its directions and colours from the default lightmap texture (mbx_prepare_ao_vectors on the
+decoded fixture); basic and distance fog are rendered (opacity within 2 of 65535).  Bars as
 tests/test_gpu_parity.py: mask and iterations equal on
 >= 99.9% of pixels (>= 99.5% iterations for the FP32 general-power Mandelbulb, whose asin / atan2 /
 pow run in FP32), depth within 1e-3 relative on >= 99.9% of common hits (the FP64 kinds, Mandelbox
@@ -59,6 +61,9 @@ SCENES = [
     # multi-ray AO with the lightmap's directions and colours
     "IFS_25.fract",
     "mandelbox33_spiral.fract",
+    # basic and volumetric distance fog
+    "IFS_23.fract",
+    "mandelbox55.fract",
 ]
 
 
@@ -88,8 +93,13 @@ def _load(rt, name, width=192):
     if p.ambientOcclusionEnabled and p.ambientOcclusionMode == abi.MBX_AO_MULTIPLE_RAYS:
         # multi-ray AO: the directions and colours of the default lightmap (the scenes name no other)
         rt.prepare_ao_vectors(p, lightmap())
-    elif p.ambientOcclusionEnabled and p.ambientOcclusionMode != abi.MBX_AO_FAST:
-        p.ambientOcclusionEnabled = 0         # screen-space AO is a post effect
+    # the fog the render refuses (the reader flags it): the scene's image without it
+    p.iterFogEnabled = 0
+    p.cloudsEnabled = 0
+    p.basicFogCastShadows = 0
+    p.distanceFogShadows = 0
+    if p.basicFogEnabled or p.volFogEnabled or p.glowEnabled:
+        p.volumetricLightDEFactor = 1.0
     return p, info
 
 
@@ -127,6 +137,10 @@ def test_loaded_example_scene_parity(rt, oracle_mod, name):
     multiray_fp32 = (p.ambientOcclusionEnabled and p.ambientOcclusionMode == abi.MBX_AO_MULTIPLE_RAYS
                      and not fp64)
     assert st["rgb_within_1e-2"] >= (0.998 if multiray_fp32 else 0.999), st
+    if p.basicFogEnabled or p.volFogEnabled:
+        assert st["opacity_within_2"] >= 0.999, st
+    if "post_within_1e-2" in st:
+        assert st["post_within_1e-2"] >= (0.998 if multiray_fp32 else 0.999), st
 
 
 def test_mandelbulb_precision_choice(rt):
@@ -146,6 +160,27 @@ def test_mandelbulb_precision_choice(rt):
     ctx.upload(scene.mandelbulb8(1920, 1080))
     assert ctx.scene_info()[3] == 0          # the 1080p benchmark scene stays on the FP32 kernel
     ctx.close()
+
+
+def test_ifs_precision_choice(rt):
+    """IFS 23 looks at the sponge from 2e-5 away (thresholds near 1e-8 at coordinates near 0.6):
+    it takes the general FP64 kernel (kind 6); the other IFS scenes stay on the FP32 IFS kernel
+    (kind 4) unless FP64 is forced."""
+    want = {"IFS_23.fract": 6, "IFS_001.fract": 4, "menger_sponge_001.fract": 4, "IFS_25.fract": 4}
+    for name, kind in want.items():
+        p, _ = _load(rt, name)
+        ctx = rt.Context(0)
+        try:
+            ctx.upload(p)
+            assert ctx.scene_info()[3] == kind, name
+            ctx.tune(rt.MBX_TUNE_PRECISION, 2)
+            ctx.upload(p)
+            assert ctx.scene_info()[3] == 6, name
+            ctx.tune(rt.MBX_TUNE_PRECISION, 1)
+            ctx.upload(p)
+            assert ctx.scene_info()[3] == 4, name
+        finally:
+            ctx.close()
 
 
 @pytest.mark.parametrize("power", [8.0, 3.0])
