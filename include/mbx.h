@@ -430,6 +430,13 @@ typedef struct mbx_outputs {
 	                    The input of mbx_render_ssao */
 	uint16_t *opacity; /* fog opacity, ushort(opacity * 65535) per sample and the integer mean
 	                      (render_worker.cpp:400, 432, 469); 0 without fog */
+	/* the image's optional channels (cImage optional buffers, render_worker.cpp:355-366, 405-420,
+	 * 474-476, 502-516), 3 floats per pixel each, of the last Monte Carlo sample: */
+	float *world;    /* the ray's end point (the hit, or where the march stopped on a miss) */
+	float *specular; /* the specular highlight of the hit; 0 on a miss */
+	float *shadow;   /* light 1's shadow factor (AuxShadow; 1 without cast shadows, 0 with the light
+	                    off or facing away); 1 on a miss */
+	float *gi;       /* the Monte Carlo GI term, the mean over the samples */
 } mbx_outputs;
 
 /* Work statistics of the last launch on a context (cStatistics analogue,
@@ -781,6 +788,32 @@ int mbx_render_ssao(mbx_ctx *ctx, const mbx_params *params, const float *depth, 
  * MBX_ERR_INVALID for bad arguments, MBX_ERR_IO when the file cannot be written. */
 int mbx_write_png(const char *path, int32_t width, int32_t height, int32_t bits, int32_t channels,
 	const void *pixels);
+
+/* ImageFileSaveTIFF::SaveTIFF (mandelbulber2/src/file_image.cpp:1590-1760): write `pixels` (host
+ * memory, row-major, `channels` contiguous samples per pixel: 1 grey, 3 RGB, 4 RGB + alpha;
+ * `bits` 8 or 16 = uint8_t / uint16_t samples, 32 = float samples) as a baseline little-endian
+ * TIFF with the fields the reference sets: strips of 64 rows deflated (compression 32946),
+ * contiguous planar configuration, MSB-to-LSB fill order, sample format unsigned or IEEE float;
+ * a fourth sample is marked unassociated alpha.  The caller composes the samples as the reference
+ * does for each channel (the 8/16-bit planes of mbx_compile_image for the colour image, those
+ * planes / 65536 as floats for 32 bits).  Host only.  MBX_OK, MBX_ERR_INVALID, MBX_ERR_IO. */
+int mbx_write_tiff(const char *path, int32_t width, int32_t height, int32_t bits, int32_t channels,
+	const void *pixels);
+
+/* ImageFileSaveEXR::SaveEXR (file_image.cpp:1249-1530): write `nchannels` channels as a
+ * single-part scanline OpenEXR file with ZIPS compression.  Channel c is named names[c] (1 to 31
+ * characters, e.g. "R", "G", "B", "A", "Z", "n.X", "s.R" as the reference names them), stored as
+ * HALF (pixel_types[c] 1, rounded to nearest even) or FLOAT (2), read from host memory at
+ * planes[c][(y * width + x) * strides[c]] (so an interleaved RGB plane is three channels with
+ * stride 3); the file lists the channels sorted by name, as the format requires.  linear sets the
+ * channels' pLinear flag (the reference's linear_colorspace).  The reference writes the raw float
+ * image (not the post-processed one) for R, G, B.  Host only.  MBX_OK, MBX_ERR_INVALID (bad size,
+ * type, stride, empty / long / repeated name), MBX_ERR_IO. */
+int mbx_write_exr(const char *path, int32_t width, int32_t height, int32_t nchannels, const char *const *names,
+	const int32_t *pixel_types, const float *const *planes, const int32_t *strides, int32_t linear);
+
+/* The HALF value mbx_write_exr stores for a float (IEEE binary16, round to nearest even). */
+uint16_t mbx_float_to_half(float f);
 
 #ifdef __cplusplus
 }

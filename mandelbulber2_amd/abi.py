@@ -299,6 +299,10 @@ class MbxOutputs(C.Structure):
         ("steps", C.c_void_p),
         ("colour", C.c_void_p),
         ("opacity", C.c_void_p),
+        ("world", C.c_void_p),
+        ("specular", C.c_void_p),
+        ("shadow", C.c_void_p),
+        ("gi", C.c_void_p),
     ]
 
 

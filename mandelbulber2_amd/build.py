@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmbx.so")
 
-SOURCES = ["mbx_render.hip", "mbx_diff.hip", "mbx_image.hip", "mbx_ssao.hip", "mbx_host.cpp", "mbx_fract.cpp", "mbx_png.cpp"]
+SOURCES = ["mbx_render.hip", "mbx_diff.hip", "mbx_image.hip", "mbx_ssao.hip", "mbx_host.cpp", "mbx_fract.cpp", "mbx_png.cpp", "mbx_imgfile.cpp"]
 HEADERS = ["mbx_frame.h", "mbx_device.h", "mbx_launch.h", os.path.join("..", "..", "include", "mbx.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

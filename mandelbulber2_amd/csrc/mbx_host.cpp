@@ -1320,6 +1320,10 @@ mbx::FrameJob make_job(const FrameConst &fc, const mbx_outputs &o, int tileBase,
 	j.steps = o.steps;
 	j.colour = o.colour;
 	j.opacity = o.opacity;
+	j.world = o.world;
+	j.specular = o.specular;
+	j.shadow = o.shadow;
+	j.gi = o.gi;
 	return j;
 }
 
@@ -1487,7 +1491,8 @@ int render_frames(mbx_ctx *ctx, const std::vector<FrameConst> &frames, const mbx
 		for (size_t i = 0; i < frames.size(); i++)
 		{
 			const mbx::Outputs o{outs[i].rgb, outs[i].depth, outs[i].iters, outs[i].mask, outs[i].normal,
-				outs[i].steps, outs[i].colour, outs[i].opacity};
+				outs[i].steps, outs[i].colour, outs[i].opacity, outs[i].world, outs[i].specular, outs[i].shadow,
+				outs[i].gi};
 			e = mbx::launch_render(kind, ctx->d_frames + i, frames[i].width, frames[i].height, o, counters,
 				ctx->tileShift, frame_variant(frames[i]), ctx->stream);
 			if (e != hipSuccess) return hip_fail(ctx, e, "render launch");
@@ -1889,7 +1894,8 @@ int mbx_render_batch_noalloc(mbx_ctx *ctx, const mbx_params *params, const mbx_o
 		for (int i = 0; i < n; i++)
 		{
 			const mbx::Outputs o{outs[i].rgb, outs[i].depth, outs[i].iters, outs[i].mask, outs[i].normal,
-				outs[i].steps, outs[i].colour, outs[i].opacity};
+				outs[i].steps, outs[i].colour, outs[i].opacity, outs[i].world, outs[i].specular, outs[i].shadow,
+				outs[i].gi};
 			e = mbx::launch_render(kind, dFrames + i, frames[i].width, frames[i].height, o, dCounters,
 				ctx->tileShift, frame_variant(frames[i]), ctx->stream);
 			if (e != hipSuccess) return hip_fail(ctx, e, "render launch");

@@ -19,6 +19,10 @@ struct Outputs
 	int32_t *steps;
 	uint8_t *colour;
 	uint16_t *opacity;
+	float *world;    // optional channels: hit point, specular, shadow, GI (3 floats per pixel)
+	float *specular;
+	float *shadow;
+	float *gi;
 };
 
 // Kernel variants: without Monte Carlo, GI, shadow and AO code; with it; and with it plus the
@@ -95,6 +99,10 @@ struct FrameJob
 	int32_t *steps;
 	uint8_t *colour;
 	uint16_t *opacity;
+	float *world;
+	float *specular;
+	float *shadow;
+	float *gi;
 };
 
 // Work counters: kCounterSlots slots of kCounterStride u64, each slot on its own 256-byte line;

@@ -575,10 +575,13 @@ __device__ __forceinline__ float specular_shade(V3 n, V3 view, V3 lv, float expo
 // shader_specular_highlight_combined.cpp:39-68).  colour is the surface colour, view the ray
 // direction; the highlight (already shadowed) goes to spec.  FULL = false compiles the shadow,
 // AO and specular paths out (frames that use none of them), which lowers the kernel's registers.
+// shadowOut (optional) receives light 1's shadow factor as the reference's outShadow holds it:
+// the AuxShadow value, 0 when neither the shade nor the highlight passes 0.001, 1 without cast
+// shadows, 0 with the light off (shader_light_shading.cpp:94-119, render_worker.cpp:1410-1411).
 template <int KIND, bool FULL = true>
 __device__ V3 object_shader(const FrameConst &f, typename PrecOf<KIND>::P p, typename PrecOf<KIND>::P nP,
 	typename PrecOf<KIND>::R thr, typename PrecOf<KIND>::R delta, V3 view, V3 colour, V3 &spec, uint32_t &seed,
-	Work &w)
+	Work &w, float *shadowOut = nullptr)
 {
 	using R = typename PrecOf<KIND>::R;
 	const V3 n = PrecOf<KIND>::tof(nP);
@@ -637,7 +640,10 @@ __device__ V3 object_shader(const FrameConst &f, typename PrecOf<KIND>::P p, typ
 		}
 		light = v3(f.lightColor[0], f.lightColor[1], f.lightColor[2]) * (shade * shadow);
 		if (FULL) spec = v3(specular.x * f.lightColor[0], specular.y * f.lightColor[1], specular.z * f.lightColor[2]);
+		if (shadowOut) *shadowOut = shadow;
 	}
+	else if (shadowOut)
+		*shadowOut = 0.0f;
 	V3 out;
 	out.x = (f.fillLight[0] + ambient.x * f.aoColor[0]) * colour.x + light.x * colour.x + f.luminosity[0];
 	out.y = (f.fillLight[1] + ambient.y * f.aoColor[1]) * colour.y + light.y * colour.y + f.luminosity[1];
@@ -831,8 +837,12 @@ __device__ __forceinline__ V3 hsv2rgb(float hue, float sat, float val)
 // One pixel: the sample loop of doWork and RayRecursion at depth 0.  VAR = 0 is the variant for
 // frames without Monte Carlo sampling, GI, shadows and AO; 1 carries them; 2 adds the Monte Carlo
 // lens, chromatic aberration and fog.
+// The optional channels of out (world position, specular, shadow, GI) are stored per sample as the
+// reference overwrites its per-pixel values (render_worker.cpp:355-366), the GI plane summed over
+// the samples and divided by their count (:474-476), so they hold no registers across the loop.
 template <int KIND, int VAR>
-__device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &po, Work &w)
+__device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &po, Work &w, const Outputs &out,
+	size_t oi)
 {
 #pragma clang fp contract(off)
 	constexpr bool FULL = VAR >= 1;
@@ -905,6 +915,17 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 			ray_marching<KIND, 0, LENS>(f, start, dir, (R)f.viewDistanceMax, seed, m, w, !lens, &fogAcc, fogOn);
 		V3 rgb;
 		V3 n = v3(0.0f, 0.0f, 0.0f);
+		if (out.world)
+		{
+			const V3 pt = Q::tof(m.point);
+			out.world[3 * oi + 0] = pt.x;
+			out.world[3 * oi + 1] = pt.y;
+			out.world[3 * oi + 2] = pt.z;
+		}
+		// the sample's specular, shadow and GI channels: a miss has none, and a shadow of 1
+		// (render_worker.cpp:1606-1612)
+		V3 specCh = v3(0.0f, 0.0f, 0.0f), giCh = v3(0.0f, 0.0f, 0.0f);
+		float shadowCh = 1.0f;
 		if (m.found)
 		{
 			const R delta = calc_delta<KIND>(f, m.point);
@@ -918,10 +939,18 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 			const V3 colour = FULL ? surface_colour<KIND>(f, m.point, w)
 								   : v3(f.surfaceColor[0], f.surfaceColor[1], f.surfaceColor[2]);
 			V3 spec;
-			rgb = object_shader<KIND, FULL>(f, m.point, nP, m.distThresh, delta, Q::tof(dir), colour, spec, seed, w);
+			rgb = object_shader<KIND, FULL>(f, m.point, nP, m.distThresh, delta, Q::tof(dir), colour, spec, seed, w,
+				&shadowCh);
 			if (FULL && f.monteCarlo && f.globalIllumination)
-				rgb = rgb + global_illumination<KIND>(f, m.point, nP, m.distThresh, colour, seed, w);
-			if (FULL) rgb = rgb + spec; // resultShader = objectShader + specular (render_worker.cpp:1458-1462)
+			{
+				giCh = global_illumination<KIND>(f, m.point, nP, m.distThresh, colour, seed, w);
+				rgb = rgb + giCh;
+			}
+			if (FULL)
+			{
+				rgb = rgb + spec; // resultShader = objectShader + specular (render_worker.cpp:1458-1462)
+				specCh = spec;
+			}
 			rgb = v3(fmaxf(rgb.x, 0.0f), fmaxf(rgb.y, 0.0f), fmaxf(rgb.z, 0.0f));
 			// the colour buffer holds the hit's surface colour; a miss keeps (0, 0, 0)
 			colR += colour_byte(colour.x);
@@ -947,6 +976,25 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 			const V3 tint = hsv2rgb(fmodf(360.0f + hue - 60.0f, 360.0f), 1.0f, 2.0f);
 			rgb = v3(rgb.x * tint.x, rgb.y * tint.y, rgb.z * tint.z);
 		}
+		if (out.specular)
+		{
+			out.specular[3 * oi + 0] = specCh.x;
+			out.specular[3 * oi + 1] = specCh.y;
+			out.specular[3 * oi + 2] = specCh.z;
+		}
+		if (out.shadow)
+		{
+			out.shadow[3 * oi + 0] = shadowCh;
+			out.shadow[3 * oi + 1] = shadowCh;
+			out.shadow[3 * oi + 2] = shadowCh;
+		}
+		if (out.gi)
+		{
+			if (rep > 0) giCh = v3(out.gi[3 * oi + 0] + giCh.x, out.gi[3 * oi + 1] + giCh.y, out.gi[3 * oi + 2] + giCh.z);
+			out.gi[3 * oi + 0] = giCh.x;
+			out.gi[3 * oi + 1] = giCh.y;
+			out.gi[3 * oi + 2] = giCh.z;
+		}
 		acc = acc + rgb;
 		if (rep == 0)
 		{
@@ -967,6 +1015,8 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 		}
 	}
 	po.rgb = repeats > 1 ? v3(acc.x / (float)repeats, acc.y / (float)repeats, acc.z / (float)repeats) : acc;
+	if (out.gi && repeats > 1)
+		for (int k = 0; k < 3; k++) out.gi[3 * oi + k] = out.gi[3 * oi + k] / (float)repeats;
 	// integer means of the colour and opacity buffers (render_worker.cpp:469-472)
 	const uint32_t reps = (uint32_t)repeats;
 	po.colour[0] = colR / reps;

@@ -39,8 +39,9 @@ __global__ __launch_bounds__(64, occupancy_target<KIND>()) void render_kernel(co
 	if (xs < f.width && ys < f.height)
 	{
 		PixelOut po;
-		render_pixel<KIND, VAR>(f, xs, ys, po, w);
-		write_pixel(out, (size_t)ys * (size_t)f.width + (size_t)xs, po);
+		const size_t pix = (size_t)ys * (size_t)f.width + (size_t)xs;
+		render_pixel<KIND, VAR>(f, xs, ys, po, w, out, pix);
+		write_pixel(out, pix, po);
 	}
 	if (counters)
 	{
@@ -91,9 +92,11 @@ __global__ __launch_bounds__(64, occupancy_target<KIND>()) void batch_kernel(con
 	if (xs < f.width && ys < f.height)
 	{
 		PixelOut po;
-		render_pixel<KIND, VAR>(f, xs, ys, po, w);
-		const Outputs out{job.rgb, job.depth, job.iters, job.mask, job.normal, job.steps, job.colour, job.opacity};
-		write_pixel(out, (size_t)ys * (size_t)f.width + (size_t)xs, po);
+		const Outputs out{job.rgb, job.depth, job.iters, job.mask, job.normal, job.steps, job.colour, job.opacity,
+			job.world, job.specular, job.shadow, job.gi};
+		const size_t pix = (size_t)ys * (size_t)f.width + (size_t)xs;
+		render_pixel<KIND, VAR>(f, xs, ys, po, w, out, pix);
+		write_pixel(out, pix, po);
 	}
 	if (counters)
 	{
@@ -153,9 +156,11 @@ __global__ __launch_bounds__(64, occupancy_target<KIND>()) void persistent_kerne
 			if (xs < f.width && ys < f.height)
 			{
 				PixelOut po;
-				render_pixel<KIND, VAR>(f, xs, ys, po, w);
-				const Outputs out{job.rgb, job.depth, job.iters, job.mask, job.normal, job.steps, job.colour, job.opacity};
-				write_pixel(out, (size_t)ys * (size_t)f.width + (size_t)xs, po);
+				const Outputs out{job.rgb, job.depth, job.iters, job.mask, job.normal, job.steps, job.colour,
+					job.opacity, job.world, job.specular, job.shadow, job.gi};
+				const size_t pix = (size_t)ys * (size_t)f.width + (size_t)xs;
+				render_pixel<KIND, VAR>(f, xs, ys, po, w, out, pix);
+				write_pixel(out, pix, po);
 			}
 		}
 	}

@@ -19,7 +19,7 @@ EXPORTS = ["mbx_version", "mbx_sizeof_params", "mbx_ctx_create", "mbx_ctx_destro
            "mbx_render_diff", "mbx_adjoint", "mbx_params_default", "mbx_load_fract",
            "mbx_compile_image", "mbx_write_png", "mbx_compile_zbuffer", "mbx_render_batch_noalloc",
            "mbx_formula_count", "mbx_formula_get", "mbx_formula_iterate", "mbx_prepare_ao_vectors",
-           "mbx_render_ssao"]
+           "mbx_render_ssao", "mbx_write_tiff", "mbx_write_exr", "mbx_float_to_half"]
 
 _lib = None
 
@@ -65,6 +65,11 @@ def load():
     L.mbx_compile_zbuffer.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                       C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_void_p]
     L.mbx_write_png.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
+    L.mbx_write_tiff.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
+    L.mbx_write_exr.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, P(C.c_char_p), P(C.c_int32),
+                                P(C.c_void_p), P(C.c_int32), C.c_int32]
+    L.mbx_float_to_half.argtypes = [C.c_float]
+    L.mbx_float_to_half.restype = C.c_uint16
     L.mbx_render_ssao.argtypes = [C.c_void_p, P(abi.MbxParams), C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p]
     L.mbx_prepare_ao_vectors.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_int, C.c_int]
@@ -143,6 +148,52 @@ def write_png(path, pixels):
         raise MbxError("mbx_write_png failed (%d)" % rc)
 
 
+def write_tiff(path, pixels):
+    """ImageFileSaveTIFF::SaveTIFF through mbx_write_tiff: `pixels` is a C-contiguous numpy array
+    of shape (H, W), (H, W, 3) or (H, W, 4), dtype uint8, uint16 or float32."""
+    import numpy as np
+    a = np.ascontiguousarray(pixels)
+    bits = {np.dtype(np.uint8): 8, np.dtype(np.uint16): 16, np.dtype(np.float32): 32}.get(a.dtype)
+    if bits is None or a.ndim not in (2, 3) or (a.ndim == 3 and a.shape[2] not in (3, 4)):
+        raise MbxError("write_tiff: (H, W), (H, W, 3) or (H, W, 4) uint8 / uint16 / float32 pixels")
+    channels = 1 if a.ndim == 2 else a.shape[2]
+    rc = load().mbx_write_tiff(str(path).encode(), a.shape[1], a.shape[0], bits, channels,
+                               a.ctypes.data_as(C.c_void_p))
+    if rc != 0:
+        raise MbxError("mbx_write_tiff failed (%d)" % rc)
+
+
+def write_exr(path, channels, half=True, linear=False):
+    """ImageFileSaveEXR::SaveEXR through mbx_write_exr: `channels` maps channel names to (H, W)
+    float arrays, or a name tuple such as ("R", "G", "B") to an (H, W, 3) array; HALF channels
+    unless half is False."""
+    import numpy as np
+    names, planes, strides, keep = [], [], [], []
+    H = W = None
+    for key, arr in channels.items():
+        a = np.ascontiguousarray(arr, np.float32)
+        keep.append(a)
+        keys = (key,) if isinstance(key, str) else tuple(key)
+        if a.ndim != (2 if len(keys) == 1 else 3) or (a.ndim == 3 and a.shape[2] != len(keys)):
+            raise MbxError("write_exr: channel %s has shape %s" % (key, a.shape))
+        if H is None:
+            H, W = a.shape[0], a.shape[1]
+        if (a.shape[0], a.shape[1]) != (H, W):
+            raise MbxError("write_exr: channels of different sizes")
+        for k, name in enumerate(keys):
+            names.append(name.encode())
+            planes.append(a.ctypes.data + 4 * k)
+            strides.append(len(keys))
+    n = len(names)
+    c_names = (C.c_char_p * n)(*names)
+    c_types = (C.c_int32 * n)(*([1 if half else 2] * n))
+    c_planes = (C.c_void_p * n)(*planes)
+    c_strides = (C.c_int32 * n)(*strides)
+    rc = load().mbx_write_exr(str(path).encode(), W, H, n, c_names, c_types, c_planes, c_strides, int(linear))
+    if rc != 0:
+        raise MbxError("mbx_write_exr failed (%d)" % rc)
+
+
 MBX_TUNE_KERNEL = 0
 MBX_TUNE_WAVES = 1
 MBX_TUNE_GRAPH = 2
@@ -206,12 +257,18 @@ class Context:
                     "mbx_scene_info")
         return m.reshape(3, 3), float(fov[0]), light, kind.value
 
-    def alloc_outputs(self, params=None):
+    def alloc_outputs(self, params=None, optional=False):
+        """Device planes for mbx_render; with optional, also the world / specular / shadow / GI
+        channels (3 floats per pixel each)."""
         torch = self.torch
         p = params or self.params
         n = p.imageWidth * p.imageHeight
         dev = torch.device("cuda", self.device)
-        return {"rgb": torch.empty((p.imageHeight, p.imageWidth, 3), dtype=torch.float32, device=dev),
+        extra = {}
+        if optional:
+            for name in ("world", "specular", "shadow", "gi"):
+                extra[name] = torch.empty((p.imageHeight, p.imageWidth, 3), dtype=torch.float32, device=dev)
+        return {**extra, "rgb": torch.empty((p.imageHeight, p.imageWidth, 3), dtype=torch.float32, device=dev),
                 "depth": torch.empty((p.imageHeight, p.imageWidth), dtype=torch.float32, device=dev),
                 "iters": torch.empty((p.imageHeight, p.imageWidth), dtype=torch.int32, device=dev),
                 "mask": torch.empty((p.imageHeight, p.imageWidth), dtype=torch.uint8, device=dev),
@@ -224,7 +281,8 @@ class Context:
     @staticmethod
     def outputs_struct(bufs):
         o = abi.MbxOutputs()
-        for name in ("rgb", "depth", "iters", "mask", "normal", "steps", "colour", "opacity"):
+        for name in ("rgb", "depth", "iters", "mask", "normal", "steps", "colour", "opacity", "world", "specular",
+                     "shadow", "gi"):
             t = bufs.get(name)
             setattr(o, name, t.data_ptr() if t is not None else None)
         return o
@@ -375,9 +433,10 @@ class Context:
         return d, it
 
 
-def render_to_host(params, device=0, kernel=KERNEL_GRID, tune=None):
+def render_to_host(params, device=0, kernel=KERNEL_GRID, tune=None, optional=False):
     """Upload, render and copy all planes back as numpy arrays (plus work statistics and the
-    kernel kind the scene was specialised for).  `tune` maps MBX_TUNE_* keys to values."""
+    kernel kind the scene was specialised for).  `tune` maps MBX_TUNE_* keys to values; with
+    optional, the world / specular / shadow / GI channels too."""
     import torch
     ctx = Context(device)
     try:
@@ -385,7 +444,7 @@ def render_to_host(params, device=0, kernel=KERNEL_GRID, tune=None):
         for k, v in (tune or {}).items():
             ctx.tune(k, v)
         ctx.upload(params)
-        bufs = ctx.alloc_outputs()
+        bufs = ctx.alloc_outputs(optional=optional)
         ctx.render(bufs)
         st = ctx.stats()
         # the post effects cRenderer runs after the render (render_image.cpp:498-508): with

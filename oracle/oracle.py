@@ -29,9 +29,7 @@ def lib(fast=False):
         build()
     L = C.CDLL(path)
     P = C.POINTER
-    L.mbxo_render.argtypes = [P(abi.MbxParams), C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
-                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                              C.c_void_p]
+    L.mbxo_render.argtypes = [P(abi.MbxParams), C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 13
     L.mbxo_ssao.argtypes = [P(abi.MbxParams), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                             C.c_int]
     L.mbxo_render.restype = C.c_int
@@ -84,8 +82,9 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
 
 
-def render(p, threads=None, rows=None, fast=False):
-    """Render the whole image (or rows=(y0, y1)).  Returns a dict of numpy planes and stats."""
+def render(p, threads=None, rows=None, fast=False, optional=False):
+    """Render the whole image (or rows=(y0, y1)).  Returns a dict of numpy planes and stats; with
+    optional, also the world / specular / shadow / GI channels."""
     L = lib(fast)
     W, H = p.imageWidth, p.imageHeight
     y0, y1 = rows if rows else (0, H)
@@ -98,11 +97,13 @@ def render(p, threads=None, rows=None, fast=False):
     steps = np.zeros(n, np.int32)
     colour = np.zeros((n, 3), np.uint8)
     opacity = np.zeros(n, np.uint16)
+    opt = {k: np.zeros((n, 3), np.float32) for k in ("world", "specular", "shadow", "gi")} if optional else {}
     stats = np.zeros(3, np.uint64)
     if threads is None:
         threads = os.cpu_count() or 1
     rc = L.mbxo_render(C.byref(p), y0, y1, threads, _ptr(rgb), _ptr(depth), _ptr(iters), _ptr(mask),
-                       _ptr(normal), _ptr(steps), _ptr(colour), _ptr(opacity), _ptr(stats))
+                       _ptr(normal), _ptr(steps), _ptr(colour), _ptr(opacity), _ptr(opt.get("world")),
+                       _ptr(opt.get("specular")), _ptr(opt.get("shadow")), _ptr(opt.get("gi")), _ptr(stats))
     if rc != 0:
         raise RuntimeError("oracle render failed: %d" % rc)
     sl = slice(y0 * W, y1 * W)
@@ -111,6 +112,8 @@ def render(p, threads=None, rows=None, fast=False):
            "normal": normal[sl].reshape(-1, W, 3), "steps": steps[sl].reshape(-1, W),
            "colour": colour[sl].reshape(-1, W, 3), "opacity": opacity[sl].reshape(-1, W),
            "rays": int(stats[0]), "de_evals": int(stats[1]), "iterations": int(stats[2])}
+    for k, v in opt.items():
+        out[k] = v[sl].reshape(-1, W, 3)
     # the post effect cRenderer runs after a whole image (render_image.cpp:498-508)
     if rows is None and p.ambientOcclusionEnabled and p.ambientOcclusionMode == abi.MBX_AO_SCREEN_SPACE:
         out["post_rgb"] = ssao(p, out["depth"], out["colour"], out["opacity"], out["rgb"], threads=threads)

@@ -1776,11 +1776,14 @@ static rgba specular_combined(ray_ctx *rc, const shin_t *in, v3 lightVector, rgb
 /* LightShading for one directional light (shader_light_shading.cpp:40-122) with its specular
  * highlight, cLight::CalculateLightVector (light.cpp:277-302) and CalculateCone (light.cpp:214:
  * 1 and white for a directional light), summed by AuxLightsShader (shader_aux_light.cpp:38-63) */
-static rgba light_shading(ray_ctx *rc, const shin_t *in, rgba surfaceColor, rgba *specOut)
+static rgba light_shading(ray_ctx *rc, const shin_t *in, rgba surfaceColor, rgba *specOut, float *shadowOut)
 {
 	const scene_t *s = rc->s;
 	const mbx_params *p = s->p;
 	*specOut = RGBA(0, 0, 0, 0);
+	/* outShadow keeps RayRecursion's default (0, 0, 0, 1) when no light is enabled
+	 * (shader_aux_light.cpp:47-55) */
+	*shadowOut = 0.0f;
 	if (!p->light.enabled) return RGBA(0, 0, 0, 0);
 	const v3 lightVector = s->lightDirection;
 	const double distance = p->light.penetrating ? in->delta / s->resolution : p->viewDistanceMax;
@@ -1822,6 +1825,7 @@ static rgba light_shading(ray_ctx *rc, const shin_t *in, rgba surfaceColor, rgba
 	specOut->G = specular.G * s->lightColor[1];
 	specOut->B = specular.B * s->lightColor[2];
 	specOut->A = 0;
+	*shadowOut = auxShadow; /* *outShadow = auxShadow (shader_light_shading.cpp:119) */
 	return out;
 }
 
@@ -1884,7 +1888,7 @@ static rgba surface_colour(const scene_t *s, v3 point)
  * iridescence and the non-surface gradients off.  Returns the shaded colour; the surface colour,
  * the specular highlight and the emissive luminosity go to the out arguments. */
 static rgba object_shader(ray_ctx *rc, const shin_t *in, rgba *surfaceColour, rgba *specularOut,
-	float luminosityEmissiveOut[3])
+	float luminosityEmissiveOut[3], float *shadowOut)
 {
 	const scene_t *s = rc->s;
 	const mbx_params *p = s->p;
@@ -1904,7 +1908,7 @@ static rgba object_shader(ray_ctx *rc, const shin_t *in, rgba *surfaceColour, rg
 	ambient2.G = ambient.G * aoI * (float)p->ambientOcclusionColor[1];
 	ambient2.B = ambient.B * aoI * (float)p->ambientOcclusionColor[2];
 	rgba auxLightsSpecular;
-	rgba auxLights = light_shading(rc, in, colour, &auxLightsSpecular);
+	rgba auxLights = light_shading(rc, in, colour, &auxLightsSpecular, shadowOut);
 	rgba luminosity;
 	const float lum = (float)p->luminosity;
 	luminosity.R = lum * (float)p->luminosityColor[0];
@@ -2035,7 +2039,8 @@ static void global_illumination(ray_ctx *rc, const shin_t *input, rgba objectCol
 		{
 			rgba oc, specular;
 			float emissive[3];
-			rgba objectShader = object_shader(rc, &ic, &oc, &specular, emissive);
+			float shadowDummy;
+			rgba objectShader = object_shader(rc, &ic, &oc, &specular, emissive, &shadowDummy);
 			newColor = oc;
 			resultShader.R = objectShader.R + specular.R;
 			resultShader.G = objectShader.G + specular.G;
@@ -2079,6 +2084,12 @@ typedef struct
 	v3 normal;
 	unsigned colour[3]; /* uchar(objectColour * 255) (render_worker.cpp:402-404) */
 	float opacity;      /* recursionOut.fogOpacity (render_worker.cpp:354, 1788) */
+	/* optional channels (render_worker.cpp:355-366): the march's end point, the specular
+	 * highlight, light 1's outShadow and the GI term of the sample */
+	v3 world;
+	float spec[3];
+	float shadow;
+	float gi[3];
 } ray_result;
 
 /* The glow part of VolumetricShader (shader_volumetric.cpp:70-204) with every other volumetric
@@ -2208,12 +2219,17 @@ static void trace_primary(ray_ctx *rc, v3 start, v3 viewVector, ray_result *res)
 	res->found = m.found;
 	res->iters = m.iters;
 	res->steps = m.steps;
+	res->world = m.point;
+	/* a miss: no specular, outShadow (1, 1, 1), no GI (render_worker.cpp:1606-1612) */
+	res->spec[0] = res->spec[1] = res->spec[2] = 0.0f;
+	res->gi[0] = res->gi[1] = res->gi[2] = 0.0f;
+	res->shadow = 1.0f;
 	if (m.found)
 	{
 		in.normal = calculate_normals(rc, &in);
 		rgba objectColour, specular;
 		float emissive[3];
-		rgba objectShader = object_shader(rc, &in, &objectColour, &specular, emissive);
+		rgba objectShader = object_shader(rc, &in, &objectColour, &specular, emissive, &res->shadow);
 		if (p->DOFMonteCarlo && p->DOFMonteCarloGlobalIllumination)
 		{
 			float gi[3];
@@ -2221,7 +2237,11 @@ static void trace_primary(ray_ctx *rc, v3 start, v3 viewVector, ray_result *res)
 			objectShader.R += gi[0];
 			objectShader.G += gi[1];
 			objectShader.B += gi[2];
+			for (int k = 0; k < 3; k++) res->gi[k] = gi[k];
 		}
+		res->spec[0] = specular.R;
+		res->spec[1] = specular.G;
+		res->spec[2] = specular.B;
 		/* resultShader = objectShader + specular, clamped at 0 (render_worker.cpp:1458-1602) */
 		float R = objectShader.R + specular.R, G = objectShader.G + specular.G, B = objectShader.B + specular.B;
 		if (R < 0.0f) R = 0.0f;
@@ -2468,6 +2488,7 @@ static void render_pixel(ray_ctx *rc, int xs, int ys, ray_result *out)
 	float acc[3] = {0, 0, 0};
 	float stdDevSum[3] = {0, 0, 0};
 	unsigned colourSum[3] = {0, 0, 0}, opacitySum = 0;
+	float giSum[3] = {0, 0, 0};
 	ray_result r;
 	memset(out, 0, sizeof(*out));
 	for (int repeat = 0; repeat < repeats; repeat++)
@@ -2496,6 +2517,14 @@ static void render_pixel(ray_ctx *rc, int xs, int ys, ray_result *out)
 		 * 400-404, 432-435) */
 		for (int k = 0; k < 3; k++) colourSum[k] += r.colour[k];
 		opacitySum += (unsigned short)(r.opacity * 65535.0);
+		/* the optional channels: the last sample's, GI summed (render_worker.cpp:355-366) */
+		out->world = r.world;
+		for (int k = 0; k < 3; k++)
+		{
+			out->spec[k] = r.spec[k];
+			giSum[k] += r.gi[k];
+		}
+		out->shadow = r.shadow;
 		/* adaptive Monte Carlo: stop once the noise estimate is low (render_worker.cpp:436-449) */
 		if (p->DOFMonteCarlo)
 		{
@@ -2519,8 +2548,9 @@ static void render_pixel(ray_ctx *rc, int xs, int ys, ray_result *out)
 		out->rgb[1] = acc[1];
 		out->rgb[2] = acc[2];
 	}
-	/* integer means over the samples (render_worker.cpp:469-472) */
+	/* integer means over the samples (render_worker.cpp:469-472), the GI mean (:474-476) */
 	for (int k = 0; k < 3; k++) out->colour[k] = colourSum[k] / (unsigned)repeats;
+	for (int k = 0; k < 3; k++) out->gi[k] = p->DOFMonteCarlo ? giSum[k] / repeats : giSum[k];
 	out->opacity = (float)(opacitySum / (unsigned)repeats);
 }
 
@@ -2534,7 +2564,7 @@ static void render_pixel(ray_ctx *rc, int xs, int ys, ray_result *out)
  * accumulated into stats[0..2] when stats is non-NULL. */
 int mbxo_render(const mbx_params *p, int y0, int y1, int threads, float *rgb, float *depth,
 	int32_t *iters, uint8_t *mask, float *normal, int32_t *steps, uint8_t *colour, uint16_t *opacity,
-	uint64_t *stats)
+	float *world, float *specular, float *shadow, float *gi, uint64_t *stats)
 {
 	scene_t s;
 	int rc0 = scene_init(&s, p);
@@ -2561,6 +2591,10 @@ int mbxo_render(const mbx_params *p, int y0, int y1, int threads, float *rgb, fl
 				if (steps) steps[i] = r.steps;
 				if (colour) { colour[3 * i] = (uint8_t)r.colour[0]; colour[3 * i + 1] = (uint8_t)r.colour[1]; colour[3 * i + 2] = (uint8_t)r.colour[2]; }
 				if (opacity) opacity[i] = (uint16_t)r.opacity;
+				if (world) { world[3 * i] = (float)r.world.x; world[3 * i + 1] = (float)r.world.y; world[3 * i + 2] = (float)r.world.z; }
+				if (specular) { specular[3 * i] = r.spec[0]; specular[3 * i + 1] = r.spec[1]; specular[3 * i + 2] = r.spec[2]; }
+				if (shadow) { shadow[3 * i] = shadow[3 * i + 1] = shadow[3 * i + 2] = r.shadow; }
+				if (gi) { gi[3 * i] = r.gi[0]; gi[3 * i + 1] = r.gi[1]; gi[3 * i + 2] = r.gi[2]; }
 				n_rays += (uint64_t)(p->DOFMonteCarlo ? p->DOFSamples : 1);
 			}
 		n_de = rc.de_evals;
@@ -2588,6 +2622,10 @@ int mbxo_render(const mbx_params *p, int y0, int y1, int threads, float *rgb, fl
 				if (steps) steps[i] = r.steps;
 				if (colour) { colour[3 * i] = (uint8_t)r.colour[0]; colour[3 * i + 1] = (uint8_t)r.colour[1]; colour[3 * i + 2] = (uint8_t)r.colour[2]; }
 				if (opacity) opacity[i] = (uint16_t)r.opacity;
+				if (world) { world[3 * i] = (float)r.world.x; world[3 * i + 1] = (float)r.world.y; world[3 * i + 2] = (float)r.world.z; }
+				if (specular) { specular[3 * i] = r.spec[0]; specular[3 * i + 1] = r.spec[1]; specular[3 * i + 2] = r.spec[2]; }
+				if (shadow) { shadow[3 * i] = shadow[3 * i + 1] = shadow[3 * i + 2] = r.shadow; }
+				if (gi) { gi[3 * i] = r.gi[0]; gi[3 * i + 1] = r.gi[1]; gi[3 * i + 2] = r.gi[2]; }
 				n_de += rc.de_evals;
 				n_it += rc.iterations;
 				n_rays += (uint64_t)(p->DOFMonteCarlo ? p->DOFSamples : 1);

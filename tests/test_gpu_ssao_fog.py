@@ -236,3 +236,39 @@ def test_unsupported_fog_kinds_are_refused(rt):
             ctx.upload(p)
     finally:
         ctx.close()
+
+
+def _channel_stats(g, r, name, mask):
+    e = np.abs(g[name].astype(np.float64) - r[name].astype(np.float64)).max(axis=-1)
+    return float((e[mask] <= 1e-2).mean()) if mask.any() else 1.0
+
+
+def test_optional_channels_match_oracle(rt, oracle_mod):
+    """The world position, specular, shadow and GI channels (mbx_outputs.world / .specular /
+    .shadow / .gi) against the oracle's: world within 1e-4 of the point's size on >= 99.9% of the
+    pixels both renders hit, the others within 1e-2 on >= 99.9% of pixels."""
+    p = _material_bulb(200, 150)
+    p.light.castShadows = 1
+    abi.set3(p.light.rotation, (30.0, 20.0, 0.0))
+    g = rt.render_to_host(p, optional=True)
+    r = oracle_mod.render(p, optional=True)
+    both = (g["mask"] > 0) & (r["mask"] > 0)
+    dw = np.abs(g["world"] - r["world"]).max(axis=-1) / np.maximum(np.abs(r["world"]).max(axis=-1), 1e-30)
+    st = {"world_1e-4": float((dw[both] <= 1e-4).mean())}
+    for name in ("specular", "shadow", "gi"):
+        st[name] = _channel_stats(g, r, name, np.ones_like(both))
+    _report("channels", st)
+    assert st["world_1e-4"] >= 0.999 and all(st[k] >= 0.999 for k in ("specular", "shadow", "gi")), st
+    assert np.abs(r["specular"]).max() > 1e-3 and (r["shadow"][both] < 1.0).any()
+    # Monte Carlo GI over 4 samples: the channel is the mean of the samples' GI
+    q = scene.plain_material(scene.mandelbulb8(96, 72))
+    q.DOFMonteCarlo = 1
+    q.DOFSamples = 4
+    q.DOFMinSamples = 1000
+    q.DOFMonteCarloGlobalIllumination = 1
+    q.reflectionsMax = 1
+    g = rt.render_to_host(q, optional=True)
+    r = oracle_mod.render(q, optional=True)
+    gi = _channel_stats(g, r, "gi", np.ones(g["mask"].shape, bool))
+    print("[channels] gi (MC 4 samples) within 1e-2: %.6f" % gi, flush=True)
+    assert gi >= 0.999 and np.abs(r["gi"]).max() > 1e-3
