@@ -615,6 +615,14 @@ int mbx_render_batch_noalloc(mbx_ctx *ctx, const mbx_params *frames, const mbx_o
 #define MBX_TUNE_TIMING 7     /* 1 (default) = bracket each mbx_render / mbx_render_batch with events so
                                  that mbx_get_stats reports its device time; 0 = no events (each event
                                  marker costs the stream ~10 us between launches; kernel_ms reads 0) */
+#define MBX_TUNE_SPLIT 8      /* grid mode, frames with shadows, AO, palette, highlights or glow and no
+                                 Monte Carlo sampling: a march launch that shades the misses and lists
+                                 the hits, then a shading launch over the hit lists (hits shaded in
+                                 full waves; hit records of up to 48 bytes per pixel of the batch).
+                                 1 (default) = for the Kaleidoscopic IFS kind, where it is faster;
+                                 2 = for every kind; 0 = never.  The geometry planes are bit for bit
+                                 those of the one-launch render, the shading planes too on the FP64
+                                 kinds and to the last bit of FP32 rounding on the FP32 kinds */
 
 /* Set a launch or numerics option (see MBX_TUNE_*); scene options apply from the next
  * mbx_scene_upload / mbx_render_batch.  No reference counterpart: the OpenCL engine's

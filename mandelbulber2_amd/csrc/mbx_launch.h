@@ -133,7 +133,14 @@ struct BatchArgs
 	unsigned long long *counters;  // work counter slots (kCounterBytes)
 	int clearFirst;                // launch the clear kernel (queue + counters) before the batch
 	unsigned long long *clearNext; // grid mode: counter slots workgroup 0 zeroes for the next launch
+	unsigned char *hits;           // grid mode, full variant without Monte Carlo: the split render's
+	                               // hit records (kMaxHitRecordBytes per pixel of the batch), or null
+	unsigned int *hitCounts;       // its per-frame hit counts (numFrames)
 };
+
+// Upper bound of the split render's hit record (the FP64 kinds': index, seed, step count,
+// threshold and point)
+constexpr size_t kMaxHitRecordBytes = 48;
 
 // Batch launch modes: one workgroup per tile of every frame (grid), or a fixed grid of
 // `numWaves` waves draining tile queues (persistent).

@@ -107,6 +107,258 @@ __global__ __launch_bounds__(64, occupancy_target<KIND>()) void batch_kernel(con
 	}
 }
 
+// ---- split render for frames with the full shading and no Monte Carlo sampling -------------
+// The full variant's shadow, AO, normal and palette work runs only on hits, but in the one-kernel
+// render a wave carries the tile's misses through it as idle lanes, and its registers hold the
+// march's state besides the shading's.  The split render marches every pixel in
+// split_march_kernel, which shades the misses (background, glow) and appends each hit to its
+// frame's list (the point, threshold, step count and the pixel's random stream after the march);
+// split_shade_kernel then shades the lists, 64 hits per wave.  Each pixel runs the same operations
+// in the same order as in render_pixel, so the planes are bit for bit those of the one-kernel
+// render.
+template <int KIND>
+struct HitRec
+{
+	uint32_t pix;      // pixel index within the frame
+	uint32_t seed;     // the pixel's random stream after the march
+	int32_t stepCount; // the march's step buffer count (glow)
+	typename PrecOf<KIND>::R thr;
+	typename PrecOf<KIND>::P point;
+};
+
+// the primary ray of pixel (xs, ys) without Monte Carlo jitter (render_pixel's image point)
+template <int KIND>
+__device__ __forceinline__ typename PrecOf<KIND>::P primary_dir(const FrameConst &f, int xs, int ys)
+{
+#pragma clang fp contract(off)
+	using Q = PrecOf<KIND>;
+	using R = typename Q::R;
+	const R aspect = Q::aspect(f);
+	const R ox = (Q::invWidth(f) * (R)xs - (R)0.5) * aspect;
+	const R oy = (R)0.5 - Q::invHeight(f) * (R)ys;
+	return Q::view(f, ox, oy);
+}
+
+template <int KIND>
+__device__ bool split_march_pixel(const FrameConst &frame, int xs, int ys, Work &w, const Outputs &out, size_t oi,
+	HitRec<KIND> &rec)
+{
+#pragma clang fp contract(off)
+	const FrameConst &f = uniform_ref(frame);
+	using Q = PrecOf<KIND>;
+	using R = typename Q::R;
+	using P = typename Q::P;
+	const uint32_t pix = (uint32_t)ys * (uint32_t)f.width + (uint32_t)xs;
+	uint32_t seed = pixel_seed(pix, f.randomSeed);
+	const P dir = primary_dir<KIND>(f, xs, ys);
+	const P cam = Q::cam(f);
+	MarchOut<KIND> m;
+	if constexpr (PosPrecision<KIND>::kFP64)
+		ray_marching<KIND, 2>(f, cam, dir, (R)f.viewDistanceMax, seed, m, w);
+	else if (f.advancedQuality)
+		ray_marching<KIND, 1>(f, cam, dir, (R)f.viewDistanceMax, seed, m, w);
+	else
+		ray_marching<KIND, 0>(f, cam, dir, (R)f.viewDistanceMax, seed, m, w);
+	if (out.world)
+	{
+		const V3 pt = Q::tof(m.point);
+		out.world[3 * oi + 0] = pt.x;
+		out.world[3 * oi + 1] = pt.y;
+		out.world[3 * oi + 2] = pt.z;
+	}
+	if (out.gi)
+		for (int k = 0; k < 3; k++) out.gi[3 * oi + k] = 0.0f;
+	if (out.depth) out.depth[oi] = m.found ? (float)m.depth : 1e20f;
+	if (out.iters) out.iters[oi] = m.iters;
+	if (out.mask) out.mask[oi] = (uint8_t)m.found;
+	if (out.steps) out.steps[oi] = m.steps;
+	if (out.opacity) out.opacity[oi] = 0;
+	if (m.found)
+	{
+		rec.pix = pix;
+		rec.seed = seed;
+		rec.stepCount = m.stepCount;
+		rec.thr = m.distThresh;
+		rec.point = m.point;
+		return true;
+	}
+	V3 rgb = v3(0.0f, 0.0f, 0.0f) + background_shader(f, Q::tof(dir));
+	if (f.glow) rgb = glow_shader(f, m.stepCount, rgb);
+	if (out.rgb)
+	{
+		out.rgb[3 * oi + 0] = rgb.x;
+		out.rgb[3 * oi + 1] = rgb.y;
+		out.rgb[3 * oi + 2] = rgb.z;
+	}
+	for (int k = 0; k < 3; k++)
+	{
+		if (out.normal) out.normal[3 * oi + k] = 0.0f;
+		if (out.colour) out.colour[3 * oi + k] = 0;
+		if (out.specular) out.specular[3 * oi + k] = 0.0f;
+		if (out.shadow) out.shadow[3 * oi + k] = 1.0f;
+	}
+	return false;
+}
+
+template <int KIND>
+__device__ void split_shade_hit(const FrameConst &frame, const HitRec<KIND> &rec, Work &w, const Outputs &out)
+{
+#pragma clang fp contract(off)
+	const FrameConst &f = uniform_ref(frame);
+	using Q = PrecOf<KIND>;
+	using P = typename Q::P;
+	using R = typename Q::R;
+	const int xs = (int)(rec.pix % (uint32_t)f.width), ys = (int)(rec.pix / (uint32_t)f.width);
+	const size_t oi = rec.pix;
+	uint32_t seed = rec.seed;
+	const P dir = primary_dir<KIND>(f, xs, ys);
+	const R delta = calc_delta<KIND>(f, rec.point);
+	P nP;
+	if (f.slowShading)
+		nP = slow_normals<KIND>(f, rec.point, (KIND == kGeneric && f.interiorMode) ? (double)rec.thr * 0.2 : (double)delta, w);
+	else
+		nP = calculate_normals<KIND>(f, rec.point, rec.thr, w);
+	const V3 n = Q::tof(nP);
+	const V3 colour = surface_colour<KIND>(f, rec.point, w);
+	V3 spec;
+	float shadowCh = 1.0f;
+	V3 rgb = object_shader<KIND, true>(f, rec.point, nP, rec.thr, delta, Q::tof(dir), colour, spec, seed, w, &shadowCh);
+	rgb = rgb + spec;
+	rgb = v3(fmaxf(rgb.x, 0.0f), fmaxf(rgb.y, 0.0f), fmaxf(rgb.z, 0.0f));
+	if (f.glow) rgb = glow_shader(f, rec.stepCount, rgb);
+	rgb = v3(0.0f, 0.0f, 0.0f) + rgb;
+	if (out.rgb)
+	{
+		out.rgb[3 * oi + 0] = rgb.x;
+		out.rgb[3 * oi + 1] = rgb.y;
+		out.rgb[3 * oi + 2] = rgb.z;
+	}
+	if (out.normal)
+	{
+		out.normal[3 * oi + 0] = n.x;
+		out.normal[3 * oi + 1] = n.y;
+		out.normal[3 * oi + 2] = n.z;
+	}
+	if (out.colour)
+	{
+		out.colour[3 * oi + 0] = (uint8_t)colour_byte(colour.x);
+		out.colour[3 * oi + 1] = (uint8_t)colour_byte(colour.y);
+		out.colour[3 * oi + 2] = (uint8_t)colour_byte(colour.z);
+	}
+	if (out.specular)
+	{
+		out.specular[3 * oi + 0] = spec.x;
+		out.specular[3 * oi + 1] = spec.y;
+		out.specular[3 * oi + 2] = spec.z;
+	}
+	if (out.shadow)
+		for (int k = 0; k < 3; k++) out.shadow[3 * oi + k] = shadowCh;
+}
+
+// The frame of global tile t: a binary search over the frames' tile ranges (wave-uniform)
+__device__ __forceinline__ int frame_of_tile(const FrameJob *__restrict__ jobs, int numFrames, unsigned int t)
+{
+	int lo = 0, hi = numFrames - 1;
+	while (lo < hi)
+	{
+		const int mid = (lo + hi) >> 1;
+		if (t < (unsigned int)jobs[mid].tileEnd)
+			hi = mid;
+		else
+			lo = mid + 1;
+	}
+	return lo;
+}
+
+template <int KIND>
+__global__ __launch_bounds__(64, occupancy_target<KIND>()) void split_march_kernel(
+	const FrameConst *__restrict__ frames, const FrameJob *__restrict__ jobs, int numFrames,
+	unsigned long long *__restrict__ counters, int tileShift, unsigned long long *__restrict__ clearNext,
+	unsigned char *__restrict__ hits, unsigned int *__restrict__ hitCounts)
+{
+	const int lane = threadIdx.x;
+	const unsigned int t = blockIdx.x;
+	if (clearNext && t == 0)
+		for (unsigned int k = 0; k < kCounters; k++) clearNext[lane * kCounterStride + k] = 0ull;
+	const int fr = frame_of_tile(jobs, numFrames, t);
+	const FrameConst &f = frames[fr];
+	const FrameJob &job = jobs[fr];
+	Work w = make_work(f);
+	const unsigned int tileBegin = fr == 0 ? 0u : (unsigned int)jobs[fr - 1].tileEnd;
+	const int lt = (int)(t - tileBegin);
+	const int tilesX = (f.width + (1 << tileShift) - 1) >> tileShift;
+	const int xs = ((lt % tilesX) << tileShift) + (lane & ((1 << tileShift) - 1));
+	const int ys = (lt / tilesX) * (64 >> tileShift) + (lane >> tileShift);
+	static_assert(sizeof(HitRec<KIND>) <= kMaxHitRecordBytes, "hit record larger than its allocation");
+	HitRec<KIND> rec;
+	bool hit = false;
+	if (xs < f.width && ys < f.height)
+	{
+		const Outputs out{job.rgb, job.depth, job.iters, job.mask, job.normal, job.steps, job.colour, job.opacity,
+			job.world, job.specular, job.shadow, job.gi};
+		hit = split_march_pixel<KIND>(f, xs, ys, w, out, (size_t)ys * (size_t)f.width + (size_t)xs, rec);
+	}
+	// one atomic per wave appends the wave's hits to the frame's list, in lane order
+	const unsigned long long ballot = __ballot(hit);
+	if (ballot)
+	{
+		unsigned int base = 0;
+		if (lane == 0) base = atomicAdd(&hitCounts[fr], (unsigned int)__popcll(ballot));
+		base = __shfl(base, 0, 64);
+		if (hit)
+		{
+			const unsigned int slot = base + (unsigned int)__popcll(ballot & ((1ull << lane) - 1ull));
+			reinterpret_cast<HitRec<KIND> *>(hits)[(size_t)job.pixBase + slot] = rec;
+		}
+	}
+	if (counters)
+	{
+		unsigned long long *slot = counters + (blockIdx.x % kCounterSlots) * kCounterStride;
+		wave_add_u64(&slot[0], w.de_evals);
+		wave_add_u64(&slot[1], w.iterations);
+		wave_add_u64(&slot[2], w.far_steps);
+	}
+}
+
+template <int KIND>
+__global__ __launch_bounds__(64, occupancy_target<KIND>()) void split_shade_kernel(
+	const FrameConst *__restrict__ frames, const FrameJob *__restrict__ jobs, int numFrames,
+	unsigned long long *__restrict__ counters, const unsigned char *__restrict__ hits,
+	const unsigned int *__restrict__ hitCounts)
+{
+	const int lane = threadIdx.x;
+	const unsigned int t = blockIdx.x;
+	const int fr = frame_of_tile(jobs, numFrames, t);
+	const unsigned int tileBegin = fr == 0 ? 0u : (unsigned int)jobs[fr - 1].tileEnd;
+	const unsigned int first = (t - tileBegin) * 64u;
+	const unsigned int count = hitCounts[fr];
+	if (first >= count) return; // a whole wave past the frame's hits
+	const FrameConst &f = frames[fr];
+	const FrameJob &job = jobs[fr];
+	Work w = make_work(f);
+	if (first + (unsigned int)lane < count)
+	{
+		const HitRec<KIND> rec = reinterpret_cast<const HitRec<KIND> *>(hits)[(size_t)job.pixBase + first + lane];
+		const Outputs out{job.rgb, job.depth, job.iters, job.mask, job.normal, job.steps, job.colour, job.opacity,
+			job.world, job.specular, job.shadow, job.gi};
+		split_shade_hit<KIND>(f, rec, w, out);
+	}
+	if (counters)
+	{
+		unsigned long long *slot = counters + (blockIdx.x % kCounterSlots) * kCounterStride;
+		wave_add_u64(&slot[0], w.de_evals);
+		wave_add_u64(&slot[1], w.iterations);
+		wave_add_u64(&slot[2], w.far_steps);
+	}
+}
+
+// Zero the split render's per-frame hit counts
+__global__ void hit_clear_kernel(unsigned int *counts, int n)
+{
+	const int i = blockIdx.x * 64 + threadIdx.x;
+	if (i < n) counts[i] = 0u;
+}
+
 // Persistent work-stealing kernel: a fixed grid of waves drains the 64-pixel tiles of every
 // frame of a batch (frames of different resolution, camera and formula parameters).  The
 // wave-uniform tile index keeps each tile's frame constants in scalar registers.  One launch
@@ -288,6 +540,16 @@ static void launch_batch_v(const BatchArgs &a, int numWaves, hipStream_t stream)
 template <int KIND>
 static hipError_t launch_batch_t(const BatchArgs &a, int numWaves, int variant, hipStream_t stream)
 {
+	if (a.hits && a.mode == kBatchGrid && variant == kVariantFull)
+	{
+		hipLaunchKernelGGL(hit_clear_kernel, dim3((a.numFrames + 63) / 64), dim3(64), 0, stream, a.hitCounts,
+			a.numFrames);
+		hipLaunchKernelGGL((split_march_kernel<KIND>), dim3(a.totalTiles), dim3(64), 0, stream, a.frames, a.jobs,
+			a.numFrames, a.counters, a.tileShift, a.clearNext, a.hits, a.hitCounts);
+		hipLaunchKernelGGL((split_shade_kernel<KIND>), dim3(a.totalTiles), dim3(64), 0, stream, a.frames, a.jobs,
+			a.numFrames, a.counters, (const unsigned char *)a.hits, (const unsigned int *)a.hitCounts);
+		return hipGetLastError();
+	}
 	if (variant == kVariantLens)
 		launch_batch_v<KIND, kVariantLens>(a, numWaves, stream);
 	else if (variant == kVariantFull)

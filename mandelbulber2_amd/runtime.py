@@ -202,6 +202,7 @@ MBX_TUNE_TILE_WIDTH = 4
 MBX_TUNE_FAR_FIELD = 5
 MBX_TUNE_PRECISION = 6
 MBX_TUNE_TIMING = 7
+MBX_TUNE_SPLIT = 8
 KERNEL_GRID = 0
 KERNEL_PLAIN = 1
 KERNEL_PERSISTENT = 2

@@ -154,3 +154,61 @@ def test_shading_settings_are_validated(rt):
     with pytest.raises(rt.MbxError, match="specular"):
         rt.render_to_host(bad)
     torch.cuda.synchronize()
+
+
+def _split_scenes():
+    from mandelbulber2_amd import scene as sc
+    bulb = sc.reference_material(sc.mandelbulb8(160, 120))
+    bulb.light.castShadows = 1
+    box = sc.mandelbox_shadow_ao(192, 108, seed=3)
+    ifs = sc.reference_material(sc.menger_ifs(128, 96))
+    glow = sc.reference_material(sc.mandelbulb8(96, 72))
+    glow.glowEnabled = 1
+    fast = sc.mandelbox_shadow_ao(160, 90, seed=5)
+    fast.ambientOcclusionMode = abi.MBX_AO_FAST
+    return {"bulb8-material-shadow": bulb, "mandelbox-shadow-ao": box, "menger-material": ifs,
+            "bulb8-glow": glow, "mandelbox-fast-ao": fast}
+
+
+@pytest.mark.parametrize("name", list(_split_scenes().keys()))
+def test_split_render_matches_the_one_kernel_render(rt, name):
+    """MBX_TUNE_SPLIT = 2: the march launch plus the shading launch over the hit lists give the
+    geometry planes of the one-kernel render bit for bit, and the same work counts; the shading
+    planes too on the FP64 kinds (their shading arithmetic has contraction off), and to the last
+    bit of FP32 rounding on the FP32 kinds (the compiler fuses their multiply-adds per kernel:
+    the shadow factor of 22 of 19,200 pixels differed by 1 ulp when measured)."""
+    p = _split_scenes()[name]
+    a = rt.render_to_host(p, tune={rt.MBX_TUNE_SPLIT: 2}, optional=True)
+    b = rt.render_to_host(p, tune={rt.MBX_TUNE_SPLIT: 0}, optional=True)
+    fp64 = a["de_kind"] in (2, 3, 5, 6)
+    for k in ("depth", "iters", "mask", "normal", "steps", "colour", "opacity", "world", "gi"):
+        assert np.array_equal(a[k], b[k]), (name, k)
+    for k in ("rgb", "specular", "shadow"):
+        if fp64:
+            assert np.array_equal(a[k], b[k]), (name, k)
+        else:
+            np.testing.assert_allclose(a[k], b[k], rtol=1e-6, atol=1e-6, err_msg=name + " " + k)
+    assert (a["de_evals"], a["iterations"]) == (b["de_evals"], b["iterations"])
+
+
+def test_split_render_batch_of_mixed_frames(rt):
+    """A batch of frames of different sizes and kinds of shading in one split render equals the
+    frames rendered one at a time without it."""
+    import torch
+    from mandelbulber2_amd import scene as sc
+    frames = [sc.mandelbox_shadow_ao(96, 54, seed=s) for s in (1, 2)]
+    frames.append(sc.mandelbox_shadow_ao(64, 64, seed=4))
+    frames[1].light.castShadows = 0
+    ctx = rt.Context(0)
+    try:
+        ctx.tune(rt.MBX_TUNE_SPLIT, 2)
+        bufs = [ctx.alloc_outputs(p, optional=True) for p in frames]
+        ctx.render_batch(frames, bufs)
+        torch.cuda.synchronize(0)
+        got = [{k: v.cpu().numpy() for k, v in b.items() if not k.startswith("_")} for b in bufs]
+    finally:
+        ctx.close()
+    for p, g in zip(frames, got):
+        r = rt.render_to_host(p, tune={rt.MBX_TUNE_SPLIT: 0}, optional=True)
+        for k in ("rgb", "depth", "iters", "mask", "normal", "colour", "specular", "shadow"):
+            assert np.array_equal(g[k], r[k]), k
