@@ -103,7 +103,9 @@ WORKLOADS = {
     "menger": {
         "metric": "Mrays/s DE raymarch of the Menger-mode kaleidoscopic IFS with the default material "
                   "and a hard shadow, 1080p, 1 GPU; % FP32 VALU peak",
-        "kernel": "batch_kernel<kIfsAnalytic> (full variant)", "dtype": "f32", "peak": FP32_PEAK_TFLOPS,
+        "kernel": "split_march_kernel<kIfsAnalytic> + split_shade_kernel<kIfsAnalytic> (split render, "
+                  "one timed launch of both)", "plain_kernel": "render_kernel<kIfsAnalytic> (full variant)",
+        "dtype": "f32", "peak": FP32_PEAK_TFLOPS,
         "config": {"formula": "kaleidoscopic IFS, Menger mode, scale 3, offset (1,1,1), abs folds, "
                               "maxiter 30, analytic IFS DE (as IFS_001.fract)",
                    "camera": "(2.2, -2.6, 1.7) to the origin, fov 53 deg", "spp": 1,
@@ -174,7 +176,7 @@ def measured_traffic(workload, w, h):
             "material": "batch_kernel_bulb8_material_1080p.md",
             "mandelbox": "batch_kernel_boxdelta_mandelbox_1080p.md",
             "hybrid": "batch_kernel_generic_hybrid002_1080p.md",
-            "menger": "batch_kernel_ifs_menger_1080p.md"}.get(workload)
+            "menger": "split_kernels_ifs_menger_1080p.md"}.get(workload)
     if not name or (w, h) != (1920, 1080):
         return None, None
     path = os.path.join(ROOT, "profiles", name)
@@ -487,6 +489,8 @@ def bench_orbit(args, torch, runtime, wl):
     out.update(legs)
     if "plain_kernel" in legs:
         out["plain_kernel"]["batch_speedup"] = legs["plain_kernel"]["ms_per_step"] / ms_per_step
+    if one_kernel is not None:
+        out["one_kernel"] = one_kernel
     if image is not None:
         out["image_compile"] = image
     if not args.no_cpu_baseline:
@@ -507,6 +511,15 @@ def bench_batch(args, torch, runtime, wl):
     wall, launch_ms = time_kernel(torch, ctx, None, stream, args.steps, args.warmup,
                                   lambda: ctx.render_batch(frames, bufs))
     st = ctx.stats()
+    one_kernel = None
+    if args.workload == "menger" and not args.no_plain:
+        # the same frame as one launch of the full-variant batch kernel (the split render off)
+        ctx.tune(runtime.MBX_TUNE_SPLIT, 0)
+        owall, oms = time_kernel(torch, ctx, bufs, stream, args.steps, args.warmup)
+        ctx.tune(runtime.MBX_TUNE_SPLIT, 1)
+        one_kernel = {"kernel": "batch_kernel<kIfsAnalytic> (full variant, one launch)",
+                      "ms_per_step": owall * 1e3 / args.steps, "avg_launch_ms": sum(oms) / len(oms),
+                      "split_speedup": owall / wall}
     plain = None
     if not args.no_plain:
         ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_PLAIN)
@@ -650,13 +663,22 @@ def main():
     wall, launch_ms = time_kernel(torch, ctx, bufs, stream, args.steps, args.warmup)
     st = ctx.stats()  # counters of the last launch (identical every frame)
 
+    one_kernel = None
+    if args.workload == "menger" and not args.no_plain:
+        # the same frame as one launch of the full-variant batch kernel (the split render off)
+        ctx.tune(runtime.MBX_TUNE_SPLIT, 0)
+        owall, oms = time_kernel(torch, ctx, bufs, stream, args.steps, args.warmup)
+        ctx.tune(runtime.MBX_TUNE_SPLIT, 1)
+        one_kernel = {"kernel": "batch_kernel<kIfsAnalytic> (full variant, one launch)",
+                      "ms_per_step": owall * 1e3 / args.steps, "avg_launch_ms": sum(oms) / len(oms),
+                      "split_speedup": owall / wall}
     plain = None
     if not args.no_plain:
         ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_PLAIN)
         psteps = max(3, args.steps // 2)
         pwall, pms = time_kernel(torch, ctx, bufs, stream, psteps, 1)
         ctx.tune(runtime.MBX_TUNE_TILE_WIDTH, 8)
-        plain = {"kernel": wl["kernel"].replace("batch_kernel", "render_kernel"),
+        plain = {"kernel": wl.get("plain_kernel", wl["kernel"].replace("batch_kernel", "render_kernel")),
                  "ms_per_step": pwall * 1e3 / psteps, "median_launch_ms": statistics.median(pms)}
         ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_PERSISTENT)
         qwall, qms = time_kernel(torch, ctx, bufs, stream, psteps, 1)
@@ -722,6 +744,8 @@ def main():
     if plain is not None:
         plain["batch_speedup"] = plain["ms_per_step"] / ms_per_step
         out["plain_kernel"] = plain
+    if one_kernel is not None:
+        out["one_kernel"] = one_kernel
     if image is not None:
         out["image_compile"] = image
     if not args.no_cpu_baseline:
