@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define MBX_ABI_VERSION 8
+#define MBX_ABI_VERSION 9
 
 /* array sizes as in mandelbulber2/src/fractal.h:52-54 */
 #define MBX_IFS_VECTOR_COUNT 9
@@ -223,7 +223,8 @@ typedef struct mbx_image_adjustments {
 typedef struct mbx_params {
 	int32_t imageWidth;             /* "image_width" */
 	int32_t imageHeight;            /* "image_height" */
-	int32_t perspectiveType;        /* "perspective_type": only 0 (three-point) */
+	int32_t perspectiveType;        /* "perspective_type": 0 three-point, 1 fish-eye, 2 equirectangular,
+	                                   3 fish-eye cut (projection_3d.cpp:41-89) */
 	int32_t maxRaymarchingSteps;    /* "max_raymarching_steps" */
 	int32_t minN;                   /* "minN" */
 	int32_t iterThreshMode;         /* "iteration_threshold_mode" */
@@ -413,6 +414,11 @@ typedef struct mbx_params {
 	int32_t iterFogEnabled;                    /* "iteration_fog_enable" (not rendered) */
 	int32_t cloudsEnabled;                     /* "clouds_enable" (not rendered) */
 	double volumetricLightDEFactor;            /* "volumetric_light_DE_Factor" (default 1) */
+	/* antialiasing (render_worker.cpp:103-104, 187-202): antialiasingSize^2 samples per pixel on a
+	 * regular sub-pixel grid, averaged; with Monte Carlo samples the grid takes the place of the
+	 * random sub-pixel jitter and the sample count is DOFSamples * antialiasingSize^2 */
+	int32_t antialiasingEnabled;               /* "antialiasing_enabled" */
+	int32_t antialiasingSize;                  /* "antialiasing_size" (1 .. 10, default 2) */
 } mbx_params;
 
 /* Per-pixel output planes (device pointers; any may be NULL except where noted).
@@ -711,14 +717,15 @@ void mbx_params_default(mbx_params *p);
                                           file_lightmap to mbx_prepare_ao_vectors */
 #define MBX_FRACT_POSTPROCESS 0x1000u  /* brightness, contrast, gamma, saturation, HDR, blur, post DOF */
 #define MBX_FRACT_MC_ADAPTIVE 0x2000u  /* (unused: the adaptive Monte Carlo stop is rendered) */
-#define MBX_FRACT_PERSPECTIVE 0x4000u  /* fish-eye or equirectangular projection, stereo */
+#define MBX_FRACT_PERSPECTIVE 0x4000u  /* stereo rendering, or the Monte Carlo lens with a fish-eye or
+                                          equirectangular projection (the projections are rendered) */
 #define MBX_FRACT_FORMULA 0x8000u      /* a formula outside the device's table (the MBX_FORMULA_* ids) */
 #define MBX_FRACT_ANIMATION 0x10000u   /* [frames] / [keyframes] sections (not read) */
 #define MBX_FRACT_MATERIAL 0x20000u    /* other material effects: iridescence, roughness, subsurface,
                                           perlin noise, luminosity gradient */
 #define MBX_FRACT_INTERIOR 0x40000u    /* (unused: interior and iteration-threshold modes are rendered) */
 #define MBX_FRACT_DISABLED 0x80000u    /* the fractal node is disabled or absent: nothing to render */
-#define MBX_FRACT_ANTIALIASING 0x100000u /* antialiasing repeats (antialiasing_size^2 per pixel) */
+#define MBX_FRACT_ANTIALIASING 0x100000u /* (unused: antialiasing is rendered) */
 #define MBX_FRACT_MC_EFFECTS 0x200000u /* (unused: Monte Carlo soft shadows and chromatic aberration are rendered) */
 #define MBX_FRACT_NEBULA 0x400000u     /* nebula mode (the OpenCL nebula engine renders the scene) */
 

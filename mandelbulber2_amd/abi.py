@@ -276,6 +276,8 @@ class MbxParams(C.Structure):
         ("iterFogEnabled", C.c_int32),
         ("cloudsEnabled", C.c_int32),
         ("volumetricLightDEFactor", C.c_double),
+        ("antialiasingEnabled", C.c_int32),
+        ("antialiasingSize", C.c_int32),
     ]
 
 

@@ -706,7 +706,7 @@ struct Prec<true>
 	{
 #pragma clang fp contract(off)
 		const R t = len(cam(f) - p) * f.resolutionD * f.fovD;
-		return f.iterThreshMode ? t : t / f.detailLevelD;
+		return (f.iterThreshMode ? t : t / f.detailLevelD) * f.threshScaleD;
 	}
 	// start + dir * scan without FMA contraction (CVector3 operator* then operator+)
 	static __device__ __forceinline__ P along(P start, P dir, R scan)

@@ -685,6 +685,8 @@ extern "C" void mbx_params_default(mbx_params *p)
 	p->iterFogEnabled = 0;
 	p->cloudsEnabled = 0;
 	p->volumetricLightDEFactor = 1.0;
+	p->antialiasingEnabled = 0;            // :69
+	p->antialiasingSize = 2;               // :70
 }
 
 extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, mbx_fract_info *info)
@@ -996,7 +998,11 @@ extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, 
 		}
 		p.fov = std::min(std::max(deg, 0.0), 720.0);
 	}
-	if (p.perspectiveType != 0 || M.b("stereo_enabled", false)) flags |= MBX_FRACT_PERSPECTIVE;
+	// stereo, and the Monte Carlo lens of the fish-eye and equirectangular projections
+	// (render_worker.cpp:1845-1866)
+	if (M.b("stereo_enabled", false) ||
+		(p.perspectiveType != 0 && M.b("DOF_monte_carlo", false) && M.b("DOF_enabled", false)))
+		flags |= MBX_FRACT_PERSPECTIVE;
 	p.sweetSpotHAngle = M.d("sweet_spot_horizontal_angle", 0.0, -180.0, 180.0);
 	p.sweetSpotVAngle = M.d("sweet_spot_vertical_angle", 0.0, -90.0, 90.0);
 	p.minN = (int)M.i("minN", 1, 0, 99999);
@@ -1209,7 +1215,8 @@ extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, 
 		M.b("post_chromatic_aberration_enabled", false))
 		flags |= MBX_FRACT_POSTPROCESS;
 	if (doc.animation) flags |= MBX_FRACT_ANIMATION;
-	if (M.b("antialiasing_enabled", false) && M.i("antialiasing_size", 2, 1, 10) > 1) flags |= MBX_FRACT_ANTIALIASING;
+	p.antialiasingEnabled = M.b("antialiasing_enabled", false);
+	p.antialiasingSize = (int)M.i("antialiasing_size", 2, 1, 10);
 
 	// ---- the fractal containers (sFractal, fractal.cpp:41-120) ---------------------------------
 	auto read_container = [&](int n, mbx_fractal &fr) {

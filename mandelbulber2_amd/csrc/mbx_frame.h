@@ -197,6 +197,12 @@ struct FrameConst
 	float volFogDensity;
 	double volFogFactorD, volFogFromSurfaceD, volFogColour1DistD, volFogColour2DistD;
 	float volFogColour[3][3];
+	// projection (projection_3d.cpp:41-89): 0 three-point, 1 fish-eye, 2 equirectangular, 3 the
+	// fish-eye cut at the hemisphere (image-plane radius fishCutD); the equirectangular image halves
+	// the distance threshold (threshScaleD = 0.5, render_worker.cpp:669); antialiasing grid size
+	// (0 = off, render_worker.cpp:187-199)
+	int32_t perspective, aaSize;
+	double fishCutD, threshScaleD;
 
 	// ambient occlusion
 	int32_t aoEnabled, aoMode, aoQuality, aoCount;

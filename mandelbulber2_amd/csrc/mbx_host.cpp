@@ -812,8 +812,13 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc, bool forceGe
 	std::memset(&fc, 0, sizeof(fc));
 	if (p->imageWidth <= 0 || p->imageHeight <= 0 || p->imageWidth > 65536 || p->imageHeight > 65536)
 		return fail(ctx, MBX_ERR_INVALID, "image size out of range");
-	if (p->perspectiveType != 0)
-		return fail(ctx, MBX_ERR_UNSUPPORTED, "only the three-point perspective is supported");
+	if (p->perspectiveType < 0 || p->perspectiveType > 3)
+		return fail(ctx, MBX_ERR_INVALID, "perspectiveType must be 0 .. 3");
+	// MonteCarloDOF offsets the lens in another plane for the other projections (render_worker.cpp:1845-1866)
+	if (p->perspectiveType != 0 && p->DOFMonteCarlo && p->DOFEnabled)
+		return fail(ctx, MBX_ERR_UNSUPPORTED, "Monte Carlo depth of field with a fish-eye or equirectangular projection");
+	if (p->antialiasingEnabled && (p->antialiasingSize < 1 || p->antialiasingSize > 10))
+		return fail(ctx, MBX_ERR_INVALID, "antialiasingSize must be 1 .. 10");
 	if (p->maxRaymarchingSteps < 1) return fail(ctx, MBX_ERR_INVALID, "max_raymarching_steps < 1");
 	if (!(p->detailLevel > 0.0)) return fail(ctx, MBX_ERR_INVALID, "detail_level must be > 0");
 	const mbx_fractal &fr = p->fractal;
@@ -839,10 +844,18 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc, bool forceGe
 	m.z(-(p->sweetSpotHAngle / 180.0 * M_PI));
 	m.x(p->sweetSpotVAngle / 180.0 * M_PI);
 	std::memcpy(ctx->mRot, m.m, sizeof(m.m));
-	const double fov = 2.0 * std::tan(p->fov / 360.0 * M_PI);
+	// CalcFOV (projection_3d.cpp:140-159); the equirectangular image spans 2 x 1 of the image plane
+	const double fov = p->perspectiveType == 0 ? 2.0 * std::tan(p->fov / 360.0 * M_PI) : p->fov / 180.0 * M_PI;
 	ctx->fov = fov;
 	const double resolution = 1.0 / p->imageHeight;
-	const double aspect = (double)p->imageWidth / p->imageHeight;
+	const double aspect = p->perspectiveType == 2 ? 2.0 : (double)p->imageWidth / p->imageHeight;
+	// the equirectangular projection halves the threshold and the delta (render_worker.cpp:669, 688):
+	// a factor of a power of two, exact wherever it is folded in
+	const double tscale = p->perspectiveType == 2 ? 0.5 : 1.0;
+	fc.perspective = p->perspectiveType;
+	fc.threshScaleD = tscale;
+	fc.fishCutD = M_PI * 0.5 / fov;
+	fc.aaSize = p->antialiasingEnabled ? p->antialiasingSize : 0;
 
 	fc.width = p->imageWidth;
 	fc.height = p->imageHeight;
@@ -858,19 +871,19 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc, bool forceGe
 	fc.aspectRatioD = aspect;
 	fc.invWidthD = 1.0 / p->imageWidth;
 	fc.invHeightD = 1.0 / p->imageHeight;
-	fc.threshKD = resolution * fov / p->detailLevel;
+	fc.threshKD = resolution * fov / p->detailLevel * tscale;
 	fc.resolutionD = resolution;
 	fc.detailLevelD = p->detailLevel;
 	fc.DEFactorD = p->DEFactor;
-	fc.deltaKD = resolution * fov;
+	fc.deltaKD = resolution * fov * tscale;
 	fc.threshK = (float)fc.threshKD;
 	fc.deltaK = (float)fc.deltaKD;
 	// iteration-threshold mode takes precedence over the constant threshold (render_worker.cpp:656-665)
 	fc.constantDEThreshold = (p->constantDEThreshold && !p->iterThreshMode) ? 1 : 0;
 	fc.interiorMode = p->interiorMode ? 1 : 0;
 	fc.iterThreshMode = p->iterThreshMode ? 1 : 0;
-	fc.DEThresh = (float)p->DEThresh;
-	fc.DEThreshD = p->DEThresh;
+	fc.DEThresh = (float)(p->DEThresh * tscale);
+	fc.DEThreshD = p->DEThresh * tscale;
 	fc.DEFactor = (float)p->DEFactor;
 	fc.searchLimit = (float)(1.0 - 0.001 * p->detailLevel);
 	fc.viewDistanceMax = (float)p->viewDistanceMax;
@@ -1355,7 +1368,7 @@ int frame_variant(const FrameConst &f)
 {
 	if (f.dofEnabled || f.chromaticAberration || f.fog) return mbx::kVariantLens;
 	return (f.monteCarlo || f.aoEnabled || (f.lightEnabled && (f.castShadows || f.specPlastic || f.specMetal)) ||
-			   (f.palette && f.paletteGradient) || f.glow || f.slowShading)
+			   (f.palette && f.paletteGradient) || f.glow || f.slowShading || f.perspective || f.aaSize)
 		? mbx::kVariantFull
 		: mbx::kVariantLite;
 }
@@ -1512,7 +1525,8 @@ int render_frames(mbx_ctx *ctx, const std::vector<FrameConst> &frames, const mbx
 		jobs.push_back(make_job(frames[i], outs[i], (int)tiles, (int)pixels, ctx->tileShift));
 		tiles = jobs.back().tileEnd;
 		pixels += (size_t)frames[i].width * frames[i].height;
-		rays += (uint64_t)frames[i].width * frames[i].height * (uint64_t)frames[i].samples;
+		const uint64_t aaRays = frames[i].aaSize ? (uint64_t)frames[i].aaSize * frames[i].aaSize : 1;
+		rays += (uint64_t)frames[i].width * frames[i].height * (uint64_t)frames[i].samples * aaRays;
 		if (tiles > (int64_t)1 << 30) return fail(ctx, MBX_ERR_INVALID, "batch too large");
 	}
 	int rc = stage_batch(ctx, frames, jobs);
@@ -1522,7 +1536,8 @@ int render_frames(mbx_ctx *ctx, const std::vector<FrameConst> &frames, const mbx
 	for (const auto &f : frames)
 	{
 		ctx->batchFull = std::max(ctx->batchFull, frame_variant(f));
-		monteCarlo = monteCarlo || f.monteCarlo;
+		// several samples per pixel, or rays that are not the three-point camera's
+		monteCarlo = monteCarlo || f.monteCarlo || f.aaSize || f.perspective;
 	}
 	// the split render takes the full variant without Monte Carlo sampling (one sample per pixel);
 	// by default only the IFS kind (measured 1.3-1.4x on the Menger scene with the default
@@ -1608,8 +1623,9 @@ std::string derive_diff(const mbx_params *p, const FrameConst &fc, const double 
 		fc.analyticFunction != MBX_ANALYTIC_LOGARITHMIC)
 		return "the differentiable render covers the Mandelbulb with the analytic logarithmic DE";
 	if (!fc.addC || fc.julia) return "the differentiable render needs c = p (no Julia mode)";
-	if (p->light.castShadows || p->ambientOcclusionEnabled || p->DOFMonteCarlo)
-		return "the differentiable render has no shadows, AO or Monte Carlo sampling";
+	if (p->light.castShadows || p->ambientOcclusionEnabled || p->DOFMonteCarlo || p->antialiasingEnabled)
+		return "the differentiable render has no shadows, AO, Monte Carlo sampling or antialiasing";
+	if (p->perspectiveType != 0) return "the differentiable render has the three-point perspective only";
 	if ((p->useColorsFromPalette && p->surfaceGradientEnable) || p->glowEnabled ||
 		(p->light.enabled && (p->specularPlasticEnable || p->metallic)))
 		return "the differentiable render has no palette colouring, specular highlights or glow";
@@ -1920,7 +1936,8 @@ int mbx_render_batch_noalloc(mbx_ctx *ctx, const mbx_params *params, const mbx_o
 		ctx->naJobs[i] = make_job(frames[i], outs[i], (int)tiles, (int)pixels, ctx->tileShift);
 		tiles = ctx->naJobs[i].tileEnd;
 		pixels += (size_t)frames[i].width * frames[i].height;
-		rays += (uint64_t)frames[i].width * frames[i].height * (uint64_t)frames[i].samples;
+		const uint64_t aaRays = frames[i].aaSize ? (uint64_t)frames[i].aaSize * frames[i].aaSize : 1;
+		rays += (uint64_t)frames[i].width * frames[i].height * (uint64_t)frames[i].samples * aaRays;
 		full = std::max(full, frame_variant(frames[i]));
 		if (tiles > (int64_t)1 << 30) return fail(ctx, MBX_ERR_INVALID, "batch too large");
 	}

@@ -834,6 +834,36 @@ __device__ __forceinline__ V3 hsv2rgb(float hue, float sat, float val)
 	}
 }
 
+// CalculateViewVector (projection_3d.cpp:41-89) of the fish-eye (1, 3) and equirectangular (2)
+// projections in FP64, normalised, then turned by the camera rotation in the kind's precision; the
+// march normalises the result again as doWork does
+template <int KIND>
+__device__ __forceinline__ typename PrecOf<KIND>::P persp_view(const FrameConst &f, double ix, double iy)
+{
+#pragma clang fp contract(off)
+	using Q = PrecOf<KIND>;
+	DV3 v;
+	if (f.perspective == 2)
+	{
+		const double vx = ix * 0.5, vy = iy * 0.5;
+		const double cy = cos(f.fovD * vy);
+		v = dv3(sin(f.fovD * vx) * cy, cos(f.fovD * vx) * cy, sin(f.fovD * vy));
+	}
+	else
+	{
+		const double r = sqrt(ix * ix + iy * iy);
+		if (r == 0.0)
+			v = dv3(0.0, 1.0, 0.0);
+		else
+		{
+			const double s = sin(r * f.fovD);
+			v = dv3(ix / r * s, cos(r * f.fovD), iy / r * s);
+		}
+	}
+	v = v * (1.0 / sqrt(ddot(v, v)));
+	return Q::cam_rot(f, Q::fromD(v.x, v.y, v.z));
+}
+
 // One pixel: the sample loop of doWork and RayRecursion at depth 0.  VAR = 0 is the variant for
 // frames without Monte Carlo sampling, GI, shadows and AO; 1 carries them; 2 adds the Monte Carlo
 // lens, chromatic aberration and fog.
@@ -858,6 +888,18 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 	const R oy = (R)0.5 - Q::invHeight(f) * (R)ys;
 	const P cam = Q::cam(f);
 	int repeats = (FULL && f.monteCarlo) ? f.samples : 1;
+	// antialiasing multiplies the samples by the grid (render_worker.cpp:185-186); the projections
+	// and the grid come with the full variants only (the host picks one for such frames)
+	const bool persp = FULL && f.perspective != 0;
+	const int aa = FULL ? f.aaSize : 0;
+	if (aa) repeats *= aa * aa;
+	// the fish-eye cut marches nothing outside the hemisphere (render_worker.cpp:165-170)
+	bool cut = false;
+	if (FULL && f.perspective == 3)
+	{
+		const double oxd = (f.invWidthD * (double)xs - 0.5) * f.aspectRatioD, oyd = 0.5 - f.invHeightD * (double)ys;
+		cut = sqrt(oxd * oxd + oyd * oyd) > f.fishCutD;
+	}
 	V3 acc = v3(0.0f, 0.0f, 0.0f);
 	V3 stdDevSum = v3(0.0f, 0.0f, 0.0f);
 	po.steps = 0;
@@ -865,7 +907,15 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 	for (int rep = 0; rep < repeats; rep++)
 	{
 		R ix = ox, iy = oy;
-		if (FULL && f.monteCarlo)
+		if (aa)
+		{
+			// the sample's place on the sub-pixel grid (render_worker.cpp:187-199); with Monte Carlo
+			// samples the x step runs on past the pixel as the reference's does
+			const int xStep = rep / aa, yStep = rep - xStep * aa;
+			ix = (R)((double)ox + (double)xStep / aa / (double)f.width * f.aspectRatioD);
+			iy = (R)((double)oy + (double)yStep / aa / (double)f.height);
+		}
+		else if (FULL && f.monteCarlo)
 		{
 			ix = ox + ((R)random_mod(seed, 1000) * (R)1e-3 - (R)0.5) * Q::invWidth(f) * aspect;
 			iy = oy + ((R)random_mod(seed, 1000) * (R)1e-3 - (R)0.5) * Q::invHeight(f);
@@ -875,7 +925,7 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 		const bool lens = LENS && (f.dofEnabled || f.chromaticAberration);
 		if (lens)
 		{
-			P v = Q::view_raw(f, ix, iy);
+			P v = persp ? persp_view<KIND>(f, (double)ix, (double)iy) : Q::view_raw(f, ix, iy);
 			if (f.dofEnabled)
 			{
 				// MonteCarloDOF (render_worker.cpp:1832-1844): a random point of the lens, turned by
@@ -901,13 +951,27 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 			}
 			dir = Q::normalize_p(v);
 		}
+		else if (persp)
+			dir = Q::normalize_p(persp_view<KIND>(f, (double)ix, (double)iy));
 		else
 			dir = Q::view(f, ix, iy);
 		MarchOut<KIND> m;
 		FogAcc fogAcc;
 		fog_init(fogAcc);
-		const bool fogOn = LENS && f.fog;
-		if constexpr (PosPrecision<KIND>::kFP64)
+		const bool fogOn = LENS && f.fog && !cut;
+		if (cut)
+		{
+			// a black sample of depth 1e20, fog opacity 1, no channels (render_worker.cpp:172-185)
+			m.point = Q::fromD(0.0, 0.0, 0.0);
+			m.depth = (R)1e20;
+			m.distThresh = (R)0;
+			m.lastDist = (R)0;
+			m.found = 0;
+			m.iters = 0;
+			m.steps = 0;
+			m.stepCount = 0;
+		}
+		else if constexpr (PosPrecision<KIND>::kFP64)
 			ray_marching<KIND, 2, LENS>(f, start, dir, (R)f.viewDistanceMax, seed, m, w, !lens, &fogAcc, fogOn);
 		else if (f.advancedQuality)
 			ray_marching<KIND, 1, LENS>(f, start, dir, (R)f.viewDistanceMax, seed, m, w, !lens, &fogAcc, fogOn);
@@ -957,9 +1021,15 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 			colG += colour_byte(colour.y);
 			colB += colour_byte(colour.z);
 		}
+		else if (cut)
+		{
+			rgb = v3(0.0f, 0.0f, 0.0f);
+			shadowCh = 0.0f;
+			opSum += 65535u;
+		}
 		else
 			rgb = background_shader(f, Q::tof(dir));
-		if (LENS && f.fog)
+		if (LENS && f.fog && !cut)
 		{
 			// the volumetric walk as accumulated by the march; the glow colour needs the step count
 			const V3 g = glow_colour(f, m.stepCount);
@@ -969,7 +1039,7 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 			op = op > 1.0f ? 1.0f : op;
 			opSum += (uint32_t)(uint16_t)(int)((double)op * 65535.0);
 		}
-		else if (FULL && f.glow)
+		else if (FULL && f.glow && !cut)
 			rgb = glow_shader(f, m.stepCount, rgb);
 		if (LENS && f.chromaticAberration)
 		{

@@ -214,6 +214,8 @@ def default_params(width=640, height=480):
     p.iterFogEnabled = 0
     p.cloudsEnabled = 0
     p.volumetricLightDEFactor = 1.0
+    p.antialiasingEnabled = 0
+    p.antialiasingSize = 2
     return p
 
 

@@ -843,7 +843,7 @@ def load(text, trace=None):
     out["imageWidth"], out["imageHeight"] = g("image_width"), g("image_height")
     out["camera"], out["target"], out["topVector"] = g("camera"), g("target"), g("camera_top")
     out["perspectiveType"], out["fov"] = g("perspective_type"), g("fov")
-    if g("perspective_type") != 0 or g("stereo_enabled"):
+    if g("stereo_enabled") or (g("perspective_type") != 0 and g("DOF_monte_carlo") and g("DOF_enabled")):
         flags |= FLAGS["perspective"]
     out["sweetSpotHAngle"] = g("sweet_spot_horizontal_angle")
     out["sweetSpotVAngle"] = g("sweet_spot_vertical_angle")
@@ -1023,8 +1023,8 @@ def load(text, trace=None):
         flags |= FLAGS["postprocess"]
     if animation:
         flags |= FLAGS["animation"]
-    if g("antialiasing_enabled") and g("antialiasing_size") > 1:
-        flags |= FLAGS["antialiasing"]
+    out["antialiasingEnabled"] = int(g("antialiasing_enabled"))
+    out["antialiasingSize"] = g("antialiasing_size")
 
     TABLE = (2, 3, 4, 5, 7, 8, 10, 11, 16, 19)  # the formulas the device path renders
 

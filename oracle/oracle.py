@@ -64,6 +64,8 @@ def lib(fast=False):
     L.mbxo_mc_noise.restype = None
     L.mbxo_sample_ray.argtypes = [P(abi.MbxParams), C.c_int, C.c_int, P(C.c_uint32), C.c_void_p, C.c_void_p,
                                   C.c_void_p]
+    L.mbxo_sample_ray_rep.argtypes = [P(abi.MbxParams), C.c_int, C.c_int, C.c_int, P(C.c_uint32), C.c_void_p,
+                                      C.c_void_p, C.c_void_p]
     L.mbxo_soft_light.argtypes = [C.c_void_p, C.c_double, P(C.c_uint32), C.c_int, C.c_void_p]
     L.mbxo_soft_light.restype = None
     L.mbxo_hsv2rgb.argtypes = [C.c_float, C.c_float, C.c_float, C.c_void_p]
@@ -185,14 +187,14 @@ def mc_noise(samples):
     return out
 
 
-def sample_ray(p, xs, ys, seed):
-    """The first sample's ray of pixel (xs, ys) from random state `seed`: (start, view vector
+def sample_ray(p, xs, ys, seed, repeat=0):
+    """The ray of sample `repeat` of pixel (xs, ys) from random state `seed`: (start, view vector
     before normalisation, tint, seed after the sample's draws)."""
     s = C.c_uint32(seed)
     st = np.zeros(3, np.float64)
     vv = np.zeros(3, np.float64)
     tint = np.zeros(3, np.float32)
-    rc = lib().mbxo_sample_ray(C.byref(p), xs, ys, C.byref(s), _ptr(st), _ptr(vv), _ptr(tint))
+    rc = lib().mbxo_sample_ray_rep(C.byref(p), xs, ys, repeat, C.byref(s), _ptr(st), _ptr(vv), _ptr(tint))
     if rc != 0:
         raise RuntimeError("oracle sample_ray failed: %d" % rc)
     return st, vv, tint, s.value

@@ -251,6 +251,7 @@ typedef struct
 	double fov;        /* CalcFOV: 2 tan(deg/360 pi) */
 	double resolution; /* 1 / imageHeight */
 	double aspectRatio;
+	int persp; /* enumPerspectiveType: 0 three-point, 1 fish-eye, 2 equirectangular, 3 fish-eye cut */
 	/* sequence (cHybridFractalSequences::sSequence, hybrid_fractal_sequences.h:58-103) */
 	int formula; /* the first container's formula */
 	int formulaMaxiter;
@@ -576,7 +577,10 @@ static int scene_init(scene_t *s, const mbx_params *p)
 	memset(s, 0, sizeof(*s));
 	s->p = p;
 	if (p->imageWidth <= 0 || p->imageHeight <= 0) return MBX_ERR_INVALID;
-	if (p->perspectiveType != 0) return MBX_ERR_UNSUPPORTED;
+	if (p->perspectiveType < 0 || p->perspectiveType > 3) return MBX_ERR_INVALID;
+	if (p->antialiasingEnabled && (p->antialiasingSize < 1 || p->antialiasingSize > 10)) return MBX_ERR_INVALID;
+	/* the lens of the other projections (render_worker.cpp:1845-1866) is not restated */
+	if (p->perspectiveType != 0 && p->DOFMonteCarlo && p->DOFEnabled) return MBX_ERR_UNSUPPORTED;
 
 	/* PrepareMainVectors (render_worker.cpp:539-564) */
 	camtarget_t ct = camera_target_top(v3_from(p->camera), v3_from(p->target), v3_from(p->topVector));
@@ -587,9 +591,12 @@ static int scene_init(scene_t *s, const mbx_params *p)
 	rot_z(&s->mRot, -(p->sweetSpotHAngle / 180.0 * M_PI));
 	rot_x(&s->mRot, p->sweetSpotVAngle / 180.0 * M_PI);
 	s->camera = v3_from(p->camera);
-	s->fov = 2.0 * tan(p->fov / 360.0 * M_PI); /* CalcFOV (projection_3d.cpp:140-159) */
-	s->resolution = 1.0 / p->imageHeight;      /* render_job.cpp:477 */
-	s->aspectRatio = (double)p->imageWidth / p->imageHeight;
+	s->persp = p->perspectiveType;
+	/* CalcFOV (projection_3d.cpp:140-159) */
+	s->fov = s->persp == 0 ? 2.0 * tan(p->fov / 360.0 * M_PI) : p->fov / 180.0 * M_PI;
+	s->resolution = 1.0 / p->imageHeight; /* render_job.cpp:477 */
+	/* the equirectangular image spans 2 x 1 of the image plane (render_worker.cpp:98-100) */
+	s->aspectRatio = s->persp == 2 ? 2.0 : (double)p->imageWidth / p->imageHeight;
 
 	rc = sequence_init(s, p);
 	if (rc != MBX_OK) return rc;
@@ -1384,6 +1391,7 @@ static double calc_dist_thresh(const scene_t *s, v3 point)
 		distThresh = p->DEThresh;
 	else
 		distThresh = v3_len(v3_sub(s->camera, point)) * s->resolution * s->fov / p->detailLevel;
+	if (s->persp == 2) distThresh *= 0.5; /* render_worker.cpp:669 */
 	if (p->advancedQuality)
 	{
 		if (distThresh > p->detailSizeMax) distThresh = p->detailSizeMax;
@@ -1396,7 +1404,9 @@ static double calc_dist_thresh(const scene_t *s, v3 point)
 /* CalcDelta (render_worker.cpp:684-690) */
 static double calc_delta(const scene_t *s, v3 point)
 {
-	return v3_len(v3_sub(s->camera, point)) * s->resolution * s->fov;
+	double delta = v3_len(v3_sub(s->camera, point)) * s->resolution * s->fov;
+	if (s->persp == 2) delta *= 0.5f; /* render_worker.cpp:688 */
+	return delta;
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -2278,10 +2288,28 @@ static void trace_primary(ray_ctx *rc, v3 start, v3 viewVector, ray_result *res)
 	free(m.stepThresh);
 }
 
-/* CalculateViewVector, three-point perspective (projection_3d.cpp:41-89) */
+/* CalculateViewVector (projection_3d.cpp:41-89): three-point, fish-eye (and its cut form) and
+ * equirectangular projections of the image point, then the camera rotation */
 static v3 view_vector(const scene_t *s, double ix, double iy)
 {
-	v3 v = V3(ix * s->fov, 1.0, iy * s->fov);
+	v3 v;
+	if (s->persp == 1 || s->persp == 3)
+	{
+		/* fish-eye: the angle from the axis grows with the distance from the image centre */
+		const double r = sqrt(ix * ix + iy * iy);
+		if (r == 0.0)
+			v = V3(0.0, 1.0, 0.0);
+		else
+			v = V3(ix / r * sin(r * s->fov), cos(r * s->fov), iy / r * sin(r * s->fov));
+	}
+	else if (s->persp == 2)
+	{
+		/* equirectangular: longitude and latitude linear in the image point */
+		const double vx = ix * 0.5, vy = iy * 0.5;
+		v = V3(sin(s->fov * vx) * cos(s->fov * vy), cos(s->fov * vx) * cos(s->fov * vy), sin(s->fov * vy));
+	}
+	else
+		v = V3(ix * s->fov, 1.0, iy * s->fov);
 	v = v3_normalize(v);
 	return rot_apply3(&s->mRot, v);
 }
@@ -2417,13 +2445,23 @@ void mbxo_hsv2rgb(float hue, float sat, float val, float *rgb) { hsv2rgb(hue, sa
 /* the ray of one sample of a pixel at image-plane point (ox, oy) (render_worker.cpp:226-264):
  * Monte Carlo anti-aliasing jitter, the lens, chromatic aberration; tint is the sample's colour
  * factor (1 without chromatic aberration) */
-static void sample_ray(ray_ctx *rc, double ox, double oy, v3 *startRay, v3 *viewVector, float tint[3])
+static void sample_ray(ray_ctx *rc, double ox, double oy, int repeat, v3 *startRay, v3 *viewVector, float tint[3])
 {
 	const scene_t *s = rc->s;
 	const mbx_params *p = s->p;
 	const int W = p->imageWidth, H = p->imageHeight;
 	double ix = ox, iy = oy;
-	if (p->DOFMonteCarlo)
+	if (p->antialiasingEnabled)
+	{
+		/* the sample's place on the sub-pixel grid (render_worker.cpp:187-199): repeat / size
+		 * steps x, repeat % size steps y, and with Monte Carlo samples the x step runs on past the
+		 * pixel as the reference's does */
+		const int aa = p->antialiasingSize;
+		const int xStep = repeat / aa, yStep = repeat % aa;
+		ix = ox + (double)xStep / aa / W * s->aspectRatio;
+		iy = oy + (double)yStep / aa / H;
+	}
+	else if (p->DOFMonteCarlo)
 	{
 		ix = ox + ((double)mbxo_random(rc->seed, 1000) / 1000.0 - 0.5) / W * s->aspectRatio;
 		iy = oy + ((double)mbxo_random(rc->seed, 1000) / 1000.0 - 0.5) / H;
@@ -2443,9 +2481,10 @@ static void sample_ray(ray_ctx *rc, double ox, double oy, v3 *startRay, v3 *view
 	}
 }
 
-/* the first sample's ray of pixel (xs, ys) from the random state *seed: origin, view vector
+/* the ray of sample `repeat` of pixel (xs, ys) from the random state *seed: origin, view vector
  * (before the march normalises it) and tint; *seed advances past the sample's draws */
-int mbxo_sample_ray(const mbx_params *p, int xs, int ys, uint32_t *seed, double *start3, double *view3, float *tint3)
+int mbxo_sample_ray_rep(const mbx_params *p, int xs, int ys, int repeat, uint32_t *seed, double *start3, double *view3,
+	float *tint3)
 {
 	scene_t s;
 	int rc0 = scene_init(&s, p);
@@ -2454,10 +2493,15 @@ int mbxo_sample_ray(const mbx_params *p, int xs, int ys, uint32_t *seed, double 
 	const double ox = ((1.0 / p->imageWidth) * xs + (-0.5)) * s.aspectRatio;
 	const double oy = (-1.0 / p->imageHeight) * ys + 0.5;
 	v3 st, vv;
-	sample_ray(&rc, ox, oy, &st, &vv, tint3);
+	sample_ray(&rc, ox, oy, repeat, &st, &vv, tint3);
 	start3[0] = st.x, start3[1] = st.y, start3[2] = st.z;
 	view3[0] = vv.x, view3[1] = vv.y, view3[2] = vv.z;
 	return 0;
+}
+
+int mbxo_sample_ray(const mbx_params *p, int xs, int ys, uint32_t *seed, double *start3, double *view3, float *tint3)
+{
+	return mbxo_sample_ray_rep(p, xs, ys, 0, seed, start3, view3, tint3);
 }
 
 /* n successive Monte Carlo soft-shadow light vectors around the unit vector light3 for a cone of
@@ -2485,6 +2529,10 @@ static void render_pixel(ray_ctx *rc, int xs, int ys, ray_result *out)
 	const double ox = ix, oy = iy;
 	int repeats = p->DOFMonteCarlo ? p->DOFSamples : 1;
 	if (repeats < 1) repeats = 1;
+	if (p->antialiasingEnabled) repeats *= p->antialiasingSize * p->antialiasingSize;
+	/* the fish-eye cut renders nothing outside the hemisphere (render_worker.cpp:165-170): no
+	 * march, a black sample of depth 1e20 and fog opacity 1 */
+	const int hemisphereCut = s->persp == 3 && sqrt(ox * ox + oy * oy) > M_PI * 0.5f / s->fov;
 	float acc[3] = {0, 0, 0};
 	float stdDevSum[3] = {0, 0, 0};
 	unsigned colourSum[3] = {0, 0, 0}, opacitySum = 0;
@@ -2495,8 +2543,15 @@ static void render_pixel(ray_ctx *rc, int xs, int ys, ray_result *out)
 	{
 		v3 viewVector, startRay;
 		float tint[3];
-		sample_ray(rc, ox, oy, &startRay, &viewVector, tint);
-		trace_primary(rc, startRay, viewVector, &r);
+		sample_ray(rc, ox, oy, repeat, &startRay, &viewVector, tint);
+		if (hemisphereCut)
+		{
+			memset(&r, 0, sizeof(r));
+			r.depth = 1e20;
+			r.opacity = 1.0f;
+		}
+		else
+			trace_primary(rc, startRay, viewVector, &r);
 		r.rgb[0] *= tint[0];
 		r.rgb[1] *= tint[1];
 		r.rgb[2] *= tint[2];
@@ -2536,7 +2591,7 @@ static void render_pixel(ray_ctx *rc, int xs, int ys, ray_result *out)
 			}
 		}
 	}
-	if (p->DOFMonteCarlo)
+	if (p->DOFMonteCarlo || p->antialiasingEnabled)
 	{
 		out->rgb[0] = acc[0] / repeats;
 		out->rgb[1] = acc[1] / repeats;
@@ -2550,7 +2605,8 @@ static void render_pixel(ray_ctx *rc, int xs, int ys, ray_result *out)
 	}
 	/* integer means over the samples (render_worker.cpp:469-472), the GI mean (:474-476) */
 	for (int k = 0; k < 3; k++) out->colour[k] = colourSum[k] / (unsigned)repeats;
-	for (int k = 0; k < 3; k++) out->gi[k] = p->DOFMonteCarlo ? giSum[k] / repeats : giSum[k];
+	for (int k = 0; k < 3; k++)
+		out->gi[k] = (p->DOFMonteCarlo || p->antialiasingEnabled) ? giSum[k] / repeats : giSum[k];
 	out->opacity = (float)(opacitySum / (unsigned)repeats);
 }
 

@@ -33,7 +33,7 @@ def test_library_exports_declared_symbols():
 
 def test_params_layout_matches_library(oracle_mod):
     lib = runtime.load()
-    assert lib.mbx_version() == 8
+    assert lib.mbx_version() == 9
     assert lib.mbx_sizeof_params() == ctypes.sizeof(abi.MbxParams)
     assert oracle_mod.lib().mbxo_sizeof_params() == ctypes.sizeof(abi.MbxParams)
 

@@ -235,7 +235,8 @@ def test_mandelbulb_power_8_global_constant_factor(load):
     assert r["DEFactor"] == 0.3 and r["fov"] == 28.07 and r["detailLevel"] == 4.0
     assert r["light.rotation"] == [-45.0, -45.0, 0.0] and r["light.intensity"] == 0.3
     assert r["background3ColorsEnable"] == 0 and r["backgroundColor1"] == [0.0, 0.0, 0.0]
-    assert {"ao_lightmap", "antialiasing"} <= r.flags() and "postprocess" not in r.flags()
+    assert "ao_lightmap" in r.flags() and "postprocess" not in r.flags() and "antialiasing" not in r.flags()
+    assert r["antialiasingEnabled"] == 1 and r["antialiasingSize"] == 2
     assert r["image.gamma"] == float(np.float32(0.7))
     assert "palette" not in r.flags()
 
@@ -351,7 +352,7 @@ def test_value_rules(load):
     assert r["backgroundColor2"] == _rgb(65535, 0x10, 0)
     assert r["camera"] == [1.0, 2.0, 2.0]
     assert r["perspectiveType"] == 1 and r.meta("errors") == 0
-    assert "perspective" in r.flags() and "interior" not in r.flags()
+    assert "perspective" not in r.flags() and "interior" not in r.flags()
 
 
 @pytest.mark.parametrize("version,fov,ptype,want", [

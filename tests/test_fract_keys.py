@@ -139,7 +139,6 @@ MAIN_RULES = [
                                           _v(p, "mat%d_transparency_of_surface" % node["material"], 0.0) > 0.0))),
     (r"MC_global_illumination_volumetric|volumetric_light_DE_Factor",
      _gate("glow", lambda p, m: _v(p, "glow_enabled") and _v(p, "glow_intensity") > 0.0)),
-    (r"antialiasing_size", _gate("antialiasing", _on("antialiasing_enabled"))),
     (r"nebula_mode", _gate("nebula", _on("nebula_mode"))),
     (r"nebula_.*", _gate("nebula", _on("nebula_mode"))),
     (r"all_lights_.*", INERT),           # consumed with light 1; without it, other lights are flagged

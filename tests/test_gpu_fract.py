@@ -64,6 +64,9 @@ SCENES = [
     # basic and volumetric distance fog
     "IFS_23.fract",
     "mandelbox55.fract",
+    # a fish-eye projection and an antialiased IFS
+    "fish_eye.fract",
+    "IFS_19_maxiter.fract",
 ]
 
 
