@@ -717,15 +717,16 @@ void mbx_params_default(mbx_params *p);
                                           file_lightmap to mbx_prepare_ao_vectors */
 #define MBX_FRACT_POSTPROCESS 0x1000u  /* brightness, contrast, gamma, saturation, HDR, blur, post DOF */
 #define MBX_FRACT_MC_ADAPTIVE 0x2000u  /* (unused: the adaptive Monte Carlo stop is rendered) */
-#define MBX_FRACT_PERSPECTIVE 0x4000u  /* stereo rendering, or the Monte Carlo lens with a fish-eye or
-                                          equirectangular projection (the projections are rendered) */
+#define MBX_FRACT_PERSPECTIVE 0x4000u  /* stereo rendering, or a fish-eye or equirectangular projection
+                                          with the Monte Carlo lens, chromatic aberration or fog */
 #define MBX_FRACT_FORMULA 0x8000u      /* a formula outside the device's table (the MBX_FORMULA_* ids) */
 #define MBX_FRACT_ANIMATION 0x10000u   /* [frames] / [keyframes] sections (not read) */
 #define MBX_FRACT_MATERIAL 0x20000u    /* other material effects: iridescence, roughness, subsurface,
                                           perlin noise, luminosity gradient */
 #define MBX_FRACT_INTERIOR 0x40000u    /* (unused: interior and iteration-threshold modes are rendered) */
 #define MBX_FRACT_DISABLED 0x80000u    /* the fractal node is disabled or absent: nothing to render */
-#define MBX_FRACT_ANTIALIASING 0x100000u /* (unused: antialiasing is rendered) */
+#define MBX_FRACT_ANTIALIASING 0x100000u /* antialiasing with the Monte Carlo lens, chromatic aberration
+                                            or fog (antialiasing alone is rendered) */
 #define MBX_FRACT_MC_EFFECTS 0x200000u /* (unused: Monte Carlo soft shadows and chromatic aberration are rendered) */
 #define MBX_FRACT_NEBULA 0x400000u     /* nebula mode (the OpenCL nebula engine renders the scene) */
 

@@ -1217,6 +1217,13 @@ extern "C" int mbx_load_fract(const char *text, size_t len, mbx_params *params, 
 	if (doc.animation) flags |= MBX_FRACT_ANIMATION;
 	p.antialiasingEnabled = M.b("antialiasing_enabled", false);
 	p.antialiasingSize = (int)M.i("antialiasing_size", 2, 1, 10);
+	{
+		// the projections and antialiasing render without the lens variant's effects
+		const bool lensFx = (p.DOFMonteCarlo && (p.DOFEnabled || p.DOFMonteCarloChromaticAberration)) ||
+			p.basicFogEnabled || p.volFogEnabled;
+		if (lensFx && p.perspectiveType != 0) flags |= MBX_FRACT_PERSPECTIVE;
+		if (lensFx && p.antialiasingEnabled) flags |= MBX_FRACT_ANTIALIASING;
+	}
 
 	// ---- the fractal containers (sFractal, fractal.cpp:41-120) ---------------------------------
 	auto read_container = [&](int n, mbx_fractal &fr) {

@@ -819,6 +819,14 @@ int derive_frame(mbx_ctx *ctx, const mbx_params *p, FrameConst &fc, bool forceGe
 		return fail(ctx, MBX_ERR_UNSUPPORTED, "Monte Carlo depth of field with a fish-eye or equirectangular projection");
 	if (p->antialiasingEnabled && (p->antialiasingSize < 1 || p->antialiasingSize > 10))
 		return fail(ctx, MBX_ERR_INVALID, "antialiasingSize must be 1 .. 10");
+	// the projections and antialiasing are built into the full kernel variant, not the lens
+	// variant that carries the Monte Carlo lens, chromatic aberration and fog
+	if ((p->perspectiveType != 0 || p->antialiasingEnabled) &&
+		((p->DOFMonteCarlo && (p->DOFEnabled || p->DOFMonteCarloChromaticAberration)) || p->basicFogEnabled ||
+			p->volFogEnabled))
+		return fail(ctx, MBX_ERR_UNSUPPORTED,
+			"a fish-eye or equirectangular projection or antialiasing with the Monte Carlo lens, chromatic "
+			"aberration or fog");
 	if (p->maxRaymarchingSteps < 1) return fail(ctx, MBX_ERR_INVALID, "max_raymarching_steps < 1");
 	if (!(p->detailLevel > 0.0)) return fail(ctx, MBX_ERR_INVALID, "detail_level must be > 0");
 	const mbx_fractal &fr = p->fractal;

@@ -889,16 +889,37 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 	const P cam = Q::cam(f);
 	int repeats = (FULL && f.monteCarlo) ? f.samples : 1;
 	// antialiasing multiplies the samples by the grid (render_worker.cpp:185-186); the projections
-	// and the grid come with the full variants only (the host picks one for such frames)
-	const bool persp = FULL && f.perspective != 0;
-	const int aa = FULL ? f.aaSize : 0;
+	// and the grid come with the full variant only (the host picks it for such frames and refuses
+	// them with the lens variant's effects)
+	constexpr bool EXTRA = VAR == 1;
+	const bool persp = EXTRA && f.perspective != 0;
+	const int aa = EXTRA ? f.aaSize : 0;
 	if (aa) repeats *= aa * aa;
-	// the fish-eye cut marches nothing outside the hemisphere (render_worker.cpp:165-170)
-	bool cut = false;
-	if (FULL && f.perspective == 3)
+	// the fish-eye cut marches nothing outside the hemisphere (render_worker.cpp:165-170): every
+	// sample is black at depth 1e20 with fog opacity 1 and no channels, whatever its random draws
+	// (the pixel's own stream), so the pixel's outputs are known without its sample loop
+	if (EXTRA && f.perspective == 3)
 	{
 		const double oxd = (f.invWidthD * (double)xs - 0.5) * f.aspectRatioD, oyd = 0.5 - f.invHeightD * (double)ys;
-		cut = sqrt(oxd * oxd + oyd * oyd) > f.fishCutD;
+		if (sqrt(oxd * oxd + oyd * oyd) > f.fishCutD)
+		{
+			po.rgb = v3(0.0f, 0.0f, 0.0f);
+			po.depth = 1e20f;
+			po.found = 0;
+			po.iters = 0;
+			po.steps = 0;
+			po.normal = v3(0.0f, 0.0f, 0.0f);
+			po.colour[0] = po.colour[1] = po.colour[2] = 0;
+			po.opacity = 65535u;
+			for (int k = 0; k < 3; k++)
+			{
+				if (out.world) out.world[3 * oi + k] = 0.0f;
+				if (out.specular) out.specular[3 * oi + k] = 0.0f;
+				if (out.shadow) out.shadow[3 * oi + k] = 0.0f;
+				if (out.gi) out.gi[3 * oi + k] = 0.0f;
+			}
+			return;
+		}
 	}
 	V3 acc = v3(0.0f, 0.0f, 0.0f);
 	V3 stdDevSum = v3(0.0f, 0.0f, 0.0f);
@@ -925,7 +946,7 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 		const bool lens = LENS && (f.dofEnabled || f.chromaticAberration);
 		if (lens)
 		{
-			P v = persp ? persp_view<KIND>(f, (double)ix, (double)iy) : Q::view_raw(f, ix, iy);
+			P v = Q::view_raw(f, ix, iy);
 			if (f.dofEnabled)
 			{
 				// MonteCarloDOF (render_worker.cpp:1832-1844): a random point of the lens, turned by
@@ -958,20 +979,8 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 		MarchOut<KIND> m;
 		FogAcc fogAcc;
 		fog_init(fogAcc);
-		const bool fogOn = LENS && f.fog && !cut;
-		if (cut)
-		{
-			// a black sample of depth 1e20, fog opacity 1, no channels (render_worker.cpp:172-185)
-			m.point = Q::fromD(0.0, 0.0, 0.0);
-			m.depth = (R)1e20;
-			m.distThresh = (R)0;
-			m.lastDist = (R)0;
-			m.found = 0;
-			m.iters = 0;
-			m.steps = 0;
-			m.stepCount = 0;
-		}
-		else if constexpr (PosPrecision<KIND>::kFP64)
+		const bool fogOn = LENS && f.fog;
+		if constexpr (PosPrecision<KIND>::kFP64)
 			ray_marching<KIND, 2, LENS>(f, start, dir, (R)f.viewDistanceMax, seed, m, w, !lens, &fogAcc, fogOn);
 		else if (f.advancedQuality)
 			ray_marching<KIND, 1, LENS>(f, start, dir, (R)f.viewDistanceMax, seed, m, w, !lens, &fogAcc, fogOn);
@@ -1021,15 +1030,9 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 			colG += colour_byte(colour.y);
 			colB += colour_byte(colour.z);
 		}
-		else if (cut)
-		{
-			rgb = v3(0.0f, 0.0f, 0.0f);
-			shadowCh = 0.0f;
-			opSum += 65535u;
-		}
 		else
 			rgb = background_shader(f, Q::tof(dir));
-		if (LENS && f.fog && !cut)
+		if (LENS && f.fog)
 		{
 			// the volumetric walk as accumulated by the march; the glow colour needs the step count
 			const V3 g = glow_colour(f, m.stepCount);
@@ -1039,7 +1042,7 @@ __device__ void render_pixel(const FrameConst &frame, int xs, int ys, PixelOut &
 			op = op > 1.0f ? 1.0f : op;
 			opSum += (uint32_t)(uint16_t)(int)((double)op * 65535.0);
 		}
-		else if (FULL && f.glow && !cut)
+		else if (FULL && f.glow)
 			rgb = glow_shader(f, m.stepCount, rgb);
 		if (LENS && f.chromaticAberration)
 		{

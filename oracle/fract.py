@@ -1025,6 +1025,12 @@ def load(text, trace=None):
         flags |= FLAGS["animation"]
     out["antialiasingEnabled"] = int(g("antialiasing_enabled"))
     out["antialiasingSize"] = g("antialiasing_size")
+    lens_fx = (g("DOF_monte_carlo") and (g("DOF_enabled") or g("DOF_MC_CA_enable"))) or \
+        g("basic_fog_enabled") or g("volumetric_fog_enabled")
+    if lens_fx and g("perspective_type") != 0:
+        flags |= FLAGS["perspective"]
+    if lens_fx and g("antialiasing_enabled"):
+        flags |= FLAGS["antialiasing"]
 
     TABLE = (2, 3, 4, 5, 7, 8, 10, 11, 16, 19)  # the formulas the device path renders
 

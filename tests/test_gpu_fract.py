@@ -103,6 +103,15 @@ def _load(rt, name, width=192):
     p.distanceFogShadows = 0
     if p.basicFogEnabled or p.volFogEnabled or p.glowEnabled:
         p.volumetricLightDEFactor = 1.0
+    # the projections and antialiasing render without the lens variant's effects (the reader
+    # flags the combination): a fish-eye scene without its lens and fog, an antialiased scene with
+    # its lens and fog but without the antialiasing grid
+    lens_fx = (p.DOFMonteCarlo and (p.DOFEnabled or p.DOFMonteCarloChromaticAberration)) or \
+        p.basicFogEnabled or p.volFogEnabled
+    if lens_fx and p.perspectiveType != 0:
+        p.DOFEnabled = p.DOFMonteCarloChromaticAberration = p.basicFogEnabled = p.volFogEnabled = 0
+    elif lens_fx:
+        p.antialiasingEnabled = 0
     return p, info
 
 

@@ -515,8 +515,24 @@ def test_interior_and_iteration_threshold_parity(rt, oracle_mod, mode):
 def test_unsupported_settings_are_rejected(rt):
     ctx = rt.Context(0)
     p = scene.mandelbulb8(64, 48)
-    p.perspectiveType = 1
+    p.perspectiveType = 4
     with pytest.raises(rt.MbxError, match="perspective"):
+        ctx.upload(p)
+    p = scene.mandelbulb8(64, 48)
+    p.perspectiveType = 1
+    p.DOFMonteCarlo = 1
+    p.DOFEnabled = 1
+    with pytest.raises(rt.MbxError, match="fish-eye"):
+        ctx.upload(p)
+    p = scene.mandelbulb8(64, 48)
+    p.antialiasingEnabled = 1
+    p.antialiasingSize = 11
+    with pytest.raises(rt.MbxError, match="antialiasing"):
+        ctx.upload(p)
+    p = scene.mandelbulb8(64, 48)
+    p.antialiasingEnabled = 1
+    p.basicFogEnabled = 1
+    with pytest.raises(rt.MbxError, match="antialiasing"):
         ctx.upload(p)
     p = scene.mandelbulb8(64, 48)
     p.rngMode = abi.MBX_RNG_GLOBAL_SEQUENTIAL
