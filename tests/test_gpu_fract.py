@@ -119,7 +119,9 @@ def _load(rt, name, width=192):
 def test_loaded_example_scene_parity(rt, oracle_mod, name):
     p, info = _load(rt, name)
     flags = abi.fract_flag_names(info.unsupported)
-    assert not {"hybrid", "primitives", "formula", "perspective", "interior", "disabled"} & set(flags)
+    # (perspective: stereo, or a projection with the lens effects, which _load turns off)
+    assert not {"hybrid", "primitives", "formula", "interior", "disabled"} & set(flags)
+    assert "perspective" not in flags or p.perspectiveType != 0
     print("\n[%s] %dx%d formula %d, not rendered: %s" % (name, p.imageWidth, p.imageHeight,
                                                         p.fractal.formula, ",".join(flags)), flush=True)
     g = rt.render_to_host(p)
