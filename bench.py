@@ -489,8 +489,6 @@ def bench_orbit(args, torch, runtime, wl):
     out.update(legs)
     if "plain_kernel" in legs:
         out["plain_kernel"]["batch_speedup"] = legs["plain_kernel"]["ms_per_step"] / ms_per_step
-    if one_kernel is not None:
-        out["one_kernel"] = one_kernel
     if image is not None:
         out["image_compile"] = image
     if not args.no_cpu_baseline:
@@ -511,15 +509,6 @@ def bench_batch(args, torch, runtime, wl):
     wall, launch_ms = time_kernel(torch, ctx, None, stream, args.steps, args.warmup,
                                   lambda: ctx.render_batch(frames, bufs))
     st = ctx.stats()
-    one_kernel = None
-    if args.workload == "menger" and not args.no_plain:
-        # the same frame as one launch of the full-variant batch kernel (the split render off)
-        ctx.tune(runtime.MBX_TUNE_SPLIT, 0)
-        owall, oms = time_kernel(torch, ctx, bufs, stream, args.steps, args.warmup)
-        ctx.tune(runtime.MBX_TUNE_SPLIT, 1)
-        one_kernel = {"kernel": "batch_kernel<kIfsAnalytic> (full variant, one launch)",
-                      "ms_per_step": owall * 1e3 / args.steps, "avg_launch_ms": sum(oms) / len(oms),
-                      "split_speedup": owall / wall}
     plain = None
     if not args.no_plain:
         ctx.tune(runtime.MBX_TUNE_KERNEL, runtime.KERNEL_PLAIN)
